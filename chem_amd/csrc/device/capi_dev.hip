@@ -1,0 +1,267 @@
+// C ABI: device context, device-resident B, metric, DF-J and DF-K.
+#include <cstring>
+#include <memory>
+
+#include "../../../include/mchip.h"
+#include "../capi_common.hpp"
+#include "dfock.hpp"
+
+using namespace mchip;
+
+struct mchip_btensor_s;   // host container (capi_host.cpp)
+namespace mchip {
+const BLayout& btensor_layout(mchip_btensor t);
+const double* btensor_data(mchip_btensor t);
+}  // namespace mchip
+
+struct mchip_ctx_s {
+  Ctx c;
+};
+struct mchip_dbt_s {
+  DevBT t;
+  mchip_ctx_s* ctx;
+  int device;
+};
+struct mchip_metric_s {
+  Metric m;
+  int device;
+};
+
+static void need_device(const mchip_ctx_s* c)
+{
+  if (!c) throw std::runtime_error("null context");
+  HIP_CHECK(hipSetDevice(c->c.device));
+}
+
+extern "C" int mchip_device_count(int* n)
+{
+  MCHIP_TRY
+  int k = 0;
+  hipError_t e = hipGetDeviceCount(&k);
+  *n = (e == hipSuccess) ? k : 0;
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_ctx_create(int device, mchip_ctx* out)
+{
+  MCHIP_TRY
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+    throw std::runtime_error("no HIP device available");
+  if (device < 0 || device >= n) throw std::runtime_error("device index out of range");
+  auto c = std::make_unique<mchip_ctx_s>();
+  ctx_init(c->c, device);
+  *out = c.release();
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_ctx_destroy(mchip_ctx c)
+{
+  MCHIP_TRY
+  if (c) {
+    (void)hipSetDevice(c->c.device);
+    ctx_free(c->c);
+    delete c;
+  }
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_ctx_sync(mchip_ctx c)
+{
+  MCHIP_TRY
+  need_device(c);
+  HIP_CHECK(hipStreamSynchronize(c->c.stream));
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_ctx_stream(mchip_ctx c, void** stream)
+{
+  MCHIP_TRY
+  *stream = (void*)c->c.stream;
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dbt_upload(mchip_ctx c, mchip_btensor host, mchip_dbt* out)
+{
+  MCHIP_TRY
+  need_device(c);
+  auto d = std::make_unique<mchip_dbt_s>();
+  d->ctx = c;
+  d->device = c->c.device;
+  dbt_upload(c->c, btensor_layout(host), btensor_data(host), d->t);
+  *out = d.release();
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dbt_destroy(mchip_dbt t)
+{
+  MCHIP_TRY
+  if (t) {
+    // the owning context may already be gone: synchronise the device instead
+    (void)hipSetDevice(t->device);
+    (void)hipDeviceSynchronize();
+    delete t;
+  }
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dbt_download(mchip_dbt t, double* host)
+{
+  MCHIP_TRY
+  need_device(t->ctx);
+  if (t->t.L.total)
+    HIP_CHECK(hipMemcpy(host, t->t.data64.p, t->t.data64.bytes, hipMemcpyDeviceToHost));
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_metric_create(mchip_ctx c, int naux, const double* V, mchip_metric* out)
+{
+  MCHIP_TRY
+  need_device(c);
+  auto m = std::make_unique<mchip_metric_s>();
+  m->device = c->c.device;
+  metric_create(c->c, naux, V, m->m);
+  *out = m.release();
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_metric_destroy(mchip_metric m)
+{
+  MCHIP_TRY
+  if (m) {
+    (void)hipSetDevice(m->device);
+    (void)hipDeviceSynchronize();
+    delete m;
+  }
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_metric_get(mchip_ctx c, mchip_metric m, double* Linv, double* Vinv)
+{
+  MCHIP_TRY
+  need_device(c);
+  const size_t bytes = (size_t)m->m.naux * m->m.naux * sizeof(double);
+  HIP_CHECK(hipStreamSynchronize(c->c.stream));
+  if (Linv) HIP_CHECK(hipMemcpy(Linv, m->m.Linv.p, bytes, hipMemcpyDeviceToHost));
+  if (Vinv) HIP_CHECK(hipMemcpy(Vinv, m->m.Vinv.p, bytes, hipMemcpyDeviceToHost));
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dbt_transform(mchip_dbt t, mchip_metric m)
+{
+  MCHIP_TRY
+  need_device(t->ctx);
+  transform_B(t->ctx->c, t->t, m->m);
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dfj_dev(mchip_dbt t, mchip_metric m, const double* P_dev, double pscale,
+                             double* J_dev)
+{
+  MCHIP_TRY
+  need_device(t->ctx);
+  dfj(t->ctx->c, t->t, m ? &m->m : nullptr, P_dev, J_dev, pscale);
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dfk_mo_dev(mchip_dbt t, mchip_metric m, const double* C_dev, int ldc, int nocc,
+                                int occ_nbatches, double* K_dev)
+{
+  MCHIP_TRY
+  need_device(t->ctx);
+  dfk_mo(t->ctx->c, t->t, m ? &m->m : nullptr, C_dev, ldc, nocc, K_dev, occ_nbatches);
+  MCHIP_CATCH
+}
+
+// Host-pointer conveniences (copy in, compute, copy out, synchronous).
+extern "C" int mchip_dfj(mchip_dbt t, mchip_metric m, const double* P, double pscale, double* J)
+{
+  MCHIP_TRY
+  need_device(t->ctx);
+  Ctx& c = t->ctx->c;
+  const int nbf = t->t.L.nbf;
+  const size_t bytes = (size_t)nbf * nbf * sizeof(double);
+  DevBuf dP(bytes), dJ(bytes);
+  HIP_CHECK(hipMemcpyAsync(dP.p, P, bytes, hipMemcpyHostToDevice, c.stream));
+  dfj(c, t->t, m ? &m->m : nullptr, dP.as<double>(), dJ.as<double>(), pscale);
+  HIP_CHECK(hipMemcpyAsync(J, dJ.p, bytes, hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dfk_mo(mchip_dbt t, mchip_metric m, const double* C, int nocc,
+                            int occ_nbatches, double* K)
+{
+  MCHIP_TRY
+  need_device(t->ctx);
+  Ctx& c = t->ctx->c;
+  const int nbf = t->t.L.nbf;
+  const size_t kb = (size_t)nbf * nbf * sizeof(double);
+  const size_t cb = (size_t)nbf * std::max(nocc, 1) * sizeof(double);
+  DevBuf dC(cb), dK(kb);
+  if (nocc > 0) HIP_CHECK(hipMemcpyAsync(dC.p, C, (size_t)nbf * nocc * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  dfk_mo(c, t->t, m ? &m->m : nullptr, dC.as<double>(), nocc, nocc, dK.as<double>(), occ_nbatches);
+  HIP_CHECK(hipMemcpyAsync(K, dK.p, kb, hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  MCHIP_CATCH
+}
+
+#include "scf.hpp"
+
+extern "C" int mchip_scf_run(mchip_dbt t, mchip_metric m, int nbf, int nocc, int nbb,
+                             const int* bsz, const double* S, const double* H, double enuc,
+                             double threshold, int max_iter, int do_diis, int diis_start,
+                             int diis_min, int diis_max, double filter_eps, int kmode,
+                             int occ_nbatches, double* energy, int* iterations, int* converged,
+                             double* C, double* eps, int max_hist, double* hist_e,
+                             double* hist_rms, double* t_j, double* t_k, int* nhist)
+{
+  MCHIP_TRY
+  need_device(t->ctx);
+  Ctx& c = t->ctx->c;
+  if (nbf != t->t.L.nbf) throw std::runtime_error("mchip_scf_run: nbf does not match B");
+  if (nocc < 0 || nocc > nbf) throw std::runtime_error("mchip_scf_run: bad nocc");
+  ScfInput in;
+  in.nbf = nbf;
+  in.nocc = nocc;
+  in.bsz.assign(bsz, bsz + nbb);
+  in.S = S;
+  in.H = H;
+  in.enuc = enuc;
+  in.threshold = threshold;
+  in.max_iter = max_iter;
+  in.do_diis = do_diis != 0;
+  in.diis_start = diis_start;
+  in.diis_min = diis_min;
+  in.diis_max = diis_max;
+  in.filter_eps = filter_eps;
+  const Metric* met = m ? &m->m : nullptr;
+  if (kmode == 0 && !met) throw std::runtime_error("mchip_scf_run: plain K needs the metric");
+  if (kmode == 1 && !t->t.transformed) throw std::runtime_error("mchip_scf_run: B is not transformed");
+  const Metric* jmet = t->t.transformed ? nullptr : met;
+  KBuildFn kb;
+  if (kmode == 0 || kmode == 1) {
+    kb = [&](int, const double*, const double* Cocc, double* K) {
+      dfk_mo(c, t->t, t->t.transformed ? nullptr : met, Cocc, nocc, nocc, K, occ_nbatches);
+    };
+  }
+  else {
+    throw std::runtime_error("mchip_scf_run: unknown K mode");
+  }
+  ScfOutput out;
+  scf_run(c, t->t, jmet, in, out, kb);
+  *energy = out.energy;
+  *iterations = out.iterations;
+  *converged = out.converged ? 1 : 0;
+  if (C) std::memcpy(C, out.C.data(), out.C.size() * sizeof(double));
+  if (eps) std::memcpy(eps, out.eps.data(), out.eps.size() * sizeof(double));
+  const int nh = std::min<int>(max_hist, (int)out.history_e.size());
+  for (int i = 0; i < nh; ++i) {
+    if (hist_e) hist_e[i] = out.history_e[i];
+    if (hist_rms) hist_rms[i] = out.history_rms[i];
+    if (t_j) t_j[i] = out.t_j[i];
+    if (t_k) t_k[i] = out.t_k[i];
+  }
+  *nhist = nh;
+  MCHIP_CATCH
+}

@@ -1,0 +1,123 @@
+// Device context and device-resident B tensor.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ints/btensor.hpp"
+
+namespace mchip {
+
+#define HIP_CHECK(x)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (x);                                                          \
+    if (e_ != hipSuccess)                                                         \
+      throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + \
+                               " at " + __FILE__ + ":" + std::to_string(__LINE__)); \
+  } while (0)
+
+#define RB_CHECK(x)                                                               \
+  do {                                                                            \
+    rocblas_status s_ = (x);                                                      \
+    if (s_ != rocblas_status_success)                                             \
+      throw std::runtime_error(std::string("rocBLAS/rocSOLVER error ") +          \
+                               rocblas_status_to_string(s_) + " at " + __FILE__ + \
+                               ":" + std::to_string(__LINE__));                   \
+  } while (0)
+
+// Owning device buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t n) { alloc(n); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept
+  {
+    release();
+    p = o.p;
+    bytes = o.bytes;
+    o.p = nullptr;
+    o.bytes = 0;
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t n)
+  {
+    release();
+    if (n) HIP_CHECK(hipMalloc(&p, n));
+    bytes = n;
+  }
+  // grow-only
+  void ensure(size_t n)
+  {
+    if (n > bytes) alloc(n);
+  }
+  void release()
+  {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+template <class T>
+DevBuf upload_vec(const std::vector<T>& v, hipStream_t s)
+{
+  DevBuf b(v.size() * sizeof(T));
+  if (!v.empty()) HIP_CHECK(hipMemcpyAsync(b.p, v.data(), b.bytes, hipMemcpyHostToDevice, s));
+  return b;
+}
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  rocblas_handle blas = nullptr;
+  DevBuf scratch_a, scratch_b, scratch_c;   // reusable workspaces
+  DevBuf scratch_partial;                   // split-K partial sums
+  DevBuf info;                              // rocsolver info
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+enum class Prec : int { F64 = 0, F32 = 1, BF16X3 = 2 };
+
+// Tile descriptor for grouped launches over slabs.
+struct Tile {
+  int slab;
+  int m0;
+  int n0;
+  int pad;
+};
+
+struct DevBT {
+  BLayout L;                 // host copy of the layout
+  DevBuf data64;             // FP64 slabs (B or B~)
+  DevBuf data32;             // FP32 slabs (optional)
+  DevBuf data_hi, data_lo;   // BF16 split slabs (optional)
+  DevBuf slab_off, ncol, bsz, boff, colmap, colmap_off, pg_off;
+  std::vector<int> h_colmap, h_colmap_off;
+  std::vector<int64_t> h_pg_off;            // prefix of bsz[N]*ncol[N]
+  int64_t colmap_total = 0;
+  int64_t pg_total = 0;                     // sum of bsz[N]*ncol[N]
+  int64_t maxR = 0;                         // max bsz[N]*ncol[N]
+  bool transformed = false;  // data64 holds L^-1 B
+  std::map<std::pair<int, int>, DevBuf> tiles;   // (BN, ng) -> tile table
+  std::map<std::pair<int, int>, int> ntiles;
+};
+
+struct Metric {
+  int naux = 0;
+  DevBuf Linv;   // column-major lower-triangular L^-1 (V = L L^T)
+  DevBuf Vinv;   // V^-1 = L^-T L^-1 (symmetric)
+};
+
+}  // namespace mchip

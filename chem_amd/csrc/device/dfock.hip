@@ -1,0 +1,306 @@
+// Device drivers of the DF Fock build: B upload, Coulomb metric, DF-J and
+// the DFMO exchange, FP64.
+//
+//   metric_create   aoloader::invert + math::LLT (aoloader.cpp:14-32,
+//                   LLT.cpp:8-109): V = L L^T (rocsolver potrf), L^-1 (trtri),
+//                   V^-1 = L^-T L^-1.
+//   dfj             DF_J::compute_J (jkdf.cpp:69-190).
+//   dfk_mo          DFMO_K::compute_K (jkdf.cpp:221-462), K returned negated.
+//   transform_B     B~ = L^-1 B (one-time), after which K = -sum_X (B~_X C)^T
+//                   (B~_X C) and J = B~^T (B~ . P) need no metric per call.
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "dfock.hpp"
+#include "kernels.hpp"
+
+namespace mchip {
+
+__global__ void k_zero_upper_cm(double* A, int n)
+{
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n * n) return;
+  const int j = (int)(idx / n), i = (int)(idx % n);   // column-major (i,j)
+  if (i < j) A[idx] = 0.0;
+}
+
+void ctx_init(Ctx& c, int device)
+{
+  c.device = device;
+  HIP_CHECK(hipSetDevice(device));
+  HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  RB_CHECK(rocblas_create_handle(&c.blas));
+  RB_CHECK(rocblas_set_stream(c.blas, c.stream));
+  c.info.alloc(sizeof(int) * 4);
+  HIP_CHECK(hipEventCreate(&c.ev0));
+  HIP_CHECK(hipEventCreate(&c.ev1));
+}
+
+void ctx_free(Ctx& c)
+{
+  if (c.stream) (void)hipStreamSynchronize(c.stream);
+  c.scratch_a.release();
+  c.scratch_b.release();
+  c.scratch_c.release();
+  c.info.release();
+  if (c.blas) rocblas_destroy_handle(c.blas);
+  if (c.stream) (void)hipStreamDestroy(c.stream);
+  if (c.ev0) (void)hipEventDestroy(c.ev0);
+  if (c.ev1) (void)hipEventDestroy(c.ev1);
+  c.blas = nullptr;
+  c.stream = nullptr;
+}
+
+void dbt_upload(Ctx& c, const BLayout& L, const double* host_data, DevBT& t)
+{
+  HIP_CHECK(hipSetDevice(c.device));
+  t.L = L;
+  t.data64.alloc((size_t)L.total * sizeof(double));
+  if (L.total)
+    HIP_CHECK(hipMemcpyAsync(t.data64.p, host_data, t.data64.bytes, hipMemcpyHostToDevice, c.stream));
+  // column -> AO function map per slab
+  t.h_colmap_off.assign(L.nbb + 1, 0);
+  t.h_colmap.clear();
+  for (int n = 0; n < L.nbb; ++n) {
+    for (int p = L.pair_ptr[n]; p < L.pair_ptr[n + 1]; ++p)
+      for (int mu = 0; mu < L.bsz[L.pair_m[p]]; ++mu) t.h_colmap.push_back(L.boff[L.pair_m[p]] + mu);
+    t.h_colmap_off[n + 1] = (int)t.h_colmap.size();
+  }
+  t.colmap_total = (int64_t)t.h_colmap.size();
+  t.h_pg_off.assign(L.nbb + 1, 0);
+  t.maxR = 0;
+  for (int n = 0; n < L.nbb; ++n) {
+    int64_t R = (int64_t)L.bsz[n] * L.ncol[n];
+    t.h_pg_off[n + 1] = t.h_pg_off[n] + R;
+    t.maxR = std::max<int64_t>(t.maxR, R);
+  }
+  t.pg_total = t.h_pg_off[L.nbb];
+  std::vector<int64_t> so(L.slab_off.begin(), L.slab_off.end());
+  t.slab_off = upload_vec(so, c.stream);
+  t.ncol = upload_vec(L.ncol, c.stream);
+  t.bsz = upload_vec(L.bsz, c.stream);
+  std::vector<int> bo(L.boff.begin(), L.boff.end());
+  t.boff = upload_vec(bo, c.stream);
+  t.colmap = upload_vec(t.h_colmap, c.stream);
+  t.colmap_off = upload_vec(t.h_colmap_off, c.stream);
+  t.pg_off = upload_vec(t.h_pg_off, c.stream);
+  t.transformed = false;
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+}
+
+// Tile table of the grouped slab GEMM for ng output columns.
+const Tile* slab_tiles(Ctx& c, DevBT& t, int ng, int* ntiles)
+{
+  auto key = std::make_pair(SG_BN, ng);
+  auto it = t.tiles.find(key);
+  if (it == t.tiles.end()) {
+    std::vector<Tile> v;
+    for (int n = 0; n < t.L.nbb; ++n) {
+      const int64_t rows = (int64_t)t.L.naux * t.L.bsz[n];
+      for (int64_t m0 = 0; m0 < rows; m0 += SG_BM)
+        for (int n0 = 0; n0 < ng; n0 += SG_BN) v.push_back(Tile{n, (int)m0, n0, 0});
+    }
+    t.ntiles[key] = (int)v.size();
+    t.tiles[key] = upload_vec(v, c.stream);
+    HIP_CHECK(hipStreamSynchronize(c.stream));   // v is pageable and dies here
+    it = t.tiles.find(key);
+  }
+  *ntiles = t.ntiles[key];
+  return it->second.as<Tile>();
+}
+
+void slab_gemm(Ctx& c, DevBT& t, const double* G, int ldg, int ng, double* out, int ldo,
+               double alpha)
+{
+  int nt = 0;
+  const Tile* tiles = slab_tiles(c, t, ng, &nt);
+  if (nt == 0) return;
+  hipLaunchKernelGGL(k_slab_gemm_f64, dim3(nt), dim3(256), 0, c.stream, t.data64.as<double>(),
+                     t.slab_off.as<int64_t>(), t.ncol.as<int>(), t.bsz.as<int>(), t.boff.as<int>(),
+                     t.colmap.as<int>(), t.colmap_off.as<int>(), tiles, t.L.naux, t.L.nbf, G, ldg,
+                     ng, out, ldo, alpha);
+  HIP_CHECK(hipGetLastError());
+}
+
+// C(m,n) = alpha * sum_s sum_k A_s(m,k) B_s(n,k), symmetric output.  When B
+// differs from A the symmetrised sum G + G^T is returned.
+void seg_syrk(Ctx& c, const double* A, const double* B, int nseg, int64_t sstride, int lda,
+              int klen, int n, double alpha, double* out, DevBuf& partial_buf, int nbatch,
+              int batch, bool reduce)
+{
+  const bool sym = (A == B);
+  const int nt = (n + SG_BM - 1) / SG_BM;
+  std::vector<int2> tl;
+  for (int i = 0; i < nt; ++i)
+    for (int j = 0; j < nt; ++j)
+      if (!sym || i <= j) tl.push_back(int2{i, j});
+  const int ntile = (int)tl.size();
+  int nch = std::max(1, std::min(nseg, 2048 / std::max(1, ntile)));
+  const int per = (nseg + nch - 1) / nch;
+  nch = (nseg + per - 1) / per;
+  // partial holds nbatch groups of nch chunks
+  const size_t need = (size_t)nbatch * nch * n * n * sizeof(double);
+  if (batch == 0) partial_buf.ensure(need);
+  double* partial = partial_buf.as<double>() + (size_t)batch * nch * n * n;
+  DevBuf tiles = upload_vec(tl, c.stream);
+  hipLaunchKernelGGL(k_seg_gemm_nt_f64, dim3(ntile * nch), dim3(256), 0, c.stream, A, B, nseg,
+                     sstride, lda, klen, n, tiles.as<int2>(), ntile, per, partial);
+  HIP_CHECK(hipGetLastError());
+  if (reduce) {
+    hipLaunchKernelGGL(k_reduce_sym, dim3(nblk((int64_t)n * n, 256)), dim3(256), 0, c.stream,
+                       partial_buf.as<double>(), nbatch * nch, n, sym ? 0 : 1, alpha, out);
+    HIP_CHECK(hipGetLastError());
+  }
+  HIP_CHECK(hipStreamSynchronize(c.stream));   // tile table lifetime
+}
+
+void metric_create(Ctx& c, int naux, const double* V_host, Metric& m)
+{
+  HIP_CHECK(hipSetDevice(c.device));
+  m.naux = naux;
+  const size_t bytes = (size_t)naux * naux * sizeof(double);
+  m.Linv.alloc(bytes);
+  m.Vinv.alloc(bytes);
+  HIP_CHECK(hipMemcpyAsync(m.Linv.p, V_host, bytes, hipMemcpyHostToDevice, c.stream));
+  double* A = m.Linv.as<double>();
+  int* info = c.info.as<int>();
+  RB_CHECK(rocsolver_dpotrf(c.blas, rocblas_fill_lower, naux, A, naux, info));
+  int h_info = 0;
+  HIP_CHECK(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  if (h_info != 0) throw std::runtime_error("metric Cholesky failed (matrix not positive definite)");
+  hipLaunchKernelGGL(k_zero_upper_cm, dim3(nblk((int64_t)naux * naux, 256)), dim3(256), 0, c.stream, A, naux);
+  RB_CHECK(rocsolver_dtrtri(c.blas, rocblas_fill_lower, rocblas_diagonal_non_unit, naux, A, naux, info));
+  HIP_CHECK(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  if (h_info != 0) throw std::runtime_error("metric triangular inverse failed");
+  hipLaunchKernelGGL(k_zero_upper_cm, dim3(nblk((int64_t)naux * naux, 256)), dim3(256), 0, c.stream, A, naux);
+  // V^-1 = L^-T L^-1 (column-major: Linv^T * Linv)
+  const double one = 1.0, zero = 0.0;
+  RB_CHECK(rocblas_dgemm(c.blas, rocblas_operation_transpose, rocblas_operation_none, naux, naux,
+                         naux, &one, A, naux, A, naux, &zero, m.Vinv.as<double>(), naux));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+}
+
+void transform_B(Ctx& c, DevBT& t, const Metric& m)
+{
+  if (t.transformed) throw std::runtime_error("B is already transformed");
+  if (m.naux != t.L.naux) throw std::runtime_error("metric and B have different aux dimensions");
+  int64_t maxslab = 0;
+  for (int n = 0; n < t.L.nbb; ++n) maxslab = std::max<int64_t>(maxslab, t.L.slab_off[n + 1] - t.L.slab_off[n]);
+  c.scratch_c.ensure((size_t)maxslab * sizeof(double));
+  const double one = 1.0;
+  for (int n = 0; n < t.L.nbb; ++n) {
+    const int64_t R = (int64_t)t.L.bsz[n] * t.L.ncol[n];
+    if (R == 0) continue;
+    double* slab = t.data64.as<double>() + t.L.slab_off[n];
+    const size_t bytes = (size_t)R * t.L.naux * sizeof(double);
+    HIP_CHECK(hipMemcpyAsync(c.scratch_c.p, slab, bytes, hipMemcpyDeviceToDevice, c.stream));
+    // column-major (R x naux) view: slab <- slab * (L^-1)^T
+    RB_CHECK(rocblas_dtrmm_64(c.blas, rocblas_side_right, rocblas_fill_lower,
+                              rocblas_operation_transpose, rocblas_diagonal_non_unit, R, t.L.naux,
+                              &one, m.Linv.as<double>(), t.L.naux, c.scratch_c.as<double>(), R,
+                              slab, R));
+  }
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  t.transformed = true;
+}
+
+void dfj(Ctx& c, DevBT& t, const Metric* m, const double* P, double* J, double pscale)
+{
+  const BLayout& L = t.L;
+  if (!t.transformed && !m) throw std::runtime_error("DF-J needs the metric for an untransformed B");
+  const int naux = L.naux, nbf = L.nbf, nbb = L.nbb;
+  const int64_t pg = t.pg_total;
+  int nch = (int)std::max<int64_t>(1, std::min<int64_t>(64, (2000000 + pg - 1) / std::max<int64_t>(pg, 1)));
+  nch = std::min(nch, naux);
+  const int xchunk = (naux + nch - 1) / nch;
+  nch = (naux + xchunk - 1) / xchunk;
+  // workspace: Pg | gpart | g | gq | Jpart | Jg
+  const size_t n_pg = (size_t)pg, n_gp = (size_t)nbb * naux, n_jp = (size_t)nch * pg;
+  c.scratch_c.ensure((n_pg + n_gp + 2 * (size_t)naux + n_jp + n_pg) * sizeof(double) + 256);
+  double* Pg = c.scratch_c.as<double>();
+  double* gpart = Pg + n_pg;
+  double* g = gpart + n_gp;
+  double* gq = g + naux;
+  double* Jpart = gq + naux;
+  double* Jg = Jpart + n_jp;
+  const unsigned gx = nblk(std::max<int64_t>(t.maxR, 1), 256);
+  if (pg > 0) {
+    hipLaunchKernelGGL(k_pair_gather, dim3(gx, nbb), dim3(256), 0, c.stream, P, nbf, t.ncol.as<int>(),
+                       t.bsz.as<int>(), t.boff.as<int>(), t.colmap.as<int>(), t.colmap_off.as<int>(),
+                       t.pg_off.as<int64_t>(), Pg);
+    hipLaunchKernelGGL(k_j_gamma, dim3(nblk(naux, J_ROWS), nbb), dim3(256), 0, c.stream,
+                       t.data64.as<double>(), t.slab_off.as<int64_t>(), t.ncol.as<int>(),
+                       t.bsz.as<int>(), t.pg_off.as<int64_t>(), Pg, naux, gpart);
+  }
+  else {
+    hipLaunchKernelGGL(k_fill, dim3(nblk(n_gp, 256)), dim3(256), 0, c.stream, gpart, (int64_t)n_gp, 0.0);
+  }
+  hipLaunchKernelGGL(k_reduce_slabs, dim3(nblk(naux, 256)), dim3(256), 0, c.stream, gpart, nbb, naux,
+                     pscale, g);
+  if (t.transformed) {
+    HIP_CHECK(hipMemcpyAsync(gq, g, naux * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  }
+  else {
+    const double one = 1.0, zero = 0.0;
+    RB_CHECK(rocblas_dgemv(c.blas, rocblas_operation_transpose, naux, naux, &one, m->Vinv.as<double>(),
+                           naux, g, 1, &zero, gq, 1));
+  }
+  hipLaunchKernelGGL(k_fill, dim3(nblk((int64_t)nbf * nbf, 256)), dim3(256), 0, c.stream, J,
+                     (int64_t)nbf * nbf, 0.0);
+  if (pg > 0) {
+    hipLaunchKernelGGL(k_j_cols, dim3(gx, nbb, nch), dim3(256), 0, c.stream, t.data64.as<double>(),
+                       t.slab_off.as<int64_t>(), t.ncol.as<int>(), t.bsz.as<int>(), t.pg_off.as<int64_t>(),
+                       gq, naux, xchunk, pg, Jpart);
+    hipLaunchKernelGGL(k_reduce_chunks, dim3(nblk(pg, 256)), dim3(256), 0, c.stream, Jpart, nch, pg, Jg);
+    hipLaunchKernelGGL(k_pair_scatter, dim3(gx, nbb), dim3(256), 0, c.stream, Jg, nbf, t.ncol.as<int>(),
+                       t.bsz.as<int>(), t.boff.as<int>(), t.colmap.as<int>(), t.colmap_off.as<int>(),
+                       t.pg_off.as<int64_t>(), J);
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+void dfk_mo(Ctx& c, DevBT& t, const Metric* m, const double* C, int ldc, int nocc, double* K,
+            int occ_nbatches)
+{
+  const BLayout& L = t.L;
+  if (!t.transformed && !m) throw std::runtime_error("DFMO-K needs the metric for an untransformed B");
+  const int naux = L.naux, nbf = L.nbf;
+  if (nocc <= 0) {
+    hipLaunchKernelGGL(k_fill, dim3(nblk((int64_t)nbf * nbf, 256)), dim3(256), 0, c.stream, K,
+                       (int64_t)nbf * nbf, 0.0);
+    return;
+  }
+  occ_nbatches = std::max(1, std::min(occ_nbatches, nocc));
+  const int per = (nocc + occ_nbatches - 1) / occ_nbatches;
+  const int nb = (nocc + per - 1) / per;
+  const size_t ht = (size_t)naux * nbf * per;
+  c.scratch_a.ensure(ht * sizeof(double));
+  if (!t.transformed) c.scratch_b.ensure(ht * sizeof(double));
+  DevBuf& partial = c.scratch_partial;
+  for (int b = 0; b < nb; ++b) {
+    const int i0 = b * per;
+    const int w = std::min(per, nocc - i0);
+    double* HT1 = c.scratch_a.as<double>();
+    // K1: HT1[x][nu][i] = sum_mu B(x|mu nu) C(mu, i0+i)
+    slab_gemm(c, t, C + i0, ldc, w, HT1, w, 1.0);
+    const double* HT = HT1;
+    if (!t.transformed) {
+      // K2: HT2 = L^-1 HT1 over x; column-major (W x naux) view times (L^-1)^T
+      double* HT2 = c.scratch_b.as<double>();
+      const int64_t W = (int64_t)nbf * w;
+      const double one = 1.0;
+      RB_CHECK(rocblas_dtrmm_64(c.blas, rocblas_side_right, rocblas_fill_lower,
+                                rocblas_operation_transpose, rocblas_diagonal_non_unit, W, naux, &one,
+                                m->Linv.as<double>(), naux, HT1, W, HT2, W));
+      HT = HT2;
+    }
+    // K3: K = -sum_{x,i} HT[x][m][i] HT[x][n][i]
+    seg_syrk(c, HT, HT, naux, (int64_t)nbf * w, w, w, nbf, -1.0, K, partial, nb, b, b == nb - 1);
+  }
+}
+
+}  // namespace mchip

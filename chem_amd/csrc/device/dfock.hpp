@@ -1,0 +1,24 @@
+// Device drivers of the DF Fock build.
+#pragma once
+
+#include "dev.hpp"
+
+namespace mchip {
+
+void ctx_init(Ctx& c, int device);
+void ctx_free(Ctx& c);
+void dbt_upload(Ctx& c, const BLayout& L, const double* host_data, DevBT& t);
+void metric_create(Ctx& c, int naux, const double* V_host, Metric& m);
+void transform_B(Ctx& c, DevBT& t, const Metric& m);
+void slab_gemm(Ctx& c, DevBT& t, const double* G, int ldg, int ng, double* out, int ldo,
+               double alpha);
+void seg_syrk(Ctx& c, const double* A, const double* B, int nseg, int64_t sstride, int lda,
+              int klen, int n, double alpha, double* out, DevBuf& partial_buf, int nbatch,
+              int batch, bool reduce);
+// J from density P (device, nbf x nbf); the total density is pscale * P.
+void dfj(Ctx& c, DevBT& t, const Metric* m, const double* P, double* J, double pscale);
+// K = -sum ... from occupied coefficients C (device, row-major nbf x ldc).
+void dfk_mo(Ctx& c, DevBT& t, const Metric* m, const double* C, int ldc, int nocc, double* K,
+            int occ_nbatches);
+
+}  // namespace mchip

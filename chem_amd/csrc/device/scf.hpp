@@ -1,0 +1,38 @@
+// Restricted HF driver on the GPU.
+#pragma once
+
+#include <functional>
+#include <vector>
+
+#include "dev.hpp"
+
+namespace mchip {
+
+struct ScfInput {
+  int nbf = 0, nocc = 0;
+  std::vector<int> bsz;          // AO block sizes (for the block filter)
+  const double* S = nullptr;     // host, row-major
+  const double* H = nullptr;     // host, T + V
+  double enuc = 0.0;
+  double threshold = 1e-6;
+  int max_iter = 100;
+  bool do_diis = true;
+  int diis_start = 1, diis_min = 2, diis_max = 10;
+  double filter_eps = 0.0;
+};
+
+struct ScfOutput {
+  bool converged = false;
+  int iterations = 0;
+  double energy = 0.0;
+  std::vector<double> C, eps, P, F;
+  std::vector<double> history_e, history_rms, t_j, t_k;
+};
+
+// K builder callback: (iteration, P_dev, Cocc_dev row-major nbf x nocc, K_dev out)
+using KBuildFn = std::function<void(int, const double*, const double*, double*)>;
+
+void scf_run(Ctx& c, DevBT& bt, const Metric* m, const ScfInput& in, ScfOutput& out,
+             const KBuildFn& kbuild);
+
+}  // namespace mchip

@@ -1,0 +1,138 @@
+"""Device-side objects: context, resident B tensor, Coulomb metric.
+
+Thin owners of the C ABI handles (include/mchip.h); every compute call goes to
+libmchip.so.
+"""
+import ctypes
+import weakref
+
+import numpy as np
+
+from . import _lib
+
+
+def device_count():
+    n = ctypes.c_int()
+    _lib.check(_lib.lib().mchip_device_count(ctypes.byref(n)))
+    return n.value
+
+
+class Context:
+    """One GPU, one HIP stream, one rocBLAS handle.
+
+    Objects created on a context (DeviceB, Metric) register here and are
+    released before the context itself."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().mchip_ctx_create(device, ctypes.byref(h)))
+        self.h = h
+        self.device = device
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, obj):
+        self._children.add(obj)
+
+    def sync(self):
+        _lib.check(_lib.lib().mchip_ctx_sync(self.h))
+
+    @property
+    def stream(self):
+        s = ctypes.c_void_p()
+        _lib.check(_lib.lib().mchip_ctx_stream(self.h, ctypes.byref(s)))
+        return s.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            for ch in list(self._children):
+                ch.release()
+            _lib.lib().mchip_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Metric:
+    """V = L L^T, L^-1 and V^-1 resident on the GPU."""
+
+    def __init__(self, ctx, V):
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().mchip_metric_create(ctx.h, V.shape[0], _lib.dptr(V), ctypes.byref(h)))
+        self.h = h
+        self.ctx = ctx
+        self.naux = V.shape[0]
+        ctx._adopt(self)
+
+    def get(self):
+        n = self.naux
+        Linv_cm = np.zeros((n, n))
+        Vinv = np.zeros((n, n))
+        _lib.check(_lib.lib().mchip_metric_get(self.ctx.h, self.h, _lib.dptr(Linv_cm), _lib.dptr(Vinv)))
+        # column-major storage -> row-major lower-triangular L^-1
+        return Linv_cm.T.copy(), Vinv
+
+    def release(self):
+        if getattr(self, "h", None):
+            _lib.lib().mchip_metric_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class DeviceB:
+    """B(X|mu nu) resident in HBM, pair-slab layout."""
+
+    def __init__(self, ctx, host_b):
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().mchip_dbt_upload(ctx.h, host_b.h, ctypes.byref(h)))
+        self.h = h
+        self.ctx = ctx
+        self.nbf = host_b.nbf
+        self.naux = host_b.naux
+        self.nelem = host_b.nelem
+        self.transformed = False
+        ctx._adopt(self)
+
+    def transform(self, metric):
+        _lib.check(_lib.lib().mchip_dbt_transform(self.h, metric.h))
+        self.transformed = True
+
+    def download(self):
+        out = np.zeros(self.nelem)
+        _lib.check(_lib.lib().mchip_dbt_download(self.h, _lib.dptr(out)))
+        return out
+
+    def dfj(self, P, metric=None, pscale=2.0):
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        J = np.zeros_like(P)
+        m = None if metric is None else metric.h
+        _lib.check(_lib.lib().mchip_dfj(self.h, m, _lib.dptr(P), pscale, _lib.dptr(J)))
+        return J
+
+    def dfk_mo(self, C_occ, metric=None, occ_nbatches=1):
+        C = np.ascontiguousarray(C_occ, dtype=np.float64)
+        K = np.zeros((self.nbf, self.nbf))
+        m = None if metric is None else metric.h
+        _lib.check(_lib.lib().mchip_dfk_mo(self.h, m, _lib.dptr(C), C.shape[1], occ_nbatches,
+                                           _lib.dptr(K)))
+        return K
+
+    def release(self):
+        if getattr(self, "h", None):
+            _lib.lib().mchip_dbt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass

@@ -1,0 +1,54 @@
+"""Restricted Hartree-Fock driver (host side of hf::hfmod, src/hf/hfscf.cpp).
+
+Integrals and descriptors are prepared here; the SCF loop itself (J, K, Fock
+assembly, DIIS, diagonalisation) runs on the GPU inside libmchip.so
+(mchip_scf_run).
+"""
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib, ints
+from .engine import DeviceB, Metric
+
+KMODES = {"dfmo": 0, "dfmo_transformed": 1}
+
+
+@dataclass
+class ScfResult:
+    energy: float
+    iterations: int
+    converged: bool
+    C: np.ndarray
+    eps: np.ndarray
+    history_e: list = field(default_factory=list)
+    history_rms: list = field(default_factory=list)
+    t_j: list = field(default_factory=list)
+    t_k: list = field(default_factory=list)
+
+
+def run_rhf(mol, dB, metric, S, H, threshold=1e-8, max_iter=100, do_diis=True, diis_start=1,
+            diis_min=2, diis_max=10, filter_eps=1e-10, kmode="dfmo", occ_nbatches=1):
+    n = mol.nbf
+    bsz = np.array(mol.b, dtype=np.int32)
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    H = np.ascontiguousarray(H, dtype=np.float64)
+    C = np.zeros((n, n))
+    eps = np.zeros(n)
+    mh = max_iter + 3
+    he, hr, tj, tk = (np.zeros(mh) for _ in range(4))
+    e = ctypes.c_double()
+    it = ctypes.c_int()
+    conv = ctypes.c_int()
+    nh = ctypes.c_int()
+    m = None if metric is None else metric.h
+    _lib.check(_lib.lib().mchip_scf_run(
+        dB.h, m, n, mol.nocc, len(bsz), _lib.iptr(bsz), _lib.dptr(S), _lib.dptr(H),
+        mol.nuclear_repulsion(), threshold, max_iter, int(do_diis), diis_start, diis_min,
+        diis_max, filter_eps, KMODES[kmode], occ_nbatches, ctypes.byref(e), ctypes.byref(it),
+        ctypes.byref(conv), _lib.dptr(C), _lib.dptr(eps), mh, _lib.dptr(he), _lib.dptr(hr),
+        _lib.dptr(tj), _lib.dptr(tk), ctypes.byref(nh)))
+    k = nh.value
+    return ScfResult(e.value, it.value, bool(conv.value), C, eps, list(he[:k]), list(hr[:k]),
+                     list(tj[:k]), list(tk[:k]))

@@ -1,0 +1,163 @@
+/*
+ * mchip.h -- C ABI of the MI355X-native DF Fock build and SOS-AO-MP2 engine.
+ *
+ * Every entry point returns MCHIP_OK (0) on success and a nonzero status on
+ * failure; mchip_last_error() then returns a thread-local message.  The C++
+ * builder layer (include/mchip_fock.hpp) rethrows such failures as
+ * std::runtime_error, which is how the reference reports every error
+ * (src/main.cpp:115-147).
+ *
+ * Matrices are dense row-major double arrays unless stated otherwise.
+ * Pointers named *_dev are device (HBM) pointers; all other pointers are host.
+ * Device entry points are asynchronous on the context's stream unless they
+ * return host data.
+ */
+#ifndef MCHIP_H
+#define MCHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCHIP_OK 0
+#define MCHIP_ERR 1
+
+const char* mchip_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Basis sets and AO integrals (host).
+ * Replaces desc::cluster_basis + ints::aofactory (src/desc/basis.cpp:278-360,
+ * src/ints/aofactory.cpp:273-310 normalisation, :380-430 libcint setup).
+ * ---------------------------------------------------------------------- */
+typedef struct mchip_basis_s* mchip_basis;
+
+/* One shell per entry; exps/coefs are concatenated per shell (nprim[i]
+ * primitives each, raw contraction coefficients as read from the basis file),
+ * centers is nshell x 3 in bohr, atom gives the owning atom of each shell.
+ * Coefficients are normalised as aofactory.cpp:273-310 does. */
+int mchip_basis_create(int nshell, const int* l, const int* nprim,
+                       const double* exps, const double* coefs,
+                       const double* centers, const int* atom,
+                       mchip_basis* out);
+int mchip_basis_destroy(mchip_basis b);
+int mchip_basis_nbf(mchip_basis b, int* nbf);
+/* Normalised contraction coefficients, same layout as coefs on input. */
+int mchip_basis_norm_coefs(mchip_basis b, double* coefs);
+
+/* Boys function F_0..F_mmax(T) (test hook for the integral engine). */
+int mchip_boys(int mmax, double T, double* F);
+
+/* S, T, V (nbf x nbf); Z/xyz describe natom point charges (bohr).
+ * aofactory::ao_overlap / ao_kinetic / ao_nuclear. */
+int mchip_int1e(mchip_basis b, int natom, const double* Z, const double* xyz,
+                double* S, double* T, double* V);
+/* Coulomb metric (X|Y), naux x naux.  aofactory::ao_2c2e(coulomb). */
+int mchip_int2c2e(mchip_basis aux, double* V);
+/* Schwarz factors: Qao nshell_ao^2 (sqrt sum|(ab|ab)|), Qaux nshell_aux.
+ * aofactory::ao_schwarz / ao_3cschwarz (integrals.cpp:511-689). */
+int mchip_schwarz(mchip_basis ao, mchip_basis aux, double* Qao, double* Qaux);
+/* One shell triple (c|ab), out[m_c][m_a][m_b] (test hook). */
+int mchip_int3c2e_shell(mchip_basis ao, mchip_basis aux, int ia, int ib, int ic,
+                        double* out);
+
+/* ------------------------------------------------------------------------
+ * Three-centre tensor B(X|mu nu), block-sparse (host container).
+ * Replaces dbcsr::btensor<3> core mode (src/dbcsrx/dbcsr_btensor.hpp) and the
+ * screened fill of aoloader::compute (src/ints/aoloader.cpp:341-377).
+ * ---------------------------------------------------------------------- */
+typedef struct mchip_btensor_s* mchip_btensor;
+
+/* Blocks are contiguous shell ranges: ao_blk_shells[i] shells in AO block i.
+ * schwarz_mask is nxb*nbb*nbb bytes (1 = compute the block triple).  After the
+ * fill, blocks with Frobenius norm < filter_eps are dropped. */
+int mchip_btensor_build(mchip_basis ao, mchip_basis aux,
+                        int nbb, const int* ao_blk_shells,
+                        int nxb, const int* aux_blk_shells,
+                        const uint8_t* schwarz_mask, double filter_eps,
+                        mchip_btensor* out);
+/* Build from a dense naux x nbf x nbf array (tests, synthetic inputs). */
+int mchip_btensor_from_dense(int nbb, const int* bsz, int nxb, const int* xsz,
+                             const uint8_t* mask, const double* dense,
+                             mchip_btensor* out);
+int mchip_btensor_destroy(mchip_btensor t);
+/* dims[0..3] = naux, nbf, nxb, nbb; *nelem = stored elements. */
+int mchip_btensor_info(mchip_btensor t, int* dims, int64_t* nelem);
+/* Final block mask after filtering, nxb*nbb*nbb bytes. */
+int mchip_btensor_mask(mchip_btensor t, uint8_t* mask);
+/* Dense copy naux x nbf x nbf (zeros outside the mask). */
+int mchip_btensor_to_dense(mchip_btensor t, double* dense);
+
+/* ------------------------------------------------------------------------
+ * Device context (one HIP stream + rocBLAS handle on one GPU).
+ * Replaces the reference's world/cart (src/megalochem.hpp:13-80): one
+ * process drives one GPU.
+ * ---------------------------------------------------------------------- */
+typedef struct mchip_ctx_s* mchip_ctx;
+typedef struct mchip_dbt_s* mchip_dbt;        /* device-resident B */
+typedef struct mchip_metric_s* mchip_metric;  /* Coulomb metric factors */
+
+int mchip_device_count(int* n);
+int mchip_ctx_create(int device, mchip_ctx* out);
+int mchip_ctx_destroy(mchip_ctx c);
+int mchip_ctx_sync(mchip_ctx c);
+/* The context's hipStream_t (for callers that enqueue their own work). */
+int mchip_ctx_stream(mchip_ctx c, void** stream);
+
+/* Copy a host B to HBM (FP64).  dbcsr::btensor decompress_init/decompress
+ * (dbcsr_btensor.hpp:1133-1348) become no-ops: B stays resident. */
+int mchip_dbt_upload(mchip_ctx c, mchip_btensor host, mchip_dbt* out);
+int mchip_dbt_destroy(mchip_dbt t);
+/* Copy the device slabs back (same layout as the host container). */
+int mchip_dbt_download(mchip_dbt t, double* host);
+
+/* Coulomb metric from host V (naux x naux): V = L L^T, L^-1, V^-1 on the GPU.
+ * Replaces aoloader::invert / math::LLT (aoloader.cpp:14-32, LLT.cpp:8-109). */
+int mchip_metric_create(mchip_ctx c, int naux, const double* V, mchip_metric* out);
+int mchip_metric_destroy(mchip_metric m);
+/* Host copies: Linv column-major lower triangular, Vinv symmetric. */
+int mchip_metric_get(mchip_ctx c, mchip_metric m, double* Linv, double* Vinv);
+/* One-time B <- L^-1 B.  Afterwards J/K calls take m = NULL. */
+int mchip_dbt_transform(mchip_dbt t, mchip_metric m);
+
+/* DF-J: J = B^T V^-1 (B . (pscale*P))  -- DF_J::compute_J (jkdf.cpp:69-190);
+ * pscale = 2 for a closed-shell alpha density (jkdf.cpp:83-90). */
+int mchip_dfj_dev(mchip_dbt t, mchip_metric m, const double* P_dev, double pscale, double* J_dev);
+int mchip_dfj(mchip_dbt t, mchip_metric m, const double* P, double pscale, double* J);
+
+/* DFMO-K: K = -(L^-1 B C)^T (L^-1 B C) for C = nbf x nocc (row-major, leading
+ * dimension ldc), returned negated as the reference does
+ * (DFMO_K::compute_K, jkdf.cpp:221-462, scale at :436-438). */
+int mchip_dfk_mo_dev(mchip_dbt t, mchip_metric m, const double* C_dev, int ldc, int nocc,
+                     int occ_nbatches, double* K_dev);
+int mchip_dfk_mo(mchip_dbt t, mchip_metric m, const double* C, int nocc, int occ_nbatches,
+                 double* K);
+
+/* ------------------------------------------------------------------------
+ * Restricted Hartree-Fock on the GPU (hf::hfmod::compute, src/hf/hfscf.cpp:
+ * 326-552): core guess, F = H + J + K, e = FPS - SPF, E = tr(HP) + tr(FP),
+ * DIIS (diis.hpp), diag via X^T F X.  S and H = T + V are host row-major.
+ * kmode: 0 = DFMO with the metric applied per iteration (reference algorithm),
+ *        1 = DFMO on the transformed B (mchip_dbt_transform).
+ * Outputs C (nbf x nbf row-major), orbital energies and per-iteration
+ * history (energy, RMS error, J and K seconds) up to max_hist entries.
+ * ---------------------------------------------------------------------- */
+int mchip_scf_run(mchip_dbt t, mchip_metric m, int nbf, int nocc, int nbb, const int* bsz,
+                  const double* S, const double* H, double enuc, double threshold,
+                  int max_iter, int do_diis, int diis_start, int diis_min, int diis_max,
+                  double filter_eps, int kmode, int occ_nbatches, double* energy,
+                  int* iterations, int* converged, double* C, double* eps, int max_hist,
+                  double* hist_e, double* hist_rms, double* t_j, double* t_k, int* nhist);
+
+/* Test hook: one v_mfma_f64_16x16x4_f64 on a single wave.  a_lane/b_lane hold
+ * the operand each of the 64 lanes supplies, out[lane*4+e] the accumulator
+ * elements (pins the lane layout the FP64 kernels assume). */
+int mchip_probe_mfma_f64(const double* a_lane, const double* b_lane, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MCHIP_H */

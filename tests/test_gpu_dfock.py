@@ -1,0 +1,141 @@
+"""GPU parity of the plain FP64 DF-J / DFMO-K path against the oracle.
+
+The contraction kernels are compared with the oracle fed the same metric
+factors the GPU produced (rocSOLVER potrf/trtri), so the bar isolates the
+kernels: max |dJ|, |dK| <= 1e-13 relative to max |J|, |K|.  The metric itself
+is pinned separately (V V^-1 = I, L^-1 against LAPACK within cond(V)*eps).
+Block masks are compared bit-exactly.
+"""
+import numpy as np
+import pytest
+
+from chem_amd import desc, ints
+from oracle import dfock
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-13
+EPS = 2.220446049250313e-16
+
+
+def _jtol(V):
+    """J = B^T V^-1 (B P) amplifies summation-order differences by cond(V)."""
+    return max(TOL, 10.0 * np.linalg.cond(V) * EPS)
+
+
+def _relerr(a, b):
+    return float(np.abs(a - b).max() / max(1e-300, np.abs(b).max()))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from chem_amd.engine import Context
+    yield Context(0)
+
+
+def _system(nwater, seed=7, eps=1e-10):
+    atoms = desc.water_cluster(nwater, seed)
+    mol = desc.Molecule(atoms)
+    B = ints.BTensorHost.build(mol, eps)
+    V = ints.int2c2e(mol)
+    return mol, B, V
+
+
+def _random_occ(nbf, nocc, seed):
+    rng = np.random.default_rng(seed)
+    C = rng.standard_normal((nbf, nocc)) * 0.3
+    return C, C @ C.T
+
+
+@pytest.mark.parametrize("nwater", [1, 3])
+def test_dfj_dfk_plain_vs_oracle(ctx, nwater):
+    from chem_amd.engine import DeviceB, Metric
+    mol, Bh, V = _system(nwater)
+    print("case: water cluster n=%d nbf=%d naux=%d" % (nwater, mol.nbf, mol.naux), flush=True)
+    Bd = Bh.dense()
+    dB = DeviceB(ctx, Bh)
+    m = Metric(ctx, V)
+    Linv, Vinv = m.get()
+    C, P = _random_occ(mol.nbf, mol.nocc, 11 + nwater)
+    Jref = dfock.df_j(Bd, Vinv, P, bsz=mol.b)
+    Kref = dfock.dfmo_k(Bd, Linv.T, C, bsz=mol.b, xsz=mol.x, osz=mol.oa())
+    J = dB.dfj(P, m)
+    K = dB.dfk_mo(C, m)
+    print("relerr J %.3e K %.3e cond %.3e" % (_relerr(J, Jref), _relerr(K, Kref), np.linalg.cond(V)))
+    assert _relerr(J, Jref) < _jtol(V)
+    assert _relerr(K, Kref) < _jtol(V)
+    assert np.array_equal(K, K.T)
+    # occupied batching does not change K beyond rounding
+    K3 = dB.dfk_mo(C, m, occ_nbatches=3)
+    assert _relerr(K3, K) < TOL
+
+
+def test_metric_factors(ctx):
+    from chem_amd.engine import Metric
+    mol, _, V = _system(1)
+    m = Metric(ctx, V)
+    Linv, Vinv = m.get()
+    Vinv_ref, Linv_ref, _ = dfock.metric_factors(V)
+    cond = np.linalg.cond(V)
+    assert np.array_equal(np.triu(Linv, 1), np.zeros_like(Linv))
+    assert _relerr(Linv, Linv_ref) < 10 * cond * 2.2e-16
+    assert np.abs(Linv @ V @ Linv.T - np.eye(V.shape[0])).max() < 1e-9
+    assert np.abs(Vinv @ V - np.eye(V.shape[0])).max() < 1e-8
+
+
+def test_transformed_B_matches_plain(ctx):
+    from chem_amd.engine import DeviceB, Metric
+    mol, Bh, V = _system(2)
+    C, P = _random_occ(mol.nbf, mol.nocc, 5)
+    m = Metric(ctx, V)
+    dB = DeviceB(ctx, Bh)
+    J0 = dB.dfj(P, m)
+    K0 = dB.dfk_mo(C, m)
+    dB.transform(m)
+    J1 = dB.dfj(P)
+    K1 = dB.dfk_mo(C)
+    # same L^-1, different association: (L^-1 B) C against L^-1 (B C)
+    print("relerr J %.3e K %.3e" % (_relerr(J1, J0), _relerr(K1, K0)))
+    assert _relerr(J1, J0) < _jtol(V)
+    assert _relerr(K1, K0) < _jtol(V)
+
+
+@pytest.mark.parametrize("nocc", [1, 5, 17, 70])
+def test_synthetic_sparse_B(ctx, nocc):
+    """Random B with a random block mask, including empty pairs and slabs."""
+    from chem_amd.engine import DeviceB, Metric
+    rng = np.random.default_rng(100 + nocc)
+    bsz = [14, 5, 5, 3, 14, 1, 7]
+    xsz = [23, 70, 9, 40]
+    nbf, naux = sum(bsz), sum(xsz)
+    mask = (rng.random((len(xsz), len(bsz), len(bsz))) < 0.6).astype(np.uint8)
+    mask = np.maximum(mask, mask.transpose(0, 2, 1))
+    mask[:, 5, :] = 0          # slab with no significant pair
+    mask[:, :, 5] = 0
+    boff = np.concatenate([[0], np.cumsum(bsz)])
+    xoff = np.concatenate([[0], np.cumsum(xsz)])
+    dense = rng.standard_normal((naux, nbf, nbf))
+    dense = 0.5 * (dense + dense.transpose(0, 2, 1))
+    for x in range(len(xsz)):
+        for a in range(len(bsz)):
+            for b in range(len(bsz)):
+                if not mask[x, a, b]:
+                    dense[xoff[x]:xoff[x + 1], boff[a]:boff[a + 1], boff[b]:boff[b + 1]] = 0
+    Bh = ints.BTensorHost.from_dense(dense, xsz, bsz, mask)
+    assert np.array_equal(Bh.mask(), mask)
+    assert np.array_equal(Bh.dense(), dense)
+    A = rng.standard_normal((naux, naux))
+    V = A @ A.T + naux * np.eye(naux)
+    C = rng.standard_normal((nbf, nocc))
+    P = C @ C.T
+    dB = DeviceB(ctx, Bh)
+    m = Metric(ctx, V)
+    Linv, Vinv = m.get()
+    J = dB.dfj(P, m)
+    K = dB.dfk_mo(C, m)
+    Jref = dfock.df_j(dense, Vinv, P)
+    Kref = dfock.dfmo_k(dense, Linv.T, C)
+    print("relerr J %.3e K %.3e" % (_relerr(J, Jref), _relerr(K, Kref)))
+    assert _relerr(J, Jref) < _jtol(V)
+    assert _relerr(K, Kref) < _jtol(V)
+    assert np.array_equal(dB.download(), dB.download())
