@@ -85,6 +85,7 @@ _SIGS = [
     ("mchip_btensor_info", c_int, [c_void_p, P_int, P_int64]),
     ("mchip_btensor_mask", c_int, [c_void_p, P_u8]),
     ("mchip_btensor_to_dense", c_int, [c_void_p, P_double]),
+    ("mchip_btensor_dense_rows", c_int, [c_void_p, c_int, c_int, P_double]),
     ("mchip_device_count", c_int, [P_int]),
     ("mchip_ctx_create", c_int, [c_int, ctypes.POINTER(c_void_p)]),
     ("mchip_ctx_destroy", c_int, [c_void_p]),
@@ -106,6 +107,8 @@ _SIGS = [
                               c_int, c_int, P_double, P_int, P_int, P_double, P_double, c_int,
                               P_double, P_double, P_double, P_double, P_int]),
     ("mchip_probe_mfma_f64", c_int, [P_double, P_double, P_double]),
+    ("mchip_ctx_set_timing", c_int, [c_void_p, c_int]),
+    ("mchip_ctx_phase_times", c_int, [c_void_p, P_double, c_int]),
 ]
 
 

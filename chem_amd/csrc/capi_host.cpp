@@ -221,3 +221,24 @@ extern "C" int mchip_btensor_to_dense(mchip_btensor t, double* dense)
   }
   MCHIP_CATCH
 }
+
+extern "C" int mchip_btensor_dense_rows(mchip_btensor t, int x0, int x1, double* dense)
+{
+  MCHIP_TRY
+  const BLayout& L = t->layout;
+  if (x0 < 0 || x1 > L.naux || x0 > x1) throw std::runtime_error("mchip_btensor_dense_rows: bad range");
+  const int nbf = L.nbf;
+  std::memset(dense, 0, sizeof(double) * (size_t)(x1 - x0) * nbf * nbf);
+  for (int n = 0; n < L.nbb; ++n) {
+    const double* slab = t->data.data() + L.slab_off[n];
+    for (int p = L.pair_ptr[n]; p < L.pair_ptr[n + 1]; ++p) {
+      const int mb = L.pair_m[p], col = L.pair_col[p];
+      for (int xi = x0; xi < x1; ++xi)
+        for (int nu = 0; nu < L.bsz[n]; ++nu)
+          for (int mu = 0; mu < L.bsz[mb]; ++mu)
+            dense[((int64_t)(xi - x0) * nbf + L.boff[mb] + mu) * nbf + L.boff[n] + nu] =
+                slab[((int64_t)xi * L.bsz[n] + nu) * L.ncol[n] + col + mu];
+    }
+  }
+  MCHIP_CATCH
+}

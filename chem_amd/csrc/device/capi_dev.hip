@@ -81,6 +81,21 @@ extern "C" int mchip_ctx_stream(mchip_ctx c, void** stream)
   MCHIP_CATCH
 }
 
+extern "C" int mchip_ctx_set_timing(mchip_ctx c, int on)
+{
+  MCHIP_TRY
+  c->c.timing = on != 0;
+  for (double& v : c->c.phase) v = 0.0;
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_ctx_phase_times(mchip_ctx c, double* out, int n)
+{
+  MCHIP_TRY
+  for (int i = 0; i < n && i < 8; ++i) out[i] = c->c.phase[i];
+  MCHIP_CATCH
+}
+
 extern "C" int mchip_dbt_upload(mchip_ctx c, mchip_btensor host, mchip_dbt* out)
 {
   MCHIP_TRY

@@ -86,7 +86,33 @@ struct Ctx {
   DevBuf scratch_partial;                   // split-K partial sums
   DevBuf info;                              // rocsolver info
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // optional per-phase kernel timing (HIP events on this stream), seconds
+  bool timing = false;
+  hipEvent_t tev[2] = {nullptr, nullptr};
+  double phase[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
+
+// Accumulates the duration of the enclosed stream work into c.phase[slot].
+struct PhaseTimer {
+  Ctx& c;
+  int slot;
+  PhaseTimer(Ctx& cc, int s) : c(cc), slot(s)
+  {
+    if (c.timing) HIP_CHECK(hipEventRecord(c.tev[0], c.stream));
+  }
+  ~PhaseTimer() noexcept(false)
+  {
+    if (!c.timing) return;
+    HIP_CHECK(hipEventRecord(c.tev[1], c.stream));
+    HIP_CHECK(hipEventSynchronize(c.tev[1]));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, c.tev[0], c.tev[1]));
+    c.phase[slot] += ms * 1e-3;
+  }
+};
+
+// phase slots
+enum { PH_K1 = 0, PH_K2 = 1, PH_K3 = 2, PH_J = 3, PH_KINC = 4, PH_SPLIT = 5 };
 
 enum class Prec : int { F64 = 0, F32 = 1, BF16X3 = 2 };
 

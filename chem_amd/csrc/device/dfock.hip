@@ -36,6 +36,8 @@ void ctx_init(Ctx& c, int device)
   c.info.alloc(sizeof(int) * 4);
   HIP_CHECK(hipEventCreate(&c.ev0));
   HIP_CHECK(hipEventCreate(&c.ev1));
+  HIP_CHECK(hipEventCreate(&c.tev[0]));
+  HIP_CHECK(hipEventCreate(&c.tev[1]));
 }
 
 void ctx_free(Ctx& c)
@@ -49,6 +51,8 @@ void ctx_free(Ctx& c)
   if (c.stream) (void)hipStreamDestroy(c.stream);
   if (c.ev0) (void)hipEventDestroy(c.ev0);
   if (c.ev1) (void)hipEventDestroy(c.ev1);
+  for (auto& e : c.tev)
+    if (e) (void)hipEventDestroy(e);
   c.blas = nullptr;
   c.stream = nullptr;
 }
@@ -210,6 +214,7 @@ void transform_B(Ctx& c, DevBT& t, const Metric& m)
 
 void dfj(Ctx& c, DevBT& t, const Metric* m, const double* P, double* J, double pscale)
 {
+  PhaseTimer timer(c, PH_J);
   const BLayout& L = t.L;
   if (!t.transformed && !m) throw std::runtime_error("DF-J needs the metric for an untransformed B");
   const int naux = L.naux, nbf = L.nbf, nbb = L.nbb;
@@ -286,9 +291,13 @@ void dfk_mo(Ctx& c, DevBT& t, const Metric* m, const double* C, int ldc, int noc
     const int w = std::min(per, nocc - i0);
     double* HT1 = c.scratch_a.as<double>();
     // K1: HT1[x][nu][i] = sum_mu B(x|mu nu) C(mu, i0+i)
-    slab_gemm(c, t, C + i0, ldc, w, HT1, w, 1.0);
+    {
+      PhaseTimer tm(c, PH_K1);
+      slab_gemm(c, t, C + i0, ldc, w, HT1, w, 1.0);
+    }
     const double* HT = HT1;
     if (!t.transformed) {
+      PhaseTimer tm(c, PH_K2);
       // K2: HT2 = L^-1 HT1 over x; column-major (W x naux) view times (L^-1)^T
       double* HT2 = c.scratch_b.as<double>();
       const int64_t W = (int64_t)nbf * w;
@@ -299,6 +308,7 @@ void dfk_mo(Ctx& c, DevBT& t, const Metric* m, const double* C, int ldc, int noc
       HT = HT2;
     }
     // K3: K = -sum_{x,i} HT[x][m][i] HT[x][n][i]
+    PhaseTimer tm(c, PH_K3);
     seg_syrk(c, HT, HT, naux, (int64_t)nbf * w, w, w, nbf, -1.0, K, partial, nb, b, b == nb - 1);
   }
 }

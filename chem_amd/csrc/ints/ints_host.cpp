@@ -157,6 +157,10 @@ struct PrimPair {
   double Ex[ESIZE], Ey[ESIZE], Ez[ESIZE];
 };
 
+// Primitive pairs with exp(-mu R_AB^2) below e^-PRIM_CUT (about 1e-18) are
+// skipped in the two-electron integrals.
+constexpr double PRIM_CUT = 41.4;
+
 void make_pairs(const Shell& a, const Shell& b, std::vector<PrimPair>& out)
 {
   out.clear();
@@ -168,6 +172,7 @@ void make_pairs(const Shell& a, const Shell& b, std::vector<PrimPair>& out)
       double al = a.alpha[i], be = b.alpha[j];
       pp.p = al + be;
       double mu = al * be / pp.p;
+      if (mu * r2 > PRIM_CUT) continue;
       pp.K = a.coeff[i] * b.coeff[j] * std::exp(-mu * r2);
       for (int d = 0; d < 3; ++d) pp.P[d] = (al * a.O[d] + be * b.O[d]) / pp.p;
       double oo2p = 0.5 / pp.p;
@@ -289,8 +294,9 @@ void ints_1e(const Basis& bas, const std::vector<Charge>& charges,
       const Shell& b = bas.shells[ib];
       const int la = a.l, lb = b.l;
       const int nca = ncart(la), ncb = ncart(lb);
-      double s_c[36], t_c[36], v_c[36];
-      for (int k = 0; k < 36; ++k) s_c[k] = t_c[k] = v_c[k] = 0.0;
+      constexpr int NC2 = ((LMAX + 1) * (LMAX + 2) / 2) * ((LMAX + 1) * (LMAX + 2) / 2);
+      double s_c[NC2], t_c[NC2], v_c[NC2];
+      for (int k = 0; k < NC2; ++k) s_c[k] = t_c[k] = v_c[k] = 0.0;
       double AB[3] = {a.O[0] - b.O[0], a.O[1] - b.O[1], a.O[2] - b.O[2]};
       double r2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
       for (size_t i = 0; i < a.alpha.size(); ++i)
@@ -378,7 +384,7 @@ void ints_2c2e(const Basis& aux, double* V_out)
       const Shell& c = aux.shells[ic];
       const int la = a.l, lc = c.l, L = la + lc;
       const int NHa = nherm(la), NHc = nherm(lc);
-      double out[7 * 7] = {0};
+      double out[(2 * LMAX + 1) * (2 * LMAX + 1)] = {0};
       for (size_t i = 0; i < a.alpha.size(); ++i) {
         aux_ket_hermite(la, a.alpha[i], a.coeff[i], Ha);
         // bra side must not carry the (-1)^{t+u+v} sign
@@ -407,15 +413,15 @@ void ints_2c2e(const Basis& aux, double* V_out)
                           s += ha * Hc[mc * NHc + herm_index(t2, u2, v2)] *
                                R[herm_index(t + t2, u + u2, v + v2)];
                   }
-              out[ma * 7 + mc] += pref * s;
+              out[ma * (2 * LMAX + 1) + mc] += pref * s;
             }
         }
       }
       const int oa = aux.shell_off[ia], oc = aux.shell_off[ic];
       for (int ma = 0; ma < nsph(la); ++ma)
         for (int mc = 0; mc < nsph(lc); ++mc) {
-          V_out[(size_t)(oa + ma) * n + (oc + mc)] = out[ma * 7 + mc];
-          V_out[(size_t)(oc + mc) * n + (oa + ma)] = out[ma * 7 + mc];
+          V_out[(size_t)(oa + ma) * n + (oc + mc)] = out[ma * (2 * LMAX + 1) + mc];
+          V_out[(size_t)(oc + mc) * n + (oa + ma)] = out[ma * (2 * LMAX + 1) + mc];
         }
     }
   }
@@ -540,10 +546,11 @@ void shell3c(const Shell& a, const Shell& b, const std::vector<PrimPair>& pairs,
   const int lab = la + lb, L = lab + lc;
   const int nca = ncart(la), ncb = ncart(lb), nmc = nsph(lc);
   const int NHab = nherm(lab), NHc = nherm(lc);
-  double acc[7 * 36];
+  constexpr int NC = (LMAX + 1) * (LMAX + 2) / 2;
+  double acc[(2 * LMAX + 1) * NC * NC];
   for (int i = 0; i < nmc * nca * ncb; ++i) acc[i] = 0.0;
   double W[(3 * LMAX + 1) * nherm(3 * LMAX)], R[nherm(3 * LMAX)];
-  double Sm[7 * nherm(2 * LMAX)];
+  double Sm[(2 * LMAX + 1) * nherm(2 * LMAX)];
   for (int ic = 0; ic < ncprim; ++ic) {
     const AuxPrim& c = cprims[ic];
     for (const PrimPair& pp : pairs) {
@@ -573,7 +580,8 @@ void shell3c(const Shell& a, const Shell& b, const std::vector<PrimPair>& pairs,
         for (int kb = 0; kb < ncb; ++kb) {
           int bx, by, bz;
           cart_exps(lb, kb, bx, by, bz);
-          double r[7] = {0, 0, 0, 0, 0, 0, 0};
+          double r[2 * LMAX + 1];
+          for (int m = 0; m < nmc; ++m) r[m] = 0.0;
           for (int t = 0; t <= ax + bx; ++t) {
             double ex = pp.Ex[eidx(ax, bx, t)];
             for (int u = 0; u <= ay + by; ++u) {
@@ -698,7 +706,7 @@ void build_B(const Basis& ao, const Basis& aux,
 #pragma omp parallel
   {
     std::vector<PrimPair> pairs;
-    double out[7 * 25];
+    double out[(2 * LMAX + 1) * (2 * LMAX + 1) * (2 * LMAX + 1)];
 #pragma omp for schedule(dynamic, 4)
     for (size_t t = 0; t < tasks.size(); ++t) {
       const Task& tk = tasks[t];

@@ -36,6 +36,15 @@ class Context:
     def sync(self):
         _lib.check(_lib.lib().mchip_ctx_sync(self.h))
 
+    def set_timing(self, on):
+        _lib.check(_lib.lib().mchip_ctx_set_timing(self.h, int(bool(on))))
+
+    def phase_times(self):
+        """Accumulated seconds: K1, K2, K3, J, K_inc, split, -, -."""
+        out = np.zeros(8)
+        _lib.check(_lib.lib().mchip_ctx_phase_times(self.h, _lib.dptr(out), 8))
+        return out
+
     @property
     def stream(self):
         s = ctypes.c_void_p()

@@ -103,6 +103,18 @@ class BTensorHost:
         _lib.check(_lib.lib().mchip_btensor_to_dense(self.h, _lib.dptr(out)))
         return out
 
+    def dense_rows(self, x0, x1):
+        """Dense B(X|mu nu) for aux functions x0 <= X < x1."""
+        out = np.zeros((x1 - x0, self.nbf, self.nbf))
+        _lib.check(_lib.lib().mchip_btensor_dense_rows(self.h, x0, x1, _lib.dptr(out)))
+        return out
+
+    def nnz_kept(self):
+        """Elements inside kept blocks (the algorithmic nnz of B)."""
+        m = self.mask().astype(np.int64)
+        return int(np.einsum("x,xmn,m,n->", np.array(self.xsz), m, np.array(self.bsz),
+                             np.array(self.bsz)))
+
     def __del__(self):
         try:
             if getattr(self, "h", None):

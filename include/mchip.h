@@ -89,6 +89,8 @@ int mchip_btensor_info(mchip_btensor t, int* dims, int64_t* nelem);
 int mchip_btensor_mask(mchip_btensor t, uint8_t* mask);
 /* Dense copy naux x nbf x nbf (zeros outside the mask). */
 int mchip_btensor_to_dense(mchip_btensor t, double* dense);
+/* Dense copy of aux rows x0 <= X < x1 ((x1-x0) x nbf x nbf). */
+int mchip_btensor_dense_rows(mchip_btensor t, int x0, int x1, double* dense);
 
 /* ------------------------------------------------------------------------
  * Device context (one HIP stream + rocBLAS handle on one GPU).
@@ -105,6 +107,13 @@ int mchip_ctx_destroy(mchip_ctx c);
 int mchip_ctx_sync(mchip_ctx c);
 /* The context's hipStream_t (for callers that enqueue their own work). */
 int mchip_ctx_stream(mchip_ctx c, void** stream);
+
+/* Per-phase kernel timing with HIP events on the context's stream (adds a
+ * synchronisation per phase; for measurement only).  Enabling resets the
+ * accumulators.  Slots: 0 K1 (B.C), 1 K2 (metric), 2 K3 (SYRK),
+ * 3 J total, 4 incremental K, 5 operand splitting. */
+int mchip_ctx_set_timing(mchip_ctx c, int on);
+int mchip_ctx_phase_times(mchip_ctx c, double* out, int n);
 
 /* Copy a host B to HBM (FP64).  dbcsr::btensor decompress_init/decompress
  * (dbcsr_btensor.hpp:1133-1348) become no-ops: B stays resident. */
