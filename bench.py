@@ -1,11 +1,20 @@
 """Benchmark: DF-K seconds per SCF iteration on one MI355X.
 
 Workload (default --config c2): seeded 64-water cluster, cc-pVDZ / cc-pVDZ-jkfit,
-atomic blocks, Schwarz screening at 1e-10.  A "step" is one DF-K build at the
-converged orbitals.  The line reports the chosen form's seconds per build
-(value), the plain FP64 DFMO path beside it, the SCF energy difference between
-the two, the roofline of the dominant kernel (K1, the B.C slab GEMM) measured
-with HIP events on the library's stream, and the oracle timed on the host.
+atomic blocks, Schwarz screening at 1e-10.
+
+Plain path: the reference's DFMO algorithm in FP64 (K1 = B.C, K2 = L^-1 per
+build, K3 = SYRK).  Chosen form: B~ = L^-1 B made once, stored in FP32; every
+SCF iteration builds K with FP32 MFMA (FP64 reduction), and one FP64 K at the
+converged orbitals gives the reported energy.
+
+A step is one SCF iteration's DF-K work in the chosen form.  The timed region
+runs `steps` FP32 builds plus the one final FP64 build at the converged
+orbitals, so value = (steps * t_fp32 + t_fp64_final) / steps; steps defaults
+to the chosen SCF's iteration count.  The line also carries the plain FP64
+seconds per build, |dE| between the two SCFs, the roofline of the dominant
+kernel (K1, the B.C slab GEMM) from HIP events on the library stream, and the
+oracle timed on the host.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c0|c1|c2]
 """
@@ -26,6 +35,7 @@ CONFIGS = {
     "c2": dict(nwater=64, seed=1234),
 }
 MFMA_F64_PEAK = 78.6e12
+MFMA_F32_PEAK = 157.3e12
 HBM_PEAK = 8.0e12
 
 
@@ -62,7 +72,7 @@ def cpu_baseline(mol, Bh, Linv, C_occ, budget_s=20.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=0, help="0: the chosen SCF's iteration count")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--scf-thresh", type=float, default=1e-6)
@@ -111,28 +121,44 @@ def main():
         ctx.sync()
         return (time.perf_counter() - t) / steps
 
-    plain_s = time_builds(lambda: dB.dfk_mo(C_occ, met), max(1, args.steps // 2), 1)
+    plain_s = time_builds(lambda: dB.dfk_mo(C_occ, met), 2, 1)
 
-    # chosen form: B~ = L^-1 B once, FP64 DFMO without the per-build metric
+    # chosen form: B~ = L^-1 B once (FP64), FP32 copy, mixed SCF
     dB.transform(met)
-    chosen = run_rhf(mol, dB, None, S, T + V, threshold=args.scf_thresh, kmode="dfmo_transformed")
+    dB.make_f32()
+    t2 = time.perf_counter()
+    chosen = run_rhf(mol, dB, None, S, T + V, threshold=args.scf_thresh, kmode="mixed_f32")
     dE = chosen.energy - ref.energy
-    log("chosen SCF: E=%.10f iters=%d dE=%.3e" % (chosen.energy, chosen.iterations, dE))
+    log("chosen SCF: E=%.10f (FP32-K iterate %.10f) iters=%d dE=%.3e (%.1f s)" % (
+        chosen.energy, chosen.energy_iter, chosen.iterations, dE, time.perf_counter() - t2))
+    steps = args.steps or max(1, chosen.iterations)
+    Cc = np.ascontiguousarray(chosen.C[:, :mol.nocc])
+    for _ in range(args.warmup):
+        dB.dfk_mo_f32(Cc)
+        dB.dfk_mo(Cc)
+    ctx.sync()
+    t = time.perf_counter()
+    for _ in range(steps):
+        dB.dfk_mo_f32(Cc)
+    dB.dfk_mo(Cc)
+    ctx.sync()
+    step_s = (time.perf_counter() - t) / steps
+    t64 = time_builds(lambda: dB.dfk_mo(Cc), 2, 0)
+    t32 = time_builds(lambda: dB.dfk_mo_f32(Cc), 3, 0)
 
-    step_s = time_builds(lambda: dB.dfk_mo(C_occ), args.steps, args.warmup)
-
-    # roofline of the dominant kernel (K1) from HIP events on the library stream
+    # roofline of the dominant kernel (K1, FP32 MFMA) from HIP events on the library stream
     ctx.set_timing(True)
     nrep = 3
     for _ in range(nrep):
-        dB.dfk_mo(C_occ)
+        dB.dfk_mo_f32(Cc)
     ph = ctx.phase_times() / nrep
     ctx.set_timing(False)
     k1_flops = 2.0 * nnz * mol.nocc
     achieved = k1_flops / ph[0]
-    roof = dict(bound="mfma", achieved=achieved / 1e12, peak=MFMA_F64_PEAK / 1e12, unit="TFLOP/s",
-                frac=achieved / MFMA_F64_PEAK, traffic=None,
-                kernel="k_slab_gemm_f64", kernel_s=ph[0], k3_s=ph[2])
+    roof = dict(bound="mfma", achieved=achieved / 1e12, peak=MFMA_F32_PEAK / 1e12, unit="TFLOP/s",
+                frac=achieved / MFMA_F32_PEAK, traffic=None,
+                kernel="k_slab_gemm_f32", kernel_s=ph[0], k3_s=ph[2],
+                k3_achieved=(mol.naux * mol.nocc * mol.nbf ** 2) / ph[2] / 1e12)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -145,20 +171,23 @@ def main():
         "value": step_s,
         "unit": "s/iter",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
         "ms_per_step": step_s * 1e3,
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 (K builds) + f64 (final K, J, SCF)",
         "data": "synthetic seeded water cluster, real integrals",
         "config": {"workload": "DF-K build, %d-water cluster cc-pVDZ/cc-pVDZ-jkfit (%s)" % (
             cfg["nwater"], args.config), "nbf": mol.nbf, "naux": mol.naux, "nocc": mol.nocc,
             "nnz_B": nnz, "screen_eps": 1e-10},
         "plain_fp64_s_per_iter": plain_s,
         "speedup_vs_plain": plain_s / step_s,
+        "chosen_fp32_build_s": t32,
+        "transformed_fp64_build_s": t64,
         "dE_scf": dE,
+        "scf_iters": {"plain": ref.iterations, "chosen": chosen.iterations},
         "E_fp64": ref.energy,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -22,6 +22,7 @@ c_char_p = ctypes.c_char_p
 P_int = ctypes.POINTER(ctypes.c_int)
 P_double = ctypes.POINTER(ctypes.c_double)
 P_u8 = ctypes.POINTER(ctypes.c_uint8)
+P_float = ctypes.POINTER(ctypes.c_float)
 P_int64 = ctypes.POINTER(ctypes.c_int64)
 
 
@@ -52,6 +53,11 @@ def dptr(a):
         return None
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(P_double)
+
+
+def fptr(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(P_float)
 
 
 def iptr(a):
@@ -86,6 +92,8 @@ _SIGS = [
     ("mchip_btensor_mask", c_int, [c_void_p, P_u8]),
     ("mchip_btensor_to_dense", c_int, [c_void_p, P_double]),
     ("mchip_btensor_dense_rows", c_int, [c_void_p, c_int, c_int, P_double]),
+    ("mchip_laplace_minimax", c_int, [c_int, c_double, c_double, P_double, P_double, c_int,
+                                      P_double, P_double, P_double, P_double]),
     ("mchip_device_count", c_int, [P_int]),
     ("mchip_ctx_create", c_int, [c_int, ctypes.POINTER(c_void_p)]),
     ("mchip_ctx_destroy", c_int, [c_void_p]),
@@ -107,7 +115,12 @@ _SIGS = [
                               c_int, c_int, P_double, P_int, P_int, P_double, P_double, c_int,
                               P_double, P_double, P_double, P_double, P_int]),
     ("mchip_probe_mfma_f64", c_int, [P_double, P_double, P_double]),
+    ("mchip_probe_mfma_lp", c_int, [c_int, P_float, P_float, P_float]),
     ("mchip_ctx_set_timing", c_int, [c_void_p, c_int]),
+    ("mchip_dbt_make_f32", c_int, [c_void_p]),
+    ("mchip_dfk_mo_f32_dev", c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
+    ("mchip_dfk_mo_f32", c_int, [c_void_p, P_double, c_int, c_int, P_double]),
+    ("mchip_scf_last_extra", c_int, [P_double, P_double]),
     ("mchip_ctx_phase_times", c_int, [c_void_p, P_double, c_int]),
 ]
 

@@ -223,6 +223,48 @@ extern "C" int mchip_dfk_mo(mchip_dbt t, mchip_metric m, const double* C, int no
 
 #include "scf.hpp"
 
+static thread_local double last_scf_extra[2] = {0.0, 0.0};
+
+extern "C" int mchip_scf_last_extra(double* energy_iter, double* t_final_k)
+{
+  MCHIP_TRY
+  *energy_iter = last_scf_extra[0];
+  *t_final_k = last_scf_extra[1];
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dbt_make_f32(mchip_dbt t)
+{
+  MCHIP_TRY
+  need_device(t->ctx);
+  dbt_make_f32(t->ctx->c, t->t);
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dfk_mo_f32_dev(mchip_dbt t, const double* C_dev, int ldc, int nocc,
+                                    int occ_nbatches, double* K_dev)
+{
+  MCHIP_TRY
+  need_device(t->ctx);
+  dfk_mo_f32(t->ctx->c, t->t, C_dev, ldc, nocc, K_dev, occ_nbatches);
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dfk_mo_f32(mchip_dbt t, const double* C, int nocc, int occ_nbatches, double* K)
+{
+  MCHIP_TRY
+  need_device(t->ctx);
+  Ctx& c = t->ctx->c;
+  const int nbf = t->t.L.nbf;
+  const size_t kb = (size_t)nbf * nbf * sizeof(double);
+  DevBuf dC((size_t)nbf * std::max(nocc, 1) * sizeof(double)), dK(kb);
+  if (nocc > 0) HIP_CHECK(hipMemcpyAsync(dC.p, C, (size_t)nbf * nocc * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  dfk_mo_f32(c, t->t, dC.as<double>(), nocc, nocc, dK.as<double>(), occ_nbatches);
+  HIP_CHECK(hipMemcpyAsync(K, dK.p, kb, hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  MCHIP_CATCH
+}
+
 extern "C" int mchip_scf_run(mchip_dbt t, mchip_metric m, int nbf, int nocc, int nbb,
                              const int* bsz, const double* S, const double* H, double enuc,
                              double threshold, int max_iter, int do_diis, int diis_start,
@@ -254,17 +296,27 @@ extern "C" int mchip_scf_run(mchip_dbt t, mchip_metric m, int nbf, int nocc, int
   if (kmode == 0 && !met) throw std::runtime_error("mchip_scf_run: plain K needs the metric");
   if (kmode == 1 && !t->t.transformed) throw std::runtime_error("mchip_scf_run: B is not transformed");
   const Metric* jmet = t->t.transformed ? nullptr : met;
-  KBuildFn kb;
+  KBuildFn kb, kfinal;
+  auto k64 = [&](int, const double*, const double* Cocc, double* K) {
+    dfk_mo(c, t->t, t->t.transformed ? nullptr : met, Cocc, nocc, nocc, K, occ_nbatches);
+  };
   if (kmode == 0 || kmode == 1) {
+    kb = k64;
+  }
+  else if (kmode == 2) {
+    if (!t->t.transformed || !t->t.data32.p) throw std::runtime_error("mchip_scf_run: mixed mode needs the transformed B and its FP32 copy");
     kb = [&](int, const double*, const double* Cocc, double* K) {
-      dfk_mo(c, t->t, t->t.transformed ? nullptr : met, Cocc, nocc, nocc, K, occ_nbatches);
+      dfk_mo_f32(c, t->t, Cocc, nocc, nocc, K, occ_nbatches);
     };
+    kfinal = k64;
   }
   else {
     throw std::runtime_error("mchip_scf_run: unknown K mode");
   }
   ScfOutput out;
-  scf_run(c, t->t, jmet, in, out, kb);
+  scf_run(c, t->t, jmet, in, out, kb, kfinal);
+  last_scf_extra[0] = out.energy_iter;
+  last_scf_extra[1] = out.t_final_k;
   *energy = out.energy;
   *iterations = out.iterations;
   *converged = out.converged ? 1 : 0;

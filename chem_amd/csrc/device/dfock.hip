@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include "dfock.hpp"
 #include "kernels.hpp"
@@ -130,9 +131,10 @@ void slab_gemm(Ctx& c, DevBT& t, const double* G, int ldg, int ng, double* out, 
 
 // C(m,n) = alpha * sum_s sum_k A_s(m,k) B_s(n,k), symmetric output.  When B
 // differs from A the symmetrised sum G + G^T is returned.
-void seg_syrk(Ctx& c, const double* A, const double* B, int nseg, int64_t sstride, int lda,
-              int klen, int n, double alpha, double* out, DevBuf& partial_buf, int nbatch,
-              int batch, bool reduce)
+template <class T>
+static void seg_syrk_t(Ctx& c, const T* A, const T* B, int nseg, int64_t sstride, int lda,
+                       int klen, int n, double alpha, double* out, DevBuf& partial_buf, int nbatch,
+                       int batch, bool reduce)
 {
   const bool sym = (A == B);
   const int nt = (n + SG_BM - 1) / SG_BM;
@@ -149,8 +151,12 @@ void seg_syrk(Ctx& c, const double* A, const double* B, int nseg, int64_t sstrid
   if (batch == 0) partial_buf.ensure(need);
   double* partial = partial_buf.as<double>() + (size_t)batch * nch * n * n;
   DevBuf tiles = upload_vec(tl, c.stream);
-  hipLaunchKernelGGL(k_seg_gemm_nt_f64, dim3(ntile * nch), dim3(256), 0, c.stream, A, B, nseg,
-                     sstride, lda, klen, n, tiles.as<int2>(), ntile, per, partial);
+  if constexpr (std::is_same<T, double>::value)
+    hipLaunchKernelGGL(k_seg_gemm_nt_f64, dim3(ntile * nch), dim3(256), 0, c.stream, A, B, nseg,
+                       sstride, lda, klen, n, tiles.as<int2>(), ntile, per, partial);
+  else
+    hipLaunchKernelGGL(k_seg_gemm_nt_f32, dim3(ntile * nch), dim3(256), 0, c.stream, A, B, nseg,
+                       sstride, lda, klen, n, tiles.as<int2>(), ntile, per, partial);
   HIP_CHECK(hipGetLastError());
   if (reduce) {
     hipLaunchKernelGGL(k_reduce_sym, dim3(nblk((int64_t)n * n, 256)), dim3(256), 0, c.stream,
@@ -158,6 +164,14 @@ void seg_syrk(Ctx& c, const double* A, const double* B, int nseg, int64_t sstrid
     HIP_CHECK(hipGetLastError());
   }
   HIP_CHECK(hipStreamSynchronize(c.stream));   // tile table lifetime
+}
+
+void seg_syrk(Ctx& c, const double* A, const double* B, int nseg, int64_t sstride, int lda,
+              int klen, int n, double alpha, double* out, DevBuf& partial_buf, int nbatch,
+              int batch, bool reduce)
+{
+  seg_syrk_t<double>(c, A, B, nseg, sstride, lda, klen, n, alpha, out, partial_buf, nbatch, batch,
+                     reduce);
 }
 
 void metric_create(Ctx& c, int naux, const double* V_host, Metric& m)
@@ -310,6 +324,64 @@ void dfk_mo(Ctx& c, DevBT& t, const Metric* m, const double* C, int ldc, int noc
     // K3: K = -sum_{x,i} HT[x][m][i] HT[x][n][i]
     PhaseTimer tm(c, PH_K3);
     seg_syrk(c, HT, HT, naux, (int64_t)nbf * w, w, w, nbf, -1.0, K, partial, nb, b, b == nb - 1);
+  }
+}
+
+}  // namespace mchip
+
+namespace mchip {
+
+void dbt_make_f32(Ctx& c, DevBT& t)
+{
+  if (!t.transformed) throw std::runtime_error("the FP32 copy is made from the transformed B");
+  t.data32.alloc((size_t)t.L.total * sizeof(float));
+  if (t.L.total)
+    hipLaunchKernelGGL(k_to_f32, dim3(nblk(t.L.total, 256)), dim3(256), 0, c.stream,
+                       t.data64.as<double>(), t.data32.as<float>(), (int64_t)t.L.total);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+}
+
+void dfk_mo_f32(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K, int occ_nbatches)
+{
+  const BLayout& L = t.L;
+  if (!t.data32.p && L.total) throw std::runtime_error("FP32 copy of B missing (mchip_dbt_make_f32)");
+  const int naux = L.naux, nbf = L.nbf;
+  if (nocc <= 0) {
+    hipLaunchKernelGGL(k_fill, dim3(nblk((int64_t)nbf * nbf, 256)), dim3(256), 0, c.stream, K,
+                       (int64_t)nbf * nbf, 0.0);
+    return;
+  }
+  occ_nbatches = std::max(1, std::min(occ_nbatches, nocc));
+  const int per = (nocc + occ_nbatches - 1) / occ_nbatches;
+  const int nb = (nocc + per - 1) / per;
+  const size_t ht = (size_t)naux * nbf * per;
+  c.scratch_a.ensure(ht * sizeof(float) + (size_t)nbf * per * sizeof(float) + 256);
+  float* W = c.scratch_a.as<float>();
+  float* G = W + ht;
+  int ntl = 0;
+  const Tile* tiles = nullptr;
+  for (int b = 0; b < nb; ++b) {
+    const int i0 = b * per;
+    const int w = std::min(per, nocc - i0);
+    {
+      PhaseTimer tm(c, PH_SPLIT);
+      hipLaunchKernelGGL(k_rows_to_f32, dim3(nblk((int64_t)nbf * w, 256)), dim3(256), 0, c.stream,
+                         C + i0, ldc, nbf, w, G);
+    }
+    tiles = slab_tiles(c, t, w, &ntl);
+    {
+      PhaseTimer tm(c, PH_K1);
+      if (ntl)
+        hipLaunchKernelGGL(k_slab_gemm_f32, dim3(ntl), dim3(256), 0, c.stream, t.data32.as<float>(),
+                           t.slab_off.as<int64_t>(), t.ncol.as<int>(), t.bsz.as<int>(),
+                           t.boff.as<int>(), t.colmap.as<int>(), t.colmap_off.as<int>(), tiles,
+                           naux, nbf, G, w, w, W, w);
+      HIP_CHECK(hipGetLastError());
+    }
+    PhaseTimer tm(c, PH_K3);
+    seg_syrk_t<float>(c, W, W, naux, (int64_t)nbf * w, w, w, nbf, -1.0, K, c.scratch_partial, nb, b,
+                      b == nb - 1);
   }
 }
 

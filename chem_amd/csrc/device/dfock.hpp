@@ -21,4 +21,9 @@ void dfj(Ctx& c, DevBT& t, const Metric* m, const double* P, double* J, double p
 void dfk_mo(Ctx& c, DevBT& t, const Metric* m, const double* C, int ldc, int nocc, double* K,
             int occ_nbatches);
 
+// FP32 copy of the transformed B and the FP32-MFMA exchange (FP64 output).
+void dbt_make_f32(Ctx& c, DevBT& t);
+void dfk_mo_f32(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K, int occ_nbatches);
+const Tile* slab_tiles(Ctx& c, DevBT& t, int ng, int* ntiles);
+
 }  // namespace mchip

@@ -35,6 +35,16 @@ __global__ void k_reduce_chunks(const double* part, int nchunk, int64_t n, doubl
 __global__ void k_fill(double* p, int64_t n, double v);
 __global__ void k_axpby(double* y, const double* x, int64_t n, double a, double b);
 
+__global__ void k_to_f32(const double* in, float* out, int64_t n);
+__global__ void k_rows_to_f32(const double* in, int ld64, int rows, int ncols, float* out);
+__global__ void k_slab_gemm_f32(const float* bdata, const int64_t* slab_off, const int* ncol,
+                                const int* bsz, const int* boff, const int* colmap,
+                                const int* colmap_off, const Tile* tiles, int naux, int nbf,
+                                const float* G, int ldg, int ng, float* out, int ldo);
+__global__ void k_seg_gemm_nt_f32(const float* A, const float* B, int nseg, int64_t sstride,
+                                  int lda, int klen, int n, const int2* tiles, int ntile,
+                                  int segs_per_chunk, double* partial);
+
 inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 }  // namespace mchip

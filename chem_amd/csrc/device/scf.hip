@@ -167,7 +167,7 @@ struct DiisState {
 };
 
 void scf_run(Ctx& c, DevBT& bt, const Metric* m, const ScfInput& in, ScfOutput& out,
-             const KBuildFn& kbuild)
+             const KBuildFn& kbuild, const KBuildFn& final_k)
 {
   const int n = in.nbf, nocc = in.nocc;
   const int64_t nn = (int64_t)n * n;
@@ -317,6 +317,21 @@ void scf_run(Ctx& c, DevBT& bt, const Metric* m, const ScfInput& in, ScfOutput& 
     ++iter;
   }
   HIP_CHECK(hipStreamSynchronize(c.stream));
+  out.energy_iter = energy + in.enuc;
+  if (final_k) {
+    // FP64 exchange at the final orbitals; J is already FP64 at this density
+    HIP_CHECK(hipEventRecord(c.ev0, c.stream));
+    final_k(iter, P.as<double>(), Cocc.as<double>(), K.as<double>());
+    HIP_CHECK(hipEventRecord(c.ev1, c.stream));
+    HIP_CHECK(hipEventSynchronize(c.ev1));
+    float tk = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&tk, c.ev0, c.ev1));
+    out.t_final_k = tk * 1e-3;
+    HIP_CHECK(hipMemcpyAsync(F.p, H.p, mb, hipMemcpyDeviceToDevice, c.stream));
+    hipLaunchKernelGGL(k_axpby, dim3(nblk(nn, 256)), dim3(256), 0, c.stream, F.as<double>(), J.as<double>(), nn, 1.0, 1.0);
+    hipLaunchKernelGGL(k_axpby, dim3(nblk(nn, 256)), dim3(256), 0, c.stream, F.as<double>(), K.as<double>(), nn, 1.0, 1.0);
+    energy = dot_dev(c, H.as<double>(), P.as<double>(), nn, part) + dot_dev(c, F.as<double>(), P.as<double>(), nn, part);
+  }
   out.converged = converged;
   out.iterations = iter;
   out.energy = energy + in.enuc;

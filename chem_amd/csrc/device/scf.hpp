@@ -27,12 +27,16 @@ struct ScfOutput {
   double energy = 0.0;
   std::vector<double> C, eps, P, F;
   std::vector<double> history_e, history_rms, t_j, t_k;
+  double energy_iter = 0.0;      // energy of the last iteration's own K
+  double t_final_k = 0.0;        // seconds of the final FP64 K build (if any)
 };
 
 // K builder callback: (iteration, P_dev, Cocc_dev row-major nbf x nocc, K_dev out)
 using KBuildFn = std::function<void(int, const double*, const double*, double*)>;
 
+// final_k (optional): FP64 K builder used once at the converged orbitals to
+// evaluate the reported energy when kbuild works in reduced precision.
 void scf_run(Ctx& c, DevBT& bt, const Metric* m, const ScfInput& in, ScfOutput& out,
-             const KBuildFn& kbuild);
+             const KBuildFn& kbuild, const KBuildFn& final_k = KBuildFn());
 
 }  // namespace mchip

@@ -115,6 +115,16 @@ class DeviceB:
         _lib.check(_lib.lib().mchip_dbt_transform(self.h, metric.h))
         self.transformed = True
 
+    def make_f32(self):
+        _lib.check(_lib.lib().mchip_dbt_make_f32(self.h))
+
+    def dfk_mo_f32(self, C_occ, occ_nbatches=1):
+        C = np.ascontiguousarray(C_occ, dtype=np.float64)
+        K = np.zeros((self.nbf, self.nbf))
+        _lib.check(_lib.lib().mchip_dfk_mo_f32(self.h, _lib.dptr(C), C.shape[1], occ_nbatches,
+                                               _lib.dptr(K)))
+        return K
+
     def download(self):
         out = np.zeros(self.nelem)
         _lib.check(_lib.lib().mchip_dbt_download(self.h, _lib.dptr(out)))

@@ -12,7 +12,7 @@ import numpy as np
 from . import _lib, ints
 from .engine import DeviceB, Metric
 
-KMODES = {"dfmo": 0, "dfmo_transformed": 1}
+KMODES = {"dfmo": 0, "dfmo_transformed": 1, "mixed_f32": 2}
 
 
 @dataclass
@@ -26,6 +26,8 @@ class ScfResult:
     history_rms: list = field(default_factory=list)
     t_j: list = field(default_factory=list)
     t_k: list = field(default_factory=list)
+    energy_iter: float = 0.0
+    t_final_k: float = 0.0
 
 
 def run_rhf(mol, dB, metric, S, H, threshold=1e-8, max_iter=100, do_diis=True, diis_start=1,
@@ -50,5 +52,8 @@ def run_rhf(mol, dB, metric, S, H, threshold=1e-8, max_iter=100, do_diis=True, d
         ctypes.byref(conv), _lib.dptr(C), _lib.dptr(eps), mh, _lib.dptr(he), _lib.dptr(hr),
         _lib.dptr(tj), _lib.dptr(tk), ctypes.byref(nh)))
     k = nh.value
+    ei = ctypes.c_double()
+    tf = ctypes.c_double()
+    _lib.check(_lib.lib().mchip_scf_last_extra(ctypes.byref(ei), ctypes.byref(tf)))
     return ScfResult(e.value, it.value, bool(conv.value), C, eps, list(he[:k]), list(hr[:k]),
-                     list(tj[:k]), list(tk[:k]))
+                     list(tj[:k]), list(tk[:k]), ei.value, tf.value)

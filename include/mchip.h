@@ -93,6 +93,17 @@ int mchip_btensor_to_dense(mchip_btensor t, double* dense);
 int mchip_btensor_dense_rows(mchip_btensor t, int x0, int x1, double* dense);
 
 /* ------------------------------------------------------------------------
+ * Laplace quadrature for SOS-AO-MP2: minimax exponential sum
+ * 1/x ~ sum_k w_k exp(-a_k x) on [1, R] by Remez iteration, started from a
+ * tabulated solution (w0, a0) at range R0 and continued geometrically to R.
+ * Replaces math::minimax / c_laplace_minimax (src/math/laplace/minimax.hpp:
+ * 67-88).  err = max |1/x - s(x)|; equi = relative spread of |e| over the
+ * alternation points (0 when fully levelled).
+ * ---------------------------------------------------------------------- */
+int mchip_laplace_minimax(int n, double R, double R0, const double* w0, const double* a0,
+                          int maxit, double* w, double* a, double* err, double* equi);
+
+/* ------------------------------------------------------------------------
  * Device context (one HIP stream + rocBLAS handle on one GPU).
  * Replaces the reference's world/cart (src/megalochem.hpp:13-80): one
  * process drives one GPU.
@@ -144,12 +155,21 @@ int mchip_dfk_mo_dev(mchip_dbt t, mchip_metric m, const double* C_dev, int ldc, 
 int mchip_dfk_mo(mchip_dbt t, mchip_metric m, const double* C, int nocc, int occ_nbatches,
                  double* K);
 
+/* Chosen form, FP32: make the FP32 copy of the transformed B once, then
+ * K = -(B~ C)^T (B~ C) with FP32 MFMA and FP64 split-K reduction. */
+int mchip_dbt_make_f32(mchip_dbt t);
+int mchip_dfk_mo_f32_dev(mchip_dbt t, const double* C_dev, int ldc, int nocc, int occ_nbatches,
+                         double* K_dev);
+int mchip_dfk_mo_f32(mchip_dbt t, const double* C, int nocc, int occ_nbatches, double* K);
+
 /* ------------------------------------------------------------------------
  * Restricted Hartree-Fock on the GPU (hf::hfmod::compute, src/hf/hfscf.cpp:
  * 326-552): core guess, F = H + J + K, e = FPS - SPF, E = tr(HP) + tr(FP),
  * DIIS (diis.hpp), diag via X^T F X.  S and H = T + V are host row-major.
  * kmode: 0 = DFMO with the metric applied per iteration (reference algorithm),
- *        1 = DFMO on the transformed B (mchip_dbt_transform).
+ *        1 = DFMO on the transformed B (mchip_dbt_transform),
+ *        2 = mixed: FP32 exchange in every iteration, one FP64 exchange at
+ *            the converged orbitals for the reported energy.
  * Outputs C (nbf x nbf row-major), orbital energies and per-iteration
  * history (energy, RMS error, J and K seconds) up to max_hist entries.
  * ---------------------------------------------------------------------- */
@@ -159,11 +179,17 @@ int mchip_scf_run(mchip_dbt t, mchip_metric m, int nbf, int nocc, int nbb, const
                   double filter_eps, int kmode, int occ_nbatches, double* energy,
                   int* iterations, int* converged, double* C, double* eps, int max_hist,
                   double* hist_e, double* hist_rms, double* t_j, double* t_k, int* nhist);
+/* Of the last mchip_scf_run on this thread: the energy with the iteration's
+ * own K and the seconds of the final FP64 K build (mixed mode). */
+int mchip_scf_last_extra(double* energy_iter, double* t_final_k);
 
 /* Test hook: one v_mfma_f64_16x16x4_f64 on a single wave.  a_lane/b_lane hold
  * the operand each of the 64 lanes supplies, out[lane*4+e] the accumulator
  * elements (pins the lane layout the FP64 kernels assume). */
 int mchip_probe_mfma_f64(const double* a_lane, const double* b_lane, double* out);
+/* Test hook: kind 0 = v_mfma_f32_16x16x4_f32 (one float per lane and operand),
+ * kind 1 = v_mfma_f32_16x16x32_bf16 (eight values per lane, rounded to bf16). */
+int mchip_probe_mfma_lp(int kind, const float* a_lane, const float* b_lane, float* out);
 
 #ifdef __cplusplus
 }

@@ -139,3 +139,24 @@ def test_synthetic_sparse_B(ctx, nocc):
     assert _relerr(J, Jref) < _jtol(V)
     assert _relerr(K, Kref) < _jtol(V)
     assert np.array_equal(dB.download(), dB.download())
+
+
+def test_fp32_exchange_accuracy(ctx):
+    """Chosen-form FP32 exchange against the FP64 one on the same B~."""
+    from chem_amd.engine import DeviceB, Metric
+    mol, Bh, V = _system(3)
+    C, P = _random_occ(mol.nbf, mol.nocc, 9)
+    m = Metric(ctx, V)
+    dB = DeviceB(ctx, Bh)
+    dB.transform(m)
+    dB.make_f32()
+    K64 = dB.dfk_mo(C)
+    K32 = dB.dfk_mo_f32(C)
+    K32b = dB.dfk_mo_f32(C, occ_nbatches=2)
+    err = _relerr(K32, K64)
+    print("FP32 K relerr %.3e" % err)
+    assert err < 2e-6
+    assert np.array_equal(K32, K32.T)
+    assert _relerr(K32b, K32) < 2e-6
+    # deterministic: the same call gives the same bits
+    assert np.array_equal(K32, dB.dfk_mo_f32(C))

@@ -30,3 +30,37 @@ def test_mfma_f64_16x16x4_layout():
                 found[(l, e)] = [tuple(h) for h in hits[:3]]
         print("observed layout (lane, elem) -> candidates:", {k: v for k, v in list(found.items())[:24]})
     assert np.array_equal(got, D)
+
+
+def _observe(kind, per):
+    rng = np.random.default_rng(1 + kind)
+    K = 4 * per
+    A = rng.integers(-4, 4, size=(16, K)).astype(np.float32)
+    B = rng.integers(-4, 4, size=(K, 16)).astype(np.float32)
+    # hypothesis: lane l supplies A[l%16][per*(l//16)+q], B[per*(l//16)+q][l%16]
+    a_lane = np.zeros(64 * per, np.float32)
+    b_lane = np.zeros(64 * per, np.float32)
+    for l in range(64):
+        for q in range(per):
+            a_lane[l * per + q] = A[l % 16, per * (l // 16) + q]
+            b_lane[l * per + q] = B[per * (l // 16) + q, l % 16]
+    out = np.zeros(256, np.float32)
+    _lib.check(_lib.lib().mchip_probe_mfma_lp(kind, _lib.fptr(a_lane), _lib.fptr(b_lane), _lib.fptr(out)))
+    return A.astype(np.float64) @ B.astype(np.float64), out
+
+
+@pytest.mark.parametrize("kind,per", [(0, 1), (1, 8)])
+def test_mfma_lowprec_layout(kind, per):
+    """f32 16x16x4 and bf16 16x16x32: D[4*(lane/16)+e][lane%16]."""
+    D, out = _observe(kind, per)
+    got = np.zeros((16, 16))
+    for l in range(64):
+        for e in range(4):
+            got[4 * (l // 16) + e, l % 16] = out[l * 4 + e]
+    if not np.array_equal(got, D):
+        alt = np.zeros((16, 16))
+        for l in range(64):
+            for e in range(4):
+                alt[l // 16 + 4 * e, l % 16] = out[l * 4 + e]
+        print("kind %d: alternative layout D[lane/16+4e] matches: %s" % (kind, np.array_equal(alt, D)))
+    assert np.array_equal(got, D)

@@ -1,6 +1,13 @@
 """GPU restricted HF against the oracle SCF (same integrals, same metric
-factors): total energies agree to 1e-10 Eh; the converged density satisfies
-P S P = P and tr(P S) = nocc."""
+factors).
+
+Bar: |E_gpu - E_oracle| < 1e-9 Eh.  The reference's DF-J evaluates
+gamma^T V^-1 gamma with an explicit V^-1 (jkdf.cpp:137-141, LLT::inverse); for
+one water with cc-pVDZ-jkfit cond(V) = 2.4e6 and one-ulp changes of gamma
+(any change of summation order) move E_J by 1e-10 - 3e-10 Eh, shown in
+tests/test_oracle_conditioning.py.  Two correct FP64 implementations therefore
+cannot agree to 1e-10; the bar is set one decade above that floor.  The
+converged density satisfies P S P = P and tr(P S) = nocc."""
 import numpy as np
 import pytest
 
@@ -38,9 +45,29 @@ def test_rhf_energy_vs_oracle(ctx, nwater, kmode):
     ref = oscf.rhf(S, T, V, mol.nuclear_repulsion(), mol.nocc, jk, bsz=mol.b, thr=1e-8)
     print("E gpu %.12f oracle %.12f diff %.2e iters %d/%d" % (
         res.energy, ref["energy"], res.energy - ref["energy"], res.iterations, ref["iterations"]))
-    assert abs(res.energy - ref["energy"]) < 1e-10
+    assert abs(res.energy - ref["energy"]) < 1e-9
     C = res.C[:, :mol.nocc]
     P = C @ C.T
     assert np.abs(P @ S @ P - P).max() < 1e-8
     assert abs(np.trace(P @ S) - mol.nocc) < 1e-10
     assert np.abs(res.eps - ref["eps"]).max() < 1e-7
+
+
+def test_mixed_scf_energy(ctx):
+    """Chosen form (FP32 exchange per iteration, FP64 exchange for the final
+    energy) against the FP64 SCF on the same B~: |dE| < 1e-8 Eh."""
+    from chem_amd.engine import DeviceB, Metric
+    from chem_amd.scf import run_rhf
+    mol = desc.Molecule(desc.water_cluster(4, 11))
+    S, T, V = ints.int1e(mol)
+    Bh = ints.BTensorHost.build(mol, 1e-10)
+    m = Metric(ctx, ints.int2c2e(mol))
+    dB = DeviceB(ctx, Bh)
+    dB.transform(m)
+    ref = run_rhf(mol, dB, None, S, T + V, threshold=1e-7, kmode="dfmo_transformed")
+    dB.make_f32()
+    mix = run_rhf(mol, dB, None, S, T + V, threshold=1e-7, kmode="mixed_f32")
+    print("E64 %.10f Emix %.10f (iterate %.10f) dE %.2e" % (ref.energy, mix.energy, mix.energy_iter,
+                                                           mix.energy - ref.energy))
+    assert ref.converged and mix.converged
+    assert abs(mix.energy - ref.energy) < 1e-8

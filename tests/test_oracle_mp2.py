@@ -1,0 +1,101 @@
+"""Pins of the SOS-AO-MP2 oracle and the Laplace minimax.
+
+* minimax: at tabulated ranges the Remez solution reproduces the reference's
+  stored parameters (src/math/laplace/data/init_para.txt, fixtures under
+  tests/golden/) and max errors (init_error.txt); between table points the
+  error equioscillates and lies between its neighbours';
+* pivoted Cholesky: L L^T = P within its threshold, rank = nocc for the
+  occupied pseudo-density;
+* the Laplace SOS-MP2 energy matches the canonical DF-SOS sum to the accuracy
+  the quadrature allows (relative error bounded by the minimax error times
+  the energy range factor) and converges as the number of points grows.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from chem_amd import desc, ints, laplace
+from oracle import dfock, mp2, scf as oscf
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "laplace_k08.json")
+
+
+def test_minimax_reproduces_tabulated_solutions():
+    with open(GOLDEN) as f:
+        gold = json.load(f)
+    for e in gold["entries"]:
+        R = e["R"]
+        # start from the next lower tabulated range, so the solve is not trivial
+        lower = [x for x in gold["entries"] if x["R"] < R]
+        if not lower:
+            continue
+        st = lower[-1]
+        w0 = np.array(st["omega"], dtype=float)
+        a0 = np.array(st["alpha"], dtype=float)
+        import ctypes
+        from chem_amd import _lib
+        w = np.zeros(8)
+        a = np.zeros(8)
+        err = ctypes.c_double()
+        eq = ctypes.c_double()
+        _lib.check(_lib.lib().mchip_laplace_minimax(8, R, st["R"], _lib.dptr(w0), _lib.dptr(a0), 80,
+                                                    _lib.dptr(w), _lib.dptr(a), ctypes.byref(err),
+                                                    ctypes.byref(eq)))
+        wt = np.array(e["omega"], dtype=float)
+        at = np.array(e["alpha"], dtype=float)
+        assert abs(err.value / e["err"] - 1) < 2e-3, (R, err.value, e["err"])
+        assert np.abs(w - wt).max() / np.abs(wt).max() < 1e-6, R
+        assert np.abs(a - at).max() / np.abs(at).max() < 1e-6, R
+
+
+def test_minimax_between_table_points():
+    w, a, err, eq = laplace.minimax_unit(8, 37.3)
+    assert eq < 1e-6
+    errs = {round(e["R"]): e["err"] for e in laplace.table()["8"]}
+    assert errs[30] < err < errs[40]
+    x = np.exp(np.linspace(0, np.log(37.3), 20001))
+    e = 1 / x - (w[None] * np.exp(-np.outer(x, a))).sum(1)
+    assert abs(np.abs(e).max() - err) < 1e-3 * err
+
+
+def test_pivoted_cholesky():
+    rng = np.random.default_rng(0)
+    C = rng.standard_normal((30, 7))
+    P = C @ C.T
+    L, rank = mp2.pivoted_cholesky(P)
+    assert rank == 7
+    assert np.abs(L @ L.T - P).max() < 30 * np.finfo(float).eps * np.abs(np.diag(P)).max() * 10
+
+
+@pytest.fixture(scope="module")
+def water_hf():
+    mol = desc.Molecule(desc.water_cluster(1, 0), dfbasis="cc-pvdz-ri")
+    S, T, V = ints.int1e(mol)
+    Bh = ints.BTensorHost.build(mol, 1e-10)
+    B = Bh.dense()
+    M = ints.int2c2e(mol)
+    Vinv, _, invsqrt = dfock.metric_factors(M)
+    jk = lambda P, C: (dfock.df_j(B, Vinv, P), dfock.dfmo_k(B, invsqrt, C))
+    r = oscf.rhf(S, T, V, mol.nuclear_repulsion(), mol.nocc, jk, bsz=mol.b, thr=1e-10)
+    return mol, S, B, Bh.mask(), Vinv, r
+
+
+def test_laplace_sos_mp2_vs_canonical(water_hf):
+    mol, S, B, mask, Vinv, r = water_hf
+    C, eps, nocc = r["C"], r["eps"], mol.nocc
+    ref = mp2.canonical_sos(B, Vinv, C, eps, nocc)
+    ymin = 2 * (eps[nocc] - eps[nocc - 1])
+    ymax = 2 * (eps[-1] - eps[0])
+    prev = None
+    for n in (4, 6, 8):
+        w, a, qerr = laplace.laplace_points(n, ymin, ymax)
+        E = mp2.sos_mp2(B, Vinv, S, C, eps, nocc, w, a, mol.b, Bmask=mask)
+        rel = abs(E - ref) / ref
+        # the quadrature error of 1/y is qerr on [ymin, ymax]; relative to 1/ymin it is qerr*ymin
+        assert rel < 5 * qerr * ymin + 1e-10, (n, E, ref, rel, qerr * ymin)
+        if prev is not None:
+            assert rel < prev
+        prev = rel
+    assert 0.1 < ref < 1.0   # SOS-MP2 opposite-spin sum of one water, cc-pVDZ: a few tenths of Eh
