@@ -101,6 +101,11 @@ _SIGS = [
     ("mchip_ctx_stream", c_int, [c_void_p, ctypes.POINTER(c_void_p)]),
     ("mchip_dbt_upload", c_int, [c_void_p, c_void_p, ctypes.POINTER(c_void_p)]),
     ("mchip_dbt_destroy", c_int, [c_void_p]),
+    ("mchip_dbt_build_gpu", c_int, [c_void_p, c_void_p, c_void_p, c_int, P_int, c_int, P_int, P_u8,
+                                    c_double, ctypes.POINTER(c_void_p)]),
+    ("mchip_dbt_info", c_int, [c_void_p, P_int, P_int64]),
+    ("mchip_dbt_mask", c_int, [c_void_p, P_u8]),
+    ("mchip_dbt_dense_rows", c_int, [c_void_p, c_int, c_int, P_double]),
     ("mchip_dbt_download", c_int, [c_void_p, P_double]),
     ("mchip_metric_create", c_int, [c_void_p, c_int, P_double, ctypes.POINTER(c_void_p)]),
     ("mchip_metric_destroy", c_int, [c_void_p]),
@@ -114,6 +119,9 @@ _SIGS = [
                               c_double, c_double, c_int, c_int, c_int, c_int, c_int, c_double,
                               c_int, c_int, P_double, P_int, P_int, P_double, P_double, c_int,
                               P_double, P_double, P_double, P_double, P_int]),
+    ("mchip_mp2_sos", c_int, [c_void_p, c_void_p, c_int, c_int, P_double, P_double, P_double, c_int,
+                              P_double, P_double, c_int, P_int, c_int, c_double, P_double, P_double,
+                              P_double]),
     ("mchip_probe_mfma_f64", c_int, [P_double, P_double, P_double]),
     ("mchip_probe_mfma_lp", c_int, [c_int, P_float, P_float, P_float]),
     ("mchip_ctx_set_timing", c_int, [c_void_p, c_int]),

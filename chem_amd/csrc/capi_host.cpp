@@ -27,6 +27,7 @@ void set_error(const std::string& s) { g_last_error = s; }
 extern "C" const char* mchip_last_error(void) { return g_last_error.c_str(); }
 
 namespace mchip {
+const Basis& basis_of(mchip_basis b) { return b->basis; }
 const BLayout& btensor_layout(mchip_btensor t) { return t->layout; }
 const double* btensor_data(mchip_btensor t) { return t->data.data(); }
 }  // namespace mchip

@@ -5,11 +5,13 @@
 #include "../../../include/mchip.h"
 #include "../capi_common.hpp"
 #include "dfock.hpp"
+#include "ints_dev.hpp"
 
 using namespace mchip;
 
 struct mchip_btensor_s;   // host container (capi_host.cpp)
 namespace mchip {
+const Basis& basis_of(mchip_basis b);
 const BLayout& btensor_layout(mchip_btensor t);
 const double* btensor_data(mchip_btensor t);
 }  // namespace mchip
@@ -105,6 +107,73 @@ extern "C" int mchip_dbt_upload(mchip_ctx c, mchip_btensor host, mchip_dbt* out)
   d->device = c->c.device;
   dbt_upload(c->c, btensor_layout(host), btensor_data(host), d->t);
   *out = d.release();
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dbt_build_gpu(mchip_ctx c, mchip_basis ao, mchip_basis aux, int nbb,
+                                   const int* ao_blk_shells, int nxb, const int* aux_blk_shells,
+                                   const uint8_t* schwarz_mask, double filter_eps, mchip_dbt* out)
+{
+  MCHIP_TRY
+  need_device(c);
+  std::vector<int> bsh(ao_blk_shells, ao_blk_shells + nbb), xsh(aux_blk_shells, aux_blk_shells + nxb);
+  int sb = 0, sx = 0;
+  for (int v : bsh) sb += v;
+  for (int v : xsh) sx += v;
+  if (sb != (int)basis_of(ao).shells.size() || sx != (int)basis_of(aux).shells.size())
+    throw std::runtime_error("mchip_dbt_build_gpu: block shell counts do not cover the basis");
+  std::vector<uint8_t> msk(schwarz_mask, schwarz_mask + (size_t)nxb * nbb * nbb);
+  auto d = std::make_unique<mchip_dbt_s>();
+  d->ctx = c;
+  d->device = c->c.device;
+  build_B_device(c->c, basis_of(ao), basis_of(aux), bsh, xsh, msk, filter_eps, d->t);
+  *out = d.release();
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dbt_info(mchip_dbt t, int* dims, int64_t* nelem)
+{
+  MCHIP_TRY
+  dims[0] = t->t.L.naux;
+  dims[1] = t->t.L.nbf;
+  dims[2] = t->t.L.nxb;
+  dims[3] = t->t.L.nbb;
+  *nelem = t->t.L.total;
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dbt_mask(mchip_dbt t, uint8_t* mask)
+{
+  MCHIP_TRY
+  std::memcpy(mask, t->t.L.mask.data(), t->t.L.mask.size());
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dbt_dense_rows(mchip_dbt t, int x0, int x1, double* dense)
+{
+  MCHIP_TRY
+  need_device(t->ctx);
+  const BLayout& L = t->t.L;
+  if (x0 < 0 || x1 > L.naux || x0 > x1) throw std::runtime_error("mchip_dbt_dense_rows: bad range");
+  const int nbf = L.nbf, nr = x1 - x0;
+  std::memset(dense, 0, sizeof(double) * (size_t)nr * nbf * nbf);
+  HIP_CHECK(hipStreamSynchronize(t->ctx->c.stream));
+  std::vector<double> rows;
+  for (int n = 0; n < L.nbb; ++n) {
+    const int64_t R = (int64_t)L.bsz[n] * L.ncol[n];
+    if (R == 0 || nr == 0) continue;
+    rows.resize((size_t)nr * R);
+    HIP_CHECK(hipMemcpy(rows.data(), t->t.data64.as<double>() + L.slab_off[n] + (int64_t)x0 * R,
+                        rows.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int p = L.pair_ptr[n]; p < L.pair_ptr[n + 1]; ++p) {
+      const int mb = L.pair_m[p], col = L.pair_col[p];
+      for (int xi = 0; xi < nr; ++xi)
+        for (int nu = 0; nu < L.bsz[n]; ++nu)
+          for (int mu = 0; mu < L.bsz[mb]; ++mu)
+            dense[((int64_t)xi * nbf + L.boff[mb] + mu) * nbf + L.boff[n] + nu] =
+                rows[((int64_t)xi * L.bsz[n] + nu) * L.ncol[n] + col + mu];
+    }
+  }
   MCHIP_CATCH
 }
 
@@ -330,5 +399,35 @@ extern "C" int mchip_scf_run(mchip_dbt t, mchip_metric m, int nbf, int nocc, int
     if (t_k) t_k[i] = out.t_k[i];
   }
   *nhist = nh;
+  MCHIP_CATCH
+}
+
+#include "mp2.hpp"
+
+extern "C" int mchip_mp2_sos(mchip_dbt t, mchip_metric m, int nbf, int nocc, const double* S,
+                             const double* C, const double* eps, int nlap, const double* omega,
+                             const double* alpha, int nbb, const int* bsz, int mo_split,
+                             double filter_eps, double* energy, double* terms, double* t_z)
+{
+  MCHIP_TRY
+  need_device(t->ctx);
+  if (!m) throw std::runtime_error("mchip_mp2_sos: the metric is required");
+  Mp2Input in;
+  in.nbf = nbf;
+  in.nocc = nocc;
+  in.S = S;
+  in.C = C;
+  in.eps = eps;
+  in.omega.assign(omega, omega + nlap);
+  in.alpha.assign(alpha, alpha + nlap);
+  in.bsz.assign(bsz, bsz + nbb);
+  in.mo_split = mo_split;
+  in.filter_eps = filter_eps;
+  std::vector<double> tv;
+  double tz = 0.0;
+  *energy = mp2_sos(t->ctx->c, t->t, m->m, in, tv, &tz);
+  if (terms)
+    for (int k = 0; k < nlap && k < (int)tv.size(); ++k) terms[k] = tv[k];
+  if (t_z) *t_z = tz;
   MCHIP_CATCH
 }

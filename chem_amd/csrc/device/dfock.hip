@@ -15,6 +15,7 @@
 #include <type_traits>
 
 #include "dfock.hpp"
+#include "ints_dev.hpp"
 #include "kernels.hpp"
 
 namespace mchip {
@@ -58,6 +59,8 @@ void ctx_free(Ctx& c)
   c.stream = nullptr;
 }
 
+static void dbt_index(Ctx& c, const BLayout& L, DevBT& t);
+
 void dbt_upload(Ctx& c, const BLayout& L, const double* host_data, DevBT& t)
 {
   HIP_CHECK(hipSetDevice(c.device));
@@ -65,6 +68,19 @@ void dbt_upload(Ctx& c, const BLayout& L, const double* host_data, DevBT& t)
   t.data64.alloc((size_t)L.total * sizeof(double));
   if (L.total)
     HIP_CHECK(hipMemcpyAsync(t.data64.p, host_data, t.data64.bytes, hipMemcpyHostToDevice, c.stream));
+  dbt_index(c, L, t);
+}
+
+void dbt_adopt(Ctx& c, const BLayout& L, DevBuf&& data, DevBT& t)
+{
+  HIP_CHECK(hipSetDevice(c.device));
+  t.L = L;
+  t.data64 = std::move(data);
+  dbt_index(c, L, t);
+}
+
+static void dbt_index(Ctx& c, const BLayout& L, DevBT& t)
+{
   // column -> AO function map per slab
   t.h_colmap_off.assign(L.nbb + 1, 0);
   t.h_colmap.clear();
@@ -95,17 +111,21 @@ void dbt_upload(Ctx& c, const BLayout& L, const double* host_data, DevBT& t)
   HIP_CHECK(hipStreamSynchronize(c.stream));
 }
 
+// Column tile of the grouped slab GEMM for ng output columns.
+static int slab_bn(int ng) { return ng <= 64 ? 64 : 128; }
+
 // Tile table of the grouped slab GEMM for ng output columns.
 const Tile* slab_tiles(Ctx& c, DevBT& t, int ng, int* ntiles)
 {
-  auto key = std::make_pair(SG_BN, ng);
+  const int BN = slab_bn(ng);
+  auto key = std::make_pair(BN, ng);
   auto it = t.tiles.find(key);
   if (it == t.tiles.end()) {
     std::vector<Tile> v;
     for (int n = 0; n < t.L.nbb; ++n) {
       const int64_t rows = (int64_t)t.L.naux * t.L.bsz[n];
       for (int64_t m0 = 0; m0 < rows; m0 += SG_BM)
-        for (int n0 = 0; n0 < ng; n0 += SG_BN) v.push_back(Tile{n, (int)m0, n0, 0});
+        for (int n0 = 0; n0 < ng; n0 += BN) v.push_back(Tile{n, (int)m0, n0, 0});
     }
     t.ntiles[key] = (int)v.size();
     t.tiles[key] = upload_vec(v, c.stream);
@@ -116,17 +136,30 @@ const Tile* slab_tiles(Ctx& c, DevBT& t, int ng, int* ntiles)
   return it->second.as<Tile>();
 }
 
-void slab_gemm(Ctx& c, DevBT& t, const double* G, int ldg, int ng, double* out, int ldo,
-               double alpha)
+template <class T>
+static void slab_gemm_t(Ctx& c, DevBT& t, const T* data, const T* G, int ldg, int ng, T* out, int ldo)
 {
   int nt = 0;
   const Tile* tiles = slab_tiles(c, t, ng, &nt);
   if (nt == 0) return;
-  hipLaunchKernelGGL(k_slab_gemm_f64, dim3(nt), dim3(256), 0, c.stream, t.data64.as<double>(),
-                     t.slab_off.as<int64_t>(), t.ncol.as<int>(), t.bsz.as<int>(), t.boff.as<int>(),
-                     t.colmap.as<int>(), t.colmap_off.as<int>(), tiles, t.L.naux, t.L.nbf, G, ldg,
-                     ng, out, ldo, alpha);
+  if (slab_bn(ng) == 64)
+    hipLaunchKernelGGL((k_slab_gemm<T, SG_BM, 64>), dim3(nt), dim3(256), 0, c.stream, data,
+                       t.slab_off.as<int64_t>(), t.ncol.as<int>(), t.bsz.as<int>(), t.boff.as<int>(),
+                       t.colmap.as<int>(), t.colmap_off.as<int>(), tiles, t.L.naux, t.L.nbf, G, ldg,
+                       ng, out, ldo);
+  else
+    hipLaunchKernelGGL((k_slab_gemm<T, SG_BM, 128>), dim3(nt), dim3(256), 0, c.stream, data,
+                       t.slab_off.as<int64_t>(), t.ncol.as<int>(), t.bsz.as<int>(), t.boff.as<int>(),
+                       t.colmap.as<int>(), t.colmap_off.as<int>(), tiles, t.L.naux, t.L.nbf, G, ldg,
+                       ng, out, ldo);
   HIP_CHECK(hipGetLastError());
+}
+
+void slab_gemm(Ctx& c, DevBT& t, const double* G, int ldg, int ng, double* out, int ldo,
+               double alpha)
+{
+  if (alpha != 1.0) throw std::runtime_error("slab_gemm: only alpha = 1 is supported");
+  slab_gemm_t<double>(c, t, t.data64.as<double>(), G, ldg, ng, out, ldo);
 }
 
 // C(m,n) = alpha * sum_s sum_k A_s(m,k) B_s(n,k), symmetric output.  When B
@@ -137,7 +170,7 @@ static void seg_syrk_t(Ctx& c, const T* A, const T* B, int nseg, int64_t sstride
                        int batch, bool reduce)
 {
   const bool sym = (A == B);
-  const int nt = (n + SG_BM - 1) / SG_BM;
+  const int nt = (n + SG_BM - 1) / SG_BM;   // 128 x 128 output tiles
   std::vector<int2> tl;
   for (int i = 0; i < nt; ++i)
     for (int j = 0; j < nt; ++j)
@@ -151,12 +184,8 @@ static void seg_syrk_t(Ctx& c, const T* A, const T* B, int nseg, int64_t sstride
   if (batch == 0) partial_buf.ensure(need);
   double* partial = partial_buf.as<double>() + (size_t)batch * nch * n * n;
   DevBuf tiles = upload_vec(tl, c.stream);
-  if constexpr (std::is_same<T, double>::value)
-    hipLaunchKernelGGL(k_seg_gemm_nt_f64, dim3(ntile * nch), dim3(256), 0, c.stream, A, B, nseg,
-                       sstride, lda, klen, n, tiles.as<int2>(), ntile, per, partial);
-  else
-    hipLaunchKernelGGL(k_seg_gemm_nt_f32, dim3(ntile * nch), dim3(256), 0, c.stream, A, B, nseg,
-                       sstride, lda, klen, n, tiles.as<int2>(), ntile, per, partial);
+  hipLaunchKernelGGL((k_seg_gemm<T, SG_BM, SG_BM>), dim3(ntile * nch), dim3(256), 0, c.stream, A, B,
+                     nseg, sstride, lda, klen, n, tiles.as<int2>(), ntile, per, partial);
   HIP_CHECK(hipGetLastError());
   if (reduce) {
     hipLaunchKernelGGL(k_reduce_sym, dim3(nblk((int64_t)n * n, 256)), dim3(256), 0, c.stream,
@@ -359,8 +388,6 @@ void dfk_mo_f32(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K,
   c.scratch_a.ensure(ht * sizeof(float) + (size_t)nbf * per * sizeof(float) + 256);
   float* W = c.scratch_a.as<float>();
   float* G = W + ht;
-  int ntl = 0;
-  const Tile* tiles = nullptr;
   for (int b = 0; b < nb; ++b) {
     const int i0 = b * per;
     const int w = std::min(per, nocc - i0);
@@ -369,20 +396,76 @@ void dfk_mo_f32(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K,
       hipLaunchKernelGGL(k_rows_to_f32, dim3(nblk((int64_t)nbf * w, 256)), dim3(256), 0, c.stream,
                          C + i0, ldc, nbf, w, G);
     }
-    tiles = slab_tiles(c, t, w, &ntl);
     {
       PhaseTimer tm(c, PH_K1);
-      if (ntl)
-        hipLaunchKernelGGL(k_slab_gemm_f32, dim3(ntl), dim3(256), 0, c.stream, t.data32.as<float>(),
-                           t.slab_off.as<int64_t>(), t.ncol.as<int>(), t.bsz.as<int>(),
-                           t.boff.as<int>(), t.colmap.as<int>(), t.colmap_off.as<int>(), tiles,
-                           naux, nbf, G, w, w, W, w);
-      HIP_CHECK(hipGetLastError());
+      slab_gemm_t<float>(c, t, t.data32.as<float>(), G, w, w, W, w);
     }
     PhaseTimer tm(c, PH_K3);
     seg_syrk_t<float>(c, W, W, naux, (int64_t)nbf * w, w, w, nbf, -1.0, K, c.scratch_partial, nb, b,
                       b == nb - 1);
   }
+}
+
+}  // namespace mchip
+
+namespace mchip {
+
+void slab_out_gemm(Ctx& c, DevBT& t, const double* A, int lda, int klen, const double* G, int ldg,
+                   double* out)
+{
+  const BLayout& L = t.L;
+  std::vector<Tile> v;
+  for (int n = 0; n < L.nbb; ++n) {
+    const int64_t rows = (int64_t)L.naux * L.bsz[n];
+    for (int64_t m0 = 0; m0 < rows; m0 += SG_BM)
+      for (int c0 = 0; c0 < L.ncol[n]; c0 += 128) v.push_back(Tile{n, (int)m0, c0, 0});
+  }
+  if (v.empty()) return;
+  DevBuf tiles = upload_vec(v, c.stream);
+  hipLaunchKernelGGL((k_slab_out_gemm<double, SG_BM, 128>), dim3((unsigned)v.size()), dim3(256), 0, c.stream,
+                     A, lda, klen, t.slab_off.as<int64_t>(), t.ncol.as<int>(), t.bsz.as<int>(),
+                     t.boff.as<int>(), t.colmap.as<int>(), t.colmap_off.as<int>(), tiles.as<Tile>(), L.naux,
+                     L.nbf, G, ldg, out);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+}
+
+void slab_syrk(Ctx& c, DevBT& t, const double* A, const double* B, double* Z, DevBuf& partial)
+{
+  const BLayout& L = t.L;
+  const int n = L.naux;
+  const int nt = (n + SG_BM - 1) / SG_BM;
+  std::vector<int2> tl;
+  for (int i = 0; i < nt; ++i)
+    for (int j = i; j < nt; ++j) tl.push_back(int2{i, j});
+  const int ntile = (int)tl.size();
+  // chunks of slabs with roughly equal contraction length
+  std::vector<int64_t> off(L.nbb), len(L.nbb);
+  int64_t tot = 0;
+  for (int s = 0; s < L.nbb; ++s) {
+    off[s] = L.slab_off[s];
+    len[s] = (int64_t)L.bsz[s] * L.ncol[s];
+    tot += len[s];
+  }
+  const int want = std::max(1, std::min(L.nbb, 2048 / std::max(1, ntile)));
+  std::vector<int> cs(1, 0);
+  int64_t acc = 0;
+  for (int s = 0; s < L.nbb; ++s) {
+    acc += len[s];
+    if ((int)cs.size() < want && acc * want >= tot * (int64_t)cs.size() && s + 1 < L.nbb) cs.push_back(s + 1);
+  }
+  cs.push_back(L.nbb);
+  const int nch = (int)cs.size() - 1;
+  partial.ensure((size_t)nch * n * n * sizeof(double));
+  DevBuf d_tl = upload_vec(tl, c.stream), d_off = upload_vec(off, c.stream), d_len = upload_vec(len, c.stream),
+         d_cs = upload_vec(cs, c.stream);
+  hipLaunchKernelGGL((k_slabseg_gemm<double, SG_BM, SG_BM>), dim3(ntile * nch), dim3(256), 0, c.stream, A, B,
+                     d_off.as<int64_t>(), d_len.as<int64_t>(), d_cs.as<int>(), n, d_tl.as<int2>(), ntile,
+                     partial.as<double>());
+  hipLaunchKernelGGL(k_reduce_sym, dim3(nblk((int64_t)n * n, 256)), dim3(256), 0, c.stream, partial.as<double>(),
+                     nch, n, 0, 1.0, Z);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(c.stream));
 }
 
 }  // namespace mchip

@@ -25,5 +25,10 @@ void dfk_mo(Ctx& c, DevBT& t, const Metric* m, const double* C, int ldc, int noc
 void dbt_make_f32(Ctx& c, DevBT& t);
 void dfk_mo_f32(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K, int occ_nbatches);
 const Tile* slab_tiles(Ctx& c, DevBT& t, int ng, int* ntiles);
+// out (slab layout of B) = rows (X, nu in N) of A (lda, klen) times gathered rows of G
+void slab_out_gemm(Ctx& c, DevBT& t, const double* A, int lda, int klen, const double* G, int ldg,
+                   double* out);
+// Z = sum over slabs of A_slab B_slab^T (naux x naux, symmetric by construction)
+void slab_syrk(Ctx& c, DevBT& t, const double* A, const double* B, double* Z, DevBuf& partial);
 
 }  // namespace mchip

@@ -5,17 +5,27 @@
 
 namespace mchip {
 
-constexpr int SG_BM = 64, SG_BN = 64, SG_BK = 16;
-constexpr int J_ROWS = 16;   // aux rows per block in k_j_gamma
+constexpr int SG_BM = 128;    // rows per slab-GEMM / SYRK tile
+constexpr int J_ROWS = 16;    // aux rows per block in k_j_gamma
 
-__global__ void k_slab_gemm_f64(const double* bdata, const int64_t* slab_off, const int* ncol,
+template <class T, int BM, int BN>
+__global__ void k_slab_gemm(const T* bdata, const int64_t* slab_off, const int* ncol, const int* bsz,
+                            const int* boff, const int* colmap, const int* colmap_off,
+                            const Tile* tiles, int naux, int nbf, const T* G, int ldg, int ng, T* out,
+                            int ldo);
+template <class T, int BM, int BN>
+__global__ void k_seg_gemm(const T* A, const T* B, int nseg, int64_t sstride, int lda, int klen, int n,
+                           const int2* tiles, int ntile, int segs_per_chunk, double* partial);
+template <class T, int BM, int BN>
+__global__ void k_slab_out_gemm(const T* A, int lda, int klen, const int64_t* slab_off, const int* ncol,
                                 const int* bsz, const int* boff, const int* colmap,
                                 const int* colmap_off, const Tile* tiles, int naux, int nbf,
-                                const double* G, int ldg, int ng, double* out, int ldo,
-                                double alpha);
-__global__ void k_seg_gemm_nt_f64(const double* A, const double* B, int nseg, int64_t sstride,
-                                  int lda, int klen, int n, const int2* tiles, int ntile,
-                                  int segs_per_chunk, double* partial);
+                                const T* G, int ldg, T* out);
+template <class T, int BM, int BN>
+__global__ void k_slabseg_gemm(const T* A, const T* B, const int64_t* seg_off, const int64_t* seg_len,
+                               const int* chunk_seg, int n, const int2* tiles, int ntile,
+                               double* partial);
+__global__ void k_zero_upper_cm(double* A, int n);
 __global__ void k_reduce_sym(const double* partial, int nchunk, int n, int sym2, double alpha,
                              double* out);
 __global__ void k_pair_gather(const double* P, int nbf, const int* ncol, const int* bsz,
@@ -37,13 +47,6 @@ __global__ void k_axpby(double* y, const double* x, int64_t n, double a, double 
 
 __global__ void k_to_f32(const double* in, float* out, int64_t n);
 __global__ void k_rows_to_f32(const double* in, int ld64, int rows, int ncols, float* out);
-__global__ void k_slab_gemm_f32(const float* bdata, const int64_t* slab_off, const int* ncol,
-                                const int* bsz, const int* boff, const int* colmap,
-                                const int* colmap_off, const Tile* tiles, int naux, int nbf,
-                                const float* G, int ldg, int ng, float* out, int ldo);
-__global__ void k_seg_gemm_nt_f32(const float* A, const float* B, int nseg, int64_t sstride,
-                                  int lda, int klen, int n, const int2* tiles, int ntile,
-                                  int segs_per_chunk, double* partial);
 
 inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 

@@ -1,170 +1,12 @@
 // FP64 kernels of the DF Fock build on gfx950.
 //
-//   k_slab_gemm_f64    grouped GEMM over B's pair slabs with gathered rows of a
-//                      dense operand: HT(X,nu,j) = sum_mu B(X|mu nu) G(mu,j).
-//                      Used for XMN,Mi->XiN (jkdf.cpp:361), mi,Xmn->Xin
-//                      (z_full.cpp:202) and XMN,NL->XML (jkdf.cpp:566).
-//   k_seg_gemm_nt_f64  split-K GEMM over a segmented contraction index,
-//                      C(m,n) = sum_s sum_k A_s(m,k) B_s(n,k): the exchange
-//                      SYRK XiM,XiN->MN (jkdf.cpp:417).
+//   k_reduce_sym       FP64 reduction of split-K partials into a symmetric K
 //   k_pair_gather / k_j_gamma / k_j_cols / k_pair_scatter
 //                      HBM-streaming DF-J (jkdf.cpp:111-170).
-//
-// MFMA: v_mfma_f64_16x16x4_f64.  Operand layout per 64-lane wave:
-//   A[i][k] from lane i + 16k, B[k][j] from lane j + 16k,
-//   D[lane/16 + 4e][lane%16] in element e of the 4-double accumulator
-//   (pinned by tests/test_gpu_mfma_layout.py).
+// The MFMA contractions live in kernels_gemm.hip.
 #include "kernels.hpp"
 
 namespace mchip {
-
-typedef double d4 __attribute__((ext_vector_type(4)));
-
-// ---------------------------------------------------------------------------
-// grouped slab GEMM
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_slab_gemm_f64(
-    const double* __restrict__ bdata, const int64_t* __restrict__ slab_off,
-    const int* __restrict__ ncol, const int* __restrict__ bsz, const int* __restrict__ boff,
-    const int* __restrict__ colmap, const int* __restrict__ colmap_off,
-    const Tile* __restrict__ tiles, int naux, int nbf,
-    const double* __restrict__ G, int ldg, int ng,
-    double* __restrict__ out, int ldo, double alpha)
-{
-  __shared__ double As[SG_BK][SG_BM + 1];
-  __shared__ double Bs[SG_BK][SG_BN + 1];
-  const Tile t = tiles[blockIdx.x];
-  const int N = t.slab;
-  const int bn = bsz[N];
-  const int K = ncol[N];
-  const int64_t rows = (int64_t)naux * bn;
-  const double* A = bdata + slab_off[N];
-  const int* cmap = colmap + colmap_off[N];
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  d4 acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-
-  const int a_row = tid >> 2, a_k = (tid & 3) * 4;
-  const int b_k = tid >> 4, b_n = (tid & 15) * 4;
-  const int64_t grow = (int64_t)t.m0 + a_row;
-  const double* arow = A + grow * K;
-
-  for (int k0 = 0; k0 < K; k0 += SG_BK) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int kk = k0 + a_k + q;
-      As[a_k + q][a_row] = (grow < rows && kk < K) ? arow[kk] : 0.0;
-    }
-    {
-      const int kk = k0 + b_k;
-      const int mu = (kk < K) ? cmap[kk] : -1;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int j = t.n0 + b_n + q;
-        Bs[b_k][b_n + q] = (mu >= 0 && j < ng) ? G[(int64_t)mu * ldg + j] : 0.0;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kq = 0; kq < SG_BK; kq += 4) {
-      const int kr = kq + (lane >> 4);
-      const double a0 = As[kr][wm * 32 + (lane & 15)];
-      const double a1 = As[kr][wm * 32 + 16 + (lane & 15)];
-      const double b0 = Bs[kr][wn * 32 + (lane & 15)];
-      const double b1 = Bs[kr][wn * 32 + 16 + (lane & 15)];
-      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int64_t r = (int64_t)t.m0 + wm * 32 + i * 16 + (lane >> 4) + 4 * e;
-        const int c = t.n0 + wn * 32 + j * 16 + (lane & 15);
-        if (r < rows && c < ng) {
-          const int64_t x = r / bn;
-          const int nu = (int)(r - x * bn);
-          out[(x * nbf + boff[N] + nu) * (int64_t)ldo + c] = alpha * acc[i][j][e];
-        }
-      }
-}
-
-// ---------------------------------------------------------------------------
-// segmented split-K NT GEMM: partial[chunk][m][n] = sum_{s in chunk} sum_k
-//   A[s*sstride + m*lda + k] * B[s*sstride + n*lda + k]
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_seg_gemm_nt_f64(
-    const double* __restrict__ A, const double* __restrict__ B, int nseg, int64_t sstride,
-    int lda, int klen, int n, const int2* __restrict__ tiles, int ntile, int segs_per_chunk,
-    double* __restrict__ partial)
-{
-  __shared__ double As[SG_BK][SG_BM + 1];
-  __shared__ double Bs[SG_BK][SG_BN + 1];
-  const int2 tl = tiles[blockIdx.x % ntile];
-  const int chunk = blockIdx.x / ntile;
-  const int s0 = chunk * segs_per_chunk;
-  const int s1 = min(nseg, s0 + segs_per_chunk);
-  const int m0 = tl.x * SG_BM, n0 = tl.y * SG_BN;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  d4 acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-  const int l_row = tid >> 2, l_k = (tid & 3) * 4;
-  const int am = m0 + l_row, bnn = n0 + l_row;
-
-  for (int s = s0; s < s1; ++s) {
-    const double* As_g = A + (int64_t)s * sstride + (int64_t)am * lda;
-    const double* Bs_g = B + (int64_t)s * sstride + (int64_t)bnn * lda;
-    for (int k0 = 0; k0 < klen; k0 += SG_BK) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int kk = k0 + l_k + q;
-        As[l_k + q][l_row] = (am < n && kk < klen) ? As_g[kk] : 0.0;
-        Bs[l_k + q][l_row] = (bnn < n && kk < klen) ? Bs_g[kk] : 0.0;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int kq = 0; kq < SG_BK; kq += 4) {
-        const int kr = kq + (lane >> 4);
-        const double a0 = As[kr][wm * 32 + (lane & 15)];
-        const double a1 = As[kr][wm * 32 + 16 + (lane & 15)];
-        const double b0 = Bs[kr][wn * 32 + (lane & 15)];
-        const double b1 = Bs[kr][wn * 32 + 16 + (lane & 15)];
-        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
-      }
-      __syncthreads();
-    }
-  }
-  double* P = partial + (int64_t)chunk * n * n;
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int r = m0 + wm * 32 + i * 16 + (lane >> 4) + 4 * e;
-        const int c = n0 + wn * 32 + j * 16 + (lane & 15);
-        if (r < n && c < n) P[(int64_t)r * n + c] = acc[i][j][e];
-      }
-}
 
 // out[r][c] = out[c][r] = alpha * sum_ch (P[ch][r][c] (+ P[ch][c][r] if sym2)), r <= c
 __global__ void k_reduce_sym(const double* __restrict__ partial, int nchunk, int n, int sym2,

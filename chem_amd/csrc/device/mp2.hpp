@@ -1,0 +1,26 @@
+// SOS-AO-MP2 device driver.
+#pragma once
+
+#include <vector>
+
+#include "dev.hpp"
+
+namespace mchip {
+
+struct Mp2Input {
+  int nbf = 0, nocc = 0;
+  const double* S = nullptr;     // host overlap, row-major
+  const double* C = nullptr;     // host MO coefficients, row-major nbf x nbf (occupied first)
+  const double* eps = nullptr;   // host orbital energies
+  std::vector<double> omega, alpha;
+  std::vector<int> bsz;          // AO block sizes
+  int mo_split = 5;
+  double filter_eps = 1e-10;
+};
+
+// Returns E = sum_k e_k (positive, unscaled); terms[k] = e_k; t_z = seconds
+// spent in the Z builder (M2-M5) over all points.
+double mp2_sos(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in, std::vector<double>& terms,
+               double* t_z);
+
+}  // namespace mchip

@@ -614,6 +614,15 @@ void shell3c(const Shell& a, const Shell& b, const std::vector<PrimPair>& pairs,
 
 }  // namespace
 
+void aux_hermite_coeffs(int l, double gamma, double coef, double* H, int stride)
+{
+  double tmp[(2 * LMAX + 1) * nherm(LMAX)];
+  aux_ket_hermite(l, gamma, coef, tmp);
+  const int NH = nherm(l);
+  for (int m = 0; m < nsph(l); ++m)
+    for (int i = 0; i < NH; ++i) H[m * stride + i] = tmp[m * NH + i];
+}
+
 void ints_3c2e_shell(const Basis& ao, const Basis& aux, int ia, int ib, int ic, double* out)
 {
   std::vector<PrimPair> pairs;

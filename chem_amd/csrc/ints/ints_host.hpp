@@ -28,6 +28,10 @@ void ints_2c2e(const Basis& aux, double* V);
 void schwarz_ao(const Basis& bas, double* Q);
 void schwarz_aux(const Basis& aux, double* Q);
 
+// Ket-side Hermite coefficients of one uncontracted aux primitive, times
+// (-1)^{t+u+v}: H[m*stride + herm_index(t,u,v)].
+void aux_hermite_coeffs(int l, double gamma, double coef, double* H, int stride);
+
 // One shell triple (c | a b), layout out[m_c][sa][sb].
 void ints_3c2e_shell(const Basis& ao, const Basis& aux, int ia, int ib, int ic, double* out);
 

@@ -203,5 +203,50 @@ MD_HD void hermite_R(int L, double alpha, double X, double Y, double Z,
   for (int i = 0; i < NH; ++i) R[i] = W[i];
 }
 
+// Same result as hermite_R with two level buffers of nherm(L) entries:
+// level n (R^n for degree <= L-n) is built from level n+1, n = L..0.
+// cur/nxt must each hold nherm(L) doubles; the result ends in R (== cur or nxt).
+MD_HD const double* hermite_R2(int L, double alpha, double X, double Y, double Z,
+                               const double* boys_table, double* cur, double* nxt)
+{
+  double F[BOYS_MMAX + 1];
+  double T = alpha * (X * X + Y * Y + Z * Z);
+  boys(L, T, boys_table, F);
+  double m2a = -2.0 * alpha;
+  double pw[BOYS_MMAX + 1];
+  pw[0] = 1.0;
+  for (int n = 1; n <= L; ++n) pw[n] = pw[n - 1] * m2a;
+  cur[0] = pw[L] * F[L];                 // level L: only R^L_000
+  for (int n = L - 1; n >= 0; --n) {
+    const int top = L - n;               // max degree at level n
+    nxt[0] = pw[n] * F[n];
+    for (int deg = 1; deg <= top; ++deg) {
+      for (int t = deg; t >= 0; --t) {
+        for (int u = deg - t; u >= 0; --u) {
+          const int v = deg - t - u;
+          double val;
+          if (t > 0) {
+            val = X * cur[herm_index(t - 1, u, v)];
+            if (t > 1) val += (t - 1) * cur[herm_index(t - 2, u, v)];
+          }
+          else if (u > 0) {
+            val = Y * cur[herm_index(t, u - 1, v)];
+            if (u > 1) val += (u - 1) * cur[herm_index(t, u - 2, v)];
+          }
+          else {
+            val = Z * cur[herm_index(t, u, v - 1)];
+            if (v > 1) val += (v - 1) * cur[herm_index(t, u, v - 2)];
+          }
+          nxt[herm_index(t, u, v)] = val;
+        }
+      }
+    }
+    double* sw = cur;
+    cur = nxt;
+    nxt = sw;
+  }
+  return cur;
+}
+
 }  // namespace md
 }  // namespace mchip

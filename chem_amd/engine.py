@@ -100,16 +100,48 @@ class Metric:
 class DeviceB:
     """B(X|mu nu) resident in HBM, pair-slab layout."""
 
-    def __init__(self, ctx, host_b):
-        h = ctypes.c_void_p()
-        _lib.check(_lib.lib().mchip_dbt_upload(ctx.h, host_b.h, ctypes.byref(h)))
+    def __init__(self, ctx, host_b=None, _handle=None):
+        if _handle is None:
+            h = ctypes.c_void_p()
+            _lib.check(_lib.lib().mchip_dbt_upload(ctx.h, host_b.h, ctypes.byref(h)))
+        else:
+            h = _handle
         self.h = h
         self.ctx = ctx
-        self.nbf = host_b.nbf
-        self.naux = host_b.naux
-        self.nelem = host_b.nelem
+        dims = (ctypes.c_int * 4)()
+        ne = ctypes.c_int64()
+        _lib.check(_lib.lib().mchip_dbt_info(h, dims, ctypes.byref(ne)))
+        self.naux, self.nbf, self.nxb, self.nbb = dims[0], dims[1], dims[2], dims[3]
+        self.nelem = ne.value
         self.transformed = False
         ctx._adopt(self)
+
+    @classmethod
+    def build_on_gpu(cls, ctx, mol, eps, mask=None):
+        """Three-centre integrals computed, screened and laid out on the GPU."""
+        from . import ints
+        if mask is None:
+            mask, _, _ = ints.schwarz_mask(mol, eps)
+        mask = np.ascontiguousarray(mask, dtype=np.uint8)
+        bsh = np.array([len(b) for b in mol.blocks], dtype=np.int32)
+        xsh = np.array([len(b) for b in mol.dfblocks], dtype=np.int32)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().mchip_dbt_build_gpu(ctx.h, mol.basis_handle.h, mol.dfbasis_handle.h,
+                                                  len(bsh), _lib.iptr(bsh), len(xsh), _lib.iptr(xsh),
+                                                  _lib.u8ptr(mask), float(eps), ctypes.byref(h)))
+        obj = cls(ctx, _handle=h)
+        obj.xsz, obj.bsz = list(mol.x), list(mol.b)
+        return obj
+
+    def mask(self):
+        m = np.zeros((self.nxb, self.nbb, self.nbb), dtype=np.uint8)
+        _lib.check(_lib.lib().mchip_dbt_mask(self.h, _lib.u8ptr(m)))
+        return m
+
+    def dense_rows(self, x0, x1):
+        out = np.zeros((x1 - x0, self.nbf, self.nbf))
+        _lib.check(_lib.lib().mchip_dbt_dense_rows(self.h, x0, x1, _lib.dptr(out)))
+        return out
 
     def transform(self, metric):
         _lib.check(_lib.lib().mchip_dbt_transform(self.h, metric.h))

@@ -1,0 +1,44 @@
+"""SOS-AO-MP2 driver (host side of mp::mpmod, src/mp/mp_compute.cpp).
+
+Laplace points from the minimax solver (chem_amd.laplace) on
+[2(e_LUMO - e_HOMO), 2(e_max - e_min)] (mp_compute.cpp:64-69); the
+pseudo-densities, pivoted Cholesky, Z builder and energy run on the GPU
+(mchip_mp2_sos).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib, laplace
+
+C_OS = 1.3   # mpmod.hpp:26
+
+
+def laplace_for(eps, nocc, nlap):
+    ymin = 2.0 * (eps[nocc] - eps[nocc - 1])
+    ymax = 2.0 * (eps[-1] - eps[0])
+    w, a, err = laplace.laplace_points(nlap, ymin, ymax)
+    return w, a, err
+
+
+def sos_mp2(mol, dB, metric, S, C, eps, nlap=5, mo_split=5, filter_eps=1e-10, points=None):
+    """Returns dict(energy (positive sum), scaled (1.3 x), terms, omega, alpha, t_z)."""
+    n, nocc = mol.nbf, mol.nocc
+    if points is None:
+        w, a, _ = laplace_for(eps, nocc, nlap)
+    else:
+        w, a = points
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    C = np.ascontiguousarray(C, dtype=np.float64)
+    eps = np.ascontiguousarray(eps, dtype=np.float64)
+    bsz = np.array(mol.b, dtype=np.int32)
+    e = ctypes.c_double()
+    tz = ctypes.c_double()
+    terms = np.zeros(len(w))
+    _lib.check(_lib.lib().mchip_mp2_sos(dB.h, metric.h, n, nocc, _lib.dptr(S), _lib.dptr(C),
+                                        _lib.dptr(eps), len(w), _lib.dptr(w), _lib.dptr(a), len(bsz),
+                                        _lib.iptr(bsz), mo_split, filter_eps, ctypes.byref(e),
+                                        _lib.dptr(terms), ctypes.byref(tz)))
+    return dict(energy=e.value, scaled=C_OS * e.value, terms=terms, omega=w, alpha=a, t_z=tz.value)

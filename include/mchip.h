@@ -130,6 +130,16 @@ int mchip_ctx_phase_times(mchip_ctx c, double* out, int n);
  * (dbcsr_btensor.hpp:1133-1348) become no-ops: B stays resident. */
 int mchip_dbt_upload(mchip_ctx c, mchip_btensor host, mchip_dbt* out);
 int mchip_dbt_destroy(mchip_dbt t);
+/* Compute B on the GPU (same screening, filter and layout as
+ * mchip_btensor_build; AO shells up to d, uncontracted aux basis). */
+int mchip_dbt_build_gpu(mchip_ctx c, mchip_basis ao, mchip_basis aux, int nbb,
+                        const int* ao_blk_shells, int nxb, const int* aux_blk_shells,
+                        const uint8_t* schwarz_mask, double filter_eps, mchip_dbt* out);
+/* dims[0..3] = naux, nbf, nxb, nbb; *nelem = stored elements. */
+int mchip_dbt_info(mchip_dbt t, int* dims, int64_t* nelem);
+int mchip_dbt_mask(mchip_dbt t, uint8_t* mask);
+/* Dense host copy of aux rows x0 <= X < x1 of the device B. */
+int mchip_dbt_dense_rows(mchip_dbt t, int x0, int x1, double* dense);
 /* Copy the device slabs back (same layout as the host container). */
 int mchip_dbt_download(mchip_dbt t, double* host);
 
@@ -182,6 +192,21 @@ int mchip_scf_run(mchip_dbt t, mchip_metric m, int nbf, int nocc, int nbb, const
 /* Of the last mchip_scf_run on this thread: the energy with the iteration's
  * own K and the seconds of the final FP64 K build (mixed mode). */
 int mchip_scf_last_extra(double* energy_iter, double* t_final_k);
+
+/* ------------------------------------------------------------------------
+ * SOS-AO-MP2 on the GPU: mp::mpmod::compute with LLMP_FULL_Z
+ * (src/mp/mp_compute.cpp:27-407, src/mp/z_full.cpp:69-397).  t must hold the
+ * untransformed B(X|mu nu) (RI auxiliary basis), m its Coulomb metric.
+ * S, C (nbf x nbf row-major, occupied columns first) and eps are host data;
+ * omega/alpha are the nlap Laplace weights/exponents (mchip_laplace_minimax
+ * scaled to [ymin, ymax]).  energy = sum_k e_k (positive, unscaled, as
+ * printed at mp_compute.cpp:384-389); terms[k] = e_k; t_z = seconds in the
+ * Z builder (M2-M5).
+ * ---------------------------------------------------------------------- */
+int mchip_mp2_sos(mchip_dbt t, mchip_metric m, int nbf, int nocc, const double* S, const double* C,
+                  const double* eps, int nlap, const double* omega, const double* alpha, int nbb,
+                  const int* bsz, int mo_split, double filter_eps, double* energy, double* terms,
+                  double* t_z);
 
 /* Test hook: one v_mfma_f64_16x16x4_f64 on a single wave.  a_lane/b_lane hold
  * the operand each of the 64 lanes supplies, out[lane*4+e] the accumulator
