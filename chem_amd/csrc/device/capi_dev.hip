@@ -193,6 +193,7 @@ extern "C" int mchip_dbt_download(mchip_dbt t, double* host)
 {
   MCHIP_TRY
   need_device(t->ctx);
+  HIP_CHECK(hipStreamSynchronize(t->ctx->c.stream));
   if (t->t.L.total)
     HIP_CHECK(hipMemcpy(host, t->t.data64.p, t->t.data64.bytes, hipMemcpyDeviceToHost));
   MCHIP_CATCH
@@ -334,6 +335,72 @@ extern "C" int mchip_dfk_mo_f32(mchip_dbt t, const double* C, int nocc, int occ_
   MCHIP_CATCH
 }
 
+extern "C" int mchip_dbt_make_bf16(mchip_dbt t)
+{
+  MCHIP_TRY
+  need_device(t->ctx);
+  dbt_make_bf16(t->ctx->c, t->t);
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dfk_mo_bf16_dev(mchip_dbt t, const double* C_dev, int ldc, int nocc,
+                                     int occ_nbatches, double* K_dev)
+{
+  MCHIP_TRY
+  need_device(t->ctx);
+  dfk_mo_bf16(t->ctx->c, t->t, C_dev, ldc, nocc, K_dev, occ_nbatches);
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dfk_mo_bf16(mchip_dbt t, const double* C, int nocc, int occ_nbatches, double* K)
+{
+  MCHIP_TRY
+  need_device(t->ctx);
+  Ctx& c = t->ctx->c;
+  const int nbf = t->t.L.nbf;
+  const size_t kb = (size_t)nbf * nbf * sizeof(double);
+  DevBuf dC((size_t)nbf * std::max(nocc, 1) * sizeof(double)), dK(kb);
+  if (nocc > 0) HIP_CHECK(hipMemcpyAsync(dC.p, C, (size_t)nbf * nocc * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  dfk_mo_bf16(c, t->t, dC.as<double>(), nocc, nocc, dK.as<double>(), occ_nbatches);
+  HIP_CHECK(hipMemcpyAsync(K, dK.p, kb, hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dfk_ao_dev(mchip_dbt t, mchip_metric m, const double* P_dev, int form,
+                                int occ_nbatches, double* K_dev)
+{
+  MCHIP_TRY
+  need_device(t->ctx);
+  if (!t->t.transformed && !m) throw std::runtime_error("mchip_dfk_ao: the metric is required for an untransformed B");
+  dfk_ao(t->ctx->c, t->t, m ? &m->m : nullptr, P_dev, K_dev, occ_nbatches, form);
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dfk_ao(mchip_dbt t, mchip_metric m, const double* P, int form, int occ_nbatches,
+                            double* K)
+{
+  MCHIP_TRY
+  need_device(t->ctx);
+  if (!t->t.transformed && !m) throw std::runtime_error("mchip_dfk_ao: the metric is required for an untransformed B");
+  Ctx& c = t->ctx->c;
+  const int nbf = t->t.L.nbf;
+  const size_t kb = (size_t)nbf * nbf * sizeof(double);
+  DevBuf dP(kb), dK(kb);
+  HIP_CHECK(hipMemcpyAsync(dP.p, P, kb, hipMemcpyHostToDevice, c.stream));
+  dfk_ao(c, t->t, m ? &m->m : nullptr, dP.as<double>(), dK.as<double>(), occ_nbatches, form);
+  HIP_CHECK(hipMemcpyAsync(K, dK.p, kb, hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dfk_bf16_tile(mchip_dbt t, int ng, int* bn)
+{
+  MCHIP_TRY
+  *bn = dfk_bf16_tuned_bn(t->t, ng);
+  MCHIP_CATCH
+}
+
 extern "C" int mchip_scf_run(mchip_dbt t, mchip_metric m, int nbf, int nocc, int nbb,
                              const int* bsz, const double* S, const double* H, double enuc,
                              double threshold, int max_iter, int do_diis, int diis_start,
@@ -376,6 +443,13 @@ extern "C" int mchip_scf_run(mchip_dbt t, mchip_metric m, int nbf, int nocc, int
     if (!t->t.transformed || !t->t.data32.p) throw std::runtime_error("mchip_scf_run: mixed mode needs the transformed B and its FP32 copy");
     kb = [&](int, const double*, const double* Cocc, double* K) {
       dfk_mo_f32(c, t->t, Cocc, nocc, nocc, K, occ_nbatches);
+    };
+    kfinal = k64;
+  }
+  else if (kmode == 3) {
+    if (!t->t.transformed || !t->t.data_hi.p) throw std::runtime_error("mchip_scf_run: BF16 mode needs the transformed B and its BF16 split copy");
+    kb = [&](int, const double*, const double* Cocc, double* K) {
+      dfk_mo_bf16(c, t->t, Cocc, nocc, nocc, K, occ_nbatches);
     };
     kfinal = k64;
   }

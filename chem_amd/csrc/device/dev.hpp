@@ -128,7 +128,11 @@ struct DevBT {
   BLayout L;                 // host copy of the layout
   DevBuf data64;             // FP64 slabs (B or B~)
   DevBuf data32;             // FP32 slabs (optional)
-  DevBuf data_hi, data_lo;   // BF16 split slabs (optional)
+  DevBuf data_hi, data_lo;   // BF16 split slabs (optional), rows padded to ldp[N]
+  DevBuf ldp, off16, ldp_pre;               // padded row length, slab offsets, prefix of ldp
+  std::vector<int> h_ldp;
+  std::vector<int64_t> h_off16, h_ldp_pre;
+  std::map<int, int> bf16_bn;               // ng -> tuned column tile of the BF16 slab GEMM
   DevBuf slab_off, ncol, bsz, boff, colmap, colmap_off, pg_off;
   std::vector<int> h_colmap, h_colmap_off;
   std::vector<int64_t> h_pg_off;            // prefix of bsz[N]*ncol[N]

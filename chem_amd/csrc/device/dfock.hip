@@ -17,6 +17,7 @@
 #include "dfock.hpp"
 #include "ints_dev.hpp"
 #include "kernels.hpp"
+#include "mp2.hpp"
 
 namespace mchip {
 
@@ -285,7 +286,7 @@ void dfj(Ctx& c, DevBT& t, const Metric* m, const double* P, double* J, double p
                        t.bsz.as<int>(), t.pg_off.as<int64_t>(), Pg, naux, gpart);
   }
   else {
-    hipLaunchKernelGGL(k_fill, dim3(nblk(n_gp, 256)), dim3(256), 0, c.stream, gpart, (int64_t)n_gp, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(gs_blocks(n_gp)), dim3(256), 0, c.stream, gpart, (int64_t)n_gp, 0.0);
   }
   hipLaunchKernelGGL(k_reduce_slabs, dim3(nblk(naux, 256)), dim3(256), 0, c.stream, gpart, nbb, naux,
                      pscale, g);
@@ -297,13 +298,13 @@ void dfj(Ctx& c, DevBT& t, const Metric* m, const double* P, double* J, double p
     RB_CHECK(rocblas_dgemv(c.blas, rocblas_operation_transpose, naux, naux, &one, m->Vinv.as<double>(),
                            naux, g, 1, &zero, gq, 1));
   }
-  hipLaunchKernelGGL(k_fill, dim3(nblk((int64_t)nbf * nbf, 256)), dim3(256), 0, c.stream, J,
+  hipLaunchKernelGGL(k_fill, dim3(gs_blocks((int64_t)nbf * nbf)), dim3(256), 0, c.stream, J,
                      (int64_t)nbf * nbf, 0.0);
   if (pg > 0) {
     hipLaunchKernelGGL(k_j_cols, dim3(gx, nbb, nch), dim3(256), 0, c.stream, t.data64.as<double>(),
                        t.slab_off.as<int64_t>(), t.ncol.as<int>(), t.bsz.as<int>(), t.pg_off.as<int64_t>(),
                        gq, naux, xchunk, pg, Jpart);
-    hipLaunchKernelGGL(k_reduce_chunks, dim3(nblk(pg, 256)), dim3(256), 0, c.stream, Jpart, nch, pg, Jg);
+    hipLaunchKernelGGL(k_reduce_chunks, dim3(gs_blocks(pg)), dim3(256), 0, c.stream, Jpart, nch, pg, Jg);
     hipLaunchKernelGGL(k_pair_scatter, dim3(gx, nbb), dim3(256), 0, c.stream, Jg, nbf, t.ncol.as<int>(),
                        t.bsz.as<int>(), t.boff.as<int>(), t.colmap.as<int>(), t.colmap_off.as<int>(),
                        t.pg_off.as<int64_t>(), J);
@@ -318,7 +319,7 @@ void dfk_mo(Ctx& c, DevBT& t, const Metric* m, const double* C, int ldc, int noc
   if (!t.transformed && !m) throw std::runtime_error("DFMO-K needs the metric for an untransformed B");
   const int naux = L.naux, nbf = L.nbf;
   if (nocc <= 0) {
-    hipLaunchKernelGGL(k_fill, dim3(nblk((int64_t)nbf * nbf, 256)), dim3(256), 0, c.stream, K,
+    hipLaunchKernelGGL(k_fill, dim3(gs_blocks((int64_t)nbf * nbf)), dim3(256), 0, c.stream, K,
                        (int64_t)nbf * nbf, 0.0);
     return;
   }
@@ -365,7 +366,7 @@ void dbt_make_f32(Ctx& c, DevBT& t)
   if (!t.transformed) throw std::runtime_error("the FP32 copy is made from the transformed B");
   t.data32.alloc((size_t)t.L.total * sizeof(float));
   if (t.L.total)
-    hipLaunchKernelGGL(k_to_f32, dim3(nblk(t.L.total, 256)), dim3(256), 0, c.stream,
+    hipLaunchKernelGGL(k_to_f32, dim3(gs_blocks(t.L.total)), dim3(256), 0, c.stream,
                        t.data64.as<double>(), t.data32.as<float>(), (int64_t)t.L.total);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(c.stream));
@@ -377,7 +378,7 @@ void dfk_mo_f32(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K,
   if (!t.data32.p && L.total) throw std::runtime_error("FP32 copy of B missing (mchip_dbt_make_f32)");
   const int naux = L.naux, nbf = L.nbf;
   if (nocc <= 0) {
-    hipLaunchKernelGGL(k_fill, dim3(nblk((int64_t)nbf * nbf, 256)), dim3(256), 0, c.stream, K,
+    hipLaunchKernelGGL(k_fill, dim3(gs_blocks((int64_t)nbf * nbf)), dim3(256), 0, c.stream, K,
                        (int64_t)nbf * nbf, 0.0);
     return;
   }
@@ -466,6 +467,220 @@ void slab_syrk(Ctx& c, DevBT& t, const double* A, const double* B, double* Z, De
                      nch, n, 0, 1.0, Z);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace mchip
+
+namespace mchip {
+
+// ---------------------------------------------------------------------------
+// BF16 split-precision exchange on the transformed B~
+// ---------------------------------------------------------------------------
+
+void dbt_make_bf16(Ctx& c, DevBT& t)
+{
+  if (!t.transformed) throw std::runtime_error("the BF16 copy is made from the transformed B");
+  const BLayout& L = t.L;
+  t.h_ldp.assign(L.nbb, 0);
+  t.h_off16.assign(L.nbb + 1, 0);
+  t.h_ldp_pre.assign(L.nbb + 1, 0);
+  for (int n = 0; n < L.nbb; ++n) {
+    t.h_ldp[n] = (L.ncol[n] + 31) / 32 * 32;
+    t.h_off16[n + 1] = t.h_off16[n] + (int64_t)L.naux * L.bsz[n] * t.h_ldp[n];
+    t.h_ldp_pre[n + 1] = t.h_ldp_pre[n] + t.h_ldp[n];
+  }
+  t.ldp = upload_vec(t.h_ldp, c.stream);
+  t.off16 = upload_vec(t.h_off16, c.stream);
+  t.ldp_pre = upload_vec(t.h_ldp_pre, c.stream);
+  const int64_t tot = t.h_off16[L.nbb];
+  t.data_hi.alloc((size_t)std::max<int64_t>(tot, 1) * sizeof(uint16_t));
+  t.data_lo.alloc((size_t)std::max<int64_t>(tot, 1) * sizeof(uint16_t));
+  if (tot > 0 && L.nbb > 0)
+    hipLaunchKernelGGL(k_split_slabs, dim3(2048, L.nbb), dim3(256), 0, c.stream, t.data64.as<double>(),
+                       t.slab_off.as<int64_t>(), t.ncol.as<int>(), t.bsz.as<int>(), L.naux,
+                       t.ldp.as<int>(), t.off16.as<int64_t>(), t.data_hi.as<uint16_t>(),
+                       t.data_lo.as<uint16_t>());
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  t.bf16_bn.clear();
+}
+
+// Tile table of the BF16 slab GEMM (BM = 128, column tile BN), padded with
+// empty tiles to a multiple of 8 for the XCD mapping.
+static const Tile* bf16_tiles(Ctx& c, DevBT& t, int BN, int ng, int* ntiles)
+{
+  auto key = std::make_pair(-BN, ng);
+  auto it = t.tiles.find(key);
+  if (it == t.tiles.end()) {
+    std::vector<Tile> v;
+    // slabs without significant pairs still get tiles: they write the zero rows of W
+    for (int n = 0; n < t.L.nbb; ++n) {
+      const int64_t rows = (int64_t)t.L.naux * t.L.bsz[n];
+      for (int64_t m0 = 0; m0 < rows; m0 += SG_BM)
+        for (int n0 = 0; n0 < ng; n0 += BN) v.push_back(Tile{n, (int)m0, n0, 0});
+    }
+    while (v.size() % 8) v.push_back(Tile{-1, 0, 0, 0});
+    t.ntiles[key] = (int)v.size();
+    t.tiles[key] = upload_vec(v, c.stream);
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    it = t.tiles.find(key);
+  }
+  *ntiles = t.ntiles[key];
+  return it->second.as<Tile>();
+}
+
+static void slab_gemm_bf16(Ctx& c, DevBT& t, int BN, const uint16_t* ghi, const uint16_t* glo, int ng,
+                           float* W, int ldw)
+{
+  int nt = 0;
+  const Tile* tiles = bf16_tiles(c, t, BN, ng, &nt);
+  if (nt == 0) return;
+#define MCHIP_BF16_SLAB(BNV)                                                                              \
+  hipLaunchKernelGGL((k_slab_gemm_bf16x3<SG_BM, BNV>), dim3(nt), dim3(256), 0, c.stream,                  \
+                     t.data_hi.as<uint16_t>(), t.data_lo.as<uint16_t>(), t.off16.as<int64_t>(),           \
+                     t.ldp.as<int>(), t.ldp_pre.as<int64_t>(), t.bsz.as<int>(), t.boff.as<int>(), tiles,  \
+                     t.L.naux, t.L.nbf, ghi, glo, ng, W, ldw)
+  switch (BN) {
+    case 64: MCHIP_BF16_SLAB(64); break;
+    case 128: MCHIP_BF16_SLAB(128); break;
+    case 160: MCHIP_BF16_SLAB(160); break;
+    default: throw std::runtime_error("BF16 slab GEMM: unsupported column tile");
+  }
+#undef MCHIP_BF16_SLAB
+  HIP_CHECK(hipGetLastError());
+}
+
+// Column tile for ng occupied columns: every candidate is timed once on the
+// live operands (after one warm launch) and the fastest is kept per ng.
+static int tune_bf16_bn(Ctx& c, DevBT& t, const uint16_t* ghi, const uint16_t* glo, int ng, float* W,
+                        int ldw)
+{
+  auto it = t.bf16_bn.find(ng);
+  if (it != t.bf16_bn.end()) return it->second;
+  const int cand[3] = {64, 128, 160};
+  int best = 128;
+  float best_ms = 1e30f;
+  for (int BN : cand) {
+    if (BN > 64 && ng <= BN / 2) continue;   // mostly empty tiles
+    slab_gemm_bf16(c, t, BN, ghi, glo, ng, W, ldw);
+    HIP_CHECK(hipEventRecord(c.ev0, c.stream));
+    slab_gemm_bf16(c, t, BN, ghi, glo, ng, W, ldw);
+    HIP_CHECK(hipEventRecord(c.ev1, c.stream));
+    HIP_CHECK(hipEventSynchronize(c.ev1));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = BN;
+    }
+  }
+  t.bf16_bn[ng] = best;
+  return best;
+}
+
+int dfk_bf16_tuned_bn(const DevBT& t, int ng)
+{
+  auto it = t.bf16_bn.find(ng);
+  return it == t.bf16_bn.end() ? 0 : it->second;
+}
+
+// K = -sum_X W_X^T W_X with W = B~ C, both contractions BF16x3 on MFMA, FP32
+// accumulation inside a chunk of aux functions, FP64 across chunks.
+void dfk_mo_bf16(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K, int occ_nbatches)
+{
+  const BLayout& L = t.L;
+  if (!t.data_hi.p || t.h_ldp.size() != (size_t)L.nbb)
+    throw std::runtime_error("BF16 copy of B missing (mchip_dbt_make_bf16)");
+  const int naux = L.naux, nbf = L.nbf;
+  if (nocc <= 0) {
+    hipLaunchKernelGGL(k_fill, dim3(gs_blocks((int64_t)nbf * nbf)), dim3(256), 0, c.stream, K,
+                       (int64_t)nbf * nbf, 0.0);
+    return;
+  }
+  occ_nbatches = std::max(1, std::min(occ_nbatches, nocc));
+  const int per = (nocc + occ_nbatches - 1) / occ_nbatches;
+  const int nb = (nocc + per - 1) / per;
+  const int ldw = (per + 31) / 32 * 32;
+  const size_t nW = (size_t)naux * nbf * ldw;
+  const size_t nG = (size_t)t.h_ldp_pre[L.nbb] * per;
+  c.scratch_a.ensure(nW * sizeof(float) + 2 * nG * sizeof(uint16_t) + 512);
+  float* W = c.scratch_a.as<float>();
+  uint16_t* ghi = reinterpret_cast<uint16_t*>(W + nW);
+  uint16_t* glo = ghi + (nG + 127) / 128 * 128;
+  // upper 128 x 128 tiles of K (table cached with the tensor); chunks of at most 256 aux functions
+  const int nt = (nbf + SG_BM - 1) / SG_BM;
+  const int ntile = nt * (nt + 1) / 2;
+  const auto tkey = std::make_pair(0, -nbf);
+  if (t.tiles.find(tkey) == t.tiles.end()) {
+    std::vector<int2> tl;
+    for (int i = 0; i < nt; ++i)
+      for (int j = i; j < nt; ++j) tl.push_back(int2{i, j});
+    t.tiles[tkey] = upload_vec(tl, c.stream);
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+  }
+  const int2* d_tl = t.tiles[tkey].as<int2>();
+  int nch = std::max(1, std::min(naux, 2048 / std::max(1, ntile)));
+  int spc = (naux + nch - 1) / nch;
+  if (spc > 256) spc = 256;
+  nch = (naux + spc - 1) / spc;
+  c.scratch_partial.ensure((size_t)nb * nch * nbf * nbf * sizeof(double));
+  int gmax = 0;
+  for (int n = 0; n < L.nbb; ++n) gmax = std::max(gmax, t.h_ldp[n]);
+  for (int b = 0; b < nb; ++b) {
+    const int i0 = b * per;
+    const int w = std::min(per, nocc - i0);
+    {
+      PhaseTimer tm(c, PH_SPLIT);
+      if (gmax > 0 && L.nbb > 0)
+        hipLaunchKernelGGL(k_gather_ct, dim3(nblk((int64_t)w * gmax, 256), L.nbb), dim3(256), 0, c.stream,
+                           C + i0, ldc, w, t.ncol.as<int>(), t.ldp.as<int>(), t.ldp_pre.as<int64_t>(),
+                           t.colmap.as<int>(), t.colmap_off.as<int>(), ghi, glo);
+      HIP_CHECK(hipGetLastError());
+    }
+    const int BN = tune_bf16_bn(c, t, ghi, glo, w, W, ldw);
+    {
+      PhaseTimer tm(c, PH_K1);
+      slab_gemm_bf16(c, t, BN, ghi, glo, w, W, ldw);
+    }
+    PhaseTimer tm(c, PH_K3);
+    double* partial = c.scratch_partial.as<double>() + (size_t)b * nch * nbf * nbf;
+    hipLaunchKernelGGL((k_seg_syrk_bf16x3<SG_BM>), dim3(ntile * nch), dim3(256), 0, c.stream, W, naux,
+                       (int64_t)nbf * ldw, ldw, nbf, d_tl, ntile, spc, partial);
+    HIP_CHECK(hipGetLastError());
+    if (b == nb - 1)
+      hipLaunchKernelGGL(k_reduce_sym, dim3(nblk((int64_t)nbf * nbf, 256)), dim3(256), 0, c.stream,
+                         c.scratch_partial.as<double>(), nb * nch, nbf, 0, -1.0, K);
+    HIP_CHECK(hipGetLastError());
+  }
+}
+
+}  // namespace mchip
+
+namespace mchip {
+
+// Exchange from a density (DFAO_K, jkdf.cpp:510-648, and DFMEM_K,
+// jkdf.cpp:719-859): K = -sum_{X,Y} B(X|m l) V^-1(X,Y) P(l,s) B(Y|s n).  The
+// density is factorised on the GPU, P = L L^T by pivoted Cholesky, and K is the
+// occupied-factor exchange with C := L, which equals the fitted-coefficient
+// contraction term by term.  form: 0 FP64, 1 FP32, 2 BF16x3 (the low-precision
+// forms need the transformed B and its copy).
+void dfk_ao(Ctx& c, DevBT& t, const Metric* m, const double* P_dev, double* K, int occ_nbatches, int form)
+{
+  const int n = t.L.nbf;
+  DevBuf U((size_t)n * n * sizeof(double));
+  HIP_CHECK(hipMemcpyAsync(U.p, P_dev, U.bytes, hipMemcpyDeviceToDevice, c.stream));
+  int rank = 0;
+  std::vector<double> hL = pivoted_cholesky_dev(c, U, n, 0.0, &rank);
+  DevBuf L((size_t)n * std::max(rank, 1) * sizeof(double));
+  if (rank > 0)
+    HIP_CHECK(hipMemcpyAsync(L.p, hL.data(), hL.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  switch (form) {
+    case 0: dfk_mo(c, t, t.transformed ? nullptr : m, L.as<double>(), rank, rank, K, occ_nbatches); break;
+    case 1: dfk_mo_f32(c, t, L.as<double>(), rank, rank, K, occ_nbatches); break;
+    case 2: dfk_mo_bf16(c, t, L.as<double>(), rank, rank, K, occ_nbatches); break;
+    default: throw std::runtime_error("dfk_ao: unknown precision form");
+  }
+  HIP_CHECK(hipStreamSynchronize(c.stream));   // L and the host factor die here
 }
 
 }  // namespace mchip

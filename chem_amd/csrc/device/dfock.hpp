@@ -24,6 +24,10 @@ void dfk_mo(Ctx& c, DevBT& t, const Metric* m, const double* C, int ldc, int noc
 // FP32 copy of the transformed B and the FP32-MFMA exchange (FP64 output).
 void dbt_make_f32(Ctx& c, DevBT& t);
 void dfk_mo_f32(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K, int occ_nbatches);
+void dbt_make_bf16(Ctx& c, DevBT& t);
+void dfk_mo_bf16(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K, int occ_nbatches);
+int dfk_bf16_tuned_bn(const DevBT& t, int ng);
+void dfk_ao(Ctx& c, DevBT& t, const Metric* m, const double* P_dev, double* K, int occ_nbatches, int form);
 const Tile* slab_tiles(Ctx& c, DevBT& t, int ng, int* ntiles);
 // out (slab layout of B) = rows (X, nu in N) of A (lda, klen) times gathered rows of G
 void slab_out_gemm(Ctx& c, DevBT& t, const double* A, int lda, int klen, const double* G, int ldg,

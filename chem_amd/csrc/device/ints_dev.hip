@@ -160,7 +160,7 @@ __global__ void k_pair_block_norm2(const double* __restrict__ tmp, const int64_t
                                    const int* __restrict__ cp_size, const int* __restrict__ xoff, int nxb,
                                    double* __restrict__ norm2)
 {
-  const int cp = blockIdx.x / nxb, xb = blockIdx.x % nxb;
+  const int cp = blockIdx.x, xb = blockIdx.y;   // grid = (pairs, aux blocks)
   const int64_t per = cp_size[cp];
   const double* src = tmp + cp_off[cp] + (int64_t)xoff[xb] * per;
   const int64_t n = (int64_t)(xoff[xb + 1] - xoff[xb]) * per;
@@ -173,7 +173,7 @@ __global__ void k_pair_block_norm2(const double* __restrict__ tmp, const int64_t
     if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
     __syncthreads();
   }
-  if (threadIdx.x == 0) norm2[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) norm2[(int64_t)cp * nxb + xb] = red[0];
 }
 
 // copy kept blocks of canonical pair cp into slab N (block M at col_mn) and
@@ -297,12 +297,13 @@ void build_B_device(Ctx& c, const Basis& ao, const Basis& aux, const std::vector
          d_boys = upload_vec(btab, c.stream), d_cpsize = upload_vec(cpsize, c.stream),
          d_xoff = upload_vec(xoff, c.stream);
   DevBuf tmp((size_t)std::max<int64_t>(ctotal, 1) * sizeof(double));
-  hipLaunchKernelGGL(k_fill, dim3(nblk(std::max<int64_t>(ctotal, 1), 256)), dim3(256), 0, c.stream,
+  hipLaunchKernelGGL(k_fill, dim3(gs_blocks(std::max<int64_t>(ctotal, 1))), dim3(256), 0, c.stream,
                      tmp.as<double>(), std::max<int64_t>(ctotal, 1), 0.0);
   const int nchunk = (nsx + 63) / 64;
   const int64_t nblocks = (int64_t)tasks.size() * nchunk;
-  // launch in slices to keep each grid within 2^31 blocks
-  const int64_t slice = 1 << 30;
+  // launch in slices of whole tasks: a dispatch holds fewer than 2^32
+  // work-items (64 per block), and each slice starts at a task boundary
+  const int64_t slice = std::max<int64_t>(1, ((int64_t)1 << 25) / nchunk) * nchunk;
   for (int64_t b0 = 0; b0 < nblocks; b0 += slice) {
     const int64_t nb = std::min(slice, nblocks - b0);
     const int64_t t0 = b0 / nchunk;
@@ -317,7 +318,7 @@ void build_B_device(Ctx& c, const Basis& ao, const Basis& aux, const std::vector
   // block norms -> final mask (host)
   DevBuf norm2((size_t)std::max(1, ncp * nxb) * sizeof(double));
   if (ncp)
-    hipLaunchKernelGGL(k_pair_block_norm2, dim3(ncp * nxb), dim3(256), 0, c.stream, tmp.as<double>(),
+    hipLaunchKernelGGL(k_pair_block_norm2, dim3(ncp, nxb), dim3(256), 0, c.stream, tmp.as<double>(),
                        d_cpoff.as<int64_t>(), d_cpsize.as<int>(), d_xoff.as<int>(), nxb, norm2.as<double>());
   std::vector<double> hn((size_t)ncp * nxb);
   if (ncp) HIP_CHECK(hipMemcpyAsync(hn.data(), norm2.p, hn.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -348,7 +349,7 @@ void build_B_device(Ctx& c, const Basis& ao, const Basis& aux, const std::vector
          d_ncol = upload_vec(L.ncol, c.stream), d_so = upload_vec(so, c.stream),
          d_keep = upload_vec(keep, c.stream), d_xblk = upload_vec(xblk, c.stream);
   DevBuf out((size_t)std::max<int64_t>(L.total, 1) * sizeof(double));
-  hipLaunchKernelGGL(k_fill, dim3(nblk(std::max<int64_t>(L.total, 1), 256)), dim3(256), 0, c.stream,
+  hipLaunchKernelGGL(k_fill, dim3(gs_blocks(std::max<int64_t>(L.total, 1))), dim3(256), 0, c.stream,
                      out.as<double>(), std::max<int64_t>(L.total, 1), 0.0);
   const int xper = 64;
   if (ncp)

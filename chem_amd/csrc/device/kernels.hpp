@@ -1,6 +1,9 @@
 // Kernel declarations shared by the device drivers.
 #pragma once
 
+#include <algorithm>
+#include <stdexcept>
+
 #include "dev.hpp"
 
 namespace mchip {
@@ -48,6 +51,38 @@ __global__ void k_axpby(double* y, const double* x, int64_t n, double a, double 
 __global__ void k_to_f32(const double* in, float* out, int64_t n);
 __global__ void k_rows_to_f32(const double* in, int ld64, int rows, int ncols, float* out);
 
-inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+// BF16 split-precision exchange (kernels_bf16.hip)
+__global__ void k_split_slabs(const double* data64, const int64_t* slab_off, const int* ncol,
+                              const int* bsz, int naux, const int* ldp, const int64_t* off16,
+                              uint16_t* hi, uint16_t* lo);
+__global__ void k_gather_ct(const double* C, int ldc, int ng, const int* ncol, const int* ldp,
+                            const int64_t* ldp_pre, const int* colmap, const int* colmap_off,
+                            uint16_t* ghi, uint16_t* glo);
+template <int BM, int BN>
+__global__ void k_slab_gemm_bf16x3(const uint16_t* bhi, const uint16_t* blo, const int64_t* off16,
+                                   const int* ldp, const int64_t* ldp_pre, const int* bsz,
+                                   const int* boff, const Tile* tiles, int naux, int nbf,
+                                   const uint16_t* ghi, const uint16_t* glo, int ng, float* out,
+                                   int ldo);
+template <int BM>
+__global__ void k_seg_syrk_bf16x3(const float* W, int nseg, int64_t sstride, int ldw, int n,
+                                  const int2* tiles, int ntile, int segs_per_chunk, double* partial);
+
+// Blocks of a one-thread-per-element launch.  A dispatch holds at most 2^32 - 1
+// work-items per grid dimension; larger element counts must use a grid-stride
+// kernel launched with gs_blocks.
+inline unsigned nblk(int64_t n, int bs)
+{
+  const int64_t b = (n + bs - 1) / bs;
+  if (b * bs >= ((int64_t)1 << 32)) throw std::runtime_error("1-D launch exceeds 2^32 work-items");
+  return (unsigned)b;
+}
+
+// Blocks of a grid-stride launch (bounded, at least 1).
+inline unsigned gs_blocks(int64_t n, int bs = 256)
+{
+  const int64_t b = (n + bs - 1) / bs;
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, 65536));
+}
 
 }  // namespace mchip

@@ -1,0 +1,338 @@
+// BF16 split-precision exchange kernels (v_mfma_f32_16x16x32_bf16, gfx950).
+//
+// Every FP32/FP64 operand x is carried as hi = bf16(x), lo = bf16(x - hi)
+// (16 significant bits), and a product is formed as lo*hi + hi*lo + hi*hi
+// with FP32 accumulation (the lo*lo term, 2^-16 relative, is dropped).
+// Chunk partials leave the kernels in FP64 and are reduced in FP64.
+//
+//   k_split_slabs         B~ (FP64 pair slabs) -> hi/lo slabs whose rows are
+//                         padded to ldp[N] = roundup(ncol[N], 32) (zeros)
+//   k_gather_ct           per-slab gathered, transposed occupied coefficients
+//                         G_N[i][c] = C(colmap_N[c], i) as hi/lo, row length ldp[N]
+//   k_slab_gemm_bf16x3    W(X,nu,i) = sum_c B~_N(X nu, c) G_N(i, c) -> FP32 W
+//                         (XMN,Mi->XiN of jkdf.cpp:361 on the transformed B)
+//   k_seg_syrk_bf16x3     partial[chunk](m,n) = sum_{X in chunk} sum_i W(X,m,i) W(X,n,i)
+//                         (XiM,XiN->MN of jkdf.cpp:417), W split to hi/lo on load
+//
+// Both GEMMs read their operands with the contraction index contiguous, 16
+// bytes (8 bf16) per lane, which is exactly the MFMA operand layout:
+//   A[l%16][8*(l/16)+q], B[8*(l/16)+q][l%16], D[4*(l/16)+e][l%16]
+// (pinned by tests/test_gpu_mfma_layout.py).  LDS rows hold 32 bf16 plus 8 of
+// padding (80 bytes): 16 lanes reading 16 consecutive rows hit distinct banks.
+#include "kernels.hpp"
+
+namespace mchip {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int LROW = 5;   // LDS row stride in 16-byte units (32 bf16 + 8 padding)
+
+__device__ __forceinline__ f32x4 mfma_bf16(const uint4& a, const uint4& b, f32x4 c)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                 __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
+__device__ __forceinline__ uint16_t bf16_bits(float x)
+{
+  return __builtin_bit_cast(uint16_t, (__bf16)x);
+}
+
+__device__ __forceinline__ float bf16_value(uint16_t b)
+{
+  return (float)__builtin_bit_cast(__bf16, b);
+}
+
+// hi/lo split of an FP64 value (hi from the FP32 rounding, lo from the exact remainder)
+__device__ __forceinline__ void split64(double x, uint16_t& hi, uint16_t& lo)
+{
+  hi = bf16_bits((float)x);
+  lo = bf16_bits((float)(x - (double)bf16_value(hi)));
+}
+
+__device__ __forceinline__ void split32(float x, uint16_t& hi, uint16_t& lo)
+{
+  hi = bf16_bits(x);
+  lo = bf16_bits(x - bf16_value(hi));
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; consecutive tile
+// indices are mapped onto one XCD so that tiles sharing operand rows share an
+// L2.  grid must be a multiple of 8.
+__device__ __forceinline__ int xcd_tile(int bid, int grid)
+{
+  return (bid & 7) * (grid >> 3) + (bid >> 3);
+}
+
+__global__ void k_split_slabs(const double* __restrict__ data64, const int64_t* __restrict__ slab_off,
+                              const int* __restrict__ ncol, const int* __restrict__ bsz, int naux,
+                              const int* __restrict__ ldp, const int64_t* __restrict__ off16,
+                              uint16_t* __restrict__ hi, uint16_t* __restrict__ lo)
+{
+  const int N = blockIdx.y;
+  const int K = ncol[N], ld = ldp[N];
+  const int64_t n = (int64_t)naux * bsz[N] * ld;
+  const double* src = data64 + slab_off[N];
+  uint16_t* h = hi + off16[N];
+  uint16_t* l = lo + off16[N];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / ld;
+    const int c = (int)(i - r * ld);
+    uint16_t a = 0, b = 0;
+    if (c < K) split64(src[r * K + c], a, b);
+    h[i] = a;
+    l[i] = b;
+  }
+}
+
+__global__ void k_gather_ct(const double* __restrict__ C, int ldc, int ng, const int* __restrict__ ncol,
+                            const int* __restrict__ ldp, const int64_t* __restrict__ ldp_pre,
+                            const int* __restrict__ colmap, const int* __restrict__ colmap_off,
+                            uint16_t* __restrict__ ghi, uint16_t* __restrict__ glo)
+{
+  const int N = blockIdx.y;
+  const int ld = ldp[N];
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)ng * ld) return;
+  const int i = (int)(idx / ld), c = (int)(idx - (int64_t)(idx / ld) * ld);
+  uint16_t a = 0, b = 0;
+  if (c < ncol[N]) split64(C[(int64_t)colmap[colmap_off[N] + c] * ldc + i], a, b);
+  const int64_t o = ldp_pre[N] * ng + idx;
+  ghi[o] = a;
+  glo[o] = b;
+}
+
+// One BK = 32 step on LDS tiles: 3 MFMAs per (i, j) sub-tile.
+template <int TM, int TN>
+__device__ __forceinline__ void bf16x3_step(const uint4* sAh, const uint4* sAl, const uint4* sBh,
+                                            const uint4* sBl, int wm0, int wn0, int lane,
+                                            f32x4 (&acc)[TM][TN])
+{
+  const int fr = lane & 15, fk = lane >> 4;
+  uint4 ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+    ah[i] = sAh[(wm0 + 16 * i + fr) * LROW + fk];
+    al[i] = sAl[(wm0 + 16 * i + fr) * LROW + fk];
+  }
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    bh[j] = sBh[(wn0 + 16 * j + fr) * LROW + fk];
+    bl[j] = sBl[(wn0 + 16 * j + fr) * LROW + fk];
+  }
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      acc[i][j] = mfma_bf16(al[i], bh[j], acc[i][j]);
+      acc[i][j] = mfma_bf16(ah[i], bl[j], acc[i][j]);
+      acc[i][j] = mfma_bf16(ah[i], bh[j], acc[i][j]);
+    }
+}
+
+// Grouped slab GEMM.  Tile (slab, m0, n0); slab < 0 marks padding of the
+// tile table to a multiple of 8.  Rows r = X*bsz[N] + nu of slab N, columns
+// n = occupied index i < ng, contraction over the slab's padded columns.
+template <int BM, int BN>
+__global__ __launch_bounds__(256) void k_slab_gemm_bf16x3(
+    const uint16_t* __restrict__ bhi, const uint16_t* __restrict__ blo, const int64_t* __restrict__ off16,
+    const int* __restrict__ ldp, const int64_t* __restrict__ ldp_pre, const int* __restrict__ bsz,
+    const int* __restrict__ boff, const Tile* __restrict__ tiles, int naux, int nbf,
+    const uint16_t* __restrict__ ghi, const uint16_t* __restrict__ glo, int ng, float* __restrict__ out,
+    int ldo)
+{
+  constexpr int TM = BM / 32, TN = BN / 32;
+  constexpr int CA = (BM * 4 + 255) / 256, CB = (BN * 4 + 255) / 256;   // 16-byte chunks per thread
+  static_assert(BM % 32 == 0 && BN % 32 == 0, "tile shape");
+  __shared__ uint4 sAh[2][BM * LROW], sAl[2][BM * LROW], sBh[2][BN * LROW], sBl[2][BN * LROW];
+  const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  if (t.slab < 0) return;
+  const int N = t.slab;
+  const int bn = bsz[N];
+  const int ld8 = ldp[N] >> 3;
+  const int64_t rows = (int64_t)naux * bn;
+  const uint4* Ah = reinterpret_cast<const uint4*>(bhi + off16[N]);
+  const uint4* Al = reinterpret_cast<const uint4*>(blo + off16[N]);
+  const uint4* Gh = reinterpret_cast<const uint4*>(ghi + ldp_pre[N] * ng);
+  const uint4* Gl = reinterpret_cast<const uint4*>(glo + ldp_pre[N] * ng);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave >> 1) * (BM / 2), wn0 = (wave & 1) * (BN / 2);
+  f32x4 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const uint4 zero = {0u, 0u, 0u, 0u};
+  uint4 rah[CA], ral[CA], rbh[CB], rbl[CB];
+  auto load = [&](int s) {
+    const int kc0 = s * 4;
+#pragma unroll
+    for (int j = 0; j < CA; ++j) {
+      const int q = tid + 256 * j;
+      const int64_t r = (int64_t)t.m0 + (q >> 2);
+      const int64_t o = r * ld8 + kc0 + (q & 3);
+      const bool ok = q < BM * 4 && r < rows;
+      rah[j] = ok ? Ah[o] : zero;
+      ral[j] = ok ? Al[o] : zero;
+    }
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+      const int q = tid + 256 * j;
+      const int c = t.n0 + (q >> 2);
+      const int64_t o = (int64_t)c * ld8 + kc0 + (q & 3);
+      const bool ok = q < BN * 4 && c < ng;
+      rbh[j] = ok ? Gh[o] : zero;
+      rbl[j] = ok ? Gl[o] : zero;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < CA; ++j) {
+      const int q = tid + 256 * j;
+      if (q < BM * 4) {
+        sAh[buf][(q >> 2) * LROW + (q & 3)] = rah[j];
+        sAl[buf][(q >> 2) * LROW + (q & 3)] = ral[j];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+      const int q = tid + 256 * j;
+      if (q < BN * 4) {
+        sBh[buf][(q >> 2) * LROW + (q & 3)] = rbh[j];
+        sBl[buf][(q >> 2) * LROW + (q & 3)] = rbl[j];
+      }
+    }
+  };
+  const int nsteps = ld8 >> 2;
+  if (nsteps > 0) {
+    load(0);
+    store(0);
+  }
+  __syncthreads();
+  for (int s = 0; s < nsteps; ++s) {
+    const int cur = s & 1;
+    if (s + 1 < nsteps) load(s + 1);
+    bf16x3_step<TM, TN>(sAh[cur], sAl[cur], sBh[cur], sBl[cur], wm0, wn0, lane, acc);
+    if (s + 1 < nsteps) store(cur ^ 1);
+    __syncthreads();
+  }
+  const int fc = lane & 15, fr4 = 4 * (lane >> 4);
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t r = (int64_t)t.m0 + wm0 + 16 * i + fr4 + e;
+      if (r >= rows) continue;
+      const int64_t x = r / bn;
+      const int nu = (int)(r - x * bn);
+      float* orow = out + (x * nbf + boff[N] + nu) * (int64_t)ldo;
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int c = t.n0 + wn0 + 16 * j + fc;
+        if (c < ldo) orow[c] = acc[i][j][e];
+      }
+    }
+}
+
+// Segmented SYRK on FP32 W (segment stride sstride, rows of length ldw, a
+// multiple of 32; columns past the occupied count are zero).  tiles[] lists
+// the upper 128 x 128 tiles; workgroup b handles tile b % ntile of chunk
+// b / ntile (segments [chunk*per, chunk*per + per)).
+template <int BM>
+__global__ __launch_bounds__(256) void k_seg_syrk_bf16x3(
+    const float* __restrict__ W, int nseg, int64_t sstride, int ldw, int n, const int2* __restrict__ tiles,
+    int ntile, int segs_per_chunk, double* __restrict__ partial)
+{
+  constexpr int TM = BM / 32;
+  constexpr int CA = BM * 8 / 256;   // float4 chunks per thread per operand
+  __shared__ uint4 sAh[2][BM * LROW], sAl[2][BM * LROW], sBh[2][BM * LROW], sBl[2][BM * LROW];
+  const int2 tl = tiles[blockIdx.x % ntile];
+  const int chunk = blockIdx.x / ntile;
+  const int s0 = chunk * segs_per_chunk;
+  const int s1 = min(nseg, s0 + segs_per_chunk);
+  const int m0 = tl.x * BM, n0 = tl.y * BM;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave >> 1) * (BM / 2), wn0 = (wave & 1) * (BM / 2);
+  f32x4 acc[TM][TM];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TM; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int ksteps = ldw >> 5;
+  const int nsteps = (s1 > s0) ? (s1 - s0) * ksteps : 0;
+  const float4 fz = {0.f, 0.f, 0.f, 0.f};
+  float4 ra[CA], rb[CA];
+  auto load = [&](int q) {
+    const int s = s0 + q / ksteps;
+    const int k0 = (q % ksteps) * 32;
+    const float* Wg = W + (int64_t)s * sstride + k0;
+#pragma unroll
+    for (int j = 0; j < CA; ++j) {
+      const int c = tid + 256 * j;
+      const int row = c >> 3, kq = (c & 7) * 4;
+      const int m = m0 + row, nn = n0 + row;
+      ra[j] = (m < n) ? *reinterpret_cast<const float4*>(Wg + (int64_t)m * ldw + kq) : fz;
+      rb[j] = (nn < n) ? *reinterpret_cast<const float4*>(Wg + (int64_t)nn * ldw + kq) : fz;
+    }
+  };
+  auto put = [&](uint4* dh, uint4* dl, int c, const float4& v) {
+    uint16_t h0, h1, h2, h3, l0, l1, l2, l3;
+    split32(v.x, h0, l0);
+    split32(v.y, h1, l1);
+    split32(v.z, h2, l2);
+    split32(v.w, h3, l3);
+    const int row = c >> 3, kq = c & 7;
+    uint2* ph = reinterpret_cast<uint2*>(dh + row * LROW) + kq;
+    uint2* pl = reinterpret_cast<uint2*>(dl + row * LROW) + kq;
+    *ph = uint2{(uint32_t)h0 | ((uint32_t)h1 << 16), (uint32_t)h2 | ((uint32_t)h3 << 16)};
+    *pl = uint2{(uint32_t)l0 | ((uint32_t)l1 << 16), (uint32_t)l2 | ((uint32_t)l3 << 16)};
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < CA; ++j) {
+      const int c = tid + 256 * j;
+      put(sAh[buf], sAl[buf], c, ra[j]);
+      put(sBh[buf], sBl[buf], c, rb[j]);
+    }
+  };
+  if (nsteps > 0) {
+    load(0);
+    store(0);
+  }
+  __syncthreads();
+  for (int q = 0; q < nsteps; ++q) {
+    const int cur = q & 1;
+    if (q + 1 < nsteps) load(q + 1);
+    bf16x3_step<TM, TM>(sAh[cur], sAl[cur], sBh[cur], sBl[cur], wm0, wn0, lane, acc);
+    if (q + 1 < nsteps) store(cur ^ 1);
+    __syncthreads();
+  }
+  double* P = partial + (int64_t)chunk * n * n;
+  const int fc = lane & 15, fr4 = 4 * (lane >> 4);
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TM; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int r = m0 + wm0 + 16 * i + fr4 + e;
+        const int c = n0 + wn0 + 16 * j + fc;
+        if (r < n && c < n) P[(int64_t)r * n + c] = (double)acc[i][j][e];
+      }
+}
+
+template __global__ void k_slab_gemm_bf16x3<128, 64>(const uint16_t*, const uint16_t*, const int64_t*,
+    const int*, const int64_t*, const int*, const int*, const Tile*, int, int, const uint16_t*,
+    const uint16_t*, int, float*, int);
+template __global__ void k_slab_gemm_bf16x3<128, 128>(const uint16_t*, const uint16_t*, const int64_t*,
+    const int*, const int64_t*, const int*, const int*, const Tile*, int, int, const uint16_t*,
+    const uint16_t*, int, float*, int);
+template __global__ void k_slab_gemm_bf16x3<128, 160>(const uint16_t*, const uint16_t*, const int64_t*,
+    const int*, const int64_t*, const int*, const int*, const Tile*, int, int, const uint16_t*,
+    const uint16_t*, int, float*, int);
+template __global__ void k_seg_syrk_bf16x3<128>(const float*, int, int64_t, int, int, const int2*, int,
+    int, double*);
+
+}  // namespace mchip

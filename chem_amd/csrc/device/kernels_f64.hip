@@ -114,27 +114,32 @@ __global__ void k_j_cols(const double* __restrict__ bdata, const int64_t* __rest
   Jpart[(int64_t)chunk * pg_total + pg_off[N] + e] = s;
 }
 
+// Element-wise kernels below are grid-stride loops: launch them with
+// gs_blocks(n), which keeps gridDim.x * blockDim.x below 2^32 for any n.
 __global__ void k_reduce_chunks(const double* __restrict__ part, int nchunk, int64_t n,
                                 double* __restrict__ out)
 {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double s = 0.0;
-  for (int c = 0; c < nchunk; ++c) s += part[(int64_t)c * n + i];
-  out[i] = s;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int c = 0; c < nchunk; ++c) s += part[(int64_t)c * n + i];
+    out[i] = s;
+  }
 }
 
 __global__ void k_fill(double* __restrict__ p, int64_t n, double v)
 {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
 }
 
 __global__ void k_axpby(double* __restrict__ y, const double* __restrict__ x, int64_t n,
                         double a, double b)
 {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] = a * x[i] + b * y[i];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = a * x[i] + b * y[i];
 }
 
 }  // namespace mchip

@@ -7,10 +7,12 @@
 
 namespace mchip {
 
+// grid-stride (launch with gs_blocks(n))
 __global__ void k_to_f32(const double* __restrict__ in, float* __restrict__ out, int64_t n)
 {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = (float)in[i];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (float)in[i];
 }
 
 // row-major G64 (rows x ld64, first ncols) -> G32 (rows x ncols)

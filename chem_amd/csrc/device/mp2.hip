@@ -150,10 +150,10 @@ __global__ void k_trace_sq(const double* __restrict__ Z, int n, double* __restri
 // ---- pivoted Cholesky driver -------------------------------------------------
 // Decomposes the row-major N x N matrix in U (destroyed); returns L (host,
 // N x rank row-major) in the original row order with LMO-ordered columns.
-static std::vector<double> pivoted_cholesky_dev(Ctx& c, DevBuf& U, int N, double filter_eps, int* rank_out)
+std::vector<double> pivoted_cholesky_dev(Ctx& c, DevBuf& U, int N, double filter_eps, int* rank_out)
 {
   DevBuf L((size_t)N * N * sizeof(double)), piv(2 * sizeof(double));
-  hipLaunchKernelGGL(k_fill, dim3(nblk((int64_t)N * N, 256)), dim3(256), 0, c.stream, L.as<double>(),
+  hipLaunchKernelGGL(k_fill, dim3(gs_blocks((int64_t)N * N)), dim3(256), 0, c.stream, L.as<double>(),
                      (int64_t)N * N, 0.0);
   std::vector<int> perm(N);
   std::iota(perm.begin(), perm.end(), 0);
@@ -189,7 +189,8 @@ static std::vector<double> pivoted_cholesky_dev(Ctx& c, DevBuf& U, int N, double
   }
   HIP_CHECK(hipGetLastError());
   std::vector<double> hL((size_t)N * N);
-  HIP_CHECK(hipMemcpy(hL.data(), L.p, hL.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpyAsync(hL.data(), L.p, hL.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
   // rows back to the original order
   std::vector<double> Lo((size_t)N * rank, 0.0);
   for (int r = 0; r < N; ++r)

@@ -18,6 +18,12 @@ struct Mp2Input {
   double filter_eps = 1e-10;
 };
 
+// Pivoted Cholesky of the symmetric positive semidefinite N x N matrix in U
+// (overwritten): returns L (N x rank, row-major, rows in the original order,
+// columns in LMO position order) with U = L L^T to N * eps * max|diag|
+// (math::pivinc_cd, piv_cd.cpp:125-483).
+std::vector<double> pivoted_cholesky_dev(Ctx& c, DevBuf& U, int N, double filter_eps, int* rank_out);
+
 // Returns E = sum_k e_k (positive, unscaled); terms[k] = e_k; t_z = seconds
 // spent in the Z builder (M2-M5) over all points.
 double mp2_sos(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in, std::vector<double>& terms,

@@ -222,7 +222,7 @@ void scf_run(Ctx& c, DevBT& bt, const Metric* m, const ScfInput& in, ScfOutput& 
                              P.as<double>(), n));
     }
     else {
-      hipLaunchKernelGGL(k_fill, dim3(nblk(nn, 256)), dim3(256), 0, c.stream, P.as<double>(), nn, 0.0);
+      hipLaunchKernelGGL(k_fill, dim3(gs_blocks(nn)), dim3(256), 0, c.stream, P.as<double>(), nn, 0.0);
     }
   };
 
@@ -253,9 +253,9 @@ void scf_run(Ctx& c, DevBT& bt, const Metric* m, const ScfInput& in, ScfOutput& 
     out.t_j.push_back(tj * 1e-3);
     out.t_k.push_back(tk * 1e-3);
     HIP_CHECK(hipMemcpyAsync(F.p, H.p, mb, hipMemcpyDeviceToDevice, c.stream));
-    hipLaunchKernelGGL(k_axpby, dim3(nblk(nn, 256)), dim3(256), 0, c.stream, W1.as<double>(), J.as<double>(), nn, 1.0, 0.0);
-    hipLaunchKernelGGL(k_axpby, dim3(nblk(nn, 256)), dim3(256), 0, c.stream, W1.as<double>(), K.as<double>(), nn, 1.0, 1.0);
-    hipLaunchKernelGGL(k_axpby, dim3(nblk(nn, 256)), dim3(256), 0, c.stream, F.as<double>(), W1.as<double>(), nn, 1.0, 1.0);
+    hipLaunchKernelGGL(k_axpby, dim3(gs_blocks(nn)), dim3(256), 0, c.stream, W1.as<double>(), J.as<double>(), nn, 1.0, 0.0);
+    hipLaunchKernelGGL(k_axpby, dim3(gs_blocks(nn)), dim3(256), 0, c.stream, W1.as<double>(), K.as<double>(), nn, 1.0, 1.0);
+    hipLaunchKernelGGL(k_axpby, dim3(gs_blocks(nn)), dim3(256), 0, c.stream, F.as<double>(), W1.as<double>(), nn, 1.0, 1.0);
     // error matrix
     gemm_rm(c, n, F.as<double>(), P.as<double>(), W1.as<double>());     // F P
     gemm_rm(c, n, W1.as<double>(), S.as<double>(), W2.as<double>());    // F P S
@@ -307,9 +307,9 @@ void scf_run(Ctx& c, DevBT& bt, const Metric* m, const ScfInput& in, ScfOutput& 
       diis.B = Bn;
       diis.coeffs = diis_solve(diis.B, nerr);
       if ((int)diis.trial.size() >= in.diis_min) {
-        hipLaunchKernelGGL(k_fill, dim3(nblk(nn, 256)), dim3(256), 0, c.stream, F.as<double>(), nn, 0.0);
+        hipLaunchKernelGGL(k_fill, dim3(gs_blocks(nn)), dim3(256), 0, c.stream, F.as<double>(), nn, 0.0);
         for (int i = 0; i < nerr; ++i)
-          hipLaunchKernelGGL(k_axpby, dim3(nblk(nn, 256)), dim3(256), 0, c.stream, F.as<double>(),
+          hipLaunchKernelGGL(k_axpby, dim3(gs_blocks(nn)), dim3(256), 0, c.stream, F.as<double>(),
                              diis.trial[i].as<double>(), nn, diis.coeffs[i], 1.0);
       }
     }
@@ -328,14 +328,15 @@ void scf_run(Ctx& c, DevBT& bt, const Metric* m, const ScfInput& in, ScfOutput& 
     HIP_CHECK(hipEventElapsedTime(&tk, c.ev0, c.ev1));
     out.t_final_k = tk * 1e-3;
     HIP_CHECK(hipMemcpyAsync(F.p, H.p, mb, hipMemcpyDeviceToDevice, c.stream));
-    hipLaunchKernelGGL(k_axpby, dim3(nblk(nn, 256)), dim3(256), 0, c.stream, F.as<double>(), J.as<double>(), nn, 1.0, 1.0);
-    hipLaunchKernelGGL(k_axpby, dim3(nblk(nn, 256)), dim3(256), 0, c.stream, F.as<double>(), K.as<double>(), nn, 1.0, 1.0);
+    hipLaunchKernelGGL(k_axpby, dim3(gs_blocks(nn)), dim3(256), 0, c.stream, F.as<double>(), J.as<double>(), nn, 1.0, 1.0);
+    hipLaunchKernelGGL(k_axpby, dim3(gs_blocks(nn)), dim3(256), 0, c.stream, F.as<double>(), K.as<double>(), nn, 1.0, 1.0);
     energy = dot_dev(c, H.as<double>(), P.as<double>(), nn, part) + dot_dev(c, F.as<double>(), P.as<double>(), nn, part);
   }
   out.converged = converged;
   out.iterations = iter;
   out.energy = energy + in.enuc;
   out.eps.resize(n);
+  HIP_CHECK(hipStreamSynchronize(c.stream));
   HIP_CHECK(hipMemcpy(out.eps.data(), w.p, n * sizeof(double), hipMemcpyDeviceToHost));
   out.C.resize(nn);
   // return C row-major (n x n): transpose of the column-major eigenvector matrix

@@ -114,6 +114,8 @@ class DeviceB:
         self.naux, self.nbf, self.nxb, self.nbb = dims[0], dims[1], dims[2], dims[3]
         self.nelem = ne.value
         self.transformed = False
+        if host_b is not None:
+            self.xsz, self.bsz = list(host_b.xsz), list(host_b.bsz)
         ctx._adopt(self)
 
     @classmethod
@@ -138,6 +140,12 @@ class DeviceB:
         _lib.check(_lib.lib().mchip_dbt_mask(self.h, _lib.u8ptr(m)))
         return m
 
+    def nnz_kept(self):
+        """Elements inside kept blocks (the algorithmic nnz of B)."""
+        m = self.mask().astype(np.int64)
+        return int(np.einsum("x,xmn,m,n->", np.array(self.xsz), m, np.array(self.bsz),
+                             np.array(self.bsz)))
+
     def dense_rows(self, x0, x1):
         out = np.zeros((x1 - x0, self.nbf, self.nbf))
         _lib.check(_lib.lib().mchip_dbt_dense_rows(self.h, x0, x1, _lib.dptr(out)))
@@ -156,6 +164,32 @@ class DeviceB:
         _lib.check(_lib.lib().mchip_dfk_mo_f32(self.h, _lib.dptr(C), C.shape[1], occ_nbatches,
                                                _lib.dptr(K)))
         return K
+
+    def make_bf16(self):
+        """BF16 hi/lo split copy of the transformed B (chosen-form exchange)."""
+        _lib.check(_lib.lib().mchip_dbt_make_bf16(self.h))
+
+    def dfk_mo_bf16(self, C_occ, occ_nbatches=1):
+        C = np.ascontiguousarray(C_occ, dtype=np.float64)
+        K = np.zeros((self.nbf, self.nbf))
+        _lib.check(_lib.lib().mchip_dfk_mo_bf16(self.h, _lib.dptr(C), C.shape[1], occ_nbatches,
+                                                _lib.dptr(K)))
+        return K
+
+    def dfk_ao(self, P, metric=None, form="fp64", occ_nbatches=1):
+        """Exchange from a density (DFAO_K / DFMEM_K): pivoted Cholesky of P on
+        the GPU, then the occupied-factor exchange.  form: fp64, f32, bf16."""
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        K = np.zeros((self.nbf, self.nbf))
+        m = None if metric is None else metric.h
+        f = {"fp64": 0, "f32": 1, "bf16": 2}[form]
+        _lib.check(_lib.lib().mchip_dfk_ao(self.h, m, _lib.dptr(P), f, occ_nbatches, _lib.dptr(K)))
+        return K
+
+    def bf16_tile(self, ng):
+        bn = ctypes.c_int()
+        _lib.check(_lib.lib().mchip_dfk_bf16_tile(self.h, ng, ctypes.byref(bn)))
+        return bn.value
 
     def download(self):
         out = np.zeros(self.nelem)

@@ -172,6 +172,30 @@ int mchip_dfk_mo_f32_dev(mchip_dbt t, const double* C_dev, int ldc, int nocc, in
                          double* K_dev);
 int mchip_dfk_mo_f32(mchip_dbt t, const double* C, int nocc, int occ_nbatches, double* K);
 
+/* BF16 split-precision copy of the transformed B (hi = bf16(x), lo = bf16(x - hi),
+ * rows padded to a multiple of 32), made once per geometry.  Chosen form of the
+ * B-contractions of DFMO_K::compute_K (src/fock/jkdf.cpp:221-462). */
+int mchip_dbt_make_bf16(mchip_dbt t);
+/* K = -sum_X (B~_X C)^T (B~_X C) with both contractions as three BF16 MFMA
+ * products (lo*hi + hi*lo + hi*hi), FP32 accumulation within chunks of at most
+ * 256 aux functions, FP64 across chunks.  Replaces the per-iteration
+ * DFMO_K::compute_K (jkdf.cpp:221-462) in the chosen form. */
+int mchip_dfk_mo_bf16_dev(mchip_dbt t, const double* C_dev, int ldc, int nocc, int occ_nbatches,
+                          double* K_dev);
+int mchip_dfk_mo_bf16(mchip_dbt t, const double* C, int nocc, int occ_nbatches, double* K);
+/* Exchange from a density matrix P (nbf x nbf, symmetric PSD): DFAO_K::compute_K
+ * (src/fock/jkdf.cpp:510-648) and DFMEM_K::compute_K (jkdf.cpp:719-859),
+ * K = -sum B(X|m l) V^-1(X,Y) P(l,s) B(Y|s n), returned negated like the
+ * reference.  P is factorised on the GPU (pivoted Cholesky, P = L L^T) and the
+ * occupied-factor exchange runs with C := L.  form 0 = FP64 (metric needed unless
+ * B is transformed), 1 = FP32, 2 = BF16x3 (transformed B with its copy). */
+int mchip_dfk_ao_dev(mchip_dbt t, mchip_metric m, const double* P_dev, int form, int occ_nbatches,
+                     double* K_dev);
+int mchip_dfk_ao(mchip_dbt t, mchip_metric m, const double* P, int form, int occ_nbatches, double* K);
+/* Column tile picked by the BF16 slab-GEMM autotuner for ng occupied columns
+ * (0 before the first build with that ng).  No counterpart in the reference. */
+int mchip_dfk_bf16_tile(mchip_dbt t, int ng, int* bn);
+
 /* ------------------------------------------------------------------------
  * Restricted Hartree-Fock on the GPU (hf::hfmod::compute, src/hf/hfscf.cpp:
  * 326-552): core guess, F = H + J + K, e = FPS - SPF, E = tr(HP) + tr(FP),

@@ -52,3 +52,42 @@ def test_overlap_normalised_and_metric_spd():
     Bh = ints.BTensorHost.build(mol, 1e-10)
     Bd = Bh.dense()
     assert np.array_equal(Bd, Bd.transpose(0, 2, 1))
+
+
+def test_cpp_builder_header(tmp_path):
+    """include/mchip_fock.hpp compiles as C++17 against the C ABI, links with
+    libmchip.so, and reports a missing required argument as std::runtime_error
+    (no device call is made)."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = tmp_path / "b.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include <cstring>
+#include "mchip_fock.hpp"
+int main() {
+  using namespace mchip::fock;
+  try {
+    auto k = create_k().method(kmethod::dfmo).occ_nbatches(2).build();
+    (void)k;
+    return 2;
+  } catch (const std::runtime_error& e) {
+    if (std::strstr(e.what(), "eri3c2e_batched") == nullptr) return 3;
+  }
+  try {
+    auto j = create_j().method(jmethod::dfao).build();
+    (void)j;
+    return 4;
+  } catch (const std::runtime_error&) {}
+  std::printf("ok\n");
+  return 0;
+}
+''')
+    exe = tmp_path / "b"
+    lib = os.path.join(root, "chem_amd", "lib")
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(root, "include"),
+                        str(src), "-L", lib, "-lmchip", "-Wl,-rpath," + lib, "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", (r.returncode, r.stdout, r.stderr)

@@ -160,3 +160,99 @@ def test_fp32_exchange_accuracy(ctx):
     assert _relerr(K32b, K32) < 2e-6
     # deterministic: the same call gives the same bits
     assert np.array_equal(K32, dB.dfk_mo_f32(C))
+
+
+# BF16x3 carries every operand with 16 significant bits (hi + lo, 2^-17
+# relative rounding) and accumulates in FP32 over chunks of <= 256 aux
+# functions: K agrees with the FP64 exchange on the same B~ to a few 1e-6
+# relative; the bar is 5e-5 of max |K|.
+BF16_TOL = 5e-5
+
+
+def test_bf16_exchange_accuracy(ctx):
+    """Chosen-form BF16 split exchange against the FP64 one on the same B~."""
+    from chem_amd.engine import DeviceB, Metric
+    mol, Bh, V = _system(3)
+    C, P = _random_occ(mol.nbf, mol.nocc, 9)
+    m = Metric(ctx, V)
+    dB = DeviceB(ctx, Bh)
+    dB.transform(m)
+    dB.make_bf16()
+    K64 = dB.dfk_mo(C)
+    K16 = dB.dfk_mo_bf16(C)
+    err = _relerr(K16, K64)
+    print("BF16x3 K relerr %.3e (column tile %d)" % (err, dB.bf16_tile(mol.nocc)))
+    assert err < BF16_TOL
+    assert np.array_equal(K16, K16.T)
+    assert dB.bf16_tile(mol.nocc) in (64, 128, 160)
+    # occupied batches (ragged: 15 = 8 + 7) and determinism
+    K16b = dB.dfk_mo_bf16(C, occ_nbatches=2)
+    assert _relerr(K16b, K64) < BF16_TOL
+    assert np.array_equal(K16, dB.dfk_mo_bf16(C))
+
+
+@pytest.mark.parametrize("nocc", [1, 17, 70])
+def test_bf16_synthetic_sparse_B(ctx, nocc):
+    """BF16 exchange on a random block-sparse B with an empty slab and ragged
+    block sizes (row padding and zero tiles) against the FP64 exchange."""
+    from chem_amd.engine import DeviceB, Metric
+    rng = np.random.default_rng(300 + nocc)
+    bsz = [14, 5, 5, 3, 14, 1, 7]
+    xsz = [23, 70, 9, 40]
+    nbf, naux = sum(bsz), sum(xsz)
+    mask = (rng.random((len(xsz), len(bsz), len(bsz))) < 0.6).astype(np.uint8)
+    mask = np.maximum(mask, mask.transpose(0, 2, 1))
+    mask[:, 5, :] = 0
+    mask[:, :, 5] = 0
+    boff = np.concatenate([[0], np.cumsum(bsz)])
+    xoff = np.concatenate([[0], np.cumsum(xsz)])
+    dense = rng.standard_normal((naux, nbf, nbf))
+    dense = 0.5 * (dense + dense.transpose(0, 2, 1))
+    for x in range(len(xsz)):
+        for a in range(len(bsz)):
+            for b in range(len(bsz)):
+                if not mask[x, a, b]:
+                    dense[xoff[x]:xoff[x + 1], boff[a]:boff[a + 1], boff[b]:boff[b + 1]] = 0
+    Bh = ints.BTensorHost.from_dense(dense, xsz, bsz, mask)
+    A = rng.standard_normal((naux, naux))
+    V = A @ A.T + naux * np.eye(naux)
+    C = rng.standard_normal((nbf, nocc))
+    dB = DeviceB(ctx, Bh)
+    m = Metric(ctx, V)
+    dB.transform(m)
+    dB.make_bf16()
+    K64 = dB.dfk_mo(C)
+    K16 = dB.dfk_mo_bf16(C)
+    err = _relerr(K16, K64)
+    print("nocc %d BF16x3 K relerr %.3e" % (nocc, err))
+    assert err < BF16_TOL
+    assert np.array_equal(K16, K16.T)
+
+
+@pytest.mark.parametrize("nwater", [1, 3])
+def test_dfao_from_density_vs_oracle(ctx, nwater):
+    """DFAO_K / DFMEM_K from a density: the GPU factorises P (pivoted Cholesky)
+    and runs the occupied-factor exchange; compared with the oracle's fitted-
+    coefficient DFAO and batched DFMEM contractions on the same metric factors.
+    The two routes differ by how V^-1 enters (L^-1 on both sides vs V^-1 on
+    one), so the bar is the metric-conditioning bound 10 cond(V) eps."""
+    from chem_amd.engine import DeviceB, Metric
+    mol, Bh, V = _system(nwater)
+    Bd = Bh.dense()
+    dB = DeviceB(ctx, Bh)
+    m = Metric(ctx, V)
+    Linv, Vinv = m.get()
+    C, P = _random_occ(mol.nbf, mol.nocc, 21 + nwater)
+    K = dB.dfk_ao(P, m)
+    Kao = dfock.dfao_k(Bd, dfock.fitting_coefficients(Bd, Vinv), P, bsz=mol.b, xsz=mol.x)
+    Kmem = dfock.dfmem_k(Bd, Vinv, P, bsz=mol.b, xsz=mol.x)
+    Kmo = dB.dfk_mo(C, m)
+    e_ao, e_mem, e_mo = _relerr(K, Kao), _relerr(K, Kmem), _relerr(K, Kmo)
+    print("DFAO from P: relerr vs oracle DFAO %.2e DFMEM %.2e, vs GPU DFMO %.2e" % (e_ao, e_mem, e_mo))
+    assert e_ao < _jtol(V) and e_mem < _jtol(V)
+    assert e_mo < 1e-12
+    # the BF16 form through the same route, on the transformed tensor
+    dB.transform(m)
+    dB.make_bf16()
+    K16 = dB.dfk_ao(P, form="bf16")
+    assert _relerr(K16, Kmo) < BF16_TOL

@@ -71,3 +71,25 @@ def test_mixed_scf_energy(ctx):
                                                            mix.energy - ref.energy))
     assert ref.converged and mix.converged
     assert abs(mix.energy - ref.energy) < 1e-8
+
+
+def test_mixed_bf16_scf_energy(ctx):
+    """Chosen form with the BF16 split exchange per iteration and the FP64
+    exchange for the final energy, against the FP64 SCF on the same B~.
+    The reported energy is variational in the density, so its error is second
+    order in the BF16 density error: bar 1e-7 Eh (the target is 1e-6)."""
+    from chem_amd.engine import DeviceB, Metric
+    from chem_amd.scf import run_rhf
+    mol = desc.Molecule(desc.water_cluster(4, 11))
+    S, T, V = ints.int1e(mol)
+    Bh = ints.BTensorHost.build(mol, 1e-10)
+    m = Metric(ctx, ints.int2c2e(mol))
+    dB = DeviceB(ctx, Bh)
+    dB.transform(m)
+    ref = run_rhf(mol, dB, None, S, T + V, threshold=1e-7, kmode="dfmo_transformed")
+    dB.make_bf16()
+    mix = run_rhf(mol, dB, None, S, T + V, threshold=1e-7, kmode="mixed_bf16")
+    print("E64 %.10f Ebf16 %.10f (iterate %.10f) dE %.2e" % (ref.energy, mix.energy, mix.energy_iter,
+                                                            mix.energy - ref.energy))
+    assert ref.converged and mix.converged
+    assert abs(mix.energy - ref.energy) < 1e-7
