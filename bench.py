@@ -4,10 +4,13 @@ Workload (default --config c2): seeded 64-water cluster, cc-pVDZ / cc-pVDZ-jkfit
 atomic blocks, Schwarz screening at 1e-10.
 
 Plain path: the reference's DFMO algorithm in FP64 (K1 = B.C, K2 = L^-1 per
-build, K3 = SYRK).  Chosen form: B~ = L^-1 B made once and stored as a BF16
-hi/lo split (--form bf16, default) or in FP32 (--form f32); every SCF
-iteration builds K on MFMA with FP32 accumulation and FP64 reduction, and one
-FP64 K at the converged orbitals gives the reported energy.
+build, K3 = SYRK).  Chosen form: low-precision copies of B~ = L^-1 B made
+once from B (which stays FP64): a BF16 hi/lo split and an FP32 copy (--form
+auto, default: BF16x3 builds until the SCF error falls below max(1e-4,
+10*thresh), FP32 builds after), or one of them alone (--form bf16 / f32).
+Every SCF iteration builds K on MFMA with FP32 accumulation and FP64
+reduction; J and one plain FP64 K at the converged orbitals give the reported
+energy with the reference's formulas.
 
 A step is one SCF iteration's DF-K work in the chosen form.  The timed region
 runs `steps` FP32 builds plus the one final FP64 build at the converged
@@ -34,6 +37,8 @@ CONFIGS = {
     "c0": dict(nwater=1, seed=0),
     "c1": dict(nwater=32, seed=1234),
     "c2": dict(nwater=64, seed=1234),
+    # serving stream: Poisson arrivals at 4/s, clusters of U{4..64} waters
+    "c4": dict(requests=200, rate=4.0, nmin=4, nmax=64, seed=2024, budget=200e9, thresh=1e-6),
 }
 MFMA_F64_PEAK = 78.6e12
 MFMA_F32_PEAK = 157.3e12
@@ -87,6 +92,61 @@ def cpu_baseline(mol, dB, Linv, C_occ, budget_s=20.0):
                            rows, naux, dt1, s, ncol, dt2))
 
 
+def serve_stream(args, log_progress=True):
+    """The DF-HF request stream: seeded Poisson arrivals, water clusters of
+    uniformly random size, the chosen form (BF16x3 escalating to FP32, FP64
+    final energy) per request, admission against the HBM budget.  Returns the
+    latency percentiles, throughput and the energy check against the plain
+    FP64 path on a sample of the requests."""
+    from chem_amd import desc
+    from chem_amd.serve import Server
+    cfg = CONFIGS["c4"]
+    nreq = args.requests or cfg["requests"]
+    rng = np.random.default_rng(cfg["seed"])
+    arrivals = np.cumsum(rng.exponential(1.0 / cfg["rate"], nreq))
+    sizes = rng.integers(cfg["nmin"], cfg["nmax"] + 1, nreq)
+    mols = {}
+    results = []
+    with Server(budget_bytes=cfg["budget"], nslots=args.slots) as srv:
+        t0 = time.perf_counter()
+        ids = []
+        for i in range(nreq):
+            wait = t0 + arrivals[i] - time.perf_counter()
+            if wait > 0:
+                time.sleep(wait)
+            mol = desc.Molecule(desc.water_cluster(int(sizes[i]), 5000 + i))
+            mols[i] = mol
+            ids.append(srv.submit(mol, threshold=cfg["thresh"], kmode="mixed_auto"))
+        for i, rid in enumerate(ids):
+            r = srv.result(rid)
+            r["nwater"] = int(sizes[i])
+            results.append(r)
+            if log_progress:
+                log("request %d (%d waters): E=%.8f iters=%d latency %.2f s (admitted after %.2f s, slot %d)" % (
+                    rid, sizes[i], r["energy"], r["iterations"], r["latency"], r["t_admit"] - r["t_submit"],
+                    r["slot"]))
+        st = srv.stats()
+    lat = np.array([r["latency"] for r in results])
+    span = max(r["t_done"] for r in results) - min(r["t_submit"] for r in results)
+    iters = sum(r["iterations"] + 1 for r in results)
+    # energy check: plain FP64 path (kmode dfmo, the reference formulas) on a sample
+    sample = sorted(set(int(i) for i in np.linspace(0, nreq - 1, min(args.serve_check, nreq)).round()))
+    dEs = []
+    with Server(budget_bytes=cfg["budget"], nslots=1) as ref:
+        for i in sample:
+            r64 = ref.result(ref.submit(mols[i], threshold=cfg["thresh"], kmode="dfmo"))
+            dEs.append(abs(results[i]["energy"] - r64["energy"]))
+            if log_progress:
+                log("check request %d (%d waters): |dE| vs plain FP64 %.2e" % (i, sizes[i], dEs[-1]))
+    return dict(requests=nreq, rate_per_s=cfg["rate"], sizes="U{%d..%d} waters" % (cfg["nmin"], cfg["nmax"]),
+                budget_gb=cfg["budget"] / 1e9, slots=args.slots, p50_s=float(np.percentile(lat, 50)),
+                p99_s=float(np.percentile(lat, 99)), mean_s=float(lat.mean()), max_s=float(lat.max()),
+                scf_iterations_per_s=iters / span, requests_per_s=nreq / span, span_s=span,
+                max_live=st["max_live"], peak_hbm_gb=st["peak_bytes"] / 1e9, failed=st["failed"],
+                all_converged=all(r["converged"] for r in results),
+                max_abs_dE_vs_fp64=float(max(dEs)) if dEs else None, dE_sample=len(dEs))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,8 +155,13 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--scf-thresh", type=float, default=1e-6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--form", default="bf16", choices=["bf16", "f32"],
-                    help="precision of the chosen-form exchange builds")
+    ap.add_argument("--requests", type=int, default=0, help="c4: number of requests (0: 200)")
+    ap.add_argument("--slots", type=int, default=4, help="c4: request workers")
+    ap.add_argument("--serve-check", type=int, default=4,
+                    help="c4: requests re-run on the plain FP64 path for the energy check")
+    ap.add_argument("--form", default="auto", choices=["auto", "bf16", "f32"],
+                    help="precision of the chosen-form exchange builds (auto: BF16x3 until the "
+                         "SCF error falls below max(1e-4, 10*thresh), then FP32)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the DF-K path runs on a single "
@@ -106,6 +171,22 @@ def main():
     from chem_amd import desc, ints
     from chem_amd.engine import Context, DeviceB, Metric
     from chem_amd.scf import run_rhf
+
+    if args.config == "c4":
+        sv = serve_stream(args)
+        out = {
+            "metric": "DF-HF serving: p99 time-to-converged-energy",
+            "value": sv["p99_s"], "unit": "s", "n_gpus": 1, "steps": sv["requests"], "warmup": 0,
+            "ms_per_step": sv["mean_s"] * 1e3, "higher_is_better": False, "scaling": "weak",
+            "vs_baseline": None, "dtype": "bf16x3 split then f32 (K builds) + f64 (J, final K, SCF)",
+            "data": "synthetic seeded water clusters, real integrals",
+            "config": {"workload": "request stream (c4): Poisson %.0f/s, %s, cc-pVDZ/cc-pVDZ-jkfit, "
+                                   "conv %.0e, HBM budget %.0f GB" % (
+                                       sv["rate_per_s"], sv["sizes"], CONFIGS["c4"]["thresh"], sv["budget_gb"])},
+            "serving": sv,
+        }
+        print(json.dumps(out), flush=True)
+        return
 
     cfg = CONFIGS[args.config]
     t0 = time.perf_counter()
@@ -150,64 +231,99 @@ def main():
         cpu = cpu_baseline(mol, dB, Linv, C_occ)
         log("cpu baseline: %.2f s/iter (%s)" % (cpu["value"], cpu["sample"]))
 
-    # chosen form: B~ = L^-1 B once (FP64), low-precision copy, mixed SCF
-    dB.transform(met)
+    # chosen form: low-precision copies of B~ = L^-1 B made once from B, which
+    # stays FP64 for J, the final exchange and the energy (reference formulas)
     form = args.form
-    if form == "bf16":
-        dB.make_bf16()
-        kfun, kmode, bpeak = dB.dfk_mo_bf16, "mixed_bf16", MFMA_BF16_PEAK
-    else:
-        dB.make_f32()
-        kfun, kmode, bpeak = dB.dfk_mo_f32, "mixed_f32", MFMA_F32_PEAK
+    tc = time.perf_counter()
+    dB.make_lowp(met, f32=form in ("auto", "f32"), bf16=form in ("auto", "bf16"))
+    ctx.sync()
+    t_copies = time.perf_counter() - tc
+    kmode = {"auto": "mixed_auto", "bf16": "mixed_bf16", "f32": "mixed_f32"}[form]
     t2 = time.perf_counter()
-    chosen = run_rhf(mol, dB, None, S, T + V, threshold=args.scf_thresh, kmode=kmode)
+    chosen = run_rhf(mol, dB, met, S, T + V, threshold=args.scf_thresh, kmode=kmode)
     dE = chosen.energy - ref.energy
-    log("chosen SCF (%s): E=%.10f (low-precision-K iterate %.10f) iters=%d dE=%.3e (%.1f s)" % (
-        form, chosen.energy, chosen.energy_iter, chosen.iterations, dE, time.perf_counter() - t2))
-    steps = args.steps or max(1, chosen.iterations)
+    nbuild = len(chosen.t_k)
+    if form == "auto":
+        # the library's escalation rule: BF16 while the previous RMS exceeds tau
+        tau = max(1e-4, 10.0 * args.scf_thresh)
+        n16 = 0
+        for i in range(nbuild):
+            prev = chosen.history_rms[i - 1] if i > 0 else float("inf")
+            if prev <= tau:
+                break
+            n16 += 1
+    else:
+        n16 = nbuild if form == "bf16" else 0
+    log("chosen SCF (%s): E=%.10f (low-precision-K iterate %.10f) iters=%d (%d BF16 + %d FP32 builds) "
+        "dE=%.3e (%.1f s)" % (form, chosen.energy, chosen.energy_iter, chosen.iterations, n16, nbuild - n16,
+                             dE, time.perf_counter() - t2))
+    steps = args.steps or max(1, nbuild)
+    n16 = min(n16, steps)
     Cc = np.ascontiguousarray(chosen.C[:, :mol.nocc])
+    k16 = dB.dfk_mo_bf16 if form in ("auto", "bf16") else None
+    k32 = dB.dfk_mo_f32 if form in ("auto", "f32") else None
     for _ in range(args.warmup):
-        kfun(Cc)
-        dB.dfk_mo(Cc)
+        if k16:
+            k16(Cc)
+        if k32:
+            k32(Cc)
     ctx.sync()
     t = time.perf_counter()
-    for _ in range(steps):
-        kfun(Cc)
-    dB.dfk_mo(Cc)
+    for s_ in range(steps):
+        (k16 if s_ < n16 else k32)(Cc)
+    dB.dfk_mo(Cc, met)      # the final FP64 exchange (plain formula) for the reported energy
     ctx.sync()
     step_s = (time.perf_counter() - t) / steps
-    t64 = time_builds(lambda: dB.dfk_mo(Cc), 2, 0)
-    tlow = time_builds(lambda: kfun(Cc), 3, 0)
+    t16 = time_builds(lambda: k16(Cc), 3, 0) if k16 else None
+    t32 = time_builds(lambda: k32(Cc), 3, 0) if k32 else None
 
-    # roofline of the dominant kernel (K1 = B~ C slab GEMM) from HIP events on the library stream
-    ctx.set_timing(True)
-    nrep = 3
-    for _ in range(nrep):
-        kfun(Cc)
-    ph = ctx.phase_times() / nrep
-    ctx.set_timing(False)
+    # roofline of the dominant kernel from HIP events on the library stream:
+    # K1 (the B~ C slab GEMM) of the form that runs most of the builds
+    def phases(fn, nrep=3):
+        ctx.set_timing(True)
+        for _ in range(nrep):
+            fn(Cc)
+        ph = ctx.phase_times() / nrep
+        ctx.set_timing(False)
+        return ph
     k1_flops = 2.0 * nnz * mol.nocc
     k3_flops = float(mol.naux) * mol.nocc * mol.nbf ** 2   # symmetric half of 2*naux*nocc*nbf^2
-    achieved = k1_flops / ph[0]
-    roof = dict(bound="mfma", achieved=achieved / 1e12, peak=bpeak / 1e12, unit="TFLOP/s",
-                frac=achieved / bpeak, traffic=None,
-                kernel="k_slab_gemm_bf16x3" if form == "bf16" else "k_slab_gemm<float>",
-                kernel_s=ph[0], k3_s=ph[2], k3_achieved=k3_flops / ph[2] / 1e12,
-                split_s=ph[5])
-    if form == "bf16":
+    kernels = {}
+    if k16:
+        ph = phases(k16)
+        kernels["k_slab_gemm_bf16x3"] = dict(s=ph[0], tflops=k1_flops / ph[0] / 1e12,
+                                             builds=n16, peak=MFMA_BF16_PEAK / 1e12)
+        kernels["k_seg_syrk_bf16x3"] = dict(s=ph[2], tflops=k3_flops / ph[2] / 1e12,
+                                            builds=n16, peak=MFMA_BF16_PEAK / 1e12)
+        kernels["launch_shape"] = dB.bf16_tile(mol.nocc)
+    if k32:
+        ph = phases(k32)
+        kernels["k_slab_gemm<float>"] = dict(s=ph[0], tflops=k1_flops / ph[0] / 1e12,
+                                             builds=steps - n16, peak=MFMA_F32_PEAK / 1e12)
+        kernels["k_seg_gemm<float>"] = dict(s=ph[2], tflops=k3_flops / ph[2] / 1e12,
+                                            builds=steps - n16, peak=MFMA_F32_PEAK / 1e12)
+    dom = max((k for k in kernels if isinstance(kernels[k], dict)),
+              key=lambda k: kernels[k]["s"] * kernels[k]["builds"])
+    d = kernels[dom]
+    roof = dict(bound="mfma", achieved=d["tflops"], peak=d["peak"], unit="TFLOP/s",
+                frac=d["tflops"] / d["peak"], traffic=None, kernel=dom, kernel_s=d["s"],
+                kernels=kernels)
+    if "bf16x3" in dom:
         # three BF16 products per useful multiply-add: issued MFMA work is 3x the algorithmic flops
-        roof["mfma_issued_frac"] = 3.0 * achieved / bpeak
-        roof["k3_mfma_issued_frac"] = 3.0 * k3_flops / ph[2] / bpeak
-        roof["column_tile"] = dB.bf16_tile(mol.nocc)
+        roof["mfma_issued_frac"] = 3.0 * roof["frac"]
 
-    # accuracy of the chosen-form K at the FP64 orbitals, and of the orbital energies
-    K_low = kfun(C_occ)
-    dK = K_low - K_plain
-    kerr = dict(max_abs=float(np.abs(dK).max()),
-                frob_rel=float(np.linalg.norm(dK) / np.linalg.norm(K_plain)))
+    # accuracy of each low-precision K at the FP64 orbitals, and of the orbital energies
+    kerr = {}
+    for name, fn in (("bf16x3", k16), ("f32", k32)):
+        if fn is None:
+            continue
+        dK = fn(C_occ) - K_plain
+        kerr[name] = dict(max_abs=float(np.abs(dK).max()),
+                          frob_rel=float(np.linalg.norm(dK) / np.linalg.norm(K_plain)))
+        log("%s K error vs FP64: max %.3e, Frobenius rel %.3e" % (
+            name, kerr[name]["max_abs"], kerr[name]["frob_rel"]))
     deps = float(np.abs(chosen.eps - ref.eps).max())
-    log("K error vs FP64: max %.3e, Frobenius rel %.3e; max |d eps| %.3e" % (
-        kerr["max_abs"], kerr["frob_rel"], deps))
+    log("max |d eps| (orbital energies of the last low-precision Fock matrix) %.3e" % deps)
 
     out = {
         "metric": "DF-K s/iter, %d H2O cc-pVDZ, chosen form vs FP64" % cfg["nwater"],
@@ -220,8 +336,8 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("bf16x3 split, f32 accumulation" if form == "bf16" else "f32") +
-                 " (K builds) + f64 (final K, J, SCF)",
+        "dtype": {"auto": "bf16x3 split then f32", "bf16": "bf16x3 split",
+                  "f32": "f32"}[form] + " (K builds, f32 accumulation) + f64 (final K, J, SCF)",
         "data": "synthetic seeded water cluster, real integrals",
         "config": {"workload": "DF-K build, %d-water cluster cc-pVDZ/cc-pVDZ-jkfit (%s)" % (
             cfg["nwater"], args.config), "nbf": mol.nbf, "naux": mol.naux, "nocc": mol.nocc,
@@ -229,8 +345,11 @@ def main():
         "plain_fp64_s_per_iter": plain_s,
         "speedup_vs_plain": plain_s / step_s,
         "chosen_form": form,
-        "chosen_build_s": tlow,
-        "transformed_fp64_build_s": t64,
+        "builds": {"bf16x3": n16, "f32": steps - n16, "fp64_final": 1},
+        "bf16x3_build_s": t16,
+        "f32_build_s": t32,
+        "final_fp64_build_s": plain_s,
+        "lowp_copies_s": t_copies,
         "dE_scf": dE,
         "dK_vs_fp64": kerr,
         "max_deps_orbital": deps,

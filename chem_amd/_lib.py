@@ -129,11 +129,18 @@ _SIGS = [
     ("mchip_dfk_mo_f32_dev", c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     ("mchip_dfk_mo_f32", c_int, [c_void_p, P_double, c_int, c_int, P_double]),
     ("mchip_dbt_make_bf16", c_int, [c_void_p]),
+    ("mchip_dbt_make_lowp", c_int, [c_void_p, c_void_p, c_int, c_int]),
     ("mchip_dfk_mo_bf16_dev", c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     ("mchip_dfk_mo_bf16", c_int, [c_void_p, P_double, c_int, c_int, P_double]),
     ("mchip_dfk_bf16_tile", c_int, [c_void_p, c_int, P_int]),
     ("mchip_dfk_ao_dev", c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
     ("mchip_dfk_ao", c_int, [c_void_p, c_void_p, P_double, c_int, c_int, P_double]),
+    ("mchip_server_create", c_int, [c_int, c_double, c_int, c_int, ctypes.POINTER(c_void_p)]),
+    ("mchip_server_destroy", c_int, [c_void_p]),
+    ("mchip_server_submit", c_int, [c_void_p, c_void_p, c_void_p, c_int, P_double, P_double, c_int, P_int,
+                                    c_int, P_int, c_int, c_double, c_double, c_int, c_double, P_int]),
+    ("mchip_server_result", c_int, [c_void_p, c_int, c_int, P_int, P_double, P_int, P_int, P_double]),
+    ("mchip_server_stats", c_int, [c_void_p, P_double]),
     ("mchip_scf_last_extra", c_int, [P_double, P_double]),
     ("mchip_ctx_phase_times", c_int, [c_void_p, P_double, c_int]),
 ]

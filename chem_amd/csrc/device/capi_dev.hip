@@ -343,6 +343,15 @@ extern "C" int mchip_dbt_make_bf16(mchip_dbt t)
   MCHIP_CATCH
 }
 
+extern "C" int mchip_dbt_make_lowp(mchip_dbt t, mchip_metric m, int f32, int bf16)
+{
+  MCHIP_TRY
+  need_device(t->ctx);
+  if (!m) throw std::runtime_error("mchip_dbt_make_lowp: the metric is required");
+  dbt_make_lowp(t->ctx->c, t->t, m->m, f32 != 0, bf16 != 0);
+  MCHIP_CATCH
+}
+
 extern "C" int mchip_dfk_mo_bf16_dev(mchip_dbt t, const double* C_dev, int ldc, int nocc,
                                      int occ_nbatches, double* K_dev)
 {
@@ -429,33 +438,9 @@ extern "C" int mchip_scf_run(mchip_dbt t, mchip_metric m, int nbf, int nocc, int
   in.diis_max = diis_max;
   in.filter_eps = filter_eps;
   const Metric* met = m ? &m->m : nullptr;
-  if (kmode == 0 && !met) throw std::runtime_error("mchip_scf_run: plain K needs the metric");
-  if (kmode == 1 && !t->t.transformed) throw std::runtime_error("mchip_scf_run: B is not transformed");
   const Metric* jmet = t->t.transformed ? nullptr : met;
   KBuildFn kb, kfinal;
-  auto k64 = [&](int, const double*, const double* Cocc, double* K) {
-    dfk_mo(c, t->t, t->t.transformed ? nullptr : met, Cocc, nocc, nocc, K, occ_nbatches);
-  };
-  if (kmode == 0 || kmode == 1) {
-    kb = k64;
-  }
-  else if (kmode == 2) {
-    if (!t->t.transformed || !t->t.data32.p) throw std::runtime_error("mchip_scf_run: mixed mode needs the transformed B and its FP32 copy");
-    kb = [&](int, const double*, const double* Cocc, double* K) {
-      dfk_mo_f32(c, t->t, Cocc, nocc, nocc, K, occ_nbatches);
-    };
-    kfinal = k64;
-  }
-  else if (kmode == 3) {
-    if (!t->t.transformed || !t->t.data_hi.p) throw std::runtime_error("mchip_scf_run: BF16 mode needs the transformed B and its BF16 split copy");
-    kb = [&](int, const double*, const double* Cocc, double* K) {
-      dfk_mo_bf16(c, t->t, Cocc, nocc, nocc, K, occ_nbatches);
-    };
-    kfinal = k64;
-  }
-  else {
-    throw std::runtime_error("mchip_scf_run: unknown K mode");
-  }
+  select_k_builders(c, t->t, met, kmode, nocc, occ_nbatches, threshold, kb, kfinal);
   ScfOutput out;
   scf_run(c, t->t, jmet, in, out, kb, kfinal);
   last_scf_extra[0] = out.energy_iter;
@@ -503,5 +488,96 @@ extern "C" int mchip_mp2_sos(mchip_dbt t, mchip_metric m, int nbf, int nocc, con
   if (terms)
     for (int k = 0; k < nlap && k < (int)tv.size(); ++k) terms[k] = tv[k];
   if (t_z) *t_z = tz;
+  MCHIP_CATCH
+}
+
+// ---------------------------------------------------------------------------
+// Request loop
+// ---------------------------------------------------------------------------
+#include "serve.hpp"
+
+struct mchip_server_s {
+  std::unique_ptr<Server> s;
+};
+
+extern "C" int mchip_server_create(int device, double budget_bytes, int nslots, int cpu_threads,
+                                   mchip_server* out)
+{
+  MCHIP_TRY
+  if (budget_bytes <= 0) throw std::runtime_error("mchip_server_create: budget must be positive");
+  auto s = std::make_unique<mchip_server_s>();
+  s->s = std::make_unique<Server>(device, (size_t)budget_bytes, nslots, std::max(1, cpu_threads));
+  *out = s.release();
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_server_destroy(mchip_server s)
+{
+  MCHIP_TRY
+  delete s;
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_server_submit(mchip_server s, mchip_basis ao, mchip_basis aux, int natom, const double* Z,
+                                   const double* xyz, int nbb, const int* ao_blk_shells, int nxb,
+                                   const int* aux_blk_shells, int nocc, double enuc, double threshold,
+                                   int kmode, double screen_eps, int* id)
+{
+  MCHIP_TRY
+  if (kmode < 0 || kmode > 4) throw std::runtime_error("mchip_server_submit: unknown K mode");
+  auto j = std::make_unique<SrvJob>();
+  j->ao = basis_of(ao);
+  j->aux = basis_of(aux);
+  j->charges.resize(natom);
+  for (int i = 0; i < natom; ++i) {
+    j->charges[i].Z = Z[i];
+    for (int d = 0; d < 3; ++d) j->charges[i].xyz[d] = xyz[3 * i + d];
+  }
+  j->ao_blk.assign(ao_blk_shells, ao_blk_shells + nbb);
+  j->aux_blk.assign(aux_blk_shells, aux_blk_shells + nxb);
+  int sb = 0, sx = 0;
+  for (int v : j->ao_blk) sb += v;
+  for (int v : j->aux_blk) sx += v;
+  if (sb != (int)j->ao.shells.size() || sx != (int)j->aux.shells.size())
+    throw std::runtime_error("mchip_server_submit: block shell counts do not cover the basis");
+  if (nocc < 0 || nocc > j->ao.nbf) throw std::runtime_error("mchip_server_submit: bad nocc");
+  j->nocc = nocc;
+  j->enuc = enuc;
+  j->threshold = threshold;
+  j->kmode = kmode;
+  j->eps = screen_eps;
+  *id = s->s->submit(std::move(j));
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_server_result(mchip_server s, int id, int wait, int* done, double* energy, int* iterations,
+                                   int* converged, double* times)
+{
+  MCHIP_TRY
+  SrvJob j;
+  const bool d = s->s->result(id, wait != 0, j);
+  *done = d ? 1 : 0;
+  if (d) {
+    if (j.failed) throw std::runtime_error("request " + std::to_string(id) + " failed: " + j.error);
+    *energy = j.energy;
+    *iterations = j.iterations;
+    *converged = j.converged ? 1 : 0;
+    if (times) {
+      times[0] = j.t_submit;
+      times[1] = j.t_admit;
+      times[2] = j.t_done;
+      times[3] = (double)j.est_bytes;
+      times[4] = (double)j.slot;
+      times[5] = j.energy_iter;
+    }
+  }
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_server_stats(mchip_server s, double* stats)
+{
+  MCHIP_TRY
+  s->s->stats(stats);
+  stats[5] = s->s->clock();
   MCHIP_CATCH
 }

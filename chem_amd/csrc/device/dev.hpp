@@ -31,29 +31,56 @@ namespace mchip {
                                ":" + std::to_string(__LINE__));                   \
   } while (0)
 
+// Allocation stream of the calling thread.  When set (the request workers of
+// the server set it to their context's stream), DevBuf allocates and frees
+// stream-ordered on it, so freeing a buffer does not wait for the work other
+// threads have queued on the device.
+inline hipStream_t& alloc_stream()
+{
+  static thread_local hipStream_t s = nullptr;
+  return s;
+}
+
 // Owning device buffer.
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  hipStream_t astream = nullptr;   // stream of a stream-ordered allocation
   DevBuf() = default;
   explicit DevBuf(size_t n) { alloc(n); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), astream(o.astream)
+  {
+    o.p = nullptr;
+    o.bytes = 0;
+    o.astream = nullptr;
+  }
   DevBuf& operator=(DevBuf&& o) noexcept
   {
     release();
     p = o.p;
     bytes = o.bytes;
+    astream = o.astream;
     o.p = nullptr;
     o.bytes = 0;
+    o.astream = nullptr;
     return *this;
   }
   ~DevBuf() { release(); }
   void alloc(size_t n)
   {
     release();
-    if (n) HIP_CHECK(hipMalloc(&p, n));
+    if (n) {
+      hipStream_t s = alloc_stream();
+      if (s) {
+        HIP_CHECK(hipMallocAsync(&p, n, s));
+        astream = s;
+      }
+      else {
+        HIP_CHECK(hipMalloc(&p, n));
+      }
+    }
     bytes = n;
   }
   // grow-only
@@ -63,9 +90,13 @@ struct DevBuf {
   }
   void release()
   {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (astream) (void)hipFreeAsync(p, astream);
+      else (void)hipFree(p);
+    }
     p = nullptr;
     bytes = 0;
+    astream = nullptr;
   }
   template <class T> T* as() const { return reinterpret_cast<T*>(p); }
 };
@@ -85,6 +116,7 @@ struct Ctx {
   DevBuf scratch_a, scratch_b, scratch_c;   // reusable workspaces
   DevBuf scratch_partial;                   // split-K partial sums
   DevBuf info;                              // rocsolver info
+  std::map<int, DevBuf> syrk_tiles;         // n -> upper-tile table of the FP32 SYRK
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // optional per-phase kernel timing (HIP events on this stream), seconds
   bool timing = false;
@@ -132,7 +164,7 @@ struct DevBT {
   DevBuf ldp, off16, ldp_pre;               // padded row length, slab offsets, prefix of ldp
   std::vector<int> h_ldp;
   std::vector<int64_t> h_off16, h_ldp_pre;
-  std::map<int, int> bf16_bn;               // ng -> tuned column tile of the BF16 slab GEMM
+  std::map<int, int> bf16_bn;               // ng -> tuned launch shape of the BF16 slab GEMM
   DevBuf slab_off, ncol, bsz, boff, colmap, colmap_off, pg_off;
   std::vector<int> h_colmap, h_colmap_off;
   std::vector<int64_t> h_pg_off;            // prefix of bsz[N]*ncol[N]
@@ -140,6 +172,7 @@ struct DevBT {
   int64_t pg_total = 0;                     // sum of bsz[N]*ncol[N]
   int64_t maxR = 0;                         // max bsz[N]*ncol[N]
   bool transformed = false;  // data64 holds L^-1 B
+  bool lowp_tilde = false;   // data32 / data_hi,lo hold L^-1 B (from either route)
   std::map<std::pair<int, int>, DevBuf> tiles;   // (BN, ng) -> tile table
   std::map<std::pair<int, int>, int> ntiles;
 };

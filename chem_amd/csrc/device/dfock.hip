@@ -49,6 +49,8 @@ void ctx_free(Ctx& c)
   c.scratch_a.release();
   c.scratch_b.release();
   c.scratch_c.release();
+  c.scratch_partial.release();
+  c.syrk_tiles.clear();
   c.info.release();
   if (c.blas) rocblas_destroy_handle(c.blas);
   if (c.stream) (void)hipStreamDestroy(c.stream);
@@ -370,12 +372,46 @@ void dbt_make_f32(Ctx& c, DevBT& t)
                        t.data64.as<double>(), t.data32.as<float>(), (int64_t)t.L.total);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(c.stream));
+  t.lowp_tilde = true;
+}
+
+// FP32 SYRK over aux segments with FP64-flushed accumulators (upper 128 x 128
+// tiles, table cached with the context per n), then the FP64 chunk reduction.
+static void seg_syrk_f32(Ctx& c, const float* A, int nseg, int64_t sstride, int lda, int klen, int n,
+                         double alpha, double* out, DevBuf& partial_buf, int nbatch, int batch, bool reduce)
+{
+  const int nt = (n + SG_BM - 1) / SG_BM;
+  const int ntile = nt * (nt + 1) / 2;
+  auto it = c.syrk_tiles.find(n);
+  if (it == c.syrk_tiles.end()) {
+    std::vector<int2> tl;
+    for (int i = 0; i < nt; ++i)
+      for (int j = i; j < nt; ++j) tl.push_back(int2{i, j});
+    c.syrk_tiles[n] = upload_vec(tl, c.stream);
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    it = c.syrk_tiles.find(n);
+  }
+  int nch = std::max(1, std::min(nseg, 2048 / std::max(1, ntile)));
+  const int per = (nseg + nch - 1) / nch;
+  nch = (nseg + per - 1) / per;
+  const size_t need = (size_t)nbatch * nch * n * n * sizeof(double);
+  if (batch == 0) partial_buf.ensure(need);
+  double* partial = partial_buf.as<double>() + (size_t)batch * nch * n * n;
+  hipLaunchKernelGGL(k_seg_syrk_f32, dim3(ntile * nch), dim3(512), 0, c.stream, A, nseg, sstride, lda, klen, n,
+                     it->second.as<int2>(), ntile, per, partial);
+  HIP_CHECK(hipGetLastError());
+  if (reduce) {
+    hipLaunchKernelGGL(k_reduce_sym, dim3(nblk((int64_t)n * n, 256)), dim3(256), 0, c.stream,
+                       partial_buf.as<double>(), nbatch * nch, n, 0, alpha, out);
+    HIP_CHECK(hipGetLastError());
+  }
 }
 
 void dfk_mo_f32(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K, int occ_nbatches)
 {
   const BLayout& L = t.L;
-  if (!t.data32.p && L.total) throw std::runtime_error("FP32 copy of B missing (mchip_dbt_make_f32)");
+  if (!t.lowp_tilde || (!t.data32.p && L.total))
+    throw std::runtime_error("FP32 copy of L^-1 B missing (mchip_dbt_make_lowp or mchip_dbt_make_f32)");
   const int naux = L.naux, nbf = L.nbf;
   if (nocc <= 0) {
     hipLaunchKernelGGL(k_fill, dim3(gs_blocks((int64_t)nbf * nbf)), dim3(256), 0, c.stream, K,
@@ -402,8 +438,7 @@ void dfk_mo_f32(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K,
       slab_gemm_t<float>(c, t, t.data32.as<float>(), G, w, w, W, w);
     }
     PhaseTimer tm(c, PH_K3);
-    seg_syrk_t<float>(c, W, W, naux, (int64_t)nbf * w, w, w, nbf, -1.0, K, c.scratch_partial, nb, b,
-                      b == nb - 1);
+    seg_syrk_f32(c, W, naux, (int64_t)nbf * w, w, w, nbf, -1.0, K, c.scratch_partial, nb, b, b == nb - 1);
   }
 }
 
@@ -477,9 +512,8 @@ namespace mchip {
 // BF16 split-precision exchange on the transformed B~
 // ---------------------------------------------------------------------------
 
-void dbt_make_bf16(Ctx& c, DevBT& t)
+static void bf16_layout(Ctx& c, DevBT& t)
 {
-  if (!t.transformed) throw std::runtime_error("the BF16 copy is made from the transformed B");
   const BLayout& L = t.L;
   t.h_ldp.assign(L.nbb, 0);
   t.h_off16.assign(L.nbb + 1, 0);
@@ -495,14 +529,59 @@ void dbt_make_bf16(Ctx& c, DevBT& t)
   const int64_t tot = t.h_off16[L.nbb];
   t.data_hi.alloc((size_t)std::max<int64_t>(tot, 1) * sizeof(uint16_t));
   t.data_lo.alloc((size_t)std::max<int64_t>(tot, 1) * sizeof(uint16_t));
-  if (tot > 0 && L.nbb > 0)
+  t.bf16_bn.clear();
+}
+
+// Low-precision copies of B~ = L^-1 B made slab by slab from the untransformed
+// B, which stays in data64: J, the reported energy and every FP64 exchange
+// keep the reference's formula (explicit V^-1 in J, L^-1 applied per build in
+// DFMO), while the per-iteration exchange reads the FP32 and/or BF16 copies.
+void dbt_make_lowp(Ctx& c, DevBT& t, const Metric& m, bool f32, bool bf16)
+{
+  if (t.transformed) throw std::runtime_error("B is transformed: use the copies of the transformed tensor");
+  if (m.naux != t.L.naux) throw std::runtime_error("metric and B have different aux dimensions");
+  const BLayout& L = t.L;
+  if (f32) t.data32.alloc((size_t)std::max<int64_t>(L.total, 1) * sizeof(float));
+  if (bf16) bf16_layout(c, t);
+  int64_t maxslab = 0;
+  for (int n = 0; n < L.nbb; ++n) maxslab = std::max<int64_t>(maxslab, L.slab_off[n + 1] - L.slab_off[n]);
+  DevBuf tmp((size_t)std::max<int64_t>(maxslab, 1) * sizeof(double));
+  const double one = 1.0;
+  for (int n = 0; n < L.nbb; ++n) {
+    const int64_t R = (int64_t)L.bsz[n] * L.ncol[n];
+    if (R == 0) continue;
+    const double* slab = t.data64.as<double>() + L.slab_off[n];
+    // column-major (R x naux) view: tmp = slab * (L^-1)^T
+    RB_CHECK(rocblas_dtrmm_64(c.blas, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                              rocblas_diagonal_non_unit, R, L.naux, &one, m.Linv.as<double>(), L.naux, slab, R,
+                              tmp.as<double>(), R));
+    const int64_t ne = R * L.naux;
+    if (f32)
+      hipLaunchKernelGGL(k_to_f32, dim3(gs_blocks(ne)), dim3(256), 0, c.stream, tmp.as<double>(),
+                         t.data32.as<float>() + L.slab_off[n], ne);
+    if (bf16)
+      hipLaunchKernelGGL(k_split_slab, dim3(gs_blocks(ne)), dim3(256), 0, c.stream, tmp.as<double>(), n,
+                         t.ncol.as<int>(), t.bsz.as<int>(), L.naux, t.ldp.as<int>(), t.off16.as<int64_t>(),
+                         t.data_hi.as<uint16_t>(), t.data_lo.as<uint16_t>());
+    HIP_CHECK(hipGetLastError());
+  }
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  t.lowp_tilde = true;
+}
+
+void dbt_make_bf16(Ctx& c, DevBT& t)
+{
+  if (!t.transformed) throw std::runtime_error("the BF16 copy is made from the transformed B");
+  const BLayout& L = t.L;
+  bf16_layout(c, t);
+  if (t.h_off16[L.nbb] > 0 && L.nbb > 0)
     hipLaunchKernelGGL(k_split_slabs, dim3(2048, L.nbb), dim3(256), 0, c.stream, t.data64.as<double>(),
                        t.slab_off.as<int64_t>(), t.ncol.as<int>(), t.bsz.as<int>(), L.naux,
                        t.ldp.as<int>(), t.off16.as<int64_t>(), t.data_hi.as<uint16_t>(),
                        t.data_lo.as<uint16_t>());
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(c.stream));
-  t.bf16_bn.clear();
+  t.lowp_tilde = true;
 }
 
 // Tile table of the BF16 slab GEMM (BM = 128, column tile BN), padded with
@@ -529,49 +608,69 @@ static const Tile* bf16_tiles(Ctx& c, DevBT& t, int BN, int ng, int* ntiles)
   return it->second.as<Tile>();
 }
 
-static void slab_gemm_bf16(Ctx& c, DevBT& t, int BN, const uint16_t* ghi, const uint16_t* glo, int ng,
+// Launch shapes of the BF16 slab GEMM: waves (WM x WN), MFMA tiles per wave
+// (TM x TN); BM = 128 rows for all, BN = 16 * WN * TN columns.
+struct Bf16Shape {
+  int WM, WN, TM, TN, NBUF;
+  int bn() const { return 16 * WN * TN; }
+  int threads() const { return 64 * WM * WN; }
+};
+static const Bf16Shape kBf16Shapes[] = {{2, 2, 4, 2, 2}, {2, 2, 4, 4, 2}, {2, 2, 4, 5, 2}, {2, 4, 4, 2, 2},
+                                        {2, 4, 4, 3, 2}, {2, 4, 4, 4, 2}, {2, 4, 4, 5, 2}, {2, 4, 4, 2, 1},
+                                        {2, 4, 4, 3, 1}};
+static constexpr int kNumBf16Shapes = sizeof(kBf16Shapes) / sizeof(kBf16Shapes[0]);
+
+static void slab_gemm_bf16(Ctx& c, DevBT& t, int shape, const uint16_t* ghi, const uint16_t* glo, int ng,
                            float* W, int ldw)
 {
+  const Bf16Shape sh = kBf16Shapes[shape];
   int nt = 0;
-  const Tile* tiles = bf16_tiles(c, t, BN, ng, &nt);
+  const Tile* tiles = bf16_tiles(c, t, sh.bn(), ng, &nt);
   if (nt == 0) return;
-#define MCHIP_BF16_SLAB(BNV)                                                                              \
-  hipLaunchKernelGGL((k_slab_gemm_bf16x3<SG_BM, BNV>), dim3(nt), dim3(256), 0, c.stream,                  \
+#define MCHIP_BF16_SLAB(WM, WN, TM, TN, NB)                                                               \
+  hipLaunchKernelGGL((k_slab_gemm_bf16x3<WM, WN, TM, TN, NB>), dim3(nt), dim3(64 * WM * WN), 0, c.stream, \
                      t.data_hi.as<uint16_t>(), t.data_lo.as<uint16_t>(), t.off16.as<int64_t>(),           \
                      t.ldp.as<int>(), t.ldp_pre.as<int64_t>(), t.bsz.as<int>(), t.boff.as<int>(), tiles,  \
                      t.L.naux, t.L.nbf, ghi, glo, ng, W, ldw)
-  switch (BN) {
-    case 64: MCHIP_BF16_SLAB(64); break;
-    case 128: MCHIP_BF16_SLAB(128); break;
-    case 160: MCHIP_BF16_SLAB(160); break;
-    default: throw std::runtime_error("BF16 slab GEMM: unsupported column tile");
+  switch (shape) {
+    case 0: MCHIP_BF16_SLAB(2, 2, 4, 2, 2); break;
+    case 1: MCHIP_BF16_SLAB(2, 2, 4, 4, 2); break;
+    case 2: MCHIP_BF16_SLAB(2, 2, 4, 5, 2); break;
+    case 3: MCHIP_BF16_SLAB(2, 4, 4, 2, 2); break;
+    case 4: MCHIP_BF16_SLAB(2, 4, 4, 3, 2); break;
+    case 5: MCHIP_BF16_SLAB(2, 4, 4, 4, 2); break;
+    case 6: MCHIP_BF16_SLAB(2, 4, 4, 5, 2); break;
+    case 7: MCHIP_BF16_SLAB(2, 4, 4, 2, 1); break;
+    case 8: MCHIP_BF16_SLAB(2, 4, 4, 3, 1); break;
+    default: throw std::runtime_error("BF16 slab GEMM: unknown launch shape");
   }
 #undef MCHIP_BF16_SLAB
   HIP_CHECK(hipGetLastError());
 }
 
-// Column tile for ng occupied columns: every candidate is timed once on the
-// live operands (after one warm launch) and the fastest is kept per ng.
-static int tune_bf16_bn(Ctx& c, DevBT& t, const uint16_t* ghi, const uint16_t* glo, int ng, float* W,
-                        int ldw)
+// Launch shape for ng occupied columns: every candidate whose column tile is
+// not mostly empty is timed once on the live operands (after one warm launch)
+// and the fastest is kept per ng.
+static int tune_bf16_shape(Ctx& c, DevBT& t, const uint16_t* ghi, const uint16_t* glo, int ng, float* W,
+                           int ldw)
 {
   auto it = t.bf16_bn.find(ng);
   if (it != t.bf16_bn.end()) return it->second;
-  const int cand[3] = {64, 128, 160};
-  int best = 128;
+  int best = 0;
   float best_ms = 1e30f;
-  for (int BN : cand) {
-    if (BN > 64 && ng <= BN / 2) continue;   // mostly empty tiles
-    slab_gemm_bf16(c, t, BN, ghi, glo, ng, W, ldw);
+  for (int v = 0; v < kNumBf16Shapes; ++v) {
+    const int bn = kBf16Shapes[v].bn();
+    if (v > 0 && bn > 64 && (ng + bn - 1) / bn * bn - ng >= bn / 2) continue;   // mostly empty tiles
+    slab_gemm_bf16(c, t, v, ghi, glo, ng, W, ldw);
     HIP_CHECK(hipEventRecord(c.ev0, c.stream));
-    slab_gemm_bf16(c, t, BN, ghi, glo, ng, W, ldw);
+    slab_gemm_bf16(c, t, v, ghi, glo, ng, W, ldw);
     HIP_CHECK(hipEventRecord(c.ev1, c.stream));
     HIP_CHECK(hipEventSynchronize(c.ev1));
     float ms = 0.f;
     HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
     if (ms < best_ms) {
       best_ms = ms;
-      best = BN;
+      best = v;
     }
   }
   t.bf16_bn[ng] = best;
@@ -581,7 +680,7 @@ static int tune_bf16_bn(Ctx& c, DevBT& t, const uint16_t* ghi, const uint16_t* g
 int dfk_bf16_tuned_bn(const DevBT& t, int ng)
 {
   auto it = t.bf16_bn.find(ng);
-  return it == t.bf16_bn.end() ? 0 : it->second;
+  return it == t.bf16_bn.end() ? 0 : kBf16Shapes[it->second].bn() * 10000 + kBf16Shapes[it->second].threads();
 }
 
 // K = -sum_X W_X^T W_X with W = B~ C, both contractions BF16x3 on MFMA, FP32
@@ -589,8 +688,8 @@ int dfk_bf16_tuned_bn(const DevBT& t, int ng)
 void dfk_mo_bf16(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K, int occ_nbatches)
 {
   const BLayout& L = t.L;
-  if (!t.data_hi.p || t.h_ldp.size() != (size_t)L.nbb)
-    throw std::runtime_error("BF16 copy of B missing (mchip_dbt_make_bf16)");
+  if (!t.lowp_tilde || !t.data_hi.p || t.h_ldp.size() != (size_t)L.nbb)
+    throw std::runtime_error("BF16 copy of L^-1 B missing (mchip_dbt_make_lowp or mchip_dbt_make_bf16)");
   const int naux = L.naux, nbf = L.nbf;
   if (nocc <= 0) {
     hipLaunchKernelGGL(k_fill, dim3(gs_blocks((int64_t)nbf * nbf)), dim3(256), 0, c.stream, K,
@@ -619,9 +718,11 @@ void dfk_mo_bf16(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K
     HIP_CHECK(hipStreamSynchronize(c.stream));
   }
   const int2* d_tl = t.tiles[tkey].as<int2>();
+  // chunks of at most 32 aux functions: the FP32 sum inside a chunk stays short
+  // (3 MFMA accumulations per 32 occupied columns per aux function)
   int nch = std::max(1, std::min(naux, 2048 / std::max(1, ntile)));
   int spc = (naux + nch - 1) / nch;
-  if (spc > 256) spc = 256;
+  if (spc > 32) spc = 32;
   nch = (naux + spc - 1) / spc;
   c.scratch_partial.ensure((size_t)nb * nch * nbf * nbf * sizeof(double));
   int gmax = 0;
@@ -637,10 +738,10 @@ void dfk_mo_bf16(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K
                            t.colmap.as<int>(), t.colmap_off.as<int>(), ghi, glo);
       HIP_CHECK(hipGetLastError());
     }
-    const int BN = tune_bf16_bn(c, t, ghi, glo, w, W, ldw);
+    const int shape = tune_bf16_shape(c, t, ghi, glo, w, W, ldw);
     {
       PhaseTimer tm(c, PH_K1);
-      slab_gemm_bf16(c, t, BN, ghi, glo, w, W, ldw);
+      slab_gemm_bf16(c, t, shape, ghi, glo, w, W, ldw);
     }
     PhaseTimer tm(c, PH_K3);
     double* partial = c.scratch_partial.as<double>() + (size_t)b * nch * nbf * nbf;

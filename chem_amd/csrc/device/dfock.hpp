@@ -25,6 +25,7 @@ void dfk_mo(Ctx& c, DevBT& t, const Metric* m, const double* C, int ldc, int noc
 void dbt_make_f32(Ctx& c, DevBT& t);
 void dfk_mo_f32(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K, int occ_nbatches);
 void dbt_make_bf16(Ctx& c, DevBT& t);
+void dbt_make_lowp(Ctx& c, DevBT& t, const Metric& m, bool f32, bool bf16);
 void dfk_mo_bf16(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K, int occ_nbatches);
 int dfk_bf16_tuned_bn(const DevBT& t, int ng);
 void dfk_ao(Ctx& c, DevBT& t, const Metric* m, const double* P_dev, double* K, int occ_nbatches, int form);

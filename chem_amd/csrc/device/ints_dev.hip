@@ -9,6 +9,8 @@
 // AO shells up to d, auxiliary shells up to f, uncontracted auxiliary basis.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <set>
 #include <vector>
 
 #include "../ints/ints_host.hpp"
@@ -283,8 +285,17 @@ void build_B_device(Ctx& c, const Basis& ao, const Basis& aux, const std::vector
     for (int d = 0; d < 3; ++d) axO[3 * s + d] = sh.O[d];
     aux_hermite_coeffs(sh.l, sh.alpha[0], sh.coeff[0], axH.data() + (size_t)s * AX_NM * nherm(LMAX), nherm(LMAX));
   }
-  const SphTable& S = sph_table();
-  HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_sph), &S.c[0][0][0], sizeof(S.c)));
+  {
+    // the constant table is written once per device (request workers share it)
+    static std::mutex mu;
+    static std::set<int> done;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!done.count(c.device)) {
+      const SphTable& S = sph_table();
+      HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_sph), &S.c[0][0][0], sizeof(S.c)));
+      done.insert(c.device);
+    }
+  }
   const auto& btab = boys_table();
   DevBuf d_aol = upload_vec(aol, c.stream), d_aonp = upload_vec(aonp, c.stream),
          d_aopo = upload_vec(aopo, c.stream), d_aoO = upload_vec(aoO, c.stream),

@@ -55,10 +55,12 @@ __global__ void k_rows_to_f32(const double* in, int ld64, int rows, int ncols, f
 __global__ void k_split_slabs(const double* data64, const int64_t* slab_off, const int* ncol,
                               const int* bsz, int naux, const int* ldp, const int64_t* off16,
                               uint16_t* hi, uint16_t* lo);
+__global__ void k_split_slab(const double* src, int N, const int* ncol, const int* bsz, int naux,
+                             const int* ldp, const int64_t* off16, uint16_t* hi, uint16_t* lo);
 __global__ void k_gather_ct(const double* C, int ldc, int ng, const int* ncol, const int* ldp,
                             const int64_t* ldp_pre, const int* colmap, const int* colmap_off,
                             uint16_t* ghi, uint16_t* glo);
-template <int BM, int BN>
+template <int WM, int WN, int TM, int TN, int NBUF>
 __global__ void k_slab_gemm_bf16x3(const uint16_t* bhi, const uint16_t* blo, const int64_t* off16,
                                    const int* ldp, const int64_t* ldp_pre, const int* bsz,
                                    const int* boff, const Tile* tiles, int naux, int nbf,
@@ -67,6 +69,8 @@ __global__ void k_slab_gemm_bf16x3(const uint16_t* bhi, const uint16_t* blo, con
 template <int BM>
 __global__ void k_seg_syrk_bf16x3(const float* W, int nseg, int64_t sstride, int ldw, int n,
                                   const int2* tiles, int ntile, int segs_per_chunk, double* partial);
+__global__ void k_seg_syrk_f32(const float* W, int nseg, int64_t sstride, int lda, int klen, int n,
+                               const int2* tiles, int ntile, int segs_per_chunk, double* partial);
 
 // Blocks of a one-thread-per-element launch.  A dispatch holds at most 2^32 - 1
 // work-items per grid dimension; larger element counts must use a grid-stride

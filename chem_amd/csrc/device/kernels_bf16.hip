@@ -87,6 +87,27 @@ __global__ void k_split_slabs(const double* __restrict__ data64, const int64_t* 
   }
 }
 
+// One slab N whose FP64 values are at src (same row layout as in data64).
+__global__ void k_split_slab(const double* __restrict__ src, int N, const int* __restrict__ ncol,
+                             const int* __restrict__ bsz, int naux, const int* __restrict__ ldp,
+                             const int64_t* __restrict__ off16, uint16_t* __restrict__ hi,
+                             uint16_t* __restrict__ lo)
+{
+  const int K = ncol[N], ld = ldp[N];
+  const int64_t n = (int64_t)naux * bsz[N] * ld;
+  uint16_t* h = hi + off16[N];
+  uint16_t* l = lo + off16[N];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / ld;
+    const int c = (int)(i - r * ld);
+    uint16_t a = 0, b = 0;
+    if (c < K) split64(src[r * K + c], a, b);
+    h[i] = a;
+    l[i] = b;
+  }
+}
+
 __global__ void k_gather_ct(const double* __restrict__ C, int ldc, int ng, const int* __restrict__ ncol,
                             const int* __restrict__ ldp, const int64_t* __restrict__ ldp_pre,
                             const int* __restrict__ colmap, const int* __restrict__ colmap_off,
@@ -135,18 +156,25 @@ __device__ __forceinline__ void bf16x3_step(const uint4* sAh, const uint4* sAl, 
 // Grouped slab GEMM.  Tile (slab, m0, n0); slab < 0 marks padding of the
 // tile table to a multiple of 8.  Rows r = X*bsz[N] + nu of slab N, columns
 // n = occupied index i < ng, contraction over the slab's padded columns.
-template <int BM, int BN>
-__global__ __launch_bounds__(256) void k_slab_gemm_bf16x3(
+// WM x WN waves, each owning TM x TN MFMA tiles of 16 x 16: the workgroup tile
+// is BM = 16*WM*TM rows by BN = 16*WN*TN columns.  B~ (operand A) streams from
+// HBM once per column tile, so wide column tiles (all occupied orbitals in one
+// tile) keep the kernel off the HBM roof.
+// NBUF = 2: double-buffered LDS, one barrier per step.  NBUF = 1: one LDS
+// buffer (two barriers per step) so that two workgroups share a CU and one's
+// prologue and epilogue overlap the other's MFMA loop.
+template <int WM, int WN, int TM, int TN, int NBUF>
+__global__ __launch_bounds__(64 * WM * WN, NBUF == 1 ? 4 : 1) void k_slab_gemm_bf16x3(
     const uint16_t* __restrict__ bhi, const uint16_t* __restrict__ blo, const int64_t* __restrict__ off16,
     const int* __restrict__ ldp, const int64_t* __restrict__ ldp_pre, const int* __restrict__ bsz,
     const int* __restrict__ boff, const Tile* __restrict__ tiles, int naux, int nbf,
     const uint16_t* __restrict__ ghi, const uint16_t* __restrict__ glo, int ng, float* __restrict__ out,
     int ldo)
 {
-  constexpr int TM = BM / 32, TN = BN / 32;
-  constexpr int CA = (BM * 4 + 255) / 256, CB = (BN * 4 + 255) / 256;   // 16-byte chunks per thread
-  static_assert(BM % 32 == 0 && BN % 32 == 0, "tile shape");
-  __shared__ uint4 sAh[2][BM * LROW], sAl[2][BM * LROW], sBh[2][BN * LROW], sBl[2][BN * LROW];
+  constexpr int NT = 64 * WM * WN;
+  constexpr int BM = 16 * WM * TM, BN = 16 * WN * TN;
+  constexpr int CA = (BM * 4 + NT - 1) / NT, CB = (BN * 4 + NT - 1) / NT;   // 16-byte chunks per thread
+  __shared__ uint4 sAh[NBUF][BM * LROW], sAl[NBUF][BM * LROW], sBh[NBUF][BN * LROW], sBl[NBUF][BN * LROW];
   const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   if (t.slab < 0) return;
   const int N = t.slab;
@@ -158,7 +186,7 @@ __global__ __launch_bounds__(256) void k_slab_gemm_bf16x3(
   const uint4* Gh = reinterpret_cast<const uint4*>(ghi + ldp_pre[N] * ng);
   const uint4* Gl = reinterpret_cast<const uint4*>(glo + ldp_pre[N] * ng);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm0 = (wave >> 1) * (BM / 2), wn0 = (wave & 1) * (BN / 2);
+  const int wm0 = (wave / WN) * (16 * TM), wn0 = (wave % WN) * (16 * TN);
   f32x4 acc[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
@@ -170,7 +198,7 @@ __global__ __launch_bounds__(256) void k_slab_gemm_bf16x3(
     const int kc0 = s * 4;
 #pragma unroll
     for (int j = 0; j < CA; ++j) {
-      const int q = tid + 256 * j;
+      const int q = tid + NT * j;
       const int64_t r = (int64_t)t.m0 + (q >> 2);
       const int64_t o = r * ld8 + kc0 + (q & 3);
       const bool ok = q < BM * 4 && r < rows;
@@ -179,7 +207,7 @@ __global__ __launch_bounds__(256) void k_slab_gemm_bf16x3(
     }
 #pragma unroll
     for (int j = 0; j < CB; ++j) {
-      const int q = tid + 256 * j;
+      const int q = tid + NT * j;
       const int c = t.n0 + (q >> 2);
       const int64_t o = (int64_t)c * ld8 + kc0 + (q & 3);
       const bool ok = q < BN * 4 && c < ng;
@@ -190,7 +218,7 @@ __global__ __launch_bounds__(256) void k_slab_gemm_bf16x3(
   auto store = [&](int buf) {
 #pragma unroll
     for (int j = 0; j < CA; ++j) {
-      const int q = tid + 256 * j;
+      const int q = tid + NT * j;
       if (q < BM * 4) {
         sAh[buf][(q >> 2) * LROW + (q & 3)] = rah[j];
         sAl[buf][(q >> 2) * LROW + (q & 3)] = ral[j];
@@ -198,7 +226,7 @@ __global__ __launch_bounds__(256) void k_slab_gemm_bf16x3(
     }
 #pragma unroll
     for (int j = 0; j < CB; ++j) {
-      const int q = tid + 256 * j;
+      const int q = tid + NT * j;
       if (q < BN * 4) {
         sBh[buf][(q >> 2) * LROW + (q & 3)] = rbh[j];
         sBl[buf][(q >> 2) * LROW + (q & 3)] = rbl[j];
@@ -206,17 +234,29 @@ __global__ __launch_bounds__(256) void k_slab_gemm_bf16x3(
     }
   };
   const int nsteps = ld8 >> 2;
-  if (nsteps > 0) {
-    load(0);
-    store(0);
-  }
-  __syncthreads();
-  for (int s = 0; s < nsteps; ++s) {
-    const int cur = s & 1;
-    if (s + 1 < nsteps) load(s + 1);
-    bf16x3_step<TM, TN>(sAh[cur], sAl[cur], sBh[cur], sBl[cur], wm0, wn0, lane, acc);
-    if (s + 1 < nsteps) store(cur ^ 1);
+  if constexpr (NBUF == 2) {
+    if (nsteps > 0) {
+      load(0);
+      store(0);
+    }
     __syncthreads();
+    for (int s = 0; s < nsteps; ++s) {
+      const int cur = s & 1;
+      if (s + 1 < nsteps) load(s + 1);
+      bf16x3_step<TM, TN>(sAh[cur], sAl[cur], sBh[cur], sBl[cur], wm0, wn0, lane, acc);
+      if (s + 1 < nsteps) store(cur ^ 1);
+      __syncthreads();
+    }
+  }
+  else {
+    if (nsteps > 0) load(0);
+    for (int s = 0; s < nsteps; ++s) {
+      if (s > 0) __syncthreads();   // every wave is done reading the previous step
+      store(0);
+      __syncthreads();
+      if (s + 1 < nsteps) load(s + 1);
+      bf16x3_step<TM, TN>(sAh[0], sAl[0], sBh[0], sBl[0], wm0, wn0, lane, acc);
+    }
   }
   const int fc = lane & 15, fr4 = 4 * (lane >> 4);
 #pragma unroll
@@ -239,7 +279,10 @@ __global__ __launch_bounds__(256) void k_slab_gemm_bf16x3(
 // Segmented SYRK on FP32 W (segment stride sstride, rows of length ldw, a
 // multiple of 32; columns past the occupied count are zero).  tiles[] lists
 // the upper 128 x 128 tiles; workgroup b handles tile b % ntile of chunk
-// b / ntile (segments [chunk*per, chunk*per + per)).
+// b / ntile (segments [chunk*per, chunk*per + per)).  Each wave owns 64 x 64
+// of the tile.  The FP32 accumulation runs over one chunk only (the driver
+// keeps chunks short: 3 MFMA accumulations per 32 occupied columns per aux
+// function) and the chunks are summed in FP64.
 template <int BM>
 __global__ __launch_bounds__(256) void k_seg_syrk_bf16x3(
     const float* __restrict__ W, int nseg, int64_t sstride, int ldw, int n, const int2* __restrict__ tiles,
@@ -323,16 +366,23 @@ __global__ __launch_bounds__(256) void k_seg_syrk_bf16x3(
       }
 }
 
-template __global__ void k_slab_gemm_bf16x3<128, 64>(const uint16_t*, const uint16_t*, const int64_t*,
-    const int*, const int64_t*, const int*, const int*, const Tile*, int, int, const uint16_t*,
-    const uint16_t*, int, float*, int);
-template __global__ void k_slab_gemm_bf16x3<128, 128>(const uint16_t*, const uint16_t*, const int64_t*,
-    const int*, const int64_t*, const int*, const int*, const Tile*, int, int, const uint16_t*,
-    const uint16_t*, int, float*, int);
-template __global__ void k_slab_gemm_bf16x3<128, 160>(const uint16_t*, const uint16_t*, const int64_t*,
-    const int*, const int64_t*, const int*, const int*, const Tile*, int, int, const uint16_t*,
-    const uint16_t*, int, float*, int);
+
 template __global__ void k_seg_syrk_bf16x3<128>(const float*, int, int64_t, int, int, const int2*, int,
-    int, double*);
+                                                 int, double*);
+
+#define MCHIP_SLAB16_INST(WM, WN, TM, TN, NB)                                                          \
+  template __global__ void k_slab_gemm_bf16x3<WM, WN, TM, TN, NB>(                                     \
+      const uint16_t*, const uint16_t*, const int64_t*, const int*, const int64_t*, const int*,         \
+      const int*, const Tile*, int, int, const uint16_t*, const uint16_t*, int, float*, int);
+MCHIP_SLAB16_INST(2, 2, 4, 2, 2)
+MCHIP_SLAB16_INST(2, 2, 4, 4, 2)
+MCHIP_SLAB16_INST(2, 2, 4, 5, 2)
+MCHIP_SLAB16_INST(2, 4, 4, 2, 2)
+MCHIP_SLAB16_INST(2, 4, 4, 3, 2)
+MCHIP_SLAB16_INST(2, 4, 4, 4, 2)
+MCHIP_SLAB16_INST(2, 4, 4, 5, 2)
+MCHIP_SLAB16_INST(2, 4, 4, 2, 1)
+MCHIP_SLAB16_INST(2, 4, 4, 3, 1)
+#undef MCHIP_SLAB16_INST
 
 }  // namespace mchip

@@ -20,6 +20,9 @@
 #include <chrono>
 #include <cmath>
 #include <deque>
+#include <limits>
+#include <memory>
+#include <stdexcept>
 
 #include "dfock.hpp"
 #include "kernels.hpp"
@@ -164,6 +167,16 @@ struct DiisState {
   std::deque<DevBuf> err, trial;
   std::vector<double> B;   // row-major nerr x nerr
   std::vector<double> coeffs;
+  std::vector<DevBuf> spare;   // dropped vectors, reused so that iterations allocate nothing
+  DevBuf take(size_t bytes)
+  {
+    if (!spare.empty()) {
+      DevBuf b = std::move(spare.back());
+      spare.pop_back();
+      return b;
+    }
+    return DevBuf(bytes);
+  }
 };
 
 void scf_run(Ctx& c, DevBT& bt, const Metric* m, const ScfInput& in, ScfOutput& out,
@@ -236,6 +249,7 @@ void scf_run(Ctx& c, DevBT& bt, const Metric* m, const ScfInput& in, ScfOutput& 
   out.t_j.clear();
   out.t_k.clear();
   bool converged = false;
+  double prev_rms = std::numeric_limits<double>::infinity();
   while (true) {
     // form_fock
     HIP_CHECK(hipEventRecord(c.ev0, c.stream));
@@ -245,7 +259,7 @@ void scf_run(Ctx& c, DevBT& bt, const Metric* m, const ScfInput& in, ScfOutput& 
     float tj = 0.f;
     HIP_CHECK(hipEventElapsedTime(&tj, c.ev0, c.ev1));
     HIP_CHECK(hipEventRecord(c.ev0, c.stream));
-    kbuild(iter, P.as<double>(), Cocc.as<double>(), K.as<double>());
+    kbuild(iter, prev_rms, P.as<double>(), Cocc.as<double>(), K.as<double>());
     HIP_CHECK(hipEventRecord(c.ev1, c.stream));
     HIP_CHECK(hipEventSynchronize(c.ev1));
     float tk = 0.f;
@@ -267,6 +281,7 @@ void scf_run(Ctx& c, DevBT& bt, const Metric* m, const ScfInput& in, ScfOutput& 
     const double rms = std::sqrt(dot_dev(c, E.as<double>(), E.as<double>(), nn, part) / ((double)n * n));
     out.history_e.push_back(energy + in.enuc);
     out.history_rms.push_back(rms);
+    prev_rms = rms;
     if (rms < in.threshold && iter > 0) {
       converged = true;
       break;
@@ -274,7 +289,7 @@ void scf_run(Ctx& c, DevBT& bt, const Metric* m, const ScfInput& in, ScfOutput& 
     if (iter > in.max_iter) break;
     if (in.do_diis && iter >= in.diis_start) {
       const bool reduce = (int)diis.err.size() >= in.diis_max;
-      DevBuf ec(mb), fc(mb);
+      DevBuf ec = diis.take(mb), fc = diis.take(mb);
       HIP_CHECK(hipMemcpyAsync(ec.p, E.p, mb, hipMemcpyDeviceToDevice, c.stream));
       HIP_CHECK(hipMemcpyAsync(fc.p, F.p, mb, hipMemcpyDeviceToDevice, c.stream));
       diis.err.push_back(std::move(ec));
@@ -288,9 +303,15 @@ void scf_run(Ctx& c, DevBT& bt, const Metric* m, const ScfInput& in, ScfOutput& 
         }
       }
       int nold = (int)diis.trial.size();
-      if (reduce) diis.err.erase(diis.err.begin() + max_pos);
+      if (reduce) {
+        diis.spare.push_back(std::move(diis.err[max_pos]));
+        diis.err.erase(diis.err.begin() + max_pos);
+      }
       diis.trial.push_back(std::move(fc));
-      if (reduce) diis.trial.erase(diis.trial.begin() + max_pos);
+      if (reduce) {
+        diis.spare.push_back(std::move(diis.trial[max_pos]));
+        diis.trial.erase(diis.trial.begin() + max_pos);
+      }
       const int nerr = (int)diis.err.size();
       std::vector<double> Bn((size_t)nerr * nerr, 0.0);
       // carry over the old B-matrix without row/column max_pos
@@ -321,7 +342,7 @@ void scf_run(Ctx& c, DevBT& bt, const Metric* m, const ScfInput& in, ScfOutput& 
   if (final_k) {
     // FP64 exchange at the final orbitals; J is already FP64 at this density
     HIP_CHECK(hipEventRecord(c.ev0, c.stream));
-    final_k(iter, P.as<double>(), Cocc.as<double>(), K.as<double>());
+    final_k(iter, prev_rms, P.as<double>(), Cocc.as<double>(), K.as<double>());
     HIP_CHECK(hipEventRecord(c.ev1, c.stream));
     HIP_CHECK(hipEventSynchronize(c.ev1));
     float tk = 0.f;
@@ -348,6 +369,55 @@ void scf_run(Ctx& c, DevBT& bt, const Metric* m, const ScfInput& in, ScfOutput& 
   HIP_CHECK(hipMemcpy(out.P.data(), P.p, mb, hipMemcpyDeviceToHost));
   out.F.resize(nn);
   HIP_CHECK(hipMemcpy(out.F.data(), F.p, mb, hipMemcpyDeviceToHost));
+}
+
+void select_k_builders(Ctx& c, DevBT& bt, const Metric* m, int kmode, int nocc, int occ_nbatches,
+                       double threshold, KBuildFn& kb, KBuildFn& kfinal)
+{
+  if (kmode == 0 && !m) throw std::runtime_error("SCF: plain K needs the metric");
+  if (kmode >= 2 && !bt.transformed && !m)
+    throw std::runtime_error("SCF: the final FP64 exchange on the untransformed B needs the metric");
+  if (kmode == 1 && !bt.transformed) throw std::runtime_error("SCF: B is not transformed");
+  DevBT* t = &bt;
+  Ctx* cp = &c;
+  const Metric* km = bt.transformed ? nullptr : m;
+  KBuildFn k64 = [cp, t, km, nocc, occ_nbatches](int, double, const double*, const double* Cocc, double* K) {
+    dfk_mo(*cp, *t, km, Cocc, nocc, nocc, K, occ_nbatches);
+  };
+  kfinal = KBuildFn();
+  if (kmode == 0 || kmode == 1) {
+    kb = k64;
+  }
+  else if (kmode == 2) {
+    if (!bt.lowp_tilde || !bt.data32.p) throw std::runtime_error("SCF: the FP32 mode needs the FP32 copy of L^-1 B");
+    kb = [cp, t, nocc, occ_nbatches](int, double, const double*, const double* Cocc, double* K) {
+      dfk_mo_f32(*cp, *t, Cocc, nocc, nocc, K, occ_nbatches);
+    };
+    kfinal = k64;
+  }
+  else if (kmode == 3) {
+    if (!bt.lowp_tilde || !bt.data_hi.p) throw std::runtime_error("SCF: the BF16 mode needs the BF16 split copy of L^-1 B");
+    kb = [cp, t, nocc, occ_nbatches](int, double, const double*, const double* Cocc, double* K) {
+      dfk_mo_bf16(*cp, *t, Cocc, nocc, nocc, K, occ_nbatches);
+    };
+    kfinal = k64;
+  }
+  else if (kmode == 4) {
+    if (!bt.lowp_tilde || !bt.data_hi.p || !bt.data32.p)
+      throw std::runtime_error("SCF: the escalation mode needs the BF16 and FP32 copies of L^-1 B");
+    const double tau = std::max(1e-4, 10.0 * threshold);
+    auto escalated = std::make_shared<bool>(false);
+    kb = [cp, t, nocc, occ_nbatches, tau, escalated](int, double prev_rms, const double*, const double* Cocc,
+                                                     double* K) {
+      if (prev_rms <= tau) *escalated = true;
+      if (*escalated) dfk_mo_f32(*cp, *t, Cocc, nocc, nocc, K, occ_nbatches);
+      else dfk_mo_bf16(*cp, *t, Cocc, nocc, nocc, K, occ_nbatches);
+    };
+    kfinal = k64;
+  }
+  else {
+    throw std::runtime_error("SCF: unknown K mode");
+  }
 }
 
 }  // namespace mchip

@@ -31,8 +31,17 @@ struct ScfOutput {
   double t_final_k = 0.0;        // seconds of the final FP64 K build (if any)
 };
 
-// K builder callback: (iteration, P_dev, Cocc_dev row-major nbf x nocc, K_dev out)
-using KBuildFn = std::function<void(int, const double*, const double*, double*)>;
+// K builder callback: (iteration, RMS of the previous iteration's error matrix
+// (infinity before the first), P_dev, Cocc_dev row-major nbf x nocc, K_dev out)
+using KBuildFn = std::function<void(int, double, const double*, const double*, double*)>;
+
+// Exchange builders of an SCF run for kmode (0 plain DFMO with the metric per
+// build, 1 DFMO on the transformed B, 2 FP32, 3 BF16x3, 4 BF16x3 escalating to
+// FP32 once the previous error RMS is below max(1e-4, 10 * threshold); modes
+// 2-4 get an FP64 final builder).  The builders capture c, bt and m by
+// reference.
+void select_k_builders(Ctx& c, DevBT& bt, const Metric* m, int kmode, int nocc, int occ_nbatches,
+                       double threshold, KBuildFn& kb, KBuildFn& kfinal);
 
 // final_k (optional): FP64 K builder used once at the converged orbitals to
 // evaluate the reported energy when kbuild works in reduced precision.

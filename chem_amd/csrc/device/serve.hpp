@@ -1,0 +1,68 @@
+// Request loop serving DF-HF requests on one GPU (see serve.hip).
+#pragma once
+
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../ints/basis.hpp"
+#include "../ints/ints_host.hpp"
+#include "dev.hpp"
+
+namespace mchip {
+
+struct SrvJob {
+  // request
+  Basis ao, aux;
+  std::vector<Charge> charges;
+  std::vector<int> ao_blk, aux_blk;   // shells per block
+  int nocc = 0;
+  double enuc = 0.0, threshold = 1e-6, eps = 1e-10;
+  int kmode = 4, max_iter = 100;
+  // bookkeeping (seconds on the server clock)
+  int id = -1, slot = -1;
+  double t_submit = 0.0, t_admit = 0.0, t_done = 0.0;
+  size_t est_bytes = 0;
+  // result
+  bool done = false, failed = false, converged = false;
+  std::string error;
+  double energy = 0.0, energy_iter = 0.0;
+  int iterations = 0, nbf = 0, naux = 0;
+};
+
+class Server {
+ public:
+  Server(int device, size_t budget_bytes, int nslots, int cpu_threads);
+  ~Server();   // finishes every submitted request, then stops the workers
+  int submit(std::unique_ptr<SrvJob> job);
+  bool result(int id, bool wait, SrvJob& out);   // copy of the bookkeeping and result
+  void stats(double* s) const;   // in_use, peak, completed, failed, max live requests
+  double clock() const;          // seconds since the server started
+
+ private:
+  int take();
+  void admit(SrvJob& j);
+  void finish(SrvJob& j, bool admitted);
+  void worker(int slot);
+  void run(Ctx& c, SrvJob& j, bool& admitted);
+
+  int device_;
+  size_t budget_;
+  int nslots_, cpu_per_worker_ = 1;
+  double t0_;
+  mutable std::mutex mu_;
+  std::condition_variable cv_;
+  std::vector<std::unique_ptr<SrvJob>> jobs_;
+  std::deque<int> pending_, waiting_;
+  size_t in_use_ = 0, peak_ = 0;
+  int live_ = 0, max_live_ = 0, completed_ = 0, failed_ = 0;
+  bool stop_ = false;
+  std::vector<std::unique_ptr<Ctx>> ctx_;
+  std::vector<std::thread> workers_;
+};
+
+}  // namespace mchip

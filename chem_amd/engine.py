@@ -165,6 +165,11 @@ class DeviceB:
                                                _lib.dptr(K)))
         return K
 
+    def make_lowp(self, metric, f32=True, bf16=True):
+        """FP32 / BF16 split copies of L^-1 B made from the untransformed B, which
+        stays FP64 for J, the final exchange and the energy (reference formulas)."""
+        _lib.check(_lib.lib().mchip_dbt_make_lowp(self.h, metric.h, int(f32), int(bf16)))
+
     def make_bf16(self):
         """BF16 hi/lo split copy of the transformed B (chosen-form exchange)."""
         _lib.check(_lib.lib().mchip_dbt_make_bf16(self.h))
@@ -187,9 +192,11 @@ class DeviceB:
         return K
 
     def bf16_tile(self, ng):
-        bn = ctypes.c_int()
-        _lib.check(_lib.lib().mchip_dfk_bf16_tile(self.h, ng, ctypes.byref(bn)))
-        return bn.value
+        """(column tile, threads per workgroup) picked by the BF16 autotuner for
+        ng occupied columns, or None before the first build with that ng."""
+        code = ctypes.c_int()
+        _lib.check(_lib.lib().mchip_dfk_bf16_tile(self.h, ng, ctypes.byref(code)))
+        return None if code.value == 0 else (code.value // 10000, code.value % 10000)
 
     def download(self):
         out = np.zeros(self.nelem)

@@ -12,7 +12,7 @@ import numpy as np
 from . import _lib, ints
 from .engine import DeviceB, Metric
 
-KMODES = {"dfmo": 0, "dfmo_transformed": 1, "mixed_f32": 2, "mixed_bf16": 3}
+KMODES = {"dfmo": 0, "dfmo_transformed": 1, "mixed_f32": 2, "mixed_bf16": 3, "mixed_auto": 4}
 
 
 @dataclass

@@ -172,6 +172,14 @@ int mchip_dfk_mo_f32_dev(mchip_dbt t, const double* C_dev, int ldc, int nocc, in
                          double* K_dev);
 int mchip_dfk_mo_f32(mchip_dbt t, const double* C, int nocc, int occ_nbatches, double* K);
 
+/* FP32 and/or BF16 split copies of B~ = L^-1 B made slab by slab from the
+ * untransformed B, which stays FP64 in place: J, the final FP64 exchange and
+ * the reported energy keep the reference's formulas (DF_J with explicit V^-1,
+ * jkdf.cpp:69-190; DFMO_K with L^-1 per build, jkdf.cpp:221-462), and only the
+ * per-iteration exchange of the chosen form (mchip_dfk_mo_f32 / _bf16, SCF
+ * kmodes 2-4) reads the copies. */
+int mchip_dbt_make_lowp(mchip_dbt t, mchip_metric m, int f32, int bf16);
+
 /* BF16 split-precision copy of the transformed B (hi = bf16(x), lo = bf16(x - hi),
  * rows padded to a multiple of 32), made once per geometry.  Chosen form of the
  * B-contractions of DFMO_K::compute_K (src/fock/jkdf.cpp:221-462). */
@@ -192,8 +200,9 @@ int mchip_dfk_mo_bf16(mchip_dbt t, const double* C, int nocc, int occ_nbatches, 
 int mchip_dfk_ao_dev(mchip_dbt t, mchip_metric m, const double* P_dev, int form, int occ_nbatches,
                      double* K_dev);
 int mchip_dfk_ao(mchip_dbt t, mchip_metric m, const double* P, int form, int occ_nbatches, double* K);
-/* Column tile picked by the BF16 slab-GEMM autotuner for ng occupied columns
- * (0 before the first build with that ng).  No counterpart in the reference. */
+/* Launch shape picked by the BF16 slab-GEMM autotuner for ng occupied columns,
+ * as column_tile * 10000 + threads_per_workgroup (0 before the first build with
+ * that ng).  No counterpart in the reference. */
 int mchip_dfk_bf16_tile(mchip_dbt t, int ng, int* bn);
 
 /* ------------------------------------------------------------------------
@@ -203,7 +212,12 @@ int mchip_dfk_bf16_tile(mchip_dbt t, int ng, int* bn);
  * kmode: 0 = DFMO with the metric applied per iteration (reference algorithm),
  *        1 = DFMO on the transformed B (mchip_dbt_transform),
  *        2 = mixed: FP32 exchange in every iteration, one FP64 exchange at
- *            the converged orbitals for the reported energy.
+ *            the converged orbitals for the reported energy,
+ *        3 = mixed with the BF16x3 exchange in every iteration (needs
+ *            mchip_dbt_make_bf16), FP64 exchange for the reported energy,
+ *        4 = precision escalation: BF16x3 while the previous iteration's error
+ *            RMS exceeds max(1e-4, 10 * threshold), FP32 afterwards, FP64 for
+ *            the reported energy (needs both copies).
  * Outputs C (nbf x nbf row-major), orbital energies and per-iteration
  * history (energy, RMS error, J and K seconds) up to max_hist entries.
  * ---------------------------------------------------------------------- */
@@ -216,6 +230,36 @@ int mchip_scf_run(mchip_dbt t, mchip_metric m, int nbf, int nocc, int nbb, const
 /* Of the last mchip_scf_run on this thread: the energy with the iteration's
  * own K and the seconds of the final FP64 K build (mixed mode). */
 int mchip_scf_last_extra(double* energy_iter, double* t_final_k);
+
+/* ------------------------------------------------------------------------
+ * Request loop (DF-HF serving).  No counterpart in the reference, which runs
+ * one molecule per process; each request runs hf::hfmod::compute with the DF
+ * builders (src/hf/hfscf.cpp:326-552).
+ *
+ * nslots worker threads (one stream and rocBLAS handle each) take requests
+ * in arrival order, prepare the host side (one-electron integrals, Coulomb
+ * metric, Schwarz block mask, HBM estimate), wait until the estimate fits
+ * budget_bytes next to the resident requests (admission in arrival order),
+ * then build B on the GPU and run the SCF with everything resident.  Work of
+ * different requests overlaps on the device.  cpu_threads is split over the
+ * workers for the host integrals.
+ * kmode as for mchip_scf_run (0-4); screen_eps is the block threshold.
+ * result: times = {submit, admit, done (server clock, s), HBM estimate (bytes),
+ * worker slot, energy of the last low-precision iteration}; wait = 1 blocks.
+ * A failed request reports its error through the return status.
+ * stats = {bytes in use, peak bytes, completed, failed, max live requests,
+ * server clock}.
+ * ---------------------------------------------------------------------- */
+typedef struct mchip_server_s* mchip_server;
+int mchip_server_create(int device, double budget_bytes, int nslots, int cpu_threads, mchip_server* out);
+int mchip_server_destroy(mchip_server s);   /* finishes every submitted request */
+int mchip_server_submit(mchip_server s, mchip_basis ao, mchip_basis aux, int natom, const double* Z,
+                        const double* xyz, int nbb, const int* ao_blk_shells, int nxb,
+                        const int* aux_blk_shells, int nocc, double enuc, double threshold, int kmode,
+                        double screen_eps, int* id);
+int mchip_server_result(mchip_server s, int id, int wait, int* done, double* energy, int* iterations,
+                        int* converged, double* times);
+int mchip_server_stats(mchip_server s, double* stats);
 
 /* ------------------------------------------------------------------------
  * SOS-AO-MP2 on the GPU: mp::mpmod::compute with LLMP_FULL_Z

@@ -181,10 +181,11 @@ def test_bf16_exchange_accuracy(ctx):
     K64 = dB.dfk_mo(C)
     K16 = dB.dfk_mo_bf16(C)
     err = _relerr(K16, K64)
-    print("BF16x3 K relerr %.3e (column tile %d)" % (err, dB.bf16_tile(mol.nocc)))
+    print("BF16x3 K relerr %.3e (column tile, threads) %s" % (err, dB.bf16_tile(mol.nocc)))
     assert err < BF16_TOL
     assert np.array_equal(K16, K16.T)
-    assert dB.bf16_tile(mol.nocc) in (64, 128, 160)
+    bn, nthr = dB.bf16_tile(mol.nocc)
+    assert bn in (64, 128, 160, 192, 256, 320) and nthr in (256, 512)
     # occupied batches (ragged: 15 = 8 + 7) and determinism
     K16b = dB.dfk_mo_bf16(C, occ_nbatches=2)
     assert _relerr(K16b, K64) < BF16_TOL
@@ -256,3 +257,25 @@ def test_dfao_from_density_vs_oracle(ctx, nwater):
     dB.make_bf16()
     K16 = dB.dfk_ao(P, form="bf16")
     assert _relerr(K16, Kmo) < BF16_TOL
+
+
+def test_lowp_copies_from_untransformed_B(ctx):
+    """The low-precision copies of L^-1 B made slab by slab from the untransformed
+    B give bitwise the same exchange as the copies of the in-place transformed
+    tensor (same rocBLAS triangular multiply per slab), and B itself stays the
+    untransformed FP64 tensor (plain J and K unchanged)."""
+    from chem_amd.engine import DeviceB, Metric
+    mol, Bh, V = _system(3)
+    C, P = _random_occ(mol.nbf, mol.nocc, 5)
+    m = Metric(ctx, V)
+    a = DeviceB(ctx, Bh)
+    J0, K0 = a.dfj(P, m), a.dfk_mo(C, m)
+    a.make_lowp(m)
+    assert np.array_equal(a.dfj(P, m), J0) and np.array_equal(a.dfk_mo(C, m), K0)
+    b = DeviceB(ctx, Bh)
+    b.transform(m)
+    b.make_f32()
+    b.make_bf16()
+    assert np.array_equal(a.dfk_mo_f32(C), b.dfk_mo_f32(C))
+    assert np.array_equal(a.dfk_mo_bf16(C), b.dfk_mo_bf16(C))
+    assert _relerr(a.dfk_mo_bf16(C), K0) < BF16_TOL

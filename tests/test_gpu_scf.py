@@ -93,3 +93,49 @@ def test_mixed_bf16_scf_energy(ctx):
                                                             mix.energy - ref.energy))
     assert ref.converged and mix.converged
     assert abs(mix.energy - ref.energy) < 1e-7
+
+
+def test_mixed_auto_scf_energy(ctx):
+    """Precision escalation (BF16x3 exchange while the error RMS is above
+    max(1e-4, 10 thr), FP32 afterwards, FP64 for the final energy): the
+    converged density is the FP32 fixed point, so |dE| < 1e-8 Eh as for the FP32
+    form."""
+    from chem_amd.engine import DeviceB, Metric
+    from chem_amd.scf import run_rhf
+    mol = desc.Molecule(desc.water_cluster(4, 11))
+    S, T, V = ints.int1e(mol)
+    Bh = ints.BTensorHost.build(mol, 1e-10)
+    m = Metric(ctx, ints.int2c2e(mol))
+    dB = DeviceB(ctx, Bh)
+    dB.transform(m)
+    ref = run_rhf(mol, dB, None, S, T + V, threshold=1e-7, kmode="dfmo_transformed")
+    dB.make_bf16()
+    dB.make_f32()
+    mix = run_rhf(mol, dB, None, S, T + V, threshold=1e-7, kmode="mixed_auto")
+    nbf16 = sum(1 for r in [float("inf")] + mix.history_rms[:-1] if r > 1e-4)
+    print("E64 %.10f Eauto %.10f dE %.2e (BF16 builds before escalation: %d of %d)" % (
+        ref.energy, mix.energy, mix.energy - ref.energy, nbf16, len(mix.t_k)))
+    assert ref.converged and mix.converged
+    assert 0 < nbf16 < len(mix.t_k)
+    assert abs(mix.energy - ref.energy) < 1e-8
+
+
+@pytest.mark.parametrize("kmode", ["mixed_auto", "mixed_f32"])
+def test_lowp_route_scf_energy(ctx, kmode):
+    """Chosen form on the untransformed B: low-precision copies of L^-1 B for the
+    per-iteration exchange, plain FP64 J and a plain FP64 final exchange, so the
+    reported energy uses the reference formulas: |dE| vs the plain FP64 SCF
+    < 1e-8 Eh."""
+    from chem_amd.engine import DeviceB, Metric
+    from chem_amd.scf import run_rhf
+    mol = desc.Molecule(desc.water_cluster(4, 11))
+    S, T, V = ints.int1e(mol)
+    Bh = ints.BTensorHost.build(mol, 1e-10)
+    m = Metric(ctx, ints.int2c2e(mol))
+    dB = DeviceB(ctx, Bh)
+    ref = run_rhf(mol, dB, m, S, T + V, threshold=1e-7, kmode="dfmo")
+    dB.make_lowp(m)
+    mix = run_rhf(mol, dB, m, S, T + V, threshold=1e-7, kmode=kmode)
+    print("%s: E64 %.10f Emix %.10f dE %.2e" % (kmode, ref.energy, mix.energy, mix.energy - ref.energy))
+    assert ref.converged and mix.converged
+    assert abs(mix.energy - ref.energy) < 1e-8
