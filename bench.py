@@ -37,6 +37,8 @@ CONFIGS = {
     "c0": dict(nwater=1, seed=0),
     "c1": dict(nwater=32, seed=1234),
     "c2": dict(nwater=64, seed=1234),
+    # batched SOS-AO-MP2: 32 waters, cc-pVDZ / cc-pVDZ-ri, 8 Laplace points
+    "c3": dict(nwater=32, seed=1234, nlap=8),
     # serving stream: Poisson arrivals at 4/s, clusters of U{4..64} waters
     "c4": dict(requests=200, rate=4.0, nmin=4, nmax=64, seed=2024, budget=200e9, thresh=1e-6),
 }
@@ -117,20 +119,30 @@ def serve_stream(args, log_progress=True):
             mol = desc.Molecule(desc.water_cluster(int(sizes[i]), 5000 + i))
             mols[i] = mol
             ids.append(srv.submit(mol, threshold=cfg["thresh"], kmode="mixed_auto"))
+        failures = []
         for i, rid in enumerate(ids):
-            r = srv.result(rid)
+            try:
+                r = srv.result(rid)
+            except Exception as e:          # a failed request is reported, not fatal
+                failures.append((rid, int(sizes[i]), str(e)))
+                log("request %d (%d waters) FAILED: %s" % (rid, sizes[i], e))
+                results.append(None)
+                continue
             r["nwater"] = int(sizes[i])
             results.append(r)
             if log_progress:
-                log("request %d (%d waters): E=%.8f iters=%d latency %.2f s (admitted after %.2f s, slot %d)" % (
-                    rid, sizes[i], r["energy"], r["iterations"], r["latency"], r["t_admit"] - r["t_submit"],
-                    r["slot"]))
+                log("request %d (%d waters): E=%.8f iters=%d latency %.2f s (admitted after %.2f s, slot %d; "
+                    "host %.2f build %.2f setup %.2f scf %.2f s)" % (
+                        rid, sizes[i], r["energy"], r["iterations"], r["latency"], r["t_admit"] - r["t_submit"],
+                        r["slot"], r["t_host"], r["t_build"], r["t_setup"], r["t_scf"]))
         st = srv.stats()
-    lat = np.array([r["latency"] for r in results])
-    span = max(r["t_done"] for r in results) - min(r["t_submit"] for r in results)
-    iters = sum(r["iterations"] + 1 for r in results)
+    ok = [r for r in results if r is not None]
+    lat = np.array([r["latency"] for r in ok])
+    span = max(r["t_done"] for r in ok) - min(r["t_submit"] for r in ok)
+    fock_builds = sum(r["iterations"] + 1 for r in ok)
     # energy check: plain FP64 path (kmode dfmo, the reference formulas) on a sample
-    sample = sorted(set(int(i) for i in np.linspace(0, nreq - 1, min(args.serve_check, nreq)).round()))
+    sample = [i for i in sorted(set(int(i) for i in np.linspace(0, nreq - 1, min(args.serve_check, nreq)).round()))
+              if results[i] is not None]
     dEs = []
     with Server(budget_bytes=cfg["budget"], nslots=1) as ref:
         for i in sample:
@@ -141,10 +153,94 @@ def serve_stream(args, log_progress=True):
     return dict(requests=nreq, rate_per_s=cfg["rate"], sizes="U{%d..%d} waters" % (cfg["nmin"], cfg["nmax"]),
                 budget_gb=cfg["budget"] / 1e9, slots=args.slots, p50_s=float(np.percentile(lat, 50)),
                 p99_s=float(np.percentile(lat, 99)), mean_s=float(lat.mean()), max_s=float(lat.max()),
-                scf_iterations_per_s=iters / span, requests_per_s=nreq / span, span_s=span,
+                scf_iterations_per_s=fock_builds / span, requests_per_s=len(ok) / span, span_s=span,
                 max_live=st["max_live"], peak_hbm_gb=st["peak_bytes"] / 1e9, failed=st["failed"],
-                all_converged=all(r["converged"] for r in results),
-                max_abs_dE_vs_fp64=float(max(dEs)) if dEs else None, dE_sample=len(dEs))
+                failures=failures[:5], all_converged=all(r["converged"] for r in ok),
+                max_abs_dE_vs_fp64=float(max(dEs)) if dEs else None, dE_sample=len(dEs),
+                mean_phase_s={k: float(np.mean([r[k] for r in ok])) for k in ("t_host", "t_build", "t_setup",
+                                                                              "t_scf")})
+
+
+def mp2_batched_bench(args):
+    """SOS-AO-MP2 Z builders on the 32-water cluster: FP64 per-point launches
+    (the reference's loop over Laplace points) against all 8 points as grouped
+    FP32 launches (FP64 accumulation) in one hipGraph.  Orbitals from the FP64
+    GPU SCF (cc-pVDZ-jkfit, 1e-8)."""
+    from chem_amd import desc, ints
+    from chem_amd.engine import Context, DeviceB, Metric
+    from chem_amd.mp2 import sos_mp2, laplace_for
+    from chem_amd.scf import run_rhf
+    cfg = CONFIGS["c3"]
+    atoms = desc.water_cluster(cfg["nwater"], cfg["seed"])
+    molj = desc.Molecule(atoms)
+    mol = desc.Molecule(atoms, dfbasis="cc-pvdz-ri")
+    ctx = Context(0)
+    S, T, V = ints.int1e(molj)
+    dBj = DeviceB.build_on_gpu(ctx, molj, 1e-10)
+    metj = Metric(ctx, ints.int2c2e(molj))
+    r = run_rhf(molj, dBj, metj, S, T + V, threshold=1e-8)
+    log("SCF: E=%.10f iters=%d" % (r.energy, r.iterations))
+    dBj.release()
+    metj.release()
+    dB = DeviceB.build_on_gpu(ctx, mol, 1e-10)
+    met = Metric(ctx, ints.int2c2e(mol))
+    nnz = dB.nnz_kept()
+    w, a, err = laplace_for(r.eps, mol.nocc, cfg["nlap"])
+    t0 = time.perf_counter()
+    ref = sos_mp2(mol, dB, met, S, r.C, r.eps, points=(w, a))
+    t_ref = time.perf_counter() - t0
+    log("FP64 per point: E=%.12f Z builders %.3f s (total %.2f s)" % (ref["energy"], ref["t_z"], t_ref))
+    t0 = time.perf_counter()
+    got = sos_mp2(mol, dB, met, S, r.C, r.eps, points=(w, a), batched=True)
+    t_b = time.perf_counter() - t0
+    log("batched graph: E=%.12f graph %.3f s (set-up %.2f s, total %.2f s)" % (
+        got["energy"], got["t_z"], got["t_prep"], t_b))
+    # algorithmic work per point: 3 * 2 nnz(B) nocc + 2 naux nnz(B)
+    flops = cfg["nlap"] * (6.0 * nnz * mol.nocc + 2.0 * mol.naux * nnz)
+    return dict(nwater=cfg["nwater"], nbf=mol.nbf, naux=mol.naux, nocc=mol.nocc, nlap=cfg["nlap"], nnz_B=nnz,
+                laplace_err=float(err), E_fp64=ref["energy"], E_batched=got["energy"],
+                dE=got["energy"] - ref["energy"], rel_err=abs(got["energy"] - ref["energy"]) / ref["energy"],
+                max_term_err=float(np.abs(got["terms"] - ref["terms"]).max()),
+                t_z_fp64_per_point=ref["t_z"], t_z_batched_graph=got["t_z"], t_batched_setup=got["t_prep"],
+                speedup_z=ref["t_z"] / got["t_z"], graph_tflops=flops / got["t_z"] / 1e12)
+
+
+def kernels_only(args):
+    """Exchange builds alone, for profiling: B, metric and the low-precision
+    copies of the 64-water cluster, then `steps` BF16x3 builds, `steps` FP32
+    builds and one plain FP64 build at orthonormalised random occupied
+    coefficients (the kernels' work does not depend on the orbitals)."""
+    from chem_amd import desc, ints
+    from chem_amd.engine import Context, DeviceB, Metric
+    cfg = CONFIGS[args.config if args.config in ("c1", "c2") else "c2"]
+    mol = desc.Molecule(desc.water_cluster(cfg["nwater"], cfg["seed"]))
+    ctx = Context(0)
+    dB = DeviceB.build_on_gpu(ctx, mol, 1e-10)
+    met = Metric(ctx, ints.int2c2e(mol))
+    dB.make_lowp(met)
+    S, _, _ = ints.int1e(mol)
+    rng = np.random.default_rng(7)
+    C = rng.standard_normal((mol.nbf, mol.nocc))
+    w, U = np.linalg.eigh(C.T @ S @ C)
+    C = np.ascontiguousarray(C @ U / np.sqrt(w))
+    steps = args.steps or 3
+    out = {}
+    for name, fn in (("bf16x3", dB.dfk_mo_bf16), ("f32", dB.dfk_mo_f32)):
+        fn(C)
+        ctx.sync()
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn(C)
+        ctx.sync()
+        out[name + "_build_s"] = (time.perf_counter() - t) / steps
+        log("%s build %.4f s" % (name, out[name + "_build_s"]))
+    t = time.perf_counter()
+    dB.dfk_mo(C, met)
+    ctx.sync()
+    out["fp64_build_s"] = time.perf_counter() - t
+    out["bf16_plan"] = dB.bf16_plan(mol.nocc)
+    log("fp64 build %.4f s" % out["fp64_build_s"])
+    return out
 
 
 def main():
@@ -155,6 +251,10 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--scf-thresh", type=float, default=1e-6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="profiling mode: exchange builds only (no SCF), one JSON line of timings")
+    ap.add_argument("--escalate", type=float, default=1e-4,
+                    help="error RMS at which the auto form switches from BF16x3 to FP32 builds")
     ap.add_argument("--requests", type=int, default=0, help="c4: number of requests (0: 200)")
     ap.add_argument("--slots", type=int, default=4, help="c4: request workers")
     ap.add_argument("--serve-check", type=int, default=4,
@@ -171,6 +271,25 @@ def main():
     from chem_amd import desc, ints
     from chem_amd.engine import Context, DeviceB, Metric
     from chem_amd.scf import run_rhf
+
+    if args.kernels_only:
+        print(json.dumps(kernels_only(args)), flush=True)
+        return
+
+    if args.config == "c3":
+        mb = mp2_batched_bench(args)
+        out = {
+            "metric": "SOS-AO-MP2 Z builders, 8 Laplace points in one grouped hipGraph (FP32, FP64 accumulation)",
+            "value": mb["t_z_batched_graph"], "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 0,
+            "ms_per_step": mb["t_z_batched_graph"] * 1e3, "higher_is_better": False, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32 contractions, f64 accumulation and energies",
+            "data": "synthetic seeded water cluster, real integrals",
+            "config": {"workload": "SOS-AO-MP2 (c3): %d waters cc-pVDZ/cc-pVDZ-ri, %d Laplace points" % (
+                mb["nwater"], mb["nlap"])},
+            "mp2": mb,
+        }
+        print(json.dumps(out), flush=True)
+        return
 
     if args.config == "c4":
         sv = serve_stream(args)
@@ -239,13 +358,14 @@ def main():
     ctx.sync()
     t_copies = time.perf_counter() - tc
     kmode = {"auto": "mixed_auto", "bf16": "mixed_bf16", "f32": "mixed_f32"}[form]
+    ctx.set_escalation(args.escalate)
     t2 = time.perf_counter()
     chosen = run_rhf(mol, dB, met, S, T + V, threshold=args.scf_thresh, kmode=kmode)
     dE = chosen.energy - ref.energy
     nbuild = len(chosen.t_k)
     if form == "auto":
         # the library's escalation rule: BF16 while the previous RMS exceeds tau
-        tau = max(1e-4, 10.0 * args.scf_thresh)
+        tau = max(args.escalate, 10.0 * args.scf_thresh)
         n16 = 0
         for i in range(nbuild):
             prev = chosen.history_rms[i - 1] if i > 0 else float("inf")
@@ -295,14 +415,14 @@ def main():
                                              builds=n16, peak=MFMA_BF16_PEAK / 1e12)
         kernels["k_seg_syrk_bf16x3"] = dict(s=ph[2], tflops=k3_flops / ph[2] / 1e12,
                                             builds=n16, peak=MFMA_BF16_PEAK / 1e12)
-        kernels["launch_shape"] = dB.bf16_tile(mol.nocc)
+        kernels["bf16_plan"] = dB.bf16_plan(mol.nocc)
     if k32:
         ph = phases(k32)
         kernels["k_slab_gemm<float>"] = dict(s=ph[0], tflops=k1_flops / ph[0] / 1e12,
                                              builds=steps - n16, peak=MFMA_F32_PEAK / 1e12)
         kernels["k_seg_gemm<float>"] = dict(s=ph[2], tflops=k3_flops / ph[2] / 1e12,
                                             builds=steps - n16, peak=MFMA_F32_PEAK / 1e12)
-    dom = max((k for k in kernels if isinstance(kernels[k], dict)),
+    dom = max((k for k in kernels if isinstance(kernels[k], dict) and "s" in kernels[k]),
               key=lambda k: kernels[k]["s"] * kernels[k]["builds"])
     d = kernels[dom]
     roof = dict(bound="mfma", achieved=d["tflops"], peak=d["peak"], unit="TFLOP/s",

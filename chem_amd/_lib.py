@@ -122,9 +122,14 @@ _SIGS = [
     ("mchip_mp2_sos", c_int, [c_void_p, c_void_p, c_int, c_int, P_double, P_double, P_double, c_int,
                               P_double, P_double, c_int, P_int, c_int, c_double, P_double, P_double,
                               P_double]),
+    ("mchip_mp2_sos_batched", c_int, [c_void_p, c_void_p, c_int, c_int, P_double, P_double, P_double, c_int,
+                              P_double, P_double, c_int, P_int, c_int, c_double, P_double, P_double,
+                              P_double, P_double]),
     ("mchip_probe_mfma_f64", c_int, [P_double, P_double, P_double]),
     ("mchip_probe_mfma_lp", c_int, [c_int, P_float, P_float, P_float]),
+    ("mchip_probe_lds_load", c_int, [ctypes.POINTER(ctypes.c_uint32), P_int, ctypes.POINTER(ctypes.c_uint32)]),
     ("mchip_ctx_set_timing", c_int, [c_void_p, c_int]),
+    ("mchip_ctx_set_escalation", c_int, [c_void_p, c_double]),
     ("mchip_dbt_make_f32", c_int, [c_void_p]),
     ("mchip_dfk_mo_f32_dev", c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     ("mchip_dfk_mo_f32", c_int, [c_void_p, P_double, c_int, c_int, P_double]),
@@ -133,6 +138,9 @@ _SIGS = [
     ("mchip_dfk_mo_bf16_dev", c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     ("mchip_dfk_mo_bf16", c_int, [c_void_p, P_double, c_int, c_int, P_double]),
     ("mchip_dfk_bf16_tile", c_int, [c_void_p, c_int, P_int]),
+    ("mchip_dfk_bf16_plan", c_int, [c_void_p, c_int, P_int, P_float, P_int]),
+    ("mchip_dfk_bf16_force", c_int, [c_void_p, c_int, c_int, c_int]),
+    ("mchip_dfk_bf16_shapes", c_int, [P_int, P_int, P_int]),
     ("mchip_dfk_ao_dev", c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
     ("mchip_dfk_ao", c_int, [c_void_p, c_void_p, P_double, c_int, c_int, P_double]),
     ("mchip_server_create", c_int, [c_int, c_double, c_int, c_int, ctypes.POINTER(c_void_p)]),

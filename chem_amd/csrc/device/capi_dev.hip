@@ -83,6 +83,14 @@ extern "C" int mchip_ctx_stream(mchip_ctx c, void** stream)
   MCHIP_CATCH
 }
 
+extern "C" int mchip_ctx_set_escalation(mchip_ctx c, double rms)
+{
+  MCHIP_TRY
+  if (!(rms > 0.0)) throw std::runtime_error("mchip_ctx_set_escalation: the threshold must be positive");
+  c->c.escalate_rms = rms;
+  MCHIP_CATCH
+}
+
 extern "C" int mchip_ctx_set_timing(mchip_ctx c, int on)
 {
   MCHIP_TRY
@@ -267,10 +275,9 @@ extern "C" int mchip_dfj(mchip_dbt t, mchip_metric m, const double* P, double ps
   const int nbf = t->t.L.nbf;
   const size_t bytes = (size_t)nbf * nbf * sizeof(double);
   DevBuf dP(bytes), dJ(bytes);
-  HIP_CHECK(hipMemcpyAsync(dP.p, P, bytes, hipMemcpyHostToDevice, c.stream));
+  h2d(dP.p, P, bytes, c.stream);
   dfj(c, t->t, m ? &m->m : nullptr, dP.as<double>(), dJ.as<double>(), pscale);
-  HIP_CHECK(hipMemcpyAsync(J, dJ.p, bytes, hipMemcpyDeviceToHost, c.stream));
-  HIP_CHECK(hipStreamSynchronize(c.stream));
+  d2h(J, dJ.p, bytes, c.stream);
   MCHIP_CATCH
 }
 
@@ -284,10 +291,9 @@ extern "C" int mchip_dfk_mo(mchip_dbt t, mchip_metric m, const double* C, int no
   const size_t kb = (size_t)nbf * nbf * sizeof(double);
   const size_t cb = (size_t)nbf * std::max(nocc, 1) * sizeof(double);
   DevBuf dC(cb), dK(kb);
-  if (nocc > 0) HIP_CHECK(hipMemcpyAsync(dC.p, C, (size_t)nbf * nocc * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  if (nocc > 0) h2d(dC.p, C, (size_t)nbf * nocc * sizeof(double), c.stream);
   dfk_mo(c, t->t, m ? &m->m : nullptr, dC.as<double>(), nocc, nocc, dK.as<double>(), occ_nbatches);
-  HIP_CHECK(hipMemcpyAsync(K, dK.p, kb, hipMemcpyDeviceToHost, c.stream));
-  HIP_CHECK(hipStreamSynchronize(c.stream));
+  d2h(K, dK.p, kb, c.stream);
   MCHIP_CATCH
 }
 
@@ -328,10 +334,9 @@ extern "C" int mchip_dfk_mo_f32(mchip_dbt t, const double* C, int nocc, int occ_
   const int nbf = t->t.L.nbf;
   const size_t kb = (size_t)nbf * nbf * sizeof(double);
   DevBuf dC((size_t)nbf * std::max(nocc, 1) * sizeof(double)), dK(kb);
-  if (nocc > 0) HIP_CHECK(hipMemcpyAsync(dC.p, C, (size_t)nbf * nocc * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  if (nocc > 0) h2d(dC.p, C, (size_t)nbf * nocc * sizeof(double), c.stream);
   dfk_mo_f32(c, t->t, dC.as<double>(), nocc, nocc, dK.as<double>(), occ_nbatches);
-  HIP_CHECK(hipMemcpyAsync(K, dK.p, kb, hipMemcpyDeviceToHost, c.stream));
-  HIP_CHECK(hipStreamSynchronize(c.stream));
+  d2h(K, dK.p, kb, c.stream);
   MCHIP_CATCH
 }
 
@@ -369,10 +374,9 @@ extern "C" int mchip_dfk_mo_bf16(mchip_dbt t, const double* C, int nocc, int occ
   const int nbf = t->t.L.nbf;
   const size_t kb = (size_t)nbf * nbf * sizeof(double);
   DevBuf dC((size_t)nbf * std::max(nocc, 1) * sizeof(double)), dK(kb);
-  if (nocc > 0) HIP_CHECK(hipMemcpyAsync(dC.p, C, (size_t)nbf * nocc * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  if (nocc > 0) h2d(dC.p, C, (size_t)nbf * nocc * sizeof(double), c.stream);
   dfk_mo_bf16(c, t->t, dC.as<double>(), nocc, nocc, dK.as<double>(), occ_nbatches);
-  HIP_CHECK(hipMemcpyAsync(K, dK.p, kb, hipMemcpyDeviceToHost, c.stream));
-  HIP_CHECK(hipStreamSynchronize(c.stream));
+  d2h(K, dK.p, kb, c.stream);
   MCHIP_CATCH
 }
 
@@ -396,10 +400,9 @@ extern "C" int mchip_dfk_ao(mchip_dbt t, mchip_metric m, const double* P, int fo
   const int nbf = t->t.L.nbf;
   const size_t kb = (size_t)nbf * nbf * sizeof(double);
   DevBuf dP(kb), dK(kb);
-  HIP_CHECK(hipMemcpyAsync(dP.p, P, kb, hipMemcpyHostToDevice, c.stream));
+  h2d(dP.p, P, kb, c.stream);
   dfk_ao(c, t->t, m ? &m->m : nullptr, dP.as<double>(), dK.as<double>(), occ_nbatches, form);
-  HIP_CHECK(hipMemcpyAsync(K, dK.p, kb, hipMemcpyDeviceToHost, c.stream));
-  HIP_CHECK(hipStreamSynchronize(c.stream));
+  d2h(K, dK.p, kb, c.stream);
   MCHIP_CATCH
 }
 
@@ -407,6 +410,29 @@ extern "C" int mchip_dfk_bf16_tile(mchip_dbt t, int ng, int* bn)
 {
   MCHIP_TRY
   *bn = dfk_bf16_tuned_bn(t->t, ng);
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dfk_bf16_force(mchip_dbt t, int ng, int gemm_shape, int syrk_variant)
+{
+  MCHIP_TRY
+  dfk_bf16_force(t->t, ng, gemm_shape, syrk_variant);
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dfk_bf16_shapes(int* ngemm, int* nsyrk, int* direct)
+{
+  MCHIP_TRY
+  *ngemm = dfk_bf16_num_shapes(nsyrk);
+  if (direct)
+    for (int i = 0; i < *ngemm; ++i) direct[i] = dfk_bf16_shape_dl(i);
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dfk_bf16_plan(mchip_dbt t, int ng, int* plan, float* ms, int* nms)
+{
+  MCHIP_TRY
+  dfk_bf16_plan(t->t, ng, plan, ms, nms);
   MCHIP_CATCH
 }
 
@@ -463,14 +489,15 @@ extern "C" int mchip_scf_run(mchip_dbt t, mchip_metric m, int nbf, int nocc, int
 
 #include "mp2.hpp"
 
-extern "C" int mchip_mp2_sos(mchip_dbt t, mchip_metric m, int nbf, int nocc, const double* S,
-                             const double* C, const double* eps, int nlap, const double* omega,
-                             const double* alpha, int nbb, const int* bsz, int mo_split,
-                             double filter_eps, double* energy, double* terms, double* t_z)
+static int mp2_entry(mchip_dbt t, mchip_metric m, int nbf, int nocc, const double* S, const double* C,
+                     const double* eps, int nlap, const double* omega, const double* alpha, int nbb,
+                     const int* bsz, int mo_split, double filter_eps, double* energy, double* terms, double* t_z,
+                     double* t_prep, bool batched)
 {
   MCHIP_TRY
   need_device(t->ctx);
   if (!m) throw std::runtime_error("mchip_mp2_sos: the metric is required");
+  if (nlap <= 0) throw std::runtime_error("mchip_mp2_sos: at least one Laplace point is required");
   Mp2Input in;
   in.nbf = nbf;
   in.nocc = nocc;
@@ -483,12 +510,35 @@ extern "C" int mchip_mp2_sos(mchip_dbt t, mchip_metric m, int nbf, int nocc, con
   in.mo_split = mo_split;
   in.filter_eps = filter_eps;
   std::vector<double> tv;
-  double tz = 0.0;
-  *energy = mp2_sos(t->ctx->c, t->t, m->m, in, tv, &tz);
+  double tz = 0.0, tp = 0.0;
+  if (batched)
+    *energy = mp2_sos_batched(t->ctx->c, t->t, m->m, in, tv, &tz, &tp);
+  else
+    *energy = mp2_sos(t->ctx->c, t->t, m->m, in, tv, &tz);
   if (terms)
     for (int k = 0; k < nlap && k < (int)tv.size(); ++k) terms[k] = tv[k];
   if (t_z) *t_z = tz;
+  if (t_prep) *t_prep = tp;
   MCHIP_CATCH
+}
+
+extern "C" int mchip_mp2_sos(mchip_dbt t, mchip_metric m, int nbf, int nocc, const double* S,
+                             const double* C, const double* eps, int nlap, const double* omega,
+                             const double* alpha, int nbb, const int* bsz, int mo_split,
+                             double filter_eps, double* energy, double* terms, double* t_z)
+{
+  return mp2_entry(t, m, nbf, nocc, S, C, eps, nlap, omega, alpha, nbb, bsz, mo_split, filter_eps, energy, terms,
+                   t_z, nullptr, false);
+}
+
+extern "C" int mchip_mp2_sos_batched(mchip_dbt t, mchip_metric m, int nbf, int nocc, const double* S,
+                                     const double* C, const double* eps, int nlap, const double* omega,
+                                     const double* alpha, int nbb, const int* bsz, int mo_split,
+                                     double filter_eps, double* energy, double* terms, double* t_z,
+                                     double* t_prep)
+{
+  return mp2_entry(t, m, nbf, nocc, S, C, eps, nlap, omega, alpha, nbb, bsz, mo_split, filter_eps, energy, terms,
+                   t_z, t_prep, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -569,6 +619,10 @@ extern "C" int mchip_server_result(mchip_server s, int id, int wait, int* done, 
       times[3] = (double)j.est_bytes;
       times[4] = (double)j.slot;
       times[5] = j.energy_iter;
+      times[6] = j.t_host;
+      times[7] = j.t_build;
+      times[8] = j.t_setup;
+      times[9] = j.t_scf;
     }
   }
   MCHIP_CATCH

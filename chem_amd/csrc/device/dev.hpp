@@ -5,6 +5,8 @@
 #include <rocblas/rocblas.h>
 
 #include <cstdint>
+#include <algorithm>
+#include <cstring>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -31,56 +33,33 @@ namespace mchip {
                                ":" + std::to_string(__LINE__));                   \
   } while (0)
 
-// Allocation stream of the calling thread.  When set (the request workers of
-// the server set it to their context's stream), DevBuf allocates and frees
-// stream-ordered on it, so freeing a buffer does not wait for the work other
-// threads have queued on the device.
-inline hipStream_t& alloc_stream()
-{
-  static thread_local hipStream_t s = nullptr;
-  return s;
-}
-
 // Owning device buffer.
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
-  hipStream_t astream = nullptr;   // stream of a stream-ordered allocation
   DevBuf() = default;
   explicit DevBuf(size_t n) { alloc(n); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), astream(o.astream)
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes)
   {
     o.p = nullptr;
     o.bytes = 0;
-    o.astream = nullptr;
   }
   DevBuf& operator=(DevBuf&& o) noexcept
   {
     release();
     p = o.p;
     bytes = o.bytes;
-    astream = o.astream;
     o.p = nullptr;
     o.bytes = 0;
-    o.astream = nullptr;
     return *this;
   }
   ~DevBuf() { release(); }
   void alloc(size_t n)
   {
     release();
-    if (n) {
-      hipStream_t s = alloc_stream();
-      if (s) {
-        HIP_CHECK(hipMallocAsync(&p, n, s));
-        astream = s;
-      }
-      else {
-        HIP_CHECK(hipMalloc(&p, n));
-      }
-    }
+    if (n) HIP_CHECK(hipMalloc(&p, n));
     bytes = n;
   }
   // grow-only
@@ -90,22 +69,82 @@ struct DevBuf {
   }
   void release()
   {
-    if (p) {
-      if (astream) (void)hipFreeAsync(p, astream);
-      else (void)hipFree(p);
-    }
+    if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
-    astream = nullptr;
   }
   template <class T> T* as() const { return reinterpret_cast<T*>(p); }
 };
+
+// Page-locked host buffer (hipHostMalloc): host data for large transfers,
+// copied by DMA without staging through the runtime's pageable path.
+template <class T>
+struct PinnedVec {
+  T* p = nullptr;
+  size_t n = 0;
+  explicit PinnedVec(size_t count) : n(count)
+  {
+    if (n) HIP_CHECK(hipHostMalloc((void**)&p, n * sizeof(T), hipHostMallocDefault));
+  }
+  PinnedVec(const PinnedVec&) = delete;
+  PinnedVec& operator=(const PinnedVec&) = delete;
+  ~PinnedVec()
+  {
+    if (p) (void)hipHostFree(p);
+  }
+  T* data() { return p; }
+  size_t size() const { return n; }
+  T& operator[](size_t i) { return p[i]; }
+};
+
+// Host<->device copies through a per-thread page-locked staging buffer, in
+// order on stream s and complete on return: the runtime's pageable-copy path
+// (staging of ordinary host memory) is never used, which keeps host transfers
+// of concurrent request workers independent of each other.
+struct HostStage {
+  static constexpr size_t kBytes = (size_t)4 << 20;
+  uint8_t* p = nullptr;
+  HostStage() { HIP_CHECK(hipHostMalloc((void**)&p, kBytes, hipHostMallocDefault)); }
+  ~HostStage()
+  {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
+inline HostStage& host_stage()
+{
+  static thread_local HostStage st;
+  return st;
+}
+
+inline void h2d(void* dev, const void* host, size_t bytes, hipStream_t s)
+{
+  HostStage& st = host_stage();
+  for (size_t off = 0; off < bytes; off += HostStage::kBytes) {
+    const size_t len = std::min(HostStage::kBytes, bytes - off);
+    std::memcpy(st.p, (const uint8_t*)host + off, len);
+    HIP_CHECK(hipMemcpyAsync((uint8_t*)dev + off, st.p, len, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
+}
+
+inline void d2h(void* host, const void* dev, size_t bytes, hipStream_t s)
+{
+  HostStage& st = host_stage();
+  for (size_t off = 0; off < bytes; off += HostStage::kBytes) {
+    const size_t len = std::min(HostStage::kBytes, bytes - off);
+    HIP_CHECK(hipMemcpyAsync(st.p, (const uint8_t*)dev + off, len, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    std::memcpy((uint8_t*)host + off, st.p, len);
+  }
+  if (bytes == 0) HIP_CHECK(hipStreamSynchronize(s));
+}
 
 template <class T>
 DevBuf upload_vec(const std::vector<T>& v, hipStream_t s)
 {
   DevBuf b(v.size() * sizeof(T));
-  if (!v.empty()) HIP_CHECK(hipMemcpyAsync(b.p, v.data(), b.bytes, hipMemcpyHostToDevice, s));
+  if (!v.empty()) h2d(b.p, v.data(), b.bytes, s);
   return b;
 }
 
@@ -117,6 +156,7 @@ struct Ctx {
   DevBuf scratch_partial;                   // split-K partial sums
   DevBuf info;                              // rocsolver info
   std::map<int, DevBuf> syrk_tiles;         // n -> upper-tile table of the FP32 SYRK
+  double escalate_rms = 1e-4;               // BF16 -> FP32 switch of the escalation mode
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // optional per-phase kernel timing (HIP events on this stream), seconds
   bool timing = false;
@@ -156,6 +196,13 @@ struct Tile {
   int pad;
 };
 
+// Autotuned BF16 exchange pipeline: slab-GEMM shape, SYRK variant and the
+// timings (ms, -1 = not tried) they were chosen from.
+struct Bf16Plan {
+  int k1 = -1, k3 = -1;
+  std::vector<float> k1_ms, k3_ms;
+};
+
 struct DevBT {
   BLayout L;                 // host copy of the layout
   DevBuf data64;             // FP64 slabs (B or B~)
@@ -164,7 +211,7 @@ struct DevBT {
   DevBuf ldp, off16, ldp_pre;               // padded row length, slab offsets, prefix of ldp
   std::vector<int> h_ldp;
   std::vector<int64_t> h_off16, h_ldp_pre;
-  std::map<int, int> bf16_bn;               // ng -> tuned launch shape of the BF16 slab GEMM
+  std::map<int, Bf16Plan> bf16_plan;        // ng -> tuned BF16 exchange pipeline
   DevBuf slab_off, ncol, bsz, boff, colmap, colmap_off, pg_off;
   std::vector<int> h_colmap, h_colmap_off;
   std::vector<int64_t> h_pg_off;            // prefix of bsz[N]*ncol[N]

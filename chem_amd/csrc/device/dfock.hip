@@ -70,7 +70,7 @@ void dbt_upload(Ctx& c, const BLayout& L, const double* host_data, DevBT& t)
   t.L = L;
   t.data64.alloc((size_t)L.total * sizeof(double));
   if (L.total)
-    HIP_CHECK(hipMemcpyAsync(t.data64.p, host_data, t.data64.bytes, hipMemcpyHostToDevice, c.stream));
+    h2d(t.data64.p, host_data, t.data64.bytes, c.stream);
   dbt_index(c, L, t);
 }
 
@@ -117,10 +117,11 @@ static void dbt_index(Ctx& c, const BLayout& L, DevBT& t)
 // Column tile of the grouped slab GEMM for ng output columns.
 static int slab_bn(int ng) { return ng <= 64 ? 64 : 128; }
 
-// Tile table of the grouped slab GEMM for ng output columns.
-const Tile* slab_tiles(Ctx& c, DevBT& t, int ng, int* ntiles)
+// Tile table of the grouped slab GEMM for ng output columns (column tile BN,
+// 0 = by ng).
+const Tile* slab_tiles(Ctx& c, DevBT& t, int ng, int* ntiles, int BN)
 {
-  const int BN = slab_bn(ng);
+  if (BN == 0) BN = slab_bn(ng);
   auto key = std::make_pair(BN, ng);
   auto it = t.tiles.find(key);
   if (it == t.tiles.end()) {
@@ -140,12 +141,14 @@ const Tile* slab_tiles(Ctx& c, DevBT& t, int ng, int* ntiles)
 }
 
 template <class T>
-static void slab_gemm_t(Ctx& c, DevBT& t, const T* data, const T* G, int ldg, int ng, T* out, int ldo)
+static void slab_gemm_t(Ctx& c, DevBT& t, const T* data, const T* G, int ldg, int ng, T* out, int ldo,
+                        int bn = 0)
 {
+  if (bn == 0) bn = slab_bn(ng);
   int nt = 0;
-  const Tile* tiles = slab_tiles(c, t, ng, &nt);
+  const Tile* tiles = slab_tiles(c, t, ng, &nt, bn);
   if (nt == 0) return;
-  if (slab_bn(ng) == 64)
+  if (bn == 64)
     hipLaunchKernelGGL((k_slab_gemm<T, SG_BM, 64>), dim3(nt), dim3(256), 0, c.stream, data,
                        t.slab_off.as<int64_t>(), t.ncol.as<int>(), t.bsz.as<int>(), t.boff.as<int>(),
                        t.colmap.as<int>(), t.colmap_off.as<int>(), tiles, t.L.naux, t.L.nbf, G, ldg,
@@ -156,6 +159,13 @@ static void slab_gemm_t(Ctx& c, DevBT& t, const T* data, const T* G, int ldg, in
                        t.colmap.as<int>(), t.colmap_off.as<int>(), tiles, t.L.naux, t.L.nbf, G, ldg,
                        ng, out, ldo);
   HIP_CHECK(hipGetLastError());
+}
+
+void slab_gemm_f32(Ctx& c, DevBT& t, const float* data, const float* G, int ldg, int ng, float* out, int ldo,
+                   int bn)
+{
+  if (bn != 0 && bn != 64 && bn != 128) throw std::runtime_error("slab_gemm_f32: column tile must be 64 or 128");
+  slab_gemm_t<float>(c, t, data, G, ldg, ng, out, ldo, bn);
 }
 
 void slab_gemm(Ctx& c, DevBT& t, const double* G, int ldg, int ng, double* out, int ldo,
@@ -213,18 +223,26 @@ void metric_create(Ctx& c, int naux, const double* V_host, Metric& m)
   const size_t bytes = (size_t)naux * naux * sizeof(double);
   m.Linv.alloc(bytes);
   m.Vinv.alloc(bytes);
-  HIP_CHECK(hipMemcpyAsync(m.Linv.p, V_host, bytes, hipMemcpyHostToDevice, c.stream));
+  h2d(m.Linv.p, V_host, bytes, c.stream);
   double* A = m.Linv.as<double>();
   int* info = c.info.as<int>();
   RB_CHECK(rocsolver_dpotrf(c.blas, rocblas_fill_lower, naux, A, naux, info));
   int h_info = 0;
-  HIP_CHECK(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-  HIP_CHECK(hipStreamSynchronize(c.stream));
-  if (h_info != 0) throw std::runtime_error("metric Cholesky failed (matrix not positive definite)");
+  d2h(&h_info, info, sizeof(int), c.stream);
+  if (h_info != 0) {
+    // diagnose before failing: is the matrix on the device still the input?
+    h2d(m.Linv.p, V_host, bytes, c.stream);
+    std::vector<double> back((size_t)naux * naux);
+    d2h(back.data(), m.Linv.p, bytes, c.stream);
+    size_t ndiff = 0;
+    for (size_t i = 0; i < back.size(); ++i) ndiff += back[i] != V_host[i];
+    throw std::runtime_error("metric Cholesky failed (matrix not positive definite; pivot " +
+                             std::to_string(h_info) + ", re-uploaded matrix differs in " + std::to_string(ndiff) +
+                             " elements)");
+  }
   hipLaunchKernelGGL(k_zero_upper_cm, dim3(nblk((int64_t)naux * naux, 256)), dim3(256), 0, c.stream, A, naux);
   RB_CHECK(rocsolver_dtrtri(c.blas, rocblas_fill_lower, rocblas_diagonal_non_unit, naux, A, naux, info));
-  HIP_CHECK(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-  HIP_CHECK(hipStreamSynchronize(c.stream));
+  d2h(&h_info, info, sizeof(int), c.stream);
   if (h_info != 0) throw std::runtime_error("metric triangular inverse failed");
   hipLaunchKernelGGL(k_zero_upper_cm, dim3(nblk((int64_t)naux * naux, 256)), dim3(256), 0, c.stream, A, naux);
   // V^-1 = L^-T L^-1 (column-major: Linv^T * Linv)
@@ -461,7 +479,7 @@ void slab_out_gemm(Ctx& c, DevBT& t, const double* A, int lda, int klen, const d
   hipLaunchKernelGGL((k_slab_out_gemm<double, SG_BM, 128>), dim3((unsigned)v.size()), dim3(256), 0, c.stream,
                      A, lda, klen, t.slab_off.as<int64_t>(), t.ncol.as<int>(), t.bsz.as<int>(),
                      t.boff.as<int>(), t.colmap.as<int>(), t.colmap_off.as<int>(), tiles.as<Tile>(), L.naux,
-                     L.nbf, G, ldg, out);
+                     L.nbf, G, ldg, out, nullptr, nullptr, (int64_t)0);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(c.stream));
 }
@@ -529,7 +547,7 @@ static void bf16_layout(Ctx& c, DevBT& t)
   const int64_t tot = t.h_off16[L.nbb];
   t.data_hi.alloc((size_t)std::max<int64_t>(tot, 1) * sizeof(uint16_t));
   t.data_lo.alloc((size_t)std::max<int64_t>(tot, 1) * sizeof(uint16_t));
-  t.bf16_bn.clear();
+  t.bf16_plan.clear();
 }
 
 // Low-precision copies of B~ = L^-1 B made slab by slab from the untransformed
@@ -584,18 +602,18 @@ void dbt_make_bf16(Ctx& c, DevBT& t)
   t.lowp_tilde = true;
 }
 
-// Tile table of the BF16 slab GEMM (BM = 128, column tile BN), padded with
+// Tile table of the BF16 slab GEMM (row tile BM, column tile BN), padded with
 // empty tiles to a multiple of 8 for the XCD mapping.
-static const Tile* bf16_tiles(Ctx& c, DevBT& t, int BN, int ng, int* ntiles)
+static const Tile* bf16_tiles(Ctx& c, DevBT& t, int BM, int BN, int ng, int* ntiles)
 {
-  auto key = std::make_pair(-BN, ng);
+  auto key = std::make_pair(-(BN * 1000 + BM), ng);
   auto it = t.tiles.find(key);
   if (it == t.tiles.end()) {
     std::vector<Tile> v;
     // slabs without significant pairs still get tiles: they write the zero rows of W
     for (int n = 0; n < t.L.nbb; ++n) {
       const int64_t rows = (int64_t)t.L.naux * t.L.bsz[n];
-      for (int64_t m0 = 0; m0 < rows; m0 += SG_BM)
+      for (int64_t m0 = 0; m0 < rows; m0 += BM)
         for (int n0 = 0; n0 < ng; n0 += BN) v.push_back(Tile{n, (int)m0, n0, 0});
     }
     while (v.size() % 8) v.push_back(Tile{-1, 0, 0, 0});
@@ -609,29 +627,41 @@ static const Tile* bf16_tiles(Ctx& c, DevBT& t, int BN, int ng, int* ntiles)
 }
 
 // Launch shapes of the BF16 slab GEMM: waves (WM x WN), MFMA tiles per wave
-// (TM x TN); BM = 128 rows for all, BN = 16 * WN * TN columns.
+// (TM x TN), BM = 16 WM TM rows, BN = 16 WN TN columns.  DL = 0: register-
+// staged loads, NB LDS buffers, FP32 W; DL = 1: direct-to-LDS loads, NB
+// pipeline stages, W written as bf16 hi/lo.
 struct Bf16Shape {
-  int WM, WN, TM, TN, NBUF;
+  int WM, WN, TM, TN, NB, DL;
+  int bm() const { return 16 * WM * TM; }
   int bn() const { return 16 * WN * TN; }
   int threads() const { return 64 * WM * WN; }
 };
-static const Bf16Shape kBf16Shapes[] = {{2, 2, 4, 2, 2}, {2, 2, 4, 4, 2}, {2, 2, 4, 5, 2}, {2, 4, 4, 2, 2},
-                                        {2, 4, 4, 3, 2}, {2, 4, 4, 4, 2}, {2, 4, 4, 5, 2}, {2, 4, 4, 2, 1},
-                                        {2, 4, 4, 3, 1}};
+static const Bf16Shape kBf16Shapes[] = {
+    {2, 2, 4, 2, 2, 0}, {2, 2, 4, 4, 2, 0}, {2, 2, 4, 5, 2, 0}, {2, 4, 4, 2, 2, 0}, {2, 4, 4, 3, 2, 0},
+    {2, 4, 4, 4, 2, 0}, {2, 4, 4, 5, 2, 0}, {2, 4, 4, 2, 1, 0}, {2, 4, 4, 3, 1, 0}, {2, 4, 4, 2, 4, 1},
+    {2, 4, 4, 3, 3, 1}, {2, 4, 4, 5, 2, 1}, {1, 4, 4, 5, 3, 1}};
 static constexpr int kNumBf16Shapes = sizeof(kBf16Shapes) / sizeof(kBf16Shapes[0]);
+// SYRK variants: 0 = register-staged on FP32 W; 1..3 = direct-to-LDS on bf16
+// hi/lo W with 2, 3, 4 stages
+static constexpr int kNumSyrkVariants = 4;
 
 static void slab_gemm_bf16(Ctx& c, DevBT& t, int shape, const uint16_t* ghi, const uint16_t* glo, int ng,
-                           float* W, int ldw)
+                           float* W, uint16_t* Whi, uint16_t* Wlo, int ldw)
 {
   const Bf16Shape sh = kBf16Shapes[shape];
   int nt = 0;
-  const Tile* tiles = bf16_tiles(c, t, sh.bn(), ng, &nt);
+  const Tile* tiles = bf16_tiles(c, t, sh.bm(), sh.bn(), ng, &nt);
   if (nt == 0) return;
 #define MCHIP_BF16_SLAB(WM, WN, TM, TN, NB)                                                               \
   hipLaunchKernelGGL((k_slab_gemm_bf16x3<WM, WN, TM, TN, NB>), dim3(nt), dim3(64 * WM * WN), 0, c.stream, \
                      t.data_hi.as<uint16_t>(), t.data_lo.as<uint16_t>(), t.off16.as<int64_t>(),           \
                      t.ldp.as<int>(), t.ldp_pre.as<int64_t>(), t.bsz.as<int>(), t.boff.as<int>(), tiles,  \
                      t.L.naux, t.L.nbf, ghi, glo, ng, W, ldw)
+#define MCHIP_BF16_SLAB_DL(WM, WN, TM, TN, S)                                                             \
+  hipLaunchKernelGGL((k_slab_gemm_bf16x3_dl<WM, WN, TM, TN, S>), dim3(nt), dim3(64 * WM * WN), 0,         \
+                     c.stream, t.data_hi.as<uint16_t>(), t.data_lo.as<uint16_t>(), t.off16.as<int64_t>(), \
+                     t.ldp.as<int>(), t.ldp_pre.as<int64_t>(), t.bsz.as<int>(), t.boff.as<int>(), tiles,  \
+                     t.L.naux, t.L.nbf, ghi, glo, ng, nullptr, Whi, Wlo, ldw)
   switch (shape) {
     case 0: MCHIP_BF16_SLAB(2, 2, 4, 2, 2); break;
     case 1: MCHIP_BF16_SLAB(2, 2, 4, 4, 2); break;
@@ -642,45 +672,153 @@ static void slab_gemm_bf16(Ctx& c, DevBT& t, int shape, const uint16_t* ghi, con
     case 6: MCHIP_BF16_SLAB(2, 4, 4, 5, 2); break;
     case 7: MCHIP_BF16_SLAB(2, 4, 4, 2, 1); break;
     case 8: MCHIP_BF16_SLAB(2, 4, 4, 3, 1); break;
+    case 9: MCHIP_BF16_SLAB_DL(2, 4, 4, 2, 4); break;
+    case 10: MCHIP_BF16_SLAB_DL(2, 4, 4, 3, 3); break;
+    case 11: MCHIP_BF16_SLAB_DL(2, 4, 4, 5, 2); break;
+    case 12: MCHIP_BF16_SLAB_DL(1, 4, 4, 5, 3); break;
     default: throw std::runtime_error("BF16 slab GEMM: unknown launch shape");
   }
 #undef MCHIP_BF16_SLAB
+#undef MCHIP_BF16_SLAB_DL
   HIP_CHECK(hipGetLastError());
 }
 
-// Launch shape for ng occupied columns: every candidate whose column tile is
-// not mostly empty is timed once on the live operands (after one warm launch)
-// and the fastest is kept per ng.
-static int tune_bf16_shape(Ctx& c, DevBT& t, const uint16_t* ghi, const uint16_t* glo, int ng, float* W,
-                           int ldw)
+static void seg_syrk_bf16(Ctx& c, int variant, const float* W, const uint16_t* Whi, const uint16_t* Wlo, int naux,
+                          int nbf, int ldw, const int2* tl, int ntile, int nch, int spc, double* partial)
 {
-  auto it = t.bf16_bn.find(ng);
-  if (it != t.bf16_bn.end()) return it->second;
-  int best = 0;
-  float best_ms = 1e30f;
+  const dim3 grid((unsigned)(ntile * nch));
+  switch (variant) {
+    case 0:
+      hipLaunchKernelGGL((k_seg_syrk_bf16x3<SG_BM>), grid, dim3(256), 0, c.stream, W, naux, (int64_t)nbf * ldw, ldw,
+                         nbf, tl, ntile, spc, partial);
+      break;
+    case 1:
+      hipLaunchKernelGGL(k_seg_syrk_bf16x3_dl<2>, grid, dim3(256), 0, c.stream, Whi, Wlo, naux, (int64_t)nbf * ldw,
+                         ldw, nbf, tl, ntile, spc, partial);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_seg_syrk_bf16x3_dl<3>, grid, dim3(256), 0, c.stream, Whi, Wlo, naux, (int64_t)nbf * ldw,
+                         ldw, nbf, tl, ntile, spc, partial);
+      break;
+    case 3:
+      hipLaunchKernelGGL(k_seg_syrk_bf16x3_dl<4>, grid, dim3(256), 0, c.stream, Whi, Wlo, naux, (int64_t)nbf * ldw,
+                         ldw, nbf, tl, ntile, spc, partial);
+      break;
+    default: throw std::runtime_error("BF16 SYRK: unknown variant");
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+// Autotuner of the BF16 exchange for ng occupied columns.  Every slab-GEMM
+// shape whose column tile is not mostly empty and every SYRK variant is timed
+// once on the live operands (after one warm launch); the pipeline (register-
+// staged with FP32 W, or direct-to-LDS with bf16 hi/lo W) with the smallest
+// sum of its best GEMM and best SYRK is kept per ng, with the timings.
+template <class F>
+static float time_launch(Ctx& c, F&& f)
+{
+  f();
+  HIP_CHECK(hipEventRecord(c.ev0, c.stream));
+  f();
+  HIP_CHECK(hipEventRecord(c.ev1, c.stream));
+  HIP_CHECK(hipEventSynchronize(c.ev1));
+  float ms = 0.f;
+  HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+  return ms;
+}
+
+static const Bf16Plan& tune_bf16(Ctx& c, DevBT& t, const uint16_t* ghi, const uint16_t* glo, int ng, float* W,
+                                 uint16_t* Whi, uint16_t* Wlo, int ldw, const int2* tl, int ntile, int nch, int spc,
+                                 double* partial)
+{
+  auto it = t.bf16_plan.find(ng);
+  if (it != t.bf16_plan.end()) return it->second;
+  Bf16Plan p;
+  p.k1_ms.assign(kNumBf16Shapes, -1.f);
+  p.k3_ms.assign(kNumSyrkVariants, -1.f);
+  float best1[2] = {1e30f, 1e30f};
+  int arg1[2] = {-1, -1};
   for (int v = 0; v < kNumBf16Shapes; ++v) {
     const int bn = kBf16Shapes[v].bn();
-    if (v > 0 && bn > 64 && (ng + bn - 1) / bn * bn - ng >= bn / 2) continue;   // mostly empty tiles
-    slab_gemm_bf16(c, t, v, ghi, glo, ng, W, ldw);
-    HIP_CHECK(hipEventRecord(c.ev0, c.stream));
-    slab_gemm_bf16(c, t, v, ghi, glo, ng, W, ldw);
-    HIP_CHECK(hipEventRecord(c.ev1, c.stream));
-    HIP_CHECK(hipEventSynchronize(c.ev1));
-    float ms = 0.f;
-    HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = v;
+    if (bn > 64 && (ng + bn - 1) / bn * bn - ng >= bn / 2) continue;   // mostly empty tiles
+    const float ms = time_launch(c, [&] { slab_gemm_bf16(c, t, v, ghi, glo, ng, W, Whi, Wlo, ldw); });
+    p.k1_ms[v] = ms;
+    const int dl = kBf16Shapes[v].DL;
+    if (ms < best1[dl]) {
+      best1[dl] = ms;
+      arg1[dl] = v;
     }
   }
-  t.bf16_bn[ng] = best;
-  return best;
+  float best3[2] = {1e30f, 1e30f};
+  int arg3[2] = {-1, -1};
+  for (int v = 0; v < kNumSyrkVariants; ++v) {
+    const float ms =
+        time_launch(c, [&] { seg_syrk_bf16(c, v, W, Whi, Wlo, t.L.naux, t.L.nbf, ldw, tl, ntile, nch, spc, partial); });
+    p.k3_ms[v] = ms;
+    const int dl = v > 0 ? 1 : 0;
+    if (ms < best3[dl]) {
+      best3[dl] = ms;
+      arg3[dl] = v;
+    }
+  }
+  const int fam = (arg1[1] >= 0 && best1[1] + best3[1] < best1[0] + best3[0]) || arg1[0] < 0 ? 1 : 0;
+  p.k1 = arg1[fam];
+  p.k3 = arg3[fam];
+  if (p.k1 < 0 || p.k3 < 0) throw std::runtime_error("BF16 autotuner found no usable launch shape");
+  return t.bf16_plan[ng] = p;
 }
+
+void dfk_bf16_plan(const DevBT& t, int ng, int* plan, float* ms, int* nms)
+{
+  auto it = t.bf16_plan.find(ng);
+  for (int i = 0; i < 8; ++i) plan[i] = 0;
+  if (nms) *nms = 0;
+  if (it == t.bf16_plan.end()) return;
+  const Bf16Plan& p = it->second;
+  const Bf16Shape& sh = kBf16Shapes[p.k1];
+  plan[0] = p.k1;
+  plan[1] = sh.bm();
+  plan[2] = sh.bn();
+  plan[3] = sh.threads();
+  plan[4] = sh.DL;
+  plan[5] = sh.NB;
+  plan[6] = p.k3;
+  plan[7] = 1;
+  if (ms && nms) {
+    int k = 0;
+    for (float v : p.k1_ms) ms[k++] = v;
+    for (float v : p.k3_ms) ms[k++] = v;
+    *nms = k;
+  }
+}
+
+void dfk_bf16_force(DevBT& t, int ng, int k1, int k3)
+{
+  if (k1 < 0 || k1 >= kNumBf16Shapes || k3 < 0 || k3 >= kNumSyrkVariants)
+    throw std::runtime_error("BF16 plan: shape or SYRK variant out of range");
+  if ((kBf16Shapes[k1].DL != 0) != (k3 != 0))
+    throw std::runtime_error("BF16 plan: direct-to-LDS GEMM shapes pair with SYRK variants 1-3, the others with 0");
+  Bf16Plan p;
+  p.k1 = k1;
+  p.k3 = k3;
+  p.k1_ms.assign(kNumBf16Shapes, -1.f);
+  p.k3_ms.assign(kNumSyrkVariants, -1.f);
+  t.bf16_plan[ng] = p;
+}
+
+int dfk_bf16_num_shapes(int* nsyrk)
+{
+  if (nsyrk) *nsyrk = kNumSyrkVariants;
+  return kNumBf16Shapes;
+}
+
+int dfk_bf16_shape_dl(int k1) { return kBf16Shapes[k1].DL; }
 
 int dfk_bf16_tuned_bn(const DevBT& t, int ng)
 {
-  auto it = t.bf16_bn.find(ng);
-  return it == t.bf16_bn.end() ? 0 : kBf16Shapes[it->second].bn() * 10000 + kBf16Shapes[it->second].threads();
+  auto it = t.bf16_plan.find(ng);
+  return it == t.bf16_plan.end() ? 0
+                                 : kBf16Shapes[it->second.k1].bn() * 10000 + kBf16Shapes[it->second.k1].threads();
 }
 
 // K = -sum_X W_X^T W_X with W = B~ C, both contractions BF16x3 on MFMA, FP32
@@ -703,10 +841,13 @@ void dfk_mo_bf16(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K
   const size_t nW = (size_t)naux * nbf * ldw;
   const size_t nG = (size_t)t.h_ldp_pre[L.nbb] * per;
   c.scratch_a.ensure(nW * sizeof(float) + 2 * nG * sizeof(uint16_t) + 512);
+  // W as FP32 or, in the same bytes, as bf16 hi and lo halves
   float* W = c.scratch_a.as<float>();
+  uint16_t* Whi = c.scratch_a.as<uint16_t>();
+  uint16_t* Wlo = Whi + nW;
   uint16_t* ghi = reinterpret_cast<uint16_t*>(W + nW);
   uint16_t* glo = ghi + (nG + 127) / 128 * 128;
-  // upper 128 x 128 tiles of K (table cached with the tensor); chunks of at most 256 aux functions
+  // upper 128 x 128 tiles of K (table cached with the tensor)
   const int nt = (nbf + SG_BM - 1) / SG_BM;
   const int ntile = nt * (nt + 1) / 2;
   const auto tkey = std::make_pair(0, -nbf);
@@ -738,16 +879,14 @@ void dfk_mo_bf16(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K
                            t.colmap.as<int>(), t.colmap_off.as<int>(), ghi, glo);
       HIP_CHECK(hipGetLastError());
     }
-    const int shape = tune_bf16_shape(c, t, ghi, glo, w, W, ldw);
+    double* partial = c.scratch_partial.as<double>() + (size_t)b * nch * nbf * nbf;
+    const Bf16Plan& plan = tune_bf16(c, t, ghi, glo, w, W, Whi, Wlo, ldw, d_tl, ntile, nch, spc, partial);
     {
       PhaseTimer tm(c, PH_K1);
-      slab_gemm_bf16(c, t, shape, ghi, glo, w, W, ldw);
+      slab_gemm_bf16(c, t, plan.k1, ghi, glo, w, W, Whi, Wlo, ldw);
     }
     PhaseTimer tm(c, PH_K3);
-    double* partial = c.scratch_partial.as<double>() + (size_t)b * nch * nbf * nbf;
-    hipLaunchKernelGGL((k_seg_syrk_bf16x3<SG_BM>), dim3(ntile * nch), dim3(256), 0, c.stream, W, naux,
-                       (int64_t)nbf * ldw, ldw, nbf, d_tl, ntile, spc, partial);
-    HIP_CHECK(hipGetLastError());
+    seg_syrk_bf16(c, plan.k3, W, Whi, Wlo, naux, nbf, ldw, d_tl, ntile, nch, spc, partial);
     if (b == nb - 1)
       hipLaunchKernelGGL(k_reduce_sym, dim3(nblk((int64_t)nbf * nbf, 256)), dim3(256), 0, c.stream,
                          c.scratch_partial.as<double>(), nb * nch, nbf, 0, -1.0, K);
@@ -774,7 +913,7 @@ void dfk_ao(Ctx& c, DevBT& t, const Metric* m, const double* P_dev, double* K, i
   std::vector<double> hL = pivoted_cholesky_dev(c, U, n, 0.0, &rank);
   DevBuf L((size_t)n * std::max(rank, 1) * sizeof(double));
   if (rank > 0)
-    HIP_CHECK(hipMemcpyAsync(L.p, hL.data(), hL.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    h2d(L.p, hL.data(), hL.size() * sizeof(double), c.stream);
   switch (form) {
     case 0: dfk_mo(c, t, t.transformed ? nullptr : m, L.as<double>(), rank, rank, K, occ_nbatches); break;
     case 1: dfk_mo_f32(c, t, L.as<double>(), rank, rank, K, occ_nbatches); break;

@@ -10,6 +10,10 @@ void ctx_free(Ctx& c);
 void dbt_upload(Ctx& c, const BLayout& L, const double* host_data, DevBT& t);
 void metric_create(Ctx& c, int naux, const double* V_host, Metric& m);
 void transform_B(Ctx& c, DevBT& t, const Metric& m);
+// FP32 slab GEMM on an FP32 copy of the slabs (data), column tile bn (64 or
+// 128; 0 picks by ng).
+void slab_gemm_f32(Ctx& c, DevBT& t, const float* data, const float* G, int ldg, int ng, float* out, int ldo,
+                   int bn);
 void slab_gemm(Ctx& c, DevBT& t, const double* G, int ldg, int ng, double* out, int ldo,
                double alpha);
 void seg_syrk(Ctx& c, const double* A, const double* B, int nseg, int64_t sstride, int lda,
@@ -28,8 +32,12 @@ void dbt_make_bf16(Ctx& c, DevBT& t);
 void dbt_make_lowp(Ctx& c, DevBT& t, const Metric& m, bool f32, bool bf16);
 void dfk_mo_bf16(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K, int occ_nbatches);
 int dfk_bf16_tuned_bn(const DevBT& t, int ng);
+void dfk_bf16_plan(const DevBT& t, int ng, int* plan, float* ms, int* nms);
+void dfk_bf16_force(DevBT& t, int ng, int k1, int k3);
+int dfk_bf16_num_shapes(int* nsyrk);
+int dfk_bf16_shape_dl(int k1);
 void dfk_ao(Ctx& c, DevBT& t, const Metric* m, const double* P_dev, double* K, int occ_nbatches, int form);
-const Tile* slab_tiles(Ctx& c, DevBT& t, int ng, int* ntiles);
+const Tile* slab_tiles(Ctx& c, DevBT& t, int ng, int* ntiles, int BN = 0);
 // out (slab layout of B) = rows (X, nu in N) of A (lda, klen) times gathered rows of G
 void slab_out_gemm(Ctx& c, DevBT& t, const double* A, int lda, int klen, const double* G, int ldg,
                    double* out);

@@ -332,7 +332,7 @@ void build_B_device(Ctx& c, const Basis& ao, const Basis& aux, const std::vector
     hipLaunchKernelGGL(k_pair_block_norm2, dim3(ncp, nxb), dim3(256), 0, c.stream, tmp.as<double>(),
                        d_cpoff.as<int64_t>(), d_cpsize.as<int>(), d_xoff.as<int>(), nxb, norm2.as<double>());
   std::vector<double> hn((size_t)ncp * nxb);
-  if (ncp) HIP_CHECK(hipMemcpyAsync(hn.data(), norm2.p, hn.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  if (ncp) d2h(hn.data(), norm2.p, hn.size() * sizeof(double), c.stream);
   HIP_CHECK(hipStreamSynchronize(c.stream));
   std::vector<uint8_t> fmask((size_t)nxb * nbb * nbb, 0), keep((size_t)std::max(1, ncp) * nxb, 0);
   for (int k = 0; k < ncp; ++k)

@@ -23,7 +23,14 @@ template <class T, int BM, int BN>
 __global__ void k_slab_out_gemm(const T* A, int lda, int klen, const int64_t* slab_off, const int* ncol,
                                 const int* bsz, const int* boff, const int* colmap,
                                 const int* colmap_off, const Tile* tiles, int naux, int nbf,
-                                const T* G, int ldg, T* out);
+                                const T* G, int ldg, T* out, const int* pcol, const int* pwidth,
+                                int64_t out_stride);
+template <int BM, int BN>
+__global__ void k_group_apply_f32(const float* P, const float* in, float* out, int n, int R, const int* pcol,
+                                  const int* pwidth, const int4* tiles);
+__global__ void k_slabseg_f32(const float* A, int64_t a_stride, const float* B, const int64_t* seg_off,
+                              const int64_t* seg_len, const int* chunk_seg, int nchunk, int n,
+                              const int2* tiles, int ntile, double* partial);
 template <class T, int BM, int BN>
 __global__ void k_slabseg_gemm(const T* A, const T* B, const int64_t* seg_off, const int64_t* seg_len,
                                const int* chunk_seg, int n, const int2* tiles, int ntile,
@@ -55,6 +62,16 @@ __global__ void k_rows_to_f32(const double* in, int ld64, int rows, int ncols, f
 __global__ void k_split_slabs(const double* data64, const int64_t* slab_off, const int* ncol,
                               const int* bsz, int naux, const int* ldp, const int64_t* off16,
                               uint16_t* hi, uint16_t* lo);
+template <int WM, int WN, int TM, int TN, int S>
+__global__ void k_slab_gemm_bf16x3_dl(const uint16_t* bhi, const uint16_t* blo, const int64_t* off16,
+                                      const int* ldp, const int64_t* ldp_pre, const int* bsz,
+                                      const int* boff, const Tile* tiles, int naux, int nbf,
+                                      const uint16_t* ghi, const uint16_t* glo, int ng, float* out,
+                                      uint16_t* whi, uint16_t* wlo, int ldo);
+template <int S>
+__global__ void k_seg_syrk_bf16x3_dl(const uint16_t* whi, const uint16_t* wlo, int nseg, int64_t sstride,
+                                     int ldw, int n, const int2* tiles, int ntile, int segs_per_chunk,
+                                     double* partial);
 __global__ void k_split_slab(const double* src, int N, const int* ncol, const int* bsz, int naux,
                              const int* ldp, const int64_t* off16, uint16_t* hi, uint16_t* lo);
 __global__ void k_gather_ct(const double* C, int ldc, int ng, const int* ncol, const int* ldp,

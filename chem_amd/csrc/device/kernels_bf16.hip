@@ -370,6 +370,273 @@ __global__ __launch_bounds__(256) void k_seg_syrk_bf16x3(
 template __global__ void k_seg_syrk_bf16x3<128>(const float*, int, int64_t, int, int, const int2*, int,
                                                  int, double*);
 
+// ---------------------------------------------------------------------------
+// Direct-to-LDS pipelined variants (global_load_lds_dwordx4, gfx950).
+//
+// Operand tiles go from HBM straight into LDS, S stages deep, with no
+// register staging: a wave-wide load writes 64 x 16 bytes contiguously (lane
+// i at +16 i), i.e. 16 rows of 32 bf16.  Lane i of the load for rows
+// [16q, 16q + 16) fetches row r = 16q + i/4, 16-byte chunk c = (i%4) ^ f(r),
+// f(r) = (r/4)%4, so that a row's chunk c sits at position c ^ f(r): the 16
+// rows an MFMA fragment read touches (one chunk each) then hit 16 distinct
+// bank groups.  Rows past the operand's end are clamped onto its last row
+// (finite data whose products are never stored, or stored as zero).
+// Every wave issues the same number of loads per stage, so a stage is
+// complete once vmcnt drops to (stages still in flight) x (loads per wave).
+// ---------------------------------------------------------------------------
+
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+
+__device__ __forceinline__ void lds_load16(const uint4* g, uint4* l)
+{
+  __builtin_amdgcn_global_load_lds((const void*)g, (lds_ptr_t)l, 16, 0, 0);
+}
+
+// s_waitcnt vmcnt(n) with the other counters left at their maximum (gfx9 encoding)
+template <int N>
+__device__ __forceinline__ void wait_vm()
+{
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
+// Wait until at most `after` stages' loads (per wave: LPW each) are in flight.
+template <int LPW, int S>
+__device__ __forceinline__ void wait_stages(int after)
+{
+  if constexpr (S >= 4) {
+    if (after >= 3) { wait_vm<3 * LPW>(); return; }
+  }
+  if constexpr (S >= 3) {
+    if (after >= 2) { wait_vm<2 * LPW>(); return; }
+  }
+  if (after >= 1) { wait_vm<LPW>(); return; }
+  wait_vm<0>();
+}
+
+__device__ __forceinline__ int swz(int r, int c) { return c ^ ((r >> 2) & 3); }
+
+// One BK = 32 step on swizzled stage tiles (uint4 units, 4 per row).
+template <int TM, int TN>
+__device__ __forceinline__ void bf16x3_step_swz(const uint4* Ah, const uint4* Al, const uint4* Bh,
+                                                const uint4* Bl, int wm0, int wn0, int lane,
+                                                f32x4 (&acc)[TM][TN])
+{
+  const int fr = lane & 15, fk = lane >> 4;
+  uint4 ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+    const int r = wm0 + 16 * i + fr;
+    ah[i] = Ah[r * 4 + swz(r, fk)];
+    al[i] = Al[r * 4 + swz(r, fk)];
+  }
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int r = wn0 + 16 * j + fr;
+    bh[j] = Bh[r * 4 + swz(r, fk)];
+    bl[j] = Bl[r * 4 + swz(r, fk)];
+  }
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      acc[i][j] = mfma_bf16(al[i], bh[j], acc[i][j]);
+      acc[i][j] = mfma_bf16(ah[i], bl[j], acc[i][j]);
+      acc[i][j] = mfma_bf16(ah[i], bh[j], acc[i][j]);
+    }
+}
+
+// Issue the loads of one stage: the wave-loads j = wave + NW*u (u < LPW) over
+// the list [A hi rows/16][A lo][B hi][B lo].
+template <int BM, int BN, int NW>
+__device__ __forceinline__ void issue_stage(uint4* st, int wave, int lane, const uint4* Ah, const uint4* Al,
+                                            const uint4* Bh, const uint4* Bl, int64_t a0, int64_t alast,
+                                            int b0, int blast, int lda8, int ldb8, int kc0)
+{
+  constexpr int QA = BM / 16, QB = BN / 16, NJ = 2 * QA + 2 * QB;
+  static_assert(NJ % NW == 0, "stage loads must split evenly over the waves");
+  const int rr = lane >> 2, pos = lane & 3;
+#pragma unroll
+  for (int u = 0; u < NJ / NW; ++u) {
+    const int j = wave + NW * u;
+    if (j < 2 * QA) {
+      const bool lo = j >= QA;
+      const int q = lo ? j - QA : j;
+      const int r = 16 * q + rr;
+      const int64_t gr = a0 + r < alast ? a0 + r : alast;
+      const uint4* src = (lo ? Al : Ah) + gr * lda8 + kc0 + swz(r, pos);
+      lds_load16(src, st + (lo ? BM * 4 : 0) + q * 64);
+    }
+    else {
+      const int jb = j - 2 * QA;
+      const bool lo = jb >= QB;
+      const int q = lo ? jb - QB : jb;
+      const int r = 16 * q + rr;
+      const int64_t gr = b0 + r < blast ? b0 + r : blast;
+      const uint4* src = (lo ? Bl : Bh) + gr * ldb8 + kc0 + swz(r, pos);
+      lds_load16(src, st + 8 * BM + (lo ? BN * 4 : 0) + q * 64);
+    }
+  }
+}
+
+// Grouped slab GEMM, direct-to-LDS pipelined.  Same operands and tile table
+// as k_slab_gemm_bf16x3 (m0 steps of BM).  W columns in [ng, ldo) are
+// written as zero.  split_out: W is written as bf16 hi/lo (whi, wlo, same
+// layout) instead of FP32 (out).
+template <int WM, int WN, int TM, int TN, int S>
+__global__ __launch_bounds__(64 * WM * WN) void k_slab_gemm_bf16x3_dl(
+    const uint16_t* __restrict__ bhi, const uint16_t* __restrict__ blo, const int64_t* __restrict__ off16,
+    const int* __restrict__ ldp, const int64_t* __restrict__ ldp_pre, const int* __restrict__ bsz,
+    const int* __restrict__ boff, const Tile* __restrict__ tiles, int naux, int nbf,
+    const uint16_t* __restrict__ ghi, const uint16_t* __restrict__ glo, int ng, float* __restrict__ out,
+    uint16_t* __restrict__ whi, uint16_t* __restrict__ wlo, int ldo)
+{
+  constexpr int NW = WM * WN;
+  constexpr int BM = 16 * WM * TM, BN = 16 * WN * TN;
+  constexpr int LPW = (2 * BM / 16 + 2 * BN / 16) / NW;
+  constexpr int STAGE = (BM + BN) * 8;   // uint4 per stage
+  __shared__ uint4 lds[S * STAGE];
+  const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  if (t.slab < 0) return;
+  const int N = t.slab;
+  const int bn = bsz[N];
+  const int ld8 = ldp[N] >> 3;
+  const int64_t rows = (int64_t)naux * bn;
+  const uint4* Ah = reinterpret_cast<const uint4*>(bhi + off16[N]);
+  const uint4* Al = reinterpret_cast<const uint4*>(blo + off16[N]);
+  const uint4* Gh = reinterpret_cast<const uint4*>(ghi + ldp_pre[N] * ng);
+  const uint4* Gl = reinterpret_cast<const uint4*>(glo + ldp_pre[N] * ng);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave / WN) * (16 * TM), wn0 = (wave % WN) * (16 * TN);
+  f32x4 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int nsteps = ld8 >> 2;
+  const int64_t alast = rows - 1;
+  const int blast = ng - 1;
+#pragma unroll
+  for (int st = 0; st < S - 1; ++st)
+    if (st < nsteps)
+      issue_stage<BM, BN, NW>(lds + st * STAGE, wave, lane, Ah, Al, Gh, Gl, t.m0, alast, t.n0, blast, ld8, ld8,
+                              st * 4);
+  for (int s = 0; s < nsteps; ++s) {
+    __syncthreads();   // every wave is done with the buffer the next issue overwrites
+    if (s + S - 1 < nsteps)
+      issue_stage<BM, BN, NW>(lds + ((s + S - 1) % S) * STAGE, wave, lane, Ah, Al, Gh, Gl, t.m0, alast, t.n0,
+                              blast, ld8, ld8, (s + S - 1) * 4);
+    wait_stages<LPW, S>(min(S - 1, nsteps - 1 - s));
+    __syncthreads();   // stage s has landed for every wave
+    const uint4* st = lds + (s % S) * STAGE;
+    bf16x3_step_swz<TM, TN>(st, st + BM * 4, st + BM * 8, st + BM * 8 + BN * 4, wm0, wn0, lane, acc);
+  }
+  const int fc = lane & 15, fr4 = 4 * (lane >> 4);
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t r = (int64_t)t.m0 + wm0 + 16 * i + fr4 + e;
+      if (r >= rows) continue;
+      const int64_t x = r / bn;
+      const int nu = (int)(r - x * bn);
+      const int64_t orow = (x * nbf + boff[N] + nu) * (int64_t)ldo;
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int c = t.n0 + wn0 + 16 * j + fc;
+        if (c >= ldo) continue;
+        const float v = c < ng ? acc[i][j][e] : 0.f;
+        if (whi) {
+          uint16_t h, l;
+          split32(v, h, l);
+          whi[orow + c] = h;
+          wlo[orow + c] = l;
+        }
+        else {
+          out[orow + c] = v;
+        }
+      }
+    }
+}
+
+// Segmented SYRK over bf16 hi/lo W, direct-to-LDS pipelined: partial[chunk](m,n)
+// = sum_{X in chunk} sum_i W(X,m,i) W(X,n,i) on the upper 128 x 128 tiles
+// (4 waves, 64 x 64 each).  Rows of W have ldw (multiple of 32) bf16.
+template <int S>
+__global__ __launch_bounds__(256) void k_seg_syrk_bf16x3_dl(
+    const uint16_t* __restrict__ whi, const uint16_t* __restrict__ wlo, int nseg, int64_t sstride, int ldw,
+    int n, const int2* __restrict__ tiles, int ntile, int segs_per_chunk, double* __restrict__ partial)
+{
+  constexpr int BM = 128, TM = 4, NW = 4;
+  constexpr int LPW = (4 * BM / 16) / NW;
+  constexpr int STAGE = 2 * BM * 8;
+  __shared__ uint4 lds[S * STAGE];
+  const int2 tl = tiles[blockIdx.x % ntile];
+  const int chunk = blockIdx.x / ntile;
+  const int s0 = chunk * segs_per_chunk;
+  const int s1 = min(nseg, s0 + segs_per_chunk);
+  const int m0 = tl.x * BM, n0 = tl.y * BM;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave >> 1) * 64, wn0 = (wave & 1) * 64;
+  f32x4 acc[TM][TM];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TM; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int ksteps = ldw >> 5;
+  const int nsteps = (s1 > s0) ? (s1 - s0) * ksteps : 0;
+  const int ld8 = ldw >> 3;
+  const uint4* Wh = reinterpret_cast<const uint4*>(whi);
+  const uint4* Wl = reinterpret_cast<const uint4*>(wlo);
+  auto issue = [&](int q, int buf) {
+    const int sg = s0 + q / ksteps;
+    const int kc0 = (q % ksteps) * 4;
+    const int64_t base = (int64_t)sg * (sstride >> 3);
+    issue_stage<BM, BM, NW>(lds + buf * STAGE, wave, lane, Wh + base, Wl + base, Wh + base, Wl + base, m0,
+                            n - 1, n0, n - 1, ld8, ld8, kc0);
+  };
+#pragma unroll
+  for (int st = 0; st < S - 1; ++st)
+    if (st < nsteps) issue(st, st);
+  for (int q = 0; q < nsteps; ++q) {
+    __syncthreads();
+    if (q + S - 1 < nsteps) issue(q + S - 1, (q + S - 1) % S);
+    wait_stages<LPW, S>(min(S - 1, nsteps - 1 - q));
+    __syncthreads();
+    const uint4* st = lds + (q % S) * STAGE;
+    bf16x3_step_swz<TM, TM>(st, st + BM * 4, st + BM * 8, st + BM * 12, wm0, wn0, lane, acc);
+  }
+  double* P = partial + (int64_t)chunk * n * n;
+  const int fc = lane & 15, fr4 = 4 * (lane >> 4);
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TM; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int r = m0 + wm0 + 16 * i + fr4 + e;
+        const int c = n0 + wn0 + 16 * j + fc;
+        if (r < n && c < n) P[(int64_t)r * n + c] = (double)acc[i][j][e];
+      }
+}
+
+#define MCHIP_SLAB16DL_INST(WM, WN, TM, TN, S)                                                         \
+  template __global__ void k_slab_gemm_bf16x3_dl<WM, WN, TM, TN, S>(                                  \
+      const uint16_t*, const uint16_t*, const int64_t*, const int*, const int64_t*, const int*,         \
+      const int*, const Tile*, int, int, const uint16_t*, const uint16_t*, int, float*, uint16_t*,      \
+      uint16_t*, int);
+MCHIP_SLAB16DL_INST(2, 4, 4, 2, 4)
+MCHIP_SLAB16DL_INST(2, 4, 4, 3, 3)
+MCHIP_SLAB16DL_INST(2, 4, 4, 5, 2)
+MCHIP_SLAB16DL_INST(1, 4, 4, 5, 3)
+#undef MCHIP_SLAB16DL_INST
+template __global__ void k_seg_syrk_bf16x3_dl<2>(const uint16_t*, const uint16_t*, int, int64_t, int, int,
+                                                 const int2*, int, int, double*);
+template __global__ void k_seg_syrk_bf16x3_dl<3>(const uint16_t*, const uint16_t*, int, int64_t, int, int,
+                                                 const int2*, int, int, double*);
+template __global__ void k_seg_syrk_bf16x3_dl<4>(const uint16_t*, const uint16_t*, int, int64_t, int, int,
+                                                 const int2*, int, int, double*);
+
 #define MCHIP_SLAB16_INST(WM, WN, TM, TN, NB)                                                          \
   template __global__ void k_slab_gemm_bf16x3<WM, WN, TM, TN, NB>(                                     \
       const uint16_t*, const uint16_t*, const int64_t*, const int*, const int64_t*, const int*,         \

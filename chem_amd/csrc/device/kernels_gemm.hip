@@ -279,8 +279,19 @@ __global__ __launch_bounds__(256) void k_slab_out_gemm(
     const T* __restrict__ A, int lda, int klen, const int64_t* __restrict__ slab_off,
     const int* __restrict__ ncol, const int* __restrict__ bsz, const int* __restrict__ boff,
     const int* __restrict__ colmap, const int* __restrict__ colmap_off, const Tile* __restrict__ tiles,
-    int naux, int nbf, const T* __restrict__ G, int ldg, T* __restrict__ out)
+    int naux, int nbf, const T* __restrict__ G, int ldg, T* __restrict__ out, const int* __restrict__ pcol,
+    const int* __restrict__ pwidth, int64_t out_stride)
 {
+  // grouped use (pcol != null): tile.pad is the group; its contraction columns
+  // are [pcol, pcol + pwidth) of A and G and its output block starts at
+  // out + group * out_stride
+  if (pcol) {
+    const int g = tiles[blockIdx.x].pad;
+    A += pcol[g];
+    G += pcol[g];
+    klen = pwidth[g];
+    out += (int64_t)g * out_stride;
+  }
   constexpr int TM = BM / 32, TN = BN / 32;
   constexpr int LDA = BM + 16, LDB = BN + 16;
   constexpr int RA = BM / 16, RB = BN / 16;
@@ -452,9 +463,201 @@ __global__ __launch_bounds__(256) void k_slabseg_gemm(
       }
 }
 
+// Grouped dense product, one group per (point g, aux function X):
+//   out[X][m][pcol[g] + j] = sum_v P[g][m][v] in[X][v][pcol[g] + j],  j < pwidth[g]
+// P: n x n row-major per group, in/out: [naux][n][R] row-major.  The virtual
+// pseudo-density applied to b for every X and Laplace point (z_full.cpp:224-240).
+template <int BM, int BN>
+__global__ __launch_bounds__(256) void k_group_apply_f32(const float* __restrict__ P, const float* __restrict__ in,
+                                                        float* __restrict__ out, int n, int R,
+                                                        const int* __restrict__ pcol,
+                                                        const int* __restrict__ pwidth,
+                                                        const int4* __restrict__ tiles)
+{
+  constexpr int TM = BM / 32, TN = BN / 32;
+  constexpr int LDA = BM + 16, LDB = BN + 16;
+  constexpr int RA = BM / 16, RB = BN / 16;
+  __shared__ float As[2][GM_BK * LDA];
+  __shared__ float Bs[2][GM_BK * LDB];
+  const int4 t = tiles[blockIdx.x];   // (group, X, m0, n0)
+  const int g = t.x;
+  const float* A = P + (int64_t)g * n * n;
+  const float* Bg = in + (int64_t)t.y * n * R + pcol[g];
+  float* Cg = out + (int64_t)t.y * n * R + pcol[g];
+  const int w = pwidth[g];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave >> 1) * (BM / 2), wn0 = (wave & 1) * (BN / 2);
+  Mfma<float>::v4 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = Mfma<float>::v4{0.f, 0.f, 0.f, 0.f};
+  const int ak = tid & 15, ar = tid >> 4;
+  const int bk = tid >> 4, bc = tid & 15;
+  float ra[RA], rb[RB];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int j = 0; j < RA; ++j) {
+      const int m = t.z + ar + 16 * j, kk = k0 + ak;
+      ra[j] = (m < n && kk < n) ? A[(int64_t)m * n + kk] : 0.f;
+    }
+#pragma unroll
+    for (int j = 0; j < RB; ++j) {
+      const int c = t.w + bc + 16 * j, kk = k0 + bk;
+      rb[j] = (c < w && kk < n) ? Bg[(int64_t)kk * R + c] : 0.f;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < RA; ++j) As[buf][ak * LDA + ar + 16 * j] = ra[j];
+#pragma unroll
+    for (int j = 0; j < RB; ++j) Bs[buf][bk * LDB + bc + 16 * j] = rb[j];
+  };
+  const int nsteps = (n + GM_BK - 1) / GM_BK;
+  if (nsteps > 0) {
+    load(0);
+    store(0);
+  }
+  __syncthreads();
+  for (int s = 0; s < nsteps; ++s) {
+    const int cur = s & 1;
+    if (s + 1 < nsteps) load((s + 1) * GM_BK);
+    mfma_step<float, TM, TN, LDA, LDB>(As[cur], Bs[cur], wm0, wn0, lane, acc);
+    if (s + 1 < nsteps) store(cur ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int r = t.z + wm0 + 16 * i + Mfma<float>::row(lane, e);
+        const int c = t.w + wn0 + 16 * j + (lane & 15);
+        if (r < n && c < w) Cg[(int64_t)r * R + c] = acc[i][j][e];
+      }
+}
+
+// Grouped slab-segment SYRK in FP32 with FP64-flushed accumulators (8 waves,
+// 64 x 32 each, flush every 16 steps):
+//   partial[g][chunk](m,n) = sum_{slabs s in chunk} sum_k A_g(off_s + m*len_s + k) B(off_s + n*len_s + k)
+// with A_g = A + g * a_stride.  Z_g(X,Y) = sum b3_g(X|mu nu) B(Y|mu nu) for all
+// Laplace points g in one launch (z_full.cpp:373-377).
+__global__ __launch_bounds__(512) void k_slabseg_f32(const float* __restrict__ A, int64_t a_stride,
+                                                     const float* __restrict__ B, const int64_t* __restrict__ seg_off,
+                                                     const int64_t* __restrict__ seg_len,
+                                                     const int* __restrict__ chunk_seg, int nchunk, int n,
+                                                     const int2* __restrict__ tiles, int ntile,
+                                                     double* __restrict__ partial)
+{
+  constexpr int BM = 128, TM = 4, TN = 2, LDA = BM + 16, RA = BM / 32, FLUSH = 16;
+  __shared__ float As[2][GM_BK * LDA];
+  __shared__ float Bs[2][GM_BK * LDA];
+  const int2 tl = tiles[blockIdx.x % ntile];
+  const int rest = blockIdx.x / ntile;
+  const int chunk = rest % nchunk, g = rest / nchunk;
+  const float* Ag = A + (int64_t)g * a_stride;
+  const int sb = chunk_seg[chunk], se = chunk_seg[chunk + 1];
+  const int m0 = tl.x * BM, n0 = tl.y * BM;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave >> 2) * 64, wn0 = (wave & 3) * 32;
+  Mfma<float>::v4 acc[TM][TN];
+  double acc64[TM][TN][4];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      acc[i][j] = Mfma<float>::v4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc64[i][j][e] = 0.0;
+    }
+  const int lk = tid & 15, lr = tid >> 4;
+  float ra[RA], rb[RA];
+  auto load = [&](int sg, int64_t kb) {
+    const int64_t len = seg_len[sg];
+    const int64_t kk = kb + lk;
+    const float* Ab = Ag + seg_off[sg];
+    const float* Bb = B + seg_off[sg];
+#pragma unroll
+    for (int j = 0; j < RA; ++j) {
+      const int m = m0 + lr + 32 * j, nn = n0 + lr + 32 * j;
+      ra[j] = (m < n && kk < len) ? Ab[(int64_t)m * len + kk] : 0.f;
+      rb[j] = (nn < n && kk < len) ? Bb[(int64_t)nn * len + kk] : 0.f;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < RA; ++j) {
+      As[buf][lk * LDA + lr + 32 * j] = ra[j];
+      Bs[buf][lk * LDA + lr + 32 * j] = rb[j];
+    }
+  };
+  auto advance = [&](int& sg, int64_t& kb) {
+    kb += GM_BK;
+    if (kb >= seg_len[sg]) {
+      kb = 0;
+      ++sg;
+      while (sg < se && seg_len[sg] == 0) ++sg;
+    }
+  };
+  auto flush = [&]() {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc64[i][j][e] += (double)acc[i][j][e];
+        acc[i][j] = Mfma<float>::v4{0.f, 0.f, 0.f, 0.f};
+      }
+  };
+  int seg = sb;
+  int64_t k0 = 0;
+  while (seg < se && seg_len[seg] == 0) ++seg;
+  bool have = seg < se;
+  if (have) {
+    load(seg, k0);
+    store(0);
+  }
+  __syncthreads();
+  int cur = 0, step = 0;
+  while (have) {
+    int nseg = seg;
+    int64_t nk = k0;
+    advance(nseg, nk);
+    const bool more = nseg < se;
+    if (more) load(nseg, nk);
+    mfma_step<float, TM, TN, LDA, LDA>(As[cur], Bs[cur], wm0, wn0, lane, acc);
+    if (more) store(cur ^ 1);
+    if (++step % FLUSH == 0) flush();
+    __syncthreads();
+    cur ^= 1;
+    seg = nseg;
+    k0 = nk;
+    have = more;
+  }
+  flush();
+  double* Pp = partial + ((int64_t)g * nchunk + chunk) * n * n;
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int r = m0 + wm0 + 16 * i + Mfma<float>::row(lane, e);
+        const int c = n0 + wn0 + 16 * j + (lane & 15);
+        if (r < n && c < n) Pp[(int64_t)r * n + c] = acc64[i][j][e];
+      }
+}
+
+template __global__ void k_group_apply_f32<128, 64>(const float*, const float*, float*, int, int, const int*,
+                                                    const int*, const int4*);
+
 template __global__ void k_slab_out_gemm<double, 128, 128>(const double*, int, int, const int64_t*,
     const int*, const int*, const int*, const int*, const int*, const Tile*, int, int, const double*, int,
-    double*);
+    double*, const int*, const int*, int64_t);
+template __global__ void k_slab_out_gemm<float, 128, 128>(const float*, int, int, const int64_t*,
+    const int*, const int*, const int*, const int*, const int*, const Tile*, int, int, const float*, int,
+    float*, const int*, const int*, int64_t);
 template __global__ void k_slabseg_gemm<double, 128, 128>(const double*, const double*, const int64_t*,
     const int64_t*, const int*, int, const int2*, int, double*);
 

@@ -14,6 +14,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <numeric>
 
@@ -159,14 +160,12 @@ std::vector<double> pivoted_cholesky_dev(Ctx& c, DevBuf& U, int N, double filter
   std::iota(perm.begin(), perm.end(), 0);
   double hp[2];
   hipLaunchKernelGGL(k_pcd_pivot, dim3(1), dim3(1024), 0, c.stream, U.as<double>(), N, 0, piv.as<double>());
-  HIP_CHECK(hipMemcpyAsync(hp, piv.p, sizeof(hp), hipMemcpyDeviceToHost, c.stream));
-  HIP_CHECK(hipStreamSynchronize(c.stream));
+  d2h(hp, piv.p, sizeof(hp), c.stream);
   const double thresh = N * 2.220446049250313e-16 * std::fabs(hp[1]);
   int rank = N;
   for (int I = 0; I < N; ++I) {
     hipLaunchKernelGGL(k_pcd_pivot, dim3(1), dim3(1024), 0, c.stream, U.as<double>(), N, I, piv.as<double>());
-    HIP_CHECK(hipMemcpyAsync(hp, piv.p, sizeof(hp), hipMemcpyDeviceToHost, c.stream));
-    HIP_CHECK(hipStreamSynchronize(c.stream));
+    d2h(hp, piv.p, sizeof(hp), c.stream);
     const int j = (int)hp[0];
     const double pv = hp[1];
     if (j != I && j < N) {
@@ -189,8 +188,7 @@ std::vector<double> pivoted_cholesky_dev(Ctx& c, DevBuf& U, int N, double filter
   }
   HIP_CHECK(hipGetLastError());
   std::vector<double> hL((size_t)N * N);
-  HIP_CHECK(hipMemcpyAsync(hL.data(), L.p, hL.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-  HIP_CHECK(hipStreamSynchronize(c.stream));
+  d2h(hL.data(), L.p, hL.size() * sizeof(double), c.stream);
   // rows back to the original order
   std::vector<double> Lo((size_t)N * rank, 0.0);
   for (int r = 0; r < N; ++r)
@@ -228,91 +226,145 @@ static std::vector<int> split_range(int n, int split)
   return v;
 }
 
+// Shared set-up of both MP2 drivers: Cholesky factor of S and its inverse,
+// orbital energies and coefficients, block offsets for the filters.
+struct Mp2Setup {
+  int n = 0, no = 0, nv = 0, nbb = 0;
+  DevBuf LS, LSinv, eo, ev, d_boff, Wo, Wv, Co, Po;
+  std::vector<double> hco, hcv;
+};
+
+static void mp2_setup(Ctx& c, const Mp2Input& in, Mp2Setup& s)
+{
+  const int n = in.nbf, no = in.nocc, nv = n - in.nocc;
+  s.n = n;
+  s.no = no;
+  s.nv = nv;
+  const size_t nn = (size_t)n * n;
+  s.LS.alloc(nn * sizeof(double));
+  s.LSinv.alloc(nn * sizeof(double));
+  s.eo.alloc(std::max(no, 1) * sizeof(double));
+  s.ev.alloc(std::max(nv, 1) * sizeof(double));
+  s.Wo.alloc((size_t)n * std::max(no, 1) * sizeof(double));
+  s.Wv.alloc((size_t)n * std::max(nv, 1) * sizeof(double));
+  s.Co.alloc((size_t)n * std::max(no, 1) * sizeof(double));
+  s.Po.alloc(nn * sizeof(double));
+  s.hco.resize((size_t)n * no);
+  s.hcv.resize((size_t)n * nv);
+  for (int r = 0; r < n; ++r) {
+    for (int i = 0; i < no; ++i) s.hco[(size_t)r * no + i] = in.C[(size_t)r * n + i];
+    for (int a = 0; a < nv; ++a) s.hcv[(size_t)r * nv + a] = in.C[(size_t)r * n + no + a];
+  }
+  h2d(s.LS.p, in.S, nn * sizeof(double), c.stream);
+  if (no) h2d(s.eo.p, in.eps, no * sizeof(double), c.stream);
+  if (nv) h2d(s.ev.p, in.eps + no, nv * sizeof(double), c.stream);
+  // S = L_S L_S^T (column-major lower), L_S^-1
+  int* info = c.info.as<int>();
+  RB_CHECK(rocsolver_dpotrf(c.blas, rocblas_fill_lower, n, s.LS.as<double>(), n, info));
+  hipLaunchKernelGGL(k_zero_upper_cm, dim3(nblk((int64_t)nn, 256)), dim3(256), 0, c.stream, s.LS.as<double>(), n);
+  HIP_CHECK(hipMemcpyAsync(s.LSinv.p, s.LS.p, nn * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  RB_CHECK(rocsolver_dtrtri(c.blas, rocblas_fill_lower, rocblas_diagonal_non_unit, n, s.LSinv.as<double>(), n, info));
+  hipLaunchKernelGGL(k_zero_upper_cm, dim3(nblk((int64_t)nn, 256)), dim3(256), 0, c.stream, s.LSinv.as<double>(), n);
+  std::vector<int> boff(in.bsz.size() + 1, 0);
+  for (size_t i = 0; i < in.bsz.size(); ++i) boff[i + 1] = boff[i] + in.bsz[i];
+  s.d_boff = upload_vec(boff, c.stream);
+  s.nbb = (int)in.bsz.size();
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+}
+
+// Pseudo-density factors of Laplace point k (mp_compute.cpp:232-302):
+// P~_o = w^1/4 (L_S^-1 C_o')(...)^T factorised by pivoted Cholesky, L_o = L_S L~
+// (n x rank row-major, in Lo), and the filtered virtual pseudo-density P_v
+// (n x n, in Pv).  Returns the rank.
+static int mp2_point_factors(Ctx& c, Mp2Setup& s, const Mp2Input& in, size_t k, DevBuf& Lo, DevBuf& Pv)
+{
+  const int n = s.n, no = s.no, nv = s.nv;
+  const size_t nn = (size_t)n * n;
+  const double w = in.omega[k], al = in.alpha[k];
+  const double w4 = std::pow(w, 0.25);
+  const double one = 1.0, zero = 0.0;
+  Pv.alloc(nn * sizeof(double));
+  if (no) {
+    h2d(s.Wo.p, s.hco.data(), (size_t)n * no * sizeof(double), c.stream);
+    hipLaunchKernelGGL(k_scale_cols_exp, dim3(nblk((int64_t)n * no, 256)), dim3(256), 0, c.stream,
+                       s.Wo.as<double>(), n, no, s.eo.as<double>(), 0.5 * al, 1.0);
+    // c_ortho = L_S^-1 C_o' : column-major (no x n) view times L_S^-T
+    RB_CHECK(rocblas_dtrmm(c.blas, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                           rocblas_diagonal_non_unit, no, n, &one, s.LSinv.as<double>(), n, s.Wo.as<double>(), no,
+                           s.Co.as<double>(), no));
+    RB_CHECK(rocblas_dgemm(c.blas, rocblas_operation_transpose, rocblas_operation_none, n, n, no, &w4,
+                           s.Co.as<double>(), no, s.Co.as<double>(), no, &zero, s.Po.as<double>(), n));
+  }
+  if (nv) {
+    h2d(s.Wv.p, s.hcv.data(), (size_t)n * nv * sizeof(double), c.stream);
+    hipLaunchKernelGGL(k_scale_cols_exp, dim3(nblk((int64_t)n * nv, 256)), dim3(256), 0, c.stream,
+                       s.Wv.as<double>(), n, nv, s.ev.as<double>(), -0.5 * al, 1.0);
+    RB_CHECK(rocblas_dgemm(c.blas, rocblas_operation_transpose, rocblas_operation_none, n, n, nv, &w4,
+                           s.Wv.as<double>(), nv, s.Wv.as<double>(), nv, &zero, Pv.as<double>(), n));
+  }
+  else {
+    hipLaunchKernelGGL(k_fill, dim3(gs_blocks((int64_t)nn)), dim3(256), 0, c.stream, Pv.as<double>(), (int64_t)nn, 0.0);
+  }
+  // M1: pivoted Cholesky of P~_o, L_o = L_S L~
+  int rank = 0;
+  std::vector<double> hLt = pivoted_cholesky_dev(c, s.Po, n, in.filter_eps, &rank);
+  if (rank == 0) return 0;
+  DevBuf Lt((size_t)n * rank * sizeof(double));
+  Lo.alloc((size_t)n * rank * sizeof(double));
+  h2d(Lt.p, hLt.data(), hLt.size() * sizeof(double), c.stream);
+  // column-major (rank x n) view: Lo^T = Lt^T L_S^T
+  RB_CHECK(rocblas_dtrmm(c.blas, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                         rocblas_diagonal_non_unit, rank, n, &one, s.LS.as<double>(), n, Lt.as<double>(), rank,
+                         Lo.as<double>(), rank));
+  std::vector<int> u = split_range(rank, in.mo_split), uoff(u.size() + 1, 0);
+  for (size_t i = 0; i < u.size(); ++i) uoff[i + 1] = uoff[i] + u[i];
+  DevBuf d_uoff = upload_vec(uoff, c.stream);
+  if (in.filter_eps > 0) {
+    hipLaunchKernelGGL(k_block_filter_rc, dim3(s.nbb * (int)u.size()), dim3(256), 0, c.stream, Lo.as<double>(),
+                       rank, s.d_boff.as<int>(), d_uoff.as<int>(), (int)u.size(), in.filter_eps);
+    hipLaunchKernelGGL(k_block_filter_rc, dim3(s.nbb * s.nbb), dim3(256), 0, c.stream, Pv.as<double>(), n,
+                       s.d_boff.as<int>(), s.d_boff.as<int>(), s.nbb, in.filter_eps);
+  }
+  HIP_CHECK(hipStreamSynchronize(c.stream));   // host factor and offsets die here
+  return rank;
+}
+
+// e = sum_xy Z~_xy Z~_yx with Z~ = V^-1 Z (mp_compute.cpp:361-389), FP64.
+static double mp2_point_energy(Ctx& c, const Metric& met, const double* Z, DevBuf& Zt, DevBuf& part)
+{
+  const int naux = met.naux;
+  const double one = 1.0, zero = 0.0;
+  RB_CHECK(rocblas_dgemm(c.blas, rocblas_operation_none, rocblas_operation_none, naux, naux, naux, &one,
+                         met.Vinv.as<double>(), naux, Z, naux, &zero, Zt.as<double>(), naux));
+  hipLaunchKernelGGL(k_trace_sq, dim3(256), dim3(256), 0, c.stream, Zt.as<double>(), naux, part.as<double>());
+  double hpart[256];
+  d2h(hpart, part.p, sizeof(hpart), c.stream);
+  double e = 0.0;
+  for (double v : hpart) e += v;
+  return e;
+}
+
 double mp2_sos(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in, std::vector<double>& terms,
                double* t_z)
 {
   const BLayout& L = bt.L;
   if (bt.transformed) throw std::runtime_error("MP2 needs the untransformed three-centre tensor");
-  const int n = in.nbf, no = in.nocc, nv = n - in.nocc, naux = L.naux;
+  const int n = in.nbf, naux = L.naux;
   if (n != L.nbf || met.naux != naux) throw std::runtime_error("MP2: dimensions do not match B / metric");
-  const size_t nn = (size_t)n * n;
-  DevBuf LS(nn * sizeof(double)), Co((size_t)n * std::max(no, 1) * sizeof(double)),
-      Cv((size_t)n * std::max(nv, 1) * sizeof(double)), Wo((size_t)n * std::max(no, 1) * sizeof(double)),
-      Wv((size_t)n * std::max(nv, 1) * sizeof(double)), Po(nn * sizeof(double)), Pv(nn * sizeof(double)),
-      eo(std::max(no, 1) * sizeof(double)), ev(std::max(nv, 1) * sizeof(double));
-  // occupied / virtual coefficients (row-major n x no / n x nv)
-  std::vector<double> hco((size_t)n * no), hcv((size_t)n * nv);
-  for (int r = 0; r < n; ++r) {
-    for (int i = 0; i < no; ++i) hco[(size_t)r * no + i] = in.C[(size_t)r * n + i];
-    for (int a = 0; a < nv; ++a) hcv[(size_t)r * nv + a] = in.C[(size_t)r * n + no + a];
-  }
-  HIP_CHECK(hipMemcpyAsync(LS.p, in.S, nn * sizeof(double), hipMemcpyHostToDevice, c.stream));
-  if (no) HIP_CHECK(hipMemcpyAsync(eo.p, in.eps, no * sizeof(double), hipMemcpyHostToDevice, c.stream));
-  if (nv) HIP_CHECK(hipMemcpyAsync(ev.p, in.eps + no, nv * sizeof(double), hipMemcpyHostToDevice, c.stream));
-  // S = L_S L_S^T (column-major lower), L_S^-1
-  DevBuf LSinv(nn * sizeof(double));
-  int* info = c.info.as<int>();
-  RB_CHECK(rocsolver_dpotrf(c.blas, rocblas_fill_lower, n, LS.as<double>(), n, info));
-  hipLaunchKernelGGL(k_zero_upper_cm, dim3(nblk((int64_t)nn, 256)), dim3(256), 0, c.stream, LS.as<double>(), n);
-  HIP_CHECK(hipMemcpyAsync(LSinv.p, LS.p, nn * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-  RB_CHECK(rocsolver_dtrtri(c.blas, rocblas_fill_lower, rocblas_diagonal_non_unit, n, LSinv.as<double>(), n, info));
-  hipLaunchKernelGGL(k_zero_upper_cm, dim3(nblk((int64_t)nn, 256)), dim3(256), 0, c.stream, LSinv.as<double>(), n);
-  // block offsets for the filters
-  std::vector<int> boff(in.bsz.size() + 1, 0);
-  for (size_t i = 0; i < in.bsz.size(); ++i) boff[i + 1] = boff[i] + in.bsz[i];
-  DevBuf d_boff = upload_vec(boff, c.stream);
-  const int nbb = (int)in.bsz.size();
-  // Z builder workspaces
-  DevBuf Zp, Zm(nn == 0 ? 8 : (size_t)naux * naux * sizeof(double)), Zt((size_t)naux * naux * sizeof(double)),
+  Mp2Setup s;
+  mp2_setup(c, in, s);
+  DevBuf Zp, Zm((size_t)naux * naux * sizeof(double)), Zt((size_t)naux * naux * sizeof(double)),
       part(256 * sizeof(double));
   DevBuf b3((size_t)std::max<int64_t>(L.total, 1) * sizeof(double));
   const double one = 1.0, zero = 0.0;
   terms.clear();
-  double total = 0.0;
-  double tz = 0.0;
+  double total = 0.0, tz = 0.0;
   for (size_t k = 0; k < in.omega.size(); ++k) {
-    const double w = in.omega[k], al = in.alpha[k];
-    const double w4 = std::pow(w, 0.25);
-    // M0: scaled coefficients
-    if (no) {
-      HIP_CHECK(hipMemcpyAsync(Wo.p, hco.data(), (size_t)n * no * sizeof(double), hipMemcpyHostToDevice, c.stream));
-      hipLaunchKernelGGL(k_scale_cols_exp, dim3(nblk((int64_t)n * no, 256)), dim3(256), 0, c.stream,
-                         Wo.as<double>(), n, no, eo.as<double>(), 0.5 * al, 1.0);
-      // c_ortho = L_S^-1 C_o' : column-major (no x n) view times L_S^-T
-      RB_CHECK(rocblas_dtrmm(c.blas, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
-                             rocblas_diagonal_non_unit, no, n, &one, LSinv.as<double>(), n, Wo.as<double>(), no,
-                             Co.as<double>(), no));
-      RB_CHECK(rocblas_dgemm(c.blas, rocblas_operation_transpose, rocblas_operation_none, n, n, no, &w4,
-                             Co.as<double>(), no, Co.as<double>(), no, &zero, Po.as<double>(), n));
-    }
-    if (nv) {
-      HIP_CHECK(hipMemcpyAsync(Wv.p, hcv.data(), (size_t)n * nv * sizeof(double), hipMemcpyHostToDevice, c.stream));
-      hipLaunchKernelGGL(k_scale_cols_exp, dim3(nblk((int64_t)n * nv, 256)), dim3(256), 0, c.stream,
-                         Wv.as<double>(), n, nv, ev.as<double>(), -0.5 * al, 1.0);
-      RB_CHECK(rocblas_dgemm(c.blas, rocblas_operation_transpose, rocblas_operation_none, n, n, nv, &w4,
-                             Wv.as<double>(), nv, Wv.as<double>(), nv, &zero, Pv.as<double>(), n));
-    }
-    // M1: pivoted Cholesky of P~_o, L_o = L_S L~
-    int rank = 0;
-    std::vector<double> hLt = pivoted_cholesky_dev(c, Po, n, in.filter_eps, &rank);
+    DevBuf Lo, Pv;
+    const int rank = mp2_point_factors(c, s, in, k, Lo, Pv);
     if (rank == 0) {
       terms.push_back(0.0);
       continue;
-    }
-    DevBuf Lt((size_t)n * rank * sizeof(double)), Lo((size_t)n * rank * sizeof(double));
-    HIP_CHECK(hipMemcpyAsync(Lt.p, hLt.data(), hLt.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
-    // column-major (rank x n) view: Lo^T = Lt^T L_S^T
-    RB_CHECK(rocblas_dtrmm(c.blas, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
-                           rocblas_diagonal_non_unit, rank, n, &one, LS.as<double>(), n, Lt.as<double>(), rank,
-                           Lo.as<double>(), rank));
-    std::vector<int> u = split_range(rank, in.mo_split), uoff(u.size() + 1, 0);
-    for (size_t i = 0; i < u.size(); ++i) uoff[i + 1] = uoff[i] + u[i];
-    DevBuf d_uoff = upload_vec(uoff, c.stream);
-    if (in.filter_eps > 0) {
-      hipLaunchKernelGGL(k_block_filter_rc, dim3(nbb * (int)u.size()), dim3(256), 0, c.stream, Lo.as<double>(),
-                         rank, d_boff.as<int>(), d_uoff.as<int>(), (int)u.size(), in.filter_eps);
-      hipLaunchKernelGGL(k_block_filter_rc, dim3(nbb * nbb), dim3(256), 0, c.stream, Pv.as<double>(), n,
-                         d_boff.as<int>(), d_boff.as<int>(), nbb, in.filter_eps);
     }
     HIP_CHECK(hipEventRecord(c.ev0, c.stream));
     // M2: b = B . L_o   [X][nu][i]
@@ -335,20 +387,184 @@ double mp2_sos(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in, std::ve
     float ms = 0.f;
     HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
     tz += ms * 1e-3;
-    // M6: Z~ = Z V^-1, e = sum Z~_xy Z~_yx
-    RB_CHECK(rocblas_dgemm(c.blas, rocblas_operation_none, rocblas_operation_none, naux, naux, naux, &one,
-                           met.Vinv.as<double>(), naux, Zm.as<double>(), naux, &zero, Zt.as<double>(), naux));
-    hipLaunchKernelGGL(k_trace_sq, dim3(256), dim3(256), 0, c.stream, Zt.as<double>(), naux, part.as<double>());
-    double hpart[256];
-    HIP_CHECK(hipMemcpyAsync(hpart, part.p, sizeof(hpart), hipMemcpyDeviceToHost, c.stream));
-    HIP_CHECK(hipStreamSynchronize(c.stream));
-    double e = 0.0;
-    for (double v : hpart) e += v;
+    const double e = mp2_point_energy(c, met, Zm.as<double>(), Zt, part);
     terms.push_back(e);
     total += e;
   }
   HIP_CHECK(hipGetLastError());
   if (t_z) *t_z = tz;
+  return total;
+}
+
+// FP32 copy of a row-major (rows x w) FP64 block into columns [c0, c0 + w) of
+// a row-major FP32 matrix with leading dimension ld.
+__global__ void k_cols_to_f32(const double* __restrict__ in, int rows, int w, float* __restrict__ out, int ld,
+                              int c0)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)rows * w) return;
+  const int r = (int)(i / w), col = (int)(i - (int64_t)r * w);
+  out[(int64_t)r * ld + c0 + col] = (float)in[i];
+}
+
+// All Laplace points at once (LLMP_FULL_Z, z_full.cpp:69-397, for every point
+// of mp_compute.cpp:232-389).  The pseudo-densities and their pivoted
+// Cholesky factors are made per point in FP64; then the four contractions of
+// all points go out as grouped launches captured in one hipGraph, in FP32
+// with FP64 accumulation:
+//   M2  b[X][nu][i]     = sum_mu B(X|mu nu) L_all(mu, i)   one slab GEMM over the
+//                                                         concatenated columns of all points
+//   M3  b2_g[X]         = P_v,g b_g[X]                     grouped over (point, X)
+//   M4  b3_g(X|M N)     = sum_i L_g(M,i) b2_g(X,i,N)        grouped over points, B's pattern
+//   M5  Z_g(X,Y)        = sum b3_g(X|mu nu) B(Y|mu nu)       grouped over points, FP64-flushed
+// and the energies e_g = tr(Z~_g Z~_g), Z~ = V^-1 Z_g, in FP64 after the graph.
+// The column tile of M2 is picked by timing both candidates on the live data.
+double mp2_sos_batched(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in, std::vector<double>& terms,
+                       double* t_z, double* t_prep)
+{
+  const BLayout& L = bt.L;
+  if (bt.transformed) throw std::runtime_error("MP2 needs the untransformed three-centre tensor");
+  const int n = in.nbf, naux = L.naux;
+  if (n != L.nbf || met.naux != naux) throw std::runtime_error("MP2: dimensions do not match B / metric");
+  const int npt = (int)in.omega.size();
+  const auto tp0 = std::chrono::steady_clock::now();
+  Mp2Setup s;
+  mp2_setup(c, in, s);
+  std::vector<DevBuf> Lo(npt), Pv(npt);
+  std::vector<int> rank(npt), pcol(npt + 1, 0);
+  for (int g = 0; g < npt; ++g) {
+    rank[g] = mp2_point_factors(c, s, in, g, Lo[g], Pv[g]);
+    pcol[g + 1] = pcol[g] + rank[g];
+  }
+  const int R = pcol[npt];
+  terms.assign(npt, 0.0);
+  if (R == 0) {
+    if (t_z) *t_z = 0.0;
+    if (t_prep) *t_prep = 0.0;
+    return 0.0;
+  }
+  // FP32 operands: all L_o side by side (n x R), the P_v (n x n each), B
+  const size_t nn = (size_t)n * n;
+  DevBuf Lall((size_t)n * R * sizeof(float)), Pv32((size_t)npt * nn * sizeof(float)),
+      B32((size_t)std::max<int64_t>(L.total, 1) * sizeof(float));
+  for (int g = 0; g < npt; ++g) {
+    if (rank[g] == 0) continue;
+    hipLaunchKernelGGL(k_cols_to_f32, dim3(nblk((int64_t)n * rank[g], 256)), dim3(256), 0, c.stream,
+                       Lo[g].as<double>(), n, rank[g], Lall.as<float>(), R, pcol[g]);
+    hipLaunchKernelGGL(k_to_f32, dim3(gs_blocks((int64_t)nn)), dim3(256), 0, c.stream, Pv[g].as<double>(),
+                       Pv32.as<float>() + (size_t)g * nn, (int64_t)nn);
+  }
+  if (L.total)
+    hipLaunchKernelGGL(k_to_f32, dim3(gs_blocks(L.total)), dim3(256), 0, c.stream, bt.data64.as<double>(),
+                       B32.as<float>(), (int64_t)L.total);
+  HIP_CHECK(hipGetLastError());
+  const size_t bsize = (size_t)naux * n * R;
+  DevBuf bb(bsize * sizeof(float)), b2(bsize * sizeof(float)),
+      b3((size_t)npt * std::max<int64_t>(L.total, 1) * sizeof(float));
+  std::vector<int> pw(rank.begin(), rank.end());
+  std::vector<int> pc(pcol.begin(), pcol.end() - 1);
+  DevBuf d_pcol = upload_vec(pc, c.stream), d_pw = upload_vec(pw, c.stream);
+  // M3 tiles (point, X, m0, n0)
+  std::vector<int4> t3;
+  for (int g = 0; g < npt; ++g)
+    for (int x = 0; x < naux; ++x)
+      for (int m0 = 0; m0 < n; m0 += 128)
+        for (int n0 = 0; n0 < rank[g]; n0 += 64) t3.push_back(int4{g, x, m0, n0});
+  // M4 tiles (slab, m0, c0, point)
+  std::vector<Tile> t4;
+  for (int g = 0; g < npt; ++g) {
+    if (rank[g] == 0) continue;
+    for (int nb = 0; nb < L.nbb; ++nb) {
+      const int64_t rows = (int64_t)naux * L.bsz[nb];
+      for (int64_t m0 = 0; m0 < rows; m0 += SG_BM)
+        for (int c0 = 0; c0 < L.ncol[nb]; c0 += 128) t4.push_back(Tile{nb, (int)m0, c0, g});
+    }
+  }
+  // M5: upper 128 x 128 tiles of Z, slabs grouped in chunks of similar length
+  const int ntz = (naux + SG_BM - 1) / SG_BM;
+  std::vector<int2> t5;
+  for (int i = 0; i < ntz; ++i)
+    for (int j = i; j < ntz; ++j) t5.push_back(int2{i, j});
+  const int ntile = (int)t5.size();
+  std::vector<int64_t> soff(L.nbb), slen(L.nbb);
+  int64_t tot = 0;
+  for (int sb = 0; sb < L.nbb; ++sb) {
+    soff[sb] = L.slab_off[sb];
+    slen[sb] = (int64_t)L.bsz[sb] * L.ncol[sb];
+    tot += slen[sb];
+  }
+  const int want = std::max(1, std::min(L.nbb, 2048 / std::max(1, ntile * npt) + 1));
+  std::vector<int> cs(1, 0);
+  int64_t acc = 0;
+  for (int sb = 0; sb < L.nbb; ++sb) {
+    acc += slen[sb];
+    if ((int)cs.size() < want && acc * want >= tot * (int64_t)cs.size() && sb + 1 < L.nbb) cs.push_back(sb + 1);
+  }
+  cs.push_back(L.nbb);
+  const int nch = (int)cs.size() - 1;
+  DevBuf d_t3 = upload_vec(t3, c.stream), d_t4 = upload_vec(t4, c.stream), d_t5 = upload_vec(t5, c.stream),
+         d_soff = upload_vec(soff, c.stream), d_slen = upload_vec(slen, c.stream), d_cs = upload_vec(cs, c.stream);
+  DevBuf partial((size_t)npt * nch * naux * naux * sizeof(double)), Z((size_t)npt * naux * naux * sizeof(double));
+  // autotune the M2 column tile on the live operands (also creates its tile table)
+  int bn = 128;
+  {
+    float best = 1e30f;
+    for (int cand : {64, 128}) {
+      slab_gemm_f32(c, bt, B32.as<float>(), Lall.as<float>(), R, R, bb.as<float>(), R, cand);
+      HIP_CHECK(hipEventRecord(c.ev0, c.stream));
+      slab_gemm_f32(c, bt, B32.as<float>(), Lall.as<float>(), R, R, bb.as<float>(), R, cand);
+      HIP_CHECK(hipEventRecord(c.ev1, c.stream));
+      HIP_CHECK(hipEventSynchronize(c.ev1));
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+      if (ms < best) {
+        best = ms;
+        bn = cand;
+      }
+    }
+  }
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  if (t_prep) *t_prep = std::chrono::duration<double>(std::chrono::steady_clock::now() - tp0).count();
+  // capture M2..M5 and the chunk reductions in one graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+  slab_gemm_f32(c, bt, B32.as<float>(), Lall.as<float>(), R, R, bb.as<float>(), R, bn);
+  hipLaunchKernelGGL((k_group_apply_f32<128, 64>), dim3((unsigned)t3.size()), dim3(256), 0, c.stream,
+                     Pv32.as<float>(), bb.as<float>(), b2.as<float>(), n, R, d_pcol.as<int>(), d_pw.as<int>(),
+                     d_t3.as<int4>());
+  hipLaunchKernelGGL((k_slab_out_gemm<float, SG_BM, 128>), dim3((unsigned)t4.size()), dim3(256), 0, c.stream,
+                     b2.as<float>(), R, 0, bt.slab_off.as<int64_t>(), bt.ncol.as<int>(), bt.bsz.as<int>(),
+                     bt.boff.as<int>(), bt.colmap.as<int>(), bt.colmap_off.as<int>(), d_t4.as<Tile>(), naux, n,
+                     Lall.as<float>(), R, b3.as<float>(), d_pcol.as<int>(), d_pw.as<int>(), (int64_t)L.total);
+  hipLaunchKernelGGL(k_slabseg_f32, dim3((unsigned)(ntile * nch * npt)), dim3(512), 0, c.stream, b3.as<float>(),
+                     (int64_t)L.total, B32.as<float>(), d_soff.as<int64_t>(), d_slen.as<int64_t>(), d_cs.as<int>(),
+                     nch, naux, d_t5.as<int2>(), ntile, partial.as<double>());
+  for (int g = 0; g < npt; ++g)
+    hipLaunchKernelGGL(k_reduce_sym, dim3(nblk((int64_t)naux * naux, 256)), dim3(256), 0, c.stream,
+                       partial.as<double>() + (size_t)g * nch * naux * naux, nch, naux, 0, 1.0,
+                       Z.as<double>() + (size_t)g * naux * naux);
+  const hipError_t cap_err = hipGetLastError();
+  HIP_CHECK(hipStreamEndCapture(c.stream, &graph));
+  HIP_CHECK(cap_err);
+  HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  HIP_CHECK(hipEventRecord(c.ev0, c.stream));
+  HIP_CHECK(hipGraphLaunch(exec, c.stream));
+  HIP_CHECK(hipEventRecord(c.ev1, c.stream));
+  HIP_CHECK(hipEventSynchronize(c.ev1));
+  float ms = 0.f;
+  HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+  if (t_z) *t_z = ms * 1e-3;
+  (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  // M6 per point in FP64
+  DevBuf Zt((size_t)naux * naux * sizeof(double)), part(256 * sizeof(double));
+  double total = 0.0;
+  for (int g = 0; g < npt; ++g) {
+    if (rank[g] == 0) continue;
+    terms[g] = mp2_point_energy(c, met, Z.as<double>() + (size_t)g * naux * naux, Zt, part);
+    total += terms[g];
+  }
   return total;
 }
 

@@ -29,4 +29,11 @@ std::vector<double> pivoted_cholesky_dev(Ctx& c, DevBuf& U, int N, double filter
 double mp2_sos(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in, std::vector<double>& terms,
                double* t_z);
 
+// Same energy with the contractions of all Laplace points grouped and captured
+// in one hipGraph, FP32 with FP64 accumulation.  t_z = seconds of the graph
+// launch (M2-M5 of every point), t_prep = seconds of the FP64 per-point set-up
+// (pseudo-densities, pivoted Cholesky, FP32 copies, autotuning).
+double mp2_sos_batched(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in, std::vector<double>& terms,
+                       double* t_z, double* t_prep);
+
 }  // namespace mchip

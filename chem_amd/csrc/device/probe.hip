@@ -84,3 +84,39 @@ extern "C" int mchip_probe_mfma_lp(int kind, const float* a_lane, const float* b
   HIP_CHECK(hipMemcpy(out, o.p, 256 * sizeof(float), hipMemcpyDeviceToHost));
   MCHIP_CATCH
 }
+
+namespace mchip {
+
+// global_load_lds_dwordx4 placement: each of the 64 lanes loads the 16 bytes
+// src[perm[lane]] into LDS; the LDS image (64 x 16 bytes from the destination
+// base) is copied out.  Expected: lane i's 16 bytes at base + 16 i.
+__global__ void k_probe_lds_load(const uint4* __restrict__ src, const int* __restrict__ perm, uint4* __restrict__ out)
+{
+  __shared__ uint4 lds[128];
+  const int lane = threadIdx.x;
+  lds[lane] = uint4{0u, 0u, 0u, 0u};
+  lds[64 + lane] = uint4{0u, 0u, 0u, 0u};
+  __syncthreads();
+  __builtin_amdgcn_global_load_lds((const void*)(src + perm[lane]),
+                                   (__attribute__((address_space(3))) void*)(lds + 64), 16, 0, 0);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  out[lane] = lds[lane];
+  out[64 + lane] = lds[64 + lane];
+}
+
+}  // namespace mchip
+
+extern "C" int mchip_probe_lds_load(const uint32_t* src, const int* perm, uint32_t* out)
+{
+  MCHIP_TRY
+  using namespace mchip;
+  DevBuf s(64 * 16), p(64 * sizeof(int)), o(128 * 16);
+  HIP_CHECK(hipMemcpy(s.p, src, 64 * 16, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(p.p, perm, 64 * sizeof(int), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_probe_lds_load, dim3(1), dim3(64), 0, 0, s.as<uint4>(), p.as<int>(), o.as<uint4>());
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipDeviceSynchronize());
+  HIP_CHECK(hipMemcpy(out, o.p, 128 * 16, hipMemcpyDeviceToHost));
+  MCHIP_CATCH
+}

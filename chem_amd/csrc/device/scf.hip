@@ -106,8 +106,7 @@ static double dot_dev(Ctx& c, const double* a, const double* b, int64_t n, DevBu
   part.ensure(nb * sizeof(double));
   hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(256), 0, c.stream, a, b, n, part.as<double>());
   double h[256];
-  HIP_CHECK(hipMemcpyAsync(h, part.p, nb * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-  HIP_CHECK(hipStreamSynchronize(c.stream));
+  d2h(h, part.p, nb * sizeof(double), c.stream);
   double s = 0.0;
   for (int i = 0; i < nb; ++i) s += h[i];
   return s;
@@ -191,8 +190,8 @@ void scf_run(Ctx& c, DevBT& bt, const Metric* m, const ScfInput& in, ScfOutput& 
   Cocc.alloc((size_t)n * std::max(nocc, 1) * sizeof(double));
   w.alloc(n * sizeof(double));
   DevBuf ework(n * sizeof(double));
-  HIP_CHECK(hipMemcpyAsync(S.p, in.S, mb, hipMemcpyHostToDevice, c.stream));
-  HIP_CHECK(hipMemcpyAsync(H.p, in.H, mb, hipMemcpyHostToDevice, c.stream));
+  h2d(S.p, in.S, mb, c.stream);
+  h2d(H.p, in.H, mb, c.stream);
   std::vector<int> hboff(in.bsz.size() + 1, 0);
   for (size_t i = 0; i < in.bsz.size(); ++i) hboff[i + 1] = hboff[i] + in.bsz[i];
   boff = upload_vec(hboff, c.stream);
@@ -357,23 +356,23 @@ void scf_run(Ctx& c, DevBT& bt, const Metric* m, const ScfInput& in, ScfOutput& 
   out.iterations = iter;
   out.energy = energy + in.enuc;
   out.eps.resize(n);
-  HIP_CHECK(hipStreamSynchronize(c.stream));
-  HIP_CHECK(hipMemcpy(out.eps.data(), w.p, n * sizeof(double), hipMemcpyDeviceToHost));
+  d2h(out.eps.data(), w.p, n * sizeof(double), c.stream);
   out.C.resize(nn);
   // return C row-major (n x n): transpose of the column-major eigenvector matrix
   std::vector<double> tmp(nn);
-  HIP_CHECK(hipMemcpy(tmp.data(), Cev.p, mb, hipMemcpyDeviceToHost));
+  d2h(tmp.data(), Cev.p, mb, c.stream);
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j) out.C[(size_t)i * n + j] = tmp[(size_t)j * n + i];
   out.P.resize(nn);
-  HIP_CHECK(hipMemcpy(out.P.data(), P.p, mb, hipMemcpyDeviceToHost));
+  d2h(out.P.data(), P.p, mb, c.stream);
   out.F.resize(nn);
-  HIP_CHECK(hipMemcpy(out.F.data(), F.p, mb, hipMemcpyDeviceToHost));
+  d2h(out.F.data(), F.p, mb, c.stream);
 }
 
 void select_k_builders(Ctx& c, DevBT& bt, const Metric* m, int kmode, int nocc, int occ_nbatches,
-                       double threshold, KBuildFn& kb, KBuildFn& kfinal)
+                       double threshold, KBuildFn& kb, KBuildFn& kfinal, int final_occ_nbatches)
 {
+  if (final_occ_nbatches <= 0) final_occ_nbatches = occ_nbatches;
   if (kmode == 0 && !m) throw std::runtime_error("SCF: plain K needs the metric");
   if (kmode >= 2 && !bt.transformed && !m)
     throw std::runtime_error("SCF: the final FP64 exchange on the untransformed B needs the metric");
@@ -385,6 +384,10 @@ void select_k_builders(Ctx& c, DevBT& bt, const Metric* m, int kmode, int nocc, 
     dfk_mo(*cp, *t, km, Cocc, nocc, nocc, K, occ_nbatches);
   };
   kfinal = KBuildFn();
+  if (kmode >= 2)   // the final FP64 exchange of the low-precision modes
+    k64 = [cp, t, km, nocc, final_occ_nbatches](int, double, const double*, const double* Cocc, double* K) {
+      dfk_mo(*cp, *t, km, Cocc, nocc, nocc, K, final_occ_nbatches);
+    };
   if (kmode == 0 || kmode == 1) {
     kb = k64;
   }
@@ -405,7 +408,7 @@ void select_k_builders(Ctx& c, DevBT& bt, const Metric* m, int kmode, int nocc, 
   else if (kmode == 4) {
     if (!bt.lowp_tilde || !bt.data_hi.p || !bt.data32.p)
       throw std::runtime_error("SCF: the escalation mode needs the BF16 and FP32 copies of L^-1 B");
-    const double tau = std::max(1e-4, 10.0 * threshold);
+    const double tau = std::max(c.escalate_rms, 10.0 * threshold);
     auto escalated = std::make_shared<bool>(false);
     kb = [cp, t, nocc, occ_nbatches, tau, escalated](int, double prev_rms, const double*, const double* Cocc,
                                                      double* K) {

@@ -37,11 +37,11 @@ using KBuildFn = std::function<void(int, double, const double*, const double*, d
 
 // Exchange builders of an SCF run for kmode (0 plain DFMO with the metric per
 // build, 1 DFMO on the transformed B, 2 FP32, 3 BF16x3, 4 BF16x3 escalating to
-// FP32 once the previous error RMS is below max(1e-4, 10 * threshold); modes
-// 2-4 get an FP64 final builder).  The builders capture c, bt and m by
-// reference.
+// FP32 once the previous error RMS is below max(c.escalate_rms, 10 * threshold); modes
+// 2-4 get an FP64 final builder, with final_occ_nbatches occupied batches,
+// 0 = occ_nbatches).  The builders capture c, bt and m by reference.
 void select_k_builders(Ctx& c, DevBT& bt, const Metric* m, int kmode, int nocc, int occ_nbatches,
-                       double threshold, KBuildFn& kb, KBuildFn& kfinal);
+                       double threshold, KBuildFn& kb, KBuildFn& kfinal, int final_occ_nbatches = 0);
 
 // final_k (optional): FP64 K builder used once at the converged orbitals to
 // evaluate the reported energy when kbuild works in reduced precision.

@@ -9,9 +9,9 @@
 // admitted only when every earlier one has been), then builds B on the GPU
 // and runs the SCF with B, the metric factors and the densities resident in
 // HBM until it converges.  Requests on different workers overlap on the
-// device through their streams; device memory of a worker is allocated and
-// freed stream-ordered on its stream, so one request's frees do not wait for
-// another request's kernels.
+// device through their streams.  Besides the budget, admission checks the
+// device's free memory, so an estimate that falls short cannot oversubscribe
+// HBM.
 //
 // The path per request is the one of hf::hfmod::compute with the DF
 // builders (src/hf/hfscf.cpp:326-552, src/fock/jkdf.cpp).  The loop itself has
@@ -147,6 +147,10 @@ bool Server::result(int id, bool wait, SrvJob& out)
   out.nbf = j.nbf;
   out.naux = j.naux;
   out.slot = j.slot;
+  out.t_host = j.t_host;
+  out.t_build = j.t_build;
+  out.t_setup = j.t_setup;
+  out.t_scf = j.t_scf;
   out.done = true;
   return true;
 }
@@ -179,7 +183,20 @@ void Server::admit(SrvJob& j)
 {
   std::unique_lock<std::mutex> lk(mu_);
   if (j.est_bytes > budget_) throw std::runtime_error("request does not fit the HBM budget");
-  cv_.wait(lk, [&] { return waiting_.front() == j.id && in_use_ + j.est_bytes <= budget_; });
+  // the estimate must fit the budget next to the resident requests, and the
+  // device must really have it free (estimates are not exact); the free-memory
+  // check is re-polled while waiting
+  const size_t margin = (size_t)4 << 30;
+  auto device_free = [&]() {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return (size_t)0;
+    return fr;
+  };
+  while (true) {
+    cv_.wait(lk, [&] { return waiting_.front() == j.id && in_use_ + j.est_bytes <= budget_; });
+    if (live_ == 0 || device_free() >= j.est_bytes + margin) break;
+    cv_.wait_for(lk, std::chrono::milliseconds(20));
+  }
   waiting_.pop_front();
   in_use_ += j.est_bytes;
   peak_ = std::max(peak_, in_use_);
@@ -229,7 +246,6 @@ void Server::worker(int slot)
       j->failed = true;
       j->error = e.what();
     }
-    alloc_stream() = nullptr;
     finish(*j, admitted);
   }
 }
@@ -237,12 +253,15 @@ void Server::worker(int slot)
 void Server::run(Ctx& c, SrvJob& j, bool& admitted)
 {
   // ---- host side: one-electron integrals, metric, Schwarz mask, estimate
+  const double th0 = now_s();
   const Basis& ao = j.ao;
   const Basis& aux = j.aux;
   const int n = ao.nbf, naux = aux.nbf, nocc = j.nocc;
   j.nbf = n;
   j.naux = naux;
-  std::vector<double> S((size_t)n * n), T((size_t)n * n), V((size_t)n * n), M((size_t)naux * naux);
+  // host arrays that go to the device live in page-locked memory (DMA copies)
+  PinnedVec<double> S((size_t)n * n), T((size_t)n * n), M((size_t)naux * naux);
+  std::vector<double> V((size_t)n * n);
   ints_1e(ao, j.charges, S.data(), T.data(), V.data());
   for (size_t i = 0; i < S.size(); ++i) T[i] += V[i];   // H = T + V
   ints_2c2e(aux, M.data());
@@ -272,25 +291,34 @@ void Server::run(Ctx& c, SrvJob& j, bool& admitted)
                              ? 4.0 * tot + 2.0 * 32.0 * naux * nbb * 8.0 + 8.0 * n * (double)n * ((naux + 31) / 32)
                              : 0.0);
   // FP64 exchange work space (HT1 and HT2 unless B is transformed; the final
-  // exchange of the low-precision modes is the plain FP64 one) and the per-slab
-  // transform buffer of the low-precision copies
-  const double ht = (j.kmode == 1 ? 1.0 : 2.0) * 8.0 * naux * (double)n * std::max(nocc, ldw / 2);
+  // exchange of the low-precision modes is the plain FP64 one, run in occupied
+  // batches of at most 8 GB per work array) and the per-slab transform buffer
+  // of the low-precision copies
+  const double ht_full = 8.0 * naux * (double)n * nocc;
+  j.final_occ_nbatches = j.kmode >= 2 ? std::max(1, (int)std::ceil(ht_full / 8e9)) : 1;
+  const double ht = (j.kmode == 1 ? 1.0 : 2.0) * ht_full / j.final_occ_nbatches +
+                    (j.kmode >= 2 ? 4.0 * naux * (double)n * ldw : 0.0);
   double maxslab = 0.0;
   for (int b = 0; b < L.nbb; ++b) maxslab = std::max(maxslab, (double)(L.slab_off[b + 1] - L.slab_off[b]));
   est += std::max(4.0 * tot, copies + ht + 8.0 * maxslab);   // build scratch vs. resident copies + work
   j.est_bytes = (size_t)est;
+  j.t_host = now_s() - th0;
 
   // ---- admission, then the device part
   admit(j);
   admitted = true;
-  alloc_stream() = c.stream;
   {
+    double tp = now_s();
     DevBT t;
     build_B_device(c, ao, aux, j.ao_blk, j.aux_blk, mask, j.eps, t);
+    j.t_build = now_s() - tp;
+    tp = now_s();
     Metric m;
     metric_create(c, naux, M.data(), m);
     if (j.kmode == 1) transform_B(c, t, m);
     if (j.kmode >= 2) dbt_make_lowp(c, t, m, j.kmode == 2 || j.kmode == 4, j.kmode == 3 || j.kmode == 4);
+    j.t_setup = now_s() - tp;
+    tp = now_s();
     ScfInput in;
     in.nbf = n;
     in.nocc = nocc;
@@ -302,9 +330,10 @@ void Server::run(Ctx& c, SrvJob& j, bool& admitted)
     in.max_iter = j.max_iter;
     in.filter_eps = j.eps;
     KBuildFn kb, kfinal;
-    select_k_builders(c, t, &m, j.kmode, nocc, 1, j.threshold, kb, kfinal);
+    select_k_builders(c, t, &m, j.kmode, nocc, 1, j.threshold, kb, kfinal, j.final_occ_nbatches);
     ScfOutput out;
     scf_run(c, t, t.transformed ? nullptr : &m, in, out, kb, kfinal);
+    j.t_scf = now_s() - tp;
     j.energy = out.energy;
     j.energy_iter = out.energy_iter;
     j.iterations = out.iterations;

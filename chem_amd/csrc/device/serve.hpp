@@ -23,9 +23,11 @@ struct SrvJob {
   int nocc = 0;
   double enuc = 0.0, threshold = 1e-6, eps = 1e-10;
   int kmode = 4, max_iter = 100;
+  int final_occ_nbatches = 1;          // occupied batches of the final FP64 exchange
   // bookkeeping (seconds on the server clock)
   int id = -1, slot = -1;
   double t_submit = 0.0, t_admit = 0.0, t_done = 0.0;
+  double t_host = 0.0, t_build = 0.0, t_setup = 0.0, t_scf = 0.0;   // phase seconds
   size_t est_bytes = 0;
   // result
   bool done = false, failed = false, converged = false;

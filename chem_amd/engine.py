@@ -39,6 +39,10 @@ class Context:
     def set_timing(self, on):
         _lib.check(_lib.lib().mchip_ctx_set_timing(self.h, int(bool(on))))
 
+    def set_escalation(self, rms):
+        """Error RMS at which the escalation K mode switches BF16x3 -> FP32."""
+        _lib.check(_lib.lib().mchip_ctx_set_escalation(self.h, float(rms)))
+
     def phase_times(self):
         """Accumulated seconds: K1, K2, K3, J, K_inc, split, -, -."""
         out = np.zeros(8)
@@ -197,6 +201,32 @@ class DeviceB:
         code = ctypes.c_int()
         _lib.check(_lib.lib().mchip_dfk_bf16_tile(self.h, ng, ctypes.byref(code)))
         return None if code.value == 0 else (code.value // 10000, code.value % 10000)
+
+    def bf16_plan(self, ng):
+        """The autotuned BF16 exchange pipeline for ng occupied columns (None
+        before the first build with that ng)."""
+        plan = np.zeros(8, dtype=np.int32)
+        ms = np.zeros(32, dtype=np.float32)
+        nms = ctypes.c_int()
+        _lib.check(_lib.lib().mchip_dfk_bf16_plan(self.h, ng, _lib.iptr(plan), _lib.fptr(ms), ctypes.byref(nms)))
+        if not plan[7]:
+            return None
+        k = nms.value
+        return dict(gemm_shape=int(plan[0]), bm=int(plan[1]), bn=int(plan[2]), threads=int(plan[3]),
+                    direct_to_lds=bool(plan[4]), stages=int(plan[5]), syrk_variant=int(plan[6]),
+                    gemm_ms=[float(v) for v in ms[:k - 4]], syrk_ms=[float(v) for v in ms[k - 4:k]])
+
+    def bf16_force(self, ng, gemm_shape, syrk_variant):
+        """Use the given BF16 pipeline for ng columns instead of the autotuner's."""
+        _lib.check(_lib.lib().mchip_dfk_bf16_force(self.h, ng, gemm_shape, syrk_variant))
+
+    @staticmethod
+    def bf16_shapes():
+        """[(gemm_shape, direct_to_lds)], number of SYRK variants."""
+        ng, ns = ctypes.c_int(), ctypes.c_int()
+        d = np.zeros(64, dtype=np.int32)
+        _lib.check(_lib.lib().mchip_dfk_bf16_shapes(ctypes.byref(ng), ctypes.byref(ns), _lib.iptr(d)))
+        return [(i, bool(d[i])) for i in range(ng.value)], ns.value
 
     def download(self):
         out = np.zeros(self.nelem)

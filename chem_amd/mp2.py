@@ -21,8 +21,12 @@ def laplace_for(eps, nocc, nlap):
     return w, a, err
 
 
-def sos_mp2(mol, dB, metric, S, C, eps, nlap=5, mo_split=5, filter_eps=1e-10, points=None):
-    """Returns dict(energy (positive sum), scaled (1.3 x), terms, omega, alpha, t_z)."""
+def sos_mp2(mol, dB, metric, S, C, eps, nlap=5, mo_split=5, filter_eps=1e-10, points=None, batched=False):
+    """Returns dict(energy (positive sum), scaled (1.3 x), terms, omega, alpha, t_z, t_prep).
+
+    batched=False: FP64, one launch sequence per Laplace point.  batched=True:
+    the contractions of all points as grouped FP32 launches (FP64
+    accumulation) captured in one hipGraph; t_z is the graph's time."""
     n, nocc = mol.nbf, mol.nocc
     if points is None:
         w, a, _ = laplace_for(eps, nocc, nlap)
@@ -36,9 +40,14 @@ def sos_mp2(mol, dB, metric, S, C, eps, nlap=5, mo_split=5, filter_eps=1e-10, po
     bsz = np.array(mol.b, dtype=np.int32)
     e = ctypes.c_double()
     tz = ctypes.c_double()
+    tp = ctypes.c_double()
     terms = np.zeros(len(w))
-    _lib.check(_lib.lib().mchip_mp2_sos(dB.h, metric.h, n, nocc, _lib.dptr(S), _lib.dptr(C),
-                                        _lib.dptr(eps), len(w), _lib.dptr(w), _lib.dptr(a), len(bsz),
-                                        _lib.iptr(bsz), mo_split, filter_eps, ctypes.byref(e),
-                                        _lib.dptr(terms), ctypes.byref(tz)))
-    return dict(energy=e.value, scaled=C_OS * e.value, terms=terms, omega=w, alpha=a, t_z=tz.value)
+    args = (dB.h, metric.h, n, nocc, _lib.dptr(S), _lib.dptr(C), _lib.dptr(eps), len(w), _lib.dptr(w),
+            _lib.dptr(a), len(bsz), _lib.iptr(bsz), mo_split, filter_eps, ctypes.byref(e), _lib.dptr(terms),
+            ctypes.byref(tz))
+    if batched:
+        _lib.check(_lib.lib().mchip_mp2_sos_batched(*args, ctypes.byref(tp)))
+    else:
+        _lib.check(_lib.lib().mchip_mp2_sos(*args))
+    return dict(energy=e.value, scaled=C_OS * e.value, terms=terms, omega=w, alpha=a, t_z=tz.value,
+                t_prep=tp.value)

@@ -44,14 +44,15 @@ class Server:
         e = ctypes.c_double()
         it = ctypes.c_int()
         conv = ctypes.c_int()
-        t = np.zeros(6)
+        t = np.zeros(10)
         _lib.check(_lib.lib().mchip_server_result(self.h, rid, int(wait), ctypes.byref(done), ctypes.byref(e),
                                                   ctypes.byref(it), ctypes.byref(conv), _lib.dptr(t)))
         if not done.value:
             return None
         return dict(id=rid, energy=e.value, iterations=it.value, converged=bool(conv.value),
                     t_submit=t[0], t_admit=t[1], t_done=t[2], latency=t[2] - t[0],
-                    est_bytes=t[3], slot=int(t[4]), energy_iter=t[5])
+                    est_bytes=t[3], slot=int(t[4]), energy_iter=t[5], t_host=t[6], t_build=t[7],
+                    t_setup=t[8], t_scf=t[9])
 
     def stats(self):
         s = np.zeros(6)

@@ -124,6 +124,10 @@ int mchip_ctx_stream(mchip_ctx c, void** stream);
  * accumulators.  Slots: 0 K1 (B.C), 1 K2 (metric), 2 K3 (SYRK),
  * 3 J total, 4 incremental K, 5 operand splitting. */
 int mchip_ctx_set_timing(mchip_ctx c, int on);
+/* Error RMS below which the escalation K mode (4) switches from the BF16x3 to
+ * the FP32 exchange (the switch point is max(rms, 10 * threshold); default
+ * 1e-4).  No counterpart in the reference. */
+int mchip_ctx_set_escalation(mchip_ctx c, double rms);
 int mchip_ctx_phase_times(mchip_ctx c, double* out, int n);
 
 /* Copy a host B to HBM (FP64).  dbcsr::btensor decompress_init/decompress
@@ -204,6 +208,19 @@ int mchip_dfk_ao(mchip_dbt t, mchip_metric m, const double* P, int form, int occ
  * as column_tile * 10000 + threads_per_workgroup (0 before the first build with
  * that ng).  No counterpart in the reference. */
 int mchip_dfk_bf16_tile(mchip_dbt t, int ng, int* bn);
+/* The full autotuned BF16 pipeline for ng occupied columns: plan[8] = {GEMM
+ * shape index, row tile, column tile, threads, direct-to-LDS (0/1), LDS
+ * buffers or pipeline stages, SYRK variant (0 register-staged, 1-3 direct-to-
+ * LDS with 2-4 stages), 1 if tuned}; ms[] (capacity 32) = the timings of every
+ * GEMM shape then every SYRK variant (-1 = not tried), *nms entries.  No
+ * counterpart in the reference. */
+int mchip_dfk_bf16_plan(mchip_dbt t, int ng, int* plan, float* ms, int* nms);
+/* Fix the BF16 pipeline for ng columns instead of tuning it (direct-to-LDS GEMM
+ * shapes pair with SYRK variants 1-3, the others with 0); shapes: count of GEMM
+ * shapes and SYRK variants and, per GEMM shape, whether it is direct-to-LDS
+ * (direct has room for ngemm entries).  No counterpart in the reference. */
+int mchip_dfk_bf16_force(mchip_dbt t, int ng, int gemm_shape, int syrk_variant);
+int mchip_dfk_bf16_shapes(int* ngemm, int* nsyrk, int* direct);
 
 /* ------------------------------------------------------------------------
  * Restricted Hartree-Fock on the GPU (hf::hfmod::compute, src/hf/hfscf.cpp:
@@ -216,8 +233,9 @@ int mchip_dfk_bf16_tile(mchip_dbt t, int ng, int* bn);
  *        3 = mixed with the BF16x3 exchange in every iteration (needs
  *            mchip_dbt_make_bf16), FP64 exchange for the reported energy,
  *        4 = precision escalation: BF16x3 while the previous iteration's error
- *            RMS exceeds max(1e-4, 10 * threshold), FP32 afterwards, FP64 for
- *            the reported energy (needs both copies).
+ *            RMS exceeds max(r, 10 * threshold) (r = 1e-4 unless set by
+ *            mchip_ctx_set_escalation), FP32 afterwards, FP64 for the reported
+ *            energy (needs both copies).
  * Outputs C (nbf x nbf row-major), orbital energies and per-iteration
  * history (energy, RMS error, J and K seconds) up to max_hist entries.
  * ---------------------------------------------------------------------- */
@@ -244,8 +262,9 @@ int mchip_scf_last_extra(double* energy_iter, double* t_final_k);
  * different requests overlaps on the device.  cpu_threads is split over the
  * workers for the host integrals.
  * kmode as for mchip_scf_run (0-4); screen_eps is the block threshold.
- * result: times = {submit, admit, done (server clock, s), HBM estimate (bytes),
- * worker slot, energy of the last low-precision iteration}; wait = 1 blocks.
+ * result: times[10] = {submit, admit, done (server clock, s), HBM estimate
+ * (bytes), worker slot, energy of the last low-precision iteration, seconds of
+ * host preparation, B build, metric and copies, SCF}; wait = 1 blocks.
  * A failed request reports its error through the return status.
  * stats = {bytes in use, peak bytes, completed, failed, max live requests,
  * server clock}.
@@ -275,6 +294,16 @@ int mchip_mp2_sos(mchip_dbt t, mchip_metric m, int nbf, int nocc, const double* 
                   const double* eps, int nlap, const double* omega, const double* alpha, int nbb,
                   const int* bsz, int mo_split, double filter_eps, double* energy, double* terms,
                   double* t_z);
+/* The same energy with the contractions of all Laplace points (b = B L_o, P_v b,
+ * b3 on B's pattern, Z = sum b3 B) as grouped launches captured in one hipGraph,
+ * FP32 with FP64 accumulation; pseudo-densities, pivoted Cholesky and the
+ * energies stay FP64.  t_z = seconds of the graph launch, t_prep = seconds of
+ * the FP64 per-point set-up and autotuning.  Chosen form of LLMP_FULL_Z over
+ * the Laplace loop of mp_compute.cpp:232-389. */
+int mchip_mp2_sos_batched(mchip_dbt t, mchip_metric m, int nbf, int nocc, const double* S, const double* C,
+                          const double* eps, int nlap, const double* omega, const double* alpha, int nbb,
+                          const int* bsz, int mo_split, double filter_eps, double* energy, double* terms,
+                          double* t_z, double* t_prep);
 
 /* Test hook: one v_mfma_f64_16x16x4_f64 on a single wave.  a_lane/b_lane hold
  * the operand each of the 64 lanes supplies, out[lane*4+e] the accumulator
@@ -283,6 +312,9 @@ int mchip_probe_mfma_f64(const double* a_lane, const double* b_lane, double* out
 /* Test hook: kind 0 = v_mfma_f32_16x16x4_f32 (one float per lane and operand),
  * kind 1 = v_mfma_f32_16x16x32_bf16 (eight values per lane, rounded to bf16). */
 int mchip_probe_mfma_lp(int kind, const float* a_lane, const float* b_lane, float* out);
+/* Test hook: placement of global_load_lds_dwordx4 (64 lanes load src[perm[lane]],
+ * 16 bytes each, to an LDS base); out = the 128 x 16-byte LDS window around it. */
+int mchip_probe_lds_load(const uint32_t* src, const int* perm, uint32_t* out);
 
 #ifdef __cplusplus
 }

@@ -186,6 +186,9 @@ def test_bf16_exchange_accuracy(ctx):
     assert np.array_equal(K16, K16.T)
     bn, nthr = dB.bf16_tile(mol.nocc)
     assert bn in (64, 128, 160, 192, 256, 320) and nthr in (256, 512)
+    plan = dB.bf16_plan(mol.nocc)
+    print("BF16 plan", plan)
+    assert plan is not None and plan["bn"] == bn
     # occupied batches (ragged: 15 = 8 + 7) and determinism
     K16b = dB.dfk_mo_bf16(C, occ_nbatches=2)
     assert _relerr(K16b, K64) < BF16_TOL
@@ -279,3 +282,47 @@ def test_lowp_copies_from_untransformed_B(ctx):
     assert np.array_equal(a.dfk_mo_f32(C), b.dfk_mo_f32(C))
     assert np.array_equal(a.dfk_mo_bf16(C), b.dfk_mo_bf16(C))
     assert _relerr(a.dfk_mo_bf16(C), K0) < BF16_TOL
+
+
+@pytest.mark.parametrize("nocc", [15, 70])
+def test_bf16_every_pipeline(ctx, nocc):
+    """Every BF16 slab-GEMM shape with every matching SYRK variant (register-
+    staged with FP32 W; direct-to-LDS pipelined with bf16 hi/lo W) gives the
+    same exchange as FP64 to the BF16 bar, on a block-sparse B with ragged
+    blocks, an empty slab and partial row / column tiles."""
+    from chem_amd.engine import DeviceB, Metric
+    rng = np.random.default_rng(500 + nocc)
+    bsz = [14, 5, 5, 3, 14, 1, 7, 9]
+    xsz = [23, 70, 9, 40, 17]
+    nbf, naux = sum(bsz), sum(xsz)
+    mask = (rng.random((len(xsz), len(bsz), len(bsz))) < 0.6).astype(np.uint8)
+    mask = np.maximum(mask, mask.transpose(0, 2, 1))
+    mask[:, 5, :] = 0
+    mask[:, :, 5] = 0
+    boff = np.concatenate([[0], np.cumsum(bsz)])
+    xoff = np.concatenate([[0], np.cumsum(xsz)])
+    dense = rng.standard_normal((naux, nbf, nbf))
+    dense = 0.5 * (dense + dense.transpose(0, 2, 1))
+    for x in range(len(xsz)):
+        for a in range(len(bsz)):
+            for b in range(len(bsz)):
+                if not mask[x, a, b]:
+                    dense[xoff[x]:xoff[x + 1], boff[a]:boff[a + 1], boff[b]:boff[b + 1]] = 0
+    Bh = ints.BTensorHost.from_dense(dense, xsz, bsz, mask)
+    A = rng.standard_normal((naux, naux))
+    V = A @ A.T + naux * np.eye(naux)
+    C = rng.standard_normal((nbf, nocc))
+    dB = DeviceB(ctx, Bh)
+    m = Metric(ctx, V)
+    dB.make_lowp(m)
+    K64 = dB.dfk_mo(C, m)
+    shapes, nsyrk = dB.bf16_shapes()
+    for shape, direct in shapes:
+        for v in (range(1, nsyrk) if direct else [0]):
+            print("case: nocc %d gemm shape %d syrk %d" % (nocc, shape, v), flush=True)
+            dB.bf16_force(nocc, shape, v)
+            K = dB.dfk_mo_bf16(C)
+            err = _relerr(K, K64)
+            print("  relerr %.3e" % err)
+            assert err < BF16_TOL
+            assert np.array_equal(K, K.T)

@@ -64,3 +64,18 @@ def test_mfma_lowprec_layout(kind, per):
                 alt[l // 16 + 4 * e, l % 16] = out[l * 4 + e]
         print("kind %d: alternative layout D[lane/16+4e] matches: %s" % (kind, np.array_equal(alt, D)))
     assert np.array_equal(got, D)
+
+
+def test_global_load_lds_placement():
+    """global_load_lds_dwordx4 writes lane i's 16 bytes at LDS base + 16 i (the
+    direct-to-LDS exchange kernels rely on it); nothing outside the 1 KB window
+    is touched."""
+    import ctypes
+    rng = np.random.default_rng(3)
+    src = rng.integers(1, 2 ** 31, size=(64, 4), dtype=np.uint32)
+    perm = rng.permutation(64).astype(np.int32)
+    out = np.zeros((128, 4), dtype=np.uint32)
+    u32 = ctypes.POINTER(ctypes.c_uint32)
+    _lib.check(_lib.lib().mchip_probe_lds_load(src.ctypes.data_as(u32), _lib.iptr(perm), out.ctypes.data_as(u32)))
+    assert np.array_equal(out[:64], np.zeros((64, 4), np.uint32))
+    assert np.array_equal(out[64:], src[perm])

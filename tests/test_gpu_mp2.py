@@ -38,3 +38,47 @@ def test_sos_mp2_vs_oracle(ctx, nwater):
     print("E_gpu %.12f E_oracle %.12f rel %.2e" % (got["energy"], E, abs(got["energy"] - E) / E))
     assert abs(got["energy"] - E) < 1e-9 * E
     assert np.abs(got["terms"] - np.array(terms)).max() < 1e-9 * E
+
+
+# FP32 operands (2^-24 rounding) with FP64-flushed accumulation.  The energy
+# e = tr(Z V^-1 Z V^-1) amplifies the rounding of every aux-indexed FP32
+# intermediate through V^-1 (cond(V) ~ 1e6-1e7 for cc-pVDZ-ri), so the
+# batched energy agrees with the FP64 path to ~2e-5 relative for two waters;
+# the bar is 1e-4 relative (energy and every per-point term).
+BATCH_TOL = 1e-4
+
+
+@pytest.mark.parametrize("nwater,nlap", [(2, 8), (4, 3)])
+def test_batched_mp2_vs_fp64(ctx, nwater, nlap):
+    """All Laplace points in one grouped FP32 hipGraph against the FP64
+    per-point path (same orbitals, points and metric), and against the oracle."""
+    from chem_amd.engine import DeviceB, Metric
+    from chem_amd.mp2 import sos_mp2, laplace_for
+    from chem_amd.scf import run_rhf
+    mol = desc.Molecule(desc.water_cluster(nwater, 9), dfbasis="cc-pvdz-ri")
+    print("case: batched MP2 water n=%d, %d points" % (nwater, nlap), flush=True)
+    S, T, V = ints.int1e(mol)
+    Bh = ints.BTensorHost.build(mol, 1e-10)
+    M = ints.int2c2e(mol)
+    dB = DeviceB(ctx, Bh)
+    met = Metric(ctx, M)
+    # orbitals from the FP64 GPU SCF in the jkfit basis
+    molj = desc.Molecule(desc.water_cluster(nwater, 9))
+    dBj = DeviceB(ctx, ints.BTensorHost.build(molj, 1e-10))
+    metj = Metric(ctx, ints.int2c2e(molj))
+    r = run_rhf(molj, dBj, metj, S, T + V, threshold=1e-9)
+    w, a, _ = laplace_for(r.eps, mol.nocc, nlap)
+    ref = sos_mp2(mol, dB, met, S, r.C, r.eps, points=(w, a))
+    got = sos_mp2(mol, dB, met, S, r.C, r.eps, points=(w, a), batched=True)
+    _, Vinv = met.get()
+    E, terms = omp2.sos_mp2(Bh.dense(), Vinv, S, r.C, r.eps, mol.nocc, w, a, mol.b, Bmask=Bh.mask(),
+                            return_terms=True)
+    rel = abs(got["energy"] - ref["energy"]) / ref["energy"]
+    print("E_fp64 %.12f E_batched %.12f rel %.2e (oracle %.12f); graph %.4f s vs per-point %.4f s" % (
+        ref["energy"], got["energy"], rel, E, got["t_z"], ref["t_z"]))
+    assert abs(ref["energy"] - E) < 1e-9 * E
+    assert rel < BATCH_TOL
+    assert np.abs(got["terms"] - ref["terms"]).max() < BATCH_TOL * ref["energy"]
+    # a second call reproduces the graph's result bit for bit
+    again = sos_mp2(mol, dB, met, S, r.C, r.eps, points=(w, a), batched=True)
+    assert again["energy"] == got["energy"]
