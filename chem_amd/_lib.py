@@ -125,6 +125,8 @@ _SIGS = [
     ("mchip_mp2_sos_batched", c_int, [c_void_p, c_void_p, c_int, c_int, P_double, P_double, P_double, c_int,
                               P_double, P_double, c_int, P_int, c_int, c_double, P_double, P_double,
                               P_double, P_double]),
+    ("mchip_mp2_z", c_int, [c_void_p, c_void_p, c_int, c_int, P_double, P_double, P_double, c_int,
+                            P_double, P_double, c_int, P_int, c_int, c_double, c_int, P_double, P_double]),
     ("mchip_probe_mfma_f64", c_int, [P_double, P_double, P_double]),
     ("mchip_probe_mfma_lp", c_int, [c_int, P_float, P_float, P_float]),
     ("mchip_probe_lds_load", c_int, [ctypes.POINTER(ctypes.c_uint32), P_int, ctypes.POINTER(ctypes.c_uint32)]),

@@ -492,7 +492,7 @@ extern "C" int mchip_scf_run(mchip_dbt t, mchip_metric m, int nbf, int nocc, int
 static int mp2_entry(mchip_dbt t, mchip_metric m, int nbf, int nocc, const double* S, const double* C,
                      const double* eps, int nlap, const double* omega, const double* alpha, int nbb,
                      const int* bsz, int mo_split, double filter_eps, double* energy, double* terms, double* t_z,
-                     double* t_prep, bool batched)
+                     double* t_prep, bool batched, double* Z = nullptr)
 {
   MCHIP_TRY
   need_device(t->ctx);
@@ -512,9 +512,9 @@ static int mp2_entry(mchip_dbt t, mchip_metric m, int nbf, int nocc, const doubl
   std::vector<double> tv;
   double tz = 0.0, tp = 0.0;
   if (batched)
-    *energy = mp2_sos_batched(t->ctx->c, t->t, m->m, in, tv, &tz, &tp);
+    *energy = mp2_sos_batched(t->ctx->c, t->t, m->m, in, tv, &tz, &tp, Z);
   else
-    *energy = mp2_sos(t->ctx->c, t->t, m->m, in, tv, &tz);
+    *energy = mp2_sos(t->ctx->c, t->t, m->m, in, tv, &tz, Z);
   if (terms)
     for (int k = 0; k < nlap && k < (int)tv.size(); ++k) terms[k] = tv[k];
   if (t_z) *t_z = tz;
@@ -539,6 +539,20 @@ extern "C" int mchip_mp2_sos_batched(mchip_dbt t, mchip_metric m, int nbf, int n
 {
   return mp2_entry(t, m, nbf, nocc, S, C, eps, nlap, omega, alpha, nbb, bsz, mo_split, filter_eps, energy, terms,
                    t_z, t_prep, true);
+}
+
+extern "C" int mchip_mp2_z(mchip_dbt t, mchip_metric m, int nbf, int nocc, const double* S, const double* C,
+                           const double* eps, int nlap, const double* omega, const double* alpha, int nbb,
+                           const int* bsz, int mo_split, double filter_eps, int batched, double* Z,
+                           double* terms)
+{
+  if (!Z) {
+    mchip::set_error("mchip_mp2_z: Z must not be null");
+    return MCHIP_ERR;
+  }
+  double e = 0.0;
+  return mp2_entry(t, m, nbf, nocc, S, C, eps, nlap, omega, alpha, nbb, bsz, mo_split, filter_eps, &e, terms,
+                   nullptr, nullptr, batched != 0, Z);
 }
 
 // ---------------------------------------------------------------------------

@@ -140,7 +140,7 @@ const Tile* slab_tiles(Ctx& c, DevBT& t, int ng, int* ntiles, int BN)
   return it->second.as<Tile>();
 }
 
-template <class T>
+template <class T, int FL = 0>
 static void slab_gemm_t(Ctx& c, DevBT& t, const T* data, const T* G, int ldg, int ng, T* out, int ldo,
                         int bn = 0)
 {
@@ -149,12 +149,12 @@ static void slab_gemm_t(Ctx& c, DevBT& t, const T* data, const T* G, int ldg, in
   const Tile* tiles = slab_tiles(c, t, ng, &nt, bn);
   if (nt == 0) return;
   if (bn == 64)
-    hipLaunchKernelGGL((k_slab_gemm<T, SG_BM, 64>), dim3(nt), dim3(256), 0, c.stream, data,
+    hipLaunchKernelGGL((k_slab_gemm<T, SG_BM, 64, FL>), dim3(nt), dim3(256), 0, c.stream, data,
                        t.slab_off.as<int64_t>(), t.ncol.as<int>(), t.bsz.as<int>(), t.boff.as<int>(),
                        t.colmap.as<int>(), t.colmap_off.as<int>(), tiles, t.L.naux, t.L.nbf, G, ldg,
                        ng, out, ldo);
   else
-    hipLaunchKernelGGL((k_slab_gemm<T, SG_BM, 128>), dim3(nt), dim3(256), 0, c.stream, data,
+    hipLaunchKernelGGL((k_slab_gemm<T, SG_BM, 128, FL>), dim3(nt), dim3(256), 0, c.stream, data,
                        t.slab_off.as<int64_t>(), t.ncol.as<int>(), t.bsz.as<int>(), t.boff.as<int>(),
                        t.colmap.as<int>(), t.colmap_off.as<int>(), tiles, t.L.naux, t.L.nbf, G, ldg,
                        ng, out, ldo);
@@ -162,10 +162,13 @@ static void slab_gemm_t(Ctx& c, DevBT& t, const T* data, const T* G, int ldg, in
 }
 
 void slab_gemm_f32(Ctx& c, DevBT& t, const float* data, const float* G, int ldg, int ng, float* out, int ldo,
-                   int bn)
+                   int bn, bool f64_sums)
 {
   if (bn != 0 && bn != 64 && bn != 128) throw std::runtime_error("slab_gemm_f32: column tile must be 64 or 128");
-  slab_gemm_t<float>(c, t, data, G, ldg, ng, out, ldo, bn);
+  if (f64_sums)
+    slab_gemm_t<float, 1>(c, t, data, G, ldg, ng, out, ldo, bn);
+  else
+    slab_gemm_t<float>(c, t, data, G, ldg, ng, out, ldo, bn);
 }
 
 void slab_gemm(Ctx& c, DevBT& t, const double* G, int ldg, int ng, double* out, int ldo,
@@ -554,13 +557,12 @@ static void bf16_layout(Ctx& c, DevBT& t)
 // B, which stays in data64: J, the reported energy and every FP64 exchange
 // keep the reference's formula (explicit V^-1 in J, L^-1 applied per build in
 // DFMO), while the per-iteration exchange reads the FP32 and/or BF16 copies.
-void dbt_make_lowp(Ctx& c, DevBT& t, const Metric& m, bool f32, bool bf16)
+// Per slab: tmp = B_slab (L^-1)^T in FP64 (the slab's rows of L^-1 B), then
+// its FP32 copy into out32 (same slab layout) if not null and its bf16 hi/lo
+// split into t's padded planes if bf16.
+static void tilde_slabs(Ctx& c, DevBT& t, const Metric& m, float* out32, bool bf16)
 {
-  if (t.transformed) throw std::runtime_error("B is transformed: use the copies of the transformed tensor");
-  if (m.naux != t.L.naux) throw std::runtime_error("metric and B have different aux dimensions");
   const BLayout& L = t.L;
-  if (f32) t.data32.alloc((size_t)std::max<int64_t>(L.total, 1) * sizeof(float));
-  if (bf16) bf16_layout(c, t);
   int64_t maxslab = 0;
   for (int n = 0; n < L.nbb; ++n) maxslab = std::max<int64_t>(maxslab, L.slab_off[n + 1] - L.slab_off[n]);
   DevBuf tmp((size_t)std::max<int64_t>(maxslab, 1) * sizeof(double));
@@ -574,9 +576,9 @@ void dbt_make_lowp(Ctx& c, DevBT& t, const Metric& m, bool f32, bool bf16)
                               rocblas_diagonal_non_unit, R, L.naux, &one, m.Linv.as<double>(), L.naux, slab, R,
                               tmp.as<double>(), R));
     const int64_t ne = R * L.naux;
-    if (f32)
+    if (out32)
       hipLaunchKernelGGL(k_to_f32, dim3(gs_blocks(ne)), dim3(256), 0, c.stream, tmp.as<double>(),
-                         t.data32.as<float>() + L.slab_off[n], ne);
+                         out32 + L.slab_off[n], ne);
     if (bf16)
       hipLaunchKernelGGL(k_split_slab, dim3(gs_blocks(ne)), dim3(256), 0, c.stream, tmp.as<double>(), n,
                          t.ncol.as<int>(), t.bsz.as<int>(), L.naux, t.ldp.as<int>(), t.off16.as<int64_t>(),
@@ -584,7 +586,23 @@ void dbt_make_lowp(Ctx& c, DevBT& t, const Metric& m, bool f32, bool bf16)
     HIP_CHECK(hipGetLastError());
   }
   HIP_CHECK(hipStreamSynchronize(c.stream));
+}
+
+void dbt_make_lowp(Ctx& c, DevBT& t, const Metric& m, bool f32, bool bf16)
+{
+  if (t.transformed) throw std::runtime_error("B is transformed: use the copies of the transformed tensor");
+  if (m.naux != t.L.naux) throw std::runtime_error("metric and B have different aux dimensions");
+  if (f32) t.data32.alloc((size_t)std::max<int64_t>(t.L.total, 1) * sizeof(float));
+  if (bf16) bf16_layout(c, t);
+  tilde_slabs(c, t, m, f32 ? t.data32.as<float>() : nullptr, bf16);
   t.lowp_tilde = true;
+}
+
+void tilde_to_f32(Ctx& c, DevBT& t, const Metric& m, float* out)
+{
+  if (t.transformed) throw std::runtime_error("B is transformed: the FP32 copy is made from the untransformed B");
+  if (m.naux != t.L.naux) throw std::runtime_error("metric and B have different aux dimensions");
+  tilde_slabs(c, t, m, out, false);
 }
 
 void dbt_make_bf16(Ctx& c, DevBT& t)

@@ -11,9 +11,10 @@ void dbt_upload(Ctx& c, const BLayout& L, const double* host_data, DevBT& t);
 void metric_create(Ctx& c, int naux, const double* V_host, Metric& m);
 void transform_B(Ctx& c, DevBT& t, const Metric& m);
 // FP32 slab GEMM on an FP32 copy of the slabs (data), column tile bn (64 or
-// 128; 0 picks by ng).
+// 128; 0 picks by ng).  f64_sums: FP64 running sums of the FP32 MFMA
+// accumulators, flushed every 16 products.
 void slab_gemm_f32(Ctx& c, DevBT& t, const float* data, const float* G, int ldg, int ng, float* out, int ldo,
-                   int bn);
+                   int bn, bool f64_sums = false);
 void slab_gemm(Ctx& c, DevBT& t, const double* G, int ldg, int ng, double* out, int ldo,
                double alpha);
 void seg_syrk(Ctx& c, const double* A, const double* B, int nseg, int64_t sstride, int lda,
@@ -30,6 +31,9 @@ void dbt_make_f32(Ctx& c, DevBT& t);
 void dfk_mo_f32(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K, int occ_nbatches);
 void dbt_make_bf16(Ctx& c, DevBT& t);
 void dbt_make_lowp(Ctx& c, DevBT& t, const Metric& m, bool f32, bool bf16);
+// FP32 copy of L^-1 B (made slab by slab in FP64 from the untransformed B)
+// into out (L.total floats, slab layout); t is left unchanged.
+void tilde_to_f32(Ctx& c, DevBT& t, const Metric& m, float* out);
 void dfk_mo_bf16(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K, int occ_nbatches);
 int dfk_bf16_tuned_bn(const DevBT& t, int ng);
 void dfk_bf16_plan(const DevBT& t, int ng, int* plan, float* ms, int* nms);

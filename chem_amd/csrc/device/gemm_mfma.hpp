@@ -41,6 +41,21 @@ struct Mfma<float> {
 
 constexpr int GM_BK = 16;
 
+// Adds FP32 MFMA accumulators into FP64 running sums and clears them: called
+// after every BK step, no FP32 sum runs over more than GM_BK products.
+template <int TM, int TN>
+__device__ __forceinline__ void flush_f64(Mfma<float>::v4 (&acc)[TM][TN], double (&acc64)[TM][TN][4])
+{
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc64[i][j][e] += (double)acc[i][j][e];
+      acc[i][j] = Mfma<float>::v4{0.f, 0.f, 0.f, 0.f};
+    }
+}
+
 // One BK step of MFMAs on LDS tiles As[k][m], Bs[k][n] (leading dims LDA, LDB).
 template <class T, int TM, int TN, int LDA, int LDB>
 __device__ __forceinline__ void mfma_step(const T* As, const T* Bs, int wm0, int wn0, int lane,

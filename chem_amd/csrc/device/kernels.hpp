@@ -11,7 +11,8 @@ namespace mchip {
 constexpr int SG_BM = 128;    // rows per slab-GEMM / SYRK tile
 constexpr int J_ROWS = 16;    // aux rows per block in k_j_gamma
 
-template <class T, int BM, int BN>
+// FL = 1 (FP32 only): FP64 running sums, flushed after every 16-product step.
+template <class T, int BM, int BN, int FL = 0>
 __global__ void k_slab_gemm(const T* bdata, const int64_t* slab_off, const int* ncol, const int* bsz,
                             const int* boff, const int* colmap, const int* colmap_off,
                             const Tile* tiles, int naux, int nbf, const T* G, int ldg, int ng, T* out,
@@ -26,8 +27,8 @@ __global__ void k_slab_out_gemm(const T* A, int lda, int klen, const int64_t* sl
                                 const T* G, int ldg, T* out, const int* pcol, const int* pwidth,
                                 int64_t out_stride);
 template <int BM, int BN>
-__global__ void k_group_apply_f32(const float* P, const float* in, float* out, int n, int R, const int* pcol,
-                                  const int* pwidth, const int4* tiles);
+__global__ void k_group_apply_f32(const float* P, const float* Plo, const float* in, float* out, int n, int R,
+                                  const int* pcol, const int* pwidth, const int4* tiles);
 __global__ void k_slabseg_f32(const float* A, int64_t a_stride, const float* B, const int64_t* seg_off,
                               const int64_t* seg_len, const int* chunk_seg, int nchunk, int n,
                               const int2* tiles, int ntile, double* partial);
@@ -56,6 +57,7 @@ __global__ void k_fill(double* p, int64_t n, double v);
 __global__ void k_axpby(double* y, const double* x, int64_t n, double a, double b);
 
 __global__ void k_to_f32(const double* in, float* out, int64_t n);
+__global__ void k_to_f32x2(const double* in, float* hi, float* lo, int64_t n);
 __global__ void k_rows_to_f32(const double* in, int ld64, int rows, int ncols, float* out);
 
 // BF16 split-precision exchange (kernels_bf16.hip)

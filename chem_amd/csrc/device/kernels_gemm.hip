@@ -13,7 +13,7 @@
 
 namespace mchip {
 
-template <class T, int BM, int BN>
+template <class T, int BM, int BN, int FL>
 __global__ __launch_bounds__(256) void k_slab_gemm(
     const T* __restrict__ bdata, const int64_t* __restrict__ slab_off, const int* __restrict__ ncol,
     const int* __restrict__ bsz, const int* __restrict__ boff, const int* __restrict__ colmap,
@@ -39,6 +39,8 @@ __global__ __launch_bounds__(256) void k_slab_gemm(
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = typename Mfma<T>::v4{0, 0, 0, 0};
+  static_assert(FL == 0 || sizeof(T) == 4, "FP64 flushing applies to the FP32 kernel");
+  double acc64[TM][TN][4] = {};
   const int ak = tid & 15, ar = tid >> 4;
   const int bk = tid >> 4, bc = tid & 15;
   T ra[RA], rb[RB];
@@ -73,6 +75,7 @@ __global__ __launch_bounds__(256) void k_slab_gemm(
     const int cur = s & 1;
     if (s + 1 < nsteps) load((s + 1) * GM_BK);
     mfma_step<T, TM, TN, LDA, LDB>(As[cur], Bs[cur], wm0, wn0, lane, acc);
+    if constexpr (FL > 0) flush_f64<TM, TN>(acc, acc64);
     if (s + 1 < nsteps) store(cur ^ 1);
     __syncthreads();
   }
@@ -87,7 +90,10 @@ __global__ __launch_bounds__(256) void k_slab_gemm(
         if (r < rows && c < ng) {
           const int64_t x = r / bn;
           const int nu = (int)(r - x * bn);
-          out[(x * nbf + boff[N] + nu) * (int64_t)ldo + c] = acc[i][j][e];
+          if constexpr (FL > 0)
+            out[(x * nbf + boff[N] + nu) * (int64_t)ldo + c] = (T)acc64[i][j][e];
+          else
+            out[(x * nbf + boff[N] + nu) * (int64_t)ldo + c] = acc[i][j][e];
         }
       }
 }
@@ -261,6 +267,10 @@ template __global__ void k_slab_gemm<float, 128, 128>(const float*, const int64_
     const int*, const int*, const int*, const Tile*, int, int, const float*, int, int, float*, int);
 template __global__ void k_slab_gemm<float, 128, 64>(const float*, const int64_t*, const int*, const int*,
     const int*, const int*, const int*, const Tile*, int, int, const float*, int, int, float*, int);
+template __global__ void k_slab_gemm<float, 128, 128, 1>(const float*, const int64_t*, const int*, const int*,
+    const int*, const int*, const int*, const Tile*, int, int, const float*, int, int, float*, int);
+template __global__ void k_slab_gemm<float, 128, 64, 1>(const float*, const int64_t*, const int*, const int*,
+    const int*, const int*, const int*, const Tile*, int, int, const float*, int, int, float*, int);
 template __global__ void k_seg_gemm<double, 128, 128>(const double*, const double*, int, int64_t, int, int,
     int, const int2*, int, int, double*);
 template __global__ void k_seg_gemm<float, 128, 128>(const float*, const float*, int, int64_t, int, int,
@@ -310,6 +320,8 @@ __global__ __launch_bounds__(256) void k_slab_out_gemm(
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = typename Mfma<T>::v4{0, 0, 0, 0};
+  constexpr bool F64 = sizeof(T) == 4;   // FP32: FP64 running sums, flushed every step
+  double acc64[TM][TN][4] = {};
   const int lk = tid & 15, lr = tid >> 4;
   int64_t arow[RA];
   int grow[RB];
@@ -353,6 +365,7 @@ __global__ __launch_bounds__(256) void k_slab_out_gemm(
     const int cur = s & 1;
     if (s + 1 < nsteps) load((s + 1) * GM_BK);
     mfma_step<T, TM, TN, LDA, LDB>(As[cur], Bs[cur], wm0, wn0, lane, acc);
+    if constexpr (F64) flush_f64<TM, TN>(acc, acc64);
     if (s + 1 < nsteps) store(cur ^ 1);
     __syncthreads();
   }
@@ -365,7 +378,12 @@ __global__ __launch_bounds__(256) void k_slab_out_gemm(
       for (int e = 0; e < 4; ++e) {
         const int64_t r = (int64_t)t.m0 + wm0 + 16 * i + Mfma<T>::row(lane, e);
         const int c = t.n0 + wn0 + 16 * j + (lane & 15);
-        if (r < rows && c < nc) o[r * nc + c] = acc[i][j][e];
+        if (r < rows && c < nc) {
+          if constexpr (F64)
+            o[r * nc + c] = (T)acc64[i][j][e];
+          else
+            o[r * nc + c] = acc[i][j][e];
+        }
       }
 }
 
@@ -467,8 +485,12 @@ __global__ __launch_bounds__(256) void k_slabseg_gemm(
 //   out[X][m][pcol[g] + j] = sum_v P[g][m][v] in[X][v][pcol[g] + j],  j < pwidth[g]
 // P: n x n row-major per group, in/out: [naux][n][R] row-major.  The virtual
 // pseudo-density applied to b for every X and Laplace point (z_full.cpp:224-240).
+// FP32 MFMA with FP64 running sums flushed after every 16-product step.  With
+// Plo (P = P + Plo as an FP32 pair) the contraction runs over both halves.
 template <int BM, int BN>
-__global__ __launch_bounds__(256) void k_group_apply_f32(const float* __restrict__ P, const float* __restrict__ in,
+__global__ __launch_bounds__(256) void k_group_apply_f32(const float* __restrict__ P,
+                                                        const float* __restrict__ Plo,
+                                                        const float* __restrict__ in,
                                                         float* __restrict__ out, int n, int R,
                                                         const int* __restrict__ pcol,
                                                         const int* __restrict__ pwidth,
@@ -481,7 +503,8 @@ __global__ __launch_bounds__(256) void k_group_apply_f32(const float* __restrict
   __shared__ float Bs[2][GM_BK * LDB];
   const int4 t = tiles[blockIdx.x];   // (group, X, m0, n0)
   const int g = t.x;
-  const float* A = P + (int64_t)g * n * n;
+  const float* Ahi = P + (int64_t)g * n * n;
+  const float* Alo = Plo ? Plo + (int64_t)g * n * n : nullptr;
   const float* Bg = in + (int64_t)t.y * n * R + pcol[g];
   float* Cg = out + (int64_t)t.y * n * R + pcol[g];
   const int w = pwidth[g];
@@ -492,10 +515,16 @@ __global__ __launch_bounds__(256) void k_group_apply_f32(const float* __restrict
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = Mfma<float>::v4{0.f, 0.f, 0.f, 0.f};
+  double acc64[TM][TN][4] = {};
   const int ak = tid & 15, ar = tid >> 4;
   const int bk = tid >> 4, bc = tid & 15;
   float ra[RA], rb[RB];
-  auto load = [&](int k0) {
+  // steps [0, nh) contract with P (hi part), steps [nh, 2 nh) with Plo
+  const int nh = (n + GM_BK - 1) / GM_BK;
+  const int nsteps = Alo ? 2 * nh : nh;
+  auto load = [&](int s) {
+    const float* A = (s < nh) ? Ahi : Alo;
+    const int k0 = (s < nh ? s : s - nh) * GM_BK;
 #pragma unroll
     for (int j = 0; j < RA; ++j) {
       const int m = t.z + ar + 16 * j, kk = k0 + ak;
@@ -513,7 +542,6 @@ __global__ __launch_bounds__(256) void k_group_apply_f32(const float* __restrict
 #pragma unroll
     for (int j = 0; j < RB; ++j) Bs[buf][bk * LDB + bc + 16 * j] = rb[j];
   };
-  const int nsteps = (n + GM_BK - 1) / GM_BK;
   if (nsteps > 0) {
     load(0);
     store(0);
@@ -521,8 +549,9 @@ __global__ __launch_bounds__(256) void k_group_apply_f32(const float* __restrict
   __syncthreads();
   for (int s = 0; s < nsteps; ++s) {
     const int cur = s & 1;
-    if (s + 1 < nsteps) load((s + 1) * GM_BK);
+    if (s + 1 < nsteps) load(s + 1);
     mfma_step<float, TM, TN, LDA, LDB>(As[cur], Bs[cur], wm0, wn0, lane, acc);
+    flush_f64<TM, TN>(acc, acc64);
     if (s + 1 < nsteps) store(cur ^ 1);
     __syncthreads();
   }
@@ -534,12 +563,12 @@ __global__ __launch_bounds__(256) void k_group_apply_f32(const float* __restrict
       for (int e = 0; e < 4; ++e) {
         const int r = t.z + wm0 + 16 * i + Mfma<float>::row(lane, e);
         const int c = t.w + wn0 + 16 * j + (lane & 15);
-        if (r < n && c < w) Cg[(int64_t)r * R + c] = acc[i][j][e];
+        if (r < n && c < w) Cg[(int64_t)r * R + c] = (float)acc64[i][j][e];
       }
 }
 
 // Grouped slab-segment SYRK in FP32 with FP64-flushed accumulators (8 waves,
-// 64 x 32 each, flush every 16 steps):
+// 64 x 32 each, flush every 4 steps = 64 products):
 //   partial[g][chunk](m,n) = sum_{slabs s in chunk} sum_k A_g(off_s + m*len_s + k) B(off_s + n*len_s + k)
 // with A_g = A + g * a_stride.  Z_g(X,Y) = sum b3_g(X|mu nu) B(Y|mu nu) for all
 // Laplace points g in one launch (z_full.cpp:373-377).
@@ -550,7 +579,7 @@ __global__ __launch_bounds__(512) void k_slabseg_f32(const float* __restrict__ A
                                                      const int2* __restrict__ tiles, int ntile,
                                                      double* __restrict__ partial)
 {
-  constexpr int BM = 128, TM = 4, TN = 2, LDA = BM + 16, RA = BM / 32, FLUSH = 16;
+  constexpr int BM = 128, TM = 4, TN = 2, LDA = BM + 16, RA = BM / 32, FLUSH = 4;
   __shared__ float As[2][GM_BK * LDA];
   __shared__ float Bs[2][GM_BK * LDA];
   const int2 tl = tiles[blockIdx.x % ntile];
@@ -649,8 +678,8 @@ __global__ __launch_bounds__(512) void k_slabseg_f32(const float* __restrict__ A
       }
 }
 
-template __global__ void k_group_apply_f32<128, 64>(const float*, const float*, float*, int, int, const int*,
-                                                    const int*, const int4*);
+template __global__ void k_group_apply_f32<128, 64>(const float*, const float*, const float*, float*, int, int,
+                                                    const int*, const int*, const int4*);
 
 template __global__ void k_slab_out_gemm<double, 128, 128>(const double*, int, int, const int64_t*,
     const int*, const int*, const int*, const int*, const int*, const Tile*, int, int, const double*, int,

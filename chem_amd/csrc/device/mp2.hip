@@ -345,7 +345,7 @@ static double mp2_point_energy(Ctx& c, const Metric& met, const double* Z, DevBu
 }
 
 double mp2_sos(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in, std::vector<double>& terms,
-               double* t_z)
+               double* t_z, double* Z_host)
 {
   const BLayout& L = bt.L;
   if (bt.transformed) throw std::runtime_error("MP2 needs the untransformed three-centre tensor");
@@ -363,6 +363,7 @@ double mp2_sos(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in, std::ve
     DevBuf Lo, Pv;
     const int rank = mp2_point_factors(c, s, in, k, Lo, Pv);
     if (rank == 0) {
+      if (Z_host) std::fill(Z_host + k * (size_t)naux * naux, Z_host + (k + 1) * (size_t)naux * naux, 0.0);
       terms.push_back(0.0);
       continue;
     }
@@ -387,6 +388,7 @@ double mp2_sos(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in, std::ve
     float ms = 0.f;
     HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
     tz += ms * 1e-3;
+    if (Z_host) d2h(Z_host + k * (size_t)naux * naux, Zm.p, (size_t)naux * naux * sizeof(double), c.stream);
     const double e = mp2_point_energy(c, met, Zm.as<double>(), Zt, part);
     terms.push_back(e);
     total += e;
@@ -411,16 +413,23 @@ __global__ void k_cols_to_f32(const double* __restrict__ in, int rows, int w, fl
 // of mp_compute.cpp:232-389).  The pseudo-densities and their pivoted
 // Cholesky factors are made per point in FP64; then the four contractions of
 // all points go out as grouped launches captured in one hipGraph, in FP32
-// with FP64 accumulation:
-//   M2  b[X][nu][i]     = sum_mu B(X|mu nu) L_all(mu, i)   one slab GEMM over the
+// with FP64 accumulation, on B~ = L^-1 B (V = L L^T; FP32 copy made slab by
+// slab in FP64 from B):
+//   M2  b[X][nu][i]     = sum_mu B~(X|mu nu) L_all(mu, i)  one slab GEMM over the
 //                                                         concatenated columns of all points
-//   M3  b2_g[X]         = P_v,g b_g[X]                     grouped over (point, X)
+//   M3  b2_g[X]         = P_v,g b_g[X]                     grouped over (point, X), P_v as an FP32 hi/lo pair
 //   M4  b3_g(X|M N)     = sum_i L_g(M,i) b2_g(X,i,N)        grouped over points, B's pattern
-//   M5  Z_g(X,Y)        = sum b3_g(X|mu nu) B(Y|mu nu)       grouped over points, FP64-flushed
-// and the energies e_g = tr(Z~_g Z~_g), Z~ = V^-1 Z_g, in FP64 after the graph.
-// The column tile of M2 is picked by timing both candidates on the live data.
+//   M5  Z'_g(X,Y)       = sum b3_g(X|mu nu) B~(Y|mu nu)      grouped over points, FP64-flushed
+// so Z'_g = L^-1 Z_g L^-T and e_g = tr(Z_g V^-1 Z_g V^-1) = sum_XY Z'_g(X,Y)^2
+// (FP64, after the graph).  The reference's form applies V^-1 after Z: Z then
+// carries the small-eigenvalue directions of V that V^-1 amplifies, and the
+// rounding of FP32 products alone moves e_g by ~1e-5 relative; in the
+// orthonormal auxiliary basis e_g is a well-conditioned function of Z'.
+// Every FP32 MFMA chain is cut into FP64 running sums after 16 products
+// (64 in M5).  The column tile of M2 is picked by timing both candidates on
+// the live data.
 double mp2_sos_batched(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in, std::vector<double>& terms,
-                       double* t_z, double* t_prep)
+                       double* t_z, double* t_prep, double* Z_host)
 {
   const BLayout& L = bt.L;
   if (bt.transformed) throw std::runtime_error("MP2 needs the untransformed three-centre tensor");
@@ -439,25 +448,26 @@ double mp2_sos_batched(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in,
   const int R = pcol[npt];
   terms.assign(npt, 0.0);
   if (R == 0) {
+    if (Z_host) std::fill(Z_host, Z_host + (size_t)npt * naux * naux, 0.0);
     if (t_z) *t_z = 0.0;
     if (t_prep) *t_prep = 0.0;
     return 0.0;
   }
-  // FP32 operands: all L_o side by side (n x R), the P_v (n x n each), B
+  // FP32 operands: all L_o side by side (n x R), the P_v (n x n each), B~
   const size_t nn = (size_t)n * n;
-  DevBuf Lall((size_t)n * R * sizeof(float)), Pv32((size_t)npt * nn * sizeof(float)),
+  // P_v as an FP32 pair (hi at [g], lo at [npt + g]): P_v b cancels heavily,
+  // and a single FP32 rounding of P_v alone costs ~1e-5 of max|Z|
+  DevBuf Lall((size_t)n * R * sizeof(float)), Pv32((size_t)2 * npt * nn * sizeof(float)),
       B32((size_t)std::max<int64_t>(L.total, 1) * sizeof(float));
   for (int g = 0; g < npt; ++g) {
     if (rank[g] == 0) continue;
     hipLaunchKernelGGL(k_cols_to_f32, dim3(nblk((int64_t)n * rank[g], 256)), dim3(256), 0, c.stream,
                        Lo[g].as<double>(), n, rank[g], Lall.as<float>(), R, pcol[g]);
-    hipLaunchKernelGGL(k_to_f32, dim3(gs_blocks((int64_t)nn)), dim3(256), 0, c.stream, Pv[g].as<double>(),
-                       Pv32.as<float>() + (size_t)g * nn, (int64_t)nn);
+    hipLaunchKernelGGL(k_to_f32x2, dim3(gs_blocks((int64_t)nn)), dim3(256), 0, c.stream, Pv[g].as<double>(),
+                       Pv32.as<float>() + (size_t)g * nn, Pv32.as<float>() + (size_t)(npt + g) * nn, (int64_t)nn);
   }
-  if (L.total)
-    hipLaunchKernelGGL(k_to_f32, dim3(gs_blocks(L.total)), dim3(256), 0, c.stream, bt.data64.as<double>(),
-                       B32.as<float>(), (int64_t)L.total);
   HIP_CHECK(hipGetLastError());
+  if (L.total) tilde_to_f32(c, bt, met, B32.as<float>());
   const size_t bsize = (size_t)naux * n * R;
   DevBuf bb(bsize * sizeof(float)), b2(bsize * sizeof(float)),
       b3((size_t)npt * std::max<int64_t>(L.total, 1) * sizeof(float));
@@ -510,9 +520,9 @@ double mp2_sos_batched(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in,
   {
     float best = 1e30f;
     for (int cand : {64, 128}) {
-      slab_gemm_f32(c, bt, B32.as<float>(), Lall.as<float>(), R, R, bb.as<float>(), R, cand);
+      slab_gemm_f32(c, bt, B32.as<float>(), Lall.as<float>(), R, R, bb.as<float>(), R, cand, true);
       HIP_CHECK(hipEventRecord(c.ev0, c.stream));
-      slab_gemm_f32(c, bt, B32.as<float>(), Lall.as<float>(), R, R, bb.as<float>(), R, cand);
+      slab_gemm_f32(c, bt, B32.as<float>(), Lall.as<float>(), R, R, bb.as<float>(), R, cand, true);
       HIP_CHECK(hipEventRecord(c.ev1, c.stream));
       HIP_CHECK(hipEventSynchronize(c.ev1));
       float ms = 0.f;
@@ -529,9 +539,10 @@ double mp2_sos_batched(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in,
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-  slab_gemm_f32(c, bt, B32.as<float>(), Lall.as<float>(), R, R, bb.as<float>(), R, bn);
+  slab_gemm_f32(c, bt, B32.as<float>(), Lall.as<float>(), R, R, bb.as<float>(), R, bn, true);
   hipLaunchKernelGGL((k_group_apply_f32<128, 64>), dim3((unsigned)t3.size()), dim3(256), 0, c.stream,
-                     Pv32.as<float>(), bb.as<float>(), b2.as<float>(), n, R, d_pcol.as<int>(), d_pw.as<int>(),
+                     Pv32.as<float>(), Pv32.as<float>() + (size_t)npt * nn, bb.as<float>(), b2.as<float>(), n, R,
+                     d_pcol.as<int>(), d_pw.as<int>(),
                      d_t3.as<int4>());
   hipLaunchKernelGGL((k_slab_out_gemm<float, SG_BM, 128>), dim3((unsigned)t4.size()), dim3(256), 0, c.stream,
                      b2.as<float>(), R, 0, bt.slab_off.as<int64_t>(), bt.ncol.as<int>(), bt.bsz.as<int>(),
@@ -557,13 +568,42 @@ double mp2_sos_batched(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in,
   if (t_z) *t_z = ms * 1e-3;
   (void)hipGraphExecDestroy(exec);
   (void)hipGraphDestroy(graph);
-  // M6 per point in FP64
-  DevBuf Zt((size_t)naux * naux * sizeof(double)), part(256 * sizeof(double));
+  // e_g = sum Z'_g(X,Y)^2 in FP64; Z_g = L Z'_g L^T only when asked for
+  DevBuf part(256 * sizeof(double)), Lf, Zg64;
+  if (Z_host) {
+    Lf.alloc((size_t)naux * naux * sizeof(double));
+    Zg64.alloc((size_t)naux * naux * sizeof(double));
+    HIP_CHECK(hipMemcpyAsync(Lf.p, met.Linv.p, (size_t)naux * naux * sizeof(double), hipMemcpyDeviceToDevice,
+                             c.stream));
+    RB_CHECK(rocsolver_dtrtri(c.blas, rocblas_fill_lower, rocblas_diagonal_non_unit, naux, Lf.as<double>(), naux,
+                              c.info.as<int>()));
+  }
   double total = 0.0;
   for (int g = 0; g < npt; ++g) {
-    if (rank[g] == 0) continue;
-    terms[g] = mp2_point_energy(c, met, Z.as<double>() + (size_t)g * naux * naux, Zt, part);
-    total += terms[g];
+    double* Zg = Z_host ? Z_host + (size_t)g * naux * naux : nullptr;
+    if (rank[g] == 0) {
+      if (Zg) std::fill(Zg, Zg + (size_t)naux * naux, 0.0);
+      continue;
+    }
+    const double* Zp = Z.as<double>() + (size_t)g * naux * naux;
+    hipLaunchKernelGGL(k_trace_sq, dim3(256), dim3(256), 0, c.stream, Zp, naux, part.as<double>());
+    double hpart[256];
+    d2h(hpart, part.p, sizeof(hpart), c.stream);
+    double e = 0.0;
+    for (double v : hpart) e += v;
+    terms[g] = e;
+    total += e;
+    if (Zg) {
+      // Z = L Z' L^T (symmetric, so row/column order does not matter)
+      const double one = 1.0;
+      RB_CHECK(rocblas_dtrmm(c.blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                             rocblas_diagonal_non_unit, naux, naux, &one, Lf.as<double>(), naux, Zp, naux,
+                             Zg64.as<double>(), naux));
+      RB_CHECK(rocblas_dtrmm(c.blas, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                             rocblas_diagonal_non_unit, naux, naux, &one, Lf.as<double>(), naux,
+                             Zg64.as<double>(), naux, Zg64.as<double>(), naux));
+      d2h(Zg, Zg64.p, (size_t)naux * naux * sizeof(double), c.stream);
+    }
   }
   return total;
 }

@@ -25,15 +25,17 @@ struct Mp2Input {
 std::vector<double> pivoted_cholesky_dev(Ctx& c, DevBuf& U, int N, double filter_eps, int* rank_out);
 
 // Returns E = sum_k e_k (positive, unscaled); terms[k] = e_k; t_z = seconds
-// spent in the Z builder (M2-M5) over all points.
+// spent in the Z builder (M2-M5) over all points.  Z_host, if not null,
+// receives Z_k(X,Y) of every point (nlap x naux x naux, zero for a point whose
+// occupied pseudo-density has rank 0).
 double mp2_sos(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in, std::vector<double>& terms,
-               double* t_z);
+               double* t_z, double* Z_host = nullptr);
 
 // Same energy with the contractions of all Laplace points grouped and captured
 // in one hipGraph, FP32 with FP64 accumulation.  t_z = seconds of the graph
 // launch (M2-M5 of every point), t_prep = seconds of the FP64 per-point set-up
 // (pseudo-densities, pivoted Cholesky, FP32 copies, autotuning).
 double mp2_sos_batched(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in, std::vector<double>& terms,
-                       double* t_z, double* t_prep);
+                       double* t_z, double* t_prep, double* Z_host = nullptr);
 
 }  // namespace mchip

@@ -51,3 +51,22 @@ def sos_mp2(mol, dB, metric, S, C, eps, nlap=5, mo_split=5, filter_eps=1e-10, po
         _lib.check(_lib.lib().mchip_mp2_sos(*args))
     return dict(energy=e.value, scaled=C_OS * e.value, terms=terms, omega=w, alpha=a, t_z=tz.value,
                 t_prep=tp.value)
+
+
+def z_builder(mol, dB, metric, S, C, eps, omega, alpha, mo_split=5, filter_eps=1e-10, batched=False):
+    """Z_k(X,Y) of every Laplace point (nlap x naux x naux) and the terms e_k:
+    the tensor LLMP_FULL_Z hands to the energy loop (z_full.cpp:373-377,
+    mp_compute.cpp:361-389).  batched selects the grouped FP32 graph."""
+    w = np.ascontiguousarray(omega, dtype=np.float64)
+    a = np.ascontiguousarray(alpha, dtype=np.float64)
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    C = np.ascontiguousarray(C, dtype=np.float64)
+    eps = np.ascontiguousarray(eps, dtype=np.float64)
+    bsz = np.array(mol.b, dtype=np.int32)
+    naux = dB.naux
+    Z = np.zeros((len(w), naux, naux))
+    terms = np.zeros(len(w))
+    _lib.check(_lib.lib().mchip_mp2_z(dB.h, metric.h, mol.nbf, mol.nocc, _lib.dptr(S), _lib.dptr(C), _lib.dptr(eps),
+                                      len(w), _lib.dptr(w), _lib.dptr(a), len(bsz), _lib.iptr(bsz), mo_split,
+                                      filter_eps, int(bool(batched)), _lib.dptr(Z), _lib.dptr(terms)))
+    return Z, terms

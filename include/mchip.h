@@ -304,6 +304,15 @@ int mchip_mp2_sos_batched(mchip_dbt t, mchip_metric m, int nbf, int nocc, const 
                           const double* eps, int nlap, const double* omega, const double* alpha, int nbb,
                           const int* bsz, int mo_split, double filter_eps, double* energy, double* terms,
                           double* t_z, double* t_prep);
+/* The Z builder alone: Z_k(X,Y) = sum_{mu nu} b3_k(X|mu nu) B(Y|mu nu) of every
+ * Laplace point k into Z (nlap x naux x naux doubles, row-major; zero for a
+ * point whose occupied pseudo-density has rank 0), with terms[k] = e_k as
+ * above.  batched = 0: the FP64 per-point path of mchip_mp2_sos; 1: the
+ * grouped FP32 graph of mchip_mp2_sos_batched.  The tensor LLMP_FULL_Z
+ * produces per point (z_full.cpp:373-377, consumed at mp_compute.cpp:361-389). */
+int mchip_mp2_z(mchip_dbt t, mchip_metric m, int nbf, int nocc, const double* S, const double* C,
+                const double* eps, int nlap, const double* omega, const double* alpha, int nbb, const int* bsz,
+                int mo_split, double filter_eps, int batched, double* Z, double* terms);
 
 /* Test hook: one v_mfma_f64_16x16x4_f64 on a single wave.  a_lane/b_lane hold
  * the operand each of the 64 lanes supplies, out[lane*4+e] the accumulator

@@ -114,8 +114,9 @@ def shellpair_mask(Bmask, bsz):
 
 
 def sos_mp2(B, Vinv, S, C, eps, nocc, omega, alpha, bsz, mo_split=5, filter_eps=1e-10,
-            Bmask=None, return_terms=False):
-    """mpmod::compute.  Returns the positive sum E = sum_k e_k (unscaled)."""
+            Bmask=None, return_terms=False, return_z=False):
+    """mpmod::compute.  Returns the positive sum E = sum_k e_k (unscaled);
+    with return_terms also the e_k, with return_z also the Z_k of every point."""
     Co, Cv = C[:, :nocc], C[:, nocc:]
     eo, ev = eps[:nocc], eps[nocc:]
     LS = np.linalg.cholesky(S)
@@ -123,7 +124,7 @@ def sos_mp2(B, Vinv, S, C, eps, nocc, omega, alpha, bsz, mo_split=5, filter_eps=
     pm = shellpair_mask(Bmask, bsz) if Bmask is not None else None
     nbf = S.shape[0]
     nvir = nbf - nocc
-    terms = []
+    terms, Zs = [], []
     for w, a in zip(omega, alpha):
         Coe = Co * np.exp(0.5 * eo * a)[None, :]
         Cve = Cv * np.exp(-0.5 * ev * a)[None, :]
@@ -135,10 +136,13 @@ def sos_mp2(B, Vinv, S, C, eps, nocc, omega, alpha, bsz, mo_split=5, filter_eps=
         Lo = block_filter(Lo, bsz, split_range(rank, mo_split), filter_eps)
         Pv = block_filter(Pv, bsz, bsz, filter_eps)
         Z = llmp_full_z(B, Lo, Pv, pm)
+        if return_z:
+            Zs.append(Z)
         Zt = Z @ Vinv
         terms.append(float(np.sum(Zt * Zt.T)))
     E = float(np.sum(terms))
-    return (E, terms) if return_terms else E
+    out = (E,) + ((terms,) if return_terms else ()) + ((np.array(Zs),) if return_z else ())
+    return out if len(out) > 1 else E
 
 
 def canonical_sos(B, Vinv, C, eps, nocc):

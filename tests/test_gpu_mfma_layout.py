@@ -79,3 +79,43 @@ def test_global_load_lds_placement():
     _lib.check(_lib.lib().mchip_probe_lds_load(src.ctypes.data_as(u32), _lib.iptr(perm), out.ctypes.data_as(u32)))
     assert np.array_equal(out[:64], np.zeros((64, 4), np.uint32))
     assert np.array_equal(out[64:], src[perm])
+
+
+def test_mfma_f32_rounding():
+    """Rounding of v_mfma_f32_16x16x4_f32's four-term sum: every case lands
+    within one ulp of the exact sum (the FP32 contractions' error model); the
+    printed line says whether each case rounded to nearest-even."""
+    A = np.zeros((16, 4), np.float64)
+    B = np.zeros((4, 16), np.float64)
+    B[:, 0] = 1.0
+    cases = [
+        (1.0, 1.25 * 2.0 ** -24, 0.0, 0.0),                  # above half an ulp: up
+        (1.0, -0.75 * 2.0 ** -25, 0.0, 0.0),                 # below 1, under half an ulp: stays 1
+        (1.0, 2.0 ** -24, 0.0, 0.0),                         # tie: even (1)
+        (1.0 + 2.0 ** -23, 2.0 ** -24, 0.0, 0.0),            # tie: even (1 + 2^-22)
+        (1.0, 2.0 ** -25, 2.0 ** -25, 2.0 ** -25),           # three sub-half-ulp terms: 1 + 2^-23 if fused
+        (2.0 ** 20, -(2.0 ** 20), 2.0 ** -30, 0.0),          # cancellation keeps the small term
+    ]
+    for i, c in enumerate(cases):
+        A[i, :] = c
+    a_lane = np.zeros(64, np.float32)
+    b_lane = np.zeros(64, np.float32)
+    for l in range(64):
+        a_lane[l] = A[l % 16, l // 16]
+        b_lane[l] = B[l // 16, l % 16]
+    assert np.array_equal(a_lane.astype(np.float64)[:64].reshape(4, 16).T, A.astype(np.float32))
+    out = np.zeros(256, np.float32)
+    _lib.check(_lib.lib().mchip_probe_mfma_lp(0, _lib.fptr(a_lane), _lib.fptr(b_lane), _lib.fptr(out)))
+    got = np.zeros((16, 16))
+    for l in range(64):
+        for e in range(4):
+            got[4 * (l // 16) + e, l % 16] = out[l * 4 + e]
+    modes = []
+    for i, c in enumerate(cases):
+        exact = float(np.sum(np.array(c, np.float64)))
+        rne = float(np.float32(exact))
+        g = got[i, 0]
+        ulp = float(np.spacing(np.float32(abs(exact)))) if exact != 0 else 2.0 ** -149
+        modes.append("case %d: exact %.17g got %.17g rne %s" % (i, exact, g, g == rne))
+        assert abs(g - exact) <= ulp
+    print("\n".join(modes))
