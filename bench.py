@@ -253,8 +253,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profiling mode: exchange builds only (no SCF), one JSON line of timings")
-    ap.add_argument("--escalate", type=float, default=1e-4,
-                    help="error RMS at which the auto form switches from BF16x3 to FP32 builds")
+    ap.add_argument("--escalate", type=float, default=0.0,
+                    help="auto form: error RMS at which BF16x3 builds give way to FP32 (0: off)")
+    ap.add_argument("--stall", type=int, default=3,
+                    help="auto form: iterations without progress before BF16x3 gives way to FP32 (0: off)")
     ap.add_argument("--requests", type=int, default=0, help="c4: number of requests (0: 200)")
     ap.add_argument("--slots", type=int, default=4, help="c4: request workers")
     ap.add_argument("--serve-check", type=int, default=4,
@@ -358,30 +360,46 @@ def main():
     ctx.sync()
     t_copies = time.perf_counter() - tc
     kmode = {"auto": "mixed_auto", "bf16": "mixed_bf16", "f32": "mixed_f32"}[form]
-    ctx.set_escalation(args.escalate)
+    ctx.set_escalation(args.escalate, args.stall)
     t2 = time.perf_counter()
     chosen = run_rhf(mol, dB, met, S, T + V, threshold=args.scf_thresh, kmode=kmode)
     dE = chosen.energy - ref.energy
     nbuild = len(chosen.t_k)
     if form == "auto":
-        # the library's escalation rule: BF16 while the previous RMS exceeds tau
-        tau = max(args.escalate, 10.0 * args.scf_thresh)
-        n16 = 0
+        # the library's escalation rule (select_k_builders, K mode 4): stage 0
+        # BF16x3, 1 FP32, 2 FP64; a stage ends at the RMS trigger (stage 0) or
+        # after args.stall iterations (below the stage's arming RMS) without the
+        # RMS falling below 0.8 x its best
+        stage_of, stage, best, flat = [], 0, float("inf"), 0
         for i in range(nbuild):
             prev = chosen.history_rms[i - 1] if i > 0 else float("inf")
-            if prev <= tau:
-                break
-            n16 += 1
+            if stage < 2 and np.isfinite(prev):
+                nxt = stage == 0 and args.escalate > 0 and prev <= args.escalate
+                if prev >= (1e-4, 1e-6)[stage]:     # stall counting is armed below these
+                    flat = 0
+                elif prev < 0.8 * best:
+                    flat = 0
+                elif args.stall > 0:
+                    flat += 1
+                    nxt = nxt or flat >= args.stall
+                best = min(best, prev)
+                if nxt:
+                    stage, best, flat = stage + 1, float("inf"), 0
+            stage_of.append(stage)
     else:
-        n16 = nbuild if form == "bf16" else 0
-    log("chosen SCF (%s): E=%.10f (low-precision-K iterate %.10f) iters=%d (%d BF16 + %d FP32 builds) "
-        "dE=%.3e (%.1f s)" % (form, chosen.energy, chosen.energy_iter, chosen.iterations, n16, nbuild - n16,
+        stage_of = [0 if form == "bf16" else 1] * nbuild
+    nb = [stage_of.count(k) for k in range(3)]
+    log("chosen SCF (%s): E=%.10f (low-precision-K iterate %.10f) iters=%d (%d BF16 + %d FP32 + %d FP64 builds) "
+        "dE=%.3e (%.1f s)" % (form, chosen.energy, chosen.energy_iter, chosen.iterations, nb[0], nb[1], nb[2],
                              dE, time.perf_counter() - t2))
     steps = args.steps or max(1, nbuild)
-    n16 = min(n16, steps)
+    stage_of = (stage_of + [stage_of[-1] if stage_of else 0] * steps)[:steps]
+    nb = [stage_of.count(k) for k in range(3)]
+    n16 = nb[0]
     Cc = np.ascontiguousarray(chosen.C[:, :mol.nocc])
     k16 = dB.dfk_mo_bf16 if form in ("auto", "bf16") else None
     k32 = dB.dfk_mo_f32 if form in ("auto", "f32") else None
+    k64 = lambda C: dB.dfk_mo(C, met)
     for _ in range(args.warmup):
         if k16:
             k16(Cc)
@@ -390,7 +408,7 @@ def main():
     ctx.sync()
     t = time.perf_counter()
     for s_ in range(steps):
-        (k16 if s_ < n16 else k32)(Cc)
+        (k16, k32, k64)[stage_of[s_]](Cc)
     dB.dfk_mo(Cc, met)      # the final FP64 exchange (plain formula) for the reported energy
     ctx.sync()
     step_s = (time.perf_counter() - t) / steps
@@ -419,9 +437,9 @@ def main():
     if k32:
         ph = phases(k32)
         kernels["k_slab_gemm<float>"] = dict(s=ph[0], tflops=k1_flops / ph[0] / 1e12,
-                                             builds=steps - n16, peak=MFMA_F32_PEAK / 1e12)
-        kernels["k_seg_gemm<float>"] = dict(s=ph[2], tflops=k3_flops / ph[2] / 1e12,
-                                            builds=steps - n16, peak=MFMA_F32_PEAK / 1e12)
+                                             builds=nb[1], peak=MFMA_F32_PEAK / 1e12)
+        kernels["k_seg_syrk_f32"] = dict(s=ph[2], tflops=k3_flops / ph[2] / 1e12,
+                                         builds=nb[1], peak=MFMA_F32_PEAK / 1e12)
     dom = max((k for k in kernels if isinstance(kernels[k], dict) and "s" in kernels[k]),
               key=lambda k: kernels[k]["s"] * kernels[k]["builds"])
     d = kernels[dom]
@@ -465,7 +483,7 @@ def main():
         "plain_fp64_s_per_iter": plain_s,
         "speedup_vs_plain": plain_s / step_s,
         "chosen_form": form,
-        "builds": {"bf16x3": n16, "f32": steps - n16, "fp64_final": 1},
+        "builds": {"bf16x3": nb[0], "f32": nb[1], "fp64": nb[2], "fp64_final": 1},
         "bf16x3_build_s": t16,
         "f32_build_s": t32,
         "final_fp64_build_s": plain_s,

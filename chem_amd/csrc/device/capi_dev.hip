@@ -83,11 +83,13 @@ extern "C" int mchip_ctx_stream(mchip_ctx c, void** stream)
   MCHIP_CATCH
 }
 
-extern "C" int mchip_ctx_set_escalation(mchip_ctx c, double rms)
+extern "C" int mchip_ctx_set_escalation(mchip_ctx c, double rms, int stall)
 {
   MCHIP_TRY
-  if (!(rms > 0.0)) throw std::runtime_error("mchip_ctx_set_escalation: the threshold must be positive");
+  if (!(rms >= 0.0)) throw std::runtime_error("mchip_ctx_set_escalation: the RMS threshold must be >= 0");
+  if (stall < 0) throw std::runtime_error("mchip_ctx_set_escalation: the stall count must be >= 0");
   c->c.escalate_rms = rms;
+  c->c.escalate_stall = stall;
   MCHIP_CATCH
 }
 

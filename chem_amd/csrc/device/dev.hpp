@@ -156,7 +156,10 @@ struct Ctx {
   DevBuf scratch_partial;                   // split-K partial sums
   DevBuf info;                              // rocsolver info
   std::map<int, DevBuf> syrk_tiles;         // n -> upper-tile table of the FP32 SYRK
-  double escalate_rms = 1e-4;               // BF16 -> FP32 switch of the escalation mode
+  // BF16 -> FP32 switch of the escalation mode: at an error RMS <= escalate_rms
+  // (0: off) or after escalate_stall iterations without progress (0: off)
+  double escalate_rms = 0.0;
+  int escalate_stall = 3;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // optional per-phase kernel timing (HIP events on this stream), seconds
   bool timing = false;

@@ -258,7 +258,14 @@ void scf_run(Ctx& c, DevBT& bt, const Metric* m, const ScfInput& in, ScfOutput& 
     float tj = 0.f;
     HIP_CHECK(hipEventElapsedTime(&tj, c.ev0, c.ev1));
     HIP_CHECK(hipEventRecord(c.ev0, c.stream));
-    kbuild(iter, prev_rms, P.as<double>(), Cocc.as<double>(), K.as<double>());
+    if (kbuild(iter, prev_rms, P.as<double>(), Cocc.as<double>(), K.as<double>())) {
+      for (auto& b : diis.err) diis.spare.push_back(std::move(b));
+      for (auto& b : diis.trial) diis.spare.push_back(std::move(b));
+      diis.err.clear();
+      diis.trial.clear();
+      diis.B.clear();
+      diis.coeffs.clear();
+    }
     HIP_CHECK(hipEventRecord(c.ev1, c.stream));
     HIP_CHECK(hipEventSynchronize(c.ev1));
     float tk = 0.f;
@@ -382,11 +389,13 @@ void select_k_builders(Ctx& c, DevBT& bt, const Metric* m, int kmode, int nocc, 
   const Metric* km = bt.transformed ? nullptr : m;
   KBuildFn k64 = [cp, t, km, nocc, occ_nbatches](int, double, const double*, const double* Cocc, double* K) {
     dfk_mo(*cp, *t, km, Cocc, nocc, nocc, K, occ_nbatches);
+    return false;
   };
   kfinal = KBuildFn();
   if (kmode >= 2)   // the final FP64 exchange of the low-precision modes
     k64 = [cp, t, km, nocc, final_occ_nbatches](int, double, const double*, const double* Cocc, double* K) {
       dfk_mo(*cp, *t, km, Cocc, nocc, nocc, K, final_occ_nbatches);
+      return false;
     };
   if (kmode == 0 || kmode == 1) {
     kb = k64;
@@ -395,6 +404,7 @@ void select_k_builders(Ctx& c, DevBT& bt, const Metric* m, int kmode, int nocc, 
     if (!bt.lowp_tilde || !bt.data32.p) throw std::runtime_error("SCF: the FP32 mode needs the FP32 copy of L^-1 B");
     kb = [cp, t, nocc, occ_nbatches](int, double, const double*, const double* Cocc, double* K) {
       dfk_mo_f32(*cp, *t, Cocc, nocc, nocc, K, occ_nbatches);
+      return false;
     };
     kfinal = k64;
   }
@@ -402,19 +412,52 @@ void select_k_builders(Ctx& c, DevBT& bt, const Metric* m, int kmode, int nocc, 
     if (!bt.lowp_tilde || !bt.data_hi.p) throw std::runtime_error("SCF: the BF16 mode needs the BF16 split copy of L^-1 B");
     kb = [cp, t, nocc, occ_nbatches](int, double, const double*, const double* Cocc, double* K) {
       dfk_mo_bf16(*cp, *t, Cocc, nocc, nocc, K, occ_nbatches);
+      return false;
     };
     kfinal = k64;
   }
   else if (kmode == 4) {
     if (!bt.lowp_tilde || !bt.data_hi.p || !bt.data32.p)
       throw std::runtime_error("SCF: the escalation mode needs the BF16 and FP32 copies of L^-1 B");
-    const double tau = std::max(c.escalate_rms, 10.0 * threshold);
-    auto escalated = std::make_shared<bool>(false);
-    kb = [cp, t, nocc, occ_nbatches, tau, escalated](int, double prev_rms, const double*, const double* Cocc,
-                                                     double* K) {
-      if (prev_rms <= tau) *escalated = true;
-      if (*escalated) dfk_mo_f32(*cp, *t, Cocc, nocc, nocc, K, occ_nbatches);
-      else dfk_mo_bf16(*cp, *t, Cocc, nocc, nocc, K, occ_nbatches);
+    // stage 0 BF16x3, 1 FP32, 2 FP64: a stage ends at the RMS trigger (BF16x3
+    // only) or after `stall` iterations without progress once the error RMS is
+    // below the stage's arming level (early SCF iterations also stall, far
+    // above any precision's noise floor)
+    struct Esc {
+      int stage = 0;
+      double best = std::numeric_limits<double>::infinity();
+      int flat = 0;
+    };
+    auto st = std::make_shared<Esc>();
+    const double tau = c.escalate_rms;
+    const int stall = c.escalate_stall;
+    kb = [cp, t, nocc, occ_nbatches, tau, stall, st, k64](int it, double prev_rms, const double* P,
+                                                          const double* Cocc, double* K) {
+      bool switched = false;
+      constexpr double arm[2] = {1e-4, 1e-6};
+      if (st->stage < 2 && std::isfinite(prev_rms)) {
+        bool next = st->stage == 0 && tau > 0.0 && prev_rms <= tau;
+        if (prev_rms >= arm[st->stage]) {
+          st->flat = 0;
+        }
+        else if (prev_rms < 0.8 * st->best) {
+          st->flat = 0;
+        }
+        else if (stall > 0 && ++st->flat >= stall) {
+          next = true;
+        }
+        st->best = std::min(st->best, prev_rms);
+        if (next) {
+          ++st->stage;
+          st->best = std::numeric_limits<double>::infinity();
+          st->flat = 0;
+          switched = true;
+        }
+      }
+      if (st->stage == 0) dfk_mo_bf16(*cp, *t, Cocc, nocc, nocc, K, occ_nbatches);
+      else if (st->stage == 1) dfk_mo_f32(*cp, *t, Cocc, nocc, nocc, K, occ_nbatches);
+      else k64(it, prev_rms, P, Cocc, K);
+      return switched;
     };
     kfinal = k64;
   }

@@ -32,12 +32,17 @@ struct ScfOutput {
 };
 
 // K builder callback: (iteration, RMS of the previous iteration's error matrix
-// (infinity before the first), P_dev, Cocc_dev row-major nbf x nocc, K_dev out)
-using KBuildFn = std::function<void(int, double, const double*, const double*, double*)>;
+// (infinity before the first), P_dev, Cocc_dev row-major nbf x nocc, K_dev out).
+// Returns true when this K comes from a different exchange operator than the
+// previous iteration's (a precision switch): the SCF then restarts DIIS, whose
+// stored Fock matrices and errors belong to the old operator.
+using KBuildFn = std::function<bool(int, double, const double*, const double*, double*)>;
 
 // Exchange builders of an SCF run for kmode (0 plain DFMO with the metric per
 // build, 1 DFMO on the transformed B, 2 FP32, 3 BF16x3, 4 BF16x3 escalating to
-// FP32 once the previous error RMS is below max(c.escalate_rms, 10 * threshold); modes
+// FP32 once the previous error RMS is <= c.escalate_rms or, below 1e-4, has not
+// fallen below 0.8 x its lowest value for c.escalate_stall iterations, and from
+// FP32 to FP64 after such a stall below 1e-6; modes
 // 2-4 get an FP64 final builder, with final_occ_nbatches occupied batches,
 // 0 = occ_nbatches).  The builders capture c, bt and m by reference.
 void select_k_builders(Ctx& c, DevBT& bt, const Metric* m, int kmode, int nocc, int occ_nbatches,

@@ -39,9 +39,10 @@ class Context:
     def set_timing(self, on):
         _lib.check(_lib.lib().mchip_ctx_set_timing(self.h, int(bool(on))))
 
-    def set_escalation(self, rms):
-        """Error RMS at which the escalation K mode switches BF16x3 -> FP32."""
-        _lib.check(_lib.lib().mchip_ctx_set_escalation(self.h, float(rms)))
+    def set_escalation(self, rms=0.0, stall=3):
+        """When the escalation K mode switches BF16x3 -> FP32: at an error RMS
+        <= rms (0: off) or after `stall` iterations without progress (0: off)."""
+        _lib.check(_lib.lib().mchip_ctx_set_escalation(self.h, float(rms), int(stall)))
 
     def phase_times(self):
         """Accumulated seconds: K1, K2, K3, J, K_inc, split, -, -."""

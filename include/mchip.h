@@ -124,10 +124,13 @@ int mchip_ctx_stream(mchip_ctx c, void** stream);
  * accumulators.  Slots: 0 K1 (B.C), 1 K2 (metric), 2 K3 (SYRK),
  * 3 J total, 4 incremental K, 5 operand splitting. */
 int mchip_ctx_set_timing(mchip_ctx c, int on);
-/* Error RMS below which the escalation K mode (4) switches from the BF16x3 to
- * the FP32 exchange (the switch point is max(rms, 10 * threshold); default
- * 1e-4).  No counterpart in the reference. */
-int mchip_ctx_set_escalation(mchip_ctx c, double rms);
+/* When the escalation K mode (4) switches from the BF16x3 to the FP32
+ * exchange: once the previous iteration's error RMS is <= rms (0 turns this
+ * trigger off; default 0), or after `stall` consecutive iterations below an
+ * error RMS of 1e-4 whose RMS stayed above 0.8 x the lowest so far (0 turns it
+ * off; default 3).  FP32 gives way to FP64 after such a stall below 1e-6.  No
+ * counterpart in the reference. */
+int mchip_ctx_set_escalation(mchip_ctx c, double rms, int stall);
 int mchip_ctx_phase_times(mchip_ctx c, double* out, int n);
 
 /* Copy a host B to HBM (FP64).  dbcsr::btensor decompress_init/decompress
@@ -232,10 +235,11 @@ int mchip_dfk_bf16_shapes(int* ngemm, int* nsyrk, int* direct);
  *            the converged orbitals for the reported energy,
  *        3 = mixed with the BF16x3 exchange in every iteration (needs
  *            mchip_dbt_make_bf16), FP64 exchange for the reported energy,
- *        4 = precision escalation: BF16x3 while the previous iteration's error
- *            RMS exceeds max(r, 10 * threshold) (r = 1e-4 unless set by
- *            mchip_ctx_set_escalation), FP32 afterwards, FP64 for the reported
- *            energy (needs both copies).
+ *        4 = precision escalation: BF16x3 until the trigger set by
+ *            mchip_ctx_set_escalation fires (by default: three iterations
+ *            without progress of the error RMS), then FP32 until it stalls in
+ *            the same way, then FP64; FP64 for the reported energy (needs both
+ *            copies).
  * Outputs C (nbf x nbf row-major), orbital energies and per-iteration
  * history (energy, RMS error, J and K seconds) up to max_hist entries.
  * ---------------------------------------------------------------------- */

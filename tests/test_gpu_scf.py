@@ -96,8 +96,8 @@ def test_mixed_bf16_scf_energy(ctx):
 
 
 def test_mixed_auto_scf_energy(ctx):
-    """Precision escalation (BF16x3 exchange while the error RMS is above
-    max(1e-4, 10 thr), FP32 afterwards, FP64 for the final energy): the
+    """Precision escalation with the RMS trigger (BF16x3 exchange while the
+    error RMS is above 1e-4, FP32 afterwards, FP64 for the final energy): the
     converged density is the FP32 fixed point, so |dE| < 1e-8 Eh as for the FP32
     form."""
     from chem_amd.engine import DeviceB, Metric
@@ -111,12 +111,40 @@ def test_mixed_auto_scf_energy(ctx):
     ref = run_rhf(mol, dB, None, S, T + V, threshold=1e-7, kmode="dfmo_transformed")
     dB.make_bf16()
     dB.make_f32()
-    mix = run_rhf(mol, dB, None, S, T + V, threshold=1e-7, kmode="mixed_auto")
+    ctx.set_escalation(1e-4, 0)
+    try:
+        mix = run_rhf(mol, dB, None, S, T + V, threshold=1e-7, kmode="mixed_auto")
+    finally:
+        ctx.set_escalation()
     nbf16 = sum(1 for r in [float("inf")] + mix.history_rms[:-1] if r > 1e-4)
     print("E64 %.10f Eauto %.10f dE %.2e (BF16 builds before escalation: %d of %d)" % (
         ref.energy, mix.energy, mix.energy - ref.energy, nbf16, len(mix.t_k)))
     assert ref.converged and mix.converged
     assert 0 < nbf16 < len(mix.t_k)
+    assert abs(mix.energy - ref.energy) < 1e-8
+
+
+def test_mixed_auto_stall_trigger(ctx):
+    """Default escalation (BF16x3 until three iterations pass without the error
+    RMS falling below 0.8 x its lowest value, then FP32 until it stalls the same
+    way, then FP64) down to a threshold below the BF16x3 and FP32 noise floors:
+    the run converges and the final FP64 energy agrees with the plain SCF to
+    1e-8 Eh."""
+    from chem_amd.engine import DeviceB, Metric
+    from chem_amd.scf import run_rhf
+    mol = desc.Molecule(desc.water_cluster(4, 11))
+    S, T, V = ints.int1e(mol)
+    Bh = ints.BTensorHost.build(mol, 1e-10)
+    m = Metric(ctx, ints.int2c2e(mol))
+    dB = DeviceB(ctx, Bh)
+    ref = run_rhf(mol, dB, m, S, T + V, threshold=1e-9, kmode="dfmo")
+    dB.make_lowp(m)
+    only16 = run_rhf(mol, dB, m, S, T + V, threshold=1e-9, kmode="mixed_bf16", max_iter=60)
+    mix = run_rhf(mol, dB, m, S, T + V, threshold=1e-9, kmode="mixed_auto", max_iter=60)
+    print("plain %d iters; bf16 only: converged %s, lowest RMS %.2e; auto: %d iters dE %.2e" % (
+        ref.iterations, only16.converged, min(only16.history_rms), mix.iterations, mix.energy - ref.energy))
+    print("auto RMS history: " + " ".join("%.1e" % r for r in mix.history_rms))
+    assert ref.converged and mix.converged
     assert abs(mix.energy - ref.energy) < 1e-8
 
 
