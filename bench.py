@@ -94,6 +94,26 @@ def cpu_baseline(mol, dB, Linv, C_occ, budget_s=20.0):
                            rows, naux, dt1, s, ncol, dt2))
 
 
+def measured_traffic():
+    """HBM bytes per launch of the exchange kernels at 64 waters, from the
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs summarised under profiles/
+    (lines `traffic_bytes_per_launch <kernel> <bytes>`, calibrated against a
+    copy kernel of known size)."""
+    out = {}
+    pdir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+    if not os.path.isdir(pdir):
+        return out
+    for f in sorted(os.listdir(pdir)):
+        if not f.endswith(".md"):
+            continue
+        with open(os.path.join(pdir, f)) as fh:
+            for line in fh:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch":
+                    out[parts[1]] = float(parts[2])
+    return out
+
+
 def serve_stream(args, log_progress=True):
     """The DF-HF request stream: seeded Poisson arrivals, water clusters of
     uniformly random size, the chosen form (BF16x3 escalating to FP32, FP64
@@ -443,9 +463,14 @@ def main():
     dom = max((k for k in kernels if isinstance(kernels[k], dict) and "s" in kernels[k]),
               key=lambda k: kernels[k]["s"] * kernels[k]["builds"])
     d = kernels[dom]
+    traffic = measured_traffic().get(dom)
     roof = dict(bound="mfma", achieved=d["tflops"], peak=d["peak"], unit="TFLOP/s",
-                frac=d["tflops"] / d["peak"], traffic=None, kernel=dom, kernel_s=d["s"],
+                frac=d["tflops"] / d["peak"], traffic=traffic, kernel=dom, kernel_s=d["s"],
                 kernels=kernels)
+    if traffic:
+        # HBM bytes per launch from the committed PMC runs, over this run's launch time
+        roof["hbm_gbs"] = traffic / d["s"] / 1e9
+        roof["hbm_frac"] = roof["hbm_gbs"] / 8000.0
     if "bf16x3" in dom:
         # three BF16 products per useful multiply-add: issued MFMA work is 3x the algorithmic flops
         roof["mfma_issued_frac"] = 3.0 * roof["frac"]

@@ -125,6 +125,9 @@ _SIGS = [
     ("mchip_mp2_sos_batched", c_int, [c_void_p, c_void_p, c_int, c_int, P_double, P_double, P_double, c_int,
                               P_double, P_double, c_int, P_int, c_int, c_double, P_double, P_double,
                               P_double, P_double]),
+    ("mchip_mp2_sos_z", c_int, [c_void_p, c_void_p, c_int, c_int, c_int, P_double, P_double, P_double, c_int,
+                                P_double, P_double, c_int, P_int, c_int, c_double, P_double, P_double, P_double]),
+    ("mchip_z_compute", c_int, [c_void_p, c_int, c_int, P_double, c_int, P_double, P_double, c_int, P_double]),
     ("mchip_mp2_z", c_int, [c_void_p, c_void_p, c_int, c_int, P_double, P_double, P_double, c_int,
                             P_double, P_double, c_int, P_int, c_int, c_double, c_int, P_double, P_double]),
     ("mchip_probe_mfma_f64", c_int, [P_double, P_double, P_double]),

@@ -494,7 +494,7 @@ extern "C" int mchip_scf_run(mchip_dbt t, mchip_metric m, int nbf, int nocc, int
 static int mp2_entry(mchip_dbt t, mchip_metric m, int nbf, int nocc, const double* S, const double* C,
                      const double* eps, int nlap, const double* omega, const double* alpha, int nbb,
                      const int* bsz, int mo_split, double filter_eps, double* energy, double* terms, double* t_z,
-                     double* t_prep, bool batched, double* Z = nullptr)
+                     double* t_prep, bool batched, double* Z = nullptr, int zmethod = 0)
 {
   MCHIP_TRY
   need_device(t->ctx);
@@ -511,6 +511,7 @@ static int mp2_entry(mchip_dbt t, mchip_metric m, int nbf, int nocc, const doubl
   in.bsz.assign(bsz, bsz + nbb);
   in.mo_split = mo_split;
   in.filter_eps = filter_eps;
+  in.zmethod = zmethod;
   std::vector<double> tv;
   double tz = 0.0, tp = 0.0;
   if (batched)
@@ -541,6 +542,48 @@ extern "C" int mchip_mp2_sos_batched(mchip_dbt t, mchip_metric m, int nbf, int n
 {
   return mp2_entry(t, m, nbf, nocc, S, C, eps, nlap, omega, alpha, nbb, bsz, mo_split, filter_eps, energy, terms,
                    t_z, t_prep, true);
+}
+
+extern "C" int mchip_mp2_sos_z(mchip_dbt t, mchip_metric m, int zmethod, int nbf, int nocc, const double* S,
+                               const double* C, const double* eps, int nlap, const double* omega,
+                               const double* alpha, int nbb, const int* bsz, int mo_split, double filter_eps,
+                               double* energy, double* terms, double* t_z)
+{
+  return mp2_entry(t, m, nbf, nocc, S, C, eps, nlap, omega, alpha, nbb, bsz, mo_split, filter_eps, energy, terms,
+                   t_z, nullptr, false, nullptr, zmethod);
+}
+
+extern "C" int mchip_z_compute(mchip_dbt t, int zmethod, int nbf, const double* Locc, int rank_o,
+                               const double* Pvir, const double* Lvir, int rank_v, double* Z)
+{
+  MCHIP_TRY
+  need_device(t->ctx);
+  Ctx& c = t->ctx->c;
+  DevBT& bt = t->t;
+  if (bt.transformed) throw std::runtime_error("mchip_z_compute: needs the untransformed B");
+  if (nbf != bt.L.nbf) throw std::runtime_error("mchip_z_compute: nbf does not match B");
+  if (zmethod < 0 || zmethod > 2) throw std::runtime_error("mchip_z_compute: unknown Z method");
+  if (rank_o < 0 || rank_v < 0) throw std::runtime_error("mchip_z_compute: negative rank");
+  if (rank_o > 0 && !Locc) throw std::runtime_error("mchip_z_compute: L_occ is required");
+  if (zmethod == 2 && rank_v > 0 && !Lvir) throw std::runtime_error("mchip_z_compute: LL_FULL needs L_vir");
+  if (zmethod != 2 && !Pvir) throw std::runtime_error("mchip_z_compute: LLMP needs P_vir");
+  const int naux = bt.L.naux;
+  const size_t nn = (size_t)nbf * nbf;
+  DevBuf dLo((size_t)nbf * std::max(rank_o, 1) * sizeof(double)), dPv, dLv,
+      dZ((size_t)naux * naux * sizeof(double));
+  if (rank_o) h2d(dLo.p, Locc, (size_t)nbf * rank_o * sizeof(double), c.stream);
+  if (zmethod != 2) {
+    dPv.alloc(nn * sizeof(double));
+    h2d(dPv.p, Pvir, nn * sizeof(double), c.stream);
+  }
+  else if (rank_v) {
+    dLv.alloc((size_t)nbf * rank_v * sizeof(double));
+    h2d(dLv.p, Lvir, (size_t)nbf * rank_v * sizeof(double), c.stream);
+  }
+  z_point(c, bt, zmethod, dLo.as<double>(), rank_o, dPv.p ? dPv.as<double>() : nullptr,
+          dLv.p ? dLv.as<double>() : nullptr, rank_v, dZ.as<double>());
+  d2h(Z, dZ.p, (size_t)naux * naux * sizeof(double), c.stream);
+  MCHIP_CATCH
 }
 
 extern "C" int mchip_mp2_z(mchip_dbt t, mchip_metric m, int nbf, int nocc, const double* S, const double* C,

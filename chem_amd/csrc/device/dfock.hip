@@ -11,6 +11,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <type_traits>
 
@@ -226,23 +227,33 @@ void metric_create(Ctx& c, int naux, const double* V_host, Metric& m)
   const size_t bytes = (size_t)naux * naux * sizeof(double);
   m.Linv.alloc(bytes);
   m.Vinv.alloc(bytes);
-  h2d(m.Linv.p, V_host, bytes, c.stream);
   double* A = m.Linv.as<double>();
   int* info = c.info.as<int>();
-  RB_CHECK(rocsolver_dpotrf(c.blas, rocblas_fill_lower, naux, A, naux, info));
+  std::vector<double> back((size_t)naux * naux);
+  // upload, read back and compare before factorising; a failed factorisation
+  // of a verified upload is retried once from a fresh upload.  Both events are
+  // reported on stderr (they have been seen only with several requests on the
+  // device at once).
   int h_info = 0;
-  d2h(&h_info, info, sizeof(int), c.stream);
-  if (h_info != 0) {
-    // diagnose before failing: is the matrix on the device still the input?
-    h2d(m.Linv.p, V_host, bytes, c.stream);
-    std::vector<double> back((size_t)naux * naux);
-    d2h(back.data(), m.Linv.p, bytes, c.stream);
-    size_t ndiff = 0;
-    for (size_t i = 0; i < back.size(); ++i) ndiff += back[i] != V_host[i];
-    throw std::runtime_error("metric Cholesky failed (matrix not positive definite; pivot " +
-                             std::to_string(h_info) + ", re-uploaded matrix differs in " + std::to_string(ndiff) +
-                             " elements)");
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    int bad = 0;
+    for (int up = 0; up < 3; ++up) {
+      h2d(m.Linv.p, V_host, bytes, c.stream);
+      d2h(back.data(), m.Linv.p, bytes, c.stream);
+      bad = std::memcmp(back.data(), V_host, bytes) != 0;
+      if (!bad) break;
+      std::fprintf(stderr, "[mchip] metric upload differs from the host matrix (naux %d, attempt %d)\n", naux, up);
+    }
+    if (bad) throw std::runtime_error("metric upload does not reproduce the host matrix");
+    RB_CHECK(rocsolver_dpotrf(c.blas, rocblas_fill_lower, naux, A, naux, info));
+    d2h(&h_info, info, sizeof(int), c.stream);
+    if (h_info == 0) break;
+    std::fprintf(stderr, "[mchip] metric Cholesky of a verified upload failed at pivot %d (naux %d, attempt %d)\n",
+                 h_info, naux, attempt);
   }
+  if (h_info != 0)
+    throw std::runtime_error("metric Cholesky failed twice on a verified upload (pivot " + std::to_string(h_info) +
+                             ", naux " + std::to_string(naux) + ")");
   hipLaunchKernelGGL(k_zero_upper_cm, dim3(nblk((int64_t)naux * naux, 256)), dim3(256), 0, c.stream, A, naux);
   RB_CHECK(rocsolver_dtrtri(c.blas, rocblas_fill_lower, rocblas_diagonal_non_unit, naux, A, naux, info));
   d2h(&h_info, info, sizeof(int), c.stream);
@@ -661,7 +672,7 @@ static const Bf16Shape kBf16Shapes[] = {
 static constexpr int kNumBf16Shapes = sizeof(kBf16Shapes) / sizeof(kBf16Shapes[0]);
 // SYRK variants: 0 = register-staged on FP32 W; 1..3 = direct-to-LDS on bf16
 // hi/lo W with 2, 3, 4 stages
-static constexpr int kNumSyrkVariants = 4;
+static constexpr int kNumSyrkVariants = 5;   // 0 staged 128, 1-3 direct-to-LDS 128 (2-4 stages), 4 direct 256
 
 static void slab_gemm_bf16(Ctx& c, DevBT& t, int shape, const uint16_t* ghi, const uint16_t* glo, int ng,
                            float* W, uint16_t* Whi, uint16_t* Wlo, int ldw)
@@ -701,27 +712,53 @@ static void slab_gemm_bf16(Ctx& c, DevBT& t, int shape, const uint16_t* ghi, con
   HIP_CHECK(hipGetLastError());
 }
 
-static void seg_syrk_bf16(Ctx& c, int variant, const float* W, const uint16_t* Whi, const uint16_t* Wlo, int naux,
-                          int nbf, int ldw, const int2* tl, int ntile, int nch, int spc, double* partial)
+// Upper-triangle tile table of an n x n SYRK with BM x BM tiles (cached in t).
+static const int2* syrk_tiles(Ctx& c, DevBT& t, int n, int BM, int* ntile)
+{
+  const auto key = std::make_pair(-1000000 - BM, n);
+  const int nt = (n + BM - 1) / BM;
+  *ntile = nt * (nt + 1) / 2;
+  if (!t.tiles.count(key)) {
+    std::vector<int2> tl;
+    for (int i = 0; i < nt; ++i)
+      for (int j = i; j < nt; ++j) tl.push_back(int2{i, j});
+    t.tiles[key] = upload_vec(tl, c.stream);
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+  }
+  return t.tiles[key].as<int2>();
+}
+
+// tl/ntile: the 128 x 128 upper tiles; the 256-tile variant uses its own table.
+static void seg_syrk_bf16(Ctx& c, DevBT& t, int variant, const float* W, const uint16_t* Whi, const uint16_t* Wlo,
+                          int naux, int nbf, int ldw, const int2* tl, int ntile, int nch, int spc, double* partial)
 {
   const dim3 grid((unsigned)(ntile * nch));
+  const dim3 grid8((unsigned)((ntile * nch + 7) / 8 * 8));   // direct-to-LDS kernels: XCD mapping
   switch (variant) {
     case 0:
       hipLaunchKernelGGL((k_seg_syrk_bf16x3<SG_BM>), grid, dim3(256), 0, c.stream, W, naux, (int64_t)nbf * ldw, ldw,
                          nbf, tl, ntile, spc, partial);
       break;
     case 1:
-      hipLaunchKernelGGL(k_seg_syrk_bf16x3_dl<2>, grid, dim3(256), 0, c.stream, Whi, Wlo, naux, (int64_t)nbf * ldw,
-                         ldw, nbf, tl, ntile, spc, partial);
+      hipLaunchKernelGGL((k_seg_syrk_bf16x3_dl<2, 128>), grid8, dim3(256), 0, c.stream, Whi, Wlo, naux,
+                         (int64_t)nbf * ldw, ldw, nbf, tl, ntile, spc, partial);
       break;
     case 2:
-      hipLaunchKernelGGL(k_seg_syrk_bf16x3_dl<3>, grid, dim3(256), 0, c.stream, Whi, Wlo, naux, (int64_t)nbf * ldw,
-                         ldw, nbf, tl, ntile, spc, partial);
+      hipLaunchKernelGGL((k_seg_syrk_bf16x3_dl<3, 128>), grid8, dim3(256), 0, c.stream, Whi, Wlo, naux,
+                         (int64_t)nbf * ldw, ldw, nbf, tl, ntile, spc, partial);
       break;
     case 3:
-      hipLaunchKernelGGL(k_seg_syrk_bf16x3_dl<4>, grid, dim3(256), 0, c.stream, Whi, Wlo, naux, (int64_t)nbf * ldw,
-                         ldw, nbf, tl, ntile, spc, partial);
+      hipLaunchKernelGGL((k_seg_syrk_bf16x3_dl<4, 128>), grid8, dim3(256), 0, c.stream, Whi, Wlo, naux,
+                         (int64_t)nbf * ldw, ldw, nbf, tl, ntile, spc, partial);
       break;
+    case 4: {
+      int nt2 = 0;
+      const int2* tl2 = syrk_tiles(c, t, nbf, 256, &nt2);
+      hipLaunchKernelGGL((k_seg_syrk_bf16x3_dl<2, 256>), dim3((unsigned)((nt2 * nch + 7) / 8 * 8)), dim3(512), 0,
+                         c.stream, Whi,
+                         Wlo, naux, (int64_t)nbf * ldw, ldw, nbf, tl2, nt2, spc, partial);
+      break;
+    }
     default: throw std::runtime_error("BF16 SYRK: unknown variant");
   }
   HIP_CHECK(hipGetLastError());
@@ -771,7 +808,7 @@ static const Bf16Plan& tune_bf16(Ctx& c, DevBT& t, const uint16_t* ghi, const ui
   int arg3[2] = {-1, -1};
   for (int v = 0; v < kNumSyrkVariants; ++v) {
     const float ms =
-        time_launch(c, [&] { seg_syrk_bf16(c, v, W, Whi, Wlo, t.L.naux, t.L.nbf, ldw, tl, ntile, nch, spc, partial); });
+        time_launch(c, [&] { seg_syrk_bf16(c, t, v, W, Whi, Wlo, t.L.naux, t.L.nbf, ldw, tl, ntile, nch, spc, partial); });
     p.k3_ms[v] = ms;
     const int dl = v > 0 ? 1 : 0;
     if (ms < best3[dl]) {
@@ -904,7 +941,7 @@ void dfk_mo_bf16(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K
       slab_gemm_bf16(c, t, plan.k1, ghi, glo, w, W, Whi, Wlo, ldw);
     }
     PhaseTimer tm(c, PH_K3);
-    seg_syrk_bf16(c, plan.k3, W, Whi, Wlo, naux, nbf, ldw, d_tl, ntile, nch, spc, partial);
+    seg_syrk_bf16(c, t, plan.k3, W, Whi, Wlo, naux, nbf, ldw, d_tl, ntile, nch, spc, partial);
     if (b == nb - 1)
       hipLaunchKernelGGL(k_reduce_sym, dim3(nblk((int64_t)nbf * nbf, 256)), dim3(256), 0, c.stream,
                          c.scratch_partial.as<double>(), nb * nch, nbf, 0, -1.0, K);

@@ -560,29 +560,38 @@ __global__ __launch_bounds__(64 * WM * WN) void k_slab_gemm_bf16x3_dl(
 }
 
 // Segmented SYRK over bf16 hi/lo W, direct-to-LDS pipelined: partial[chunk](m,n)
-// = sum_{X in chunk} sum_i W(X,m,i) W(X,n,i) on the upper 128 x 128 tiles
-// (4 waves, 64 x 64 each).  Rows of W have ldw (multiple of 32) bf16.
-template <int S>
-__global__ __launch_bounds__(256) void k_seg_syrk_bf16x3_dl(
+// = sum_{X in chunk} sum_i W(X,m,i) W(X,n,i) on the upper BM x BM tiles.
+// BM = 128: 4 waves of 64 x 64; BM = 256: 8 waves of 128 x 64 (each W row
+// block is fetched by half as many tiles).  Rows of W have ldw (multiple of
+// 32) bf16.
+template <int S, int BM>
+__global__ __launch_bounds__(BM * 2) void k_seg_syrk_bf16x3_dl(
     const uint16_t* __restrict__ whi, const uint16_t* __restrict__ wlo, int nseg, int64_t sstride, int ldw,
     int n, const int2* __restrict__ tiles, int ntile, int segs_per_chunk, double* __restrict__ partial)
 {
-  constexpr int BM = 128, TM = 4, NW = 4;
+  static_assert(BM == 128 || BM == 256, "SYRK tile");
+  constexpr int NW = BM / 32, WN = NW / 2;
+  constexpr int TM = BM / 32, TN = BM / WN / 16;
   constexpr int LPW = (4 * BM / 16) / NW;
   constexpr int STAGE = 2 * BM * 8;
   __shared__ uint4 lds[S * STAGE];
-  const int2 tl = tiles[blockIdx.x % ntile];
-  const int chunk = blockIdx.x / ntile;
+  // all tiles of a chunk run on one XCD, dispatched together: the W rows of
+  // the chunk come from HBM about once per XCD and are re-read from its L2
+  // (grid padded to a multiple of 8; padding blocks exit)
+  const int lid = xcd_tile(blockIdx.x, gridDim.x);
+  const int chunk = lid / ntile;
   const int s0 = chunk * segs_per_chunk;
+  if (s0 >= nseg) return;
+  const int2 tl = tiles[lid % ntile];
   const int s1 = min(nseg, s0 + segs_per_chunk);
   const int m0 = tl.x * BM, n0 = tl.y * BM;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm0 = (wave >> 1) * 64, wn0 = (wave & 1) * 64;
-  f32x4 acc[TM][TM];
+  const int wm0 = (wave / WN) * (BM / 2), wn0 = (wave % WN) * (16 * TN);
+  f32x4 acc[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < TM; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
   const int ksteps = ldw >> 5;
   const int nsteps = (s1 > s0) ? (s1 - s0) * ksteps : 0;
   const int ld8 = ldw >> 3;
@@ -604,14 +613,14 @@ __global__ __launch_bounds__(256) void k_seg_syrk_bf16x3_dl(
     wait_stages<LPW, S>(min(S - 1, nsteps - 1 - q));
     __syncthreads();
     const uint4* st = lds + (q % S) * STAGE;
-    bf16x3_step_swz<TM, TM>(st, st + BM * 4, st + BM * 8, st + BM * 12, wm0, wn0, lane, acc);
+    bf16x3_step_swz<TM, TN>(st, st + BM * 4, st + BM * 8, st + BM * 12, wm0, wn0, lane, acc);
   }
   double* P = partial + (int64_t)chunk * n * n;
   const int fc = lane & 15, fr4 = 4 * (lane >> 4);
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < TM; ++j)
+    for (int j = 0; j < TN; ++j)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int r = m0 + wm0 + 16 * i + fr4 + e;
@@ -630,12 +639,14 @@ MCHIP_SLAB16DL_INST(2, 4, 4, 3, 3)
 MCHIP_SLAB16DL_INST(2, 4, 4, 5, 2)
 MCHIP_SLAB16DL_INST(1, 4, 4, 5, 3)
 #undef MCHIP_SLAB16DL_INST
-template __global__ void k_seg_syrk_bf16x3_dl<2>(const uint16_t*, const uint16_t*, int, int64_t, int, int,
-                                                 const int2*, int, int, double*);
-template __global__ void k_seg_syrk_bf16x3_dl<3>(const uint16_t*, const uint16_t*, int, int64_t, int, int,
-                                                 const int2*, int, int, double*);
-template __global__ void k_seg_syrk_bf16x3_dl<4>(const uint16_t*, const uint16_t*, int, int64_t, int, int,
-                                                 const int2*, int, int, double*);
+#define MCHIP_SYRK16DL_INST(S, BM)                                                                     \
+  template __global__ void k_seg_syrk_bf16x3_dl<S, BM>(const uint16_t*, const uint16_t*, int, int64_t, int, \
+                                                       int, const int2*, int, int, double*);
+MCHIP_SYRK16DL_INST(2, 128)
+MCHIP_SYRK16DL_INST(3, 128)
+MCHIP_SYRK16DL_INST(4, 128)
+MCHIP_SYRK16DL_INST(2, 256)
+#undef MCHIP_SYRK16DL_INST
 
 #define MCHIP_SLAB16_INST(WM, WN, TM, TN, NB)                                                          \
   template __global__ void k_slab_gemm_bf16x3<WM, WN, TM, TN, NB>(                                     \

@@ -11,6 +11,12 @@
 //   M4  b3(X|M N)  = sum_i L_o(M,i) b2(X,N,i) on B's pair pattern   k_slab_out_gemm
 //   M5  Z(X,Y)     = sum_{mu nu} b3(X|mu nu) B(Y|mu nu)  k_slabseg_gemm (upper tiles)
 //   M6  e = tr(Z~ Z~), Z~ = Z V^-1
+// LLMP_MEM_Z (z_mem.cpp) runs the same contractions in memory batches; with B
+// resident in HBM it is the same GPU path.  LL_FULL_Z (z_full.cpp:428-692)
+// replaces M3-M5 by a pivoted Cholesky factor L_v of the orthogonalised
+// virtual pseudo-density (mp_compute.cpp:309-336):
+//   b(X,i,a) = sum_N L_v(N,a) b(X,N,i)   rocBLAS strided batch
+//   Z(X,Y)   = sum_ia b(X,i,a) b(Y,i,a)   rocBLAS GEMM
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
@@ -276,7 +282,8 @@ static void mp2_setup(Ctx& c, const Mp2Input& in, Mp2Setup& s)
 // P~_o = w^1/4 (L_S^-1 C_o')(...)^T factorised by pivoted Cholesky, L_o = L_S L~
 // (n x rank row-major, in Lo), and the filtered virtual pseudo-density P_v
 // (n x n, in Pv).  Returns the rank.
-static int mp2_point_factors(Ctx& c, Mp2Setup& s, const Mp2Input& in, size_t k, DevBuf& Lo, DevBuf& Pv)
+static int mp2_point_factors(Ctx& c, Mp2Setup& s, const Mp2Input& in, size_t k, DevBuf& Lo, DevBuf& Pv,
+                             DevBuf* Lv = nullptr, int* rank_v = nullptr)
 {
   const int n = s.n, no = s.no, nv = s.nv;
   const size_t nn = (size_t)n * n;
@@ -304,6 +311,39 @@ static int mp2_point_factors(Ctx& c, Mp2Setup& s, const Mp2Input& in, size_t k, 
   }
   else {
     hipLaunchKernelGGL(k_fill, dim3(gs_blocks((int64_t)nn)), dim3(256), 0, c.stream, Pv.as<double>(), (int64_t)nn, 0.0);
+  }
+  // LL_FULL_Z: L_v = L_S CD(P~_v), P~_v = omega^{1/4} (L_S^-1 C_v')(L_S^-1 C_v')^T
+  // (mp_compute.cpp:309-336); made before P_v is filtered, from the same C_v'
+  if (Lv && rank_v) {
+    *rank_v = 0;
+    if (nv) {
+      DevBuf Cv((size_t)n * nv * sizeof(double)), Pvt(nn * sizeof(double));
+      RB_CHECK(rocblas_dtrmm(c.blas, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                             rocblas_diagonal_non_unit, nv, n, &one, s.LSinv.as<double>(), n, s.Wv.as<double>(), nv,
+                             Cv.as<double>(), nv));
+      RB_CHECK(rocblas_dgemm(c.blas, rocblas_operation_transpose, rocblas_operation_none, n, n, nv, &w4,
+                             Cv.as<double>(), nv, Cv.as<double>(), nv, &zero, Pvt.as<double>(), n));
+      int rv = 0;
+      std::vector<double> hLv = pivoted_cholesky_dev(c, Pvt, n, in.filter_eps, &rv);
+      *rank_v = rv;
+      if (rv > 0) {
+        DevBuf Lvt((size_t)n * rv * sizeof(double));
+        Lv->alloc((size_t)n * rv * sizeof(double));
+        h2d(Lvt.p, hLv.data(), hLv.size() * sizeof(double), c.stream);
+        RB_CHECK(rocblas_dtrmm(c.blas, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                               rocblas_diagonal_non_unit, rv, n, &one, s.LS.as<double>(), n, Lvt.as<double>(), rv,
+                               Lv->as<double>(), rv));
+        if (in.filter_eps > 0) {
+          std::vector<int> r = split_range(rv, in.mo_split), roff(r.size() + 1, 0);
+          for (size_t i = 0; i < r.size(); ++i) roff[i + 1] = roff[i] + r[i];
+          DevBuf d_roff = upload_vec(roff, c.stream);
+          hipLaunchKernelGGL(k_block_filter_rc, dim3(s.nbb * (int)r.size()), dim3(256), 0, c.stream,
+                             Lv->as<double>(), rv, s.d_boff.as<int>(), d_roff.as<int>(), (int)r.size(),
+                             in.filter_eps);
+        }
+        HIP_CHECK(hipStreamSynchronize(c.stream));   // host factor and offsets die here
+      }
+    }
   }
   // M1: pivoted Cholesky of P~_o, L_o = L_S L~
   int rank = 0;
@@ -344,6 +384,51 @@ static double mp2_point_energy(Ctx& c, const Metric& met, const double* Z, DevBu
   return e;
 }
 
+void z_point(Ctx& c, DevBT& bt, int zmethod, const double* Lo, int ro, const double* Pv, const double* Lv, int rv,
+             double* Z)
+{
+  const BLayout& L = bt.L;
+  const int n = L.nbf, naux = L.naux;
+  const double one = 1.0, zero = 0.0;
+  if (zmethod < 0 || zmethod > 2) throw std::runtime_error("Z builder: unknown method");
+  if (ro == 0 || (zmethod == 2 && rv == 0)) {
+    hipLaunchKernelGGL(k_fill, dim3(gs_blocks((int64_t)naux * naux)), dim3(256), 0, c.stream, Z,
+                       (int64_t)naux * naux, 0.0);
+    return;
+  }
+  // M2: b = B . L_o   [X][nu][i]
+  const size_t bsize = (size_t)naux * n * ro;
+  c.scratch_a.ensure(bsize * sizeof(double));
+  double* bb = c.scratch_a.as<double>();
+  slab_gemm(c, bt, Lo, ro, ro, bb, ro, 1.0);
+  if (zmethod == 2) {
+    // b(X,i,a) = sum_N L_v(N,a) b(X,N,i)  ("Na, XiN -> Xia"): per X the
+    // column-major (rv x ro) product L_v^T b_X
+    const size_t csize = (size_t)naux * ro * rv;
+    c.scratch_b.ensure(csize * sizeof(double));
+    double* cc = c.scratch_b.as<double>();
+    RB_CHECK(rocblas_dgemm_strided_batched(c.blas, rocblas_operation_none, rocblas_operation_transpose, rv, ro, n,
+                                           &one, Lv, rv, 0, bb, ro, (rocblas_stride)n * ro, &zero, cc, rv,
+                                           (rocblas_stride)ro * rv, naux));
+    // Z(X,Y) = sum_ia b(X,i,a) b(Y,i,a)  ("Mia, Nia -> MN")
+    RB_CHECK(rocblas_dgemm(c.blas, rocblas_operation_transpose, rocblas_operation_none, naux, naux, ro * rv, &one,
+                           cc, ro * rv, cc, ro * rv, &zero, Z, naux));
+    return;
+  }
+  // M3: b2_X = P_v b_X for every X (column-major ro x n blocks)
+  c.scratch_b.ensure(bsize * sizeof(double));
+  double* b2 = c.scratch_b.as<double>();
+  RB_CHECK(rocblas_dgemm_strided_batched(c.blas, rocblas_operation_none, rocblas_operation_none, ro, n, n, &one, bb,
+                                         ro, (rocblas_stride)n * ro, Pv, n, 0, &zero, b2, ro,
+                                         (rocblas_stride)n * ro, naux));
+  // M4: b3 on the pair slabs
+  c.scratch_c.ensure((size_t)std::max<int64_t>(L.total, 1) * sizeof(double));
+  double* b3 = c.scratch_c.as<double>();
+  slab_out_gemm(c, bt, b2, ro, ro, Lo, ro, b3);
+  // M5: Z = sum_slabs b3 B^T (symmetric)
+  slab_syrk(c, bt, b3, bt.data64.as<double>(), Z, c.scratch_partial);
+}
+
 double mp2_sos(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in, std::vector<double>& terms,
                double* t_z, double* Z_host)
 {
@@ -351,38 +436,26 @@ double mp2_sos(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in, std::ve
   if (bt.transformed) throw std::runtime_error("MP2 needs the untransformed three-centre tensor");
   const int n = in.nbf, naux = L.naux;
   if (n != L.nbf || met.naux != naux) throw std::runtime_error("MP2: dimensions do not match B / metric");
+  if (in.zmethod < 0 || in.zmethod > 2) throw std::runtime_error("MP2: unknown Z method");
   Mp2Setup s;
   mp2_setup(c, in, s);
-  DevBuf Zp, Zm((size_t)naux * naux * sizeof(double)), Zt((size_t)naux * naux * sizeof(double)),
+  DevBuf Zm((size_t)naux * naux * sizeof(double)), Zt((size_t)naux * naux * sizeof(double)),
       part(256 * sizeof(double));
-  DevBuf b3((size_t)std::max<int64_t>(L.total, 1) * sizeof(double));
-  const double one = 1.0, zero = 0.0;
   terms.clear();
   double total = 0.0, tz = 0.0;
+  const bool ll = in.zmethod == 2;
   for (size_t k = 0; k < in.omega.size(); ++k) {
-    DevBuf Lo, Pv;
-    const int rank = mp2_point_factors(c, s, in, k, Lo, Pv);
-    if (rank == 0) {
+    DevBuf Lo, Pv, Lv;
+    int rv = 0;
+    const int rank = mp2_point_factors(c, s, in, k, Lo, Pv, ll ? &Lv : nullptr, ll ? &rv : nullptr);
+    if (rank == 0 || (ll && rv == 0)) {
       if (Z_host) std::fill(Z_host + k * (size_t)naux * naux, Z_host + (k + 1) * (size_t)naux * naux, 0.0);
       terms.push_back(0.0);
       continue;
     }
     HIP_CHECK(hipEventRecord(c.ev0, c.stream));
-    // M2: b = B . L_o   [X][nu][i]
-    const size_t bsize = (size_t)naux * n * rank;
-    c.scratch_a.ensure(bsize * sizeof(double));
-    c.scratch_b.ensure(bsize * sizeof(double));
-    double* bb = c.scratch_a.as<double>();
-    double* b2 = c.scratch_b.as<double>();
-    slab_gemm(c, bt, Lo.as<double>(), rank, rank, bb, rank, 1.0);
-    // M3: b2_X = P_v b_X for every X (column-major rank x n blocks)
-    RB_CHECK(rocblas_dgemm_strided_batched(c.blas, rocblas_operation_none, rocblas_operation_none, rank, n, n,
-                                           &one, bb, rank, (rocblas_stride)n * rank, Pv.as<double>(), n, 0, &zero,
-                                           b2, rank, (rocblas_stride)n * rank, naux));
-    // M4: b3 on the pair slabs
-    slab_out_gemm(c, bt, b2, rank, rank, Lo.as<double>(), rank, b3.as<double>());
-    // M5: Z = sum_slabs b3 B^T (symmetric)
-    slab_syrk(c, bt, b3.as<double>(), bt.data64.as<double>(), Zm.as<double>(), Zp);
+    z_point(c, bt, in.zmethod, Lo.as<double>(), rank, Pv.as<double>(), ll ? Lv.as<double>() : nullptr, rv,
+            Zm.as<double>());
     HIP_CHECK(hipEventRecord(c.ev1, c.stream));
     HIP_CHECK(hipEventSynchronize(c.ev1));
     float ms = 0.f;
@@ -435,6 +508,7 @@ double mp2_sos_batched(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in,
   if (bt.transformed) throw std::runtime_error("MP2 needs the untransformed three-centre tensor");
   const int n = in.nbf, naux = L.naux;
   if (n != L.nbf || met.naux != naux) throw std::runtime_error("MP2: dimensions do not match B / metric");
+  if (in.zmethod == 2) throw std::runtime_error("MP2: the grouped FP32 graph implements the LLMP Z builders");
   const int npt = (int)in.omega.size();
   const auto tp0 = std::chrono::steady_clock::now();
   Mp2Setup s;

@@ -16,7 +16,19 @@ struct Mp2Input {
   std::vector<int> bsz;          // AO block sizes
   int mo_split = 5;
   double filter_eps = 1e-10;
+  // Z builder (mp::zmethod, z_builder.hpp): 0 LLMP_FULL, 1 LLMP_MEM (the same
+  // contractions batched differently; one GPU path), 2 LL_FULL (B(X|ia) route
+  // with a pivoted Cholesky factor of the virtual pseudo-density)
+  int zmethod = 0;
 };
+
+// Z of one Laplace point from given factors (mp::Z::compute, z_builder.hpp):
+//   zmethod 0/1: Z = LLMP_FULL_Z(L_o, P_v)   (z_full.cpp:69-397, z_mem.cpp:37-)
+//   zmethod 2:   Z = LL_FULL_Z(L_o, L_v)     (z_full.cpp:428-692)
+// Lo: device n x ro row-major; Pv: device n x n (zmethod 0/1); Lv: device
+// n x rv row-major (zmethod 2).  Z: device naux x naux, full symmetric.
+void z_point(Ctx& c, DevBT& bt, int zmethod, const double* Lo, int ro, const double* Pv, const double* Lv, int rv,
+             double* Z);
 
 // Pivoted Cholesky of the symmetric positive semidefinite N x N matrix in U
 // (overwritten): returns L (N x rank, row-major, rows in the original order,

@@ -213,9 +213,10 @@ class DeviceB:
         if not plan[7]:
             return None
         k = nms.value
+        ns = self.bf16_shapes()[1]
         return dict(gemm_shape=int(plan[0]), bm=int(plan[1]), bn=int(plan[2]), threads=int(plan[3]),
                     direct_to_lds=bool(plan[4]), stages=int(plan[5]), syrk_variant=int(plan[6]),
-                    gemm_ms=[float(v) for v in ms[:k - 4]], syrk_ms=[float(v) for v in ms[k - 4:k]])
+                    gemm_ms=[float(v) for v in ms[:k - ns]], syrk_ms=[float(v) for v in ms[k - ns:k]])
 
     def bf16_force(self, ng, gemm_shape, syrk_variant):
         """Use the given BF16 pipeline for ng columns instead of the autotuner's."""

@@ -21,8 +21,13 @@ def laplace_for(eps, nocc, nlap):
     return w, a, err
 
 
-def sos_mp2(mol, dB, metric, S, C, eps, nlap=5, mo_split=5, filter_eps=1e-10, points=None, batched=False):
+ZMETHODS = {"llmp_full": 0, "llmp_mem": 1, "ll_full": 2}   # mp::zmethod (z_builder.hpp)
+
+
+def sos_mp2(mol, dB, metric, S, C, eps, nlap=5, mo_split=5, filter_eps=1e-10, points=None, batched=False,
+            zmethod="llmp_full"):
     """Returns dict(energy (positive sum), scaled (1.3 x), terms, omega, alpha, t_z, t_prep).
+    zmethod: "llmp_full" / "llmp_mem" (one GPU path) or "ll_full" (FP64 only).
 
     batched=False: FP64, one launch sequence per Laplace point.  batched=True:
     the contractions of all points as grouped FP32 launches (FP64
@@ -45,10 +50,15 @@ def sos_mp2(mol, dB, metric, S, C, eps, nlap=5, mo_split=5, filter_eps=1e-10, po
     args = (dB.h, metric.h, n, nocc, _lib.dptr(S), _lib.dptr(C), _lib.dptr(eps), len(w), _lib.dptr(w),
             _lib.dptr(a), len(bsz), _lib.iptr(bsz), mo_split, filter_eps, ctypes.byref(e), _lib.dptr(terms),
             ctypes.byref(tz))
+    zm = ZMETHODS[zmethod]
     if batched:
+        if zm == 2:
+            raise ValueError("the grouped FP32 graph implements the LLMP Z builders")
         _lib.check(_lib.lib().mchip_mp2_sos_batched(*args, ctypes.byref(tp)))
-    else:
+    elif zm == 0:
         _lib.check(_lib.lib().mchip_mp2_sos(*args))
+    else:
+        _lib.check(_lib.lib().mchip_mp2_sos_z(dB.h, metric.h, zm, *args[2:]))
     return dict(energy=e.value, scaled=C_OS * e.value, terms=terms, omega=w, alpha=a, t_z=tz.value,
                 t_prep=tp.value)
 
@@ -70,3 +80,19 @@ def z_builder(mol, dB, metric, S, C, eps, omega, alpha, mo_split=5, filter_eps=1
                                       len(w), _lib.dptr(w), _lib.dptr(a), len(bsz), _lib.iptr(bsz), mo_split,
                                       filter_eps, int(bool(batched)), _lib.dptr(Z), _lib.dptr(terms)))
     return Z, terms
+
+
+def z_compute(dB, Locc, Pvir=None, Lvir=None, zmethod="llmp_full"):
+    """One Z (naux x naux) from given factors, FP64 (mp::Z::compute,
+    z_builder.hpp:43-116): LLMP builders take the virtual pseudo-density Pvir,
+    LL_FULL its Cholesky factor Lvir."""
+    zm = ZMETHODS[zmethod]
+    Lo = np.ascontiguousarray(Locc, dtype=np.float64)
+    n, ro = Lo.shape
+    Pv = None if Pvir is None else np.ascontiguousarray(Pvir, dtype=np.float64)
+    Lv = None if Lvir is None else np.ascontiguousarray(Lvir, dtype=np.float64)
+    rv = 0 if Lv is None else Lv.shape[1]
+    Z = np.zeros((dB.naux, dB.naux))
+    _lib.check(_lib.lib().mchip_z_compute(dB.h, zm, n, _lib.dptr(Lo), ro, _lib.dptr(Pv), _lib.dptr(Lv), rv,
+                                          _lib.dptr(Z)))
+    return Z

@@ -308,6 +308,21 @@ int mchip_mp2_sos_batched(mchip_dbt t, mchip_metric m, int nbf, int nocc, const 
                           const double* eps, int nlap, const double* omega, const double* alpha, int nbb,
                           const int* bsz, int mo_split, double filter_eps, double* energy, double* terms,
                           double* t_z, double* t_prep);
+/* mchip_mp2_sos with a choice of Z builder (mp::zmethod, z_builder.hpp):
+ * 0 LLMP_FULL, 1 LLMP_MEM (the same contractions; one GPU path), 2 LL_FULL
+ * (z_full.cpp:428-692: L_v = L_S CD(P~_v) from the orthogonalised virtual
+ * pseudo-density, mp_compute.cpp:309-336; b(X,i,a) = L_v^T b(X,:,i);
+ * Z = sum_ia b(X,i,a) b(Y,i,a)).  FP64. */
+int mchip_mp2_sos_z(mchip_dbt t, mchip_metric m, int zmethod, int nbf, int nocc, const double* S, const double* C,
+                    const double* eps, int nlap, const double* omega, const double* alpha, int nbb, const int* bsz,
+                    int mo_split, double filter_eps, double* energy, double* terms, double* t_z);
+/* One Z from given factors (mp::Z: set_occ_coeff / set_vir_density /
+ * set_vir_coeff, compute, zmat; z_builder.hpp:43-116).  Locc: nbf x rank_o
+ * row-major; Pvir: nbf x nbf (zmethod 0/1, ignored for 2); Lvir: nbf x rank_v
+ * row-major (zmethod 2, ignored otherwise).  Z: naux x naux row-major, full
+ * symmetric, FP64.  t holds the untransformed B. */
+int mchip_z_compute(mchip_dbt t, int zmethod, int nbf, const double* Locc, int rank_o, const double* Pvir,
+                    const double* Lvir, int rank_v, double* Z);
 /* The Z builder alone: Z_k(X,Y) = sum_{mu nu} b3_k(X|mu nu) B(Y|mu nu) of every
  * Laplace point k into Z (nlap x naux x naux doubles, row-major; zero for a
  * point whose occupied pseudo-density has rank 0), with terms[k] = e_k as

@@ -11,6 +11,7 @@ Follows src/mp/mp_compute.cpp:27-407 and src/mp/z_full.cpp:69-397:
       L_o  = L_S L~, then block filter(eps) of L_o and P_v
       Z    = LLMP_FULL_Z: b = L_o^T B; b2 = P_v b; b3 = L_o b2 on B's
              shell-pair pattern; Z_XY = sum b3(X|mn) B(Y|mn)
+             (LL_FULL_Z: L_v = L_S CD(P~_v), b(X,i,a) = L_v^T b; Z = sum_ia b b)
       e_k  = tr(Z~ Z~) with Z~ = Z V^-1      (:361-381)
   * E = sum_k e_k (printed positive), scaled E = 1.3 E (:384-389).
 
@@ -99,6 +100,16 @@ def llmp_full_z(B, Lo, Pv, pair_mask=None):
     return b3.reshape(naux, -1) @ B.reshape(naux, -1).T            # Mmn, Nmn -> MN
 
 
+def ll_full_z(B, Lo, Lv):
+    """LL_FULL_Z (z_full.cpp:428-692): b(X,i,N) = sum_m L_o(m,i) B(X|mN);
+    b(X,i,a) = sum_N L_v(N,a) b(X,i,N); Z_XY = sum_ia b(X,i,a) b(Y,i,a)."""
+    b = np.einsum("mi,xmn->xin", Lo, B, optimize=True)            # mi, Xmn -> Xin
+    c = np.einsum("na,xin->xia", Lv, b, optimize=True)            # Na, XiN -> Xia
+    naux = B.shape[0]
+    c = c.reshape(naux, -1)
+    return c @ c.T                                                # Mia, Nia -> MN
+
+
 def shellpair_mask(Bmask, bsz):
     """get_shellpairs (z_common.cpp:7-60): (M,N) on if any X block is kept,
     expanded to functions."""
@@ -114,9 +125,11 @@ def shellpair_mask(Bmask, bsz):
 
 
 def sos_mp2(B, Vinv, S, C, eps, nocc, omega, alpha, bsz, mo_split=5, filter_eps=1e-10,
-            Bmask=None, return_terms=False, return_z=False):
+            Bmask=None, return_terms=False, return_z=False, zmethod="llmp_full"):
     """mpmod::compute.  Returns the positive sum E = sum_k e_k (unscaled);
-    with return_terms also the e_k, with return_z also the Z_k of every point."""
+    with return_terms also the e_k, with return_z also the Z_k of every point.
+    zmethod "ll_full" factors the orthogonalised virtual pseudo-density too
+    (mp_compute.cpp:309-336) and builds Z with ll_full_z."""
     Co, Cv = C[:, :nocc], C[:, nocc:]
     eo, ev = eps[:nocc], eps[nocc:]
     LS = np.linalg.cholesky(S)
@@ -135,7 +148,13 @@ def sos_mp2(B, Vinv, S, C, eps, nocc, omega, alpha, bsz, mo_split=5, filter_eps=
         Lo = LS @ Lt
         Lo = block_filter(Lo, bsz, split_range(rank, mo_split), filter_eps)
         Pv = block_filter(Pv, bsz, bsz, filter_eps)
-        Z = llmp_full_z(B, Lo, Pv, pm)
+        if zmethod == "ll_full":
+            cvo = LSinv @ Cve
+            Lvt, rank_v = pivoted_cholesky(w ** 0.25 * cvo @ cvo.T, filter_eps=filter_eps)
+            Lv = block_filter(LS @ Lvt, bsz, split_range(rank_v, mo_split), filter_eps)
+            Z = ll_full_z(B, Lo, Lv)
+        else:
+            Z = llmp_full_z(B, Lo, Pv, pm)
         if return_z:
             Zs.append(Z)
         Zt = Z @ Vinv

@@ -123,3 +123,62 @@ def test_z_builder_vs_oracle(ctx):
         assert e32 < 5e-5
     assert np.abs(t64 - np.array(terms)).max() < 1e-9 * E
     assert np.abs(t32 - t64).max() < BATCH_TOL * E
+
+
+@pytest.mark.parametrize("nwater", [1, 3])
+def test_ll_full_and_llmp_mem_vs_oracle(ctx, nwater):
+    """The other Z builders behind mp::zmethod: LL_FULL_Z (pivoted Cholesky of
+    the virtual pseudo-density, B(X|ia) route) against the oracle's LL_FULL
+    restatement to 1e-9 relative; LLMP_MEM_Z (the same contractions as
+    LLMP_FULL) reproduces the LLMP_FULL energy bit for bit."""
+    from chem_amd.engine import DeviceB, Metric
+    from chem_amd.mp2 import sos_mp2, laplace_for
+    mol = desc.Molecule(desc.water_cluster(nwater, 5), dfbasis="cc-pvdz-ri")
+    print("case: LL_FULL / LLMP_MEM water n=%d" % nwater, flush=True)
+    S, T, V = ints.int1e(mol)
+    Bh = ints.BTensorHost.build(mol, 1e-10)
+    B = Bh.dense()
+    M = ints.int2c2e(mol)
+    Vinv0, _, invsqrt0 = dfock.metric_factors(M)
+    jk = lambda P, C: (dfock.df_j(B, Vinv0, P), dfock.dfmo_k(B, invsqrt0, C))
+    r = oscf.rhf(S, T, V, mol.nuclear_repulsion(), mol.nocc, jk, bsz=mol.b, thr=1e-9)
+    dB = DeviceB(ctx, Bh)
+    met = Metric(ctx, M)
+    _, Vinv = met.get()
+    w, a, _ = laplace_for(r["eps"], mol.nocc, 6)
+    ll = sos_mp2(mol, dB, met, S, r["C"], r["eps"], points=(w, a), zmethod="ll_full")
+    full = sos_mp2(mol, dB, met, S, r["C"], r["eps"], points=(w, a))
+    mem = sos_mp2(mol, dB, met, S, r["C"], r["eps"], points=(w, a), zmethod="llmp_mem")
+    E, terms = omp2.sos_mp2(B, Vinv, S, r["C"], r["eps"], mol.nocc, w, a, mol.b, Bmask=Bh.mask(),
+                            return_terms=True, zmethod="ll_full")
+    print("LL_FULL gpu %.12f oracle %.12f rel %.2e; LLMP_FULL %.12f" % (
+        ll["energy"], E, abs(ll["energy"] - E) / E, full["energy"]))
+    assert abs(ll["energy"] - E) < 1e-9 * E
+    assert np.abs(ll["terms"] - np.array(terms)).max() < 1e-9 * E
+    assert abs(ll["energy"] - full["energy"]) < 1e-8 * E
+    assert mem["energy"] == full["energy"]
+
+
+def test_z_compute_from_factors(ctx):
+    """mp::Z from given factors (set_occ_coeff / set_vir_density /
+    set_vir_coeff): LLMP and LL_FULL Z against the oracle's restatements, with
+    random factors on two waters, to 1e-12 of max|Z|."""
+    from chem_amd.engine import DeviceB
+    from chem_amd.mp2 import z_compute
+    mol = desc.Molecule(desc.water_cluster(2, 5), dfbasis="cc-pvdz-ri")
+    print("case: Z from factors, 2 waters", flush=True)
+    Bh = ints.BTensorHost.build(mol, 1e-10)
+    B = Bh.dense()
+    dB = DeviceB(ctx, Bh)
+    rng = np.random.default_rng(11)
+    n = mol.nbf
+    Lo = rng.standard_normal((n, 6))
+    Lv = rng.standard_normal((n, 9))
+    Pv = Lv @ Lv.T
+    pm = omp2.shellpair_mask(Bh.mask(), mol.b)
+    Zl = z_compute(dB, Lo, Pvir=Pv)
+    Zo = omp2.llmp_full_z(B, Lo, Pv, pm)
+    assert np.abs(Zl - Zo).max() < 1e-12 * np.abs(Zo).max()
+    Zf = z_compute(dB, Lo, Lvir=Lv, zmethod="ll_full")
+    Zfo = omp2.ll_full_z(B, Lo, Lv)
+    assert np.abs(Zf - Zfo).max() < 1e-12 * np.abs(Zfo).max()

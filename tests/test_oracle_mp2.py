@@ -99,3 +99,37 @@ def test_laplace_sos_mp2_vs_canonical(water_hf):
             assert rel < prev
         prev = rel
     assert 0.1 < ref < 1.0   # SOS-MP2 opposite-spin sum of one water, cc-pVDZ: a few tenths of Eh
+
+
+def test_ll_full_z_equals_llmp_full_z(water_hf):
+    """Z of LL_FULL (z_full.cpp:428-692) from L_v equals Z of LLMP_FULL with
+    P_v = L_v L_v^T (no pair mask): the two reference routes compute the same
+    tensor, tr(P_o B_X P_v B_Y).  A transposed or mis-indexed contraction in
+    either restatement breaks the equality."""
+    mol, S, B, mask, Vinv, r = water_hf
+    rng = np.random.default_rng(3)
+    n = B.shape[1]
+    Lo = rng.standard_normal((n, 4))
+    Lv = rng.standard_normal((n, 7))
+    Z1 = mp2.ll_full_z(B, Lo, Lv)
+    Z2 = mp2.llmp_full_z(B, Lo, Lv @ Lv.T)
+    assert np.abs(Z1 - Z2).max() < 1e-12 * np.abs(Z2).max()
+    # and against a direct sum over (i, a) for a few elements
+    for X, Y in ((0, 0), (3, 11), (len(B) - 1, 5)):
+        direct = sum((Lo[:, i] @ B[X] @ Lv[:, a]) * (Lo[:, i] @ B[Y] @ Lv[:, a])
+                     for i in range(4) for a in range(7))
+        assert abs(Z1[X, Y] - direct) < 1e-12 * np.abs(Z1).max()
+
+
+def test_ll_full_energy_matches_llmp(water_hf):
+    """The SOS-MP2 energy through LL_FULL_Z (pivoted Cholesky of the
+    orthogonalised virtual pseudo-density) equals the LLMP_FULL_Z energy to the
+    Cholesky threshold."""
+    mol, S, B, mask, Vinv, r = water_hf
+    C, eps, nocc = r["C"], r["eps"], mol.nocc
+    ymin = 2 * (eps[nocc] - eps[nocc - 1])
+    ymax = 2 * (eps[-1] - eps[0])
+    w, a, _ = laplace.laplace_points(6, ymin, ymax)
+    E1 = mp2.sos_mp2(B, Vinv, S, C, eps, nocc, w, a, mol.b, Bmask=mask)
+    E2 = mp2.sos_mp2(B, Vinv, S, C, eps, nocc, w, a, mol.b, Bmask=mask, zmethod="ll_full")
+    assert abs(E1 - E2) < 1e-9 * E1, (E1, E2)
