@@ -273,7 +273,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profiling mode: exchange builds only (no SCF), one JSON line of timings")
-    ap.add_argument("--escalate", type=float, default=0.0,
+    ap.add_argument("--escalate", type=float, default=1e-5,
                     help="auto form: error RMS at which BF16x3 builds give way to FP32 (0: off)")
     ap.add_argument("--stall", type=int, default=3,
                     help="auto form: iterations without progress before BF16x3 gives way to FP32 (0: off)")

@@ -18,6 +18,7 @@
 #include "dfock.hpp"
 #include "ints_dev.hpp"
 #include "kernels.hpp"
+#include "linalg.hpp"
 #include "mp2.hpp"
 
 namespace mchip {
@@ -53,6 +54,7 @@ void ctx_free(Ctx& c)
   c.scratch_partial.release();
   c.syrk_tiles.clear();
   c.info.release();
+  if (c.stream) (void)hipStreamSynchronize(c.stream);   // stream-ordered frees land first
   if (c.blas) rocblas_destroy_handle(c.blas);
   if (c.stream) (void)hipStreamDestroy(c.stream);
   if (c.ev0) (void)hipEventDestroy(c.ev0);
@@ -228,36 +230,19 @@ void metric_create(Ctx& c, int naux, const double* V_host, Metric& m)
   m.Linv.alloc(bytes);
   m.Vinv.alloc(bytes);
   double* A = m.Linv.as<double>();
-  int* info = c.info.as<int>();
+  // upload, read back and compare before factorising (cheap next to the
+  // factorisation, and the factor feeds every later step)
   std::vector<double> back((size_t)naux * naux);
-  // upload, read back and compare before factorising; a failed factorisation
-  // of a verified upload is retried once from a fresh upload.  Both events are
-  // reported on stderr (they have been seen only with several requests on the
-  // device at once).
-  int h_info = 0;
-  for (int attempt = 0; attempt < 2; ++attempt) {
-    int bad = 0;
-    for (int up = 0; up < 3; ++up) {
-      h2d(m.Linv.p, V_host, bytes, c.stream);
-      d2h(back.data(), m.Linv.p, bytes, c.stream);
-      bad = std::memcmp(back.data(), V_host, bytes) != 0;
-      if (!bad) break;
-      std::fprintf(stderr, "[mchip] metric upload differs from the host matrix (naux %d, attempt %d)\n", naux, up);
-    }
-    if (bad) throw std::runtime_error("metric upload does not reproduce the host matrix");
-    RB_CHECK(rocsolver_dpotrf(c.blas, rocblas_fill_lower, naux, A, naux, info));
-    d2h(&h_info, info, sizeof(int), c.stream);
-    if (h_info == 0) break;
-    std::fprintf(stderr, "[mchip] metric Cholesky of a verified upload failed at pivot %d (naux %d, attempt %d)\n",
-                 h_info, naux, attempt);
-  }
+  h2d(m.Linv.p, V_host, bytes, c.stream);
+  d2h(back.data(), m.Linv.p, bytes, c.stream);
+  if (std::memcmp(back.data(), V_host, bytes) != 0)
+    throw std::runtime_error("metric upload does not reproduce the host matrix");
+  const int h_info = chol_lower(c, A, naux, naux);
   if (h_info != 0)
-    throw std::runtime_error("metric Cholesky failed twice on a verified upload (pivot " + std::to_string(h_info) +
-                             ", naux " + std::to_string(naux) + ")");
+    throw std::runtime_error("metric Cholesky failed (matrix not positive definite at pivot " +
+                             std::to_string(h_info) + " of " + std::to_string(naux) + ")");
   hipLaunchKernelGGL(k_zero_upper_cm, dim3(nblk((int64_t)naux * naux, 256)), dim3(256), 0, c.stream, A, naux);
-  RB_CHECK(rocsolver_dtrtri(c.blas, rocblas_fill_lower, rocblas_diagonal_non_unit, naux, A, naux, info));
-  d2h(&h_info, info, sizeof(int), c.stream);
-  if (h_info != 0) throw std::runtime_error("metric triangular inverse failed");
+  trtri_lower(c, A, naux, naux);
   hipLaunchKernelGGL(k_zero_upper_cm, dim3(nblk((int64_t)naux * naux, 256)), dim3(256), 0, c.stream, A, naux);
   // V^-1 = L^-T L^-1 (column-major: Linv^T * Linv)
   const double one = 1.0, zero = 0.0;
@@ -668,7 +653,7 @@ struct Bf16Shape {
 static const Bf16Shape kBf16Shapes[] = {
     {2, 2, 4, 2, 2, 0}, {2, 2, 4, 4, 2, 0}, {2, 2, 4, 5, 2, 0}, {2, 4, 4, 2, 2, 0}, {2, 4, 4, 3, 2, 0},
     {2, 4, 4, 4, 2, 0}, {2, 4, 4, 5, 2, 0}, {2, 4, 4, 2, 1, 0}, {2, 4, 4, 3, 1, 0}, {2, 4, 4, 2, 4, 1},
-    {2, 4, 4, 3, 3, 1}, {2, 4, 4, 5, 2, 1}, {1, 4, 4, 5, 3, 1}};
+    {2, 4, 4, 3, 3, 1}, {2, 4, 4, 5, 2, 1}, {1, 4, 4, 5, 3, 1}, {2, 2, 8, 5, 3, 1}, {2, 2, 8, 5, 2, 1}};
 static constexpr int kNumBf16Shapes = sizeof(kBf16Shapes) / sizeof(kBf16Shapes[0]);
 // SYRK variants: 0 = register-staged on FP32 W; 1..3 = direct-to-LDS on bf16
 // hi/lo W with 2, 3, 4 stages
@@ -705,6 +690,8 @@ static void slab_gemm_bf16(Ctx& c, DevBT& t, int shape, const uint16_t* ghi, con
     case 10: MCHIP_BF16_SLAB_DL(2, 4, 4, 3, 3); break;
     case 11: MCHIP_BF16_SLAB_DL(2, 4, 4, 5, 2); break;
     case 12: MCHIP_BF16_SLAB_DL(1, 4, 4, 5, 3); break;
+    case 13: MCHIP_BF16_SLAB_DL(2, 2, 8, 5, 3); break;
+    case 14: MCHIP_BF16_SLAB_DL(2, 2, 8, 5, 2); break;
     default: throw std::runtime_error("BF16 slab GEMM: unknown launch shape");
   }
 #undef MCHIP_BF16_SLAB

@@ -638,6 +638,8 @@ MCHIP_SLAB16DL_INST(2, 4, 4, 2, 4)
 MCHIP_SLAB16DL_INST(2, 4, 4, 3, 3)
 MCHIP_SLAB16DL_INST(2, 4, 4, 5, 2)
 MCHIP_SLAB16DL_INST(1, 4, 4, 5, 3)
+MCHIP_SLAB16DL_INST(2, 2, 8, 5, 3)
+MCHIP_SLAB16DL_INST(2, 2, 8, 5, 2)
 #undef MCHIP_SLAB16DL_INST
 #define MCHIP_SYRK16DL_INST(S, BM)                                                                     \
   template __global__ void k_seg_syrk_bf16x3_dl<S, BM>(const uint16_t*, const uint16_t*, int, int64_t, int, \

@@ -26,6 +26,7 @@
 
 #include "dfock.hpp"
 #include "kernels.hpp"
+#include "linalg.hpp"
 #include "mp2.hpp"
 
 namespace mchip {
@@ -265,11 +266,11 @@ static void mp2_setup(Ctx& c, const Mp2Input& in, Mp2Setup& s)
   if (no) h2d(s.eo.p, in.eps, no * sizeof(double), c.stream);
   if (nv) h2d(s.ev.p, in.eps + no, nv * sizeof(double), c.stream);
   // S = L_S L_S^T (column-major lower), L_S^-1
-  int* info = c.info.as<int>();
-  RB_CHECK(rocsolver_dpotrf(c.blas, rocblas_fill_lower, n, s.LS.as<double>(), n, info));
+  if (chol_lower(c, s.LS.as<double>(), n, n) != 0)
+    throw std::runtime_error("MP2: the overlap matrix is not positive definite");
   hipLaunchKernelGGL(k_zero_upper_cm, dim3(nblk((int64_t)nn, 256)), dim3(256), 0, c.stream, s.LS.as<double>(), n);
   HIP_CHECK(hipMemcpyAsync(s.LSinv.p, s.LS.p, nn * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-  RB_CHECK(rocsolver_dtrtri(c.blas, rocblas_fill_lower, rocblas_diagonal_non_unit, n, s.LSinv.as<double>(), n, info));
+  trtri_lower(c, s.LSinv.as<double>(), n, n);
   hipLaunchKernelGGL(k_zero_upper_cm, dim3(nblk((int64_t)nn, 256)), dim3(256), 0, c.stream, s.LSinv.as<double>(), n);
   std::vector<int> boff(in.bsz.size() + 1, 0);
   for (size_t i = 0; i < in.bsz.size(); ++i) boff[i + 1] = boff[i] + in.bsz[i];
@@ -649,8 +650,7 @@ double mp2_sos_batched(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in,
     Zg64.alloc((size_t)naux * naux * sizeof(double));
     HIP_CHECK(hipMemcpyAsync(Lf.p, met.Linv.p, (size_t)naux * naux * sizeof(double), hipMemcpyDeviceToDevice,
                              c.stream));
-    RB_CHECK(rocsolver_dtrtri(c.blas, rocblas_fill_lower, rocblas_diagonal_non_unit, naux, Lf.as<double>(), naux,
-                              c.info.as<int>()));
+    trtri_lower(c, Lf.as<double>(), naux, naux);
   }
   double total = 0.0;
   for (int g = 0; g < npt; ++g) {

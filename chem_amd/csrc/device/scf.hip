@@ -417,8 +417,10 @@ void select_k_builders(Ctx& c, DevBT& bt, const Metric* m, int kmode, int nocc, 
     kfinal = k64;
   }
   else if (kmode == 4) {
-    if (!bt.lowp_tilde || !bt.data_hi.p || !bt.data32.p)
-      throw std::runtime_error("SCF: the escalation mode needs the BF16 and FP32 copies of L^-1 B");
+    if (!bt.lowp_tilde || !bt.data_hi.p)
+      throw std::runtime_error("SCF: the escalation mode needs the BF16 split copy of L^-1 B");
+    // without the FP32 copy the FP32 stage is skipped (BF16x3 -> FP64)
+    const bool have32 = bt.data32.p != nullptr;
     // stage 0 BF16x3, 1 FP32, 2 FP64: a stage ends at the RMS trigger (BF16x3
     // only) or after `stall` iterations without progress once the error RMS is
     // below the stage's arming level (early SCF iterations also stall, far
@@ -431,8 +433,8 @@ void select_k_builders(Ctx& c, DevBT& bt, const Metric* m, int kmode, int nocc, 
     auto st = std::make_shared<Esc>();
     const double tau = c.escalate_rms;
     const int stall = c.escalate_stall;
-    kb = [cp, t, nocc, occ_nbatches, tau, stall, st, k64](int it, double prev_rms, const double* P,
-                                                          const double* Cocc, double* K) {
+    kb = [cp, t, nocc, occ_nbatches, tau, stall, st, k64, have32](int it, double prev_rms, const double* P,
+                                                                  const double* Cocc, double* K) {
       bool switched = false;
       constexpr double arm[2] = {1e-4, 1e-6};
       if (st->stage < 2 && std::isfinite(prev_rms)) {
@@ -448,7 +450,7 @@ void select_k_builders(Ctx& c, DevBT& bt, const Metric* m, int kmode, int nocc, 
         }
         st->best = std::min(st->best, prev_rms);
         if (next) {
-          ++st->stage;
+          st->stage += (st->stage == 0 && !have32) ? 2 : 1;
           st->best = std::numeric_limits<double>::infinity();
           st->flat = 0;
           switched = true;

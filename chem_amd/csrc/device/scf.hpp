@@ -42,8 +42,8 @@ using KBuildFn = std::function<bool(int, double, const double*, const double*, d
 // build, 1 DFMO on the transformed B, 2 FP32, 3 BF16x3, 4 BF16x3 escalating to
 // FP32 once the previous error RMS is <= c.escalate_rms or, below 1e-4, has not
 // fallen below 0.8 x its lowest value for c.escalate_stall iterations, and from
-// FP32 to FP64 after such a stall below 1e-6; modes
-// 2-4 get an FP64 final builder, with final_occ_nbatches occupied batches,
+// FP32 to FP64 after such a stall below 1e-6 (straight from BF16x3 to FP64 when
+// the tensor has no FP32 copy); modes 2-4 get an FP64 final builder, with final_occ_nbatches occupied batches,
 // 0 = occ_nbatches).  The builders capture c, bt and m by reference.
 void select_k_builders(Ctx& c, DevBT& bt, const Metric* m, int kmode, int nocc, int occ_nbatches,
                        double threshold, KBuildFn& kb, KBuildFn& kfinal, int final_occ_nbatches = 0);

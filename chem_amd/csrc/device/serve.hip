@@ -224,10 +224,42 @@ void Server::finish(SrvJob& j, bool admitted)
   cv_.notify_all();
 }
 
+// Page-locked host memory that only grows; HostArray views its first n
+// doubles.
+struct PinnedArena {
+  double* p = nullptr;
+  size_t n = 0;
+  double* get(size_t m)
+  {
+    if (m > n) {
+      if (p) (void)hipHostFree(p);
+      p = nullptr;
+      n = 0;
+      HIP_CHECK(hipHostMalloc((void**)&p, m * sizeof(double), hipHostMallocDefault));
+      n = m;
+    }
+    return p;
+  }
+  ~PinnedArena()
+  {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
+struct HostArray {
+  double* p;
+  size_t n;
+  HostArray(PinnedArena& a, size_t m) : p(a.get(m)), n(m) {}
+  double* data() { return p; }
+  size_t size() const { return n; }
+  double& operator[](size_t i) { return p[i]; }
+};
+
 void Server::worker(int slot)
 {
   HIP_CHECK(hipSetDevice(device_));
   Ctx& c = *ctx_[slot];
+  alloc_stream() = c.stream;   // stream-ordered device allocations for this worker
   omp_set_num_threads(cpu_per_worker_);
   while (true) {
     const int id = take();
@@ -245,6 +277,14 @@ void Server::worker(int slot)
     catch (const std::exception& e) {
       j->failed = true;
       j->error = e.what();
+      // the request's buffers are freed in stream order: let the frees land
+      // before its memory leaves the budget
+      c.scratch_a.release();
+      c.scratch_b.release();
+      c.scratch_c.release();
+      c.scratch_partial.release();
+      c.syrk_tiles.clear();
+      (void)hipStreamSynchronize(c.stream);
     }
     finish(*j, admitted);
   }
@@ -260,7 +300,9 @@ void Server::run(Ctx& c, SrvJob& j, bool& admitted)
   j.nbf = n;
   j.naux = naux;
   // host arrays that go to the device live in page-locked memory (DMA copies)
-  PinnedVec<double> S((size_t)n * n), T((size_t)n * n), M((size_t)naux * naux);
+  // (kept per worker across requests: hipHostFree synchronises the whole device)
+  static thread_local PinnedArena pS, pT, pM;
+  HostArray S(pS, (size_t)n * n), T(pT, (size_t)n * n), M(pM, (size_t)naux * naux);
   std::vector<double> V((size_t)n * n);
   ints_1e(ao, j.charges, S.data(), T.data(), V.data());
   for (size_t i = 0; i < S.size(); ++i) T[i] += V[i];   // H = T + V
@@ -286,7 +328,8 @@ void Server::run(Ctx& c, SrvJob& j, bool& admitted)
                + 64.0 * 1048576.0;                         // tables and small buffers
   // FP32 copy; BF16 hi/lo copy (rows padded to 32) and its split-K partials
   // (one n x n FP64 block per 32 aux functions)
-  const double copies = (j.kmode == 2 || j.kmode == 4 ? 4.0 * tot : 0.0) +
+  // (the escalation mode runs without the FP32 copy here: BF16x3, then FP64)
+  const double copies = (j.kmode == 2 ? 4.0 * tot : 0.0) +
                         (j.kmode == 3 || j.kmode == 4
                              ? 4.0 * tot + 2.0 * 32.0 * naux * nbb * 8.0 + 8.0 * n * (double)n * ((naux + 31) / 32)
                              : 0.0);
@@ -316,7 +359,7 @@ void Server::run(Ctx& c, SrvJob& j, bool& admitted)
     Metric m;
     metric_create(c, naux, M.data(), m);
     if (j.kmode == 1) transform_B(c, t, m);
-    if (j.kmode >= 2) dbt_make_lowp(c, t, m, j.kmode == 2 || j.kmode == 4, j.kmode == 3 || j.kmode == 4);
+    if (j.kmode >= 2) dbt_make_lowp(c, t, m, j.kmode == 2, j.kmode == 3 || j.kmode == 4);
     j.t_setup = now_s() - tp;
     tp = now_s();
     ScfInput in;

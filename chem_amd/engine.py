@@ -39,7 +39,7 @@ class Context:
     def set_timing(self, on):
         _lib.check(_lib.lib().mchip_ctx_set_timing(self.h, int(bool(on))))
 
-    def set_escalation(self, rms=0.0, stall=3):
+    def set_escalation(self, rms=1e-5, stall=3):
         """When the escalation K mode switches BF16x3 -> FP32: at an error RMS
         <= rms (0: off) or after `stall` iterations without progress (0: off)."""
         _lib.check(_lib.lib().mchip_ctx_set_escalation(self.h, float(rms), int(stall)))

@@ -126,7 +126,7 @@ int mchip_ctx_stream(mchip_ctx c, void** stream);
 int mchip_ctx_set_timing(mchip_ctx c, int on);
 /* When the escalation K mode (4) switches from the BF16x3 to the FP32
  * exchange: once the previous iteration's error RMS is <= rms (0 turns this
- * trigger off; default 0), or after `stall` consecutive iterations below an
+ * trigger off; default 1e-5), or after `stall` consecutive iterations below an
  * error RMS of 1e-4 whose RMS stayed above 0.8 x the lowest so far (0 turns it
  * off; default 3).  FP32 gives way to FP64 after such a stall below 1e-6.  No
  * counterpart in the reference. */
@@ -236,10 +236,10 @@ int mchip_dfk_bf16_shapes(int* ngemm, int* nsyrk, int* direct);
  *        3 = mixed with the BF16x3 exchange in every iteration (needs
  *            mchip_dbt_make_bf16), FP64 exchange for the reported energy,
  *        4 = precision escalation: BF16x3 until the trigger set by
- *            mchip_ctx_set_escalation fires (by default: three iterations
- *            without progress of the error RMS), then FP32 until it stalls in
- *            the same way, then FP64; FP64 for the reported energy (needs both
- *            copies).
+ *            mchip_ctx_set_escalation fires (by default: error RMS <= 1e-5,
+ *            or three iterations without progress), then FP32 until it stalls in
+ *            the same way, then FP64; FP64 for the reported energy (needs the
+ *            BF16 copy; without the FP32 copy BF16x3 gives way to FP64).
  * Outputs C (nbf x nbf row-major), orbital energies and per-iteration
  * history (energy, RMS error, J and K seconds) up to max_hist entries.
  * ---------------------------------------------------------------------- */

@@ -225,4 +225,69 @@ class create_k_base {
 inline create_k_base create_k() { return create_k_base(); }
 
 }  // namespace fock
+
+// Z builders of SOS-AO-MP2 with the names and call pattern of
+// src/mp/z_builder.hpp (mp::Z, zmethod, create_z):
+//
+//   auto z = mchip::mp::create_z().method(mchip::mp::zmethod::llmp_full)
+//                .eri3c2e_batched(b).build();   // b: untransformed B (RI basis)
+//   z->set_occ_coeff(locc).set_vir_density(pvir);    // LLMP builders
+//   z->compute();
+//   auto Z = z->zmat();                               // naux x naux
+namespace mp {
+
+using fock::matrix;
+using fock::shared_matrix;
+
+// z_builder.hpp:32-36
+enum class zmethod { llmp_full, llmp_mem, ll_full };
+
+// mp::Z (z_builder.hpp:43-116): occupied Cholesky factor L_occ (nbf x rank),
+// virtual pseudo-density P_vir (LLMP builders) or its factor L_vir (LL_FULL).
+class Z {
+  mchip_dbt m_b;
+  zmethod m_method;
+  shared_matrix m_locc, m_lvir, m_pvir, m_zmat;
+
+ public:
+  Z(mchip_dbt b, zmethod m) : m_b(b), m_method(m) {}
+  Z& set_occ_coeff(shared_matrix locc) { m_locc = std::move(locc); return *this; }
+  Z& set_vir_coeff(shared_matrix lvir) { m_lvir = std::move(lvir); return *this; }
+  Z& set_vir_density(shared_matrix pvir) { m_pvir = std::move(pvir); return *this; }
+  void init() {}
+  void compute()
+  {
+    if (!m_locc) throw std::runtime_error("Z: occupied coefficients not set");
+    const int nbf = m_locc->rows;
+    const int zm = m_method == zmethod::llmp_full ? 0 : m_method == zmethod::llmp_mem ? 1 : 2;
+    if (zm == 2 && !m_lvir) throw std::runtime_error("Z: LL_FULL needs the virtual coefficients");
+    if (zm != 2 && !m_pvir) throw std::runtime_error("Z: LLMP needs the virtual density");
+    int dims[4] = {0, 0, 0, 0};
+    int64_t nelem = 0;
+    fock::check(mchip_dbt_info(m_b, dims, &nelem));
+    const int naux = dims[0];
+    if (!m_zmat || m_zmat->rows != naux) m_zmat = std::make_shared<matrix>(naux, naux);
+    fock::check(mchip_z_compute(m_b, zm, nbf, m_locc->ptr(), m_locc->cols, m_pvir ? m_pvir->ptr() : nullptr,
+                                m_lvir ? m_lvir->ptr() : nullptr, m_lvir ? m_lvir->cols : 0, m_zmat->ptr()));
+  }
+  shared_matrix zmat() { return m_zmat; }
+};
+
+// create_z (z_builder.hpp:324-406)
+class create_z_base {
+  zmethod c_method = zmethod::llmp_full;
+  mchip_dbt c_b = nullptr;
+
+ public:
+  create_z_base& method(zmethod m) { c_method = m; return *this; }
+  create_z_base& eri3c2e_batched(mchip_dbt b) { c_b = b; return *this; }
+  std::shared_ptr<Z> build()
+  {
+    if (!c_b) throw std::runtime_error("create_z: eri3c2e_batched is required");
+    return std::make_shared<Z>(c_b, c_method);
+  }
+};
+inline create_z_base create_z() { return create_z_base(); }
+
+}  // namespace mp
 }  // namespace mchip

@@ -79,6 +79,13 @@ int main() {
     (void)j;
     return 4;
   } catch (const std::runtime_error&) {}
+  try {
+    auto z = mchip::mp::create_z().method(mchip::mp::zmethod::ll_full).build();
+    (void)z;
+    return 5;
+  } catch (const std::runtime_error& e) {
+    if (std::strstr(e.what(), "eri3c2e_batched") == nullptr) return 6;
+  }
   std::printf("ok\n");
   return 0;
 }

@@ -125,11 +125,11 @@ def test_mixed_auto_scf_energy(ctx):
 
 
 def test_mixed_auto_stall_trigger(ctx):
-    """Default escalation (BF16x3 until three iterations pass without the error
-    RMS falling below 0.8 x its lowest value, then FP32 until it stalls the same
-    way, then FP64) down to a threshold below the BF16x3 and FP32 noise floors:
-    the run converges and the final FP64 energy agrees with the plain SCF to
-    1e-8 Eh."""
+    """Default escalation (BF16x3 until the error RMS reaches 1e-5 or three
+    iterations pass without it falling below 0.8 x its lowest value, then FP32
+    until it stalls that way, then FP64) down to a threshold below the BF16x3
+    and FP32 noise floors: the run converges and the final FP64 energy agrees
+    with the plain SCF to 1e-8 Eh."""
     from chem_amd.engine import DeviceB, Metric
     from chem_amd.scf import run_rhf
     mol = desc.Molecule(desc.water_cluster(4, 11))
@@ -165,5 +165,25 @@ def test_lowp_route_scf_energy(ctx, kmode):
     dB.make_lowp(m)
     mix = run_rhf(mol, dB, m, S, T + V, threshold=1e-7, kmode=kmode)
     print("%s: E64 %.10f Emix %.10f dE %.2e" % (kmode, ref.energy, mix.energy, mix.energy - ref.energy))
+    assert ref.converged and mix.converged
+    assert abs(mix.energy - ref.energy) < 1e-8
+
+
+def test_mixed_auto_without_fp32_copy(ctx):
+    """The escalation mode on a tensor with only the BF16 copy (as the request
+    loop runs it): BF16x3 until the error stalls, then the plain FP64 exchange;
+    it converges below the BF16x3 noise floor with |dE| < 1e-8 Eh."""
+    from chem_amd.engine import DeviceB, Metric
+    from chem_amd.scf import run_rhf
+    mol = desc.Molecule(desc.water_cluster(4, 11))
+    S, T, V = ints.int1e(mol)
+    Bh = ints.BTensorHost.build(mol, 1e-10)
+    m = Metric(ctx, ints.int2c2e(mol))
+    dB = DeviceB(ctx, Bh)
+    ref = run_rhf(mol, dB, m, S, T + V, threshold=1e-9, kmode="dfmo")
+    dB.make_lowp(m, f32=False, bf16=True)
+    mix = run_rhf(mol, dB, m, S, T + V, threshold=1e-9, kmode="mixed_auto", max_iter=60)
+    print("plain %d iters; auto without FP32 copy: %d iters dE %.2e" % (
+        ref.iterations, mix.iterations, mix.energy - ref.energy))
     assert ref.converged and mix.converged
     assert abs(mix.energy - ref.energy) < 1e-8
