@@ -139,7 +139,8 @@ _SIGS = [
     ("mchip_dfk_mo_f32_dev", c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     ("mchip_dfk_mo_f32", c_int, [c_void_p, P_double, c_int, c_int, P_double]),
     ("mchip_dbt_make_bf16", c_int, [c_void_p]),
-    ("mchip_dbt_make_lowp", c_int, [c_void_p, c_void_p, c_int, c_int]),
+    ("mchip_dbt_make_lowp", c_int, [c_void_p, c_void_p, c_int, c_int, c_int]),
+    ("mchip_dfk_mo_tilde64", c_int, [c_void_p, P_double, c_int, c_int, P_double]),
     ("mchip_dfk_mo_bf16_dev", c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     ("mchip_dfk_mo_bf16", c_int, [c_void_p, P_double, c_int, c_int, P_double]),
     ("mchip_dfk_bf16_tile", c_int, [c_void_p, c_int, P_int]),

@@ -350,12 +350,26 @@ extern "C" int mchip_dbt_make_bf16(mchip_dbt t)
   MCHIP_CATCH
 }
 
-extern "C" int mchip_dbt_make_lowp(mchip_dbt t, mchip_metric m, int f32, int bf16)
+extern "C" int mchip_dbt_make_lowp(mchip_dbt t, mchip_metric m, int f32, int bf16, int f64)
 {
   MCHIP_TRY
   need_device(t->ctx);
   if (!m) throw std::runtime_error("mchip_dbt_make_lowp: the metric is required");
-  dbt_make_lowp(t->ctx->c, t->t, m->m, f32 != 0, bf16 != 0);
+  dbt_make_lowp(t->ctx->c, t->t, m->m, f32 != 0, bf16 != 0, f64 != 0);
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dfk_mo_tilde64(mchip_dbt t, const double* C, int nocc, int occ_nbatches, double* K)
+{
+  MCHIP_TRY
+  need_device(t->ctx);
+  if (!t->t.data64t.p) throw std::runtime_error("mchip_dfk_mo_tilde64: no FP64 L^-1 B copy (mchip_dbt_make_lowp f64)");
+  Ctx& c = t->ctx->c;
+  const int n = t->t.L.nbf;
+  DevBuf dC((size_t)n * std::max(nocc, 1) * sizeof(double)), dK((size_t)n * n * sizeof(double));
+  if (nocc) h2d(dC.p, C, (size_t)n * nocc * sizeof(double), c.stream);
+  dfk_mo(c, t->t, nullptr, dC.as<double>(), nocc, nocc, dK.as<double>(), occ_nbatches, t->t.data64t.as<double>());
+  d2h(K, dK.p, (size_t)n * n * sizeof(double), c.stream);
   MCHIP_CATCH
 }
 

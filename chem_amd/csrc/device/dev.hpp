@@ -239,6 +239,7 @@ struct DevBT {
   BLayout L;                 // host copy of the layout
   DevBuf data64;             // FP64 slabs (B or B~)
   DevBuf data32;             // FP32 slabs (optional)
+  DevBuf data64t;            // FP64 L^-1 B slabs beside the untransformed B (optional)
   DevBuf data_hi, data_lo;   // BF16 split slabs (optional), rows padded to ldp[N]
   DevBuf ldp, off16, ldp_pre;               // padded row length, slab offsets, prefix of ldp
   std::vector<int> h_ldp;

@@ -332,10 +332,13 @@ void dfj(Ctx& c, DevBT& t, const Metric* m, const double* P, double* J, double p
 }
 
 void dfk_mo(Ctx& c, DevBT& t, const Metric* m, const double* C, int ldc, int nocc, double* K,
-            int occ_nbatches)
+            int occ_nbatches, const double* tilde64)
 {
   const BLayout& L = t.L;
-  if (!t.transformed && !m) throw std::runtime_error("DFMO-K needs the metric for an untransformed B");
+  // tilde64: FP64 slabs of L^-1 B (same layout as B): K1 on them, no K2
+  const bool pre = t.transformed || tilde64 != nullptr;
+  const double* bdata = tilde64 ? tilde64 : t.data64.as<double>();
+  if (!pre && !m) throw std::runtime_error("DFMO-K needs the metric for an untransformed B");
   const int naux = L.naux, nbf = L.nbf;
   if (nocc <= 0) {
     hipLaunchKernelGGL(k_fill, dim3(gs_blocks((int64_t)nbf * nbf)), dim3(256), 0, c.stream, K,
@@ -347,7 +350,7 @@ void dfk_mo(Ctx& c, DevBT& t, const Metric* m, const double* C, int ldc, int noc
   const int nb = (nocc + per - 1) / per;
   const size_t ht = (size_t)naux * nbf * per;
   c.scratch_a.ensure(ht * sizeof(double));
-  if (!t.transformed) c.scratch_b.ensure(ht * sizeof(double));
+  if (!pre) c.scratch_b.ensure(ht * sizeof(double));
   DevBuf& partial = c.scratch_partial;
   for (int b = 0; b < nb; ++b) {
     const int i0 = b * per;
@@ -356,10 +359,10 @@ void dfk_mo(Ctx& c, DevBT& t, const Metric* m, const double* C, int ldc, int noc
     // K1: HT1[x][nu][i] = sum_mu B(x|mu nu) C(mu, i0+i)
     {
       PhaseTimer tm(c, PH_K1);
-      slab_gemm(c, t, C + i0, ldc, w, HT1, w, 1.0);
+      slab_gemm_t<double>(c, t, bdata, C + i0, ldc, w, HT1, w);
     }
     const double* HT = HT1;
-    if (!t.transformed) {
+    if (!pre) {
       PhaseTimer tm(c, PH_K2);
       // K2: HT2 = L^-1 HT1 over x; column-major (W x naux) view times (L^-1)^T
       double* HT2 = c.scratch_b.as<double>();
@@ -553,30 +556,31 @@ static void bf16_layout(Ctx& c, DevBT& t)
 // B, which stays in data64: J, the reported energy and every FP64 exchange
 // keep the reference's formula (explicit V^-1 in J, L^-1 applied per build in
 // DFMO), while the per-iteration exchange reads the FP32 and/or BF16 copies.
-// Per slab: tmp = B_slab (L^-1)^T in FP64 (the slab's rows of L^-1 B), then
-// its FP32 copy into out32 (same slab layout) if not null and its bf16 hi/lo
-// split into t's padded planes if bf16.
-static void tilde_slabs(Ctx& c, DevBT& t, const Metric& m, float* out32, bool bf16)
+// Per slab: B_slab (L^-1)^T in FP64 (the slab's rows of L^-1 B) into out64
+// (same slab layout) or a temporary, then its FP32 copy into out32 if not null
+// and its bf16 hi/lo split into t's padded planes if bf16.
+static void tilde_slabs(Ctx& c, DevBT& t, const Metric& m, float* out32, bool bf16, double* out64 = nullptr)
 {
   const BLayout& L = t.L;
   int64_t maxslab = 0;
   for (int n = 0; n < L.nbb; ++n) maxslab = std::max<int64_t>(maxslab, L.slab_off[n + 1] - L.slab_off[n]);
-  DevBuf tmp((size_t)std::max<int64_t>(maxslab, 1) * sizeof(double));
+  DevBuf tmp(out64 ? 0 : (size_t)std::max<int64_t>(maxslab, 1) * sizeof(double));
   const double one = 1.0;
   for (int n = 0; n < L.nbb; ++n) {
     const int64_t R = (int64_t)L.bsz[n] * L.ncol[n];
     if (R == 0) continue;
     const double* slab = t.data64.as<double>() + L.slab_off[n];
-    // column-major (R x naux) view: tmp = slab * (L^-1)^T
+    double* dst = out64 ? out64 + L.slab_off[n] : tmp.as<double>();
+    // column-major (R x naux) view: dst = slab * (L^-1)^T
     RB_CHECK(rocblas_dtrmm_64(c.blas, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
                               rocblas_diagonal_non_unit, R, L.naux, &one, m.Linv.as<double>(), L.naux, slab, R,
-                              tmp.as<double>(), R));
+                              dst, R));
     const int64_t ne = R * L.naux;
     if (out32)
-      hipLaunchKernelGGL(k_to_f32, dim3(gs_blocks(ne)), dim3(256), 0, c.stream, tmp.as<double>(),
+      hipLaunchKernelGGL(k_to_f32, dim3(gs_blocks(ne)), dim3(256), 0, c.stream, dst,
                          out32 + L.slab_off[n], ne);
     if (bf16)
-      hipLaunchKernelGGL(k_split_slab, dim3(gs_blocks(ne)), dim3(256), 0, c.stream, tmp.as<double>(), n,
+      hipLaunchKernelGGL(k_split_slab, dim3(gs_blocks(ne)), dim3(256), 0, c.stream, dst, n,
                          t.ncol.as<int>(), t.bsz.as<int>(), L.naux, t.ldp.as<int>(), t.off16.as<int64_t>(),
                          t.data_hi.as<uint16_t>(), t.data_lo.as<uint16_t>());
     HIP_CHECK(hipGetLastError());
@@ -584,13 +588,14 @@ static void tilde_slabs(Ctx& c, DevBT& t, const Metric& m, float* out32, bool bf
   HIP_CHECK(hipStreamSynchronize(c.stream));
 }
 
-void dbt_make_lowp(Ctx& c, DevBT& t, const Metric& m, bool f32, bool bf16)
+void dbt_make_lowp(Ctx& c, DevBT& t, const Metric& m, bool f32, bool bf16, bool f64)
 {
   if (t.transformed) throw std::runtime_error("B is transformed: use the copies of the transformed tensor");
   if (m.naux != t.L.naux) throw std::runtime_error("metric and B have different aux dimensions");
   if (f32) t.data32.alloc((size_t)std::max<int64_t>(t.L.total, 1) * sizeof(float));
   if (bf16) bf16_layout(c, t);
-  tilde_slabs(c, t, m, f32 ? t.data32.as<float>() : nullptr, bf16);
+  if (f64) t.data64t.alloc((size_t)std::max<int64_t>(t.L.total, 1) * sizeof(double));
+  tilde_slabs(c, t, m, f32 ? t.data32.as<float>() : nullptr, bf16, f64 ? t.data64t.as<double>() : nullptr);
   t.lowp_tilde = true;
 }
 

@@ -23,14 +23,17 @@ void seg_syrk(Ctx& c, const double* A, const double* B, int nseg, int64_t sstrid
 // J from density P (device, nbf x nbf); the total density is pscale * P.
 void dfj(Ctx& c, DevBT& t, const Metric* m, const double* P, double* J, double pscale);
 // K = -sum ... from occupied coefficients C (device, row-major nbf x ldc).
+// tilde64: FP64 slabs of L^-1 B (t.data64t) to contract instead of applying
+// L^-1 to B C per build (the same K; K1 and K3 only).
 void dfk_mo(Ctx& c, DevBT& t, const Metric* m, const double* C, int ldc, int nocc, double* K,
-            int occ_nbatches);
+            int occ_nbatches, const double* tilde64 = nullptr);
 
 // FP32 copy of the transformed B and the FP32-MFMA exchange (FP64 output).
 void dbt_make_f32(Ctx& c, DevBT& t);
 void dfk_mo_f32(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K, int occ_nbatches);
 void dbt_make_bf16(Ctx& c, DevBT& t);
-void dbt_make_lowp(Ctx& c, DevBT& t, const Metric& m, bool f32, bool bf16);
+// f64: also keep the FP64 L^-1 B slabs (t.data64t) for the FP64 exchange.
+void dbt_make_lowp(Ctx& c, DevBT& t, const Metric& m, bool f32, bool bf16, bool f64 = false);
 // FP32 copy of L^-1 B (made slab by slab in FP64 from the untransformed B)
 // into out (L.total floats, slab layout); t is left unchanged.
 void tilde_to_f32(Ctx& c, DevBT& t, const Metric& m, float* out);

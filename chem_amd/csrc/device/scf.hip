@@ -392,11 +392,14 @@ void select_k_builders(Ctx& c, DevBT& bt, const Metric* m, int kmode, int nocc, 
     return false;
   };
   kfinal = KBuildFn();
-  if (kmode >= 2)   // the final FP64 exchange of the low-precision modes
-    k64 = [cp, t, km, nocc, final_occ_nbatches](int, double, const double*, const double* Cocc, double* K) {
-      dfk_mo(*cp, *t, km, Cocc, nocc, nocc, K, final_occ_nbatches);
+  if (kmode >= 2) {   // the final FP64 exchange of the low-precision modes
+    // on the FP64 L^-1 B copy when it exists ((L^-1 B) C instead of L^-1 (B C))
+    const double* t64 = bt.data64t.p ? bt.data64t.as<double>() : nullptr;
+    k64 = [cp, t, km, nocc, final_occ_nbatches, t64](int, double, const double*, const double* Cocc, double* K) {
+      dfk_mo(*cp, *t, km, Cocc, nocc, nocc, K, final_occ_nbatches, t64);
       return false;
     };
+  }
   if (kmode == 0 || kmode == 1) {
     kb = k64;
   }

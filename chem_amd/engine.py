@@ -170,10 +170,18 @@ class DeviceB:
                                                _lib.dptr(K)))
         return K
 
-    def make_lowp(self, metric, f32=True, bf16=True):
+    def make_lowp(self, metric, f32=True, bf16=True, f64=False):
         """FP32 / BF16 split copies of L^-1 B made from the untransformed B, which
-        stays FP64 for J, the final exchange and the energy (reference formulas)."""
-        _lib.check(_lib.lib().mchip_dbt_make_lowp(self.h, metric.h, int(f32), int(bf16)))
+        stays FP64 for J and the energy (reference formulas).  f64 also keeps the
+        FP64 L^-1 B slabs, which the FP64 exchanges of the mixed modes then use."""
+        _lib.check(_lib.lib().mchip_dbt_make_lowp(self.h, metric.h, int(f32), int(bf16), int(f64)))
+
+    def dfk_mo_tilde64(self, C_occ, occ_nbatches=1):
+        """K from the FP64 L^-1 B copy (make_lowp(..., f64=True))."""
+        C = np.ascontiguousarray(C_occ, dtype=np.float64)
+        K = np.zeros((self.nbf, self.nbf))
+        _lib.check(_lib.lib().mchip_dfk_mo_tilde64(self.h, _lib.dptr(C), C.shape[1], occ_nbatches, _lib.dptr(K)))
+        return K
 
     def make_bf16(self):
         """BF16 hi/lo split copy of the transformed B (chosen-form exchange)."""

@@ -180,12 +180,16 @@ int mchip_dfk_mo_f32_dev(mchip_dbt t, const double* C_dev, int ldc, int nocc, in
 int mchip_dfk_mo_f32(mchip_dbt t, const double* C, int nocc, int occ_nbatches, double* K);
 
 /* FP32 and/or BF16 split copies of B~ = L^-1 B made slab by slab from the
- * untransformed B, which stays FP64 in place: J, the final FP64 exchange and
- * the reported energy keep the reference's formulas (DF_J with explicit V^-1,
- * jkdf.cpp:69-190; DFMO_K with L^-1 per build, jkdf.cpp:221-462), and only the
+ * untransformed B, which stays FP64 in place: J and the reported energy keep
+ * the reference's formulas (DF_J with explicit V^-1, jkdf.cpp:69-190), and the
  * per-iteration exchange of the chosen form (mchip_dfk_mo_f32 / _bf16, SCF
- * kmodes 2-4) reads the copies. */
-int mchip_dbt_make_lowp(mchip_dbt t, mchip_metric m, int f32, int bf16);
+ * kmodes 2-4) reads the copies.  f64 also keeps the FP64 slabs of L^-1 B: the
+ * FP64 exchanges of kmodes 2-4 (the final one, and the last escalation stage)
+ * then contract (L^-1 B) C instead of applying L^-1 to B C per build
+ * (jkdf.cpp:361-417 with the L^-1 product associated the other way). */
+int mchip_dbt_make_lowp(mchip_dbt t, mchip_metric m, int f32, int bf16, int f64);
+/* K from the FP64 L^-1 B copy (host C nbf x nocc row-major, K nbf x nbf). */
+int mchip_dfk_mo_tilde64(mchip_dbt t, const double* C, int nocc, int occ_nbatches, double* K);
 
 /* BF16 split-precision copy of the transformed B (hi = bf16(x), lo = bf16(x - hi),
  * rows padded to a multiple of 32), made once per geometry.  Chosen form of the

@@ -187,3 +187,26 @@ def test_mixed_auto_without_fp32_copy(ctx):
         ref.iterations, mix.iterations, mix.energy - ref.energy))
     assert ref.converged and mix.converged
     assert abs(mix.energy - ref.energy) < 1e-8
+
+
+def test_fp64_copy_exchange_and_scf(ctx):
+    """The FP64 L^-1 B copy: its exchange equals the plain DFMO exchange (L^-1
+    applied per build) to 1e-12 relative, and the escalation SCF whose FP64
+    exchanges use it reproduces the plain SCF energy to 1e-8 Eh."""
+    from chem_amd.engine import DeviceB, Metric
+    from chem_amd.scf import run_rhf
+    mol = desc.Molecule(desc.water_cluster(4, 11))
+    S, T, V = ints.int1e(mol)
+    Bh = ints.BTensorHost.build(mol, 1e-10)
+    m = Metric(ctx, ints.int2c2e(mol))
+    dB = DeviceB(ctx, Bh)
+    ref = run_rhf(mol, dB, m, S, T + V, threshold=1e-9, kmode="dfmo")
+    dB.make_lowp(m, f32=True, bf16=True, f64=True)
+    C = np.ascontiguousarray(ref.C[:, :mol.nocc])
+    K0 = dB.dfk_mo(C, m)
+    K1 = dB.dfk_mo_tilde64(C)
+    rel = np.abs(K1 - K0).max() / np.abs(K0).max()
+    mix = run_rhf(mol, dB, m, S, T + V, threshold=1e-9, kmode="mixed_auto", max_iter=60)
+    print("K rel %.2e; dE %.2e (%d iterations)" % (rel, mix.energy - ref.energy, mix.iterations))
+    assert rel < 1e-12
+    assert mix.converged and abs(mix.energy - ref.energy) < 1e-8
