@@ -263,10 +263,13 @@ def kernels_only(args):
         out[name + "_build_s"] = (time.perf_counter() - t) / steps
         log("%s build %.4f s" % (name, out[name + "_build_s"]))
     t = time.perf_counter()
-    dB.dfk_mo(C, met)
+    K64 = dB.dfk_mo(C, met)
     ctx.sync()
     out["fp64_build_s"] = time.perf_counter() - t
+    for name, fn in (("bf16x3", dB.dfk_mo_bf16), ("f32", dB.dfk_mo_f32)):
+        out[name + "_frob_rel_vs_fp64"] = float(np.linalg.norm(fn(C) - K64) / np.linalg.norm(K64))
     out["bf16_plan"] = dB.bf16_plan(mol.nocc)
+    out["f32_plan"] = dB.f32_plan(mol.nocc)
     log("fp64 build %.4f s" % out["fp64_build_s"])
     return out
 
@@ -467,10 +470,11 @@ def main():
         kernels["bf16_plan"] = dB.bf16_plan(mol.nocc)
     if k32:
         ph = phases(k32)
-        kernels["k_slab_gemm<float>"] = dict(s=ph[0], tflops=k1_flops / ph[0] / 1e12,
+        kernels["k_slab_gemm_f32_dl"] = dict(s=ph[0], tflops=k1_flops / ph[0] / 1e12,
                                              builds=nb[1], peak=MFMA_F32_PEAK / 1e12)
-        kernels["k_seg_syrk_f32"] = dict(s=ph[2], tflops=k3_flops / ph[2] / 1e12,
-                                         builds=nb[1], peak=MFMA_F32_PEAK / 1e12)
+        kernels["k_seg_syrk_f32_dl"] = dict(s=ph[2], tflops=k3_flops / ph[2] / 1e12,
+                                            builds=nb[1], peak=MFMA_F32_PEAK / 1e12)
+        kernels["f32_plan"] = dB.f32_plan(mol.nocc)
     dom = max((k for k in kernels if isinstance(kernels[k], dict) and "s" in kernels[k]),
               key=lambda k: kernels[k]["s"] * kernels[k]["builds"])
     d = kernels[dom]

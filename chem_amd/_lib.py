@@ -145,6 +145,9 @@ _SIGS = [
     ("mchip_dfk_mo_bf16", c_int, [c_void_p, P_double, c_int, c_int, P_double]),
     ("mchip_dfk_bf16_tile", c_int, [c_void_p, c_int, P_int]),
     ("mchip_dfk_bf16_plan", c_int, [c_void_p, c_int, P_int, P_float, P_int]),
+    ("mchip_dfk_f32_plan", c_int, [c_void_p, c_int, P_int, P_float, P_int]),
+    ("mchip_dfk_f32_force", c_int, [c_void_p, c_int, c_int, c_int]),
+    ("mchip_dfk_f32_shapes", c_int, [P_int, P_int]),
     ("mchip_dfk_bf16_force", c_int, [c_void_p, c_int, c_int, c_int]),
     ("mchip_dfk_bf16_shapes", c_int, [P_int, P_int, P_int]),
     ("mchip_dfk_ao_dev", c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),

@@ -383,89 +383,6 @@ void dfk_mo(Ctx& c, DevBT& t, const Metric* m, const double* C, int ldc, int noc
 
 namespace mchip {
 
-void dbt_make_f32(Ctx& c, DevBT& t)
-{
-  if (!t.transformed) throw std::runtime_error("the FP32 copy is made from the transformed B");
-  t.data32.alloc((size_t)t.L.total * sizeof(float));
-  if (t.L.total)
-    hipLaunchKernelGGL(k_to_f32, dim3(gs_blocks(t.L.total)), dim3(256), 0, c.stream,
-                       t.data64.as<double>(), t.data32.as<float>(), (int64_t)t.L.total);
-  HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(c.stream));
-  t.lowp_tilde = true;
-}
-
-// FP32 SYRK over aux segments with FP64-flushed accumulators (upper 128 x 128
-// tiles, table cached with the context per n), then the FP64 chunk reduction.
-static void seg_syrk_f32(Ctx& c, const float* A, int nseg, int64_t sstride, int lda, int klen, int n,
-                         double alpha, double* out, DevBuf& partial_buf, int nbatch, int batch, bool reduce)
-{
-  const int nt = (n + SG_BM - 1) / SG_BM;
-  const int ntile = nt * (nt + 1) / 2;
-  auto it = c.syrk_tiles.find(n);
-  if (it == c.syrk_tiles.end()) {
-    std::vector<int2> tl;
-    for (int i = 0; i < nt; ++i)
-      for (int j = i; j < nt; ++j) tl.push_back(int2{i, j});
-    c.syrk_tiles[n] = upload_vec(tl, c.stream);
-    HIP_CHECK(hipStreamSynchronize(c.stream));
-    it = c.syrk_tiles.find(n);
-  }
-  int nch = std::max(1, std::min(nseg, 2048 / std::max(1, ntile)));
-  const int per = (nseg + nch - 1) / nch;
-  nch = (nseg + per - 1) / per;
-  const size_t need = (size_t)nbatch * nch * n * n * sizeof(double);
-  if (batch == 0) partial_buf.ensure(need);
-  double* partial = partial_buf.as<double>() + (size_t)batch * nch * n * n;
-  hipLaunchKernelGGL(k_seg_syrk_f32, dim3(ntile * nch), dim3(512), 0, c.stream, A, nseg, sstride, lda, klen, n,
-                     it->second.as<int2>(), ntile, per, partial);
-  HIP_CHECK(hipGetLastError());
-  if (reduce) {
-    hipLaunchKernelGGL(k_reduce_sym, dim3(nblk((int64_t)n * n, 256)), dim3(256), 0, c.stream,
-                       partial_buf.as<double>(), nbatch * nch, n, 0, alpha, out);
-    HIP_CHECK(hipGetLastError());
-  }
-}
-
-void dfk_mo_f32(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K, int occ_nbatches)
-{
-  const BLayout& L = t.L;
-  if (!t.lowp_tilde || (!t.data32.p && L.total))
-    throw std::runtime_error("FP32 copy of L^-1 B missing (mchip_dbt_make_lowp or mchip_dbt_make_f32)");
-  const int naux = L.naux, nbf = L.nbf;
-  if (nocc <= 0) {
-    hipLaunchKernelGGL(k_fill, dim3(gs_blocks((int64_t)nbf * nbf)), dim3(256), 0, c.stream, K,
-                       (int64_t)nbf * nbf, 0.0);
-    return;
-  }
-  occ_nbatches = std::max(1, std::min(occ_nbatches, nocc));
-  const int per = (nocc + occ_nbatches - 1) / occ_nbatches;
-  const int nb = (nocc + per - 1) / per;
-  const size_t ht = (size_t)naux * nbf * per;
-  c.scratch_a.ensure(ht * sizeof(float) + (size_t)nbf * per * sizeof(float) + 256);
-  float* W = c.scratch_a.as<float>();
-  float* G = W + ht;
-  for (int b = 0; b < nb; ++b) {
-    const int i0 = b * per;
-    const int w = std::min(per, nocc - i0);
-    {
-      PhaseTimer tm(c, PH_SPLIT);
-      hipLaunchKernelGGL(k_rows_to_f32, dim3(nblk((int64_t)nbf * w, 256)), dim3(256), 0, c.stream,
-                         C + i0, ldc, nbf, w, G);
-    }
-    {
-      PhaseTimer tm(c, PH_K1);
-      slab_gemm_t<float>(c, t, t.data32.as<float>(), G, w, w, W, w);
-    }
-    PhaseTimer tm(c, PH_K3);
-    seg_syrk_f32(c, W, naux, (int64_t)nbf * w, w, w, nbf, -1.0, K, c.scratch_partial, nb, b, b == nb - 1);
-  }
-}
-
-}  // namespace mchip
-
-namespace mchip {
-
 void slab_out_gemm(Ctx& c, DevBT& t, const double* A, int lda, int klen, const double* G, int ldg,
                    double* out)
 {
@@ -532,9 +449,13 @@ namespace mchip {
 // BF16 split-precision exchange on the transformed B~
 // ---------------------------------------------------------------------------
 
-static void bf16_layout(Ctx& c, DevBT& t)
+// Padded slab layout shared by the BF16 and FP32 copies of B~ (computed once
+// per tensor): rows of slab N padded to ldp[N], a multiple of 32 elements, at
+// element offset off16[N]; gathered coefficients of slab N at ldp_pre[N] * ng.
+static void pad_layout(Ctx& c, DevBT& t)
 {
   const BLayout& L = t.L;
+  if (t.h_ldp_pre.size() == (size_t)L.nbb + 1) return;
   t.h_ldp.assign(L.nbb, 0);
   t.h_off16.assign(L.nbb + 1, 0);
   t.h_ldp_pre.assign(L.nbb + 1, 0);
@@ -546,7 +467,14 @@ static void bf16_layout(Ctx& c, DevBT& t)
   t.ldp = upload_vec(t.h_ldp, c.stream);
   t.off16 = upload_vec(t.h_off16, c.stream);
   t.ldp_pre = upload_vec(t.h_ldp_pre, c.stream);
-  const int64_t tot = t.h_off16[L.nbb];
+}
+
+static int64_t padded_elems(const DevBT& t) { return t.h_off16.empty() ? 0 : t.h_off16.back(); }
+
+static void bf16_layout(Ctx& c, DevBT& t)
+{
+  pad_layout(c, t);
+  const int64_t tot = padded_elems(t);
   t.data_hi.alloc((size_t)std::max<int64_t>(tot, 1) * sizeof(uint16_t));
   t.data_lo.alloc((size_t)std::max<int64_t>(tot, 1) * sizeof(uint16_t));
   t.bf16_plan.clear();
@@ -558,8 +486,10 @@ static void bf16_layout(Ctx& c, DevBT& t)
 // DFMO), while the per-iteration exchange reads the FP32 and/or BF16 copies.
 // Per slab: B_slab (L^-1)^T in FP64 (the slab's rows of L^-1 B) into out64
 // (same slab layout) or a temporary, then its FP32 copy into out32 if not null
-// and its bf16 hi/lo split into t's padded planes if bf16.
-static void tilde_slabs(Ctx& c, DevBT& t, const Metric& m, float* out32, bool bf16, double* out64 = nullptr)
+// (in the padded layout if pad32, else in the slab layout of data64) and its
+// bf16 hi/lo split into t's padded planes if bf16.
+static void tilde_slabs(Ctx& c, DevBT& t, const Metric& m, float* out32, bool pad32, bool bf16,
+                        double* out64 = nullptr)
 {
   const BLayout& L = t.L;
   int64_t maxslab = 0;
@@ -576,7 +506,11 @@ static void tilde_slabs(Ctx& c, DevBT& t, const Metric& m, float* out32, bool bf
                               rocblas_diagonal_non_unit, R, L.naux, &one, m.Linv.as<double>(), L.naux, slab, R,
                               dst, R));
     const int64_t ne = R * L.naux;
-    if (out32)
+    if (out32 && pad32)
+      hipLaunchKernelGGL(k_pad_f32_slab, dim3(gs_blocks((int64_t)L.naux * L.bsz[n] * t.h_ldp[n])), dim3(256), 0,
+                         c.stream, dst, n, t.ncol.as<int>(), t.bsz.as<int>(), L.naux, t.ldp.as<int>(),
+                         t.off16.as<int64_t>(), out32);
+    else if (out32)
       hipLaunchKernelGGL(k_to_f32, dim3(gs_blocks(ne)), dim3(256), 0, c.stream, dst,
                          out32 + L.slab_off[n], ne);
     if (bf16)
@@ -592,10 +526,12 @@ void dbt_make_lowp(Ctx& c, DevBT& t, const Metric& m, bool f32, bool bf16, bool 
 {
   if (t.transformed) throw std::runtime_error("B is transformed: use the copies of the transformed tensor");
   if (m.naux != t.L.naux) throw std::runtime_error("metric and B have different aux dimensions");
-  if (f32) t.data32.alloc((size_t)std::max<int64_t>(t.L.total, 1) * sizeof(float));
+  if (f32 || bf16) pad_layout(c, t);
+  if (f32) t.data32.alloc((size_t)std::max<int64_t>(padded_elems(t), 1) * sizeof(float));
   if (bf16) bf16_layout(c, t);
   if (f64) t.data64t.alloc((size_t)std::max<int64_t>(t.L.total, 1) * sizeof(double));
-  tilde_slabs(c, t, m, f32 ? t.data32.as<float>() : nullptr, bf16, f64 ? t.data64t.as<double>() : nullptr);
+  tilde_slabs(c, t, m, f32 ? t.data32.as<float>() : nullptr, true, bf16, f64 ? t.data64t.as<double>() : nullptr);
+  if (f32) t.f32_plan.clear();
   t.lowp_tilde = true;
 }
 
@@ -603,7 +539,7 @@ void tilde_to_f32(Ctx& c, DevBT& t, const Metric& m, float* out)
 {
   if (t.transformed) throw std::runtime_error("B is transformed: the FP32 copy is made from the untransformed B");
   if (m.naux != t.L.naux) throw std::runtime_error("metric and B have different aux dimensions");
-  tilde_slabs(c, t, m, out, false);
+  tilde_slabs(c, t, m, out, false, false);
 }
 
 void dbt_make_bf16(Ctx& c, DevBT& t)
@@ -611,7 +547,7 @@ void dbt_make_bf16(Ctx& c, DevBT& t)
   if (!t.transformed) throw std::runtime_error("the BF16 copy is made from the transformed B");
   const BLayout& L = t.L;
   bf16_layout(c, t);
-  if (t.h_off16[L.nbb] > 0 && L.nbb > 0)
+  if (L.nbb > 0 && t.h_off16[L.nbb] > 0)
     hipLaunchKernelGGL(k_split_slabs, dim3(2048, L.nbb), dim3(256), 0, c.stream, t.data64.as<double>(),
                        t.slab_off.as<int64_t>(), t.ncol.as<int>(), t.bsz.as<int>(), L.naux,
                        t.ldp.as<int>(), t.off16.as<int64_t>(), t.data_hi.as<uint16_t>(),
@@ -934,6 +870,223 @@ void dfk_mo_bf16(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K
     }
     PhaseTimer tm(c, PH_K3);
     seg_syrk_bf16(c, t, plan.k3, W, Whi, Wlo, naux, nbf, ldw, d_tl, ntile, nch, spc, partial);
+    if (b == nb - 1)
+      hipLaunchKernelGGL(k_reduce_sym, dim3(nblk((int64_t)nbf * nbf, 256)), dim3(256), 0, c.stream,
+                         c.scratch_partial.as<double>(), nb * nch, nbf, 0, -1.0, K);
+    HIP_CHECK(hipGetLastError());
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 exchange on the padded FP32 copy of B~
+// ---------------------------------------------------------------------------
+
+void dbt_make_f32(Ctx& c, DevBT& t)
+{
+  if (!t.transformed) throw std::runtime_error("the FP32 copy is made from the transformed B");
+  const BLayout& L = t.L;
+  pad_layout(c, t);
+  t.data32.alloc((size_t)std::max<int64_t>(padded_elems(t), 1) * sizeof(float));
+  for (int n = 0; n < L.nbb; ++n) {
+    const int64_t ne = (int64_t)L.naux * L.bsz[n] * t.h_ldp[n];
+    if (ne == 0) continue;
+    hipLaunchKernelGGL(k_pad_f32_slab, dim3(gs_blocks(ne)), dim3(256), 0, c.stream,
+                       t.data64.as<double>() + L.slab_off[n], n, t.ncol.as<int>(), t.bsz.as<int>(), L.naux,
+                       t.ldp.as<int>(), t.off16.as<int64_t>(), t.data32.as<float>());
+    HIP_CHECK(hipGetLastError());
+  }
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  t.f32_plan.clear();
+  t.lowp_tilde = true;
+}
+
+// Launch shapes of the FP32 slab GEMM (direct-to-LDS): waves WM x WN, MFMA
+// tiles per wave TM x TN, S pipeline stages.
+struct F32Shape {
+  int WM, WN, TM, TN, S;
+  int bm() const { return 16 * WM * TM; }
+  int bn() const { return 16 * WN * TN; }
+  int threads() const { return 64 * WM * WN; }
+};
+static const F32Shape kF32Shapes[] = {{2, 2, 4, 2, 3}, {2, 4, 4, 2, 3}, {2, 2, 4, 5, 3},
+                                      {2, 4, 4, 5, 3}, {2, 4, 6, 5, 3}, {2, 4, 4, 5, 2}};
+static constexpr int kNumF32Shapes = sizeof(kF32Shapes) / sizeof(kF32Shapes[0]);
+static constexpr int kNumF32Syrk = 3;   // direct-to-LDS SYRK with 2, 3, 4 stages
+
+static void slab_gemm_f32dl(Ctx& c, DevBT& t, int shape, const float* G, int ng, float* W, int ldw)
+{
+  const F32Shape sh = kF32Shapes[shape];
+  int nt = 0;
+  const Tile* tiles = bf16_tiles(c, t, sh.bm(), sh.bn(), ng, &nt);
+  if (nt == 0) return;
+#define MCHIP_F32_SLAB(WM, WN, TM, TN, S)                                                                     \
+  hipLaunchKernelGGL((k_slab_gemm_f32_dl<WM, WN, TM, TN, S>), dim3(nt), dim3(64 * WM * WN), 0, c.stream,      \
+                     t.data32.as<float>(), t.off16.as<int64_t>(), t.ldp.as<int>(), t.ldp_pre.as<int64_t>(),   \
+                     t.bsz.as<int>(), t.boff.as<int>(), tiles, t.L.naux, t.L.nbf, G, ng, W, ldw)
+  switch (shape) {
+    case 0: MCHIP_F32_SLAB(2, 2, 4, 2, 3); break;
+    case 1: MCHIP_F32_SLAB(2, 4, 4, 2, 3); break;
+    case 2: MCHIP_F32_SLAB(2, 2, 4, 5, 3); break;
+    case 3: MCHIP_F32_SLAB(2, 4, 4, 5, 3); break;
+    case 4: MCHIP_F32_SLAB(2, 4, 6, 5, 3); break;
+    case 5: MCHIP_F32_SLAB(2, 4, 4, 5, 2); break;
+    default: throw std::runtime_error("FP32 slab GEMM: unknown launch shape");
+  }
+#undef MCHIP_F32_SLAB
+  HIP_CHECK(hipGetLastError());
+}
+
+static void seg_syrk_f32dl(Ctx& c, int variant, const float* W, int naux, int nbf, int ldw, const int2* tl,
+                           int ntile, int nch, int spc, double* partial)
+{
+  const dim3 grid8((unsigned)((ntile * nch + 7) / 8 * 8));
+  const int64_t ss = (int64_t)nbf * ldw;
+  switch (variant) {
+    case 0:
+      hipLaunchKernelGGL(k_seg_syrk_f32_dl<2>, grid8, dim3(512), 0, c.stream, W, naux, ss, ldw, nbf, tl, ntile, spc,
+                         partial);
+      break;
+    case 1:
+      hipLaunchKernelGGL(k_seg_syrk_f32_dl<3>, grid8, dim3(512), 0, c.stream, W, naux, ss, ldw, nbf, tl, ntile, spc,
+                         partial);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_seg_syrk_f32_dl<4>, grid8, dim3(512), 0, c.stream, W, naux, ss, ldw, nbf, tl, ntile, spc,
+                         partial);
+      break;
+    default: throw std::runtime_error("FP32 SYRK: unknown variant");
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+// Autotuner of the FP32 exchange for ng occupied columns: every slab-GEMM shape
+// whose column tile is not mostly empty and every SYRK variant is timed once
+// on the live operands (after one warm launch); the fastest of each is kept
+// per ng, with the timings.
+static const Bf16Plan& tune_f32(Ctx& c, DevBT& t, const float* G, int ng, float* W, int ldw, const int2* tl,
+                                int ntile, int nch, int spc, double* partial)
+{
+  auto it = t.f32_plan.find(ng);
+  if (it != t.f32_plan.end()) return it->second;
+  Bf16Plan p;
+  p.k1_ms.assign(kNumF32Shapes, -1.f);
+  p.k3_ms.assign(kNumF32Syrk, -1.f);
+  float best = 1e30f;
+  for (int v = 0; v < kNumF32Shapes; ++v) {
+    const int bn = kF32Shapes[v].bn();
+    if (bn > 64 && (ng + bn - 1) / bn * bn - ng >= bn / 2) continue;   // mostly empty tiles
+    const float ms = time_launch(c, [&] { slab_gemm_f32dl(c, t, v, G, ng, W, ldw); });
+    p.k1_ms[v] = ms;
+    if (ms < best) {
+      best = ms;
+      p.k1 = v;
+    }
+  }
+  best = 1e30f;
+  for (int v = 0; v < kNumF32Syrk; ++v) {
+    const float ms = time_launch(c, [&] { seg_syrk_f32dl(c, v, W, t.L.naux, t.L.nbf, ldw, tl, ntile, nch, spc, partial); });
+    p.k3_ms[v] = ms;
+    if (ms < best) {
+      best = ms;
+      p.k3 = v;
+    }
+  }
+  if (p.k1 < 0 || p.k3 < 0) throw std::runtime_error("FP32 autotuner found no usable launch shape");
+  return t.f32_plan[ng] = p;
+}
+
+void dfk_f32_plan(const DevBT& t, int ng, int* plan, float* ms, int* nms)
+{
+  auto it = t.f32_plan.find(ng);
+  for (int i = 0; i < 8; ++i) plan[i] = 0;
+  if (nms) *nms = 0;
+  if (it == t.f32_plan.end()) return;
+  const Bf16Plan& p = it->second;
+  const F32Shape& sh = kF32Shapes[p.k1];
+  plan[0] = p.k1;
+  plan[1] = sh.bm();
+  plan[2] = sh.bn();
+  plan[3] = sh.threads();
+  plan[4] = 1;
+  plan[5] = sh.S;
+  plan[6] = p.k3;
+  plan[7] = 1;
+  if (ms && nms) {
+    int k = 0;
+    for (float v : p.k1_ms) ms[k++] = v;
+    for (float v : p.k3_ms) ms[k++] = v;
+    *nms = k;
+  }
+}
+
+void dfk_f32_force(DevBT& t, int ng, int k1, int k3)
+{
+  if (k1 < 0 || k1 >= kNumF32Shapes || k3 < 0 || k3 >= kNumF32Syrk)
+    throw std::runtime_error("FP32 plan: shape or SYRK variant out of range");
+  Bf16Plan p;
+  p.k1 = k1;
+  p.k3 = k3;
+  p.k1_ms.assign(kNumF32Shapes, -1.f);
+  p.k3_ms.assign(kNumF32Syrk, -1.f);
+  t.f32_plan[ng] = p;
+}
+
+int dfk_f32_num_shapes(int* nsyrk)
+{
+  if (nsyrk) *nsyrk = kNumF32Syrk;
+  return kNumF32Shapes;
+}
+
+// K = -sum_X W_X^T W_X with W = B~ C (DFMO_K::compute_K, jkdf.cpp:221-462, on L^-1 B): both
+// contractions in FP32 on MFMA; the SYRK's FP32 sums are flushed into FP64
+// every 256 products and its aux chunks are summed in FP64.
+void dfk_mo_f32(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K, int occ_nbatches)
+{
+  const BLayout& L = t.L;
+  if (!t.lowp_tilde || (!t.data32.p && L.total) || t.h_ldp_pre.size() != (size_t)L.nbb + 1)
+    throw std::runtime_error("FP32 copy of L^-1 B missing (mchip_dbt_make_lowp or mchip_dbt_make_f32)");
+  const int naux = L.naux, nbf = L.nbf;
+  if (nocc <= 0) {
+    hipLaunchKernelGGL(k_fill, dim3(gs_blocks((int64_t)nbf * nbf)), dim3(256), 0, c.stream, K,
+                       (int64_t)nbf * nbf, 0.0);
+    return;
+  }
+  occ_nbatches = std::max(1, std::min(occ_nbatches, nocc));
+  const int per = (nocc + occ_nbatches - 1) / occ_nbatches;
+  const int nb = (nocc + per - 1) / per;
+  const int ldw = (per + 15) / 16 * 16;   // W rows of whole 64-byte steps, zero past the batch width
+  const size_t nW = (size_t)naux * nbf * ldw;
+  const size_t nG = (size_t)t.h_ldp_pre[L.nbb] * per;
+  c.scratch_a.ensure((nW + nG) * sizeof(float) + 256);
+  float* W = c.scratch_a.as<float>();
+  float* G = W + nW;   // 64-byte aligned: nW is a multiple of 16
+  int ntile = 0;
+  const int2* tl = syrk_tiles(c, t, nbf, 128, &ntile);
+  int nch = std::max(1, std::min(naux, 2048 / std::max(1, ntile)));
+  const int spc = (naux + nch - 1) / nch;
+  nch = (naux + spc - 1) / spc;
+  c.scratch_partial.ensure((size_t)nb * nch * nbf * nbf * sizeof(double));
+  int gmax = 0;
+  for (int n = 0; n < L.nbb; ++n) gmax = std::max(gmax, t.h_ldp[n]);
+  for (int b = 0; b < nb; ++b) {
+    const int i0 = b * per;
+    const int w = std::min(per, nocc - i0);
+    {
+      PhaseTimer tm(c, PH_SPLIT);
+      if (gmax > 0 && L.nbb > 0)
+        hipLaunchKernelGGL(k_gather_ct_f32, dim3(nblk((int64_t)w * gmax, 256), L.nbb), dim3(256), 0, c.stream,
+                           C + i0, ldc, w, t.ncol.as<int>(), t.ldp.as<int>(), t.ldp_pre.as<int64_t>(),
+                           t.colmap.as<int>(), t.colmap_off.as<int>(), G);
+      HIP_CHECK(hipGetLastError());
+    }
+    double* partial = c.scratch_partial.as<double>() + (size_t)b * nch * nbf * nbf;
+    const Bf16Plan& plan = tune_f32(c, t, G, w, W, ldw, tl, ntile, nch, spc, partial);
+    {
+      PhaseTimer tm(c, PH_K1);
+      slab_gemm_f32dl(c, t, plan.k1, G, w, W, ldw);
+    }
+    PhaseTimer tm(c, PH_K3);
+    seg_syrk_f32dl(c, plan.k3, W, naux, nbf, ldw, tl, ntile, nch, spc, partial);
     if (b == nb - 1)
       hipLaunchKernelGGL(k_reduce_sym, dim3(nblk((int64_t)nbf * nbf, 256)), dim3(256), 0, c.stream,
                          c.scratch_partial.as<double>(), nb * nch, nbf, 0, -1.0, K);

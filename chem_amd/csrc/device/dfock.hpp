@@ -40,6 +40,10 @@ void tilde_to_f32(Ctx& c, DevBT& t, const Metric& m, float* out);
 void dfk_mo_bf16(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K, int occ_nbatches);
 int dfk_bf16_tuned_bn(const DevBT& t, int ng);
 void dfk_bf16_plan(const DevBT& t, int ng, int* plan, float* ms, int* nms);
+// Tuned FP32 exchange pipeline for ng (same fields as dfk_bf16_plan; plan[4] = 1, plan[5] = stages).
+void dfk_f32_plan(const DevBT& t, int ng, int* plan, float* ms, int* nms);
+void dfk_f32_force(DevBT& t, int ng, int k1, int k3);
+int dfk_f32_num_shapes(int* nsyrk);
 void dfk_bf16_force(DevBT& t, int ng, int k1, int k3);
 int dfk_bf16_num_shapes(int* nsyrk);
 int dfk_bf16_shape_dl(int k1);

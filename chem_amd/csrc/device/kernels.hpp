@@ -58,7 +58,6 @@ __global__ void k_axpby(double* y, const double* x, int64_t n, double a, double 
 
 __global__ void k_to_f32(const double* in, float* out, int64_t n);
 __global__ void k_to_f32x2(const double* in, float* hi, float* lo, int64_t n);
-__global__ void k_rows_to_f32(const double* in, int ld64, int rows, int ncols, float* out);
 
 // BF16 split-precision exchange (kernels_bf16.hip)
 __global__ void k_split_slabs(const double* data64, const int64_t* slab_off, const int* ncol,
@@ -88,8 +87,19 @@ __global__ void k_slab_gemm_bf16x3(const uint16_t* bhi, const uint16_t* blo, con
 template <int BM>
 __global__ void k_seg_syrk_bf16x3(const float* W, int nseg, int64_t sstride, int ldw, int n,
                                   const int2* tiles, int ntile, int segs_per_chunk, double* partial);
-__global__ void k_seg_syrk_f32(const float* W, int nseg, int64_t sstride, int lda, int klen, int n,
-                               const int2* tiles, int ntile, int segs_per_chunk, double* partial);
+
+// FP32 exchange on the padded copy of B~ (kernels_f32dl.hip)
+__global__ void k_pad_f32_slab(const double* src, int N, const int* ncol, const int* bsz, int naux,
+                               const int* ldp, const int64_t* off, float* out);
+__global__ void k_gather_ct_f32(const double* C, int ldc, int ng, const int* ncol, const int* ldp,
+                                const int64_t* ldp_pre, const int* colmap, const int* colmap_off, float* g);
+template <int WM, int WN, int TM, int TN, int S>
+__global__ void k_slab_gemm_f32_dl(const float* data, const int64_t* off, const int* ldp, const int64_t* ldp_pre,
+                                   const int* bsz, const int* boff, const Tile* tiles, int naux, int nbf,
+                                   const float* g, int ng, float* out, int ldo);
+template <int S>
+__global__ void k_seg_syrk_f32_dl(const float* W, int nseg, int64_t sstride, int ldw, int n, const int2* tiles,
+                                  int ntile, int segs_per_chunk, double* partial);
 
 // Blocks of a one-thread-per-element launch.  A dispatch holds at most 2^32 - 1
 // work-items per grid dimension; larger element counts must use a grid-stride

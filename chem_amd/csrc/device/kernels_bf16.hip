@@ -19,6 +19,7 @@
 //   A[l%16][8*(l/16)+q], B[8*(l/16)+q][l%16], D[4*(l/16)+e][l%16]
 // (pinned by tests/test_gpu_mfma_layout.py).  LDS rows hold 32 bf16 plus 8 of
 // padding (80 bytes): 16 lanes reading 16 consecutive rows hit distinct banks.
+#include "dl_common.hpp"
 #include "kernels.hpp"
 
 namespace mchip {
@@ -55,14 +56,6 @@ __device__ __forceinline__ void split32(float x, uint16_t& hi, uint16_t& lo)
 {
   hi = bf16_bits(x);
   lo = bf16_bits(x - bf16_value(hi));
-}
-
-// Workgroups are dispatched round-robin over the 8 XCDs; consecutive tile
-// indices are mapped onto one XCD so that tiles sharing operand rows share an
-// L2.  grid must be a multiple of 8.
-__device__ __forceinline__ int xcd_tile(int bid, int grid)
-{
-  return (bid & 7) * (grid >> 3) + (bid >> 3);
 }
 
 __global__ void k_split_slabs(const double* __restrict__ data64, const int64_t* __restrict__ slab_off,
@@ -384,37 +377,6 @@ template __global__ void k_seg_syrk_bf16x3<128>(const float*, int, int64_t, int,
 // Every wave issues the same number of loads per stage, so a stage is
 // complete once vmcnt drops to (stages still in flight) x (loads per wave).
 // ---------------------------------------------------------------------------
-
-typedef __attribute__((address_space(3))) void* lds_ptr_t;
-
-__device__ __forceinline__ void lds_load16(const uint4* g, uint4* l)
-{
-  __builtin_amdgcn_global_load_lds((const void*)g, (lds_ptr_t)l, 16, 0, 0);
-}
-
-// s_waitcnt vmcnt(n) with the other counters left at their maximum (gfx9 encoding)
-template <int N>
-__device__ __forceinline__ void wait_vm()
-{
-  static_assert(N >= 0 && N < 64, "vmcnt range");
-  __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
-}
-
-// Wait until at most `after` stages' loads (per wave: LPW each) are in flight.
-template <int LPW, int S>
-__device__ __forceinline__ void wait_stages(int after)
-{
-  if constexpr (S >= 4) {
-    if (after >= 3) { wait_vm<3 * LPW>(); return; }
-  }
-  if constexpr (S >= 3) {
-    if (after >= 2) { wait_vm<2 * LPW>(); return; }
-  }
-  if (after >= 1) { wait_vm<LPW>(); return; }
-  wait_vm<0>();
-}
-
-__device__ __forceinline__ int swz(int r, int c) { return c ^ ((r >> 2) & 3); }
 
 // One BK = 32 step on swizzled stage tiles (uint4 units, 4 per row).
 template <int TM, int TN>

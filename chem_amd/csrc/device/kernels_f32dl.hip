@@ -1,0 +1,264 @@
+// FP32 exchange on the transformed B~ = L^-1 B, direct-to-LDS pipelined (gfx950):
+//   k_pad_f32_slab      one FP64 slab of B~ -> FP32 rows padded to ldp (zeros past ncol)
+//   k_gather_ct_f32     per-slab gathered, transposed occupied coefficients in FP32
+//   k_slab_gemm_f32_dl  W(X, mu, i) = sum_nu B~(X, mu nu) C(nu, i) per slab (K1)
+//   k_seg_syrk_f32_dl   partial[chunk] = sum_{X in chunk} W_X W_X^T, upper tiles (K3)
+//
+// Operands use the padded slab layout of the BF16 copy: slab N has naux*bsz[N]
+// rows of ldp[N] (multiple of 32) elements at element offset off[N]; the
+// gathered coefficients of slab N are ng rows of ldp[N] at ldp_pre[N]*ng.
+// A contraction step covers 16 FP32 = 64 bytes of a row, the same LDS row
+// geometry (4 chunks of 16 bytes, XOR-swizzled) as one BF16 step of 32
+// elements.  Lane (fr, fk) of a 16x16 MFMA fragment reads chunk fk of row fr,
+// i.e. k = 4 fk + q for q = 0..3, and feeds element q to the q-th
+// v_mfma_f32_16x16x4_f32: the four MFMAs together sum over all 16 k.
+#include "dl_common.hpp"
+#include "gemm_mfma.hpp"
+#include "kernels.hpp"
+
+namespace mchip {
+
+typedef Mfma<float>::v4 f32v4;
+
+__global__ void k_pad_f32_slab(const double* __restrict__ src, int N, const int* __restrict__ ncol,
+                               const int* __restrict__ bsz, int naux, const int* __restrict__ ldp,
+                               const int64_t* __restrict__ off, float* __restrict__ out)
+{
+  const int K = ncol[N], ld = ldp[N];
+  const int64_t n = (int64_t)naux * bsz[N] * ld;
+  float* o = out + off[N];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / ld;
+    const int c = (int)(i - r * ld);
+    o[i] = c < K ? (float)src[r * K + c] : 0.f;
+  }
+}
+
+__global__ void k_gather_ct_f32(const double* __restrict__ C, int ldc, int ng, const int* __restrict__ ncol,
+                                const int* __restrict__ ldp, const int64_t* __restrict__ ldp_pre,
+                                const int* __restrict__ colmap, const int* __restrict__ colmap_off,
+                                float* __restrict__ g)
+{
+  const int N = blockIdx.y;
+  const int ld = ldp[N];
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)ng * ld) return;
+  const int i = (int)(idx / ld), c = (int)(idx - (int64_t)(idx / ld) * ld);
+  g[ldp_pre[N] * ng + idx] = c < ncol[N] ? (float)C[(int64_t)colmap[colmap_off[N] + c] * ldc + i] : 0.f;
+}
+
+// One 16-element step on swizzled stage tiles (uint4 units, 4 per row).
+template <int TM, int TN>
+__device__ __forceinline__ void f32_step_swz(const uint4* A, const uint4* B, int wm0, int wn0, int lane,
+                                             f32v4 (&acc)[TM][TN])
+{
+  const int fr = lane & 15, fk = lane >> 4;
+  float4 a[TM], b[TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+    const int r = wm0 + 16 * i + fr;
+    const uint4 v = A[r * 4 + swz(r, fk)];
+    a[i] = make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
+  }
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int r = wn0 + 16 * j + fr;
+    const uint4 v = B[r * 4 + swz(r, fk)];
+    b[j] = make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
+  }
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      acc[i][j] = Mfma<float>::op(a[i].x, b[j].x, acc[i][j]);
+      acc[i][j] = Mfma<float>::op(a[i].y, b[j].y, acc[i][j]);
+      acc[i][j] = Mfma<float>::op(a[i].z, b[j].z, acc[i][j]);
+      acc[i][j] = Mfma<float>::op(a[i].w, b[j].w, acc[i][j]);
+    }
+}
+
+// Loads of one stage: wave-loads j = wave + NW*u over [A rows/16][B rows/16];
+// a wave-load brings 16 rows x 64 bytes.  When NW does not divide the count,
+// the first waves issue one load more than the others.
+template <int BM, int BN, int NW>
+__device__ __forceinline__ void issue_stage_f32(uint4* st, int wave, int lane, const uint4* A, const uint4* B,
+                                                int64_t a0, int64_t alast, int b0, int blast, int lda4, int ldb4,
+                                                int kc0)
+{
+  constexpr int QA = BM / 16, QB = BN / 16, NJ = QA + QB;
+  constexpr int UMAX = (NJ + NW - 1) / NW;
+  const int rr = lane >> 2, pos = lane & 3;
+#pragma unroll
+  for (int u = 0; u < UMAX; ++u) {
+    const int j = wave + NW * u;
+    if (j >= NJ) break;
+    if (j < QA) {
+      const int r = 16 * j + rr;
+      const int64_t gr = a0 + r < alast ? a0 + r : alast;
+      lds_load16(A + gr * lda4 + kc0 + swz(r, pos), st + j * 64);
+    }
+    else {
+      const int q = j - QA;
+      const int r = 16 * q + rr;
+      const int64_t gr = b0 + r < blast ? b0 + r : blast;
+      lds_load16(B + gr * ldb4 + kc0 + swz(r, pos), st + BM * 4 + q * 64);
+    }
+  }
+}
+
+// Grouped slab GEMM in FP32: W rows (X, boff[N] + nu) for the rows X*bsz + nu
+// of slab N, columns [0, ldo), zero in [ng, ldo).  Tile table as for the BF16
+// kernel (m0 steps of BM, padded to a multiple of 8 with slab -1).  FP32
+// accumulation over the slab's columns.
+template <int WM, int WN, int TM, int TN, int S>
+__global__ __launch_bounds__(64 * WM * WN) void k_slab_gemm_f32_dl(
+    const float* __restrict__ data, const int64_t* __restrict__ off, const int* __restrict__ ldp,
+    const int64_t* __restrict__ ldp_pre, const int* __restrict__ bsz, const int* __restrict__ boff,
+    const Tile* __restrict__ tiles, int naux, int nbf, const float* __restrict__ g, int ng,
+    float* __restrict__ out, int ldo)
+{
+  constexpr int NW = WM * WN;
+  constexpr int BM = 16 * WM * TM, BN = 16 * WN * TN;
+  // per-wave loads per stage, rounded down: a wave with one load more waits
+  // for one stage-load more than it needs, never less
+  constexpr int LPW = (BM / 16 + BN / 16) / NW;
+  constexpr int STAGE = (BM + BN) * 4;   // uint4 per stage
+  __shared__ uint4 lds[S * STAGE];
+  const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  if (t.slab < 0) return;
+  const int N = t.slab;
+  const int bn = bsz[N];
+  const int ld4 = ldp[N] >> 2;
+  const int64_t rows = (int64_t)naux * bn;
+  const uint4* A = reinterpret_cast<const uint4*>(data + off[N]);
+  const uint4* G = reinterpret_cast<const uint4*>(g + ldp_pre[N] * ng);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave / WN) * (16 * TM), wn0 = (wave % WN) * (16 * TN);
+  f32v4 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = f32v4{0.f, 0.f, 0.f, 0.f};
+  const int nsteps = ld4 >> 2;
+  const int64_t alast = rows - 1;
+  const int blast = ng - 1;
+#pragma unroll
+  for (int st = 0; st < S - 1; ++st)
+    if (st < nsteps)
+      issue_stage_f32<BM, BN, NW>(lds + st * STAGE, wave, lane, A, G, t.m0, alast, t.n0, blast, ld4, ld4, st * 4);
+  for (int s = 0; s < nsteps; ++s) {
+    __syncthreads();   // every wave is done with the buffer the next issue overwrites
+    if (s + S - 1 < nsteps)
+      issue_stage_f32<BM, BN, NW>(lds + ((s + S - 1) % S) * STAGE, wave, lane, A, G, t.m0, alast, t.n0, blast,
+                                  ld4, ld4, (s + S - 1) * 4);
+    wait_stages<LPW, S>(min(S - 1, nsteps - 1 - s));
+    __syncthreads();   // stage s has landed for every wave
+    const uint4* st = lds + (s % S) * STAGE;
+    f32_step_swz<TM, TN>(st, st + BM * 4, wm0, wn0, lane, acc);
+  }
+  const int fc = lane & 15, fr4 = 4 * (lane >> 4);
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t r = (int64_t)t.m0 + wm0 + 16 * i + fr4 + e;
+      if (r >= rows) continue;
+      const int64_t x = r / bn;
+      const int nu = (int)(r - x * bn);
+      const int64_t orow = (x * nbf + boff[N] + nu) * (int64_t)ldo;
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int c = t.n0 + wn0 + 16 * j + fc;
+        if (c < ldo) out[orow + c] = c < ng ? acc[i][j][e] : 0.f;
+      }
+    }
+}
+
+// Segmented SYRK in FP32 over W (rows of ldw, a multiple of 16, floats; the
+// columns past the occupied count are zero): partial[chunk](m, n) =
+// sum_{X in chunk} sum_i W(X,m,i) W(X,n,i) on the upper 128 x 128 tiles, 8
+// waves of 64 x 32.  The FP32 MFMA accumulators are added into FP64 registers
+// every 16 steps (64 MFMA accumulations) and restarted.  All tiles of a chunk
+// run on one XCD (grid padded to a multiple of 8; padding blocks exit).
+template <int S>
+__global__ __launch_bounds__(512) void k_seg_syrk_f32_dl(const float* __restrict__ W, int nseg, int64_t sstride,
+                                                         int ldw, int n, const int2* __restrict__ tiles, int ntile,
+                                                         int segs_per_chunk, double* __restrict__ partial)
+{
+  constexpr int BM = 128, NW = 8, WN = 4, TM = 4, TN = 2, FLUSH = 16;
+  constexpr int LPW = (2 * BM / 16) / NW;
+  constexpr int STAGE = 2 * BM * 4;
+  __shared__ uint4 lds[S * STAGE];
+  const int lid = xcd_tile(blockIdx.x, gridDim.x);
+  const int chunk = lid / ntile;
+  const int s0 = chunk * segs_per_chunk;
+  if (s0 >= nseg) return;
+  const int2 tl = tiles[lid % ntile];
+  const int s1 = min(nseg, s0 + segs_per_chunk);
+  const int m0 = tl.x * BM, n0 = tl.y * BM;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave / WN) * 64, wn0 = (wave % WN) * 32;
+  f32v4 acc[TM][TN];
+  double acc64[TM][TN][4];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      acc[i][j] = f32v4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc64[i][j][e] = 0.0;
+    }
+  const int ksteps = ldw >> 4;
+  const int nsteps = (s1 - s0) * ksteps;
+  const int ld4 = ldw >> 2;
+  const uint4* W4 = reinterpret_cast<const uint4*>(W);
+  auto issue = [&](int q, int buf) {
+    const int sg = s0 + q / ksteps;
+    const int kc0 = (q % ksteps) * 4;
+    const uint4* base = W4 + (int64_t)sg * (sstride >> 2);
+    issue_stage_f32<BM, BM, NW>(lds + buf * STAGE, wave, lane, base, base, m0, n - 1, n0, n - 1, ld4, ld4, kc0);
+  };
+#pragma unroll
+  for (int st = 0; st < S - 1; ++st)
+    if (st < nsteps) issue(st, st);
+  for (int q = 0; q < nsteps; ++q) {
+    __syncthreads();
+    if (q + S - 1 < nsteps) issue(q + S - 1, (q + S - 1) % S);
+    wait_stages<LPW, S>(min(S - 1, nsteps - 1 - q));
+    __syncthreads();
+    const uint4* st = lds + (q % S) * STAGE;
+    f32_step_swz<TM, TN>(st, st + BM * 4, wm0, wn0, lane, acc);
+    if ((q + 1) % FLUSH == 0) flush_f64<TM, TN>(acc, acc64);
+  }
+  flush_f64<TM, TN>(acc, acc64);
+  double* P = partial + (int64_t)chunk * n * n;
+  const int fc = lane & 15, fr4 = 4 * (lane >> 4);
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int r = m0 + wm0 + 16 * i + fr4 + e;
+        const int c = n0 + wn0 + 16 * j + fc;
+        if (r < n && c < n) P[(int64_t)r * n + c] = acc64[i][j][e];
+      }
+}
+
+#define MCHIP_SLABF32DL_INST(WM, WN, TM, TN, S)                                                            \
+  template __global__ void k_slab_gemm_f32_dl<WM, WN, TM, TN, S>(                                          \
+      const float*, const int64_t*, const int*, const int64_t*, const int*, const int*, const Tile*, int, int, \
+      const float*, int, float*, int);
+MCHIP_SLABF32DL_INST(2, 2, 4, 2, 3)
+MCHIP_SLABF32DL_INST(2, 4, 4, 2, 3)
+MCHIP_SLABF32DL_INST(2, 2, 4, 5, 3)
+MCHIP_SLABF32DL_INST(2, 4, 4, 5, 3)
+MCHIP_SLABF32DL_INST(2, 4, 6, 5, 3)
+MCHIP_SLABF32DL_INST(2, 4, 4, 5, 2)
+#undef MCHIP_SLABF32DL_INST
+template __global__ void k_seg_syrk_f32_dl<2>(const float*, int, int64_t, int, int, const int2*, int, int, double*);
+template __global__ void k_seg_syrk_f32_dl<3>(const float*, int, int64_t, int, int, const int2*, int, int, double*);
+template __global__ void k_seg_syrk_f32_dl<4>(const float*, int, int64_t, int, int, const int2*, int, int, double*);
+
+}  // namespace mchip

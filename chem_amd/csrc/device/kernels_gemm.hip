@@ -171,93 +171,6 @@ __global__ __launch_bounds__(256) void k_seg_gemm(
       }
 }
 
-// FP32 segmented SYRK for the chosen-form exchange: 8 waves (2 x 4, 64 x 32
-// each) on a 128 x 128 upper tile; the FP32 MFMA accumulators are added into
-// FP64 registers every 16 steps (64 MFMA accumulations) and restarted, so the
-// FP32 rounding does not grow with the aux-chunk length.
-__global__ __launch_bounds__(512) void k_seg_syrk_f32(
-    const float* __restrict__ A, int nseg, int64_t sstride, int lda, int klen, int n,
-    const int2* __restrict__ tiles, int ntile, int segs_per_chunk, double* __restrict__ partial)
-{
-  constexpr int BM = 128, TM = 4, TN = 2, LDA = BM + 16, RA = BM / 32, FLUSH = 16;
-  __shared__ float As[2][GM_BK * LDA];
-  __shared__ float Bs[2][GM_BK * LDA];
-  const int2 tl = tiles[blockIdx.x % ntile];
-  const int chunk = blockIdx.x / ntile;
-  const int s0 = chunk * segs_per_chunk;
-  const int s1 = min(nseg, s0 + segs_per_chunk);
-  const int m0 = tl.x * BM, n0 = tl.y * BM;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm0 = (wave >> 2) * 64, wn0 = (wave & 3) * 32;
-  Mfma<float>::v4 acc[TM][TN];
-  double acc64[TM][TN][4];
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      acc[i][j] = Mfma<float>::v4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int e = 0; e < 4; ++e) acc64[i][j][e] = 0.0;
-    }
-  const int lk = tid & 15, lr = tid >> 4;
-  const int ksteps = (klen + GM_BK - 1) / GM_BK;
-  const int nsteps = (s1 > s0) ? (s1 - s0) * ksteps : 0;
-  float ra[RA], rb[RA];
-  auto load = [&](int q) {
-    const int s = s0 + q / ksteps;
-    const int kk = (q % ksteps) * GM_BK + lk;
-    const float* Ag = A + (int64_t)s * sstride;
-#pragma unroll
-    for (int j = 0; j < RA; ++j) {
-      const int m = m0 + lr + 32 * j, nn = n0 + lr + 32 * j;
-      ra[j] = (m < n && kk < klen) ? Ag[(int64_t)m * lda + kk] : 0.f;
-      rb[j] = (nn < n && kk < klen) ? Ag[(int64_t)nn * lda + kk] : 0.f;
-    }
-  };
-  auto store = [&](int buf) {
-#pragma unroll
-    for (int j = 0; j < RA; ++j) {
-      As[buf][lk * LDA + lr + 32 * j] = ra[j];
-      Bs[buf][lk * LDA + lr + 32 * j] = rb[j];
-    }
-  };
-  auto flush = [&]() {
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-      for (int j = 0; j < TN; ++j) {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) acc64[i][j][e] += (double)acc[i][j][e];
-        acc[i][j] = Mfma<float>::v4{0.f, 0.f, 0.f, 0.f};
-      }
-  };
-  if (nsteps > 0) {
-    load(0);
-    store(0);
-  }
-  __syncthreads();
-  for (int q = 0; q < nsteps; ++q) {
-    const int cur = q & 1;
-    if (q + 1 < nsteps) load(q + 1);
-    mfma_step<float, TM, TN, LDA, LDA>(As[cur], Bs[cur], wm0, wn0, lane, acc);
-    if (q + 1 < nsteps) store(cur ^ 1);
-    if ((q + 1) % FLUSH == 0) flush();
-    __syncthreads();
-  }
-  flush();
-  double* P = partial + (int64_t)chunk * n * n;
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int r = m0 + wm0 + 16 * i + Mfma<float>::row(lane, e);
-        const int c = n0 + wn0 + 16 * j + (lane & 15);
-        if (r < n && c < n) P[(int64_t)r * n + c] = acc64[i][j][e];
-      }
-}
-
 // explicit instantiations
 template __global__ void k_slab_gemm<double, 128, 128>(const double*, const int64_t*, const int*, const int*,
     const int*, const int*, const int*, const Tile*, int, int, const double*, int, int, double*, int);

@@ -1,9 +1,9 @@
 // Reduced-precision kernels of the chosen DF-K form on gfx950.
 //
 //   k_to_f32             B~ (FP64) -> FP32 copy, once per geometry
-//   k_rows_to_f32        dense FP64 operand -> FP32
 //   k_to_f32x2           FP64 -> FP32 pair hi + lo (x = hi + lo to ~2^-48)
-// The FP32 MFMA contractions are the FP32 instantiations in kernels_gemm.hip.
+// The FP32 MFMA contractions of the exchange are in kernels_f32dl.hip, those
+// of MP2 are the FP32 instantiations in kernels_gemm.hip.
 #include "kernels.hpp"
 
 namespace mchip {
@@ -28,16 +28,6 @@ __global__ void k_to_f32x2(const double* __restrict__ in, float* __restrict__ hi
     hi[i] = h;
     lo[i] = (float)(x - (double)h);
   }
-}
-
-// row-major G64 (rows x ld64, first ncols) -> G32 (rows x ncols)
-__global__ void k_rows_to_f32(const double* __restrict__ in, int ld64, int rows, int ncols,
-                              float* __restrict__ out)
-{
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)rows * ncols) return;
-  const int r = (int)(i / ncols), c = (int)(i % ncols);
-  out[i] = (float)in[(int64_t)r * ld64 + c];
 }
 
 }  // namespace mchip

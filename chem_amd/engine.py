@@ -226,6 +226,32 @@ class DeviceB:
                     direct_to_lds=bool(plan[4]), stages=int(plan[5]), syrk_variant=int(plan[6]),
                     gemm_ms=[float(v) for v in ms[:k - ns]], syrk_ms=[float(v) for v in ms[k - ns:k]])
 
+    def f32_plan(self, ng):
+        """The autotuned FP32 exchange pipeline for ng occupied columns (None
+        before the first FP32 build with that ng)."""
+        plan = np.zeros(8, dtype=np.int32)
+        ms = np.zeros(32, dtype=np.float32)
+        nms = ctypes.c_int()
+        _lib.check(_lib.lib().mchip_dfk_f32_plan(self.h, ng, _lib.iptr(plan), _lib.fptr(ms), ctypes.byref(nms)))
+        if not plan[7]:
+            return None
+        k = nms.value
+        ns = self.f32_shapes()[1]
+        return dict(gemm_shape=int(plan[0]), bm=int(plan[1]), bn=int(plan[2]), threads=int(plan[3]),
+                    stages=int(plan[5]), syrk_stages=int(plan[6]) + 2,
+                    gemm_ms=[float(v) for v in ms[:k - ns]], syrk_ms=[float(v) for v in ms[k - ns:k]])
+
+    def f32_force(self, ng, gemm_shape, syrk_variant):
+        """Use the given FP32 pipeline for ng columns instead of the autotuner's."""
+        _lib.check(_lib.lib().mchip_dfk_f32_force(self.h, ng, gemm_shape, syrk_variant))
+
+    @staticmethod
+    def f32_shapes():
+        """Number of FP32 slab-GEMM shapes and of FP32 SYRK variants."""
+        ng, ns = ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.lib().mchip_dfk_f32_shapes(ctypes.byref(ng), ctypes.byref(ns)))
+        return ng.value, ns.value
+
     def bf16_force(self, ng, gemm_shape, syrk_variant):
         """Use the given BF16 pipeline for ng columns instead of the autotuner's."""
         _lib.check(_lib.lib().mchip_dfk_bf16_force(self.h, ng, gemm_shape, syrk_variant))

@@ -222,6 +222,17 @@ int mchip_dfk_bf16_tile(mchip_dbt t, int ng, int* bn);
  * GEMM shape then every SYRK variant (-1 = not tried), *nms entries.  No
  * counterpart in the reference. */
 int mchip_dfk_bf16_plan(mchip_dbt t, int ng, int* plan, float* ms, int* nms);
+/* The autotuned FP32 exchange pipeline (direct-to-LDS FP32 MFMA slab GEMM and
+ * SYRK on the padded FP32 copy) for ng occupied columns, chosen on the first
+ * FP32 build with that ng: plan[8] = {GEMM shape index, row tile, column tile,
+ * threads, 1, pipeline stages, SYRK variant (0-2: 2-4 stages), 1 if tuned};
+ * ms[] (capacity 32) = the timings of every GEMM shape then every SYRK variant
+ * (-1 = not tried), *nms entries.  No counterpart in the reference. */
+int mchip_dfk_f32_plan(mchip_dbt t, int ng, int* plan, float* ms, int* nms);
+/* Fix the FP32 pipeline for ng columns instead of tuning it; shapes: the
+ * number of GEMM shapes and SYRK variants.  No counterpart in the reference. */
+int mchip_dfk_f32_force(mchip_dbt t, int ng, int gemm_shape, int syrk_variant);
+int mchip_dfk_f32_shapes(int* ngemm, int* nsyrk);
 /* Fix the BF16 pipeline for ng columns instead of tuning it (direct-to-LDS GEMM
  * shapes pair with SYRK variants 1-3, the others with 0); shapes: count of GEMM
  * shapes and SYRK variants and, per GEMM shape, whether it is direct-to-LDS

@@ -162,6 +162,62 @@ def test_fp32_exchange_accuracy(ctx):
     assert np.array_equal(K32, dB.dfk_mo_f32(C))
 
 
+@pytest.mark.parametrize("nocc", [15, 70, 333])
+def test_f32_every_pipeline(ctx, nocc):
+    """Every FP32 slab-GEMM shape with every SYRK variant (direct-to-LDS, 2-4
+    stages) on a block-sparse B with ragged blocks, an empty slab, several
+    128-row SYRK tiles and partial row / column tiles: all give bitwise the
+    same exchange (only the tiling of the output differs, not the order of any
+    sum) and agree with FP64 to 2e-6 relative."""
+    from chem_amd.engine import DeviceB, Metric
+    rng = np.random.default_rng(700 + nocc)
+    bsz = [14, 5, 5, 3, 14, 1, 7, 9, 30, 40, 50]
+    xsz = [23, 70, 9, 40, 17]
+    nbf, naux = sum(bsz), sum(xsz)
+    mask = (rng.random((len(xsz), len(bsz), len(bsz))) < 0.6).astype(np.uint8)
+    mask = np.maximum(mask, mask.transpose(0, 2, 1))
+    mask[:, 5, :] = 0
+    mask[:, :, 5] = 0
+    boff = np.concatenate([[0], np.cumsum(bsz)])
+    xoff = np.concatenate([[0], np.cumsum(xsz)])
+    dense = rng.standard_normal((naux, nbf, nbf))
+    dense = 0.5 * (dense + dense.transpose(0, 2, 1))
+    for x in range(len(xsz)):
+        for a in range(len(bsz)):
+            for b in range(len(bsz)):
+                if not mask[x, a, b]:
+                    dense[xoff[x]:xoff[x + 1], boff[a]:boff[a + 1], boff[b]:boff[b + 1]] = 0
+    Bh = ints.BTensorHost.from_dense(dense, xsz, bsz, mask)
+    A = rng.standard_normal((naux, naux))
+    V = A @ A.T + naux * np.eye(naux)
+    C = rng.standard_normal((nbf, nocc))
+    dB = DeviceB(ctx, Bh)
+    m = Metric(ctx, V)
+    dB.make_lowp(m, f32=True, bf16=False)
+    K64 = dB.dfk_mo(C, m)
+    ngemm, nsyrk = dB.f32_shapes()
+    Kfirst = None
+    for shape in range(ngemm):
+        for v in range(nsyrk):
+            print("case: nocc %d FP32 gemm shape %d syrk %d" % (nocc, shape, v), flush=True)
+            dB.f32_force(nocc, shape, v)
+            K = dB.dfk_mo_f32(C)
+            err = _relerr(K, K64)
+            print("  relerr %.3e" % err)
+            assert err < 2e-6
+            assert np.array_equal(K, K.T)
+            if Kfirst is None:
+                Kfirst = K
+            assert np.array_equal(K, Kfirst)
+    # the autotuner picks one of them
+    dB2 = DeviceB(ctx, Bh)
+    dB2.make_lowp(m, f32=True, bf16=False)
+    assert np.array_equal(dB2.dfk_mo_f32(C), Kfirst)
+    plan = dB2.f32_plan(nocc)
+    assert plan is not None and 0 <= plan["gemm_shape"] < ngemm
+    assert len(plan["gemm_ms"]) == ngemm and len(plan["syrk_ms"]) == nsyrk
+
+
 # BF16x3 carries every operand with 16 significant bits (hi + lo, 2^-17
 # relative rounding) and accumulates in FP32 over chunks of <= 256 aux
 # functions: K agrees with the FP64 exchange on the same B~ to a few 1e-6
