@@ -584,7 +584,8 @@ static const Tile* bf16_tiles(Ctx& c, DevBT& t, int BM, int BN, int ng, int* nti
 // Launch shapes of the BF16 slab GEMM: waves (WM x WN), MFMA tiles per wave
 // (TM x TN), BM = 16 WM TM rows, BN = 16 WN TN columns.  DL = 0: register-
 // staged loads, NB LDS buffers, FP32 W; DL = 1: direct-to-LDS loads, NB
-// pipeline stages, W written as bf16 hi/lo.
+// pipeline stages, W written as bf16 hi/lo; DL = 2: as 1 with the B~ rows in
+// an NB-stage pipeline of their own and the gathered coefficients in two.
 struct Bf16Shape {
   int WM, WN, TM, TN, NB, DL;
   int bm() const { return 16 * WM * TM; }
@@ -594,7 +595,8 @@ struct Bf16Shape {
 static const Bf16Shape kBf16Shapes[] = {
     {2, 2, 4, 2, 2, 0}, {2, 2, 4, 4, 2, 0}, {2, 2, 4, 5, 2, 0}, {2, 4, 4, 2, 2, 0}, {2, 4, 4, 3, 2, 0},
     {2, 4, 4, 4, 2, 0}, {2, 4, 4, 5, 2, 0}, {2, 4, 4, 2, 1, 0}, {2, 4, 4, 3, 1, 0}, {2, 4, 4, 2, 4, 1},
-    {2, 4, 4, 3, 3, 1}, {2, 4, 4, 5, 2, 1}, {1, 4, 4, 5, 3, 1}, {2, 2, 8, 5, 3, 1}, {2, 2, 8, 5, 2, 1}};
+    {2, 4, 4, 3, 3, 1}, {2, 4, 4, 5, 2, 1}, {1, 4, 4, 5, 3, 1}, {2, 2, 8, 5, 3, 1}, {2, 2, 8, 5, 2, 1},
+    {2, 4, 4, 5, 4, 2}, {2, 4, 4, 5, 3, 2}, {2, 4, 4, 3, 4, 2}, {2, 2, 8, 5, 3, 2}};
 static constexpr int kNumBf16Shapes = sizeof(kBf16Shapes) / sizeof(kBf16Shapes[0]);
 // SYRK variants: 0 = register-staged on FP32 W; 1..3 = direct-to-LDS on bf16
 // hi/lo W with 2, 3, 4 stages
@@ -617,6 +619,11 @@ static void slab_gemm_bf16(Ctx& c, DevBT& t, int shape, const uint16_t* ghi, con
                      c.stream, t.data_hi.as<uint16_t>(), t.data_lo.as<uint16_t>(), t.off16.as<int64_t>(), \
                      t.ldp.as<int>(), t.ldp_pre.as<int64_t>(), t.bsz.as<int>(), t.boff.as<int>(), tiles,  \
                      t.L.naux, t.L.nbf, ghi, glo, ng, nullptr, Whi, Wlo, ldw)
+#define MCHIP_BF16_SLAB_DLA(WM, WN, TM, TN, SA)                                                           \
+  hipLaunchKernelGGL((k_slab_gemm_bf16x3_dla<WM, WN, TM, TN, SA>), dim3(nt), dim3(64 * WM * WN), 0,       \
+                     c.stream, t.data_hi.as<uint16_t>(), t.data_lo.as<uint16_t>(), t.off16.as<int64_t>(), \
+                     t.ldp.as<int>(), t.ldp_pre.as<int64_t>(), t.bsz.as<int>(), t.boff.as<int>(), tiles,  \
+                     t.L.naux, t.L.nbf, ghi, glo, ng, Whi, Wlo, ldw)
   switch (shape) {
     case 0: MCHIP_BF16_SLAB(2, 2, 4, 2, 2); break;
     case 1: MCHIP_BF16_SLAB(2, 2, 4, 4, 2); break;
@@ -633,10 +640,15 @@ static void slab_gemm_bf16(Ctx& c, DevBT& t, int shape, const uint16_t* ghi, con
     case 12: MCHIP_BF16_SLAB_DL(1, 4, 4, 5, 3); break;
     case 13: MCHIP_BF16_SLAB_DL(2, 2, 8, 5, 3); break;
     case 14: MCHIP_BF16_SLAB_DL(2, 2, 8, 5, 2); break;
+    case 15: MCHIP_BF16_SLAB_DLA(2, 4, 4, 5, 4); break;
+    case 16: MCHIP_BF16_SLAB_DLA(2, 4, 4, 5, 3); break;
+    case 17: MCHIP_BF16_SLAB_DLA(2, 4, 4, 3, 4); break;
+    case 18: MCHIP_BF16_SLAB_DLA(2, 2, 8, 5, 3); break;
     default: throw std::runtime_error("BF16 slab GEMM: unknown launch shape");
   }
 #undef MCHIP_BF16_SLAB
 #undef MCHIP_BF16_SLAB_DL
+#undef MCHIP_BF16_SLAB_DLA
   HIP_CHECK(hipGetLastError());
 }
 
@@ -726,7 +738,7 @@ static const Bf16Plan& tune_bf16(Ctx& c, DevBT& t, const uint16_t* ghi, const ui
     if (bn > 64 && (ng + bn - 1) / bn * bn - ng >= bn / 2) continue;   // mostly empty tiles
     const float ms = time_launch(c, [&] { slab_gemm_bf16(c, t, v, ghi, glo, ng, W, Whi, Wlo, ldw); });
     p.k1_ms[v] = ms;
-    const int dl = kBf16Shapes[v].DL;
+    const int dl = kBf16Shapes[v].DL ? 1 : 0;   // W as bf16 hi/lo: pairs with the direct-to-LDS SYRKs
     if (ms < best1[dl]) {
       best1[dl] = ms;
       arg1[dl] = v;
@@ -780,7 +792,7 @@ void dfk_bf16_force(DevBT& t, int ng, int k1, int k3)
   if (k1 < 0 || k1 >= kNumBf16Shapes || k3 < 0 || k3 >= kNumSyrkVariants)
     throw std::runtime_error("BF16 plan: shape or SYRK variant out of range");
   if ((kBf16Shapes[k1].DL != 0) != (k3 != 0))
-    throw std::runtime_error("BF16 plan: direct-to-LDS GEMM shapes pair with SYRK variants 1-3, the others with 0");
+    throw std::runtime_error("BF16 plan: direct-to-LDS GEMM shapes pair with SYRK variants 1-4, the others with 0");
   Bf16Plan p;
   p.k1 = k1;
   p.k3 = k3;

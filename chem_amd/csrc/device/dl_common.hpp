@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 namespace mchip {
 
 // Workgroups are dispatched round-robin over the 8 XCDs; consecutive tile
@@ -44,6 +46,93 @@ __device__ __forceinline__ void wait_stages(int after)
   wait_vm<0>();
 }
 
-__device__ __forceinline__ int swz(int r, int c) { return c ^ ((r >> 2) & 3); }
+// Workgroup barrier without the memory fence of __syncthreads(): that fence
+// makes every wave wait for all of its outstanding global loads (vmcnt(0)),
+// the prefetched pipeline stages included, at every step.  The pipelined
+// kernels order their direct-to-LDS loads themselves (wait_vm / wait_stages
+// before the barrier that publishes a stage; the barrier before a stage is
+// overwritten comes after the MFMAs that consumed its LDS reads); the empty
+// asm statements keep the compiler from moving memory accesses across it.
+__device__ __forceinline__ void block_sync()
+{
+  asm volatile("" ::: "memory");
+  // lgkmcnt(0), vmcnt and expcnt at their maximum: this wave's LDS reads are done
+  __builtin_amdgcn_s_waitcnt((0xF) | (0x3 << 14) | (0x7 << 4));
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// f(integral_constant<int, 0>), ..., f(integral_constant<int, N - 1>): a loop
+// unrolled at compile time, so that each pipeline stage can live in an LDS
+// array of its own.  The compiler's wait-count insertion tells LDS arrays
+// apart and then waits, before a stage's LDS reads, only for the loads into
+// that stage; with one array indexed by a run-time stage number it waits for
+// every outstanding load.
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>)
+{
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// Runs an S-stage direct-to-LDS pipeline over nsteps steps.  issue(s, k)
+// starts the loads of step s into stage k (an integral_constant; the stage
+// is k mod S, each stage an LDS array of its own), LPW loads per wave;
+// compute(s, k) consumes step s from stage k.  Whole stage cycles in which
+// every step issues run first, without branches (the compiler's wait-count
+// model then stays exact across the loop's back edge and adds no waits of
+// its own); the last steps follow with the issues and waits guarded.
+template <int S, int LPW, class Issue, class Compute>
+__device__ __forceinline__ void dl_pipeline(int nsteps, Issue&& issue, Compute&& compute)
+{
+  static_for<S - 1>([&](auto k) {
+    if (k.value < nsteps) issue(k.value, k);
+  });
+  int s0 = 0;
+  for (; s0 + 2 * S - 1 <= nsteps; s0 += S) {
+    static_for<S>([&](auto k) {
+      const int s = s0 + k.value;
+      block_sync();   // every wave is done with the stage the next issue overwrites
+      issue(s + S - 1, std::integral_constant<int, k.value + S - 1>{});
+      wait_vm<(S - 1) * LPW>();
+      block_sync();   // step s has landed for every wave
+      compute(s, k);
+    });
+  }
+  for (; s0 < nsteps; s0 += S) {
+    static_for<S>([&](auto k) {
+      const int s = s0 + k.value;
+      if (s < nsteps) {
+        block_sync();
+        if (s + S - 1 < nsteps) issue(s + S - 1, std::integral_constant<int, k.value + S - 1>{});
+        wait_stages<LPW, S>(min(S - 1, nsteps - 1 - s));
+        block_sync();
+        compute(s, k);
+      }
+    });
+  }
+}
+
+// Stage k mod S of up to four LDS arrays.
+template <int S, class K, class T>
+__device__ __forceinline__ T* pick_stage(K, T* a0, T* a1, T* a2, T* a3)
+{
+  constexpr int i = K::value % S;
+  if constexpr (i == 0) return a0;
+  else if constexpr (i == 1) return a1;
+  else if constexpr (i == 2) return a2;
+  else return a3;
+}
+
+// Position of 16-byte chunk c of row r (64-byte rows) in an LDS stage tile.
+// A fragment read takes one chunk from each of 16 consecutive rows (from a
+// multiple of 16); with f(r) = ((r >> 1) ^ (r >> 2)) & 3 those 16 chunks fill
+// 16 distinct 16-byte bank groups of a 64-bank LDS, and every 8 of them
+// (lanes 0-7, 8-15) 8 distinct groups of a 32-bank LDS: conflict-free either way.
+__device__ __forceinline__ int swz(int r, int c) { return c ^ (((r >> 1) ^ (r >> 2)) & 3); }
 
 }  // namespace mchip

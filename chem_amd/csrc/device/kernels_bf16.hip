@@ -369,10 +369,10 @@ template __global__ void k_seg_syrk_bf16x3<128>(const float*, int, int64_t, int,
 // Operand tiles go from HBM straight into LDS, S stages deep, with no
 // register staging: a wave-wide load writes 64 x 16 bytes contiguously (lane
 // i at +16 i), i.e. 16 rows of 32 bf16.  Lane i of the load for rows
-// [16q, 16q + 16) fetches row r = 16q + i/4, 16-byte chunk c = (i%4) ^ f(r),
-// f(r) = (r/4)%4, so that a row's chunk c sits at position c ^ f(r): the 16
-// rows an MFMA fragment read touches (one chunk each) then hit 16 distinct
-// bank groups.  Rows past the operand's end are clamped onto its last row
+// [16q, 16q + 16) fetches row r = 16q + i/4, 16-byte chunk c = swz(r, i%4),
+// so that a row's chunk c sits at position swz(r, c) (an XOR, its own
+// inverse): the 16 rows an MFMA fragment read touches (one chunk each) then
+// hit distinct bank groups (dl_common.hpp).  Rows past the operand's end are clamped onto its last row
 // (finite data whose products are never stored, or stored as zero).
 // Every wave issues the same number of loads per stage, so a stage is
 // complete once vmcnt drops to (stages still in flight) x (loads per wave).
@@ -441,6 +441,42 @@ __device__ __forceinline__ void issue_stage(uint4* st, int wave, int lane, const
   }
 }
 
+// W rows (X, boff + nu) for the tile's rows X*bn + nu of the slab, columns
+// [n0, n0 + BN) below ldo, zero at and past ng: FP32 (out) or bf16 hi/lo
+// (whi, wlo).  Accumulator layout of the 16x16 MFMA: D[4*(lane/16) + e][lane%16].
+template <int TM, int TN>
+__device__ __forceinline__ void store_w_tile(const f32x4 (&acc)[TM][TN], const Tile& t, int wm0, int wn0, int lane,
+                                             int64_t rows, int bn, int nbf, int boffN, int ng, float* out,
+                                             uint16_t* whi, uint16_t* wlo, int ldo)
+{
+  const int fc = lane & 15, fr4 = 4 * (lane >> 4);
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t r = (int64_t)t.m0 + wm0 + 16 * i + fr4 + e;
+      if (r >= rows) continue;
+      const int64_t x = r / bn;
+      const int nu = (int)(r - x * bn);
+      const int64_t orow = (x * nbf + boffN + nu) * (int64_t)ldo;
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int c = t.n0 + wn0 + 16 * j + fc;
+        if (c >= ldo) continue;
+        const float v = c < ng ? acc[i][j][e] : 0.f;
+        if (whi) {
+          uint16_t h, l;
+          split32(v, h, l);
+          whi[orow + c] = h;
+          wlo[orow + c] = l;
+        }
+        else {
+          out[orow + c] = v;
+        }
+      }
+    }
+}
+
 // Grouped slab GEMM, direct-to-LDS pipelined.  Same operands and tile table
 // as k_slab_gemm_bf16x3 (m0 steps of BM).  W columns in [ng, ldo) are
 // written as zero.  split_out: W is written as bf16 hi/lo (whi, wlo, same
@@ -457,7 +493,8 @@ __global__ __launch_bounds__(64 * WM * WN) void k_slab_gemm_bf16x3_dl(
   constexpr int BM = 16 * WM * TM, BN = 16 * WN * TN;
   constexpr int LPW = (2 * BM / 16 + 2 * BN / 16) / NW;
   constexpr int STAGE = (BM + BN) * 8;   // uint4 per stage
-  __shared__ uint4 lds[S * STAGE];
+  static_assert(S >= 2 && S <= 4, "pipeline depth");
+  __shared__ uint4 lds0[STAGE], lds1[STAGE], lds2[STAGE], lds3[STAGE];
   const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   if (t.slab < 0) return;
   const int N = t.slab;
@@ -478,47 +515,146 @@ __global__ __launch_bounds__(64 * WM * WN) void k_slab_gemm_bf16x3_dl(
   const int nsteps = ld8 >> 2;
   const int64_t alast = rows - 1;
   const int blast = ng - 1;
-#pragma unroll
-  for (int st = 0; st < S - 1; ++st)
-    if (st < nsteps)
-      issue_stage<BM, BN, NW>(lds + st * STAGE, wave, lane, Ah, Al, Gh, Gl, t.m0, alast, t.n0, blast, ld8, ld8,
-                              st * 4);
-  for (int s = 0; s < nsteps; ++s) {
-    __syncthreads();   // every wave is done with the buffer the next issue overwrites
-    if (s + S - 1 < nsteps)
-      issue_stage<BM, BN, NW>(lds + ((s + S - 1) % S) * STAGE, wave, lane, Ah, Al, Gh, Gl, t.m0, alast, t.n0,
-                              blast, ld8, ld8, (s + S - 1) * 4);
-    wait_stages<LPW, S>(min(S - 1, nsteps - 1 - s));
-    __syncthreads();   // stage s has landed for every wave
-    const uint4* st = lds + (s % S) * STAGE;
-    bf16x3_step_swz<TM, TN>(st, st + BM * 4, st + BM * 8, st + BM * 8 + BN * 4, wm0, wn0, lane, acc);
-  }
-  const int fc = lane & 15, fr4 = 4 * (lane >> 4);
+  dl_pipeline<S, LPW>(
+      nsteps,
+      [&](int s, auto k) {
+        issue_stage<BM, BN, NW>(pick_stage<S>(k, lds0, lds1, lds2, lds3), wave, lane, Ah, Al, Gh, Gl, t.m0, alast,
+                                t.n0, blast, ld8, ld8, s * 4);
+      },
+      [&](int, auto k) {
+        const uint4* st = pick_stage<S>(k, lds0, lds1, lds2, lds3);
+        bf16x3_step_swz<TM, TN>(st, st + BM * 4, st + BM * 8, st + BM * 8 + BN * 4, wm0, wn0, lane, acc);
+      });
+  store_w_tile<TM, TN>(acc, t, wm0, wn0, lane, rows, bn, nbf, boff[N], ng, out, whi, wlo, ldo);
+}
+
+// Grouped slab GEMM with separate pipelines for the two operands: the B~ rows
+// (streamed once from HBM) are loaded SA - 1 steps ahead of use, the gathered
+// coefficients (re-read by every row tile of a slab, served from L2) one.  At
+// BN = 320 a stage of both would take 56 KB and allow only double buffering;
+// split, the B~ stream gets SA - 1 steps of latency cover in the same LDS.
+// Every stage is an LDS array of its own and the step loop is unrolled over a
+// whole cycle of stages, so a step waits only for its own stage's loads.
+// Same operands, tile table and output as k_slab_gemm_bf16x3_dl (bf16 hi/lo W).
+template <int WM, int WN, int TM, int TN, int SA>
+__global__ __launch_bounds__(64 * WM * WN) void k_slab_gemm_bf16x3_dla(
+    const uint16_t* __restrict__ bhi, const uint16_t* __restrict__ blo, const int64_t* __restrict__ off16,
+    const int* __restrict__ ldp, const int64_t* __restrict__ ldp_pre, const int* __restrict__ bsz,
+    const int* __restrict__ boff, const Tile* __restrict__ tiles, int naux, int nbf,
+    const uint16_t* __restrict__ ghi, const uint16_t* __restrict__ glo, int ng, uint16_t* __restrict__ whi,
+    uint16_t* __restrict__ wlo, int ldo)
+{
+  static_assert(SA >= 2 && SA <= 4, "B~ pipeline depth");
+  constexpr int NW = WM * WN;
+  constexpr int BM = 16 * WM * TM, BN = 16 * WN * TN;
+  constexpr int QA = BM / 16, QB = BN / 16;
+  static_assert((2 * QA) % NW == 0 && (2 * QB) % NW == 0, "stage loads must split evenly over the waves");
+  constexpr int LA = 2 * QA / NW, LB = 2 * QB / NW;   // wave-loads per wave per stage
+  constexpr int SGA = BM * 8, SGB = BN * 8;           // uint4 per stage: hi rows, then lo rows
+  constexpr int CYC = SA % 2 == 0 ? SA : 2 * SA;      // steps until both stage cycles repeat
+  __shared__ uint4 sa0[SGA], sa1[SGA], sa2[SGA], sa3[SGA];
+  __shared__ uint4 sb0[SGB], sb1[SGB];
+  auto stage_a = [&](auto k) -> uint4* {
+    constexpr int i = decltype(k)::value % SA;
+    if constexpr (i == 0) return sa0;
+    else if constexpr (i == 1) return sa1;
+    else if constexpr (i == 2) return sa2;
+    else return sa3;
+  };
+  auto stage_b = [&](auto k) -> uint4* {
+    if constexpr (decltype(k)::value % 2 == 0) return sb0;
+    else return sb1;
+  };
+  const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  if (t.slab < 0) return;
+  const int N = t.slab;
+  const int bn = bsz[N];
+  const int ld8 = ldp[N] >> 3;
+  const int64_t rows = (int64_t)naux * bn;
+  const uint4* Ah = reinterpret_cast<const uint4*>(bhi + off16[N]);
+  const uint4* Al = reinterpret_cast<const uint4*>(blo + off16[N]);
+  const uint4* Gh = reinterpret_cast<const uint4*>(ghi + ldp_pre[N] * ng);
+  const uint4* Gl = reinterpret_cast<const uint4*>(glo + ldp_pre[N] * ng);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave / WN) * (16 * TM), wn0 = (wave % WN) * (16 * TN);
+  const int rr = lane >> 2, pos = lane & 3;
+  f32x4 acc[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int64_t r = (int64_t)t.m0 + wm0 + 16 * i + fr4 + e;
-      if (r >= rows) continue;
-      const int64_t x = r / bn;
-      const int nu = (int)(r - x * bn);
-      const int64_t orow = (x * nbf + boff[N] + nu) * (int64_t)ldo;
+    for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int nsteps = ld8 >> 2;
+  const int64_t alast = rows - 1;
+  const int blast = ng - 1;
+  // wave-loads j = wave + NW*u over [hi rows/16][lo rows/16] of one operand
+  auto issue_a = [&](int s, uint4* st) {
 #pragma unroll
-      for (int j = 0; j < TN; ++j) {
-        const int c = t.n0 + wn0 + 16 * j + fc;
-        if (c >= ldo) continue;
-        const float v = c < ng ? acc[i][j][e] : 0.f;
-        if (whi) {
-          uint16_t h, l;
-          split32(v, h, l);
-          whi[orow + c] = h;
-          wlo[orow + c] = l;
-        }
-        else {
-          out[orow + c] = v;
-        }
-      }
+    for (int u = 0; u < LA; ++u) {
+      const int j = wave + NW * u;
+      const bool lo = j >= QA;
+      const int q = lo ? j - QA : j;
+      const int r = 16 * q + rr;
+      const int64_t gr = t.m0 + r < alast ? t.m0 + r : alast;
+      lds_load16((lo ? Al : Ah) + gr * ld8 + s * 4 + swz(r, pos), st + (lo ? BM * 4 : 0) + q * 64);
     }
+  };
+  auto issue_b = [&](int s, uint4* st) {
+#pragma unroll
+    for (int u = 0; u < LB; ++u) {
+      const int j = wave + NW * u;
+      const bool lo = j >= QB;
+      const int q = lo ? j - QB : j;
+      const int r = 16 * q + rr;
+      const int gr = t.n0 + r < blast ? t.n0 + r : blast;
+      lds_load16((lo ? Gl : Gh) + (int64_t)gr * ld8 + s * 4 + swz(r, pos), st + (lo ? BN * 4 : 0) + q * 64);
+    }
+  };
+  // issue order A(0..SA-2), B(0), then per step A(s+SA-1), B(s+1): every load
+  // that step s needs precedes B(s), so B(s) landing (vmcnt in order) is enough
+  static_for<SA - 1>([&](auto k) {
+    if (k.value < nsteps) issue_a(k.value, stage_a(k));
+  });
+  if (nsteps > 0) issue_b(0, stage_b(std::integral_constant<int, 0>{}));
+  auto compute = [&](auto k) {
+    const uint4* sa = stage_a(k);
+    const uint4* sb = stage_b(k);
+    bf16x3_step_swz<TM, TN>(sa, sa + BM * 4, sb, sb + BN * 4, wm0, wn0, lane, acc);
+  };
+  // whole stage cycles in which every step issues both loads (no branches:
+  // the wait-count model stays exact across the loop's back edge), then the
+  // remaining steps with the issues and waits guarded
+  int s0 = 0;
+  for (; s0 + CYC + SA - 1 <= nsteps; s0 += CYC) {
+    static_for<CYC>([&](auto k) {
+      const int s = s0 + k.value;
+      block_sync();   // every wave is done with step s - 1's buffers
+      issue_a(s + SA - 1, stage_a(std::integral_constant<int, k.value + SA - 1>{}));
+      issue_b(s + 1, stage_b(std::integral_constant<int, k.value + 1>{}));
+      wait_vm<LA + LB>();
+      block_sync();   // step s's operands have landed for every wave
+      compute(k);
+    });
+  }
+  for (; s0 < nsteps; s0 += CYC) {
+    static_for<CYC>([&](auto k) {
+      const int s = s0 + k.value;
+      if (s < nsteps) {
+        block_sync();
+        const bool na = s + SA - 1 < nsteps, nb = s + 1 < nsteps;
+        if (na) issue_a(s + SA - 1, stage_a(std::integral_constant<int, k.value + SA - 1>{}));
+        if (nb) issue_b(s + 1, stage_b(std::integral_constant<int, k.value + 1>{}));
+        if (na)
+          wait_vm<LA + LB>();
+        else if (nb)
+          wait_vm<LB>();
+        else
+          wait_vm<0>();
+        block_sync();
+        compute(k);
+      }
+    });
+  }
+  store_w_tile<TM, TN>(acc, t, wm0, wn0, lane, rows, bn, nbf, boff[N], ng, nullptr, whi, wlo, ldo);
 }
 
 // Segmented SYRK over bf16 hi/lo W, direct-to-LDS pipelined: partial[chunk](m,n)
@@ -536,7 +672,8 @@ __global__ __launch_bounds__(BM * 2) void k_seg_syrk_bf16x3_dl(
   constexpr int TM = BM / 32, TN = BM / WN / 16;
   constexpr int LPW = (4 * BM / 16) / NW;
   constexpr int STAGE = 2 * BM * 8;
-  __shared__ uint4 lds[S * STAGE];
+  static_assert(S >= 2 && S <= 4, "pipeline depth");
+  __shared__ uint4 lds0[STAGE], lds1[STAGE], lds2[STAGE], lds3[STAGE];
   // all tiles of a chunk run on one XCD, dispatched together: the W rows of
   // the chunk come from HBM about once per XCD and are re-read from its L2
   // (grid padded to a multiple of 8; padding blocks exit)
@@ -559,24 +696,19 @@ __global__ __launch_bounds__(BM * 2) void k_seg_syrk_bf16x3_dl(
   const int ld8 = ldw >> 3;
   const uint4* Wh = reinterpret_cast<const uint4*>(whi);
   const uint4* Wl = reinterpret_cast<const uint4*>(wlo);
-  auto issue = [&](int q, int buf) {
-    const int sg = s0 + q / ksteps;
-    const int kc0 = (q % ksteps) * 4;
-    const int64_t base = (int64_t)sg * (sstride >> 3);
-    issue_stage<BM, BM, NW>(lds + buf * STAGE, wave, lane, Wh + base, Wl + base, Wh + base, Wl + base, m0,
-                            n - 1, n0, n - 1, ld8, ld8, kc0);
-  };
-#pragma unroll
-  for (int st = 0; st < S - 1; ++st)
-    if (st < nsteps) issue(st, st);
-  for (int q = 0; q < nsteps; ++q) {
-    __syncthreads();
-    if (q + S - 1 < nsteps) issue(q + S - 1, (q + S - 1) % S);
-    wait_stages<LPW, S>(min(S - 1, nsteps - 1 - q));
-    __syncthreads();
-    const uint4* st = lds + (q % S) * STAGE;
-    bf16x3_step_swz<TM, TN>(st, st + BM * 4, st + BM * 8, st + BM * 12, wm0, wn0, lane, acc);
-  }
+  dl_pipeline<S, LPW>(
+      nsteps,
+      [&](int q, auto k) {
+        const int sg = s0 + q / ksteps;
+        const int kc0 = (q % ksteps) * 4;
+        const int64_t base = (int64_t)sg * (sstride >> 3);
+        issue_stage<BM, BM, NW>(pick_stage<S>(k, lds0, lds1, lds2, lds3), wave, lane, Wh + base, Wl + base,
+                                Wh + base, Wl + base, m0, n - 1, n0, n - 1, ld8, ld8, kc0);
+      },
+      [&](int, auto k) {
+        const uint4* st = pick_stage<S>(k, lds0, lds1, lds2, lds3);
+        bf16x3_step_swz<TM, TN>(st, st + BM * 4, st + BM * 8, st + BM * 12, wm0, wn0, lane, acc);
+      });
   double* P = partial + (int64_t)chunk * n * n;
   const int fc = lane & 15, fr4 = 4 * (lane >> 4);
 #pragma unroll
@@ -603,6 +735,15 @@ MCHIP_SLAB16DL_INST(1, 4, 4, 5, 3)
 MCHIP_SLAB16DL_INST(2, 2, 8, 5, 3)
 MCHIP_SLAB16DL_INST(2, 2, 8, 5, 2)
 #undef MCHIP_SLAB16DL_INST
+#define MCHIP_SLAB16DLA_INST(WM, WN, TM, TN, SA)                                                       \
+  template __global__ void k_slab_gemm_bf16x3_dla<WM, WN, TM, TN, SA>(                                \
+      const uint16_t*, const uint16_t*, const int64_t*, const int*, const int64_t*, const int*,         \
+      const int*, const Tile*, int, int, const uint16_t*, const uint16_t*, int, uint16_t*, uint16_t*, int);
+MCHIP_SLAB16DLA_INST(2, 4, 4, 5, 4)
+MCHIP_SLAB16DLA_INST(2, 4, 4, 5, 3)
+MCHIP_SLAB16DLA_INST(2, 4, 4, 3, 4)
+MCHIP_SLAB16DLA_INST(2, 2, 8, 5, 3)
+#undef MCHIP_SLAB16DLA_INST
 #define MCHIP_SYRK16DL_INST(S, BM)                                                                     \
   template __global__ void k_seg_syrk_bf16x3_dl<S, BM>(const uint16_t*, const uint16_t*, int, int64_t, int, \
                                                        int, const int2*, int, int, double*);

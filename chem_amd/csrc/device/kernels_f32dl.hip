@@ -124,7 +124,8 @@ __global__ __launch_bounds__(64 * WM * WN) void k_slab_gemm_f32_dl(
   // for one stage-load more than it needs, never less
   constexpr int LPW = (BM / 16 + BN / 16) / NW;
   constexpr int STAGE = (BM + BN) * 4;   // uint4 per stage
-  __shared__ uint4 lds[S * STAGE];
+  static_assert(S >= 2 && S <= 4, "pipeline depth");
+  __shared__ uint4 lds0[STAGE], lds1[STAGE], lds2[STAGE], lds3[STAGE];
   const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   if (t.slab < 0) return;
   const int N = t.slab;
@@ -143,20 +144,16 @@ __global__ __launch_bounds__(64 * WM * WN) void k_slab_gemm_f32_dl(
   const int nsteps = ld4 >> 2;
   const int64_t alast = rows - 1;
   const int blast = ng - 1;
-#pragma unroll
-  for (int st = 0; st < S - 1; ++st)
-    if (st < nsteps)
-      issue_stage_f32<BM, BN, NW>(lds + st * STAGE, wave, lane, A, G, t.m0, alast, t.n0, blast, ld4, ld4, st * 4);
-  for (int s = 0; s < nsteps; ++s) {
-    __syncthreads();   // every wave is done with the buffer the next issue overwrites
-    if (s + S - 1 < nsteps)
-      issue_stage_f32<BM, BN, NW>(lds + ((s + S - 1) % S) * STAGE, wave, lane, A, G, t.m0, alast, t.n0, blast,
-                                  ld4, ld4, (s + S - 1) * 4);
-    wait_stages<LPW, S>(min(S - 1, nsteps - 1 - s));
-    __syncthreads();   // stage s has landed for every wave
-    const uint4* st = lds + (s % S) * STAGE;
-    f32_step_swz<TM, TN>(st, st + BM * 4, wm0, wn0, lane, acc);
-  }
+  dl_pipeline<S, LPW>(
+      nsteps,
+      [&](int s, auto k) {
+        issue_stage_f32<BM, BN, NW>(pick_stage<S>(k, lds0, lds1, lds2, lds3), wave, lane, A, G, t.m0, alast, t.n0,
+                                    blast, ld4, ld4, s * 4);
+      },
+      [&](int, auto k) {
+        const uint4* st = pick_stage<S>(k, lds0, lds1, lds2, lds3);
+        f32_step_swz<TM, TN>(st, st + BM * 4, wm0, wn0, lane, acc);
+      });
   const int fc = lane & 15, fr4 = 4 * (lane >> 4);
 #pragma unroll
   for (int i = 0; i < TM; ++i)
@@ -189,7 +186,8 @@ __global__ __launch_bounds__(512) void k_seg_syrk_f32_dl(const float* __restrict
   constexpr int BM = 128, NW = 8, WN = 4, TM = 4, TN = 2, FLUSH = 16;
   constexpr int LPW = (2 * BM / 16) / NW;
   constexpr int STAGE = 2 * BM * 4;
-  __shared__ uint4 lds[S * STAGE];
+  static_assert(S >= 2 && S <= 4, "pipeline depth");
+  __shared__ uint4 lds0[STAGE], lds1[STAGE], lds2[STAGE], lds3[STAGE];
   const int lid = xcd_tile(blockIdx.x, gridDim.x);
   const int chunk = lid / ntile;
   const int s0 = chunk * segs_per_chunk;
@@ -213,24 +211,20 @@ __global__ __launch_bounds__(512) void k_seg_syrk_f32_dl(const float* __restrict
   const int nsteps = (s1 - s0) * ksteps;
   const int ld4 = ldw >> 2;
   const uint4* W4 = reinterpret_cast<const uint4*>(W);
-  auto issue = [&](int q, int buf) {
-    const int sg = s0 + q / ksteps;
-    const int kc0 = (q % ksteps) * 4;
-    const uint4* base = W4 + (int64_t)sg * (sstride >> 2);
-    issue_stage_f32<BM, BM, NW>(lds + buf * STAGE, wave, lane, base, base, m0, n - 1, n0, n - 1, ld4, ld4, kc0);
-  };
-#pragma unroll
-  for (int st = 0; st < S - 1; ++st)
-    if (st < nsteps) issue(st, st);
-  for (int q = 0; q < nsteps; ++q) {
-    __syncthreads();
-    if (q + S - 1 < nsteps) issue(q + S - 1, (q + S - 1) % S);
-    wait_stages<LPW, S>(min(S - 1, nsteps - 1 - q));
-    __syncthreads();
-    const uint4* st = lds + (q % S) * STAGE;
-    f32_step_swz<TM, TN>(st, st + BM * 4, wm0, wn0, lane, acc);
-    if ((q + 1) % FLUSH == 0) flush_f64<TM, TN>(acc, acc64);
-  }
+  dl_pipeline<S, LPW>(
+      nsteps,
+      [&](int q, auto k) {
+        const int sg = s0 + q / ksteps;
+        const int kc0 = (q % ksteps) * 4;
+        const uint4* base = W4 + (int64_t)sg * (sstride >> 2);
+        issue_stage_f32<BM, BM, NW>(pick_stage<S>(k, lds0, lds1, lds2, lds3), wave, lane, base, base, m0, n - 1, n0,
+                                    n - 1, ld4, ld4, kc0);
+      },
+      [&](int q, auto k) {
+        const uint4* st = pick_stage<S>(k, lds0, lds1, lds2, lds3);
+        f32_step_swz<TM, TN>(st, st + BM * 4, wm0, wn0, lane, acc);
+        if ((q + 1) % FLUSH == 0) flush_f64<TM, TN>(acc, acc64);
+      });
   flush_f64<TM, TN>(acc, acc64);
   double* P = partial + (int64_t)chunk * n * n;
   const int fc = lane & 15, fr4 = 4 * (lane >> 4);

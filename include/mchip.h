@@ -216,9 +216,11 @@ int mchip_dfk_ao(mchip_dbt t, mchip_metric m, const double* P, int form, int occ
  * that ng).  No counterpart in the reference. */
 int mchip_dfk_bf16_tile(mchip_dbt t, int ng, int* bn);
 /* The full autotuned BF16 pipeline for ng occupied columns: plan[8] = {GEMM
- * shape index, row tile, column tile, threads, direct-to-LDS (0/1), LDS
- * buffers or pipeline stages, SYRK variant (0 register-staged, 1-3 direct-to-
- * LDS with 2-4 stages), 1 if tuned}; ms[] (capacity 32) = the timings of every
+ * shape index, row tile, column tile, threads, direct-to-LDS (0 register-
+ * staged, 1 one pipeline for both operands, 2 the B~ rows in a pipeline of
+ * their own), LDS buffers or pipeline stages, SYRK variant (0 register-staged,
+ * 1-3 direct-to-LDS 128-row tiles with 2-4 stages, 4 direct-to-LDS 256-row
+ * tiles), 1 if tuned}; ms[] (capacity 32) = the timings of every
  * GEMM shape then every SYRK variant (-1 = not tried), *nms entries.  No
  * counterpart in the reference. */
 int mchip_dfk_bf16_plan(mchip_dbt t, int ng, int* plan, float* ms, int* nms);
@@ -234,7 +236,7 @@ int mchip_dfk_f32_plan(mchip_dbt t, int ng, int* plan, float* ms, int* nms);
 int mchip_dfk_f32_force(mchip_dbt t, int ng, int gemm_shape, int syrk_variant);
 int mchip_dfk_f32_shapes(int* ngemm, int* nsyrk);
 /* Fix the BF16 pipeline for ng columns instead of tuning it (direct-to-LDS GEMM
- * shapes pair with SYRK variants 1-3, the others with 0); shapes: count of GEMM
+ * shapes pair with SYRK variants 1-4, the others with 0); shapes: count of GEMM
  * shapes and SYRK variants and, per GEMM shape, whether it is direct-to-LDS
  * (direct has room for ngemm entries).  No counterpart in the reference. */
 int mchip_dfk_bf16_force(mchip_dbt t, int ng, int gemm_shape, int syrk_variant);
