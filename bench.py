@@ -4,21 +4,21 @@ Workload (default --config c2): seeded 64-water cluster, cc-pVDZ / cc-pVDZ-jkfit
 atomic blocks, Schwarz screening at 1e-10.
 
 Plain path: the reference's DFMO algorithm in FP64 (K1 = B.C, K2 = L^-1 per
-build, K3 = SYRK).  Chosen form: low-precision copies of B~ = L^-1 B made
-once from B (which stays FP64): a BF16 hi/lo split and an FP32 copy (--form
-auto, default: BF16x3 builds until the SCF error falls below max(1e-4,
-10*thresh), FP32 builds after), or one of them alone (--form bf16 / f32).
-Every SCF iteration builds K on MFMA with FP32 accumulation and FP64
-reduction; J and one plain FP64 K at the converged orbitals give the reported
-energy with the reference's formulas.
+build, K3 = SYRK).  Chosen form: copies of B~ = L^-1 B made once from B (which
+stays FP64 for J and the energy): a BF16 hi/lo split, an FP32 copy and an FP64
+copy.  --form auto (default) runs the library's precision escalation: BF16x3
+builds until the SCF error falls below --escalate (or stops improving), then
+FP32 builds, then FP64 ones if FP32 stalls; every build runs on MFMA.  J and
+one FP64 exchange (on the FP64 copy) at the converged orbitals give the
+reported energy with the reference's formulas.
 
 A step is one SCF iteration's DF-K work in the chosen form.  The timed region
-runs `steps` FP32 builds plus the one final FP64 build at the converged
-orbitals, so value = (steps * t_fp32 + t_fp64_final) / steps; steps defaults
-to the chosen SCF's iteration count.  The line also carries the plain FP64
-seconds per build, |dE| between the two SCFs, the roofline of the dominant
-kernel (K1, the B.C slab GEMM) from HIP events on the library stream, and the
-oracle timed on the host.
+replays the chosen SCF's sequence of builds (same forms, same count; steps
+defaults to its iteration count) plus the final FP64 build, so value =
+(sum of the step builds + final build) / steps.  The line also carries the
+plain FP64 seconds per build, |dE| between the two SCFs, the roofline of the
+dominant kernel (K1 of the BF16x3 form) from HIP events on the library stream,
+and the oracle timed on the host.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c0|c1|c2]
 """
@@ -245,7 +245,7 @@ def kernels_only(args):
     ctx = Context(0)
     dB = DeviceB.build_on_gpu(ctx, mol, 1e-10)
     met = Metric(ctx, ints.int2c2e(mol))
-    dB.make_lowp(met)
+    dB.make_lowp(met, f64=True)
     S, _, _ = ints.int1e(mol)
     rng = np.random.default_rng(7)
     C = rng.standard_normal((mol.nbf, mol.nocc))
@@ -253,7 +253,7 @@ def kernels_only(args):
     C = np.ascontiguousarray(C @ U / np.sqrt(w))
     steps = args.steps or 3
     out = {}
-    for name, fn in (("bf16x3", dB.dfk_mo_bf16), ("f32", dB.dfk_mo_f32)):
+    for name, fn in (("bf16x3", dB.dfk_mo_bf16), ("f32", dB.dfk_mo_f32), ("fp64_copy", dB.dfk_mo_tilde64)):
         fn(C)
         ctx.sync()
         t = time.perf_counter()
@@ -270,6 +270,7 @@ def kernels_only(args):
         out[name + "_frob_rel_vs_fp64"] = float(np.linalg.norm(fn(C) - K64) / np.linalg.norm(K64))
     out["bf16_plan"] = dB.bf16_plan(mol.nocc)
     out["f32_plan"] = dB.f32_plan(mol.nocc)
+    out["f64_plan"] = dB.f64_plan(mol.nocc)
     log("fp64 build %.4f s" % out["fp64_build_s"])
     return out
 
@@ -475,6 +476,12 @@ def main():
         kernels["k_seg_syrk_f32_dl"] = dict(s=ph[2], tflops=k3_flops / ph[2] / 1e12,
                                             builds=nb[1], peak=MFMA_F32_PEAK / 1e12)
         kernels["f32_plan"] = dB.f32_plan(mol.nocc)
+    ph = phases(k64, nrep=2)
+    kernels["k_slab_gemm_f64_dl"] = dict(s=ph[0], tflops=k1_flops / ph[0] / 1e12, builds=1 + nb[2],
+                                         peak=MFMA_F64_PEAK / 1e12)
+    kernels["k_seg_syrk_f64_dl"] = dict(s=ph[2], tflops=k3_flops / ph[2] / 1e12, builds=1 + nb[2],
+                                        peak=MFMA_F64_PEAK / 1e12)
+    kernels["f64_plan"] = dB.f64_plan(mol.nocc)
     dom = max((k for k in kernels if isinstance(kernels[k], dict) and "s" in kernels[k]),
               key=lambda k: kernels[k]["s"] * kernels[k]["builds"])
     d = kernels[dom]

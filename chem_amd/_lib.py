@@ -148,6 +148,9 @@ _SIGS = [
     ("mchip_dfk_f32_plan", c_int, [c_void_p, c_int, P_int, P_float, P_int]),
     ("mchip_dfk_f32_force", c_int, [c_void_p, c_int, c_int, c_int]),
     ("mchip_dfk_f32_shapes", c_int, [P_int, P_int]),
+    ("mchip_dfk_f64_plan", c_int, [c_void_p, c_int, P_int, P_float, P_int]),
+    ("mchip_dfk_f64_force", c_int, [c_void_p, c_int, c_int, c_int]),
+    ("mchip_dfk_f64_shapes", c_int, [P_int, P_int]),
     ("mchip_dfk_bf16_force", c_int, [c_void_p, c_int, c_int, c_int]),
     ("mchip_dfk_bf16_shapes", c_int, [P_int, P_int, P_int]),
     ("mchip_dfk_ao_dev", c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),

@@ -473,6 +473,27 @@ extern "C" int mchip_dfk_f32_shapes(int* ngemm, int* nsyrk)
   MCHIP_CATCH
 }
 
+extern "C" int mchip_dfk_f64_plan(mchip_dbt t, int ng, int* plan, float* ms, int* nms)
+{
+  MCHIP_TRY
+  dfk_f64_plan(t->t, ng, plan, ms, nms);
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dfk_f64_force(mchip_dbt t, int ng, int gemm_shape, int syrk_variant)
+{
+  MCHIP_TRY
+  dfk_f64_force(t->t, ng, gemm_shape, syrk_variant);
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dfk_f64_shapes(int* ngemm, int* nsyrk)
+{
+  MCHIP_TRY
+  *ngemm = dfk_f64_num_shapes(nsyrk);
+  MCHIP_CATCH
+}
+
 extern "C" int mchip_scf_run(mchip_dbt t, mchip_metric m, int nbf, int nocc, int nbb,
                              const int* bsz, const double* S, const double* H, double enuc,
                              double threshold, int max_iter, int do_diis, int diis_start,

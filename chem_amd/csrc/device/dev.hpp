@@ -239,13 +239,14 @@ struct DevBT {
   BLayout L;                 // host copy of the layout
   DevBuf data64;             // FP64 slabs (B or B~)
   DevBuf data32;             // FP32 slabs (optional), rows padded to ldp[N] at off16[N]
-  DevBuf data64t;            // FP64 L^-1 B slabs beside the untransformed B (optional)
+  DevBuf data64t;            // FP64 L^-1 B slabs beside the untransformed B (optional), rows padded like data32
   DevBuf data_hi, data_lo;   // BF16 split slabs (optional), rows padded to ldp[N]
   DevBuf ldp, off16, ldp_pre;               // padded row length, slab offsets (elements), prefix of ldp
   std::vector<int> h_ldp;
   std::vector<int64_t> h_off16, h_ldp_pre;
   std::map<int, Bf16Plan> bf16_plan;        // ng -> tuned BF16 exchange pipeline
   std::map<int, Bf16Plan> f32_plan;         // ng -> tuned FP32 exchange pipeline
+  std::map<int, Bf16Plan> f64_plan;         // ng -> tuned FP64 slab GEMM / SYRK (-1: not yet)
   DevBuf slab_off, ncol, bsz, boff, colmap, colmap_off, pg_off;
   std::vector<int> h_colmap, h_colmap_off;
   std::vector<int64_t> h_pg_off;            // prefix of bsz[N]*ncol[N]

@@ -331,13 +331,22 @@ void dfj(Ctx& c, DevBT& t, const Metric* m, const double* P, double* J, double p
   HIP_CHECK(hipGetLastError());
 }
 
+// FP64 direct-to-LDS exchange pieces (defined with the other precision forms below)
+static void dfk_mo_tilde64_dl(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K, int occ_nbatches);
+static void syrk_f64_exchange(Ctx& c, DevBT& t, const double* W, int ng, int ldw, int nb, int b, double* K);
+
 void dfk_mo(Ctx& c, DevBT& t, const Metric* m, const double* C, int ldc, int nocc, double* K,
             int occ_nbatches, const double* tilde64)
 {
   const BLayout& L = t.L;
-  // tilde64: FP64 slabs of L^-1 B (same layout as B): K1 on them, no K2
-  const bool pre = t.transformed || tilde64 != nullptr;
-  const double* bdata = tilde64 ? tilde64 : t.data64.as<double>();
+  // tilde64: the padded FP64 copy of L^-1 B: K1 on it, no K2
+  if (tilde64) {
+    if (tilde64 != t.data64t.as<double>()) throw std::runtime_error("DFMO-K: tilde64 must be the tensor's FP64 copy");
+    dfk_mo_tilde64_dl(c, t, C, ldc, nocc, K, occ_nbatches);
+    return;
+  }
+  const bool pre = t.transformed;
+  const double* bdata = t.data64.as<double>();
   if (!pre && !m) throw std::runtime_error("DFMO-K needs the metric for an untransformed B");
   const int naux = L.naux, nbf = L.nbf;
   if (nocc <= 0) {
@@ -348,10 +357,11 @@ void dfk_mo(Ctx& c, DevBT& t, const Metric* m, const double* C, int ldc, int noc
   occ_nbatches = std::max(1, std::min(occ_nbatches, nocc));
   const int per = (nocc + occ_nbatches - 1) / occ_nbatches;
   const int nb = (nocc + per - 1) / per;
-  const size_t ht = (size_t)naux * nbf * per;
+  // HT rows padded to whole 64-byte steps of the SYRK, zero past the batch width
+  const int ldw = (per + 7) / 8 * 8;
+  const size_t ht = (size_t)naux * nbf * ldw;
   c.scratch_a.ensure(ht * sizeof(double));
   if (!pre) c.scratch_b.ensure(ht * sizeof(double));
-  DevBuf& partial = c.scratch_partial;
   for (int b = 0; b < nb; ++b) {
     const int i0 = b * per;
     const int w = std::min(per, nocc - i0);
@@ -359,14 +369,16 @@ void dfk_mo(Ctx& c, DevBT& t, const Metric* m, const double* C, int ldc, int noc
     // K1: HT1[x][nu][i] = sum_mu B(x|mu nu) C(mu, i0+i)
     {
       PhaseTimer tm(c, PH_K1);
-      slab_gemm_t<double>(c, t, bdata, C + i0, ldc, w, HT1, w);
+      if (ldw != w)
+        hipLaunchKernelGGL(k_fill, dim3(gs_blocks((int64_t)ht)), dim3(256), 0, c.stream, HT1, (int64_t)ht, 0.0);
+      slab_gemm_t<double>(c, t, bdata, C + i0, ldc, w, HT1, ldw);
     }
     const double* HT = HT1;
     if (!pre) {
       PhaseTimer tm(c, PH_K2);
       // K2: HT2 = L^-1 HT1 over x; column-major (W x naux) view times (L^-1)^T
       double* HT2 = c.scratch_b.as<double>();
-      const int64_t W = (int64_t)nbf * w;
+      const int64_t W = (int64_t)nbf * ldw;
       const double one = 1.0;
       RB_CHECK(rocblas_dtrmm_64(c.blas, rocblas_side_right, rocblas_fill_lower,
                                 rocblas_operation_transpose, rocblas_diagonal_non_unit, W, naux, &one,
@@ -375,7 +387,7 @@ void dfk_mo(Ctx& c, DevBT& t, const Metric* m, const double* C, int ldc, int noc
     }
     // K3: K = -sum_{x,i} HT[x][m][i] HT[x][n][i]
     PhaseTimer tm(c, PH_K3);
-    seg_syrk(c, HT, HT, naux, (int64_t)nbf * w, w, w, nbf, -1.0, K, partial, nb, b, b == nb - 1);
+    syrk_f64_exchange(c, t, HT, w, ldw, nb, b, K);
   }
 }
 
@@ -484,28 +496,32 @@ static void bf16_layout(Ctx& c, DevBT& t)
 // B, which stays in data64: J, the reported energy and every FP64 exchange
 // keep the reference's formula (explicit V^-1 in J, L^-1 applied per build in
 // DFMO), while the per-iteration exchange reads the FP32 and/or BF16 copies.
-// Per slab: B_slab (L^-1)^T in FP64 (the slab's rows of L^-1 B) into out64
-// (same slab layout) or a temporary, then its FP32 copy into out32 if not null
-// (in the padded layout if pad32, else in the slab layout of data64) and its
-// bf16 hi/lo split into t's padded planes if bf16.
+// Per slab: B_slab (L^-1)^T in FP64 (the slab's rows of L^-1 B) into a
+// temporary, then its padded FP64 copy into out64 if not null, its FP32 copy
+// into out32 if not null (in the padded layout if pad32, else in the slab
+// layout of data64) and its bf16 hi/lo split into t's padded planes if bf16.
 static void tilde_slabs(Ctx& c, DevBT& t, const Metric& m, float* out32, bool pad32, bool bf16,
                         double* out64 = nullptr)
 {
   const BLayout& L = t.L;
   int64_t maxslab = 0;
   for (int n = 0; n < L.nbb; ++n) maxslab = std::max<int64_t>(maxslab, L.slab_off[n + 1] - L.slab_off[n]);
-  DevBuf tmp(out64 ? 0 : (size_t)std::max<int64_t>(maxslab, 1) * sizeof(double));
+  DevBuf tmp((size_t)std::max<int64_t>(maxslab, 1) * sizeof(double));
   const double one = 1.0;
   for (int n = 0; n < L.nbb; ++n) {
     const int64_t R = (int64_t)L.bsz[n] * L.ncol[n];
     if (R == 0) continue;
     const double* slab = t.data64.as<double>() + L.slab_off[n];
-    double* dst = out64 ? out64 + L.slab_off[n] : tmp.as<double>();
+    double* dst = tmp.as<double>();
     // column-major (R x naux) view: dst = slab * (L^-1)^T
     RB_CHECK(rocblas_dtrmm_64(c.blas, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
                               rocblas_diagonal_non_unit, R, L.naux, &one, m.Linv.as<double>(), L.naux, slab, R,
                               dst, R));
     const int64_t ne = R * L.naux;
+    if (out64)
+      hipLaunchKernelGGL(k_pad_f64_slab, dim3(gs_blocks((int64_t)L.naux * L.bsz[n] * t.h_ldp[n])), dim3(256), 0,
+                         c.stream, dst, n, t.ncol.as<int>(), t.bsz.as<int>(), L.naux, t.ldp.as<int>(),
+                         t.off16.as<int64_t>(), out64);
     if (out32 && pad32)
       hipLaunchKernelGGL(k_pad_f32_slab, dim3(gs_blocks((int64_t)L.naux * L.bsz[n] * t.h_ldp[n])), dim3(256), 0,
                          c.stream, dst, n, t.ncol.as<int>(), t.bsz.as<int>(), L.naux, t.ldp.as<int>(),
@@ -526,12 +542,13 @@ void dbt_make_lowp(Ctx& c, DevBT& t, const Metric& m, bool f32, bool bf16, bool 
 {
   if (t.transformed) throw std::runtime_error("B is transformed: use the copies of the transformed tensor");
   if (m.naux != t.L.naux) throw std::runtime_error("metric and B have different aux dimensions");
-  if (f32 || bf16) pad_layout(c, t);
+  if (f32 || bf16 || f64) pad_layout(c, t);
   if (f32) t.data32.alloc((size_t)std::max<int64_t>(padded_elems(t), 1) * sizeof(float));
   if (bf16) bf16_layout(c, t);
-  if (f64) t.data64t.alloc((size_t)std::max<int64_t>(t.L.total, 1) * sizeof(double));
+  if (f64) t.data64t.alloc((size_t)std::max<int64_t>(padded_elems(t), 1) * sizeof(double));
   tilde_slabs(c, t, m, f32 ? t.data32.as<float>() : nullptr, true, bf16, f64 ? t.data64t.as<double>() : nullptr);
   if (f32) t.f32_plan.clear();
+  t.f64_plan.clear();
   t.lowp_tilde = true;
 }
 
@@ -912,22 +929,22 @@ void dbt_make_f32(Ctx& c, DevBT& t)
   t.lowp_tilde = true;
 }
 
-// Launch shapes of the FP32 slab GEMM (direct-to-LDS): waves WM x WN, MFMA
-// tiles per wave TM x TN, S pipeline stages.
-struct F32Shape {
+// Launch shapes of the FP32 and FP64 slab GEMMs (direct-to-LDS): waves
+// WM x WN, MFMA tiles per wave TM x TN, S pipeline stages.
+struct DlShape {
   int WM, WN, TM, TN, S;
   int bm() const { return 16 * WM * TM; }
   int bn() const { return 16 * WN * TN; }
   int threads() const { return 64 * WM * WN; }
 };
-static const F32Shape kF32Shapes[] = {{2, 2, 4, 2, 3}, {2, 4, 4, 2, 3}, {2, 2, 4, 5, 3},
+static const DlShape kDlShapes[] = {{2, 2, 4, 2, 3}, {2, 4, 4, 2, 3}, {2, 2, 4, 5, 3},
                                       {2, 4, 4, 5, 3}, {2, 4, 6, 5, 3}, {2, 4, 4, 5, 2}};
-static constexpr int kNumF32Shapes = sizeof(kF32Shapes) / sizeof(kF32Shapes[0]);
+static constexpr int kNumDlShapes = sizeof(kDlShapes) / sizeof(kDlShapes[0]);
 static constexpr int kNumF32Syrk = 3;   // direct-to-LDS SYRK with 2, 3, 4 stages
 
 static void slab_gemm_f32dl(Ctx& c, DevBT& t, int shape, const float* G, int ng, float* W, int ldw)
 {
-  const F32Shape sh = kF32Shapes[shape];
+  const DlShape sh = kDlShapes[shape];
   int nt = 0;
   const Tile* tiles = bf16_tiles(c, t, sh.bm(), sh.bn(), ng, &nt);
   if (nt == 0) return;
@@ -981,11 +998,11 @@ static const Bf16Plan& tune_f32(Ctx& c, DevBT& t, const float* G, int ng, float*
   auto it = t.f32_plan.find(ng);
   if (it != t.f32_plan.end()) return it->second;
   Bf16Plan p;
-  p.k1_ms.assign(kNumF32Shapes, -1.f);
+  p.k1_ms.assign(kNumDlShapes, -1.f);
   p.k3_ms.assign(kNumF32Syrk, -1.f);
   float best = 1e30f;
-  for (int v = 0; v < kNumF32Shapes; ++v) {
-    const int bn = kF32Shapes[v].bn();
+  for (int v = 0; v < kNumDlShapes; ++v) {
+    const int bn = kDlShapes[v].bn();
     if (bn > 64 && (ng + bn - 1) / bn * bn - ng >= bn / 2) continue;   // mostly empty tiles
     const float ms = time_launch(c, [&] { slab_gemm_f32dl(c, t, v, G, ng, W, ldw); });
     p.k1_ms[v] = ms;
@@ -1014,7 +1031,7 @@ void dfk_f32_plan(const DevBT& t, int ng, int* plan, float* ms, int* nms)
   if (nms) *nms = 0;
   if (it == t.f32_plan.end()) return;
   const Bf16Plan& p = it->second;
-  const F32Shape& sh = kF32Shapes[p.k1];
+  const DlShape& sh = kDlShapes[p.k1];
   plan[0] = p.k1;
   plan[1] = sh.bm();
   plan[2] = sh.bn();
@@ -1033,12 +1050,12 @@ void dfk_f32_plan(const DevBT& t, int ng, int* plan, float* ms, int* nms)
 
 void dfk_f32_force(DevBT& t, int ng, int k1, int k3)
 {
-  if (k1 < 0 || k1 >= kNumF32Shapes || k3 < 0 || k3 >= kNumF32Syrk)
+  if (k1 < 0 || k1 >= kNumDlShapes || k3 < 0 || k3 >= kNumF32Syrk)
     throw std::runtime_error("FP32 plan: shape or SYRK variant out of range");
   Bf16Plan p;
   p.k1 = k1;
   p.k3 = k3;
-  p.k1_ms.assign(kNumF32Shapes, -1.f);
+  p.k1_ms.assign(kNumDlShapes, -1.f);
   p.k3_ms.assign(kNumF32Syrk, -1.f);
   t.f32_plan[ng] = p;
 }
@@ -1046,7 +1063,7 @@ void dfk_f32_force(DevBT& t, int ng, int k1, int k3)
 int dfk_f32_num_shapes(int* nsyrk)
 {
   if (nsyrk) *nsyrk = kNumF32Syrk;
-  return kNumF32Shapes;
+  return kNumDlShapes;
 }
 
 // K = -sum_X W_X^T W_X with W = B~ C (DFMO_K::compute_K, jkdf.cpp:221-462, on L^-1 B): both
@@ -1104,6 +1121,205 @@ void dfk_mo_f32(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K,
                          c.scratch_partial.as<double>(), nb * nch, nbf, 0, -1.0, K);
     HIP_CHECK(hipGetLastError());
   }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 exchange pieces on padded operands (direct-to-LDS)
+// ---------------------------------------------------------------------------
+
+// Launch shapes of the FP64 slab GEMM: waves WM x WN, MFMA tiles per wave
+// TM x TN, S pipeline stages.
+static const DlShape kF64Shapes[] = {{2, 2, 4, 2, 3}, {2, 4, 4, 2, 3}, {2, 2, 4, 5, 3}, {2, 4, 4, 5, 3},
+                                      {2, 4, 4, 5, 2}};
+static constexpr int kNumF64Shapes = sizeof(kF64Shapes) / sizeof(kF64Shapes[0]);
+static constexpr int kNumF64Syrk = 3;   // direct-to-LDS SYRK with 2, 3, 4 stages
+
+static void slab_gemm_f64dl(Ctx& c, DevBT& t, int shape, const double* G, int ng, double* W, int ldw)
+{
+  int nt = 0;
+  const Tile* tiles = bf16_tiles(c, t, kF64Shapes[shape].bm(), kF64Shapes[shape].bn(), ng, &nt);
+  if (nt == 0) return;
+#define MCHIP_F64_SLAB(WM, WN, TM, TN, S)                                                                     \
+  hipLaunchKernelGGL((k_slab_gemm_f64_dl<WM, WN, TM, TN, S>), dim3(nt), dim3(64 * WM * WN), 0, c.stream,      \
+                     t.data64t.as<double>(), t.off16.as<int64_t>(), t.ldp.as<int>(), t.ldp_pre.as<int64_t>(), \
+                     t.bsz.as<int>(), t.boff.as<int>(), tiles, t.L.naux, t.L.nbf, G, ng, W, ldw)
+  switch (shape) {
+    case 0: MCHIP_F64_SLAB(2, 2, 4, 2, 3); break;
+    case 1: MCHIP_F64_SLAB(2, 4, 4, 2, 3); break;
+    case 2: MCHIP_F64_SLAB(2, 2, 4, 5, 3); break;
+    case 3: MCHIP_F64_SLAB(2, 4, 4, 5, 3); break;
+    case 4: MCHIP_F64_SLAB(2, 4, 4, 5, 2); break;
+    default: throw std::runtime_error("FP64 slab GEMM: unknown launch shape");
+  }
+#undef MCHIP_F64_SLAB
+  HIP_CHECK(hipGetLastError());
+}
+
+static void seg_syrk_f64dl(Ctx& c, int variant, const double* W, int naux, int nbf, int ldw, const int2* tl,
+                           int ntile, int nch, int spc, double* partial)
+{
+  const dim3 grid8((unsigned)((ntile * nch + 7) / 8 * 8));
+  const int64_t ss = (int64_t)nbf * ldw;
+  switch (variant) {
+    case 0:
+      hipLaunchKernelGGL(k_seg_syrk_f64_dl<2>, grid8, dim3(512), 0, c.stream, W, naux, ss, ldw, nbf, tl, ntile, spc,
+                         partial);
+      break;
+    case 1:
+      hipLaunchKernelGGL(k_seg_syrk_f64_dl<3>, grid8, dim3(512), 0, c.stream, W, naux, ss, ldw, nbf, tl, ntile, spc,
+                         partial);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_seg_syrk_f64_dl<4>, grid8, dim3(512), 0, c.stream, W, naux, ss, ldw, nbf, tl, ntile, spc,
+                         partial);
+      break;
+    default: throw std::runtime_error("FP64 SYRK: unknown variant");
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+// FP64 plan entry for ng columns (k1 / k3 = -1 until tuned or forced).
+static Bf16Plan& f64_plan_entry(DevBT& t, int ng)
+{
+  auto it = t.f64_plan.find(ng);
+  if (it != t.f64_plan.end()) return it->second;
+  Bf16Plan p;
+  p.k1_ms.assign(kNumF64Shapes, -1.f);
+  p.k3_ms.assign(kNumF64Syrk, -1.f);
+  return t.f64_plan[ng] = p;
+}
+
+// K3 of an FP64 exchange batch: partial sums of -W_X W_X^T over aux chunks on
+// the upper 128 x 128 tiles (FP64 MFMA), reduced into K after the last batch.
+// The SYRK variant is timed once per ng on the live operand (autotuner).
+static void syrk_f64_exchange(Ctx& c, DevBT& t, const double* W, int ng, int ldw, int nb, int b, double* K)
+{
+  const int naux = t.L.naux, nbf = t.L.nbf;
+  int ntile = 0;
+  const int2* tl = syrk_tiles(c, t, nbf, 128, &ntile);
+  int nch = std::max(1, std::min(naux, 2048 / std::max(1, ntile)));
+  const int spc = (naux + nch - 1) / nch;
+  nch = (naux + spc - 1) / spc;
+  if (b == 0) c.scratch_partial.ensure((size_t)nb * nch * nbf * nbf * sizeof(double));
+  double* partial = c.scratch_partial.as<double>() + (size_t)b * nch * nbf * nbf;
+  Bf16Plan& p = f64_plan_entry(t, ng);
+  if (p.k3 < 0) {
+    float best = 1e30f;
+    for (int v = 0; v < kNumF64Syrk; ++v) {
+      const float ms = time_launch(c, [&] { seg_syrk_f64dl(c, v, W, naux, nbf, ldw, tl, ntile, nch, spc, partial); });
+      p.k3_ms[v] = ms;
+      if (ms < best) {
+        best = ms;
+        p.k3 = v;
+      }
+    }
+  }
+  seg_syrk_f64dl(c, p.k3, W, naux, nbf, ldw, tl, ntile, nch, spc, partial);
+  if (b == nb - 1)
+    hipLaunchKernelGGL(k_reduce_sym, dim3(nblk((int64_t)nbf * nbf, 256)), dim3(256), 0, c.stream,
+                       c.scratch_partial.as<double>(), nb * nch, nbf, 0, -1.0, K);
+  HIP_CHECK(hipGetLastError());
+}
+
+// DFMO-K (DFMO_K::compute_K, jkdf.cpp:221-462) on the padded FP64 copy of
+// L^-1 B: K1 = direct-to-LDS FP64 slab GEMM with the gathered occupied
+// coefficients, no K2, K3 = FP64 SYRK.  The slab-GEMM shape is timed once per
+// ng on the live operands (autotuner).
+static void dfk_mo_tilde64_dl(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K, int occ_nbatches)
+{
+  const BLayout& L = t.L;
+  if (!t.data64t.p || t.h_ldp_pre.size() != (size_t)L.nbb + 1)
+    throw std::runtime_error("FP64 copy of L^-1 B missing (mchip_dbt_make_lowp with f64)");
+  const int naux = L.naux, nbf = L.nbf;
+  if (nocc <= 0) {
+    hipLaunchKernelGGL(k_fill, dim3(gs_blocks((int64_t)nbf * nbf)), dim3(256), 0, c.stream, K,
+                       (int64_t)nbf * nbf, 0.0);
+    return;
+  }
+  occ_nbatches = std::max(1, std::min(occ_nbatches, nocc));
+  const int per = (nocc + occ_nbatches - 1) / occ_nbatches;
+  const int nb = (nocc + per - 1) / per;
+  const int ldw = (per + 7) / 8 * 8;
+  const size_t nW = (size_t)naux * nbf * ldw;
+  const size_t nG = (size_t)t.h_ldp_pre[L.nbb] * per;
+  c.scratch_a.ensure((nW + nG) * sizeof(double) + 256);
+  double* W = c.scratch_a.as<double>();
+  double* G = W + nW;   // 64-byte aligned: nW is a multiple of 8
+  int gmax = 0;
+  for (int n = 0; n < L.nbb; ++n) gmax = std::max(gmax, t.h_ldp[n]);
+  for (int b = 0; b < nb; ++b) {
+    const int i0 = b * per;
+    const int w = std::min(per, nocc - i0);
+    {
+      PhaseTimer tm(c, PH_SPLIT);
+      if (gmax > 0 && L.nbb > 0)
+        hipLaunchKernelGGL(k_gather_ct_f64, dim3(nblk((int64_t)w * gmax, 256), L.nbb), dim3(256), 0, c.stream,
+                           C + i0, ldc, w, t.ncol.as<int>(), t.ldp.as<int>(), t.ldp_pre.as<int64_t>(),
+                           t.colmap.as<int>(), t.colmap_off.as<int>(), G);
+      HIP_CHECK(hipGetLastError());
+    }
+    Bf16Plan& p = f64_plan_entry(t, w);
+    if (p.k1 < 0) {
+      float best = 1e30f;
+      for (int v = 0; v < kNumF64Shapes; ++v) {
+        const int bn = kF64Shapes[v].bn();
+        if (bn > 64 && (w + bn - 1) / bn * bn - w >= bn / 2) continue;   // mostly empty tiles
+        const float ms = time_launch(c, [&] { slab_gemm_f64dl(c, t, v, G, w, W, ldw); });
+        p.k1_ms[v] = ms;
+        if (ms < best) {
+          best = ms;
+          p.k1 = v;
+        }
+      }
+    }
+    {
+      PhaseTimer tm(c, PH_K1);
+      slab_gemm_f64dl(c, t, p.k1, G, w, W, ldw);
+    }
+    PhaseTimer tm(c, PH_K3);
+    syrk_f64_exchange(c, t, W, w, ldw, nb, b, K);
+  }
+}
+
+void dfk_f64_plan(const DevBT& t, int ng, int* plan, float* ms, int* nms)
+{
+  auto it = t.f64_plan.find(ng);
+  for (int i = 0; i < 8; ++i) plan[i] = 0;
+  if (nms) *nms = 0;
+  if (it == t.f64_plan.end()) return;
+  const Bf16Plan& p = it->second;
+  plan[0] = p.k1;
+  if (p.k1 >= 0) {
+    const DlShape& sh = kF64Shapes[p.k1];
+    plan[1] = sh.bm();
+    plan[2] = sh.bn();
+    plan[3] = sh.threads();
+    plan[5] = sh.S;
+  }
+  plan[4] = 1;
+  plan[6] = p.k3;
+  plan[7] = 1;
+  if (ms && nms) {
+    int k = 0;
+    for (float v : p.k1_ms) ms[k++] = v;
+    for (float v : p.k3_ms) ms[k++] = v;
+    *nms = k;
+  }
+}
+
+void dfk_f64_force(DevBT& t, int ng, int k1, int k3)
+{
+  if (k1 < 0 || k1 >= kNumF64Shapes || k3 < 0 || k3 >= kNumF64Syrk)
+    throw std::runtime_error("FP64 plan: shape or SYRK variant out of range");
+  Bf16Plan& p = f64_plan_entry(t, ng);
+  p.k1 = k1;
+  p.k3 = k3;
+}
+
+int dfk_f64_num_shapes(int* nsyrk)
+{
+  if (nsyrk) *nsyrk = kNumF64Syrk;
+  return kNumF64Shapes;
 }
 
 }  // namespace mchip

@@ -44,6 +44,10 @@ void dfk_bf16_plan(const DevBT& t, int ng, int* plan, float* ms, int* nms);
 void dfk_f32_plan(const DevBT& t, int ng, int* plan, float* ms, int* nms);
 void dfk_f32_force(DevBT& t, int ng, int k1, int k3);
 int dfk_f32_num_shapes(int* nsyrk);
+// Tuned FP64 pipeline (same fields; plan[0] / plan[6] = -1 until tuned).
+void dfk_f64_plan(const DevBT& t, int ng, int* plan, float* ms, int* nms);
+void dfk_f64_force(DevBT& t, int ng, int k1, int k3);
+int dfk_f64_num_shapes(int* nsyrk);
 void dfk_bf16_force(DevBT& t, int ng, int k1, int k3);
 int dfk_bf16_num_shapes(int* nsyrk);
 int dfk_bf16_shape_dl(int k1);

@@ -17,6 +17,13 @@ __device__ __forceinline__ int xcd_tile(int bid, int grid)
   return (bid & 7) * (grid >> 3) + (bid >> 3);
 }
 
+// Position of 16-byte chunk c of row r (64-byte rows) in an LDS stage tile.
+// A fragment read takes one chunk from each of 16 consecutive rows (from a
+// multiple of 16); with f(r) = ((r >> 1) ^ (r >> 2)) & 3 those 16 chunks fill
+// 16 distinct 16-byte bank groups of a 64-bank LDS, and every 8 of them
+// (lanes 0-7, 8-15) 8 distinct groups of a 32-bank LDS: conflict-free either way.
+__device__ __forceinline__ int swz(int r, int c) { return c ^ (((r >> 1) ^ (r >> 2)) & 3); }
+
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
 
 __device__ __forceinline__ void lds_load16(const uint4* g, uint4* l)
@@ -117,6 +124,37 @@ __device__ __forceinline__ void dl_pipeline(int nsteps, Issue&& issue, Compute&&
   }
 }
 
+// Loads of one stage of two single-plane operands (64-byte rows: 16 FP32 or
+// 8 FP64 of each row): wave-loads j = wave + NW*u over [A rows/16][B rows/16];
+// a wave-load brings 16 rows x 64 bytes.  Rows past alast / blast are
+// clamped.  When NW does not divide the count, the first waves issue one load
+// more than the others.
+template <int BM, int BN, int NW>
+__device__ __forceinline__ void issue_rows(uint4* st, int wave, int lane, const uint4* A, const uint4* B,
+                                                int64_t a0, int64_t alast, int b0, int blast, int lda4, int ldb4,
+                                                int kc0)
+{
+  constexpr int QA = BM / 16, QB = BN / 16, NJ = QA + QB;
+  constexpr int UMAX = (NJ + NW - 1) / NW;
+  const int rr = lane >> 2, pos = lane & 3;
+#pragma unroll
+  for (int u = 0; u < UMAX; ++u) {
+    const int j = wave + NW * u;
+    if (j >= NJ) break;
+    if (j < QA) {
+      const int r = 16 * j + rr;
+      const int64_t gr = a0 + r < alast ? a0 + r : alast;
+      lds_load16(A + gr * lda4 + kc0 + swz(r, pos), st + j * 64);
+    }
+    else {
+      const int q = j - QA;
+      const int r = 16 * q + rr;
+      const int64_t gr = b0 + r < blast ? b0 + r : blast;
+      lds_load16(B + gr * ldb4 + kc0 + swz(r, pos), st + BM * 4 + q * 64);
+    }
+  }
+}
+
 // Stage k mod S of up to four LDS arrays.
 template <int S, class K, class T>
 __device__ __forceinline__ T* pick_stage(K, T* a0, T* a1, T* a2, T* a3)
@@ -127,12 +165,5 @@ __device__ __forceinline__ T* pick_stage(K, T* a0, T* a1, T* a2, T* a3)
   else if constexpr (i == 2) return a2;
   else return a3;
 }
-
-// Position of 16-byte chunk c of row r (64-byte rows) in an LDS stage tile.
-// A fragment read takes one chunk from each of 16 consecutive rows (from a
-// multiple of 16); with f(r) = ((r >> 1) ^ (r >> 2)) & 3 those 16 chunks fill
-// 16 distinct 16-byte bank groups of a 64-bank LDS, and every 8 of them
-// (lanes 0-7, 8-15) 8 distinct groups of a 32-bank LDS: conflict-free either way.
-__device__ __forceinline__ int swz(int r, int c) { return c ^ (((r >> 1) ^ (r >> 2)) & 3); }
 
 }  // namespace mchip

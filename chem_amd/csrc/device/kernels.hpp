@@ -103,6 +103,19 @@ template <int WM, int WN, int TM, int TN, int S>
 __global__ void k_slab_gemm_f32_dl(const float* data, const int64_t* off, const int* ldp, const int64_t* ldp_pre,
                                    const int* bsz, const int* boff, const Tile* tiles, int naux, int nbf,
                                    const float* g, int ng, float* out, int ldo);
+
+// FP64 exchange on padded operands (kernels_f64dl.hip)
+__global__ void k_pad_f64_slab(const double* src, int N, const int* ncol, const int* bsz, int naux,
+                               const int* ldp, const int64_t* off, double* out);
+__global__ void k_gather_ct_f64(const double* C, int ldc, int ng, const int* ncol, const int* ldp,
+                                const int64_t* ldp_pre, const int* colmap, const int* colmap_off, double* g);
+template <int WM, int WN, int TM, int TN, int S>
+__global__ void k_slab_gemm_f64_dl(const double* data, const int64_t* off, const int* ldp, const int64_t* ldp_pre,
+                                   const int* bsz, const int* boff, const Tile* tiles, int naux, int nbf,
+                                   const double* g, int ng, double* out, int ldo);
+template <int S>
+__global__ void k_seg_syrk_f64_dl(const double* W, int nseg, int64_t sstride, int ldw, int n, const int2* tiles,
+                                  int ntile, int segs_per_chunk, double* partial);
 template <int S>
 __global__ void k_seg_syrk_f32_dl(const float* W, int nseg, int64_t sstride, int ldw, int n, const int2* tiles,
                                   int ntile, int segs_per_chunk, double* partial);

@@ -78,35 +78,6 @@ __device__ __forceinline__ void f32_step_swz(const uint4* A, const uint4* B, int
     }
 }
 
-// Loads of one stage: wave-loads j = wave + NW*u over [A rows/16][B rows/16];
-// a wave-load brings 16 rows x 64 bytes.  When NW does not divide the count,
-// the first waves issue one load more than the others.
-template <int BM, int BN, int NW>
-__device__ __forceinline__ void issue_stage_f32(uint4* st, int wave, int lane, const uint4* A, const uint4* B,
-                                                int64_t a0, int64_t alast, int b0, int blast, int lda4, int ldb4,
-                                                int kc0)
-{
-  constexpr int QA = BM / 16, QB = BN / 16, NJ = QA + QB;
-  constexpr int UMAX = (NJ + NW - 1) / NW;
-  const int rr = lane >> 2, pos = lane & 3;
-#pragma unroll
-  for (int u = 0; u < UMAX; ++u) {
-    const int j = wave + NW * u;
-    if (j >= NJ) break;
-    if (j < QA) {
-      const int r = 16 * j + rr;
-      const int64_t gr = a0 + r < alast ? a0 + r : alast;
-      lds_load16(A + gr * lda4 + kc0 + swz(r, pos), st + j * 64);
-    }
-    else {
-      const int q = j - QA;
-      const int r = 16 * q + rr;
-      const int64_t gr = b0 + r < blast ? b0 + r : blast;
-      lds_load16(B + gr * ldb4 + kc0 + swz(r, pos), st + BM * 4 + q * 64);
-    }
-  }
-}
-
 // Grouped slab GEMM in FP32: W rows (X, boff[N] + nu) for the rows X*bsz + nu
 // of slab N, columns [0, ldo), zero in [ng, ldo).  Tile table as for the BF16
 // kernel (m0 steps of BM, padded to a multiple of 8 with slab -1).  FP32
@@ -147,7 +118,7 @@ __global__ __launch_bounds__(64 * WM * WN) void k_slab_gemm_f32_dl(
   dl_pipeline<S, LPW>(
       nsteps,
       [&](int s, auto k) {
-        issue_stage_f32<BM, BN, NW>(pick_stage<S>(k, lds0, lds1, lds2, lds3), wave, lane, A, G, t.m0, alast, t.n0,
+        issue_rows<BM, BN, NW>(pick_stage<S>(k, lds0, lds1, lds2, lds3), wave, lane, A, G, t.m0, alast, t.n0,
                                     blast, ld4, ld4, s * 4);
       },
       [&](int, auto k) {
@@ -217,7 +188,7 @@ __global__ __launch_bounds__(512) void k_seg_syrk_f32_dl(const float* __restrict
         const int sg = s0 + q / ksteps;
         const int kc0 = (q % ksteps) * 4;
         const uint4* base = W4 + (int64_t)sg * (sstride >> 2);
-        issue_stage_f32<BM, BM, NW>(pick_stage<S>(k, lds0, lds1, lds2, lds3), wave, lane, base, base, m0, n - 1, n0,
+        issue_rows<BM, BM, NW>(pick_stage<S>(k, lds0, lds1, lds2, lds3), wave, lane, base, base, m0, n - 1, n0,
                                     n - 1, ld4, ld4, kc0);
       },
       [&](int q, auto k) {

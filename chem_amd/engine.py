@@ -252,6 +252,33 @@ class DeviceB:
         _lib.check(_lib.lib().mchip_dfk_f32_shapes(ctypes.byref(ng), ctypes.byref(ns)))
         return ng.value, ns.value
 
+    def f64_plan(self, ng):
+        """The autotuned FP64 exchange pieces for ng occupied columns: slab-GEMM
+        shape of the FP64 L^-1 B copy route and SYRK variant (None if neither is
+        tuned yet; -1 for the one not yet tuned)."""
+        plan = np.zeros(8, dtype=np.int32)
+        ms = np.zeros(32, dtype=np.float32)
+        nms = ctypes.c_int()
+        _lib.check(_lib.lib().mchip_dfk_f64_plan(self.h, ng, _lib.iptr(plan), _lib.fptr(ms), ctypes.byref(nms)))
+        if not plan[7]:
+            return None
+        k = nms.value
+        ns = self.f64_shapes()[1]
+        return dict(gemm_shape=int(plan[0]), bm=int(plan[1]), bn=int(plan[2]), threads=int(plan[3]),
+                    stages=int(plan[5]), syrk_variant=int(plan[6]),
+                    gemm_ms=[float(v) for v in ms[:k - ns]], syrk_ms=[float(v) for v in ms[k - ns:k]])
+
+    def f64_force(self, ng, gemm_shape, syrk_variant):
+        """Use the given FP64 slab-GEMM shape and SYRK variant for ng columns."""
+        _lib.check(_lib.lib().mchip_dfk_f64_force(self.h, ng, gemm_shape, syrk_variant))
+
+    @staticmethod
+    def f64_shapes():
+        """Number of FP64 slab-GEMM shapes and of FP64 SYRK variants."""
+        ng, ns = ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.lib().mchip_dfk_f64_shapes(ctypes.byref(ng), ctypes.byref(ns)))
+        return ng.value, ns.value
+
     def bf16_force(self, ng, gemm_shape, syrk_variant):
         """Use the given BF16 pipeline for ng columns instead of the autotuner's."""
         _lib.check(_lib.lib().mchip_dfk_bf16_force(self.h, ng, gemm_shape, syrk_variant))

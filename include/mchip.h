@@ -235,6 +235,15 @@ int mchip_dfk_f32_plan(mchip_dbt t, int ng, int* plan, float* ms, int* nms);
  * number of GEMM shapes and SYRK variants.  No counterpart in the reference. */
 int mchip_dfk_f32_force(mchip_dbt t, int ng, int gemm_shape, int syrk_variant);
 int mchip_dfk_f32_shapes(int* ngemm, int* nsyrk);
+/* The autotuned FP64 exchange pieces for ng occupied columns: the slab GEMM
+ * on the padded FP64 copy of L^-1 B (mchip_dfk_mo_tilde64) and the FP64 SYRK
+ * every FP64 exchange ends with; same fields as mchip_dfk_f32_plan, with
+ * plan[0] (GEMM shape) and plan[6] (SYRK variant, 0-2: 2-4 stages) -1 until
+ * tuned.  force fixes both; shapes gives their counts.  No counterpart in the
+ * reference. */
+int mchip_dfk_f64_plan(mchip_dbt t, int ng, int* plan, float* ms, int* nms);
+int mchip_dfk_f64_force(mchip_dbt t, int ng, int gemm_shape, int syrk_variant);
+int mchip_dfk_f64_shapes(int* ngemm, int* nsyrk);
 /* Fix the BF16 pipeline for ng columns instead of tuning it (direct-to-LDS GEMM
  * shapes pair with SYRK variants 1-4, the others with 0); shapes: count of GEMM
  * shapes and SYRK variants and, per GEMM shape, whether it is direct-to-LDS

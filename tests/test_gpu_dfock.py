@@ -218,6 +218,68 @@ def test_f32_every_pipeline(ctx, nocc):
     assert len(plan["gemm_ms"]) == ngemm and len(plan["syrk_ms"]) == nsyrk
 
 
+@pytest.mark.parametrize("nocc", [15, 70, 333])
+def test_f64_every_pipeline(ctx, nocc):
+    """Every FP64 slab-GEMM shape on the padded FP64 copy of L^-1 B with every
+    FP64 SYRK variant (direct-to-LDS, 2-4 stages), on a block-sparse B with
+    ragged blocks, an empty slab, several 128-row SYRK tiles and partial row /
+    column tiles: all give bitwise the same exchange, equal to the plain DFMO
+    exchange (L^-1 applied per build, same SYRK) to 1e-12 relative, also in
+    occupied batches whose width is not a multiple of the 8-column step."""
+    from chem_amd.engine import DeviceB, Metric
+    rng = np.random.default_rng(900 + nocc)
+    bsz = [14, 5, 5, 3, 14, 1, 7, 9, 30, 40, 50]
+    xsz = [23, 70, 9, 40, 17]
+    nbf, naux = sum(bsz), sum(xsz)
+    mask = (rng.random((len(xsz), len(bsz), len(bsz))) < 0.6).astype(np.uint8)
+    mask = np.maximum(mask, mask.transpose(0, 2, 1))
+    mask[:, 5, :] = 0
+    mask[:, :, 5] = 0
+    boff = np.concatenate([[0], np.cumsum(bsz)])
+    xoff = np.concatenate([[0], np.cumsum(xsz)])
+    dense = rng.standard_normal((naux, nbf, nbf))
+    dense = 0.5 * (dense + dense.transpose(0, 2, 1))
+    for x in range(len(xsz)):
+        for a in range(len(bsz)):
+            for b in range(len(bsz)):
+                if not mask[x, a, b]:
+                    dense[xoff[x]:xoff[x + 1], boff[a]:boff[a + 1], boff[b]:boff[b + 1]] = 0
+    Bh = ints.BTensorHost.from_dense(dense, xsz, bsz, mask)
+    A = rng.standard_normal((naux, naux))
+    V = A @ A.T + naux * np.eye(naux)
+    C = rng.standard_normal((nbf, nocc))
+    dB = DeviceB(ctx, Bh)
+    m = Metric(ctx, V)
+    Kplain = dB.dfk_mo(C, m)
+    Kplain3 = dB.dfk_mo(C, m, occ_nbatches=3)
+    Linv, _ = m.get()
+    Kref = dfock.dfmo_k(dense, Linv.T, C)
+    assert _relerr(Kplain, Kref) < _jtol(V)
+    assert np.abs(Kplain3 - Kplain).max() < 1e-12 * np.abs(Kplain).max()
+    dB.make_lowp(m, f32=False, bf16=False, f64=True)
+    ngemm, nsyrk = dB.f64_shapes()
+    Kfirst = None
+    for shape in range(ngemm):
+        for v in range(nsyrk):
+            print("case: nocc %d FP64 gemm shape %d syrk %d" % (nocc, shape, v), flush=True)
+            dB.f64_force(nocc, shape, v)
+            K = dB.dfk_mo_tilde64(C)
+            assert np.array_equal(K, K.T)
+            if Kfirst is None:
+                Kfirst = K
+                rel = np.abs(K - Kplain).max() / np.abs(Kplain).max()
+                print("  rel vs plain %.3e" % rel)
+                assert rel < 1e-12
+            assert np.array_equal(K, Kfirst)
+    dB2 = DeviceB(ctx, Bh)
+    dB2.make_lowp(m, f32=False, bf16=False, f64=True)
+    assert np.array_equal(dB2.dfk_mo_tilde64(C), Kfirst)
+    plan = dB2.f64_plan(nocc)
+    assert plan is not None and 0 <= plan["gemm_shape"] < ngemm and 0 <= plan["syrk_variant"] < nsyrk
+    K2 = dB2.dfk_mo_tilde64(C, occ_nbatches=2)
+    assert np.abs(K2 - Kfirst).max() < 1e-12 * np.abs(Kfirst).max()
+
+
 # BF16x3 carries every operand with 16 significant bits (hi + lo, 2^-17
 # relative rounding) and accumulates in FP32 over chunks of <= 256 aux
 # functions: K agrees with the FP64 exchange on the same B~ to a few 1e-6
