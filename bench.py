@@ -384,6 +384,9 @@ def main():
         cpu = cpu_baseline(mol, dB, Linv, C_occ)
         log("cpu baseline: %.2f s/iter (%s)" % (cpu["value"], cpu["sample"]))
 
+    # the plain exchange's two naux x nbf x nocc work arrays are not needed from
+    # here on; at 64 waters B and its three copies leave no room for them
+    ctx.trim()
     # chosen form: low-precision copies of B~ = L^-1 B made once from B, which
     # stays FP64 for J, the final exchange and the energy (reference formulas)
     form = args.form

@@ -135,6 +135,7 @@ _SIGS = [
     ("mchip_probe_lds_load", c_int, [ctypes.POINTER(ctypes.c_uint32), P_int, ctypes.POINTER(ctypes.c_uint32)]),
     ("mchip_ctx_set_timing", c_int, [c_void_p, c_int]),
     ("mchip_ctx_set_escalation", c_int, [c_void_p, c_double, c_int]),
+    ("mchip_ctx_trim", c_int, [c_void_p]),
     ("mchip_dbt_make_f32", c_int, [c_void_p]),
     ("mchip_dfk_mo_f32_dev", c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     ("mchip_dfk_mo_f32", c_int, [c_void_p, P_double, c_int, c_int, P_double]),

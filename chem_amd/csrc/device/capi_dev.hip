@@ -93,6 +93,17 @@ extern "C" int mchip_ctx_set_escalation(mchip_ctx c, double rms, int stall)
   MCHIP_CATCH
 }
 
+extern "C" int mchip_ctx_trim(mchip_ctx c)
+{
+  MCHIP_TRY
+  HIP_CHECK(hipStreamSynchronize(c->c.stream));
+  c->c.scratch_a.release();
+  c->c.scratch_b.release();
+  c->c.scratch_c.release();
+  c->c.scratch_partial.release();
+  MCHIP_CATCH
+}
+
 extern "C" int mchip_ctx_set_timing(mchip_ctx c, int on)
 {
   MCHIP_TRY

@@ -44,6 +44,10 @@ class Context:
         <= rms (0: off) or after `stall` iterations without progress (0: off)."""
         _lib.check(_lib.lib().mchip_ctx_set_escalation(self.h, float(rms), int(stall)))
 
+    def trim(self):
+        """Release the work arrays kept for reuse by earlier builds."""
+        _lib.check(_lib.lib().mchip_ctx_trim(self.h))
+
     def phase_times(self):
         """Accumulated seconds: K1, K2, K3, J, K_inc, split, -, -."""
         out = np.zeros(8)

@@ -131,6 +131,11 @@ int mchip_ctx_set_timing(mchip_ctx c, int on);
  * off; default 3).  FP32 gives way to FP64 after such a stall below 1e-6.  No
  * counterpart in the reference. */
 int mchip_ctx_set_escalation(mchip_ctx c, double rms, int stall);
+/* Release the context's work arrays (they grow to the largest exchange run on
+ * the context and are kept for reuse; the plain FP64 exchange holds two of
+ * naux * nbf * nocc doubles).  The next build allocates what it needs again.
+ * No counterpart in the reference. */
+int mchip_ctx_trim(mchip_ctx c);
 int mchip_ctx_phase_times(mchip_ctx c, double* out, int n);
 
 /* Copy a host B to HBM (FP64).  dbcsr::btensor decompress_init/decompress

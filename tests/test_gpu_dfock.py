@@ -278,6 +278,10 @@ def test_f64_every_pipeline(ctx, nocc):
     assert plan is not None and 0 <= plan["gemm_shape"] < ngemm and 0 <= plan["syrk_variant"] < nsyrk
     K2 = dB2.dfk_mo_tilde64(C, occ_nbatches=2)
     assert np.abs(K2 - Kfirst).max() < 1e-12 * np.abs(Kfirst).max()
+    # released work arrays are allocated again by the next build
+    ctx.trim()
+    assert np.array_equal(dB2.dfk_mo_tilde64(C), Kfirst)
+    assert np.array_equal(dB.dfk_mo(C, m), Kplain)
 
 
 # BF16x3 carries every operand with 16 significant bits (hi + lo, 2^-17
