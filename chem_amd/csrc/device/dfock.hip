@@ -603,6 +603,7 @@ static const Tile* bf16_tiles(Ctx& c, DevBT& t, int BM, int BN, int ng, int* nti
 // staged loads, NB LDS buffers, FP32 W; DL = 1: direct-to-LDS loads, NB
 // pipeline stages, W written as bf16 hi/lo; DL = 2: as 1 with the B~ rows in
 // an NB-stage pipeline of their own and the gathered coefficients in two.
+// The list keeps the shapes that were fastest for some width on MI355X.
 struct Bf16Shape {
   int WM, WN, TM, TN, NB, DL;
   int bm() const { return 16 * WM * TM; }
@@ -610,10 +611,9 @@ struct Bf16Shape {
   int threads() const { return 64 * WM * WN; }
 };
 static const Bf16Shape kBf16Shapes[] = {
-    {2, 2, 4, 2, 2, 0}, {2, 2, 4, 4, 2, 0}, {2, 2, 4, 5, 2, 0}, {2, 4, 4, 2, 2, 0}, {2, 4, 4, 3, 2, 0},
-    {2, 4, 4, 4, 2, 0}, {2, 4, 4, 5, 2, 0}, {2, 4, 4, 2, 1, 0}, {2, 4, 4, 3, 1, 0}, {2, 4, 4, 2, 4, 1},
-    {2, 4, 4, 3, 3, 1}, {2, 4, 4, 5, 2, 1}, {1, 4, 4, 5, 3, 1}, {2, 2, 8, 5, 3, 1}, {2, 2, 8, 5, 2, 1},
-    {2, 4, 4, 5, 4, 2}, {2, 4, 4, 5, 3, 2}, {2, 4, 4, 3, 4, 2}, {2, 2, 8, 5, 3, 2}};
+    {2, 2, 4, 2, 2, 0}, {2, 4, 4, 2, 2, 0}, {2, 4, 4, 2, 4, 1}, {2, 4, 4, 3, 3, 1}, {2, 4, 4, 5, 2, 1},
+    {2, 2, 8, 5, 3, 1}, {2, 2, 4, 5, 2, 1}, {2, 4, 4, 5, 4, 2}, {2, 4, 4, 5, 3, 2}, {2, 4, 4, 3, 4, 2},
+    {2, 2, 4, 5, 2, 2}};
 static constexpr int kNumBf16Shapes = sizeof(kBf16Shapes) / sizeof(kBf16Shapes[0]);
 // SYRK variants: 0 = register-staged on FP32 W; 1..3 = direct-to-LDS on bf16
 // hi/lo W with 2, 3, 4 stages
@@ -643,24 +643,16 @@ static void slab_gemm_bf16(Ctx& c, DevBT& t, int shape, const uint16_t* ghi, con
                      t.L.naux, t.L.nbf, ghi, glo, ng, Whi, Wlo, ldw)
   switch (shape) {
     case 0: MCHIP_BF16_SLAB(2, 2, 4, 2, 2); break;
-    case 1: MCHIP_BF16_SLAB(2, 2, 4, 4, 2); break;
-    case 2: MCHIP_BF16_SLAB(2, 2, 4, 5, 2); break;
-    case 3: MCHIP_BF16_SLAB(2, 4, 4, 2, 2); break;
-    case 4: MCHIP_BF16_SLAB(2, 4, 4, 3, 2); break;
-    case 5: MCHIP_BF16_SLAB(2, 4, 4, 4, 2); break;
-    case 6: MCHIP_BF16_SLAB(2, 4, 4, 5, 2); break;
-    case 7: MCHIP_BF16_SLAB(2, 4, 4, 2, 1); break;
-    case 8: MCHIP_BF16_SLAB(2, 4, 4, 3, 1); break;
-    case 9: MCHIP_BF16_SLAB_DL(2, 4, 4, 2, 4); break;
-    case 10: MCHIP_BF16_SLAB_DL(2, 4, 4, 3, 3); break;
-    case 11: MCHIP_BF16_SLAB_DL(2, 4, 4, 5, 2); break;
-    case 12: MCHIP_BF16_SLAB_DL(1, 4, 4, 5, 3); break;
-    case 13: MCHIP_BF16_SLAB_DL(2, 2, 8, 5, 3); break;
-    case 14: MCHIP_BF16_SLAB_DL(2, 2, 8, 5, 2); break;
-    case 15: MCHIP_BF16_SLAB_DLA(2, 4, 4, 5, 4); break;
-    case 16: MCHIP_BF16_SLAB_DLA(2, 4, 4, 5, 3); break;
-    case 17: MCHIP_BF16_SLAB_DLA(2, 4, 4, 3, 4); break;
-    case 18: MCHIP_BF16_SLAB_DLA(2, 2, 8, 5, 3); break;
+    case 1: MCHIP_BF16_SLAB(2, 4, 4, 2, 2); break;
+    case 2: MCHIP_BF16_SLAB_DL(2, 4, 4, 2, 4); break;
+    case 3: MCHIP_BF16_SLAB_DL(2, 4, 4, 3, 3); break;
+    case 4: MCHIP_BF16_SLAB_DL(2, 4, 4, 5, 2); break;
+    case 5: MCHIP_BF16_SLAB_DL(2, 2, 8, 5, 3); break;
+    case 6: MCHIP_BF16_SLAB_DL(2, 2, 4, 5, 2); break;
+    case 7: MCHIP_BF16_SLAB_DLA(2, 4, 4, 5, 4); break;
+    case 8: MCHIP_BF16_SLAB_DLA(2, 4, 4, 5, 3); break;
+    case 9: MCHIP_BF16_SLAB_DLA(2, 4, 4, 3, 4); break;
+    case 10: MCHIP_BF16_SLAB_DLA(2, 2, 4, 5, 2); break;
     default: throw std::runtime_error("BF16 slab GEMM: unknown launch shape");
   }
 #undef MCHIP_BF16_SLAB

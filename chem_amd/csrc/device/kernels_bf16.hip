@@ -441,15 +441,79 @@ __device__ __forceinline__ void issue_stage(uint4* st, int wave, int lane, const
   }
 }
 
+// Quad lane exchange: lane i of each group of 4 lanes receives v from lane
+// i ^ T (DPP quad_perm, no LDS traffic).
+template <int T>
+__device__ __forceinline__ uint32_t quad_xor(uint32_t v)
+{
+  constexpr int perm = (0 ^ T) | ((1 ^ T) << 2) | ((2 ^ T) << 4) | ((3 ^ T) << 6);
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, perm, 0xF, 0xF, false);
+}
+
+__device__ __forceinline__ uint32_t sel4(const uint32_t (&v)[4], int k)
+{
+  return k == 0 ? v[0] : k == 1 ? v[1] : k == 2 ? v[2] : v[3];
+}
+
 // W rows (X, boff + nu) for the tile's rows X*bn + nu of the slab, columns
 // [n0, n0 + BN) below ldo, zero at and past ng: FP32 (out) or bf16 hi/lo
 // (whi, wlo).  Accumulator layout of the 16x16 MFMA: D[4*(lane/16) + e][lane%16].
+// bf16 output: the 4 x 4 blocks (rows 4g..4g+3 of lane group g, columns of a
+// lane quad) are transposed across the quad with DPP, hi and lo packed into
+// one 32-bit word, so that each lane stores 4 consecutive columns of one row
+// as one 8-byte store per plane (ldo is a multiple of 32).
 template <int TM, int TN>
 __device__ __forceinline__ void store_w_tile(const f32x4 (&acc)[TM][TN], const Tile& t, int wm0, int wn0, int lane,
                                              int64_t rows, int bn, int nbf, int boffN, int ng, float* out,
                                              uint16_t* whi, uint16_t* wlo, int ldo)
 {
-  const int fc = lane & 15, fr4 = 4 * (lane >> 4);
+  const int fc = lane & 15, g = lane >> 4;
+  if (whi) {
+    const int cq = fc & 3, c4 = fc & ~3;
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int r = t.m0 + wm0 + 16 * i + 4 * g + cq;   // the row this lane stores after the transpose
+      const bool rok = r < rows;
+      const int x = r / bn;
+      const int nu = r - x * bn;
+      const int64_t orow = ((int64_t)x * nbf + boffN + nu) * (int64_t)ldo;
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int col = t.n0 + wn0 + 16 * j + fc;   // the column this lane holds before the transpose
+        uint32_t p[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          uint16_t h, l;
+          split32(col < ng ? acc[i][j][e] : 0.f, h, l);
+          p[e] = (uint32_t)h | ((uint32_t)l << 16);
+        }
+        // o[k] = element (row 4g + cq, column c4 + k): lane cq ^ T holds it in p[cq]
+        uint32_t o[4];
+        o[0] = o[1] = o[2] = o[3] = 0u;
+        const uint32_t r1 = quad_xor<1>(sel4(p, cq ^ 1));
+        const uint32_t r2 = quad_xor<2>(sel4(p, cq ^ 2));
+        const uint32_t r3 = quad_xor<3>(sel4(p, cq ^ 3));
+        const uint32_t own = sel4(p, cq);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int d = k ^ cq;
+          o[k] = d == 0 ? own : d == 1 ? r1 : d == 2 ? r2 : r3;
+        }
+        const int c0 = t.n0 + wn0 + 16 * j + c4;
+        if (rok && c0 < ldo) {
+          uint2 hv, lv;
+          hv.x = (o[0] & 0xFFFFu) | (o[1] << 16);
+          hv.y = (o[2] & 0xFFFFu) | (o[3] << 16);
+          lv.x = (o[0] >> 16) | (o[1] & 0xFFFF0000u);
+          lv.y = (o[2] >> 16) | (o[3] & 0xFFFF0000u);
+          *reinterpret_cast<uint2*>(whi + orow + c0) = hv;
+          *reinterpret_cast<uint2*>(wlo + orow + c0) = lv;
+        }
+      }
+    }
+    return;
+  }
+  const int fr4 = 4 * g;
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -463,16 +527,7 @@ __device__ __forceinline__ void store_w_tile(const f32x4 (&acc)[TM][TN], const T
       for (int j = 0; j < TN; ++j) {
         const int c = t.n0 + wn0 + 16 * j + fc;
         if (c >= ldo) continue;
-        const float v = c < ng ? acc[i][j][e] : 0.f;
-        if (whi) {
-          uint16_t h, l;
-          split32(v, h, l);
-          whi[orow + c] = h;
-          wlo[orow + c] = l;
-        }
-        else {
-          out[orow + c] = v;
-        }
+        out[orow + c] = c < ng ? acc[i][j][e] : 0.f;
       }
     }
 }
@@ -481,8 +536,16 @@ __device__ __forceinline__ void store_w_tile(const f32x4 (&acc)[TM][TN], const T
 // as k_slab_gemm_bf16x3 (m0 steps of BM).  W columns in [ng, ldo) are
 // written as zero.  split_out: W is written as bf16 hi/lo (whi, wlo, same
 // layout) instead of FP32 (out).
+// Four-wave shapes of at most 20 MFMA tiles per wave fit two workgroups per
+// CU (LDS and registers): ask the compiler for two waves per SIMD.
+template <int WM, int WN, int TM, int TN>
+constexpr int dl_min_blocks()
+{
+  return (WM * WN == 4 && TM * TN <= 20) ? 2 : 1;
+}
+
 template <int WM, int WN, int TM, int TN, int S>
-__global__ __launch_bounds__(64 * WM * WN) void k_slab_gemm_bf16x3_dl(
+__global__ __launch_bounds__(64 * WM * WN, (dl_min_blocks<WM, WN, TM, TN>())) void k_slab_gemm_bf16x3_dl(
     const uint16_t* __restrict__ bhi, const uint16_t* __restrict__ blo, const int64_t* __restrict__ off16,
     const int* __restrict__ ldp, const int64_t* __restrict__ ldp_pre, const int* __restrict__ bsz,
     const int* __restrict__ boff, const Tile* __restrict__ tiles, int naux, int nbf,
@@ -537,7 +600,7 @@ __global__ __launch_bounds__(64 * WM * WN) void k_slab_gemm_bf16x3_dl(
 // whole cycle of stages, so a step waits only for its own stage's loads.
 // Same operands, tile table and output as k_slab_gemm_bf16x3_dl (bf16 hi/lo W).
 template <int WM, int WN, int TM, int TN, int SA>
-__global__ __launch_bounds__(64 * WM * WN) void k_slab_gemm_bf16x3_dla(
+__global__ __launch_bounds__(64 * WM * WN, (dl_min_blocks<WM, WN, TM, TN>())) void k_slab_gemm_bf16x3_dla(
     const uint16_t* __restrict__ bhi, const uint16_t* __restrict__ blo, const int64_t* __restrict__ off16,
     const int* __restrict__ ldp, const int64_t* __restrict__ ldp_pre, const int* __restrict__ bsz,
     const int* __restrict__ boff, const Tile* __restrict__ tiles, int naux, int nbf,
@@ -731,9 +794,8 @@ __global__ __launch_bounds__(BM * 2) void k_seg_syrk_bf16x3_dl(
 MCHIP_SLAB16DL_INST(2, 4, 4, 2, 4)
 MCHIP_SLAB16DL_INST(2, 4, 4, 3, 3)
 MCHIP_SLAB16DL_INST(2, 4, 4, 5, 2)
-MCHIP_SLAB16DL_INST(1, 4, 4, 5, 3)
 MCHIP_SLAB16DL_INST(2, 2, 8, 5, 3)
-MCHIP_SLAB16DL_INST(2, 2, 8, 5, 2)
+MCHIP_SLAB16DL_INST(2, 2, 4, 5, 2)
 #undef MCHIP_SLAB16DL_INST
 #define MCHIP_SLAB16DLA_INST(WM, WN, TM, TN, SA)                                                       \
   template __global__ void k_slab_gemm_bf16x3_dla<WM, WN, TM, TN, SA>(                                \
@@ -742,7 +804,7 @@ MCHIP_SLAB16DL_INST(2, 2, 8, 5, 2)
 MCHIP_SLAB16DLA_INST(2, 4, 4, 5, 4)
 MCHIP_SLAB16DLA_INST(2, 4, 4, 5, 3)
 MCHIP_SLAB16DLA_INST(2, 4, 4, 3, 4)
-MCHIP_SLAB16DLA_INST(2, 2, 8, 5, 3)
+MCHIP_SLAB16DLA_INST(2, 2, 4, 5, 2)
 #undef MCHIP_SLAB16DLA_INST
 #define MCHIP_SYRK16DL_INST(S, BM)                                                                     \
   template __global__ void k_seg_syrk_bf16x3_dl<S, BM>(const uint16_t*, const uint16_t*, int, int64_t, int, \
@@ -758,14 +820,7 @@ MCHIP_SYRK16DL_INST(2, 256)
       const uint16_t*, const uint16_t*, const int64_t*, const int*, const int64_t*, const int*,         \
       const int*, const Tile*, int, int, const uint16_t*, const uint16_t*, int, float*, int);
 MCHIP_SLAB16_INST(2, 2, 4, 2, 2)
-MCHIP_SLAB16_INST(2, 2, 4, 4, 2)
-MCHIP_SLAB16_INST(2, 2, 4, 5, 2)
 MCHIP_SLAB16_INST(2, 4, 4, 2, 2)
-MCHIP_SLAB16_INST(2, 4, 4, 3, 2)
-MCHIP_SLAB16_INST(2, 4, 4, 4, 2)
-MCHIP_SLAB16_INST(2, 4, 4, 5, 2)
-MCHIP_SLAB16_INST(2, 4, 4, 2, 1)
-MCHIP_SLAB16_INST(2, 4, 4, 3, 1)
 #undef MCHIP_SLAB16_INST
 
 }  // namespace mchip
