@@ -303,7 +303,7 @@ def main():
         sys.exit(2)
 
     from chem_amd import desc, ints
-    from chem_amd.engine import Context, DeviceB, Metric
+    from chem_amd.engine import Context, DeviceArray, DeviceB, Metric
     from chem_amd.scf import run_rhf
 
     if args.kernels_only:
@@ -375,7 +375,12 @@ def main():
         ctx.sync()
         return (time.perf_counter() - t) / steps
 
-    plain_s = time_builds(lambda: dB.dfk_mo(C_occ, met), 2, 1)
+    # plain FP64 build with the occupied coefficients and K resident in HBM
+    C0d = DeviceArray(ctx, C_occ.shape, C_occ)
+    K0d = DeviceArray(ctx, (mol.nbf, mol.nbf))
+    plain_s = time_builds(lambda: dB.dfk_dev("fp64", C0d, mol.nocc, K0d, metric=met), 2, 1)
+    C0d.release()
+    K0d.release()
 
     K_plain = dB.dfk_mo(C_occ, met)
     cpu = None
@@ -434,31 +439,38 @@ def main():
     nb = [stage_of.count(k) for k in range(3)]
     n16 = nb[0]
     Cc = np.ascontiguousarray(chosen.C[:, :mol.nocc])
-    k16 = dB.dfk_mo_bf16 if form in ("auto", "bf16") else None
-    k32 = dB.dfk_mo_f32 if form in ("auto", "f32") else None
-    k64 = dB.dfk_mo_tilde64
+    # the occupied coefficients and K stay in HBM for the timed builds (as in
+    # the library's SCF loop): no host copies inside the timed region
+    Cd = DeviceArray(ctx, Cc.shape, Cc)
+    Kd = DeviceArray(ctx, (mol.nbf, mol.nbf))
+
+    def dev_build(kind):
+        return lambda: dB.dfk_dev(kind, Cd, mol.nocc, Kd)
+    k16 = dev_build("bf16x3") if form in ("auto", "bf16") else None
+    k32 = dev_build("f32") if form in ("auto", "f32") else None
+    k64 = dev_build("fp64_copy")
     for _ in range(args.warmup):
         if k16:
-            k16(Cc)
+            k16()
         if k32:
-            k32(Cc)
+            k32()
     ctx.sync()
     t = time.perf_counter()
     for s_ in range(steps):
-        (k16, k32, k64)[stage_of[s_]](Cc)
-    dB.dfk_mo_tilde64(Cc)   # the final FP64 exchange for the reported energy
+        (k16, k32, k64)[stage_of[s_]]()
+    k64()   # the final FP64 exchange for the reported energy
     ctx.sync()
     step_s = (time.perf_counter() - t) / steps
-    final_s = time_builds(lambda: dB.dfk_mo_tilde64(Cc), 2, 0)
-    t16 = time_builds(lambda: k16(Cc), 3, 0) if k16 else None
-    t32 = time_builds(lambda: k32(Cc), 3, 0) if k32 else None
+    final_s = time_builds(k64, 2, 0)
+    t16 = time_builds(k16, 3, 0) if k16 else None
+    t32 = time_builds(k32, 3, 0) if k32 else None
 
     # roofline of the dominant kernel from HIP events on the library stream:
     # K1 (the B~ C slab GEMM) of the form that runs most of the builds
     def phases(fn, nrep=3):
         ctx.set_timing(True)
         for _ in range(nrep):
-            fn(Cc)
+            fn()
         ph = ctx.phase_times() / nrep
         ctx.set_timing(False)
         return ph
@@ -502,7 +514,7 @@ def main():
 
     # accuracy of each low-precision K at the FP64 orbitals, and of the orbital energies
     kerr = {}
-    for name, fn in (("bf16x3", k16), ("f32", k32)):
+    for name, fn in (("bf16x3", dB.dfk_mo_bf16 if k16 else None), ("f32", dB.dfk_mo_f32 if k32 else None)):
         if fn is None:
             continue
         dK = fn(C_occ) - K_plain

@@ -136,6 +136,10 @@ _SIGS = [
     ("mchip_ctx_set_timing", c_int, [c_void_p, c_int]),
     ("mchip_ctx_set_escalation", c_int, [c_void_p, c_double, c_int]),
     ("mchip_ctx_trim", c_int, [c_void_p]),
+    ("mchip_dev_alloc", c_int, [c_void_p, ctypes.c_size_t, ctypes.POINTER(c_void_p)]),
+    ("mchip_dev_free", c_int, [c_void_p, c_void_p]),
+    ("mchip_dev_copy_in", c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_size_t]),
+    ("mchip_dev_copy_out", c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_size_t]),
     ("mchip_dbt_make_f32", c_int, [c_void_p]),
     ("mchip_dfk_mo_f32_dev", c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     ("mchip_dfk_mo_f32", c_int, [c_void_p, P_double, c_int, c_int, P_double]),
@@ -143,6 +147,7 @@ _SIGS = [
     ("mchip_dbt_make_lowp", c_int, [c_void_p, c_void_p, c_int, c_int, c_int]),
     ("mchip_dfk_mo_tilde64", c_int, [c_void_p, P_double, c_int, c_int, P_double]),
     ("mchip_dfk_mo_bf16_dev", c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
+    ("mchip_dfk_mo_tilde64_dev", c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     ("mchip_dfk_mo_bf16", c_int, [c_void_p, P_double, c_int, c_int, P_double]),
     ("mchip_dfk_bf16_tile", c_int, [c_void_p, c_int, P_int]),
     ("mchip_dfk_bf16_plan", c_int, [c_void_p, c_int, P_int, P_float, P_int]),

@@ -93,6 +93,41 @@ extern "C" int mchip_ctx_set_escalation(mchip_ctx c, double rms, int stall)
   MCHIP_CATCH
 }
 
+extern "C" int mchip_dev_alloc(mchip_ctx c, size_t bytes, void** p)
+{
+  MCHIP_TRY
+  need_device(c);
+  if (!p) throw std::runtime_error("mchip_dev_alloc: null output pointer");
+  *p = nullptr;
+  if (bytes) HIP_CHECK(hipMalloc(p, bytes));
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dev_free(mchip_ctx c, void* p)
+{
+  MCHIP_TRY
+  need_device(c);
+  HIP_CHECK(hipStreamSynchronize(c->c.stream));
+  if (p) HIP_CHECK(hipFree(p));
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dev_copy_in(mchip_ctx c, void* dst, const void* src, size_t bytes)
+{
+  MCHIP_TRY
+  need_device(c);
+  if (bytes) h2d(dst, src, bytes, c->c.stream);
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dev_copy_out(mchip_ctx c, void* dst, const void* src, size_t bytes)
+{
+  MCHIP_TRY
+  need_device(c);
+  if (bytes) d2h(dst, src, bytes, c->c.stream);
+  MCHIP_CATCH
+}
+
 extern "C" int mchip_ctx_trim(mchip_ctx c)
 {
   MCHIP_TRY
@@ -381,6 +416,16 @@ extern "C" int mchip_dfk_mo_tilde64(mchip_dbt t, const double* C, int nocc, int 
   if (nocc) h2d(dC.p, C, (size_t)n * nocc * sizeof(double), c.stream);
   dfk_mo(c, t->t, nullptr, dC.as<double>(), nocc, nocc, dK.as<double>(), occ_nbatches, t->t.data64t.as<double>());
   d2h(K, dK.p, (size_t)n * n * sizeof(double), c.stream);
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dfk_mo_tilde64_dev(mchip_dbt t, const double* C_dev, int ldc, int nocc, int occ_nbatches,
+                                        double* K_dev)
+{
+  MCHIP_TRY
+  need_device(t->ctx);
+  if (!t->t.data64t.p) throw std::runtime_error("mchip_dfk_mo_tilde64_dev: no FP64 L^-1 B copy (mchip_dbt_make_lowp f64)");
+  dfk_mo(t->ctx->c, t->t, nullptr, C_dev, ldc, nocc, K_dev, occ_nbatches, t->t.data64t.as<double>());
   MCHIP_CATCH
 }
 

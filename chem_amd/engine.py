@@ -74,6 +74,45 @@ class Context:
             pass
 
 
+class DeviceArray:
+    """A float64 array in the context's GPU memory (for the _dev exchange entry
+    points, which keep the occupied coefficients and K resident)."""
+
+    def __init__(self, ctx, shape, host=None):
+        self.ctx = ctx
+        self.shape = tuple(shape)
+        self.nbytes = int(np.prod(self.shape)) * 8
+        p = ctypes.c_void_p()
+        _lib.check(_lib.lib().mchip_dev_alloc(ctx.h, self.nbytes, ctypes.byref(p)))
+        self.ptr = p
+        if host is not None:
+            self.upload(host)
+
+    def upload(self, host):
+        a = np.ascontiguousarray(host, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError("DeviceArray.upload: shape %s, expected %s" % (a.shape, self.shape))
+        _lib.check(_lib.lib().mchip_dev_copy_in(self.ctx.h, self.ptr, a.ctypes.data_as(ctypes.c_void_p),
+                                                self.nbytes))
+
+    def download(self):
+        a = np.empty(self.shape)
+        _lib.check(_lib.lib().mchip_dev_copy_out(self.ctx.h, a.ctypes.data_as(ctypes.c_void_p), self.ptr,
+                                                 self.nbytes))
+        return a
+
+    def release(self):
+        if getattr(self, "ptr", None) is not None and self.ptr.value:
+            _lib.lib().mchip_dev_free(self.ctx.h, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
 class Metric:
     """V = L L^T, L^-1 and V^-1 resident on the GPU."""
 
@@ -197,6 +236,22 @@ class DeviceB:
         _lib.check(_lib.lib().mchip_dfk_mo_bf16(self.h, _lib.dptr(C), C.shape[1], occ_nbatches,
                                                 _lib.dptr(K)))
         return K
+
+    def dfk_dev(self, form, C_dev, nocc, K_dev, metric=None, occ_nbatches=1):
+        """K into the DeviceArray K_dev (nbf x nbf) from the DeviceArray C_dev
+        (nbf x ldc, first nocc columns), without host copies or a
+        synchronisation: form "fp64" (plain, with the metric unless B is
+        transformed), "fp64_copy" (the FP64 copy of L^-1 B), "f32" or "bf16x3"."""
+        ldc = C_dev.shape[1]
+        L = _lib.lib()
+        if form == "fp64":
+            m = None if metric is None else metric.h
+            rc = L.mchip_dfk_mo_dev(self.h, m, C_dev.ptr, ldc, nocc, occ_nbatches, K_dev.ptr)
+        else:
+            fn = {"fp64_copy": L.mchip_dfk_mo_tilde64_dev, "f32": L.mchip_dfk_mo_f32_dev,
+                  "bf16x3": L.mchip_dfk_mo_bf16_dev}[form]
+            rc = fn(self.h, C_dev.ptr, ldc, nocc, occ_nbatches, K_dev.ptr)
+        _lib.check(rc)
 
     def dfk_ao(self, P, metric=None, form="fp64", occ_nbatches=1):
         """Exchange from a density (DFAO_K / DFMEM_K): pivoted Cholesky of P on

@@ -136,6 +136,15 @@ int mchip_ctx_set_escalation(mchip_ctx c, double rms, int stall);
  * naux * nbf * nocc doubles).  The next build allocates what it needs again.
  * No counterpart in the reference. */
 int mchip_ctx_trim(mchip_ctx c);
+/* Device arrays on the context's GPU for callers that keep the occupied
+ * coefficients and K resident between builds (the _dev entry points):
+ * allocate, free (after the context's pending work), and synchronous copies
+ * in and out on the context's stream.  No counterpart in the reference (its
+ * matrices live in DBCSR on the host). */
+int mchip_dev_alloc(mchip_ctx c, size_t bytes, void** p);
+int mchip_dev_free(mchip_ctx c, void* p);
+int mchip_dev_copy_in(mchip_ctx c, void* dst, const void* src, size_t bytes);
+int mchip_dev_copy_out(mchip_ctx c, void* dst, const void* src, size_t bytes);
 int mchip_ctx_phase_times(mchip_ctx c, double* out, int n);
 
 /* Copy a host B to HBM (FP64).  dbcsr::btensor decompress_init/decompress
@@ -204,6 +213,11 @@ int mchip_dbt_make_bf16(mchip_dbt t);
  * products (lo*hi + hi*lo + hi*hi), FP32 accumulation within chunks of at most
  * 256 aux functions, FP64 across chunks.  Replaces the per-iteration
  * DFMO_K::compute_K (jkdf.cpp:221-462) in the chosen form. */
+/* The FP64 exchange on the FP64 copy of L^-1 B with device-resident C
+ * (row-major nbf x ldc, first nocc columns) and K (nbf x nbf); no
+ * synchronisation (the context's stream). */
+int mchip_dfk_mo_tilde64_dev(mchip_dbt t, const double* C_dev, int ldc, int nocc, int occ_nbatches,
+                             double* K_dev);
 int mchip_dfk_mo_bf16_dev(mchip_dbt t, const double* C_dev, int ldc, int nocc, int occ_nbatches,
                           double* K_dev);
 int mchip_dfk_mo_bf16(mchip_dbt t, const double* C, int nocc, int occ_nbatches, double* K);

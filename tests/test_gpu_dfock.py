@@ -284,6 +284,32 @@ def test_f64_every_pipeline(ctx, nocc):
     assert np.array_equal(dB.dfk_mo(C, m), Kplain)
 
 
+def test_device_resident_exchange_matches_host(ctx):
+    """The _dev entry points (occupied coefficients and K in device arrays,
+    a row stride larger than the occupied count) give bitwise the host-pointer
+    results for every form."""
+    from chem_amd.engine import DeviceArray, DeviceB, Metric
+    mol, Bh, V = _system(3)
+    C, _ = _random_occ(mol.nbf, mol.nocc, 13)
+    m = Metric(ctx, V)
+    dB = DeviceB(ctx, Bh)
+    dB.make_lowp(m, f32=True, bf16=True, f64=True)
+    ld = mol.nocc + 3
+    Cwide = np.zeros((mol.nbf, ld))
+    Cwide[:, :mol.nocc] = C
+    Cd = DeviceArray(ctx, Cwide.shape, Cwide)
+    Kd = DeviceArray(ctx, (mol.nbf, mol.nbf))
+    host = {"fp64": lambda: dB.dfk_mo(C, m), "fp64_copy": lambda: dB.dfk_mo_tilde64(C),
+            "f32": lambda: dB.dfk_mo_f32(C), "bf16x3": lambda: dB.dfk_mo_bf16(C)}
+    for form, fn in host.items():
+        print("case: device-resident %s" % form, flush=True)
+        dB.dfk_dev(form, Cd, mol.nocc, Kd, metric=m)
+        ctx.sync()
+        assert np.array_equal(Kd.download(), fn()), form
+    Cd.release()
+    Kd.release()
+
+
 # BF16x3 carries every operand with 16 significant bits (hi + lo, 2^-17
 # relative rounding) and accumulates in FP32 over chunks of <= 256 aux
 # functions: K agrees with the FP64 exchange on the same B~ to a few 1e-6
