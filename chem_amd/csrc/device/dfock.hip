@@ -207,7 +207,7 @@ static void seg_syrk_t(Ctx& c, const T* A, const T* B, int nseg, int64_t sstride
                      nseg, sstride, lda, klen, n, tiles.as<int2>(), ntile, per, partial);
   HIP_CHECK(hipGetLastError());
   if (reduce) {
-    hipLaunchKernelGGL(k_reduce_sym, dim3(nblk((int64_t)n * n, 256)), dim3(256), 0, c.stream,
+    hipLaunchKernelGGL(k_reduce_sym<double>, dim3(nblk((int64_t)n * n, 256)), dim3(256), 0, c.stream,
                        partial_buf.as<double>(), nbatch * nch, n, sym ? 0 : 1, alpha, out);
     HIP_CHECK(hipGetLastError());
   }
@@ -447,7 +447,7 @@ void slab_syrk(Ctx& c, DevBT& t, const double* A, const double* B, double* Z, De
   hipLaunchKernelGGL((k_slabseg_gemm<double, SG_BM, SG_BM>), dim3(ntile * nch), dim3(256), 0, c.stream, A, B,
                      d_off.as<int64_t>(), d_len.as<int64_t>(), d_cs.as<int>(), n, d_tl.as<int2>(), ntile,
                      partial.as<double>());
-  hipLaunchKernelGGL(k_reduce_sym, dim3(nblk((int64_t)n * n, 256)), dim3(256), 0, c.stream, partial.as<double>(),
+  hipLaunchKernelGGL(k_reduce_sym<double>, dim3(nblk((int64_t)n * n, 256)), dim3(256), 0, c.stream, partial.as<double>(),
                      nch, n, 0, 1.0, Z);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(c.stream));
@@ -679,7 +679,7 @@ static const int2* syrk_tiles(Ctx& c, DevBT& t, int n, int BM, int* ntile)
 
 // tl/ntile: the 128 x 128 upper tiles; the 256-tile variant uses its own table.
 static void seg_syrk_bf16(Ctx& c, DevBT& t, int variant, const float* W, const uint16_t* Whi, const uint16_t* Wlo,
-                          int naux, int nbf, int ldw, const int2* tl, int ntile, int nch, int spc, double* partial)
+                          int naux, int nbf, int ldw, const int2* tl, int ntile, int nch, int spc, float* partial)
 {
   const dim3 grid((unsigned)(ntile * nch));
   const dim3 grid8((unsigned)((ntile * nch + 7) / 8 * 8));   // direct-to-LDS kernels: XCD mapping
@@ -733,7 +733,7 @@ static float time_launch(Ctx& c, F&& f)
 
 static const Bf16Plan& tune_bf16(Ctx& c, DevBT& t, const uint16_t* ghi, const uint16_t* glo, int ng, float* W,
                                  uint16_t* Whi, uint16_t* Wlo, int ldw, const int2* tl, int ntile, int nch, int spc,
-                                 double* partial)
+                                 float* partial)
 {
   auto it = t.bf16_plan.find(ng);
   if (it != t.bf16_plan.end()) return it->second;
@@ -869,7 +869,8 @@ void dfk_mo_bf16(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K
   int spc = (naux + nch - 1) / nch;
   if (spc > 32) spc = 32;
   nch = (naux + spc - 1) / spc;
-  c.scratch_partial.ensure((size_t)nb * nch * nbf * nbf * sizeof(double));
+  // chunk partials are the FP32 chunk sums, kept exactly in FP32
+  c.scratch_partial.ensure((size_t)nb * nch * nbf * nbf * sizeof(float));
   int gmax = 0;
   for (int n = 0; n < L.nbb; ++n) gmax = std::max(gmax, t.h_ldp[n]);
   for (int b = 0; b < nb; ++b) {
@@ -883,7 +884,7 @@ void dfk_mo_bf16(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K
                            t.colmap.as<int>(), t.colmap_off.as<int>(), ghi, glo);
       HIP_CHECK(hipGetLastError());
     }
-    double* partial = c.scratch_partial.as<double>() + (size_t)b * nch * nbf * nbf;
+    float* partial = c.scratch_partial.as<float>() + (size_t)b * nch * nbf * nbf;
     const Bf16Plan& plan = tune_bf16(c, t, ghi, glo, w, W, Whi, Wlo, ldw, d_tl, ntile, nch, spc, partial);
     {
       PhaseTimer tm(c, PH_K1);
@@ -892,8 +893,8 @@ void dfk_mo_bf16(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K
     PhaseTimer tm(c, PH_K3);
     seg_syrk_bf16(c, t, plan.k3, W, Whi, Wlo, naux, nbf, ldw, d_tl, ntile, nch, spc, partial);
     if (b == nb - 1)
-      hipLaunchKernelGGL(k_reduce_sym, dim3(nblk((int64_t)nbf * nbf, 256)), dim3(256), 0, c.stream,
-                         c.scratch_partial.as<double>(), nb * nch, nbf, 0, -1.0, K);
+      hipLaunchKernelGGL(k_reduce_sym<float>, dim3(nblk((int64_t)nbf * nbf, 256)), dim3(256), 0, c.stream,
+                         c.scratch_partial.as<float>(), nb * nch, nbf, 0, -1.0, K);
     HIP_CHECK(hipGetLastError());
   }
 }
@@ -1109,7 +1110,7 @@ void dfk_mo_f32(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K,
     PhaseTimer tm(c, PH_K3);
     seg_syrk_f32dl(c, plan.k3, W, naux, nbf, ldw, tl, ntile, nch, spc, partial);
     if (b == nb - 1)
-      hipLaunchKernelGGL(k_reduce_sym, dim3(nblk((int64_t)nbf * nbf, 256)), dim3(256), 0, c.stream,
+      hipLaunchKernelGGL(k_reduce_sym<double>, dim3(nblk((int64_t)nbf * nbf, 256)), dim3(256), 0, c.stream,
                          c.scratch_partial.as<double>(), nb * nch, nbf, 0, -1.0, K);
     HIP_CHECK(hipGetLastError());
   }
@@ -1208,7 +1209,7 @@ static void syrk_f64_exchange(Ctx& c, DevBT& t, const double* W, int ng, int ldw
   }
   seg_syrk_f64dl(c, p.k3, W, naux, nbf, ldw, tl, ntile, nch, spc, partial);
   if (b == nb - 1)
-    hipLaunchKernelGGL(k_reduce_sym, dim3(nblk((int64_t)nbf * nbf, 256)), dim3(256), 0, c.stream,
+    hipLaunchKernelGGL(k_reduce_sym<double>, dim3(nblk((int64_t)nbf * nbf, 256)), dim3(256), 0, c.stream,
                        c.scratch_partial.as<double>(), nb * nch, nbf, 0, -1.0, K);
   HIP_CHECK(hipGetLastError());
 }

@@ -37,7 +37,8 @@ __global__ void k_slabseg_gemm(const T* A, const T* B, const int64_t* seg_off, c
                                const int* chunk_seg, int n, const int2* tiles, int ntile,
                                double* partial);
 __global__ void k_zero_upper_cm(double* A, int n);
-__global__ void k_reduce_sym(const double* partial, int nchunk, int n, int sym2, double alpha,
+template <class T>
+__global__ void k_reduce_sym(const T* partial, int nchunk, int n, int sym2, double alpha,
                              double* out);
 __global__ void k_pair_gather(const double* P, int nbf, const int* ncol, const int* bsz,
                               const int* boff, const int* colmap, const int* colmap_off,
@@ -78,7 +79,7 @@ __global__ void k_slab_gemm_bf16x3_dla(const uint16_t* bhi, const uint16_t* blo,
 template <int S, int BM>
 __global__ void k_seg_syrk_bf16x3_dl(const uint16_t* whi, const uint16_t* wlo, int nseg, int64_t sstride,
                                      int ldw, int n, const int2* tiles, int ntile, int segs_per_chunk,
-                                     double* partial);
+                                     float* partial);
 __global__ void k_split_slab(const double* src, int N, const int* ncol, const int* bsz, int naux,
                              const int* ldp, const int64_t* off16, uint16_t* hi, uint16_t* lo);
 __global__ void k_gather_ct(const double* C, int ldc, int ng, const int* ncol, const int* ldp,
@@ -92,7 +93,7 @@ __global__ void k_slab_gemm_bf16x3(const uint16_t* bhi, const uint16_t* blo, con
                                    int ldo);
 template <int BM>
 __global__ void k_seg_syrk_bf16x3(const float* W, int nseg, int64_t sstride, int ldw, int n,
-                                  const int2* tiles, int ntile, int segs_per_chunk, double* partial);
+                                  const int2* tiles, int ntile, int segs_per_chunk, float* partial);
 
 // FP32 exchange on the padded copy of B~ (kernels_f32dl.hip)
 __global__ void k_pad_f32_slab(const double* src, int N, const int* ncol, const int* bsz, int naux,

@@ -279,7 +279,7 @@ __global__ __launch_bounds__(64 * WM * WN, NBUF == 1 ? 4 : 1) void k_slab_gemm_b
 template <int BM>
 __global__ __launch_bounds__(256) void k_seg_syrk_bf16x3(
     const float* __restrict__ W, int nseg, int64_t sstride, int ldw, int n, const int2* __restrict__ tiles,
-    int ntile, int segs_per_chunk, double* __restrict__ partial)
+    int ntile, int segs_per_chunk, float* __restrict__ partial)
 {
   constexpr int TM = BM / 32;
   constexpr int CA = BM * 8 / 256;   // float4 chunks per thread per operand
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(256) void k_seg_syrk_bf16x3(
     if (q + 1 < nsteps) store(cur ^ 1);
     __syncthreads();
   }
-  double* P = partial + (int64_t)chunk * n * n;
+  float* P = partial + (int64_t)chunk * n * n;   // the FP32 chunk sum, exactly
   const int fc = lane & 15, fr4 = 4 * (lane >> 4);
 #pragma unroll
   for (int i = 0; i < TM; ++i)
@@ -355,13 +355,13 @@ __global__ __launch_bounds__(256) void k_seg_syrk_bf16x3(
       for (int e = 0; e < 4; ++e) {
         const int r = m0 + wm0 + 16 * i + fr4 + e;
         const int c = n0 + wn0 + 16 * j + fc;
-        if (r < n && c < n) P[(int64_t)r * n + c] = (double)acc[i][j][e];
+        if (r < n && c < n) P[(int64_t)r * n + c] = acc[i][j][e];
       }
 }
 
 
 template __global__ void k_seg_syrk_bf16x3<128>(const float*, int, int64_t, int, int, const int2*, int,
-                                                 int, double*);
+                                                 int, float*);
 
 // ---------------------------------------------------------------------------
 // Direct-to-LDS pipelined variants (global_load_lds_dwordx4, gfx950).
@@ -728,7 +728,7 @@ __global__ __launch_bounds__(64 * WM * WN, (dl_min_blocks<WM, WN, TM, TN>())) vo
 template <int S, int BM>
 __global__ __launch_bounds__(BM * 2) void k_seg_syrk_bf16x3_dl(
     const uint16_t* __restrict__ whi, const uint16_t* __restrict__ wlo, int nseg, int64_t sstride, int ldw,
-    int n, const int2* __restrict__ tiles, int ntile, int segs_per_chunk, double* __restrict__ partial)
+    int n, const int2* __restrict__ tiles, int ntile, int segs_per_chunk, float* __restrict__ partial)
 {
   static_assert(BM == 128 || BM == 256, "SYRK tile");
   constexpr int NW = BM / 32, WN = NW / 2;
@@ -772,7 +772,7 @@ __global__ __launch_bounds__(BM * 2) void k_seg_syrk_bf16x3_dl(
         const uint4* st = pick_stage<S>(k, lds0, lds1, lds2, lds3);
         bf16x3_step_swz<TM, TN>(st, st + BM * 4, st + BM * 8, st + BM * 12, wm0, wn0, lane, acc);
       });
-  double* P = partial + (int64_t)chunk * n * n;
+  float* P = partial + (int64_t)chunk * n * n;   // the FP32 chunk sum, exactly
   const int fc = lane & 15, fr4 = 4 * (lane >> 4);
 #pragma unroll
   for (int i = 0; i < TM; ++i)
@@ -782,7 +782,7 @@ __global__ __launch_bounds__(BM * 2) void k_seg_syrk_bf16x3_dl(
       for (int e = 0; e < 4; ++e) {
         const int r = m0 + wm0 + 16 * i + fr4 + e;
         const int c = n0 + wn0 + 16 * j + fc;
-        if (r < n && c < n) P[(int64_t)r * n + c] = (double)acc[i][j][e];
+        if (r < n && c < n) P[(int64_t)r * n + c] = acc[i][j][e];
       }
 }
 
@@ -808,7 +808,7 @@ MCHIP_SLAB16DLA_INST(2, 2, 4, 5, 2)
 #undef MCHIP_SLAB16DLA_INST
 #define MCHIP_SYRK16DL_INST(S, BM)                                                                     \
   template __global__ void k_seg_syrk_bf16x3_dl<S, BM>(const uint16_t*, const uint16_t*, int, int64_t, int, \
-                                                       int, const int2*, int, int, double*);
+                                                       int, const int2*, int, int, float*);
 MCHIP_SYRK16DL_INST(2, 128)
 MCHIP_SYRK16DL_INST(3, 128)
 MCHIP_SYRK16DL_INST(4, 128)

@@ -1,6 +1,6 @@
 // FP64 kernels of the DF Fock build on gfx950.
 //
-//   k_reduce_sym       FP64 reduction of split-K partials into a symmetric K
+//   k_reduce_sym       FP64 reduction of split-K partials (FP64 or FP32) into a symmetric K
 //   k_pair_gather / k_j_gamma / k_j_cols / k_pair_scatter
 //                      HBM-streaming DF-J (jkdf.cpp:111-170).
 // The MFMA contractions live in kernels_gemm.hip.
@@ -9,7 +9,8 @@
 namespace mchip {
 
 // out[r][c] = out[c][r] = alpha * sum_ch (P[ch][r][c] (+ P[ch][c][r] if sym2)), r <= c
-__global__ void k_reduce_sym(const double* __restrict__ partial, int nchunk, int n, int sym2,
+template <class T>
+__global__ void k_reduce_sym(const T* __restrict__ partial, int nchunk, int n, int sym2,
                              double alpha, double* __restrict__ out)
 {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -18,14 +19,16 @@ __global__ void k_reduce_sym(const double* __restrict__ partial, int nchunk, int
   if (r > c) return;
   double v = 0.0;
   for (int ch = 0; ch < nchunk; ++ch) {
-    const double* P = partial + (int64_t)ch * n * n;
-    v += P[(int64_t)r * n + c];
-    if (sym2) v += P[(int64_t)c * n + r];
+    const T* P = partial + (int64_t)ch * n * n;
+    v += (double)P[(int64_t)r * n + c];
+    if (sym2) v += (double)P[(int64_t)c * n + r];
   }
   v *= alpha;
   out[(int64_t)r * n + c] = v;
   out[(int64_t)c * n + r] = v;
 }
+template __global__ void k_reduce_sym<double>(const double*, int, int, int, double, double*);
+template __global__ void k_reduce_sym<float>(const float*, int, int, int, double, double*);
 
 // ---------------------------------------------------------------------------
 // DF-J streaming kernels

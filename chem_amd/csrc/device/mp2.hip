@@ -627,7 +627,7 @@ double mp2_sos_batched(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in,
                      (int64_t)L.total, B32.as<float>(), d_soff.as<int64_t>(), d_slen.as<int64_t>(), d_cs.as<int>(),
                      nch, naux, d_t5.as<int2>(), ntile, partial.as<double>());
   for (int g = 0; g < npt; ++g)
-    hipLaunchKernelGGL(k_reduce_sym, dim3(nblk((int64_t)naux * naux, 256)), dim3(256), 0, c.stream,
+    hipLaunchKernelGGL(k_reduce_sym<double>, dim3(nblk((int64_t)naux * naux, 256)), dim3(256), 0, c.stream,
                        partial.as<double>() + (size_t)g * nch * naux * naux, nch, naux, 0, 1.0,
                        Z.as<double>() + (size_t)g * naux * naux);
   const hipError_t cap_err = hipGetLastError();

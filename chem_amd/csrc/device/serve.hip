@@ -327,13 +327,13 @@ void Server::run(Ctx& c, SrvJob& j, bool& admitted)
                + 60.0 * 8.0 * n * (double)n                // SCF matrices, DIIS, split-K partials
                + 64.0 * 1048576.0;                         // tables and small buffers
   // FP32 copy and BF16 hi/lo copy (both with rows padded to 32 elements), the
-  // BF16 split-K partials (one n x n FP64 block per 32 aux functions)
+  // BF16 split-K partials (one n x n FP32 block per 32 aux functions)
   // (the escalation mode runs without the FP32 copy here: BF16x3, then FP64)
   double ptot = 0.0;
   for (int b = 0; b < L.nbb; ++b) ptot += (double)naux * L.bsz[b] * ((L.ncol[b] + 31) / 32 * 32);
   const double copies = (j.kmode == 2 ? 4.0 * ptot : 0.0) +
                         (j.kmode == 3 || j.kmode == 4
-                             ? 4.0 * ptot + 2.0 * 32.0 * naux * nbb * 8.0 + 8.0 * n * (double)n * ((naux + 31) / 32)
+                             ? 4.0 * ptot + 2.0 * 32.0 * naux * nbb * 8.0 + 4.0 * n * (double)n * ((naux + 31) / 32)
                              : 0.0);
   // FP64 exchange work space (HT1 and HT2 unless B is transformed; the final
   // exchange of the low-precision modes is the plain FP64 one, run in occupied
