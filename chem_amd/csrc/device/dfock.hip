@@ -181,46 +181,6 @@ void slab_gemm(Ctx& c, DevBT& t, const double* G, int ldg, int ng, double* out, 
   slab_gemm_t<double>(c, t, t.data64.as<double>(), G, ldg, ng, out, ldo);
 }
 
-// C(m,n) = alpha * sum_s sum_k A_s(m,k) B_s(n,k), symmetric output.  When B
-// differs from A the symmetrised sum G + G^T is returned.
-template <class T>
-static void seg_syrk_t(Ctx& c, const T* A, const T* B, int nseg, int64_t sstride, int lda,
-                       int klen, int n, double alpha, double* out, DevBuf& partial_buf, int nbatch,
-                       int batch, bool reduce)
-{
-  const bool sym = (A == B);
-  const int nt = (n + SG_BM - 1) / SG_BM;   // 128 x 128 output tiles
-  std::vector<int2> tl;
-  for (int i = 0; i < nt; ++i)
-    for (int j = 0; j < nt; ++j)
-      if (!sym || i <= j) tl.push_back(int2{i, j});
-  const int ntile = (int)tl.size();
-  int nch = std::max(1, std::min(nseg, 2048 / std::max(1, ntile)));
-  const int per = (nseg + nch - 1) / nch;
-  nch = (nseg + per - 1) / per;
-  // partial holds nbatch groups of nch chunks
-  const size_t need = (size_t)nbatch * nch * n * n * sizeof(double);
-  if (batch == 0) partial_buf.ensure(need);
-  double* partial = partial_buf.as<double>() + (size_t)batch * nch * n * n;
-  DevBuf tiles = upload_vec(tl, c.stream);
-  hipLaunchKernelGGL((k_seg_gemm<T, SG_BM, SG_BM>), dim3(ntile * nch), dim3(256), 0, c.stream, A, B,
-                     nseg, sstride, lda, klen, n, tiles.as<int2>(), ntile, per, partial);
-  HIP_CHECK(hipGetLastError());
-  if (reduce) {
-    hipLaunchKernelGGL(k_reduce_sym<double>, dim3(nblk((int64_t)n * n, 256)), dim3(256), 0, c.stream,
-                       partial_buf.as<double>(), nbatch * nch, n, sym ? 0 : 1, alpha, out);
-    HIP_CHECK(hipGetLastError());
-  }
-  HIP_CHECK(hipStreamSynchronize(c.stream));   // tile table lifetime
-}
-
-void seg_syrk(Ctx& c, const double* A, const double* B, int nseg, int64_t sstride, int lda,
-              int klen, int n, double alpha, double* out, DevBuf& partial_buf, int nbatch,
-              int batch, bool reduce)
-{
-  seg_syrk_t<double>(c, A, B, nseg, sstride, lda, klen, n, alpha, out, partial_buf, nbatch, batch,
-                     reduce);
-}
 
 void metric_create(Ctx& c, int naux, const double* V_host, Metric& m)
 {

@@ -17,9 +17,6 @@ void slab_gemm_f32(Ctx& c, DevBT& t, const float* data, const float* G, int ldg,
                    int bn, bool f64_sums = false);
 void slab_gemm(Ctx& c, DevBT& t, const double* G, int ldg, int ng, double* out, int ldo,
                double alpha);
-void seg_syrk(Ctx& c, const double* A, const double* B, int nseg, int64_t sstride, int lda,
-              int klen, int n, double alpha, double* out, DevBuf& partial_buf, int nbatch,
-              int batch, bool reduce);
 // J from density P (device, nbf x nbf); the total density is pscale * P.
 void dfj(Ctx& c, DevBT& t, const Metric* m, const double* P, double* J, double pscale);
 // K = -sum ... from occupied coefficients C (device, row-major nbf x ldc).

@@ -18,9 +18,6 @@ __global__ void k_slab_gemm(const T* bdata, const int64_t* slab_off, const int* 
                             const Tile* tiles, int naux, int nbf, const T* G, int ldg, int ng, T* out,
                             int ldo);
 template <class T, int BM, int BN>
-__global__ void k_seg_gemm(const T* A, const T* B, int nseg, int64_t sstride, int lda, int klen, int n,
-                           const int2* tiles, int ntile, int segs_per_chunk, double* partial);
-template <class T, int BM, int BN>
 __global__ void k_slab_out_gemm(const T* A, int lda, int klen, const int64_t* slab_off, const int* ncol,
                                 const int* bsz, const int* boff, const int* colmap,
                                 const int* colmap_off, const Tile* tiles, int naux, int nbf,

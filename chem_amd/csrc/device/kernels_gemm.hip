@@ -4,10 +4,8 @@
 //                         of a dense operand: out(X,nu,j) = sum_mu B(X|mu nu) G(mu,j).
 //                         XMN,Mi->XiN (jkdf.cpp:361) and mi,Xmn->Xin
 //                         (z_full.cpp:202).  One tile table entry per block.
-//   k_seg_gemm<T,BM,BN>   split-K GEMM over a segmented contraction index,
-//                         partial[chunk](m,n) = sum_{s in chunk} sum_k
-//                         A_s(m,k) B_s(n,k): the exchange SYRK XiM,XiN->MN
-//                         (jkdf.cpp:417).  FP64 chunk partials.
+// The exchange SYRK XiM,XiN->MN (jkdf.cpp:417) of every precision is in the
+// direct-to-LDS kernel files.
 #include "gemm_mfma.hpp"
 #include "kernels.hpp"
 
@@ -98,78 +96,6 @@ __global__ __launch_bounds__(256) void k_slab_gemm(
       }
 }
 
-template <class T, int BM, int BN>
-__global__ __launch_bounds__(256) void k_seg_gemm(
-    const T* __restrict__ A, const T* __restrict__ B, int nseg, int64_t sstride, int lda, int klen,
-    int n, const int2* __restrict__ tiles, int ntile, int segs_per_chunk, double* __restrict__ partial)
-{
-  constexpr int TM = BM / 32, TN = BN / 32;
-  constexpr int LDA = BM + 16, LDB = BN + 16;
-  constexpr int RA = BM / 16, RB = BN / 16;
-  __shared__ T As[2][GM_BK * LDA];
-  __shared__ T Bs[2][GM_BK * LDB];
-  const int2 tl = tiles[blockIdx.x % ntile];
-  const int chunk = blockIdx.x / ntile;
-  const int s0 = chunk * segs_per_chunk;
-  const int s1 = min(nseg, s0 + segs_per_chunk);
-  const int m0 = tl.x * BM, n0 = tl.y * BN;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm0 = (wave >> 1) * (BM / 2), wn0 = (wave & 1) * (BN / 2);
-  typename Mfma<T>::v4 acc[TM][TN];
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j) acc[i][j] = typename Mfma<T>::v4{0, 0, 0, 0};
-  const int lk = tid & 15, lr = tid >> 4;
-  const int ksteps = (klen + GM_BK - 1) / GM_BK;
-  const int nsteps = (s1 > s0) ? (s1 - s0) * ksteps : 0;
-  T ra[RA], rb[RB];
-  auto load = [&](int q) {
-    const int s = s0 + q / ksteps;
-    const int kk = (q % ksteps) * GM_BK + lk;
-    const T* As_g = A + (int64_t)s * sstride;
-    const T* Bs_g = B + (int64_t)s * sstride;
-#pragma unroll
-    for (int j = 0; j < RA; ++j) {
-      const int m = m0 + lr + 16 * j;
-      ra[j] = (m < n && kk < klen) ? As_g[(int64_t)m * lda + kk] : T(0);
-    }
-#pragma unroll
-    for (int j = 0; j < RB; ++j) {
-      const int nn = n0 + lr + 16 * j;
-      rb[j] = (nn < n && kk < klen) ? Bs_g[(int64_t)nn * lda + kk] : T(0);
-    }
-  };
-  auto store = [&](int buf) {
-#pragma unroll
-    for (int j = 0; j < RA; ++j) As[buf][lk * LDA + lr + 16 * j] = ra[j];
-#pragma unroll
-    for (int j = 0; j < RB; ++j) Bs[buf][lk * LDB + lr + 16 * j] = rb[j];
-  };
-  if (nsteps > 0) {
-    load(0);
-    store(0);
-  }
-  __syncthreads();
-  for (int q = 0; q < nsteps; ++q) {
-    const int cur = q & 1;
-    if (q + 1 < nsteps) load(q + 1);
-    mfma_step<T, TM, TN, LDA, LDB>(As[cur], Bs[cur], wm0, wn0, lane, acc);
-    if (q + 1 < nsteps) store(cur ^ 1);
-    __syncthreads();
-  }
-  double* P = partial + (int64_t)chunk * n * n;
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int r = m0 + wm0 + 16 * i + Mfma<T>::row(lane, e);
-        const int c = n0 + wn0 + 16 * j + (lane & 15);
-        if (r < n && c < n) P[(int64_t)r * n + c] = (double)acc[i][j][e];
-      }
-}
 
 // explicit instantiations
 template __global__ void k_slab_gemm<double, 128, 128>(const double*, const int64_t*, const int*, const int*,
@@ -184,10 +110,6 @@ template __global__ void k_slab_gemm<float, 128, 128, 1>(const float*, const int
     const int*, const int*, const int*, const Tile*, int, int, const float*, int, int, float*, int);
 template __global__ void k_slab_gemm<float, 128, 64, 1>(const float*, const int64_t*, const int*, const int*,
     const int*, const int*, const int*, const Tile*, int, int, const float*, int, int, float*, int);
-template __global__ void k_seg_gemm<double, 128, 128>(const double*, const double*, int, int64_t, int, int,
-    int, const int2*, int, int, double*);
-template __global__ void k_seg_gemm<float, 128, 128>(const float*, const float*, int, int64_t, int, int,
-    int, const int2*, int, int, double*);
 
 }  // namespace mchip
 
