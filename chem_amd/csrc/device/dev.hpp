@@ -228,9 +228,10 @@ struct Tile {
   int pad;
 };
 
-// Autotuned BF16 exchange pipeline: slab-GEMM shape, SYRK variant and the
-// timings (ms, -1 = not tried) they were chosen from.
-struct Bf16Plan {
+// Autotuned exchange pipeline of one precision (BF16x3, FP32 or FP64): slab-GEMM
+// shape, SYRK variant (-1 = not chosen yet) and the timings (ms, -1 = not
+// tried) they were chosen from.
+struct ExchangePlan {
   int k1 = -1, k3 = -1;
   std::vector<float> k1_ms, k3_ms;
 };
@@ -244,9 +245,9 @@ struct DevBT {
   DevBuf ldp, off16, ldp_pre;               // padded row length, slab offsets (elements), prefix of ldp
   std::vector<int> h_ldp;
   std::vector<int64_t> h_off16, h_ldp_pre;
-  std::map<int, Bf16Plan> bf16_plan;        // ng -> tuned BF16 exchange pipeline
-  std::map<int, Bf16Plan> f32_plan;         // ng -> tuned FP32 exchange pipeline
-  std::map<int, Bf16Plan> f64_plan;         // ng -> tuned FP64 slab GEMM / SYRK (-1: not yet)
+  std::map<int, ExchangePlan> bf16_plan;        // ng -> tuned BF16 exchange pipeline
+  std::map<int, ExchangePlan> f32_plan;         // ng -> tuned FP32 exchange pipeline
+  std::map<int, ExchangePlan> f64_plan;         // ng -> tuned FP64 slab GEMM / SYRK (-1: not yet)
   DevBuf slab_off, ncol, bsz, boff, colmap, colmap_off, pg_off;
   std::vector<int> h_colmap, h_colmap_off;
   std::vector<int64_t> h_pg_off;            // prefix of bsz[N]*ncol[N]

@@ -691,13 +691,13 @@ static float time_launch(Ctx& c, F&& f)
   return ms;
 }
 
-static const Bf16Plan& tune_bf16(Ctx& c, DevBT& t, const uint16_t* ghi, const uint16_t* glo, int ng, float* W,
+static const ExchangePlan& tune_bf16(Ctx& c, DevBT& t, const uint16_t* ghi, const uint16_t* glo, int ng, float* W,
                                  uint16_t* Whi, uint16_t* Wlo, int ldw, const int2* tl, int ntile, int nch, int spc,
                                  float* partial)
 {
   auto it = t.bf16_plan.find(ng);
   if (it != t.bf16_plan.end()) return it->second;
-  Bf16Plan p;
+  ExchangePlan p;
   p.k1_ms.assign(kNumBf16Shapes, -1.f);
   p.k3_ms.assign(kNumSyrkVariants, -1.f);
   float best1[2] = {1e30f, 1e30f};
@@ -738,7 +738,7 @@ void dfk_bf16_plan(const DevBT& t, int ng, int* plan, float* ms, int* nms)
   for (int i = 0; i < 8; ++i) plan[i] = 0;
   if (nms) *nms = 0;
   if (it == t.bf16_plan.end()) return;
-  const Bf16Plan& p = it->second;
+  const ExchangePlan& p = it->second;
   const Bf16Shape& sh = kBf16Shapes[p.k1];
   plan[0] = p.k1;
   plan[1] = sh.bm();
@@ -762,7 +762,7 @@ void dfk_bf16_force(DevBT& t, int ng, int k1, int k3)
     throw std::runtime_error("BF16 plan: shape or SYRK variant out of range");
   if ((kBf16Shapes[k1].DL != 0) != (k3 != 0))
     throw std::runtime_error("BF16 plan: direct-to-LDS GEMM shapes pair with SYRK variants 1-4, the others with 0");
-  Bf16Plan p;
+  ExchangePlan p;
   p.k1 = k1;
   p.k3 = k3;
   p.k1_ms.assign(kNumBf16Shapes, -1.f);
@@ -845,7 +845,7 @@ void dfk_mo_bf16(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K
       HIP_CHECK(hipGetLastError());
     }
     float* partial = c.scratch_partial.as<float>() + (size_t)b * nch * nbf * nbf;
-    const Bf16Plan& plan = tune_bf16(c, t, ghi, glo, w, W, Whi, Wlo, ldw, d_tl, ntile, nch, spc, partial);
+    const ExchangePlan& plan = tune_bf16(c, t, ghi, glo, w, W, Whi, Wlo, ldw, d_tl, ntile, nch, spc, partial);
     {
       PhaseTimer tm(c, PH_K1);
       slab_gemm_bf16(c, t, plan.k1, ghi, glo, w, W, Whi, Wlo, ldw);
@@ -945,12 +945,12 @@ static void seg_syrk_f32dl(Ctx& c, int variant, const float* W, int naux, int nb
 // whose column tile is not mostly empty and every SYRK variant is timed once
 // on the live operands (after one warm launch); the fastest of each is kept
 // per ng, with the timings.
-static const Bf16Plan& tune_f32(Ctx& c, DevBT& t, const float* G, int ng, float* W, int ldw, const int2* tl,
+static const ExchangePlan& tune_f32(Ctx& c, DevBT& t, const float* G, int ng, float* W, int ldw, const int2* tl,
                                 int ntile, int nch, int spc, double* partial)
 {
   auto it = t.f32_plan.find(ng);
   if (it != t.f32_plan.end()) return it->second;
-  Bf16Plan p;
+  ExchangePlan p;
   p.k1_ms.assign(kNumDlShapes, -1.f);
   p.k3_ms.assign(kNumF32Syrk, -1.f);
   float best = 1e30f;
@@ -983,7 +983,7 @@ void dfk_f32_plan(const DevBT& t, int ng, int* plan, float* ms, int* nms)
   for (int i = 0; i < 8; ++i) plan[i] = 0;
   if (nms) *nms = 0;
   if (it == t.f32_plan.end()) return;
-  const Bf16Plan& p = it->second;
+  const ExchangePlan& p = it->second;
   const DlShape& sh = kDlShapes[p.k1];
   plan[0] = p.k1;
   plan[1] = sh.bm();
@@ -1005,7 +1005,7 @@ void dfk_f32_force(DevBT& t, int ng, int k1, int k3)
 {
   if (k1 < 0 || k1 >= kNumDlShapes || k3 < 0 || k3 >= kNumF32Syrk)
     throw std::runtime_error("FP32 plan: shape or SYRK variant out of range");
-  Bf16Plan p;
+  ExchangePlan p;
   p.k1 = k1;
   p.k3 = k3;
   p.k1_ms.assign(kNumDlShapes, -1.f);
@@ -1062,7 +1062,7 @@ void dfk_mo_f32(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K,
       HIP_CHECK(hipGetLastError());
     }
     double* partial = c.scratch_partial.as<double>() + (size_t)b * nch * nbf * nbf;
-    const Bf16Plan& plan = tune_f32(c, t, G, w, W, ldw, tl, ntile, nch, spc, partial);
+    const ExchangePlan& plan = tune_f32(c, t, G, w, W, ldw, tl, ntile, nch, spc, partial);
     {
       PhaseTimer tm(c, PH_K1);
       slab_gemm_f32dl(c, t, plan.k1, G, w, W, ldw);
@@ -1132,11 +1132,11 @@ static void seg_syrk_f64dl(Ctx& c, int variant, const double* W, int naux, int n
 }
 
 // FP64 plan entry for ng columns (k1 / k3 = -1 until tuned or forced).
-static Bf16Plan& f64_plan_entry(DevBT& t, int ng)
+static ExchangePlan& f64_plan_entry(DevBT& t, int ng)
 {
   auto it = t.f64_plan.find(ng);
   if (it != t.f64_plan.end()) return it->second;
-  Bf16Plan p;
+  ExchangePlan p;
   p.k1_ms.assign(kNumF64Shapes, -1.f);
   p.k3_ms.assign(kNumF64Syrk, -1.f);
   return t.f64_plan[ng] = p;
@@ -1155,7 +1155,7 @@ static void syrk_f64_exchange(Ctx& c, DevBT& t, const double* W, int ng, int ldw
   nch = (naux + spc - 1) / spc;
   if (b == 0) c.scratch_partial.ensure((size_t)nb * nch * nbf * nbf * sizeof(double));
   double* partial = c.scratch_partial.as<double>() + (size_t)b * nch * nbf * nbf;
-  Bf16Plan& p = f64_plan_entry(t, ng);
+  ExchangePlan& p = f64_plan_entry(t, ng);
   if (p.k3 < 0) {
     float best = 1e30f;
     for (int v = 0; v < kNumF64Syrk; ++v) {
@@ -1211,7 +1211,7 @@ static void dfk_mo_tilde64_dl(Ctx& c, DevBT& t, const double* C, int ldc, int no
                            t.colmap.as<int>(), t.colmap_off.as<int>(), G);
       HIP_CHECK(hipGetLastError());
     }
-    Bf16Plan& p = f64_plan_entry(t, w);
+    ExchangePlan& p = f64_plan_entry(t, w);
     if (p.k1 < 0) {
       float best = 1e30f;
       for (int v = 0; v < kNumF64Shapes; ++v) {
@@ -1240,7 +1240,7 @@ void dfk_f64_plan(const DevBT& t, int ng, int* plan, float* ms, int* nms)
   for (int i = 0; i < 8; ++i) plan[i] = 0;
   if (nms) *nms = 0;
   if (it == t.f64_plan.end()) return;
-  const Bf16Plan& p = it->second;
+  const ExchangePlan& p = it->second;
   plan[0] = p.k1;
   if (p.k1 >= 0) {
     const DlShape& sh = kF64Shapes[p.k1];
@@ -1264,7 +1264,7 @@ void dfk_f64_force(DevBT& t, int ng, int k1, int k3)
 {
   if (k1 < 0 || k1 >= kNumF64Shapes || k3 < 0 || k3 >= kNumF64Syrk)
     throw std::runtime_error("FP64 plan: shape or SYRK variant out of range");
-  Bf16Plan& p = f64_plan_entry(t, ng);
+  ExchangePlan& p = f64_plan_entry(t, ng);
   p.k1 = k1;
   p.k3 = k3;
 }

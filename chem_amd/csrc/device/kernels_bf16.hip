@@ -3,22 +3,27 @@
 // Every FP32/FP64 operand x is carried as hi = bf16(x), lo = bf16(x - hi)
 // (16 significant bits), and a product is formed as lo*hi + hi*lo + hi*hi
 // with FP32 accumulation (the lo*lo term, 2^-16 relative, is dropped).
-// Chunk partials leave the kernels in FP64 and are reduced in FP64.
+// Chunk partials (FP32 sums over at most 32 aux functions) are reduced in FP64.
 //
-//   k_split_slabs         B~ (FP64 pair slabs) -> hi/lo slabs whose rows are
-//                         padded to ldp[N] = roundup(ncol[N], 32) (zeros)
+//   k_split_slabs / k_split_slab   B~ (FP64 pair slabs) -> hi/lo slabs whose rows
+//                         are padded to ldp[N] = roundup(ncol[N], 32) (zeros)
 //   k_gather_ct           per-slab gathered, transposed occupied coefficients
 //                         G_N[i][c] = C(colmap_N[c], i) as hi/lo, row length ldp[N]
-//   k_slab_gemm_bf16x3    W(X,nu,i) = sum_c B~_N(X nu, c) G_N(i, c) -> FP32 W
-//                         (XMN,Mi->XiN of jkdf.cpp:361 on the transformed B)
+//   k_slab_gemm_bf16x3    W(X,nu,i) = sum_c B~_N(X nu, c) G_N(i, c) -> FP32 W,
+//                         register-staged (XMN,Mi->XiN of jkdf.cpp:361 on B~)
+//   k_slab_gemm_bf16x3_dl / _dla   the same, direct-to-LDS pipelined (one
+//                         pipeline for both operands / separate ones), W as
+//                         bf16 hi/lo
 //   k_seg_syrk_bf16x3     partial[chunk](m,n) = sum_{X in chunk} sum_i W(X,m,i) W(X,n,i)
 //                         (XiM,XiN->MN of jkdf.cpp:417), W split to hi/lo on load
+//   k_seg_syrk_bf16x3_dl  the same on bf16 hi/lo W, direct-to-LDS pipelined
 //
 // Both GEMMs read their operands with the contraction index contiguous, 16
 // bytes (8 bf16) per lane, which is exactly the MFMA operand layout:
 //   A[l%16][8*(l/16)+q], B[8*(l/16)+q][l%16], D[4*(l/16)+e][l%16]
-// (pinned by tests/test_gpu_mfma_layout.py).  LDS rows hold 32 bf16 plus 8 of
-// padding (80 bytes): 16 lanes reading 16 consecutive rows hit distinct banks.
+// (pinned by tests/test_gpu_mfma_layout.py).  In the register-staged kernels
+// LDS rows hold 32 bf16 plus 8 of padding (80 bytes); the direct-to-LDS ones
+// use 64-byte rows with XOR-swizzled 16-byte chunks (dl_common.hpp).
 #include "dl_common.hpp"
 #include "kernels.hpp"
 
