@@ -331,7 +331,7 @@ def main():
             "metric": "DF-HF serving: p99 time-to-converged-energy",
             "value": sv["p99_s"], "unit": "s", "n_gpus": 1, "steps": sv["requests"], "warmup": 0,
             "ms_per_step": sv["mean_s"] * 1e3, "higher_is_better": False, "scaling": "weak",
-            "vs_baseline": None, "dtype": "bf16x3 split then f32 (K builds) + f64 (J, final K, SCF)",
+            "vs_baseline": None, "dtype": "bf16x3 split (K builds) + f64 (J, final K, SCF)",
             "data": "synthetic seeded water clusters, real integrals",
             "config": {"workload": "request stream (c4): Poisson %.0f/s, %s, cc-pVDZ/cc-pVDZ-jkfit, "
                                    "conv %.0e, HBM budget %.0f GB" % (

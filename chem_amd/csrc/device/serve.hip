@@ -97,6 +97,11 @@ Server::Server(int device, size_t budget_bytes, int nslots, int cpu_threads)
   for (int i = 0; i < nslots_; ++i) {
     ctx_[i] = std::make_unique<Ctx>();
     ctx_init(*ctx_[i], device_);
+    // Requests carry no FP32 copy, so the escalation mode's RMS trigger would
+    // hand the last iterations to the plain FP64 exchange (L^-1 applied per
+    // build, about 15x a BF16x3 build).  BF16x3 runs until the SCF converges
+    // or stalls instead; the energy comes from the final FP64 exchange.
+    ctx_[i]->escalate_rms = 0.0;
   }
   for (int i = 0; i < nslots_; ++i) workers_.emplace_back([this, i] { worker(i); });
 }

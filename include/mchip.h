@@ -311,6 +311,8 @@ int mchip_scf_last_extra(double* energy_iter, double* t_final_k);
  * different requests overlaps on the device.  cpu_threads is split over the
  * workers for the host integrals.
  * kmode as for mchip_scf_run (0-4); screen_eps is the block threshold.
+ * Requests get no FP32 copy, and kmode 4 here escalates on a stall only (no
+ * RMS trigger): BF16x3 until the SCF converges or stalls, then FP64.
  * result: times[10] = {submit, admit, done (server clock, s), HBM estimate
  * (bytes), worker slot, energy of the last low-precision iteration, seconds of
  * host preparation, B build, metric and copies, SCF}; wait = 1 blocks.
