@@ -20,7 +20,10 @@ plain FP64 seconds per build, |dE| between the two SCFs, the roofline of the
 dominant kernel (K1 of the BF16x3 form) from HIP events on the library stream,
 and the oracle timed on the host.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c0|c1|c2]
+--config c3 times the batched SOS-MP2 graph and --config c4 the request stream;
+each prints its own JSON line.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c0|c1|c2|c3|c4]
 """
 import argparse
 import json
@@ -295,11 +298,11 @@ def main():
                     help="c4: requests re-run on the plain FP64 path for the energy check")
     ap.add_argument("--form", default="auto", choices=["auto", "bf16", "f32"],
                     help="precision of the chosen-form exchange builds (auto: BF16x3 until the "
-                         "SCF error falls below max(1e-4, 10*thresh), then FP32)")
+                         "SCF error reaches --escalate or stalls, then FP32, then FP64 if FP32 stalls)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the DF-K path runs on a single "
-              "MI355X); --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        print("bench.py measures one GPU only (the DF-K path runs on a single MI355X); "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     from chem_amd import desc, ints
