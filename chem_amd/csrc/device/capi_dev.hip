@@ -1,6 +1,7 @@
 // C ABI: device context, device-resident B, metric, DF-J and DF-K.
 #include <cstring>
 #include <memory>
+#include <string>
 
 #include "../../../include/mchip.h"
 #include "../capi_common.hpp"
@@ -305,11 +306,21 @@ extern "C" int mchip_dfj_dev(mchip_dbt t, mchip_metric m, const double* P_dev, d
   MCHIP_CATCH
 }
 
+// The _dev entry points read C through a raw device pointer: the occupied
+// count must fit the row stride, the one size they can check.
+static void check_occ(const char* fn, int ldc, int nocc)
+{
+  if (nocc < 0 || nocc > ldc)
+    throw std::runtime_error(std::string(fn) + ": nocc " + std::to_string(nocc) + " outside 0..ldc (" +
+                             std::to_string(ldc) + ")");
+}
+
 extern "C" int mchip_dfk_mo_dev(mchip_dbt t, mchip_metric m, const double* C_dev, int ldc, int nocc,
                                 int occ_nbatches, double* K_dev)
 {
   MCHIP_TRY
   need_device(t->ctx);
+  check_occ("mchip_dfk_mo_dev", ldc, nocc);
   dfk_mo(t->ctx->c, t->t, m ? &m->m : nullptr, C_dev, ldc, nocc, K_dev, occ_nbatches);
   MCHIP_CATCH
 }
@@ -370,6 +381,7 @@ extern "C" int mchip_dfk_mo_f32_dev(mchip_dbt t, const double* C_dev, int ldc, i
 {
   MCHIP_TRY
   need_device(t->ctx);
+  check_occ("mchip_dfk_mo_f32_dev", ldc, nocc);
   dfk_mo_f32(t->ctx->c, t->t, C_dev, ldc, nocc, K_dev, occ_nbatches);
   MCHIP_CATCH
 }
@@ -424,6 +436,7 @@ extern "C" int mchip_dfk_mo_tilde64_dev(mchip_dbt t, const double* C_dev, int ld
 {
   MCHIP_TRY
   need_device(t->ctx);
+  check_occ("mchip_dfk_mo_tilde64_dev", ldc, nocc);
   if (!t->t.data64t.p) throw std::runtime_error("mchip_dfk_mo_tilde64_dev: no FP64 L^-1 B copy (mchip_dbt_make_lowp f64)");
   dfk_mo(t->ctx->c, t->t, nullptr, C_dev, ldc, nocc, K_dev, occ_nbatches, t->t.data64t.as<double>());
   MCHIP_CATCH
@@ -434,6 +447,7 @@ extern "C" int mchip_dfk_mo_bf16_dev(mchip_dbt t, const double* C_dev, int ldc, 
 {
   MCHIP_TRY
   need_device(t->ctx);
+  check_occ("mchip_dfk_mo_bf16_dev", ldc, nocc);
   dfk_mo_bf16(t->ctx->c, t->t, C_dev, ldc, nocc, K_dev, occ_nbatches);
   MCHIP_CATCH
 }

@@ -242,7 +242,20 @@ class DeviceB:
         (nbf x ldc, first nocc columns), without host copies or a
         synchronisation: form "fp64" (plain, with the metric unless B is
         transformed), "fp64_copy" (the FP64 copy of L^-1 B), "f32" or "bf16x3"."""
+        # the kernels read nbf x ldc and write nbf x nbf through raw device
+        # pointers: check the arrays on the host before any launch
+        for name, a in (("C_dev", C_dev), ("K_dev", K_dev)):
+            if getattr(a, "ptr", None) is None or not a.ptr.value:
+                raise ValueError("dfk_dev: %s has been released" % name)
+            if len(a.shape) != 2 or a.shape[0] != self.nbf:
+                raise ValueError("dfk_dev: %s has shape %s, expected (%d, n)" % (name, a.shape, self.nbf))
+        if K_dev.shape != (self.nbf, self.nbf):
+            raise ValueError("dfk_dev: K_dev has shape %s, expected (%d, %d)" % (K_dev.shape, self.nbf, self.nbf))
         ldc = C_dev.shape[1]
+        if not 0 <= nocc <= ldc:
+            raise ValueError("dfk_dev: nocc %d outside 0..%d (columns of C_dev)" % (nocc, ldc))
+        if form not in ("fp64", "fp64_copy", "f32", "bf16x3"):
+            raise ValueError("dfk_dev: unknown form %r" % (form,))
         L = _lib.lib()
         if form == "fp64":
             m = None if metric is None else metric.h

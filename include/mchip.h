@@ -180,7 +180,9 @@ int mchip_dfj(mchip_dbt t, mchip_metric m, const double* P, double pscale, doubl
 
 /* DFMO-K: K = -(L^-1 B C)^T (L^-1 B C) for C = nbf x nocc (row-major, leading
  * dimension ldc), returned negated as the reference does
- * (DFMO_K::compute_K, jkdf.cpp:221-462, scale at :436-438). */
+ * (DFMO_K::compute_K, jkdf.cpp:221-462, scale at :436-438).
+ * The _dev exchange entry points take device pointers (C: nbf x ldc, K:
+ * nbf x nbf, the caller's to size) and reject nocc outside 0..ldc. */
 int mchip_dfk_mo_dev(mchip_dbt t, mchip_metric m, const double* C_dev, int ldc, int nocc,
                      int occ_nbatches, double* K_dev);
 int mchip_dfk_mo(mchip_dbt t, mchip_metric m, const double* C, int nocc, int occ_nbatches,

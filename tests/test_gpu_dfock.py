@@ -310,6 +310,50 @@ def test_device_resident_exchange_matches_host(ctx):
     Kd.release()
 
 
+def test_device_resident_exchange_rejects_bad_arguments(ctx):
+    """The _dev exchange paths check what they can before any launch: array
+    shapes and released arrays in the Python layer, the occupied count against
+    the row stride in the C entry points (called directly here)."""
+    import ctypes
+    from chem_amd import _lib
+    from chem_amd.engine import DeviceArray, DeviceB, Metric
+    mol, Bh, V = _system(1)
+    m = Metric(ctx, V)
+    dB = DeviceB(ctx, Bh)
+    dB.make_lowp(m, f32=True, bf16=True, f64=True)
+    n, no = mol.nbf, mol.nocc
+    Cd = DeviceArray(ctx, (n, no), np.zeros((n, no)))
+    Kd = DeviceArray(ctx, (n, n))
+    Kbad = DeviceArray(ctx, (n, n - 1))
+    Cbad = DeviceArray(ctx, (n + 1, no))
+    for form in ("fp64", "fp64_copy", "f32", "bf16x3"):
+        print("case: bad arguments, %s" % form, flush=True)
+        with pytest.raises(ValueError):
+            dB.dfk_dev(form, Cd, no, Kbad, metric=m)
+        with pytest.raises(ValueError):
+            dB.dfk_dev(form, Cbad, no, Kd, metric=m)
+        with pytest.raises(ValueError):
+            dB.dfk_dev(form, Cd, no + 1, Kd, metric=m)
+    with pytest.raises(ValueError):
+        dB.dfk_dev("fp16", Cd, no, Kd, metric=m)
+    L = _lib.lib()
+    for name in ("mchip_dfk_mo_f32_dev", "mchip_dfk_mo_tilde64_dev", "mchip_dfk_mo_bf16_dev"):
+        rc = getattr(L, name)(dB.h, Cd.ptr, no, no + 1, 1, Kd.ptr)
+        assert rc != 0, name
+        assert b"outside 0..ldc" in L.mchip_last_error(), name
+    rc = L.mchip_dfk_mo_dev(dB.h, m.h, Cd.ptr, no, -1, 1, Kd.ptr)
+    assert rc != 0 and b"outside 0..ldc" in L.mchip_last_error()
+    # the checks left the context usable: a valid call still works
+    dB.dfk_dev("bf16x3", Cd, no, Kd)
+    ctx.sync()
+    assert np.all(Kd.download() == 0.0)   # C = 0 gives K = 0
+    Cd.release()
+    with pytest.raises(ValueError):
+        dB.dfk_dev("f32", Cd, no, Kd)
+    for a in (Kd, Kbad, Cbad):
+        a.release()
+
+
 # BF16x3 carries every operand with 16 significant bits (hi + lo, 2^-17
 # relative rounding) and accumulates in FP32 over chunks of <= 256 aux
 # functions: K agrees with the FP64 exchange on the same B~ to a few 1e-6
