@@ -122,3 +122,66 @@ def test_bench_help_lists_workloads():
         assert c in r.stdout
     for opt in ("--steps", "--warmup", "--form", "--escalate", "--stall"):
         assert opt in r.stdout
+
+
+def test_split_range_partitions():
+    """split_range (dbcsr_common.hpp:216-230): full blocks of `split` and one
+    remainder block, summing to n; brute force over small n and split."""
+    for n in range(0, 60):
+        for split in range(1, 20):
+            r = desc.split_range(n, split)
+            assert sum(r) == n
+            assert len(r) == -(-n // split)
+            assert all(x == split for x in r[:-1])
+            if r:
+                assert 0 < r[-1] <= split
+
+
+def test_make_blk_bounds_invariants():
+    """make_blk_bounds (dbcsr_btensor.hpp:35-95) on random block sizes and
+    centre maps: batches are consecutive from block 0, never split a centre,
+    number at most min(nbatches, nblocks), and cover every block whenever the
+    reference's floating-point test can close batch min(nbatches, nblocks)."""
+    rng = np.random.default_rng(5)
+    dropped = 0
+    for _ in range(3000):
+        use_map = rng.random() < 0.5
+        if use_map:
+            ncen = int(rng.integers(1, 12))
+            blkmap = sorted(int(c) for c in rng.integers(0, ncen, size=int(rng.integers(ncen, 3 * ncen + 1))))
+            nblk = len(blkmap)
+        else:
+            blkmap = None
+            nblk = int(rng.integers(1, 30))
+        bs = [int(x) for x in rng.integers(1, 15, size=nblk)]
+        nb = int(rng.integers(1, 12))
+        out = desc.make_blk_bounds(bs, nb, blkmap)
+        assert out[0][0] == 0
+        for (a0, a1), (b0, _) in zip(out, out[1:]):
+            assert a0 <= a1 and b0 == a1 + 1
+        assert len(out) <= min(nb, nblk)
+        if blkmap is not None:
+            assert all(e == nblk - 1 or blkmap[e] != blkmap[e + 1] for _, e in out)
+        k = min(nb, nblk)
+        closes = float(sum(bs)) >= k * (sum(bs) / k)
+        covers = out[-1][1] == nblk - 1
+        # a run that ends below batch k closes at the last block anyway; only
+        # a threshold k * (nele / k) rounded above nele can leave blocks out
+        if closes:
+            assert covers
+        dropped += not covers
+    assert dropped > 0   # the rounding case occurs in this sample
+
+
+def test_make_blk_bounds_known_cases():
+    """Hand-checked splits: equal blocks into equal batches; a centre map that
+    forces a batch to run to the centre's end; 211 elements in 11 batches,
+    where 11 * (211 / 11) rounds to 211.00000000000003 and the last batch
+    is never closed (the reference's rule, kept)."""
+    assert desc.make_blk_bounds([2] * 6, 3) == [[0, 1], [2, 3], [4, 5]]
+    assert desc.make_blk_bounds([1, 1, 1, 1], 2, [0, 0, 0, 1]) == [[0, 2], [3, 3]]
+    assert desc.make_blk_bounds([5, 5], 4) == [[0, 0], [1, 1]]
+    assert 11 * (211 / 11) > 211
+    bs = [10] * 20 + [11]
+    out = desc.make_blk_bounds(bs, 11)
+    assert out[-1][1] < len(bs) - 1
