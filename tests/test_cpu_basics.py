@@ -185,3 +185,21 @@ def test_make_blk_bounds_known_cases():
     bs = [10] * 20 + [11]
     out = desc.make_blk_bounds(bs, 11)
     assert out[-1][1] < len(bs) - 1
+
+
+def test_product_code_does_not_use_the_oracle():
+    """The oracle is the checker only: nothing under chem_amd/ or include/
+    imports, links or names the oracle package, and bench.py touches it only
+    inside its CPU-baseline function."""
+    pat = re.compile(r"^\s*(from\s+oracle\b|import\s+oracle\b)|[\"']oracle[/.]", re.M)
+    for top in ("chem_amd", "include"):
+        for dirpath, _, files in os.walk(os.path.join(ROOT, top)):
+            for f in files:
+                if f.endswith((".py", ".hip", ".hpp", ".cpp", ".h", ".c")):
+                    text = open(os.path.join(dirpath, f), encoding="utf-8", errors="replace").read()
+                    assert not pat.search(text), os.path.join(dirpath, f)
+    src = open(os.path.join(ROOT, "bench.py")).read()
+    body = src.split("\ndef cpu_baseline(", 1)
+    assert len(body) == 2
+    outside = body[0] + "\n".join(l for l in body[1].split("\ndef ", 1)[1:])
+    assert not re.search(r"^\s*(from\s+oracle\b|import\s+oracle\b)", outside, re.M)
