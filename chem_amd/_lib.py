@@ -130,9 +130,6 @@ _SIGS = [
     ("mchip_z_compute", c_int, [c_void_p, c_int, c_int, P_double, c_int, P_double, P_double, c_int, P_double]),
     ("mchip_mp2_z", c_int, [c_void_p, c_void_p, c_int, c_int, P_double, P_double, P_double, c_int,
                             P_double, P_double, c_int, P_int, c_int, c_double, c_int, P_double, P_double]),
-    ("mchip_probe_mfma_f64", c_int, [P_double, P_double, P_double]),
-    ("mchip_probe_mfma_lp", c_int, [c_int, P_float, P_float, P_float]),
-    ("mchip_probe_lds_load", c_int, [ctypes.POINTER(ctypes.c_uint32), P_int, ctypes.POINTER(ctypes.c_uint32)]),
     ("mchip_ctx_set_timing", c_int, [c_void_p, c_int]),
     ("mchip_ctx_set_escalation", c_int, [c_void_p, c_double, c_int]),
     ("mchip_ctx_trim", c_int, [c_void_p]),

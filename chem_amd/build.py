@@ -49,6 +49,24 @@ def _compile(src):
     return obj
 
 
+PROBE_SRC = os.path.join(ROOT, "tests", "native", "probe.hip")
+PROBE_LIB = os.path.join(ROOT, "tests", "native", "libmchip_probe.so")
+
+
+def build_probe():
+    """Test-only library of builtin-layout probes (tests/native/probe.hip),
+    loaded by the tests alone."""
+    if not os.path.exists(PROBE_SRC):
+        return None
+    tmp = PROBE_LIB + ".tmp"
+    cmd = [HIPCC] + COMMON + DEVICE + ["-x", "hip", PROBE_SRC, "-shared", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("probe build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(tmp, PROBE_LIB)
+    return PROBE_LIB
+
+
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
@@ -62,6 +80,7 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
     os.replace(tmp, LIB)
+    build_probe()
     if verbose:
         print("built", LIB)
     return LIB

@@ -246,15 +246,19 @@ void dfj(Ctx& c, DevBT& t, const Metric* m, const double* P, double* J, double p
   nch = std::min(nch, naux);
   const int xchunk = (naux + nch - 1) / nch;
   nch = (naux + xchunk - 1) / xchunk;
-  // workspace: Pg | gpart | g | gq | Jpart | Jg
+  // workspace: Pg | gpart, gpart_c | g | gq | Jpart, Jpart_c | Jg; every sum is
+  // compensated (value + rounding-error pair), so J carries no summation-order
+  // noise for V^-1 to amplify
   const size_t n_pg = (size_t)pg, n_gp = (size_t)nbb * naux, n_jp = (size_t)nch * pg;
-  c.scratch_c.ensure((n_pg + n_gp + 2 * (size_t)naux + n_jp + n_pg) * sizeof(double) + 256);
+  c.scratch_c.ensure((n_pg + 2 * n_gp + 2 * (size_t)naux + 2 * n_jp + n_pg) * sizeof(double) + 256);
   double* Pg = c.scratch_c.as<double>();
   double* gpart = Pg + n_pg;
-  double* g = gpart + n_gp;
+  double* gpart_c = gpart + n_gp;
+  double* g = gpart_c + n_gp;
   double* gq = g + naux;
   double* Jpart = gq + naux;
-  double* Jg = Jpart + n_jp;
+  double* Jpart_c = Jpart + n_jp;
+  double* Jg = Jpart_c + n_jp;
   const unsigned gx = nblk(std::max<int64_t>(t.maxR, 1), 256);
   if (pg > 0) {
     hipLaunchKernelGGL(k_pair_gather, dim3(gx, nbb), dim3(256), 0, c.stream, P, nbf, t.ncol.as<int>(),
@@ -262,28 +266,28 @@ void dfj(Ctx& c, DevBT& t, const Metric* m, const double* P, double* J, double p
                        t.pg_off.as<int64_t>(), Pg);
     hipLaunchKernelGGL(k_j_gamma, dim3(nblk(naux, J_ROWS), nbb), dim3(256), 0, c.stream,
                        t.data64.as<double>(), t.slab_off.as<int64_t>(), t.ncol.as<int>(),
-                       t.bsz.as<int>(), t.pg_off.as<int64_t>(), Pg, naux, gpart);
+                       t.bsz.as<int>(), t.pg_off.as<int64_t>(), Pg, naux, gpart, gpart_c);
   }
   else {
-    hipLaunchKernelGGL(k_fill, dim3(gs_blocks(n_gp)), dim3(256), 0, c.stream, gpart, (int64_t)n_gp, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(gs_blocks(2 * n_gp)), dim3(256), 0, c.stream, gpart, (int64_t)(2 * n_gp), 0.0);
   }
-  hipLaunchKernelGGL(k_reduce_slabs, dim3(nblk(naux, 256)), dim3(256), 0, c.stream, gpart, nbb, naux,
+  hipLaunchKernelGGL(k_reduce_slabs, dim3(nblk(naux, 256)), dim3(256), 0, c.stream, gpart, gpart_c, nbb, naux,
                      pscale, g);
   if (t.transformed) {
     HIP_CHECK(hipMemcpyAsync(gq, g, naux * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
   }
   else {
-    const double one = 1.0, zero = 0.0;
-    RB_CHECK(rocblas_dgemv(c.blas, rocblas_operation_transpose, naux, naux, &one, m->Vinv.as<double>(),
-                           naux, g, 1, &zero, gq, 1));
+    // gq = V^-1 g (V^-1 symmetric: its rows are its columns)
+    hipLaunchKernelGGL(k_symv_dot2, dim3(nblk(naux, 4)), dim3(256), 0, c.stream, m->Vinv.as<double>(), g, naux, gq);
   }
   hipLaunchKernelGGL(k_fill, dim3(gs_blocks((int64_t)nbf * nbf)), dim3(256), 0, c.stream, J,
                      (int64_t)nbf * nbf, 0.0);
   if (pg > 0) {
     hipLaunchKernelGGL(k_j_cols, dim3(gx, nbb, nch), dim3(256), 0, c.stream, t.data64.as<double>(),
                        t.slab_off.as<int64_t>(), t.ncol.as<int>(), t.bsz.as<int>(), t.pg_off.as<int64_t>(),
-                       gq, naux, xchunk, pg, Jpart);
-    hipLaunchKernelGGL(k_reduce_chunks, dim3(gs_blocks(pg)), dim3(256), 0, c.stream, Jpart, nch, pg, Jg);
+                       gq, naux, xchunk, pg, Jpart, Jpart_c);
+    hipLaunchKernelGGL(k_reduce_chunks_dd, dim3(gs_blocks(pg)), dim3(256), 0, c.stream, Jpart, Jpart_c, nch, pg,
+                       Jg);
     hipLaunchKernelGGL(k_pair_scatter, dim3(gx, nbb), dim3(256), 0, c.stream, Jg, nbf, t.ncol.as<int>(),
                        t.bsz.as<int>(), t.boff.as<int>(), t.colmap.as<int>(), t.colmap_off.as<int>(),
                        t.pg_off.as<int64_t>(), J);
@@ -532,6 +536,15 @@ void dbt_make_bf16(Ctx& c, DevBT& t)
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(c.stream));
   t.lowp_tilde = true;
+}
+
+// Zero an exchange work array of the given byte count (a multiple of 8): the
+// slab GEMMs write W only up to the column-tile edge past their width.
+static void zero_w(Ctx& c, void* W, size_t bytes)
+{
+  const int64_t n = (int64_t)(bytes / sizeof(double));
+  hipLaunchKernelGGL(k_fill, dim3(gs_blocks(n)), dim3(256), 0, c.stream, static_cast<double*>(W), n, 0.0);
+  HIP_CHECK(hipGetLastError());
 }
 
 // Tile table of the BF16 slab GEMM (row tile BM, column tile BN), padded with
@@ -848,6 +861,9 @@ void dfk_mo_bf16(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K
     const ExchangePlan& plan = tune_bf16(c, t, ghi, glo, w, W, Whi, Wlo, ldw, d_tl, ntile, nch, spc, partial);
     {
       PhaseTimer tm(c, PH_K1);
+      // a narrower last batch: its tiles stop at the first column-tile edge
+      // past w, so the columns up to ldw still hold the previous batch
+      if (w < per) zero_w(c, W, nW * sizeof(float));
       slab_gemm_bf16(c, t, plan.k1, ghi, glo, w, W, Whi, Wlo, ldw);
     }
     PhaseTimer tm(c, PH_K3);
@@ -1065,6 +1081,7 @@ void dfk_mo_f32(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K,
     const ExchangePlan& plan = tune_f32(c, t, G, w, W, ldw, tl, ntile, nch, spc, partial);
     {
       PhaseTimer tm(c, PH_K1);
+      if (w < per) zero_w(c, W, nW * sizeof(float));   // see dfk_mo_bf16
       slab_gemm_f32dl(c, t, plan.k1, G, w, W, ldw);
     }
     PhaseTimer tm(c, PH_K3);
@@ -1227,6 +1244,7 @@ static void dfk_mo_tilde64_dl(Ctx& c, DevBT& t, const double* C, int ldc, int no
     }
     {
       PhaseTimer tm(c, PH_K1);
+      if (w < per) zero_w(c, W, nW * sizeof(double));   // see dfk_mo_bf16
       slab_gemm_f64dl(c, t, p.k1, G, w, W, ldw);
     }
     PhaseTimer tm(c, PH_K3);

@@ -45,12 +45,15 @@ __global__ void k_pair_scatter(const double* Jg, int nbf, const int* ncol, const
                                const int64_t* pg_off, double* J);
 __global__ void k_j_gamma(const double* bdata, const int64_t* slab_off, const int* ncol,
                           const int* bsz, const int64_t* pg_off, const double* Pg, int naux,
-                          double* gpart);
-__global__ void k_reduce_slabs(const double* gpart, int nslab, int naux, double scale, double* g);
+                          double* gpart, double* gpart_c);
+__global__ void k_reduce_slabs(const double* gpart, const double* gpart_c, int nslab, int naux, double scale,
+                               double* g);
+__global__ void k_symv_dot2(const double* A, const double* x, int n, double* y);
 __global__ void k_j_cols(const double* bdata, const int64_t* slab_off, const int* ncol,
                          const int* bsz, const int64_t* pg_off, const double* gq, int naux,
-                         int xchunk, int64_t pg_total, double* Jpart);
+                         int xchunk, int64_t pg_total, double* Jpart, double* Jpart_c);
 __global__ void k_reduce_chunks(const double* part, int nchunk, int64_t n, double* out);
+__global__ void k_reduce_chunks_dd(const double* part, const double* part_c, int nchunk, int64_t n, double* out);
 __global__ void k_fill(double* p, int64_t n, double v);
 __global__ void k_axpby(double* y, const double* x, int64_t n, double a, double b);
 

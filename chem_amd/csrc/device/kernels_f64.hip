@@ -2,8 +2,12 @@
 //
 //   k_reduce_sym       FP64 reduction of split-K partials (FP64 or FP32) into a symmetric K
 //   k_pair_gather / k_j_gamma / k_j_cols / k_pair_scatter
-//                      HBM-streaming DF-J (jkdf.cpp:111-170).
+//                      HBM-streaming DF-J (jkdf.cpp:111-170); every sum of
+//                      J (gamma = B.P, V^-1 gamma, J = B^T gamma~) is
+//                      compensated (dd.hpp), so J is as accurate as a
+//                      double-FP64 evaluation rounded once.
 // The MFMA contractions live in kernels_gemm.hip.
+#include "dd.hpp"
 #include "kernels.hpp"
 
 namespace mchip {
@@ -65,11 +69,12 @@ __global__ void k_pair_scatter(const double* __restrict__ Jg, int nbf, const int
   J[(int64_t)mu * nbf + boff[N] + nu] = Jg[pg_off[N] + e];
 }
 
-// gpart[N*naux + x] = sum_e slab_N[x][e] * Pg_N[e]; one wave per row, J_ROWS rows per block
+// gpart[N*naux + x] = sum_e slab_N[x][e] * Pg_N[e] as a compensated pair (sum
+// in gpart, rounding errors in gpart_c); one wave per row, J_ROWS rows per block
 __global__ __launch_bounds__(256) void k_j_gamma(
     const double* __restrict__ bdata, const int64_t* __restrict__ slab_off,
     const int* __restrict__ ncol, const int* __restrict__ bsz, const int64_t* __restrict__ pg_off,
-    const double* __restrict__ Pg, int naux, double* __restrict__ gpart)
+    const double* __restrict__ Pg, int naux, double* __restrict__ gpart, double* __restrict__ gpart_c)
 {
   const int N = blockIdx.y;
   const int R = bsz[N] * ncol[N];
@@ -80,30 +85,53 @@ __global__ __launch_bounds__(256) void k_j_gamma(
     const int x = blockIdx.x * J_ROWS + rr;
     if (x >= naux) break;
     const double* row = slab + (int64_t)x * R;
-    double s = 0.0;
-    for (int e = lane; e < R; e += 64) s += row[e] * pg[e];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0) gpart[(int64_t)N * naux + x] = s;
+    dsum s;
+    for (int e = lane; e < R; e += 64) dot2_add(s, row[e], pg[e]);
+    dsum_wave(s);
+    if (lane == 0) {
+      gpart[(int64_t)N * naux + x] = s.s;
+      gpart_c[(int64_t)N * naux + x] = s.c;
+    }
   }
 }
 
-// g[x] = scale * sum_N gpart[N*naux + x]   (fixed order)
-__global__ void k_reduce_slabs(const double* __restrict__ gpart, int nslab, int naux, double scale,
-                               double* __restrict__ g)
+// g[x] = scale * sum_N (gpart + gpart_c)[N*naux + x]  (compensated, fixed order;
+// scale is a power of two)
+__global__ void k_reduce_slabs(const double* __restrict__ gpart, const double* __restrict__ gpart_c, int nslab,
+                               int naux, double scale, double* __restrict__ g)
 {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= naux) return;
-  double s = 0.0;
-  for (int N = 0; N < nslab; ++N) s += gpart[(int64_t)N * naux + x];
-  g[x] = scale * s;
+  dsum s;
+  for (int N = 0; N < nslab; ++N) {
+    sum2_add(s, gpart[(int64_t)N * naux + x]);
+    s.c += gpart_c[(int64_t)N * naux + x];
+  }
+  g[x] = scale * dsum_value(s);
 }
 
-// Jpart[chunk][pg_off[N] + e] = sum_{x in chunk} gq[x] * slab_N[x][e]
+// y = A x for a symmetric n x n A (row r of A is column r), compensated; one
+// wave per row
+__global__ __launch_bounds__(256) void k_symv_dot2(const double* __restrict__ A, const double* __restrict__ x, int n,
+                                                   double* __restrict__ y)
+{
+  const int lane = threadIdx.x & 63;
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= n) return;
+  const double* row = A + (int64_t)r * n;
+  dsum s;
+  for (int e = lane; e < n; e += 64) dot2_add(s, row[e], x[e]);
+  dsum_wave(s);
+  if (lane == 0) y[r] = dsum_value(s);
+}
+
+// Jpart[chunk][pg_off[N] + e] = sum_{x in chunk} gq[x] * slab_N[x][e] as a
+// compensated pair (sums in Jpart, errors in Jpart_c)
 __global__ void k_j_cols(const double* __restrict__ bdata, const int64_t* __restrict__ slab_off,
                          const int* __restrict__ ncol, const int* __restrict__ bsz,
                          const int64_t* __restrict__ pg_off, const double* __restrict__ gq,
-                         int naux, int xchunk, int64_t pg_total, double* __restrict__ Jpart)
+                         int naux, int xchunk, int64_t pg_total, double* __restrict__ Jpart,
+                         double* __restrict__ Jpart_c)
 {
   const int N = blockIdx.y;
   const int chunk = blockIdx.z;
@@ -112,9 +140,25 @@ __global__ void k_j_cols(const double* __restrict__ bdata, const int64_t* __rest
   if (e >= R) return;
   const double* slab = bdata + slab_off[N];
   const int x0 = chunk * xchunk, x1 = min(naux, x0 + xchunk);
-  double s = 0.0;
-  for (int x = x0; x < x1; ++x) s += gq[x] * slab[(int64_t)x * R + e];
-  Jpart[(int64_t)chunk * pg_total + pg_off[N] + e] = s;
+  dsum s;
+  for (int x = x0; x < x1; ++x) dot2_add(s, gq[x], slab[(int64_t)x * R + e]);
+  Jpart[(int64_t)chunk * pg_total + pg_off[N] + e] = s.s;
+  Jpart_c[(int64_t)chunk * pg_total + pg_off[N] + e] = s.c;
+}
+
+// out[i] = sum_chunk (part + part_c)[chunk][i], compensated, fixed order
+__global__ void k_reduce_chunks_dd(const double* __restrict__ part, const double* __restrict__ part_c, int nchunk,
+                                   int64_t n, double* __restrict__ out)
+{
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    dsum s;
+    for (int c = 0; c < nchunk; ++c) {
+      sum2_add(s, part[(int64_t)c * n + i]);
+      s.c += part_c[(int64_t)c * n + i];
+    }
+    out[i] = dsum_value(s);
+  }
 }
 
 // Element-wise kernels below are grid-stride loops: launch them with

@@ -186,7 +186,12 @@ def make_blk_bounds(blksizes, nbatches, blkmap=None):
 def rcm_order(atoms, cutoff=5.0):
     """Reverse Cuthill-McKee ordering of atoms (rcm.hpp:20-230): connectivity by
     distance < cutoff (bohr); start from the lower-degree end of the farthest
-    pair not yet placed; neighbours queued by decreasing degree (stable)."""
+    pair not yet placed; neighbours queued by decreasing degree.  The
+    reference sorts each neighbour list with std::sort (rcm.hpp:121), which for
+    lists of at most 16 entries (libstdc++'s insertion-sort threshold; a water
+    cluster's 5-bohr lists are far shorter) keeps equal degrees in index
+    order: the stable sort here gives the same order for such lists.  Returns
+    the new order as indices into atoms (rcm::reordered_idx)."""
     n = len(atoms)
     xyz = np.array([a.xyz for a in atoms])
     dist = np.sqrt(((xyz[:, None, :] - xyz[None, :, :]) ** 2).sum(-1))
@@ -340,10 +345,15 @@ def water_cluster(n, seed, min_oo=2.8, box=None, max_tries=1000000):
 # molecule
 # ---------------------------------------------------------------------------
 class Molecule:
-    """Atoms + orbital/aux basis with atomic blocks (molecule.hpp:23-318)."""
+    """Atoms + orbital/aux basis with atomic blocks (molecule.hpp:23-318).
+
+    reorder (default on, as the reference's "reorder" key,
+    megalochem_driver.cpp:380-399): atoms in reverse Cuthill-McKee order of
+    their 5-bohr connectivity (rcm_order) before the basis is built, which fixes
+    the block order of every tensor."""
 
     def __init__(self, atoms, basis="cc-pvdz", dfbasis="cc-pvdz-jkfit", charge=0,
-                 mult=1, mo_split=5, reorder=False, name="mol"):
+                 mult=1, mo_split=5, reorder=True, name="mol"):
         if reorder:
             order = rcm_order(atoms)
             atoms = [atoms[i] for i in order]

@@ -382,17 +382,6 @@ int mchip_mp2_z(mchip_dbt t, mchip_metric m, int nbf, int nocc, const double* S,
                 const double* eps, int nlap, const double* omega, const double* alpha, int nbb, const int* bsz,
                 int mo_split, double filter_eps, int batched, double* Z, double* terms);
 
-/* Test hook: one v_mfma_f64_16x16x4_f64 on a single wave.  a_lane/b_lane hold
- * the operand each of the 64 lanes supplies, out[lane*4+e] the accumulator
- * elements (pins the lane layout the FP64 kernels assume). */
-int mchip_probe_mfma_f64(const double* a_lane, const double* b_lane, double* out);
-/* Test hook: kind 0 = v_mfma_f32_16x16x4_f32 (one float per lane and operand),
- * kind 1 = v_mfma_f32_16x16x32_bf16 (eight values per lane, rounded to bf16). */
-int mchip_probe_mfma_lp(int kind, const float* a_lane, const float* b_lane, float* out);
-/* Test hook: placement of global_load_lds_dwordx4 (64 lanes load src[perm[lane]],
- * 16 bytes each, to an LDS base); out = the 128 x 16-byte LDS window around it. */
-int mchip_probe_lds_load(const uint32_t* src, const int* perm, uint32_t* out);
-
 #ifdef __cplusplus
 }
 #endif

@@ -86,6 +86,42 @@ def df_j(B, Vinv, P_alpha, P_beta=None, bsz=None, nbatches_b=5, blkmap=None):
     return J
 
 
+def df_j_ext(B, Vinv, P_alpha, P_beta=None, x_rows=None):
+    """DF_J::compute_J (jkdf.cpp:69-190) with every sum carried in x86 extended
+    precision (numpy longdouble, 64-bit significand) and J rounded to FP64 once:
+    gamma = B.P_tot (:111-128), gq(Y) = sum_X gamma(X) V^-1(X,Y) (:137-141),
+    J = sum_X gq(X) B(X|..) (:153-170).  Summation order then no longer matters
+    (the reference's batch loops only reorder these sums), so this is the
+    order-free value the reference's FP64 sums scatter around; V^-1 amplifies
+    that scatter by up to cond(V).
+
+    B is a dense [naux, nbf, nbf] array, or a callable (x0, x1) -> the dense
+    rows x0..x1-1 with x_rows giving (naux, rows per call), for tensors too
+    large to hold dense."""
+    ld = np.longdouble
+    ptot = (2.0 * P_alpha if P_beta is None else P_alpha + P_beta).astype(ld)
+    if callable(B):
+        naux, step = x_rows
+        rows = lambda x0, x1: B(x0, x1)
+    else:
+        naux, step = B.shape[0], max(1, B.shape[0])
+        rows = lambda x0, x1: B[x0:x1]
+    nbf = ptot.shape[0]
+    gamma = np.zeros(naux, dtype=ld)
+    for x0 in range(0, naux, step):
+        x1 = min(naux, x0 + step)
+        b = rows(x0, x1).reshape(x1 - x0, -1).astype(ld)
+        gamma[x0:x1] = b @ ptot.reshape(-1)
+    gq = gamma.astype(np.float64).astype(ld) @ np.asarray(Vinv, dtype=ld)
+    gq = gq.astype(np.float64).astype(ld)
+    J = np.zeros(nbf * nbf, dtype=ld)
+    for x0 in range(0, naux, step):
+        x1 = min(naux, x0 + step)
+        b = rows(x0, x1).reshape(x1 - x0, -1).astype(ld)
+        J += gq[x0:x1] @ b
+    return J.astype(np.float64).reshape(nbf, nbf)
+
+
 def dfmo_k(B, invsqrt, C_occ, bsz=None, xsz=None, osz=None, occ_nbatches=1,
            nbatches_b=5, nbatches_x=5, bmap=None, xmap=None):
     """DFMO_K::compute_K for one spin (jkdf.cpp:221-462); returns K (negated)."""

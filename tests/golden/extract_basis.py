@@ -3,9 +3,9 @@
 The reference keeps Basis Set Exchange JSON files in ``basis/*.json``
 (read by ``src/desc/basis.cpp:278-360``).  The GPU host has no copy of the
 reference, so the three sets used by the DF-HF / SOS-MP2 path are trimmed to
-elements H..Ne and committed next to this script as data.
+elements H..Ne and committed as data under chem_amd/basis/.
 
-Usage: python chem_amd/basis/extract_basis.py /root/reference/basis
+Usage: python tests/golden/extract_basis.py /root/reference/basis
 """
 import json
 import os
@@ -16,7 +16,7 @@ ELEMENTS = [str(z) for z in range(1, 11)]
 
 
 def main(src):
-    here = os.path.dirname(os.path.abspath(__file__))
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "..", "chem_amd", "basis")
     for name in SETS:
         with open(os.path.join(src, name + ".json")) as f:
             data = json.load(f)

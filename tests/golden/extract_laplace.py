@@ -4,7 +4,7 @@ on [1, R] that the reference's Laplace module reads
 http://www.mis.mpg.de/scicomp/EXP_SUM/1_x/), orders k = 1..12, into
 chem_amd/data/laplace_init.json.
 
-Usage: python chem_amd/data/extract_laplace.py /root/reference/src/math/laplace/data
+Usage: python tests/golden/extract_laplace.py /root/reference/src/math/laplace/data
 """
 import json
 import os
@@ -20,7 +20,7 @@ def parse_token_range(tok):
 
 
 def main(src):
-    here = os.path.dirname(os.path.abspath(__file__))
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "..", "chem_amd", "data")
     errs = {}
     for line in open(os.path.join(src, "init_error.txt")):
         if line.startswith("1_xk"):

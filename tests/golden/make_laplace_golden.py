@@ -9,8 +9,8 @@ import os
 import sys
 
 here = os.path.dirname(os.path.abspath(__file__))
-sys.path.insert(0, os.path.join(here, "..", ".."))
-from chem_amd.data.extract_laplace import parse_token_range  # noqa: E402
+sys.path.insert(0, here)
+from extract_laplace import parse_token_range  # noqa: E402
 
 
 def main(src):

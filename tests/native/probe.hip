@@ -1,9 +1,27 @@
-// Lane-layout probe of the FP64 MFMA builtin (one wave, fixed 16x4 and 4x16
-// operands).  Used by the tests to pin the operand/accumulator layout the
-// kernels assume.
-#include "../../../include/mchip.h"
-#include "../capi_common.hpp"
-#include "dev.hpp"
+// Test-only probes of the gfx950 builtins the kernels of libmchip.so rely on:
+// the lane layouts of the FP64, FP32 and BF16 MFMA instructions and the LDS
+// placement of global_load_lds_dwordx4.  Built by the package build into
+// tests/native/libmchip_probe.so, which only the tests load; nothing of it is
+// part of libmchip.so.  Each entry point returns 0, or 1 with a message for
+// mchip_probe_last_error().
+#include <stdexcept>
+#include <string>
+
+#include "device/dev.hpp"
+
+static thread_local std::string g_probe_err;
+
+extern "C" const char* mchip_probe_last_error(void) { return g_probe_err.c_str(); }
+
+#define MCHIP_TRY try {
+#define MCHIP_CATCH                  \
+  }                                  \
+  catch (const std::exception& e)    \
+  {                                  \
+    g_probe_err = e.what();          \
+    return 1;                        \
+  }                                  \
+  return 0;
 
 namespace mchip {
 

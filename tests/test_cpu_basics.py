@@ -25,7 +25,11 @@ def test_water_geometry_and_dims():
     atoms = desc.water_cluster(1, 0)
     mol = desc.Molecule(atoms)
     assert (mol.nbf, mol.naux, mol.nocc) == (24, 116, 5)
-    assert mol.b == [14, 5, 5] and mol.x == [70, 23, 23]
+    # RCM (on by default, as in the reference): all three atoms are within 5
+    # bohr of each other with degree 2; the farthest pair is H1-H2, the start
+    # H1, then O and H2 in index order: R = [1, 0, 2], reversed H2, O, H1
+    assert mol.b == [5, 14, 5] and mol.x == [23, 70, 23]
+    assert desc.Molecule(atoms, reorder=False).b == [14, 5, 5]
     d = np.linalg.norm(atoms[1].xyz - atoms[0].xyz) * desc.BOHR_RADIUS
     assert abs(d - 0.9572) < 1e-12
     v1, v2 = atoms[1].xyz - atoms[0].xyz, atoms[2].xyz - atoms[0].xyz
@@ -203,3 +207,60 @@ def test_product_code_does_not_use_the_oracle():
     assert len(body) == 2
     outside = body[0] + "\n".join(l for l in body[1].split("\ndef ", 1)[1:])
     assert not re.search(r"^\s*(from\s+oracle\b|import\s+oracle\b)", outside, re.M)
+
+
+def _atoms_at(xs):
+    from chem_amd.desc import Atom
+    return [Atom(1, x, 0.0, 0.0) for x in xs]
+
+
+def test_rcm_order_hand_checked_chain():
+    """Four atoms on a line 3 bohr apart (5-bohr cutoff: nearest neighbours
+    only), given in the scrambled order x = 6, 0, 9, 3.  By hand
+    (rcm.hpp:137-210): the farthest pair is (1, 2) (x = 0 and 9), both of
+    degree 1, so the start is node 1; its neighbour 3; then 3's neighbours by
+    decreasing degree [0 (deg 2), 1]; then 0's [3, 2]; R = [1, 3, 0, 2],
+    reversed [2, 0, 3, 1]: the atoms come out sorted by decreasing x."""
+    from chem_amd.desc import rcm_order
+    atoms = _atoms_at([6.0, 0.0, 9.0, 3.0])
+    assert rcm_order(atoms) == [2, 0, 3, 1]
+
+
+def test_rcm_order_ties_keep_index_order():
+    """A centre with four arms (4 bohr from the centre, 5.7 bohr or more from
+    each other) and a tail atom linked to arm 4 only: three arms tie in degree
+    and enter the queue in index order (std::sort on at most 16 entries keeps
+    ties), so the order is fixed by the tie rule."""
+    from chem_amd.desc import Atom, rcm_order
+    atoms = [Atom(1, 0.0, 0.0, 0.0),          # 0 centre
+             Atom(1, 4.0, 0.0, 0.0),          # 1 arm +x
+             Atom(1, -4.0, 0.0, 0.0),         # 2 arm -x
+             Atom(1, 0.0, 4.0, 0.0),          # 3 arm +y
+             Atom(1, 0.0, -4.0, 0.0),         # 4 arm -y
+             Atom(1, 0.0, -8.5, 0.0)]         # 5 tail of arm 4
+    # degrees: centre 4, arms 1 except arm 4 (2), tail 1.  Farthest pair not
+    # placed: (3, 5) at 12.5 bohr, both degree 1 -> start at 3; its neighbour 0;
+    # 0's neighbours by decreasing degree: 4 (deg 2), then 1, 2, 3 in index
+    # order; 4's neighbours [0, 5]: R = [3, 0, 4, 1, 2, 5], reversed below.
+    assert rcm_order(atoms) == [5, 2, 1, 4, 0, 3]
+
+
+def test_rcm_reduces_water_cluster_bandwidth():
+    """On seeded water clusters the reordered 5-bohr connectivity has a
+    bandwidth no larger than in the generator's order, and the order is a
+    permutation that the Molecule applies by default (the reference's
+    "reorder": true)."""
+    from chem_amd import desc
+    for n, seed in ((8, 1), (16, 2), (32, 3)):
+        atoms = desc.water_cluster(n, seed)
+        order = desc.rcm_order(atoms)
+        assert sorted(order) == list(range(len(atoms)))
+
+        def bandwidth(idx):
+            xyz = np.array([atoms[i].xyz for i in idx])
+            d = np.sqrt(((xyz[:, None] - xyz[None]) ** 2).sum(-1))
+            i, j = np.nonzero(d < 5.0)
+            return int(np.abs(i - j).max())
+        assert bandwidth(order) <= bandwidth(range(len(atoms)))
+        mol = desc.Molecule(atoms)
+        assert [a.xyz.tolist() for a in mol.atoms] == [atoms[i].xyz.tolist() for i in order]

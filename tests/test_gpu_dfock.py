@@ -518,3 +518,58 @@ def test_bf16_every_pipeline(ctx, nocc):
             print("  relerr %.3e" % err)
             assert err < BF16_TOL
             assert np.array_equal(K, K.T)
+
+
+def _synthetic_B(rng, bsz, xsz, density=0.6):
+    nbf, naux = sum(bsz), sum(xsz)
+    mask = (rng.random((len(xsz), len(bsz), len(bsz))) < density).astype(np.uint8)
+    mask = np.maximum(mask, mask.transpose(0, 2, 1))
+    boff = np.concatenate([[0], np.cumsum(bsz)])
+    xoff = np.concatenate([[0], np.cumsum(xsz)])
+    dense = rng.standard_normal((naux, nbf, nbf))
+    dense = 0.5 * (dense + dense.transpose(0, 2, 1))
+    for x in range(len(xsz)):
+        for a in range(len(bsz)):
+            for b in range(len(bsz)):
+                if not mask[x, a, b]:
+                    dense[xoff[x]:xoff[x + 1], boff[a]:boff[a + 1], boff[b]:boff[b + 1]] = 0
+    A = rng.standard_normal((naux, naux))
+    V = A @ A.T + naux * np.eye(naux)
+    return ints.BTensorHost.from_dense(dense, xsz, bsz, mask), dense, V
+
+
+@pytest.mark.parametrize("nocc,nbatch", [(129, 2), (193, 3), (97, 2)])
+def test_batched_exchange_crosses_tile_edge(ctx, nocc, nbatch):
+    """Occupied batches whose last batch is narrower than the others by enough
+    that its column tiles end before the work array's padded width (129 = 65 +
+    64: BN 64 tiles of the last batch stop at column 64 of a 72/80/96-wide W).
+    Every form must zero those columns itself, or the previous batch's
+    orbitals are counted twice.  Each batched K equals its unbatched K."""
+    from chem_amd.engine import DeviceB, Metric
+    rng = np.random.default_rng(1300 + nocc)
+    bsz = [14, 5, 5, 14, 5, 5, 14, 5, 5, 14, 5, 5, 14, 5, 5, 14, 5, 5, 14, 5, 5, 14, 5, 5, 14, 5]
+    xsz = [40, 17, 17, 40, 17]
+    Bh, dense, V = _synthetic_B(rng, bsz, xsz)
+    nbf = sum(bsz)
+    assert nbf >= nocc
+    C = rng.standard_normal((nbf, nocc)) * 0.3
+    P = C @ C.T
+    dB = DeviceB(ctx, Bh)
+    m = Metric(ctx, V)
+    dB.make_lowp(m, f32=True, bf16=True, f64=True)
+    Kplain = dB.dfk_mo(C, m)
+    Linv, _ = m.get()
+    Kref = dfock.dfmo_k(dense, Linv.T, C)
+    assert _relerr(Kplain, Kref) < _jtol(V)
+    forms = {"plain": (lambda nb: dB.dfk_mo(C, m, occ_nbatches=nb), 1e-12),
+             "fp64_copy": (lambda nb: dB.dfk_mo_tilde64(C, occ_nbatches=nb), 1e-12),
+             "f32": (lambda nb: dB.dfk_mo_f32(C, occ_nbatches=nb), 2e-6),
+             "bf16x3": (lambda nb: dB.dfk_mo_bf16(C, occ_nbatches=nb), BF16_TOL),
+             "ao_fp64": (lambda nb: dB.dfk_ao(P, m, occ_nbatches=nb), 1e-10)}
+    for name, (fn, tol) in forms.items():
+        print("case: batched exchange nocc %d nbatch %d form %s" % (nocc, nbatch, name), flush=True)
+        K1 = fn(1)
+        Kb = fn(nbatch)
+        e1, eb, ebb = _relerr(K1, Kref), _relerr(Kb, Kref), _relerr(Kb, K1)
+        print("  relerr unbatched %.2e batched %.2e batched vs unbatched %.2e" % (e1, eb, ebb))
+        assert e1 < tol and eb < tol and ebb < tol

@@ -8,6 +8,33 @@ from chem_amd import _lib
 pytestmark = pytest.mark.gpu
 
 
+class _Probe:
+    """The test-only probe library tests/native/libmchip_probe.so (built with
+    the package); the probes are not part of libmchip.so."""
+
+    _h = None
+
+    def __getattr__(self, name):
+        import ctypes
+        import os
+        if _Probe._h is None:
+            path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "native", "libmchip_probe.so")
+            if not os.path.exists(path):
+                raise _lib.MchipError("%s not built; run __graft_entry__.build()" % path)
+            h = ctypes.CDLL(path)
+            h.mchip_probe_last_error.restype = ctypes.c_char_p
+            _Probe._h = h
+        return getattr(_Probe._h, name)
+
+
+_probe = _Probe()
+
+
+def _check(rc):
+    if rc != 0:
+        raise _lib.MchipError(_probe.mchip_probe_last_error().decode())
+
+
 def test_mfma_f64_16x16x4_layout():
     rng = np.random.default_rng(0)
     A = rng.integers(-8, 8, size=(16, 4)).astype(np.float64)
@@ -15,7 +42,7 @@ def test_mfma_f64_16x16x4_layout():
     a_lane = np.array([A[l % 16, l // 16] for l in range(64)])
     b_lane = np.array([B[l // 16, l % 16] for l in range(64)])
     out = np.zeros(256)
-    _lib.check(_lib.lib().mchip_probe_mfma_f64(_lib.dptr(a_lane), _lib.dptr(b_lane), _lib.dptr(out)))
+    _check(_probe.mchip_probe_mfma_f64(_lib.dptr(a_lane), _lib.dptr(b_lane), _lib.dptr(out)))
     D = A @ B
     got = np.zeros((16, 16))
     for l in range(64):
@@ -45,7 +72,7 @@ def _observe(kind, per):
             a_lane[l * per + q] = A[l % 16, per * (l // 16) + q]
             b_lane[l * per + q] = B[per * (l // 16) + q, l % 16]
     out = np.zeros(256, np.float32)
-    _lib.check(_lib.lib().mchip_probe_mfma_lp(kind, _lib.fptr(a_lane), _lib.fptr(b_lane), _lib.fptr(out)))
+    _check(_probe.mchip_probe_mfma_lp(kind, _lib.fptr(a_lane), _lib.fptr(b_lane), _lib.fptr(out)))
     return A.astype(np.float64) @ B.astype(np.float64), out
 
 
@@ -76,7 +103,7 @@ def test_global_load_lds_placement():
     perm = rng.permutation(64).astype(np.int32)
     out = np.zeros((128, 4), dtype=np.uint32)
     u32 = ctypes.POINTER(ctypes.c_uint32)
-    _lib.check(_lib.lib().mchip_probe_lds_load(src.ctypes.data_as(u32), _lib.iptr(perm), out.ctypes.data_as(u32)))
+    _check(_probe.mchip_probe_lds_load(src.ctypes.data_as(u32), _lib.iptr(perm), out.ctypes.data_as(u32)))
     assert np.array_equal(out[:64], np.zeros((64, 4), np.uint32))
     assert np.array_equal(out[64:], src[perm])
 
@@ -105,7 +132,7 @@ def test_mfma_f32_rounding():
         b_lane[l] = B[l // 16, l % 16]
     assert np.array_equal(a_lane.astype(np.float64)[:64].reshape(4, 16).T, A.astype(np.float32))
     out = np.zeros(256, np.float32)
-    _lib.check(_lib.lib().mchip_probe_mfma_lp(0, _lib.fptr(a_lane), _lib.fptr(b_lane), _lib.fptr(out)))
+    _check(_probe.mchip_probe_mfma_lp(0, _lib.fptr(a_lane), _lib.fptr(b_lane), _lib.fptr(out)))
     got = np.zeros((16, 16))
     for l in range(64):
         for e in range(4):

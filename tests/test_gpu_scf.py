@@ -1,13 +1,19 @@
 """GPU restricted HF against the oracle SCF (same integrals, same metric
 factors).
 
-Bar: |E_gpu - E_oracle| < 1e-9 Eh.  The reference's DF-J evaluates
-gamma^T V^-1 gamma with an explicit V^-1 (jkdf.cpp:137-141, LLT::inverse); for
-one water with cc-pVDZ-jkfit cond(V) = 2.4e6 and one-ulp changes of gamma
-(any change of summation order) move E_J by 1e-10 - 3e-10 Eh, shown in
-tests/test_oracle_conditioning.py.  Two correct FP64 implementations therefore
-cannot agree to 1e-10; the bar is set one decade above that floor.  The
-converged density satisfies P S P = P and tr(P S) = nocc."""
+Bar for the plain FP64 path (kmode dfmo): |E_gpu - E_oracle| < 1e-10 Eh.  The
+reference's DF-J evaluates gamma^T V^-1 gamma with an explicit V^-1
+(jkdf.cpp:137-141, LLT::inverse); for one water with cc-pVDZ-jkfit cond(V) =
+2.4e6, and one-ulp changes of gamma (any change of summation order) move E_J
+by 1e-10 - 3e-10 Eh (tests/test_oracle_conditioning.py).  The GPU therefore
+carries every J sum compensated, and the oracle carries them in extended
+precision (dfock.df_j_ext): both are then the order-free value, and their
+energies agree far inside 1e-10 Eh.
+
+The transformed route (J = B~^T (B~ . P) with B~ = L^-1 B, no explicit V^-1)
+is a different association of the same algebra, so it moves the energy by the
+cond(V) floor itself; its bar stays at 1e-9 Eh.  The converged density
+satisfies P S P = P and tr(P S) = nocc."""
 import numpy as np
 import pytest
 
@@ -40,12 +46,12 @@ def test_rhf_energy_vs_oracle(ctx, nwater, kmode):
     res = run_rhf(mol, dB, m, S, T + V, threshold=1e-8, kmode=kmode)
     assert res.converged
     Bd = Bh.dense()
-    jk = lambda P, C: (dfock.df_j(Bd, Vinv, P, bsz=mol.b),
+    jk = lambda P, C: (dfock.df_j_ext(Bd, Vinv, P),
                        dfock.dfmo_k(Bd, Linv.T, C, bsz=mol.b, xsz=mol.x, osz=mol.oa()))
     ref = oscf.rhf(S, T, V, mol.nuclear_repulsion(), mol.nocc, jk, bsz=mol.b, thr=1e-8)
     print("E gpu %.12f oracle %.12f diff %.2e iters %d/%d" % (
         res.energy, ref["energy"], res.energy - ref["energy"], res.iterations, ref["iterations"]))
-    assert abs(res.energy - ref["energy"]) < 1e-9
+    assert abs(res.energy - ref["energy"]) < (1e-10 if kmode == "dfmo" else 1e-9)
     C = res.C[:, :mol.nocc]
     P = C @ C.T
     assert np.abs(P @ S @ P - P).max() < 1e-8

@@ -135,3 +135,57 @@ def test_orthogonalizer_and_filter():
     M[2:, 2:] = 1e-9
     f = oscf.block_filter(M, [2, 2], 1e-10)
     assert not f[:2, :2].any() and f[2:, 2:].any()
+
+
+def _exact_round(fr):
+    """Nearest double to a Fraction (float() of a Fraction rounds correctly)."""
+    return float(fr)
+
+
+def test_df_j_ext_matches_exact_rational():
+    """The extended-precision DF-J equals exact rational arithmetic (rounded to
+    FP64 at the same three points: gamma, V^-1 gamma, J) to within one ulp of
+    each J element, on a tiny ill-conditioned case where plain FP64 sums in a
+    different order disagree far beyond that."""
+    from fractions import Fraction as F
+    rng = np.random.default_rng(5)
+    naux, nbf = 7, 4
+    B = rng.standard_normal((naux, nbf, nbf))
+    B = 0.5 * (B + B.transpose(0, 2, 1))
+    A = rng.standard_normal((naux, naux))
+    # nearly singular metric: V^-1 has entries of order 1e8
+    w = np.array([1e-8, 1e-6, 1e-3, 1, 2, 3, 4])
+    Q, _ = np.linalg.qr(A)
+    V = (Q * w) @ Q.T
+    Vinv = (Q / w) @ Q.T
+    Vinv = 0.5 * (Vinv + Vinv.T)
+    C = rng.standard_normal((nbf, 2))
+    P = C @ C.T
+    J = dfock.df_j_ext(B, Vinv, P)
+    Pt = [[F(2 * P[m, n]) for n in range(nbf)] for m in range(nbf)]
+    gamma = [_exact_round(sum(F(B[x, m, n]) * Pt[m][n] for m in range(nbf) for n in range(nbf)))
+             for x in range(naux)]
+    gq = [_exact_round(sum(F(gamma[x]) * F(Vinv[x, y]) for x in range(naux))) for y in range(naux)]
+    Jx = np.array([[_exact_round(sum(F(gq[x]) * F(B[x, m, n]) for x in range(naux))) for n in range(nbf)]
+                   for m in range(nbf)])
+    ulp = np.spacing(np.abs(Jx))
+    assert np.all(np.abs(J - Jx) <= ulp)
+    # the plain FP64 restatement misses that bound here (the check is sharp)
+    Jp = dfock.df_j(B, Vinv, P)
+    assert (np.abs(Jp - Jx) > ulp).any()
+
+
+def test_df_j_ext_row_batches():
+    """The callable (X-row batch) form agrees with the dense one to one ulp
+    (only the extended-precision association of the X sum differs), and with
+    the plain FP64 restatement to its cond(V) bound."""
+    mol, B, V = _water_system(1, 0)
+    Vinv, _, _ = dfock.metric_factors(V)
+    rng = np.random.default_rng(3)
+    C = rng.standard_normal((mol.nbf, mol.nocc)) * 0.2
+    P = C @ C.T
+    J = dfock.df_j_ext(B, Vinv, P)
+    Jb = dfock.df_j_ext(lambda x0, x1: B[x0:x1], Vinv, P, x_rows=(B.shape[0], 13))
+    assert np.all(np.abs(J - Jb) <= np.spacing(np.abs(J)))
+    Jp = dfock.df_j(B, Vinv, P, bsz=mol.b)
+    assert np.abs(J - Jp).max() < 10 * np.linalg.cond(V) * 2.2e-16 * np.abs(J).max()
