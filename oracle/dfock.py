@@ -147,6 +147,24 @@ def dfmo_k(B, invsqrt, C_occ, bsz=None, xsz=None, osz=None, occ_nbatches=1,
     return -K
 
 
+def dfmo_k_rows(rows, naux, step, Linv, C_occ):
+    """DFMO_K::compute_K (jkdf.cpp:221-462) for one spin with B supplied in
+    X-row batches, for tensors too large to hold dense: HT1 = B C
+    ("XMN, Mi -> XiN", :361), HT2 = L^-1 HT1 ("XiN, XY -> YiN" with
+    invsqrt = L^-T, :384), K = -HT2^T HT2 ("XiM, XiN -> MN", :417).
+    rows(x0, x1) returns the dense rows x0..x1-1 of B; Linv is the lower
+    triangular L^-1 (row-major).  Returns K (negated)."""
+    nbf, nocc = C_occ.shape
+    HT1 = np.empty((naux, nocc, nbf))
+    Ct = np.ascontiguousarray(C_occ.T)
+    for x0 in range(0, naux, step):
+        x1 = min(naux, x0 + step)
+        HT1[x0:x1] = np.matmul(Ct, rows(x0, x1))          # sum_m C(m,i) B(x|m n)
+    HT2 = (np.asarray(Linv) @ HT1.reshape(naux, -1)).reshape(naux * nocc, nbf)
+    del HT1
+    return -(HT2.T @ HT2)
+
+
 def fitting_coefficients(B, Vinv):
     """dfitting::compute: C_fit(X|mn) = sum_Y Vinv(X,Y) B(Y|mn) (fitting.cpp:124-128)."""
     naux, nbf, _ = B.shape

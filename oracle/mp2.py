@@ -124,31 +124,57 @@ def shellpair_mask(Bmask, bsz):
     return m
 
 
+def point_factors(S, C, eps, nocc, w, a, bsz, mo_split=5, filter_eps=1e-10):
+    """The pseudo-density factors of one Laplace point (mp_compute.cpp:232-302):
+    L_o = L_S CD(omega^(1/4) L_S^-1 C_o' C_o'^T L_S^-T) and P_v = omega^(1/4)
+    C_v' C_v'^T, both block-filtered.  Returns (L_o, P_v, rank)."""
+    Co, Cv = C[:, :nocc], C[:, nocc:]
+    eo, ev = eps[:nocc], eps[nocc:]
+    LS = np.linalg.cholesky(S)
+    LSinv = np.linalg.solve(LS, np.eye(S.shape[0]))
+    Coe = Co * np.exp(0.5 * eo * a)[None, :]
+    Cve = Cv * np.exp(-0.5 * ev * a)[None, :]
+    cortho = LSinv @ Coe
+    Po_t = w ** 0.25 * cortho @ cortho.T
+    Pv = w ** 0.25 * Cve @ Cve.T
+    Lt, rank = pivoted_cholesky(Po_t, filter_eps=filter_eps)
+    Lo = block_filter(LS @ Lt, bsz, split_range(rank, mo_split), filter_eps)
+    Pv = block_filter(Pv, bsz, bsz, filter_eps)
+    return Lo, Pv, rank
+
+
+def llmp_z_rows(rows, naux, step, Lo, Pv, pair_mask, xs):
+    """Rows xs of LLMP_FULL_Z's Z (z_full.cpp:69-397) with B supplied in X-row
+    batches (rows(x0, x1) -> dense rows): b3(X) = pair_mask * (L_o L_o^T B(X)
+    P_v) for X in xs (b = L_o^T B, b2 = P_v b, b3 = L_o b2 restricted to B's
+    shell-pair pattern), then Z(X, Y) = sum_mn b3(X|mn) B(Y|mn) for every Y."""
+    xs = list(xs)
+    nbf = Lo.shape[0]
+    LL = Lo @ Lo.T
+    b3 = np.empty((len(xs), nbf * nbf))
+    for k, x in enumerate(xs):
+        b3[k] = (pair_mask * (LL @ rows(x, x + 1)[0] @ Pv.T)).reshape(-1)
+    Z = np.empty((len(xs), naux))
+    for y0 in range(0, naux, step):
+        y1 = min(naux, y0 + step)
+        Z[:, y0:y1] = b3 @ rows(y0, y1).reshape(y1 - y0, -1).T
+    return Z
+
+
 def sos_mp2(B, Vinv, S, C, eps, nocc, omega, alpha, bsz, mo_split=5, filter_eps=1e-10,
             Bmask=None, return_terms=False, return_z=False, zmethod="llmp_full"):
     """mpmod::compute.  Returns the positive sum E = sum_k e_k (unscaled);
     with return_terms also the e_k, with return_z also the Z_k of every point.
     zmethod "ll_full" factors the orthogonalised virtual pseudo-density too
     (mp_compute.cpp:309-336) and builds Z with ll_full_z."""
-    Co, Cv = C[:, :nocc], C[:, nocc:]
-    eo, ev = eps[:nocc], eps[nocc:]
-    LS = np.linalg.cholesky(S)
-    LSinv = np.linalg.solve(LS, np.eye(S.shape[0]))
     pm = shellpair_mask(Bmask, bsz) if Bmask is not None else None
-    nbf = S.shape[0]
-    nvir = nbf - nocc
     terms, Zs = [], []
     for w, a in zip(omega, alpha):
-        Coe = Co * np.exp(0.5 * eo * a)[None, :]
-        Cve = Cv * np.exp(-0.5 * ev * a)[None, :]
-        cortho = LSinv @ Coe
-        Po_t = w ** 0.25 * cortho @ cortho.T
-        Pv = w ** 0.25 * Cve @ Cve.T
-        Lt, rank = pivoted_cholesky(Po_t, filter_eps=filter_eps)
-        Lo = LS @ Lt
-        Lo = block_filter(Lo, bsz, split_range(rank, mo_split), filter_eps)
-        Pv = block_filter(Pv, bsz, bsz, filter_eps)
+        Lo, Pv, rank = point_factors(S, C, eps, nocc, w, a, bsz, mo_split, filter_eps)
         if zmethod == "ll_full":
+            LS = np.linalg.cholesky(S)
+            LSinv = np.linalg.solve(LS, np.eye(S.shape[0]))
+            Cve = C[:, nocc:] * np.exp(-0.5 * eps[nocc:] * a)[None, :]
             cvo = LSinv @ Cve
             Lvt, rank_v = pivoted_cholesky(w ** 0.25 * cvo @ cvo.T, filter_eps=filter_eps)
             Lv = block_filter(LS @ Lvt, bsz, split_range(rank_v, mo_split), filter_eps)

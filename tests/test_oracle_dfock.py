@@ -189,3 +189,14 @@ def test_df_j_ext_row_batches():
     assert np.all(np.abs(J - Jb) <= np.spacing(np.abs(J)))
     Jp = dfock.df_j(B, Vinv, P, bsz=mol.b)
     assert np.abs(J - Jp).max() < 10 * np.linalg.cond(V) * 2.2e-16 * np.abs(J).max()
+
+
+def test_dfmo_k_rows_matches_dfmo_k():
+    """The X-row-batched DFMO restatement (for tensors too large to hold
+    dense) equals the batched dense one to rounding."""
+    mol, B, V = _water_system(2, 3)
+    _, Linv, invsqrt = dfock.metric_factors(V)
+    C = np.random.default_rng(4).standard_normal((mol.nbf, mol.nocc)) * 0.2
+    K = dfock.dfmo_k(B, invsqrt, C, bsz=mol.b, xsz=mol.x, osz=mol.oa(), occ_nbatches=3)
+    Kr = dfock.dfmo_k_rows(lambda x0, x1: B[x0:x1], B.shape[0], 17, Linv, C)
+    assert np.abs(Kr - K).max() < 1e-13 * np.abs(K).max()

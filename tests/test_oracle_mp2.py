@@ -133,3 +133,24 @@ def test_ll_full_energy_matches_llmp(water_hf):
     E1 = mp2.sos_mp2(B, Vinv, S, C, eps, nocc, w, a, mol.b, Bmask=mask)
     E2 = mp2.sos_mp2(B, Vinv, S, C, eps, nocc, w, a, mol.b, Bmask=mask, zmethod="ll_full")
     assert abs(E1 - E2) < 1e-9 * E1, (E1, E2)
+
+
+def test_llmp_z_rows_and_point_factors(water_hf):
+    """The row-sampled LLMP_FULL Z (B streamed in X-row batches) equals the
+    dense restatement's rows, and the per-point factors reproduce the energy
+    term of sos_mp2 for that point (same pseudo-density code path)."""
+    mol, S, B, mask, Vinv, r = water_hf
+    C, eps, nocc = r["C"], r["eps"], mol.nocc
+    ymin = 2 * (eps[nocc] - eps[nocc - 1])
+    ymax = 2 * (eps[-1] - eps[0])
+    w, a, _ = laplace.laplace_points(4, ymin, ymax)
+    pm = mp2.shellpair_mask(mask, mol.b)
+    Lo, Pv, rank = mp2.point_factors(S, C, eps, nocc, w[1], a[1], mol.b)
+    assert rank == nocc
+    Z = mp2.llmp_full_z(B, Lo, Pv, pm)
+    xs = [0, 7, len(B) - 1]
+    Zr = mp2.llmp_z_rows(lambda x0, x1: B[x0:x1], B.shape[0], 13, Lo, Pv, pm, xs)
+    assert np.abs(Zr - Z[xs]).max() < 1e-13 * np.abs(Z).max()
+    _, terms = mp2.sos_mp2(B, Vinv, S, C, eps, nocc, w, a, mol.b, Bmask=mask, return_terms=True)
+    Zt = Z @ Vinv
+    assert abs(np.sum(Zt * Zt.T) - terms[1]) < 1e-13 * terms[1]
