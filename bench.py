@@ -135,6 +135,25 @@ def measured_mfma_util():
     return out
 
 
+def exact_kernels(form, plan):
+    """Template names (as the profiler prints them, spaces removed) of the slab
+    GEMM and SYRK an autotuned exchange plan runs: the keys of the PMC lines."""
+    if not plan:
+        return None, None
+    wn = plan["threads"] // 128
+    tm, tn = plan["bm"] // 32, plan["bn"] // (16 * wn)
+    if form == "bf16x3":
+        base = {0: "k_slab_gemm_bf16x3", 1: "k_slab_gemm_bf16x3_dl", 2: "k_slab_gemm_bf16x3_dla"}[plan["pipeline"]]
+        k1 = "%s<2,%d,%d,%d,%d>" % (base, wn, tm, tn, plan["stages"])
+        v = plan["syrk_variant"]
+        k3 = "k_seg_syrk_bf16x3<128>" if v == 0 else (
+            "k_seg_syrk_bf16x3_dl<2,256>" if v == 4 else "k_seg_syrk_bf16x3_dl<%d,128>" % (v + 1))
+    else:
+        k1 = "k_slab_gemm_%s_dl<2,%d,%d,%d,%d>" % (form, wn, tm, tn, plan["stages"])
+        k3 = "k_seg_syrk_%s_dl<%d>" % (form, (plan["syrk_stages"] if form == "f32" else plan["syrk_variant"] + 2))
+    return k1, k3
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -266,14 +285,15 @@ def mp2_batched_bench(args):
 
 
 def kernels_only(args):
-    """Exchange builds alone, for profiling: B, metric and the low-precision
-    copies of the 64-water cluster, then `steps` BF16x3 builds, `steps` FP32
-    builds and one plain FP64 build at orthonormalised random occupied
-    coefficients (the kernels' work does not depend on the orbitals)."""
+    """The chosen form's exchange builds alone, for profiling: B, metric and
+    the copies of L^-1 B of the 64-water cluster, then `steps` BF16x3, FP32 and
+    FP64-copy builds each at orthonormalised random occupied coefficients (the
+    kernels' work does not depend on the orbitals).  Everything in this mode
+    runs on the library's own kernels (no vendor BLAS call)."""
     from chem_amd import desc, ints
     from chem_amd.engine import Context, DeviceB, Metric
     cfg = CONFIGS[args.config if args.config in ("c1", "c2") else "c2"]
-    mol = desc.Molecule(desc.water_cluster(cfg["nwater"], cfg["seed"]))
+    mol = desc.Molecule(desc.water_cluster(cfg["nwater"], cfg["seed"]), reorder=not args.no_reorder)
     ctx = Context(0)
     dB = DeviceB.build_on_gpu(ctx, mol, 1e-10)
     met = Metric(ctx, ints.int2c2e(mol))
@@ -294,16 +314,12 @@ def kernels_only(args):
         ctx.sync()
         out[name + "_build_s"] = (time.perf_counter() - t) / steps
         log("%s build %.4f s" % (name, out[name + "_build_s"]))
-    t = time.perf_counter()
-    K64 = dB.dfk_mo(C, met)
-    ctx.sync()
-    out["fp64_build_s"] = time.perf_counter() - t
+    K64 = dB.dfk_mo_tilde64(C)
     for name, fn in (("bf16x3", dB.dfk_mo_bf16), ("f32", dB.dfk_mo_f32)):
         out[name + "_frob_rel_vs_fp64"] = float(np.linalg.norm(fn(C) - K64) / np.linalg.norm(K64))
     out["bf16_plan"] = dB.bf16_plan(mol.nocc)
     out["f32_plan"] = dB.f32_plan(mol.nocc)
     out["f64_plan"] = dB.f64_plan(mol.nocc)
-    log("fp64 build %.4f s" % out["fp64_build_s"])
     return out
 
 
@@ -328,6 +344,8 @@ def main():
     ap.add_argument("--slots", type=int, default=4, help="c4: request workers")
     ap.add_argument("--serve-check", type=int, default=4,
                     help="c4: requests re-run on the plain FP64 path for the energy check")
+    ap.add_argument("--no-reorder", action="store_true",
+                    help="keep the generator's atom order (the reference's \"reorder\": false)")
     ap.add_argument("--form", default="auto", choices=["auto", "bf16", "f32"],
                     help="precision of the chosen-form exchange builds (auto: BF16x3 until the "
                          "SCF error reaches --escalate or stalls, then FP32, then FP64 if FP32 stalls)")
@@ -378,7 +396,7 @@ def main():
 
     cfg = CONFIGS[args.config]
     t0 = time.perf_counter()
-    mol = desc.Molecule(desc.water_cluster(cfg["nwater"], cfg["seed"]))
+    mol = desc.Molecule(desc.water_cluster(cfg["nwater"], cfg["seed"]), reorder=not args.no_reorder)
     log("molecule: %d waters nbf=%d naux=%d nocc=%d" % (cfg["nwater"], mol.nbf, mol.naux, mol.nocc))
     S, T, V = ints.int1e(mol)
     M = ints.int2c2e(mol)
@@ -553,7 +571,16 @@ def main():
     dom = max((k for k in kernels if isinstance(kernels[k], dict) and "s" in kernels[k]),
               key=lambda k: kernels[k]["s"] * kernels[k]["builds"])
     d = kernels[dom]
-    traffic = measured_traffic().get(dom)
+    # the exact kernel instances the autotuned plans run, for the PMC figures
+    exact = {}
+    for key, form, pname in (("k_slab_gemm_bf16x3", "bf16x3", "bf16_plan"), ("k_seg_syrk_bf16x3", "bf16x3", "bf16_plan"),
+                             ("k_slab_gemm_f32_dl", "f32", "f32_plan"), ("k_seg_syrk_f32_dl", "f32", "f32_plan"),
+                             ("k_slab_gemm_f64_dl", "f64", "f64_plan"), ("k_seg_syrk_f64_dl", "f64", "f64_plan")):
+        if key in kernels and kernels.get(pname):
+            k1, k3 = exact_kernels(form, kernels[pname])
+            exact[key] = k1 if key.startswith("k_slab") else k3
+            kernels[key]["instance"] = exact[key]
+    traffic = measured_traffic().get(exact.get(dom, dom))
     roof = dict(bound="mfma", achieved=d["tflops"], peak=d["peak"], unit="TFLOP/s",
                 frac=d["tflops"] / d["peak"], traffic=traffic, kernel=dom, kernel_s=d["s"],
                 kernels=kernels)
@@ -561,7 +588,7 @@ def main():
         # HBM bytes per launch from the committed PMC runs, over this run's launch time
         roof["hbm_gbs"] = traffic / d["s"] / 1e9
         roof["hbm_frac"] = roof["hbm_gbs"] / 8000.0
-    util = measured_mfma_util().get(dom)
+    util = measured_mfma_util().get(exact.get(dom, dom))
     if util is not None:
         roof["mfma_util_counter"] = util
 
@@ -579,14 +606,21 @@ def main():
     # converged density (F = H + J + K with the reference's J and an FP64 K),
     # against the plain SCF's, occupied and virtual separately
     from scipy.linalg import eigh
-    Pc = Cc @ Cc.T
-    Ff = T + V + dB.dfj(Pc, met) + dB.dfk_mo_tilde64(Cc)
-    eps_f = eigh(Ff, S, eigvals_only=True)
     no = mol.nocc
-    deps = {"occupied": float(np.abs(eps_f[:no] - ref.eps[:no]).max()),
-            "virtual": float(np.abs(eps_f[no:] - ref.eps[no:]).max())}
-    log("orbital energies of the final FP64 Fock vs plain FP64 SCF: max |d eps| occupied %.3e virtual %.3e" % (
-        deps["occupied"], deps["virtual"]))
+
+    def final_eps(res):
+        Co = np.ascontiguousarray(res.C[:, :no])
+        F = T + V + dB.dfj(Co @ Co.T, met) + dB.dfk_mo_tilde64(Co)
+        return eigh(F, S, eigvals_only=True)
+    Cp = np.ascontiguousarray(ref.C[:, :no])
+    eps_ref = eigh(T + V + dB.dfj(Cp @ Cp.T, met) + dB.dfk_mo_tilde64(Cp), S, eigvals_only=True)
+    deps = {}
+    for tag, res in (("scf_thresh", chosen), ("both_tight", chosen_tight)):
+        e = final_eps(res)
+        deps[tag] = {"occupied": float(np.abs(e[:no] - eps_ref[:no]).max()),
+                     "virtual": float(np.abs(e[no:] - eps_ref[no:]).max())}
+        log("orbital energies of the final FP64 Fock (chosen SCF %s) vs the plain SCF's: max |d eps| "
+            "occupied %.3e virtual %.3e" % (tag, deps[tag]["occupied"], deps[tag]["virtual"]))
 
     out = {
         "metric": "DF-K s/iter, %d H2O cc-pVDZ, chosen form vs FP64" % cfg["nwater"],
@@ -604,7 +638,7 @@ def main():
         "data": "synthetic seeded water cluster, real integrals",
         "config": {"workload": "DF-K build, %d-water cluster cc-pVDZ/cc-pVDZ-jkfit (%s)" % (
             cfg["nwater"], args.config), "nbf": mol.nbf, "naux": mol.naux, "nocc": mol.nocc,
-            "nnz_B": nnz, "screen_eps": 1e-10, "rcm_reorder": True, "scf_thresh": args.scf_thresh},
+            "nnz_B": nnz, "screen_eps": 1e-10, "rcm_reorder": not args.no_reorder, "scf_thresh": args.scf_thresh},
         "plain_fp64_s_per_iter": plain_s,
         "speedup_vs_plain": plain_s / step_s,
         "chosen_form": form,

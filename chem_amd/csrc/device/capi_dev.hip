@@ -356,6 +356,31 @@ extern "C" int mchip_dfk_mo(mchip_dbt t, mchip_metric m, const double* C, int no
   MCHIP_CATCH
 }
 
+extern "C" int mchip_dfk_lmo_uv(mchip_dbt t, mchip_metric m, const double* U, const double* V, int nocc,
+                                int occ_nbatches, int fp64_copy, double* K)
+{
+  MCHIP_TRY
+  need_device(t->ctx);
+  if (nocc < 0) throw std::runtime_error("two-sided exchange: nocc < 0");
+  Ctx& c = t->ctx->c;
+  const int nbf = t->t.L.nbf;
+  const size_t kb = (size_t)nbf * nbf * sizeof(double);
+  const size_t cb = (size_t)nbf * std::max(nocc, 1) * sizeof(double);
+  // v arrives as nocc x nbf (the reference's v_mb); the contraction wants v^T
+  std::vector<double> vt((size_t)nbf * std::max(nocc, 1));
+  for (int i = 0; i < nocc; ++i)
+    for (int n = 0; n < nbf; ++n) vt[(size_t)n * nocc + i] = V[(size_t)i * nbf + n];
+  DevBuf dU(cb), dVt(cb), dK(kb);
+  if (nocc > 0) {
+    h2d(dU.p, U, (size_t)nbf * nocc * sizeof(double), c.stream);
+    h2d(dVt.p, vt.data(), (size_t)nbf * nocc * sizeof(double), c.stream);
+  }
+  dfk_uv(c, t->t, m ? &m->m : nullptr, dU.as<double>(), nocc, dVt.as<double>(), nocc, nocc, dK.as<double>(),
+         occ_nbatches, fp64_copy != 0);
+  d2h(K, dK.p, kb, c.stream);
+  MCHIP_CATCH
+}
+
 #include "scf.hpp"
 
 static thread_local double last_scf_extra[2] = {0.0, 0.0};

@@ -2,14 +2,12 @@
 // the DFMO exchange, FP64.
 //
 //   metric_create   aoloader::invert + math::LLT (aoloader.cpp:14-32,
-//                   LLT.cpp:8-109): V = L L^T (rocsolver potrf), L^-1 (trtri),
-//                   V^-1 = L^-T L^-1.
+//                   LLT.cpp:8-109): V = L L^T, L^-1, V^-1 = L^-T L^-1 on the
+//                   library's own blocked FP64 kernels (linalg.hip).
 //   dfj             DF_J::compute_J (jkdf.cpp:69-190).
 //   dfk_mo          DFMO_K::compute_K (jkdf.cpp:221-462), K returned negated.
 //   transform_B     B~ = L^-1 B (one-time), after which K = -sum_X (B~_X C)^T
 //                   (B~_X C) and J = B~^T (B~ . P) need no metric per call.
-#include <rocsolver/rocsolver.h>
-
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -204,11 +202,22 @@ void metric_create(Ctx& c, int naux, const double* V_host, Metric& m)
   hipLaunchKernelGGL(k_zero_upper_cm, dim3(nblk((int64_t)naux * naux, 256)), dim3(256), 0, c.stream, A, naux);
   trtri_lower(c, A, naux, naux);
   hipLaunchKernelGGL(k_zero_upper_cm, dim3(nblk((int64_t)naux * naux, 256)), dim3(256), 0, c.stream, A, naux);
-  // V^-1 = L^-T L^-1 (column-major: Linv^T * Linv)
-  const double one = 1.0, zero = 0.0;
-  RB_CHECK(rocblas_dgemm(c.blas, rocblas_operation_transpose, rocblas_operation_none, naux, naux,
-                         naux, &one, A, naux, A, naux, &zero, m.Vinv.as<double>(), naux));
+  // V^-1 = L^-T L^-1: V^-1(i, j) = sum_{k >= max(i, j)} L^-1(k, i) L^-1(k, j), both
+  // operands read down the columns of L^-1; exactly symmetric (the (i, j) and
+  // (j, i) tiles sum the same products in the same order)
+  dgemm_rm(c.stream, true, true, naux, naux, naux, A, naux, A, naux, m.Vinv.as<double>(), naux, 1.0, 0.0,
+           DG_TRI_A_UPPER | DG_TRI_B_LOWER);
   HIP_CHECK(hipStreamSynchronize(c.stream));
+}
+
+// dst[x][r] = sum_{y <= x} L^-1(x, y) src[y][r] for naux rows x of R elements
+// (L^-1 lower triangular, stored column-major): the rows of L^-1 B for a
+// slab of R = bsz * ncol elements per aux function, or HT2 = L^-1 HT1 of the
+// exchange (DFMO_K K2, jkdf.cpp:384).  Asynchronous.
+void linv_apply(Ctx& c, const Metric& m, const double* src, int64_t R, double* dst)
+{
+  dgemm_rm(c.stream, false, false, m.naux, R, m.naux, m.Linv.as<double>(), m.naux, src, R, dst, R, 1.0, 0.0,
+           DG_TRI_A_LOWER);
 }
 
 void transform_B(Ctx& c, DevBT& t, const Metric& m)
@@ -218,18 +227,14 @@ void transform_B(Ctx& c, DevBT& t, const Metric& m)
   int64_t maxslab = 0;
   for (int n = 0; n < t.L.nbb; ++n) maxslab = std::max<int64_t>(maxslab, t.L.slab_off[n + 1] - t.L.slab_off[n]);
   c.scratch_c.ensure((size_t)maxslab * sizeof(double));
-  const double one = 1.0;
   for (int n = 0; n < t.L.nbb; ++n) {
     const int64_t R = (int64_t)t.L.bsz[n] * t.L.ncol[n];
     if (R == 0) continue;
     double* slab = t.data64.as<double>() + t.L.slab_off[n];
     const size_t bytes = (size_t)R * t.L.naux * sizeof(double);
     HIP_CHECK(hipMemcpyAsync(c.scratch_c.p, slab, bytes, hipMemcpyDeviceToDevice, c.stream));
-    // column-major (R x naux) view: slab <- slab * (L^-1)^T
-    RB_CHECK(rocblas_dtrmm_64(c.blas, rocblas_side_right, rocblas_fill_lower,
-                              rocblas_operation_transpose, rocblas_diagonal_non_unit, R, t.L.naux,
-                              &one, m.Linv.as<double>(), t.L.naux, c.scratch_c.as<double>(), R,
-                              slab, R));
+    // slab <- L^-1 slab over x (out of place, from the copy)
+    linv_apply(c, m, c.scratch_c.as<double>(), R, slab);
   }
   HIP_CHECK(hipStreamSynchronize(c.stream));
   t.transformed = true;
@@ -340,7 +345,11 @@ void dfk_mo(Ctx& c, DevBT& t, const Metric* m, const double* C, int ldc, int noc
     const double* HT = HT1;
     if (!pre) {
       PhaseTimer tm(c, PH_K2);
-      // K2: HT2 = L^-1 HT1 over x; column-major (W x naux) view times (L^-1)^T
+      // K2: HT2 = L^-1 HT1 over x; column-major (W x naux) view times (L^-1)^T.
+      // The plain path's metric step stays on rocBLAS's triangular multiply:
+      // the vendor kernel (0.67 of the FP64 MFMA peak at 64 waters) is the
+      // faster of the two, and this path is the FP64 reference the chosen
+      // form is measured against.
       double* HT2 = c.scratch_b.as<double>();
       const int64_t W = (int64_t)nbf * ldw;
       const double one = 1.0;
@@ -471,16 +480,12 @@ static void tilde_slabs(Ctx& c, DevBT& t, const Metric& m, float* out32, bool pa
   int64_t maxslab = 0;
   for (int n = 0; n < L.nbb; ++n) maxslab = std::max<int64_t>(maxslab, L.slab_off[n + 1] - L.slab_off[n]);
   DevBuf tmp((size_t)std::max<int64_t>(maxslab, 1) * sizeof(double));
-  const double one = 1.0;
   for (int n = 0; n < L.nbb; ++n) {
     const int64_t R = (int64_t)L.bsz[n] * L.ncol[n];
     if (R == 0) continue;
     const double* slab = t.data64.as<double>() + L.slab_off[n];
     double* dst = tmp.as<double>();
-    // column-major (R x naux) view: dst = slab * (L^-1)^T
-    RB_CHECK(rocblas_dtrmm_64(c.blas, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
-                              rocblas_diagonal_non_unit, R, L.naux, &one, m.Linv.as<double>(), L.naux, slab, R,
-                              dst, R));
+    linv_apply(c, m, slab, R, dst);
     const int64_t ne = R * L.naux;
     if (out64)
       hipLaunchKernelGGL(k_pad_f64_slab, dim3(gs_blocks((int64_t)L.naux * L.bsz[n] * t.h_ldp[n])), dim3(256), 0,
@@ -547,21 +552,53 @@ static void zero_w(Ctx& c, void* W, size_t bytes)
   HIP_CHECK(hipGetLastError());
 }
 
-// Tile table of the BF16 slab GEMM (row tile BM, column tile BN), padded with
-// empty tiles to a multiple of 8 for the XCD mapping.
+// Tile table of the direct-to-LDS slab GEMMs (row tile BM, column tile BN),
+// laid out for the XCD mapping of the kernels (xcd_tile): the table is cut
+// into 8 equal segments and segment x runs on XCD x.  Whole slabs are dealt to
+// the segments heaviest first, each to the segment with the least work so far
+// (a tile of slab N costs its ldp[N] / 32 contraction steps plus one step for
+// its loads and stores), and the segments are padded with empty tiles (slab
+// -1) to equal length.  The work per XCD is then balanced whatever the atom
+// order, and a slab's tiles share one XCD's L2 for their coefficient rows.
 static const Tile* bf16_tiles(Ctx& c, DevBT& t, int BM, int BN, int ng, int* ntiles)
 {
   auto key = std::make_pair(-(BN * 1000 + BM), ng);
   auto it = t.tiles.find(key);
   if (it == t.tiles.end()) {
-    std::vector<Tile> v;
-    // slabs without significant pairs still get tiles: they write the zero rows of W
-    for (int n = 0; n < t.L.nbb; ++n) {
+    const int nbb = t.L.nbb;
+    std::vector<int64_t> ntl(nbb);
+    std::vector<double> work(nbb);
+    const int ncb = (ng + BN - 1) / BN;
+    for (int n = 0; n < nbb; ++n) {
+      const int64_t rows = (int64_t)t.L.naux * t.L.bsz[n];
+      ntl[n] = (rows + BM - 1) / BM * ncb;
+      const int ldp = t.h_ldp.size() == (size_t)nbb ? t.h_ldp[n] : (t.L.ncol[n] + 31) / 32 * 32;
+      work[n] = (double)ntl[n] * (ldp / 32 + 1);
+    }
+    std::vector<int> order(nbb);
+    for (int n = 0; n < nbb; ++n) order[n] = n;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+    std::vector<std::vector<Tile>> seg(8);
+    double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int n : order) {
+      int best = 0;
+      for (int x = 1; x < 8; ++x)
+        if (load[x] < load[best]) best = x;
+      load[best] += work[n];
+      // slabs without significant pairs still get tiles: they write the zero rows of W
       const int64_t rows = (int64_t)t.L.naux * t.L.bsz[n];
       for (int64_t m0 = 0; m0 < rows; m0 += BM)
-        for (int n0 = 0; n0 < ng; n0 += BN) v.push_back(Tile{n, (int)m0, n0, 0});
+        for (int n0 = 0; n0 < ng; n0 += BN) seg[best].push_back(Tile{n, (int)m0, n0, 0});
     }
-    while (v.size() % 8) v.push_back(Tile{-1, 0, 0, 0});
+    size_t len = 0;
+    for (auto& v : seg) len = std::max(len, v.size());
+    std::vector<Tile> v;
+    v.reserve(8 * len);
+    for (auto& sv : seg) {
+      v.insert(v.end(), sv.begin(), sv.end());
+      v.resize((v.size() + len - sv.size()), Tile{-1, 0, 0, 0});
+    }
+    if (v.size() >= ((size_t)1 << 31)) throw std::runtime_error("slab GEMM: too many tiles");
     t.ntiles[key] = (int)v.size();
     t.tiles[key] = upload_vec(v, c.stream);
     HIP_CHECK(hipStreamSynchronize(c.stream));
@@ -1126,22 +1163,23 @@ static void slab_gemm_f64dl(Ctx& c, DevBT& t, int shape, const double* G, int ng
 }
 
 static void seg_syrk_f64dl(Ctx& c, int variant, const double* W, int naux, int nbf, int ldw, const int2* tl,
-                           int ntile, int nch, int spc, double* partial)
+                           int ntile, int nch, int spc, double* partial, const double* W2 = nullptr)
 {
   const dim3 grid8((unsigned)((ntile * nch + 7) / 8 * 8));
   const int64_t ss = (int64_t)nbf * ldw;
+  if (!W2) W2 = W;
   switch (variant) {
     case 0:
-      hipLaunchKernelGGL(k_seg_syrk_f64_dl<2>, grid8, dim3(512), 0, c.stream, W, naux, ss, ldw, nbf, tl, ntile, spc,
-                         partial);
+      hipLaunchKernelGGL(k_seg_syrk_f64_dl<2>, grid8, dim3(512), 0, c.stream, W, W2, naux, ss, ldw, nbf, tl, ntile,
+                         spc, partial);
       break;
     case 1:
-      hipLaunchKernelGGL(k_seg_syrk_f64_dl<3>, grid8, dim3(512), 0, c.stream, W, naux, ss, ldw, nbf, tl, ntile, spc,
-                         partial);
+      hipLaunchKernelGGL(k_seg_syrk_f64_dl<3>, grid8, dim3(512), 0, c.stream, W, W2, naux, ss, ldw, nbf, tl, ntile,
+                         spc, partial);
       break;
     case 2:
-      hipLaunchKernelGGL(k_seg_syrk_f64_dl<4>, grid8, dim3(512), 0, c.stream, W, naux, ss, ldw, nbf, tl, ntile, spc,
-                         partial);
+      hipLaunchKernelGGL(k_seg_syrk_f64_dl<4>, grid8, dim3(512), 0, c.stream, W, W2, naux, ss, ldw, nbf, tl, ntile,
+                         spc, partial);
       break;
     default: throw std::runtime_error("FP64 SYRK: unknown variant");
   }
@@ -1250,6 +1288,106 @@ static void dfk_mo_tilde64_dl(Ctx& c, DevBT& t, const double* C, int ldc, int no
     PhaseTimer tm(c, PH_K3);
     syrk_f64_exchange(c, t, W, w, ldw, nb, b, K);
   }
+}
+
+// Full 128 x 128 tile table of an n x n product (cached in t).
+static const int2* full_tiles(Ctx& c, DevBT& t, int n, int* ntile)
+{
+  const auto key = std::make_pair(-2000000, n);
+  const int nt = (n + 127) / 128;
+  *ntile = nt * nt;
+  if (!t.tiles.count(key)) {
+    std::vector<int2> tl;
+    for (int i = 0; i < nt; ++i)
+      for (int j = 0; j < nt; ++j) tl.push_back(int2{i, j});
+    t.tiles[key] = upload_vec(tl, c.stream);
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+  }
+  return t.tiles[key].as<int2>();
+}
+
+// The two-sided exchange of DFLMO_K (jkdf.cpp:1093-1331, set_sym(false)):
+// K(m, n) = -sum_{X,Y,i} (B u)(X|m i) V^-1(X, Y) (B v^T)(Y|n i), with
+// V^-1 = L^-T L^-1 it is -sum_{Z,i} Wu(Z, m, i) Wv(Z, n, i) for Wu = L^-1 B u and
+// Wv = L^-1 B v^T.  U and Vt are device arrays nbf x ldu / nbf x ldv (row-major,
+// first nocc columns; Vt is v transposed).  FP64 throughout: on the FP64 copy
+// of L^-1 B when fp64_copy (no metric step), else on B with L^-1 applied to
+// each half-transformed tensor (as K2 of the plain DFMO path).  K is nbf x nbf
+// row-major and not symmetric.
+void dfk_uv(Ctx& c, DevBT& t, const Metric* m, const double* U, int ldu, const double* Vt, int ldv, int nocc,
+            double* K, int occ_nbatches, bool fp64_copy)
+{
+  const BLayout& L = t.L;
+  const int naux = L.naux, nbf = L.nbf;
+  if (fp64_copy && (!t.data64t.p || t.h_ldp_pre.size() != (size_t)L.nbb + 1))
+    throw std::runtime_error("two-sided exchange: FP64 copy of L^-1 B missing (mchip_dbt_make_lowp with f64)");
+  if (!fp64_copy && !t.transformed && !m)
+    throw std::runtime_error("two-sided exchange: the metric is needed for an untransformed B");
+  if (nocc <= 0) {
+    hipLaunchKernelGGL(k_fill, dim3(gs_blocks((int64_t)nbf * nbf)), dim3(256), 0, c.stream, K,
+                       (int64_t)nbf * nbf, 0.0);
+    return;
+  }
+  occ_nbatches = std::max(1, std::min(occ_nbatches, nocc));
+  const int per = (nocc + occ_nbatches - 1) / occ_nbatches;
+  const int nb = (nocc + per - 1) / per;
+  const int ldw = (per + 7) / 8 * 8;
+  const size_t nW = (size_t)naux * nbf * ldw;
+  const bool metric_step = !fp64_copy && !t.transformed;
+  // Wu | Wv (| half-transformed staging for the metric step)
+  c.scratch_a.ensure(nW * sizeof(double) * (metric_step ? 3 : 2) + 256);
+  double* Wu = c.scratch_a.as<double>();
+  double* Wv = Wu + nW;
+  double* HT1 = Wv + nW;
+  size_t nG = 0;
+  if (fp64_copy) nG = (size_t)t.h_ldp_pre[L.nbb] * per;
+  c.scratch_b.ensure(std::max<size_t>(nG, 1) * sizeof(double));
+  double* G = c.scratch_b.as<double>();
+  int gmax = 0;
+  for (int n = 0; n < (int)t.h_ldp.size(); ++n) gmax = std::max(gmax, t.h_ldp[n]);
+  int ntile = 0;
+  const int2* tl = full_tiles(c, t, nbf, &ntile);
+  int nch = std::max(1, std::min(naux, 2048 / std::max(1, ntile)));
+  const int spc = (naux + nch - 1) / nch;
+  nch = (naux + spc - 1) / spc;
+  c.scratch_partial.ensure((size_t)nb * nch * nbf * nbf * sizeof(double));
+  // W = (L^-1) B C for one batch of columns of C into out (zero past w)
+  auto half = [&](const double* C, int ldc, int i0, int w, double* out) {
+    zero_w(c, out, nW * sizeof(double));
+    if (fp64_copy) {
+      if (gmax > 0 && L.nbb > 0)
+        hipLaunchKernelGGL(k_gather_ct_f64, dim3(nblk((int64_t)w * gmax, 256), L.nbb), dim3(256), 0, c.stream,
+                           C + i0, ldc, w, t.ncol.as<int>(), t.ldp.as<int>(), t.ldp_pre.as<int64_t>(),
+                           t.colmap.as<int>(), t.colmap_off.as<int>(), G);
+      ExchangePlan& p = f64_plan_entry(t, w);
+      if (p.k1 < 0) p.k1 = 0;
+      slab_gemm_f64dl(c, t, p.k1, G, w, out, ldw);
+    }
+    else if (metric_step) {
+      zero_w(c, HT1, nW * sizeof(double));
+      slab_gemm_t<double>(c, t, t.data64.as<double>(), C + i0, ldc, w, HT1, ldw);
+      // L^-1 over x, as the plain DFMO path's K2
+      const double one = 1.0;
+      RB_CHECK(rocblas_dtrmm_64(c.blas, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                                rocblas_diagonal_non_unit, (int64_t)nbf * ldw, naux, &one, m->Linv.as<double>(), naux,
+                                HT1, (int64_t)nbf * ldw, out, (int64_t)nbf * ldw));
+    }
+    else {
+      slab_gemm_t<double>(c, t, t.data64.as<double>(), C + i0, ldc, w, out, ldw);
+    }
+    HIP_CHECK(hipGetLastError());
+  };
+  for (int b = 0; b < nb; ++b) {
+    const int i0 = b * per;
+    const int w = std::min(per, nocc - i0);
+    half(U, ldu, i0, w, Wu);
+    half(Vt, ldv, i0, w, Wv);
+    double* partial = c.scratch_partial.as<double>() + (size_t)b * nch * nbf * nbf;
+    seg_syrk_f64dl(c, 0, Wu, naux, nbf, ldw, tl, ntile, nch, spc, partial, Wv);
+  }
+  hipLaunchKernelGGL(k_reduce_full, dim3(gs_blocks((int64_t)nbf * nbf)), dim3(256), 0, c.stream,
+                     c.scratch_partial.as<double>(), nb * nch, (int64_t)nbf * nbf, -1.0, K);
+  HIP_CHECK(hipGetLastError());
 }
 
 void dfk_f64_plan(const DevBT& t, int ng, int* plan, float* ms, int* nms)

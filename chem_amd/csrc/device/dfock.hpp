@@ -10,6 +10,7 @@ void ctx_free(Ctx& c);
 void dbt_upload(Ctx& c, const BLayout& L, const double* host_data, DevBT& t);
 void metric_create(Ctx& c, int naux, const double* V_host, Metric& m);
 void transform_B(Ctx& c, DevBT& t, const Metric& m);
+void linv_apply(Ctx& c, const Metric& m, const double* src, int64_t R, double* dst);
 // FP32 slab GEMM on an FP32 copy of the slabs (data), column tile bn (64 or
 // 128; 0 picks by ng).  f64_sums: FP64 running sums of the FP32 MFMA
 // accumulators, flushed every 16 products.
@@ -49,6 +50,10 @@ void dfk_bf16_force(DevBT& t, int ng, int k1, int k3);
 int dfk_bf16_num_shapes(int* nsyrk);
 int dfk_bf16_shape_dl(int k1);
 void dfk_ao(Ctx& c, DevBT& t, const Metric* m, const double* P_dev, double* K, int occ_nbatches, int form);
+// Two-sided exchange (DFLMO_K, set_sym(false)): K = -(L^-1 B U)^T (L^-1 B Vt)
+// per aux function, summed; U, Vt device nbf x ld (row-major), K not symmetric.
+void dfk_uv(Ctx& c, DevBT& t, const Metric* m, const double* U, int ldu, const double* Vt, int ldv, int nocc,
+            double* K, int occ_nbatches, bool fp64_copy);
 const Tile* slab_tiles(Ctx& c, DevBT& t, int ng, int* ntiles, int BN = 0);
 // out (slab layout of B) = rows (X, nu in N) of A (lda, klen) times gathered rows of G
 void slab_out_gemm(Ctx& c, DevBT& t, const double* A, int lda, int klen, const double* G, int ldg,

@@ -115,11 +115,27 @@ __global__ void k_slab_gemm_f64_dl(const double* data, const int64_t* off, const
                                    const int* bsz, const int* boff, const Tile* tiles, int naux, int nbf,
                                    const double* g, int ng, double* out, int ldo);
 template <int S>
-__global__ void k_seg_syrk_f64_dl(const double* W, int nseg, int64_t sstride, int ldw, int n, const int2* tiles,
-                                  int ntile, int segs_per_chunk, double* partial);
+__global__ void k_seg_syrk_f64_dl(const double* W, const double* W2, int nseg, int64_t sstride, int ldw, int n,
+                                  const int2* tiles, int ntile, int segs_per_chunk, double* partial);
+__global__ void k_reduce_full(const double* partial, int nchunk, int64_t n2, double alpha, double* out);
 template <int S>
 __global__ void k_seg_syrk_f32_dl(const float* W, int nseg, int64_t sstride, int ldw, int n, const int2* tiles,
                                   int ntile, int segs_per_chunk, double* partial);
+
+// Dense FP64 GEMM on MFMA (kernels_dgemm.hip): C = alpha op(A) op(B) + beta C,
+// C row-major (ldc); A(m,k) = ak ? A[m*lda + k] : A[k*lda + m]; B(k,n) =
+// bk ? B[n*ldb + k] : B[k*ldb + n].  tri: zeros of triangular operands whose
+// contraction range a tile skips (DG_TRI_*).  Asynchronous on st.
+constexpr int DG_TRI_A_LOWER = 1;   // A(m,k) = 0 for k > m
+constexpr int DG_TRI_A_UPPER = 2;   // A(m,k) = 0 for k < m
+constexpr int DG_TRI_B_LOWER = 4;   // B(k,n) = 0 for k < n
+constexpr int DG_TRI_B_UPPER = 8;   // B(k,n) = 0 for k > n
+void dgemm_rm(hipStream_t st, bool ak, bool bk, int64_t M, int64_t N, int K, const double* A, int64_t lda,
+              const double* B, int64_t ldb, double* C, int64_t ldc, double alpha, double beta, int tri);
+// Y = alpha X Wop for a column-major m x w panel X (w <= 64) and a lower-
+// triangular w x w W (column-major, ldw); Wop = W^T if trans; Y may be X.
+void trmm_panel(hipStream_t st, int m, int w, const double* X, int ldx, const double* W, int ldw, bool trans,
+                double alpha, double* Y, int ldy);
 
 // Blocks of a one-thread-per-element launch.  A dispatch holds at most 2^32 - 1
 // work-items per grid dimension; larger element counts must use a grid-stride

@@ -32,6 +32,17 @@ __global__ void k_reduce_sym(const T* __restrict__ partial, int nchunk, int n, i
   out[(int64_t)c * n + r] = v;
 }
 template __global__ void k_reduce_sym<double>(const double*, int, int, int, double, double*);
+
+// out[i] = alpha * sum_ch partial[ch][i], i < n2 (fixed order; no symmetrisation)
+__global__ void k_reduce_full(const double* __restrict__ partial, int nchunk, int64_t n2, double alpha,
+                              double* __restrict__ out)
+{
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (int64_t)gridDim.x * blockDim.x) {
+    double v = 0.0;
+    for (int ch = 0; ch < nchunk; ++ch) v += partial[(int64_t)ch * n2 + i];
+    out[i] = alpha * v;
+  }
+}
 template __global__ void k_reduce_sym<float>(const float*, int, int, int, double, double*);
 
 // ---------------------------------------------------------------------------

@@ -2,7 +2,7 @@
 //   k_pad_f64_slab      one FP64 slab (rows of ncol) -> rows padded to ldp (zeros past ncol)
 //   k_gather_ct_f64     per-slab gathered, transposed occupied coefficients, padded rows
 //   k_slab_gemm_f64_dl  W(X, mu, i) = sum_nu B~(X, mu nu) C(nu, i) per slab (K1)
-//   k_seg_syrk_f64_dl   partial[chunk] = sum_{X in chunk} W_X W_X^T, upper tiles (K3)
+//   k_seg_syrk_f64_dl   partial[chunk] = sum_{X in chunk} W_X W2_X^T (K3; W2 = W: upper tiles)
 //
 // Same padded slab layout and LDS row geometry as the FP32 kernels: a
 // contraction step covers 8 FP64 = 64 bytes of a row, 4 chunks of 16 bytes.
@@ -139,12 +139,14 @@ __global__ __launch_bounds__(64 * WM * WN) void k_slab_gemm_f64_dl(
 
 // Segmented SYRK in FP64 over W (rows of ldw, a multiple of 8, doubles; the
 // columns past the occupied count are zero): partial[chunk](m, n) =
-// sum_{X in chunk} sum_i W(X,m,i) W(X,n,i) on the upper 128 x 128 tiles, 8
-// waves of 64 x 32.  All tiles of a chunk run on one XCD (grid padded to a
-// multiple of 8; padding blocks exit).
+// sum_{X in chunk} sum_i W(X,m,i) W2(X,n,i) on the 128 x 128 tiles of the
+// table (the upper ones when W2 = W, the exchange being symmetric; all of
+// them for the two-sided exchange), 8 waves of 64 x 32.  All tiles of a
+// chunk run on one XCD (grid padded to a multiple of 8; padding blocks exit).
 template <int S>
-__global__ __launch_bounds__(512) void k_seg_syrk_f64_dl(const double* __restrict__ W, int nseg, int64_t sstride,
-                                                         int ldw, int n, const int2* __restrict__ tiles, int ntile,
+__global__ __launch_bounds__(512) void k_seg_syrk_f64_dl(const double* __restrict__ W, const double* __restrict__ W2,
+                                                         int nseg, int64_t sstride, int ldw, int n,
+                                                         const int2* __restrict__ tiles, int ntile,
                                                          int segs_per_chunk, double* __restrict__ partial)
 {
   constexpr int BM = 128, NW = 8, WN = 4, TM = 4, TN = 2;
@@ -169,15 +171,16 @@ __global__ __launch_bounds__(512) void k_seg_syrk_f64_dl(const double* __restric
   const int ksteps = ldw >> 3;
   const int nsteps = (s1 - s0) * ksteps;
   const int ld2 = ldw >> 1;
-  const uint4* W2 = reinterpret_cast<const uint4*>(W);
+  const uint4* Wa = reinterpret_cast<const uint4*>(W);
+  const uint4* Wb = reinterpret_cast<const uint4*>(W2);
   dl_pipeline<S, LPW>(
       nsteps,
       [&](int q, auto k) {
         const int sg = s0 + q / ksteps;
         const int kc0 = (q % ksteps) * 4;
-        const uint4* base = W2 + (int64_t)sg * (sstride >> 1);
-        issue_rows<BM, BM, NW>(pick_stage<S>(k, lds0, lds1, lds2, lds3), wave, lane, base, base, m0, n - 1, n0,
-                               n - 1, ld2, ld2, kc0);
+        const int64_t so = (int64_t)sg * (sstride >> 1);
+        issue_rows<BM, BM, NW>(pick_stage<S>(k, lds0, lds1, lds2, lds3), wave, lane, Wa + so, Wb + so, m0, n - 1,
+                               n0, n - 1, ld2, ld2, kc0);
       },
       [&](int, auto k) {
         const uint4* st = pick_stage<S>(k, lds0, lds1, lds2, lds3);
@@ -207,8 +210,11 @@ MCHIP_SLABF64DL_INST(2, 2, 4, 5, 3)
 MCHIP_SLABF64DL_INST(2, 4, 4, 5, 3)
 MCHIP_SLABF64DL_INST(2, 4, 4, 5, 2)
 #undef MCHIP_SLABF64DL_INST
-template __global__ void k_seg_syrk_f64_dl<2>(const double*, int, int64_t, int, int, const int2*, int, int, double*);
-template __global__ void k_seg_syrk_f64_dl<3>(const double*, int, int64_t, int, int, const int2*, int, int, double*);
-template __global__ void k_seg_syrk_f64_dl<4>(const double*, int, int64_t, int, int, const int2*, int, int, double*);
+template __global__ void k_seg_syrk_f64_dl<2>(const double*, const double*, int, int64_t, int, int, const int2*, int,
+                                               int, double*);
+template __global__ void k_seg_syrk_f64_dl<3>(const double*, const double*, int, int64_t, int, int, const int2*, int,
+                                               int, double*);
+template __global__ void k_seg_syrk_f64_dl<4>(const double*, const double*, int, int64_t, int, int, const int2*, int,
+                                               int, double*);
 
 }  // namespace mchip

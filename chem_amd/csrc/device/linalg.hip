@@ -2,15 +2,11 @@
 // V = L L^T and L^-1 of aoloader.cpp:14-32 / LLT.cpp; the overlap factor of
 // mp_compute.cpp:167-178).
 //
-// rocsolver_dpotrf / rocsolver_dtrtri return wrong factors, with and without a
-// failure flag, in 5-20 % of calls when other streams of the same process run
-// FP32/FP64 MFMA work at the same time (measured on MI355X with the request
-// loop's traffic; never when the device is otherwise idle or runs BF16
-// kernels).  Their results are inputs to every exchange and energy, so both
-// are done here with blocked right-looking (Cholesky) and backward (inverse)
-// algorithms of block size 64: the diagonal blocks by one workgroup each in
-// LDS, the off-diagonal work by rocBLAS dtrmm and dgemm, whose results stayed
-// bit-for-bit reproducible under the same concurrent load.
+// Blocked right-looking Cholesky and backward blocked inverse with block size
+// 64: the diagonal blocks by one workgroup each in LDS (k_chol_diag,
+// k_trti_diag), the panels by k_trmm_panel and the trailing updates by the
+// library's FP64 MFMA GEMM (kernels_dgemm.hip), every product out of place or
+// staged through LDS.  No vendor BLAS or solver runs on this path.
 #include <algorithm>
 #include <cmath>
 
@@ -113,7 +109,6 @@ int chol_lower(Ctx& c, double* A, int n, int lda)
   DevBuf winv((size_t)NB * NB * sizeof(double));
   int* info = c.info.as<int>();
   HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int), c.stream));
-  const double one = 1.0, mone = -1.0;
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int kb = std::min(NB, n - k0);
     hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(256), 0, c.stream, A, lda, k0, kb, winv.as<double>(), info);
@@ -122,12 +117,11 @@ int chol_lower(Ctx& c, double* A, int n, int lda)
     if (m <= 0) break;
     double* A21 = A + (int64_t)k0 * lda + (k0 + kb);
     double* A22 = A + (int64_t)(k0 + kb) * lda + (k0 + kb);
-    // A21 <- A21 L11^-T
-    RB_CHECK(rocblas_dtrmm(c.blas, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
-                           rocblas_diagonal_non_unit, m, kb, &one, winv.as<double>(), NB, A21, lda, A21, lda));
-    // A22 <- A22 - A21 A21^T (both triangles; only the lower one is used)
-    RB_CHECK(rocblas_dgemm(c.blas, rocblas_operation_none, rocblas_operation_transpose, m, m, kb, &mone, A21, lda,
-                           A21, lda, &one, A22, lda));
+    // A21 <- A21 L11^-T (the panel is read into LDS before it is overwritten)
+    trmm_panel(c.stream, m, kb, A21, lda, winv.as<double>(), NB, true, 1.0, A21, lda);
+    // A22 <- A22 - A21 A21^T: the row-major view of the column-major A22 is
+    // its transpose, and the update is symmetric
+    dgemm_rm(c.stream, false, false, m, m, kb, A21, lda, A21, lda, A22, lda, -1.0, 1.0, 0);
   }
   int h_info = 0;
   d2h(&h_info, info, sizeof(int), c.stream);
@@ -137,7 +131,7 @@ int chol_lower(Ctx& c, double* A, int n, int lda)
 void trtri_lower(Ctx& c, double* A, int n, int lda)
 {
   if (n <= 0) return;
-  const double one = 1.0, mone = -1.0;
+  DevBuf tmp((size_t)n * NB * sizeof(double));
   const int last = (n - 1) / NB * NB;
   for (int j0 = last; j0 >= 0; j0 -= NB) {
     const int jb = std::min(NB, n - j0);
@@ -145,14 +139,15 @@ void trtri_lower(Ctx& c, double* A, int n, int lda)
     HIP_CHECK(hipGetLastError());
     const int m = n - j0 - jb;
     if (m <= 0) continue;
-    double* A11 = A + (int64_t)j0 * lda + j0;
+    const double* A11 = A + (int64_t)j0 * lda + j0;
     double* A21 = A + (int64_t)j0 * lda + (j0 + jb);
-    double* A22 = A + (int64_t)(j0 + jb) * lda + (j0 + jb);
-    // A21 <- L22^-1 A21 (A22 already inverted), then A21 <- -A21 L11^-1 (A11 inverted above)
-    RB_CHECK(rocblas_dtrmm(c.blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
-                           rocblas_diagonal_non_unit, m, jb, &one, A22, lda, A21, lda, A21, lda));
-    RB_CHECK(rocblas_dtrmm(c.blas, rocblas_side_right, rocblas_fill_lower, rocblas_operation_none,
-                           rocblas_diagonal_non_unit, m, jb, &mone, A11, lda, A21, lda, A21, lda));
+    const double* A22 = A + (int64_t)(j0 + jb) * lda + (j0 + jb);
+    // T = L22^-1 A21 (A22 already inverted, lower triangular), column-major
+    // m x jb in tmp: row-major T^T(c, r) = sum_k A21(k, c) L22^-1(r, k)
+    double* T = tmp.as<double>();
+    dgemm_rm(c.stream, true, false, jb, m, m, A21, lda, A22, lda, T, m, 1.0, 0.0, DG_TRI_B_UPPER);
+    // A21 <- -T L11^-1 (A11 inverted above)
+    trmm_panel(c.stream, m, jb, T, m, A11, lda, false, -1.0, A21, lda);
   }
   HIP_CHECK(hipStreamSynchronize(c.stream));
 }

@@ -276,6 +276,20 @@ class DeviceB:
         _lib.check(_lib.lib().mchip_dfk_ao(self.h, m, _lib.dptr(P), f, occ_nbatches, _lib.dptr(K)))
         return K
 
+    def dfk_lmo_uv(self, U, V, metric=None, occ_nbatches=1, fp64_copy=False):
+        """Two-sided exchange of DFLMO_K (set_sym(false)): U nbf x nocc (left
+        coefficients), V nocc x nbf (right coefficients, the reference's v_mb);
+        K (negated, not symmetric)."""
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        if U.shape[0] != self.nbf or V.shape != (U.shape[1], self.nbf):
+            raise ValueError("U must be nbf x nocc and V nocc x nbf")
+        K = np.zeros((self.nbf, self.nbf))
+        m = None if metric is None else metric.h
+        _lib.check(_lib.lib().mchip_dfk_lmo_uv(self.h, m, _lib.dptr(U), _lib.dptr(V), U.shape[1], occ_nbatches,
+                                               int(bool(fp64_copy)), _lib.dptr(K)))
+        return K
+
     def bf16_tile(self, ng):
         """(column tile, threads per workgroup) picked by the BF16 autotuner for
         ng occupied columns, or None before the first build with that ng."""
@@ -295,7 +309,8 @@ class DeviceB:
         k = nms.value
         ns = self.bf16_shapes()[1]
         return dict(gemm_shape=int(plan[0]), bm=int(plan[1]), bn=int(plan[2]), threads=int(plan[3]),
-                    direct_to_lds=bool(plan[4]), stages=int(plan[5]), syrk_variant=int(plan[6]),
+                    direct_to_lds=bool(plan[4]), pipeline=int(plan[4]), stages=int(plan[5]),
+                    syrk_variant=int(plan[6]),
                     gemm_ms=[float(v) for v in ms[:k - ns]], syrk_ms=[float(v) for v in ms[k - ns:k]])
 
     def f32_plan(self, ng):

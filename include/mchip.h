@@ -232,6 +232,16 @@ int mchip_dfk_mo_bf16(mchip_dbt t, const double* C, int nocc, int occ_nbatches, 
 int mchip_dfk_ao_dev(mchip_dbt t, mchip_metric m, const double* P_dev, int form, int occ_nbatches,
                      double* K_dev);
 int mchip_dfk_ao(mchip_dbt t, mchip_metric m, const double* P, int form, int occ_nbatches, double* K);
+/* The two-sided exchange of DFLMO_K (src/fock/jkdf.cpp:1093-1331, taken when
+ * the builder has set_sym(false), as the ADC drivers use it):
+ * K(m,n) = -sum_{X,Y,i} (B u)(X|m i) V^-1(X,Y) (B v^T)(Y|n i), u nbf x nocc
+ * (the reference's u_bm, set_coeff_left_*) and v nocc x nbf (v_mb,
+ * set_coeff_right_*), both row-major; K nbf x nbf row-major, returned negated
+ * like the reference and not symmetric.  fp64_copy = 1: on the FP64 copy of
+ * L^-1 B (mchip_dbt_make_lowp with f64); 0: L^-1 applied to each
+ * half-transformed tensor (m needed unless B is transformed).  FP64. */
+int mchip_dfk_lmo_uv(mchip_dbt t, mchip_metric m, const double* U, const double* V, int nocc, int occ_nbatches,
+                     int fp64_copy, double* K);
 /* Launch shape picked by the BF16 slab-GEMM autotuner for ng occupied columns,
  * as column_tile * 10000 + threads_per_workgroup (0 before the first build with
  * that ng).  No counterpart in the reference. */

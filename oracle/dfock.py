@@ -165,6 +165,17 @@ def dfmo_k_rows(rows, naux, step, Linv, C_occ):
     return -(HT2.T @ HT2)
 
 
+def dflmo_k_uv(B, Vinv, U, V):
+    """DFLMO_K non-symmetric branch (jkdf.cpp:1093-1331): htv = "Xmn, in -> Xmi"
+    with v_mb = V (nocc x nbf), htu = "Xmn, ni -> Xmi" with u_bm = U (nbf x
+    nocc), htvfit = "XY, Ymi -> Xmi" with V^-1, K = "Xmi, Xni -> mn" of htu and
+    htvfit, negated (:1323).  Returns K (not symmetric)."""
+    htu = np.einsum("xmn,ni->xmi", B, U, optimize=True)
+    htv = np.einsum("xmn,in->xmi", B, V, optimize=True)
+    htvfit = np.einsum("xy,ymi->xmi", Vinv, htv, optimize=True)
+    return -np.einsum("xmi,xni->mn", htu, htvfit, optimize=True)
+
+
 def fitting_coefficients(B, Vinv):
     """dfitting::compute: C_fit(X|mn) = sum_Y Vinv(X,Y) B(Y|mn) (fitting.cpp:124-128)."""
     naux, nbf, _ = B.shape

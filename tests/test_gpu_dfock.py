@@ -573,3 +573,39 @@ def test_batched_exchange_crosses_tile_edge(ctx, nocc, nbatch):
         e1, eb, ebb = _relerr(K1, Kref), _relerr(Kb, Kref), _relerr(Kb, K1)
         print("  relerr unbatched %.2e batched %.2e batched vs unbatched %.2e" % (e1, eb, ebb))
         assert e1 < tol and eb < tol and ebb < tol
+
+
+@pytest.mark.parametrize("nbatch", [1, 3])
+def test_two_sided_exchange_vs_oracle(ctx, nbatch):
+    """DFLMO_K's two-sided exchange (set_sym(false), jkdf.cpp:1093-1331) with
+    left coefficients u (nbf x nocc) and right coefficients v (nocc x nbf):
+    against the oracle's restatement on a 3-water system and a ragged sparse
+    synthetic B, through the plain route (L^-1 per build) and the FP64 copy of
+    L^-1 B, in one and three occupied batches.  With u = C and v = C^T it is
+    the symmetric DFMO exchange."""
+    from chem_amd.engine import DeviceB, Metric
+    rng = np.random.default_rng(40 + nbatch)
+    cases = []
+    mol, Bh, V = _system(3)
+    cases.append(("3 waters", Bh, Bh.dense(), V, mol.nbf))
+    Bs, dense, Vs = _synthetic_B(rng, [14, 5, 5, 3, 14, 1, 7, 9, 30], [23, 70, 9, 40, 17])
+    cases.append(("synthetic", Bs, dense, Vs, sum([14, 5, 5, 3, 14, 1, 7, 9, 30])))
+    for name, Bh_, dense_, V_, nbf in cases:
+        print("case: two-sided exchange %s, %d batches" % (name, nbatch), flush=True)
+        nocc = 13
+        U = rng.standard_normal((nbf, nocc)) * 0.3
+        W = rng.standard_normal((nocc, nbf)) * 0.3
+        dB = DeviceB(ctx, Bh_)
+        m = Metric(ctx, V_)
+        _, Vinv = m.get()
+        Kref = dfock.dflmo_k_uv(dense_, Vinv, U, W)
+        Kp = dB.dfk_lmo_uv(U, W, metric=m, occ_nbatches=nbatch)
+        dB.make_lowp(m, f32=False, bf16=False, f64=True)
+        Kc = dB.dfk_lmo_uv(U, W, occ_nbatches=nbatch, fp64_copy=True)
+        ep, ec = _relerr(Kp, Kref), _relerr(Kc, Kref)
+        print("  relerr plain %.2e fp64 copy %.2e (asym %.2e)" % (ep, ec, np.abs(Kref - Kref.T).max()))
+        assert ep < _jtol(V_) and ec < _jtol(V_)
+        assert _relerr(Kc, Kp) < 1e-12
+        # u = C, v = C^T: the symmetric exchange
+        Ks = dB.dfk_lmo_uv(U, np.ascontiguousarray(U.T), metric=m, occ_nbatches=nbatch)
+        assert _relerr(Ks, dB.dfk_mo(U, m)) < 1e-12

@@ -200,3 +200,30 @@ def test_dfmo_k_rows_matches_dfmo_k():
     K = dfock.dfmo_k(B, invsqrt, C, bsz=mol.b, xsz=mol.x, osz=mol.oa(), occ_nbatches=3)
     Kr = dfock.dfmo_k_rows(lambda x0, x1: B[x0:x1], B.shape[0], 17, Linv, C)
     assert np.abs(Kr - K).max() < 1e-13 * np.abs(K).max()
+
+
+def test_dflmo_uv_pins():
+    """The two-sided DFLMO restatement: with u = C and v = C^T it is the
+    symmetric DFLMO (= DFMO) exchange; each element equals a direct triple sum
+    (a transposed u or v, or V^-1 applied on the wrong side, breaks either);
+    swapping u and v transposes K."""
+    mol, B, V = _water_system(1, 2)
+    Vinv, _, invsqrt = dfock.metric_factors(V)
+    rng = np.random.default_rng(8)
+    C = rng.standard_normal((mol.nbf, mol.nocc)) * 0.3
+    Ksym = dfock.dflmo_k_sym(B, Vinv, C)
+    Kuv = dfock.dflmo_k_uv(B, Vinv, C, C.T)
+    assert np.abs(Kuv - Ksym).max() < 1e-12 * np.abs(Ksym).max()
+    U = rng.standard_normal((mol.nbf, 3))
+    W = rng.standard_normal((3, mol.nbf))
+    K = dfock.dflmo_k_uv(B, Vinv, U, W)
+    assert np.abs(K - K.T).max() > 1e-3 * np.abs(K).max()       # genuinely not symmetric
+    for m_, n_ in ((0, 1), (5, 17), (23, 2)):
+        bu = np.array([[B[x, m_, :] @ U[:, i] for i in range(3)] for x in range(B.shape[0])])
+        bv = np.array([[B[x, n_, :] @ W[i, :] for i in range(3)] for x in range(B.shape[0])])
+        direct = -sum(bu[x, i] * Vinv[x, y] * bv[y, i] for x in range(B.shape[0]) for y in range(B.shape[0])
+                      for i in range(3))
+        assert abs(K[m_, n_] - direct) < 1e-11 * np.abs(K).max()
+    # (V^-1 now meets the other factor: the explicit inverse's cond(V) eps noise)
+    Kt = dfock.dflmo_k_uv(B, Vinv, W.T, U.T)
+    assert np.abs(Kt - K.T).max() < 10 * np.linalg.cond(V) * 2.2e-16 * np.abs(K).max()
