@@ -113,6 +113,7 @@ _SIGS = [
     ("mchip_dbt_transform", c_int, [c_void_p, c_void_p]),
     ("mchip_dfj_dev", c_int, [c_void_p, c_void_p, c_void_p, c_double, c_void_p]),
     ("mchip_dfj", c_int, [c_void_p, c_void_p, P_double, c_double, P_double]),
+    ("mchip_dfj_dev_noalloc", c_int, [c_void_p, c_void_p, c_void_p, c_double, c_void_p]),
     ("mchip_dfk_mo_dev", c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     ("mchip_dfk_mo", c_int, [c_void_p, c_void_p, P_double, c_int, c_int, P_double]),
     ("mchip_scf_run", c_int, [c_void_p, c_void_p, c_int, c_int, c_int, P_int, P_double, P_double,

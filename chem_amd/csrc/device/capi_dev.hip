@@ -1,6 +1,7 @@
 // C ABI: device context, device-resident B, metric, DF-J and DF-K.
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 
 #include "../../../include/mchip.h"
@@ -45,6 +46,31 @@ extern "C" int mchip_device_count(int* n)
   MCHIP_CATCH
 }
 
+// Workspace of the allocation-free DF-J (mchip_dfj_dev_noalloc): a fixed
+// device array, one copy per GPU for the life of the process.  Host code
+// only ever uses its device address, looked up per device when a context is
+// created on it (with that device current); the array's name is not a
+// device address in host code.
+static constexpr size_t kNoallocBytes = (size_t)128 << 20;
+static __device__ __align__(256) double scratch[kNoallocBytes / sizeof(double)];
+static std::mutex g_noalloc_mu;
+static void* g_noalloc_ptr[64] = {};
+
+// Device address of this GPU's copy of the array (the current device is
+// `device`).
+static double* noalloc_scratch(int device)
+{
+  if (device < 0 || device >= 64) throw std::runtime_error("device index out of range for the fixed workspace");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[device]) {
+    void* p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw std::runtime_error("fixed workspace: no device address");
+    g_noalloc_ptr[device] = p;
+  }
+  return static_cast<double*>(g_noalloc_ptr[device]);
+}
+
 extern "C" int mchip_ctx_create(int device, mchip_ctx* out)
 {
   MCHIP_TRY
@@ -53,7 +79,8 @@ extern "C" int mchip_ctx_create(int device, mchip_ctx* out)
     throw std::runtime_error("no HIP device available");
   if (device < 0 || device >= n) throw std::runtime_error("device index out of range");
   auto c = std::make_unique<mchip_ctx_s>();
-  ctx_init(c->c, device);
+  ctx_init(c->c, device);   // makes `device` current
+  (void)noalloc_scratch(device);
   *out = c.release();
   MCHIP_CATCH
 }
@@ -303,6 +330,20 @@ extern "C" int mchip_dfj_dev(mchip_dbt t, mchip_metric m, const double* P_dev, d
   MCHIP_TRY
   need_device(t->ctx);
   dfj(t->ctx->c, t->t, m ? &m->m : nullptr, P_dev, J_dev, pscale);
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dfj_dev_noalloc(mchip_dbt t, mchip_metric m, const double* P_dev, double pscale,
+                                     double* J_dev)
+{
+  MCHIP_TRY
+  need_device(t->ctx);
+  const size_t need = dfj_workspace_bytes(t->t);
+  if (need > kNoallocBytes)
+    throw std::runtime_error("mchip_dfj_dev_noalloc: this tensor's DF-J needs " + std::to_string(need) +
+                             " bytes of workspace, more than the fixed " + std::to_string(kNoallocBytes) +
+                             " (use mchip_dfj_dev)");
+  dfj_ws(t->ctx->c, t->t, m ? &m->m : nullptr, P_dev, J_dev, pscale, noalloc_scratch(t->ctx->c.device));
   MCHIP_CATCH
 }
 

@@ -240,23 +240,47 @@ void transform_B(Ctx& c, DevBT& t, const Metric& m)
   t.transformed = true;
 }
 
+// DF-J's chunking of the aux index for the column pass (k_j_cols): about
+// 2e6 elements of the pair-gathered slabs per chunk, at most 64 chunks.
+static void dfj_chunks(const DevBT& t, int* nch_out, int* xchunk_out)
+{
+  const int naux = t.L.naux;
+  const int64_t pg = t.pg_total;
+  int nch = (int)std::max<int64_t>(1, std::min<int64_t>(64, (2000000 + pg - 1) / std::max<int64_t>(pg, 1)));
+  nch = std::max(1, std::min(nch, naux));
+  const int xchunk = (naux + nch - 1) / nch;
+  *nch_out = std::max(1, (naux + xchunk - 1) / std::max(1, xchunk));
+  *xchunk_out = xchunk;
+}
+
+size_t dfj_workspace_bytes(const DevBT& t)
+{
+  int nch = 1, xchunk = 1;
+  dfj_chunks(t, &nch, &xchunk);
+  const size_t n_pg = (size_t)t.pg_total, n_gp = (size_t)t.L.nbb * t.L.naux, n_jp = (size_t)nch * t.pg_total;
+  return (n_pg + 2 * n_gp + 2 * (size_t)t.L.naux + 2 * n_jp + n_pg) * sizeof(double);
+}
+
 void dfj(Ctx& c, DevBT& t, const Metric* m, const double* P, double* J, double pscale)
+{
+  c.scratch_c.ensure(dfj_workspace_bytes(t) + 256);
+  dfj_ws(c, t, m, P, J, pscale, c.scratch_c.as<double>());
+}
+
+void dfj_ws(Ctx& c, DevBT& t, const Metric* m, const double* P, double* J, double pscale, double* ws)
 {
   PhaseTimer timer(c, PH_J);
   const BLayout& L = t.L;
   if (!t.transformed && !m) throw std::runtime_error("DF-J needs the metric for an untransformed B");
   const int naux = L.naux, nbf = L.nbf, nbb = L.nbb;
   const int64_t pg = t.pg_total;
-  int nch = (int)std::max<int64_t>(1, std::min<int64_t>(64, (2000000 + pg - 1) / std::max<int64_t>(pg, 1)));
-  nch = std::min(nch, naux);
-  const int xchunk = (naux + nch - 1) / nch;
-  nch = (naux + xchunk - 1) / xchunk;
+  int nch = 1, xchunk = 1;
+  dfj_chunks(t, &nch, &xchunk);
   // workspace: Pg | gpart, gpart_c | g | gq | Jpart, Jpart_c | Jg; every sum is
   // compensated (value + rounding-error pair), so J carries no summation-order
   // noise for V^-1 to amplify
   const size_t n_pg = (size_t)pg, n_gp = (size_t)nbb * naux, n_jp = (size_t)nch * pg;
-  c.scratch_c.ensure((n_pg + 2 * n_gp + 2 * (size_t)naux + 2 * n_jp + n_pg) * sizeof(double) + 256);
-  double* Pg = c.scratch_c.as<double>();
+  double* Pg = ws;
   double* gpart = Pg + n_pg;
   double* gpart_c = gpart + n_gp;
   double* g = gpart_c + n_gp;

@@ -20,6 +20,10 @@ void slab_gemm(Ctx& c, DevBT& t, const double* G, int ldg, int ng, double* out, 
                double alpha);
 // J from density P (device, nbf x nbf); the total density is pscale * P.
 void dfj(Ctx& c, DevBT& t, const Metric* m, const double* P, double* J, double pscale);
+// The same with the caller's workspace ws (dfj_workspace_bytes(t) bytes,
+// 8-byte aligned; every element is written before it is read).
+size_t dfj_workspace_bytes(const DevBT& t);
+void dfj_ws(Ctx& c, DevBT& t, const Metric* m, const double* P, double* J, double pscale, double* ws);
 // K = -sum ... from occupied coefficients C (device, row-major nbf x ldc).
 // tilde64: FP64 slabs of L^-1 B (t.data64t) to contract instead of applying
 // L^-1 to B C per build (the same K; K1 and K3 only).

@@ -177,6 +177,15 @@ int mchip_dbt_transform(mchip_dbt t, mchip_metric m);
  * pscale = 2 for a closed-shell alpha density (jkdf.cpp:83-90). */
 int mchip_dfj_dev(mchip_dbt t, mchip_metric m, const double* P_dev, double pscale, double* J_dev);
 int mchip_dfj(mchip_dbt t, mchip_metric m, const double* P, double pscale, double* J);
+/* mchip_dfj_dev without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Its workspace is a
+ * fixed 128 MiB device array per GPU, held until the process ends; a tensor
+ * whose DF-J workspace is larger is rejected with an error (about 83 MB at 64
+ * waters, cc-pVDZ-jkfit).  Same kernels, same J bit for bit as mchip_dfj_dev;
+ * every call on one GPU shares the array, so calls that overlap on one GPU
+ * (two host threads, or two streams) race. */
+int mchip_dfj_dev_noalloc(mchip_dbt t, mchip_metric m, const double* P_dev, double pscale, double* J_dev);
 
 /* DFMO-K: K = -(L^-1 B C)^T (L^-1 B C) for C = nbf x nocc (row-major, leading
  * dimension ldc), returned negated as the reference does
