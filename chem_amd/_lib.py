@@ -159,6 +159,8 @@ _SIGS = [
     ("mchip_dfk_bf16_shapes", c_int, [P_int, P_int, P_int]),
     ("mchip_dfk_ao_dev", c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
     ("mchip_dfk_ao", c_int, [c_void_p, c_void_p, P_double, c_int, c_int, P_double]),
+    ("mchip_schwarz_mask", c_int, [c_void_p, c_void_p, c_int, P_int, c_int, P_int, c_double, P_u8, P_double,
+                                   P_double]),
     ("mchip_dfk_lmo_uv", c_int, [c_void_p, c_void_p, P_double, P_double, c_int, c_int, c_int, P_double]),
     ("mchip_server_create", c_int, [c_int, c_double, c_int, c_int, ctypes.POINTER(c_void_p)]),
     ("mchip_server_destroy", c_int, [c_void_p]),

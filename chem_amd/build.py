@@ -67,6 +67,25 @@ def build_probe():
     return PROBE_LIB
 
 
+DRIVER_SRC = os.path.join(ROOT, "tests", "native", "fock_driver.cpp")
+DRIVER_EXE = os.path.join(ROOT, "tests", "native", "fock_driver")
+
+
+def build_driver():
+    """Test driver of the C++ builder layer (tests/native/fock_driver.cpp),
+    linked against the in-tree libmchip.so with a run path relative to itself."""
+    if not os.path.exists(DRIVER_SRC):
+        return None
+    tmp = DRIVER_EXE + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), DRIVER_SRC,
+           "-L", LIBDIR, "-lmchip", "-Wl,-rpath,$ORIGIN/../../chem_amd/lib", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("driver build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(tmp, DRIVER_EXE)
+    return DRIVER_EXE
+
+
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
@@ -81,6 +100,7 @@ def build(verbose=False):
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
     os.replace(tmp, LIB)
     build_probe()
+    build_driver()
     if verbose:
         print("built", LIB)
     return LIB

@@ -1,4 +1,6 @@
 // C ABI: basis sets, AO integrals and the host B container.
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -113,6 +115,44 @@ extern "C" int mchip_schwarz(mchip_basis ao, mchip_basis aux, double* Qao, doubl
   MCHIP_TRY
   if (Qao) schwarz_ao(ao->basis, Qao);
   if (Qaux) schwarz_aux(aux->basis, Qaux);
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_schwarz_mask(mchip_basis ao, mchip_basis aux, int nbb, const int* ao_blk_shells, int nxb,
+                                  const int* aux_blk_shells, double eps, uint8_t* mask, double* norm_mn,
+                                  double* norm_x)
+{
+  MCHIP_TRY
+  const int nsa = (int)ao->basis.shells.size(), nsx = (int)aux->basis.shells.size();
+  std::vector<int> boff(nbb + 1, 0), xoff(nxb + 1, 0);
+  for (int i = 0; i < nbb; ++i) boff[i + 1] = boff[i] + ao_blk_shells[i];
+  for (int i = 0; i < nxb; ++i) xoff[i + 1] = xoff[i] + aux_blk_shells[i];
+  if (boff[nbb] != nsa || xoff[nxb] != nsx)
+    throw std::runtime_error("mchip_schwarz_mask: block shell counts do not cover the basis");
+  std::vector<double> Qao((size_t)nsa * nsa), Qx(nsx);
+  schwarz_ao(ao->basis, Qao.data());
+  schwarz_aux(aux->basis, Qx.data());
+  // Frobenius block norms (sums in shell order, row by row)
+  std::vector<double> nmn((size_t)nbb * nbb), nx(nxb);
+  for (int M = 0; M < nbb; ++M)
+    for (int N = 0; N < nbb; ++N) {
+      double s = 0.0;
+      for (int i = boff[M]; i < boff[M + 1]; ++i)
+        for (int j = boff[N]; j < boff[N + 1]; ++j) s += Qao[(size_t)i * nsa + j] * Qao[(size_t)i * nsa + j];
+      nmn[(size_t)M * nbb + N] = std::sqrt(s);
+    }
+  for (int X = 0; X < nxb; ++X) {
+    double s = 0.0;
+    for (int i = xoff[X]; i < xoff[X + 1]; ++i) s += Qx[i] * Qx[i];
+    nx[X] = std::sqrt(s);
+  }
+  // keep (X, M, N) iff (float)(|Q_X| |Q_MN|) > eps: the product in double,
+  // rounded to float, compared in double (screening.cpp:23-30)
+  for (int X = 0; X < nxb; ++X)
+    for (int MN = 0; MN < nbb * nbb; ++MN)
+      mask[(size_t)X * nbb * nbb + MN] = (double)(float)(nx[X] * nmn[MN]) > eps ? 1 : 0;
+  if (norm_mn) std::copy(nmn.begin(), nmn.end(), norm_mn);
+  if (norm_x) std::copy(nx.begin(), nx.end(), norm_x);
   MCHIP_CATCH
 }
 

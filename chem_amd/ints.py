@@ -48,11 +48,20 @@ def int3c2e_shell(mol, ia, ib, ic):
 
 
 def schwarz_mask(mol, eps):
-    """Block mask of the Schwarz screener (screening.cpp:8-30)."""
-    Qao, Qx = schwarz(mol)
-    nmn = block_norms(Qao, mol.blocks)
-    nx = block_norms(Qx, mol.dfblocks)
-    return schwarz_block_mask(nmn, nx, eps), nmn, nx
+    """Block mask of the Schwarz screener (screening.cpp:8-30), computed by the
+    library (mchip_schwarz_mask, the same code the C++ layer uses).  Returns
+    (mask, block norms of the AO pair factors, block norms of the aux
+    factors)."""
+    nbb, nxb = len(mol.blocks), len(mol.dfblocks)
+    bsh = np.array([len(b) for b in mol.blocks], dtype=np.int32)
+    xsh = np.array([len(b) for b in mol.dfblocks], dtype=np.int32)
+    mask = np.zeros((nxb, nbb, nbb), dtype=np.uint8)
+    nmn = np.zeros((nbb, nbb))
+    nx = np.zeros(nxb)
+    _lib.check(_lib.lib().mchip_schwarz_mask(mol.basis_handle.h, mol.dfbasis_handle.h, nbb, _lib.iptr(bsh), nxb,
+                                             _lib.iptr(xsh), float(eps), _lib.u8ptr(mask), _lib.dptr(nmn),
+                                             _lib.dptr(nx)))
+    return mask, nmn, nx
 
 
 class BTensorHost:

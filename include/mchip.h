@@ -59,6 +59,13 @@ int mchip_int2c2e(mchip_basis aux, double* V);
 /* Schwarz factors: Qao nshell_ao^2 (sqrt sum|(ab|ab)|), Qaux nshell_aux.
  * aofactory::ao_schwarz / ao_3cschwarz (integrals.cpp:511-689). */
 int mchip_schwarz(mchip_basis ao, mchip_basis aux, double* Qao, double* Qaux);
+/* The Schwarz block mask of schwarz_screener (screening.cpp:8-30) for the
+ * given atomic blocking (shells per block, contiguous): Frobenius block norms
+ * of the shell factors, norm_mn nbb x nbb and norm_x nxb (either may be NULL),
+ * and mask[X][M][N] = (float)(norm_x[X] * norm_mn[M][N]) > eps, nxb*nbb*nbb
+ * bytes.  The one implementation the host and GPU tensor builds use. */
+int mchip_schwarz_mask(mchip_basis ao, mchip_basis aux, int nbb, const int* ao_blk_shells, int nxb,
+                       const int* aux_blk_shells, double eps, uint8_t* mask, double* norm_mn, double* norm_x);
 /* One shell triple (c|ab), out[m_c][m_a][m_b] (test hook). */
 int mchip_int3c2e_shell(mchip_basis ao, mchip_basis aux, int ia, int ib, int ic,
                         double* out);

@@ -60,8 +60,9 @@ def test_overlap_normalised_and_metric_spd():
 
 def test_cpp_builder_header(tmp_path):
     """include/mchip_fock.hpp compiles as C++17 against the C ABI, links with
-    libmchip.so, and reports a missing required argument as std::runtime_error
-    (no device call is made)."""
+    libmchip.so, and reports a missing required factory argument (the
+    reference's CHECK_REQUIRED) as std::runtime_error naming it, without a
+    device call; unsupported methods and metrics are refused the same way."""
     import subprocess
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     src = tmp_path / "b.cpp"
@@ -69,27 +70,24 @@ def test_cpp_builder_header(tmp_path):
 #include <cstdio>
 #include <cstring>
 #include "mchip_fock.hpp"
+static int expect(const char* what, const std::runtime_error& e) { return std::strstr(e.what(), what) ? 0 : 1; }
 int main() {
-  using namespace mchip::fock;
-  try {
-    auto k = create_k().method(kmethod::dfmo).occ_nbatches(2).build();
-    (void)k;
-    return 2;
-  } catch (const std::runtime_error& e) {
-    if (std::strstr(e.what(), "eri3c2e_batched") == nullptr) return 3;
-  }
-  try {
-    auto j = create_j().method(jmethod::dfao).build();
-    (void)j;
-    return 4;
-  } catch (const std::runtime_error&) {}
-  try {
-    auto z = mchip::mp::create_z().method(mchip::mp::zmethod::ll_full).build();
-    (void)z;
-    return 5;
-  } catch (const std::runtime_error& e) {
-    if (std::strstr(e.what(), "eri3c2e_batched") == nullptr) return 6;
-  }
+  using namespace mchip;
+  int bad = 0;
+  try { fock::create_k().method(fock::kmethod::dfmo).occ_nbatches(2).build(); return 2; }
+  catch (const std::runtime_error& e) { bad += expect("set_world", e); }
+  try { fock::create_j().build(); return 3; }
+  catch (const std::runtime_error& e) { bad += expect("method", e); }
+  try { mp::create_z().method(mp::zmethod::ll_full).build(); return 4; }
+  catch (const std::runtime_error& e) { bad += expect("set_world", e); }
+  try { ints::aoloader::create().build(); return 5; }
+  catch (const std::runtime_error& e) { bad += expect("set_world", e); }
+  try { fock::str_to_kmethod("nope"); return 6; }
+  catch (const std::runtime_error& e) { bad += expect("Invalid kmethod", e); }
+  if (fock::str_to_kmethod("dflmo") != fock::kmethod::dflmo || fock::str_to_jmethod("dfao") != fock::jmethod::dfao ||
+      mp::str_to_zmethod("llmp_mem") != mp::zmethod::llmp_mem) return 7;
+  if (desc::split_range(12, 5) != std::vector<int>({5, 5, 2})) return 8;
+  if (bad) return 9;
   std::printf("ok\n");
   return 0;
 }

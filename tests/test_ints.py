@@ -290,3 +290,17 @@ def test_schwarz_mask_rule():
                 assert m[x, i, j] == (1 if f > eps else 0)
     # the float rounding matters at the boundary: 1e-5 * 1e-5 rounds above 1e-10
     assert float(np.float32(1e-10)) > 1e-10 and m[0, 0, 0] == 1
+
+
+def test_library_schwarz_mask_matches_rule():
+    """The library's block mask (mchip_schwarz_mask, the one the host and GPU
+    tensor builds and the C++ layer use) is the float-cast rule applied to its
+    block norms, and those norms are the Frobenius norms of the shell factors."""
+    for n, seed in ((1, 0), (4, 2)):
+        mol = desc.Molecule(desc.water_cluster(n, seed))
+        mask, nmn, nx = ints.schwarz_mask(mol, 1e-10)
+        assert np.array_equal(mask, desc.schwarz_block_mask(nmn, nx, 1e-10))
+        Qao, Qx = ints.schwarz(mol)
+        assert np.allclose(nmn, desc.block_norms(Qao, mol.blocks), rtol=1e-14, atol=0)
+        assert np.allclose(nx, desc.block_norms(Qx, mol.dfblocks), rtol=1e-14, atol=0)
+        assert mask.sum() > 0
