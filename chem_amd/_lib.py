@@ -134,6 +134,7 @@ _SIGS = [
     ("mchip_ctx_set_timing", c_int, [c_void_p, c_int]),
     ("mchip_ctx_set_escalation", c_int, [c_void_p, c_double, c_int]),
     ("mchip_ctx_trim", c_int, [c_void_p]),
+    ("mchip_ctx_set_guess", c_int, [c_void_p, c_int, P_double]),
     ("mchip_dev_alloc", c_int, [c_void_p, ctypes.c_size_t, ctypes.POINTER(c_void_p)]),
     ("mchip_dev_free", c_int, [c_void_p, c_void_p]),
     ("mchip_dev_copy_in", c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_size_t]),
@@ -159,6 +160,10 @@ _SIGS = [
     ("mchip_dfk_bf16_shapes", c_int, [P_int, P_int, P_int]),
     ("mchip_dfk_ao_dev", c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
     ("mchip_dfk_ao", c_int, [c_void_p, c_void_p, P_double, c_int, c_int, P_double]),
+    ("mchip_int4c2e", c_int, [c_void_p, P_double]),
+    ("mchip_sad_guess", c_int, [c_void_p, c_int, P_double, P_double, P_double]),
+    ("mchip_sad_atom", c_int, [c_void_p, c_double, P_double, c_double, P_double, P_double, P_int, P_int,
+                               P_double]),
     ("mchip_schwarz_mask", c_int, [c_void_p, c_void_p, c_int, P_int, c_int, P_int, c_double, P_u8, P_double,
                                    P_double]),
     ("mchip_dfk_lmo_uv", c_int, [c_void_p, c_void_p, P_double, P_double, c_int, c_int, c_int, P_double]),

@@ -20,6 +20,7 @@ const double* btensor_data(mchip_btensor t);
 
 struct mchip_ctx_s {
   Ctx c;
+  std::vector<double> guess_P;   // first density of the next SCF on this context (empty: core guess)
 };
 struct mchip_dbt_s {
   DevBT t;
@@ -82,6 +83,19 @@ extern "C" int mchip_ctx_create(int device, mchip_ctx* out)
   ctx_init(c->c, device);   // makes `device` current
   (void)noalloc_scratch(device);
   *out = c.release();
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_ctx_set_guess(mchip_ctx c, int nbf, const double* P_alpha)
+{
+  MCHIP_TRY
+  if (!c) throw std::runtime_error("null context");
+  if (!P_alpha || nbf <= 0) {
+    c->guess_P.clear();
+  }
+  else {
+    c->guess_P.assign(P_alpha, P_alpha + (size_t)nbf * nbf);
+  }
   MCHIP_CATCH
 }
 
@@ -657,6 +671,11 @@ extern "C" int mchip_scf_run(mchip_dbt t, mchip_metric m, int nbf, int nocc, int
   in.diis_min = diis_min;
   in.diis_max = diis_max;
   in.filter_eps = filter_eps;
+  if (!t->ctx->guess_P.empty()) {
+    if ((int64_t)t->ctx->guess_P.size() != (int64_t)nbf * nbf)
+      throw std::runtime_error("mchip_scf_run: the guess density set on the context is not nbf x nbf");
+    in.guess_P.swap(t->ctx->guess_P);
+  }
   const Metric* met = m ? &m->m : nullptr;
   const Metric* jmet = t->t.transformed ? nullptr : met;
   KBuildFn kb, kfinal;

@@ -4,7 +4,9 @@
 //   one_electron   :131-181   X = orthogonalizer(S) (orthogonalizer.cpp:14-45,
 //                             eigenvalues < 0 -> 0, else 1/sqrt), H = T + V,
 //                             block filter(eps) of S, X, H
-//   guess "core"   hfguess.cpp:127-141
+//   guess "core"   hfguess.cpp:127-141; "SAD" hfguess.cpp:142-452 (density
+//                  from the host's atomic SCFs, first exchange on its
+//                  pivoted Cholesky factor)
 //   form_fock      :228-276   F = H + J + K
 //   compute_errmat :298-324   e = F P S - S P F  (formed as A - A^T, A = F P S)
 //   energy         :278-296   E = tr(H P) + tr(F P)
@@ -26,6 +28,7 @@
 
 #include "dfock.hpp"
 #include "kernels.hpp"
+#include "mp2.hpp"
 #include "scf.hpp"
 
 namespace mchip {
@@ -238,7 +241,23 @@ void scf_run(Ctx& c, DevBT& bt, const Metric* m, const ScfInput& in, ScfOutput& 
     }
   };
 
-  diag(H.as<double>());
+  // first density: the core guess, or the SAD density with its Cholesky
+  // factor for the first exchange (hfguess.cpp:423-452)
+  DevBuf Lsad;
+  int sad_rank = 0;
+  if (!in.guess_P.empty()) {
+    if ((int64_t)in.guess_P.size() != nn) throw std::runtime_error("SCF: the guess density is not nbf x nbf");
+    h2d(P.p, in.guess_P.data(), mb, c.stream);
+    DevBuf U(mb);
+    HIP_CHECK(hipMemcpyAsync(U.p, P.p, mb, hipMemcpyDeviceToDevice, c.stream));
+    std::vector<double> hL = pivoted_cholesky_dev(c, U, n, in.filter_eps, &sad_rank);
+    Lsad.alloc((size_t)n * std::max(sad_rank, 1) * sizeof(double));
+    if (sad_rank > 0) h2d(Lsad.p, hL.data(), hL.size() * sizeof(double), c.stream);
+  }
+  else {
+    diag(H.as<double>());
+  }
+  const double* t64 = bt.data64t.p ? bt.data64t.as<double>() : nullptr;
 
   DiisState diis;
   double energy = 0.0;
@@ -258,7 +277,11 @@ void scf_run(Ctx& c, DevBT& bt, const Metric* m, const ScfInput& in, ScfOutput& 
     float tj = 0.f;
     HIP_CHECK(hipEventElapsedTime(&tj, c.ev0, c.ev1));
     HIP_CHECK(hipEventRecord(c.ev0, c.stream));
-    if (kbuild(iter, prev_rms, P.as<double>(), Cocc.as<double>(), K.as<double>())) {
+    if (iter == 0 && !in.guess_P.empty()) {
+      // the SAD iteration: every column of the guess density's factor, FP64
+      dfk_mo(c, bt, m, Lsad.as<double>(), sad_rank, sad_rank, K.as<double>(), 1, t64);
+    }
+    else if (kbuild(iter, prev_rms, P.as<double>(), Cocc.as<double>(), K.as<double>())) {
       for (auto& b : diis.err) diis.spare.push_back(std::move(b));
       for (auto& b : diis.trial) diis.spare.push_back(std::move(b));
       diis.err.clear();

@@ -19,6 +19,11 @@ struct ScfInput {
   bool do_diis = true;
   int diis_start = 1, diis_min = 2, diis_max = 10;
   double filter_eps = 0.0;
+  // SAD guess (hfguess.cpp:142-452): the alpha density of the first
+  // iteration (row-major nbf x nbf; empty: the core guess).  Its first
+  // exchange contracts every column of the density's pivoted Cholesky factor
+  // (the SAD iteration of DFMO_K, jkdf.cpp:243-246).
+  std::vector<double> guess_P;
 };
 
 struct ScfOutput {

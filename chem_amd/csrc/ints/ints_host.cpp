@@ -485,6 +485,96 @@ void schwarz_ao(const Basis& bas, double* Q)
   }
 }
 
+// ---------------------------------------------------------------------------
+// four-centre integrals (ab|cd) (the reference's int2e_sph through libcint,
+// used by its exact J/K for the atomic SCFs of the SAD guess, hfguess.cpp:142-452)
+// ---------------------------------------------------------------------------
+// (ab|cd) = sum_{pairs} 2 pi^{5/2} / (p q sqrt(p+q)) K_ab K_cd
+//           sum_{tuv} G_ab[tuv] sum_{t'u'v'} (-1)^{t'+u'+v'} G_cd[t'u'v'] R_{t+t', u+u', v+v'}
+// (Helgaker et al., eq. 9.9.33), spherical components, layout out[sa][sb][sc][sd].
+void ints_4c2e_shell(const Basis& bas, int ia, int ib, int ic, int id, double* out)
+{
+  const double* btab = boys_table().data();
+  const Shell& a = bas.shells[ia];
+  const Shell& b = bas.shells[ib];
+  const Shell& cs = bas.shells[ic];
+  const Shell& ds = bas.shells[id];
+  const int L1 = a.l + b.l, L2 = cs.l + ds.l, NH1 = nherm(L1), NH2 = nherm(L2);
+  const int na = nsph(a.l), nb = nsph(b.l), nc = nsph(cs.l), nd = nsph(ds.l);
+  std::vector<PrimPair> p1, p2;
+  make_pairs(a, b, p1);
+  make_pairs(cs, ds, p2);
+  std::vector<double> G1(p1.size() * na * nb * NH1), G2(p2.size() * nc * nd * NH2);
+  for (size_t k = 0; k < p1.size(); ++k) pair_sph_hermite(a.l, b.l, p1[k], G1.data() + k * na * nb * NH1);
+  for (size_t k = 0; k < p2.size(); ++k) {
+    double* g = G2.data() + k * nc * nd * NH2;
+    pair_sph_hermite(cs.l, ds.l, p2[k], g);
+    for (int s = 0; s < nc * nd; ++s)
+      for (int t = 0; t <= L2; ++t)
+        for (int u = 0; u <= L2 - t; ++u)
+          for (int v = 0; v <= L2 - t - u; ++v)
+            if ((t + u + v) & 1) g[s * NH2 + herm_index(t, u, v)] = -g[s * NH2 + herm_index(t, u, v)];
+  }
+  std::fill(out, out + (size_t)na * nb * nc * nd, 0.0);
+  double W[(4 * LMAX + 1) * nherm(4 * LMAX)], R[nherm(4 * LMAX)];
+  std::vector<double> T(NH1);
+  for (size_t k1 = 0; k1 < p1.size(); ++k1)
+    for (size_t k2 = 0; k2 < p2.size(); ++k2) {
+      const PrimPair& q1 = p1[k1];
+      const PrimPair& q2 = p2[k2];
+      const double p = q1.p, q = q2.p, alpha = p * q / (p + q);
+      hermite_R(L1 + L2, alpha, q1.P[0] - q2.P[0], q1.P[1] - q2.P[1], q1.P[2] - q2.P[2], btab, W, R);
+      const double pref = 2.0 * std::pow(PI, 2.5) / (p * q * std::sqrt(p + q)) * q1.K * q2.K;
+      for (int cd = 0; cd < nc * nd; ++cd) {
+        const double* g2 = G2.data() + (k2 * nc * nd + cd) * NH2;
+        // T[tuv] = sum_{t'u'v'} g2[t'u'v'] R[t+t', u+u', v+v']
+        for (int t = 0; t <= L1; ++t)
+          for (int u = 0; u <= L1 - t; ++u)
+            for (int v = 0; v <= L1 - t - u; ++v) {
+              double acc = 0.0;
+              for (int t2 = 0; t2 <= L2; ++t2)
+                for (int u2 = 0; u2 <= L2 - t2; ++u2)
+                  for (int v2 = 0; v2 <= L2 - t2 - u2; ++v2)
+                    acc += g2[herm_index(t2, u2, v2)] * R[herm_index(t + t2, u + u2, v + v2)];
+              T[herm_index(t, u, v)] = acc;
+            }
+        for (int ab = 0; ab < na * nb; ++ab) {
+          const double* g1 = G1.data() + (k1 * na * nb + ab) * NH1;
+          double acc = 0.0;
+          for (int h = 0; h < NH1; ++h) acc += g1[h] * T[h];
+          out[(size_t)ab * nc * nd + cd] += pref * acc;
+        }
+      }
+    }
+}
+
+// Every (mu nu|lambda sigma) of a (small) basis, row-major nbf^4.
+void ints_4c2e_all(const Basis& bas, double* out)
+{
+  const int ns = (int)bas.shells.size(), n = bas.nbf;
+  std::vector<int> off(ns);
+  for (int i = 0; i < ns; ++i) off[i] = bas.shell_off[i];
+#pragma omp parallel for collapse(2) schedule(dynamic)
+  for (int ia = 0; ia < ns; ++ia)
+    for (int ib = 0; ib < ns; ++ib) {
+      std::vector<double> buf;
+      for (int ic = 0; ic < ns; ++ic)
+        for (int id = 0; id < ns; ++id) {
+          const int na = nsph(bas.shells[ia].l), nb = nsph(bas.shells[ib].l), nc = nsph(bas.shells[ic].l),
+                    nd = nsph(bas.shells[id].l);
+          buf.assign((size_t)na * nb * nc * nd, 0.0);
+          ints_4c2e_shell(bas, ia, ib, ic, id, buf.data());
+          for (int sa = 0; sa < na; ++sa)
+            for (int sb = 0; sb < nb; ++sb)
+              for (int sc = 0; sc < nc; ++sc)
+                for (int sd = 0; sd < nd; ++sd) {
+                  const size_t i = off[ia] + sa, j = off[ib] + sb, k = off[ic] + sc, l = off[id] + sd;
+                  out[((i * n + j) * n + k) * n + l] = buf[((sa * nb + sb) * nc + sc) * nd + sd];
+                }
+        }
+    }
+}
+
 void schwarz_aux(const Basis& aux, double* Q)
 {
   // diagonal of (X|X) summed per shell

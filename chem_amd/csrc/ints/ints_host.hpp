@@ -35,6 +35,11 @@ void aux_hermite_coeffs(int l, double gamma, double coef, double* H, int stride)
 // One shell triple (c | a b), layout out[m_c][sa][sb].
 void ints_3c2e_shell(const Basis& ao, const Basis& aux, int ia, int ib, int ic, double* out);
 
+// Four-centre (ab|cd) of one shell quartet, out[sa][sb][sc][sd], and every
+// (mu nu|lambda sigma) of a small basis (row-major nbf^4).
+void ints_4c2e_shell(const Basis& bas, int ia, int ib, int ic, int id, double* out);
+void ints_4c2e_all(const Basis& bas, double* out);
+
 // Screened, filtered, slab-laid-out B(X|mu nu).
 void build_B(const Basis& ao, const Basis& aux,
              const std::vector<int>& ao_blk_shells,

@@ -31,6 +31,15 @@ def int2c2e(mol):
     return M
 
 
+def int4c2e(mol):
+    """Exact four-centre ERIs (ij|kl) over the AO basis, nbf^4 row-major
+    (host; the SAD guess's atomic SCFs use them)."""
+    n = mol.nbf
+    out = np.zeros((n, n, n, n))
+    _lib.check(_lib.lib().mchip_int4c2e(mol.basis_handle.h, _lib.dptr(out)))
+    return out
+
+
 def schwarz(mol):
     Qao = np.zeros((len(mol.shells), len(mol.shells)))
     Qx = np.zeros(len(mol.dfshells))

@@ -30,9 +30,26 @@ class ScfResult:
     t_final_k: float = 0.0
 
 
+def sad_density(mol):
+    """SAD guess density (alpha, nbf x nbf) of the molecule: atomic fractional-
+    occupation SCFs with exact integrals on the host (mchip_sad_guess,
+    hfguess.cpp:142-452)."""
+    Z, xyz = mol.charges()
+    P = np.zeros((mol.nbf, mol.nbf))
+    _lib.check(_lib.lib().mchip_sad_guess(mol.basis_handle.h, len(Z), _lib.dptr(Z), _lib.dptr(xyz), _lib.dptr(P)))
+    return P
+
+
 def run_rhf(mol, dB, metric, S, H, threshold=1e-8, max_iter=100, do_diis=True, diis_start=1,
-            diis_min=2, diis_max=10, filter_eps=1e-10, kmode="dfmo", occ_nbatches=1):
+            diis_min=2, diis_max=10, filter_eps=1e-10, kmode="dfmo", occ_nbatches=1, guess="core"):
+    """guess: "core" (diagonalised core Hamiltonian) or "sad" (superposition of
+    atomic densities, the reference's default)."""
     n = mol.nbf
+    if guess == "sad":
+        Pg = np.ascontiguousarray(sad_density(mol))
+        _lib.check(_lib.lib().mchip_ctx_set_guess(dB.ctx.h, n, _lib.dptr(Pg)))
+    elif guess != "core":
+        raise ValueError("guess must be core or sad")
     bsz = np.array(mol.b, dtype=np.int32)
     S = np.ascontiguousarray(S, dtype=np.float64)
     H = np.ascontiguousarray(H, dtype=np.float64)

@@ -70,6 +70,24 @@ int mchip_schwarz_mask(mchip_basis ao, mchip_basis aux, int nbb, const int* ao_b
 int mchip_int3c2e_shell(mchip_basis ao, mchip_basis aux, int ia, int ib, int ic,
                         double* out);
 
+/* Four-centre (mu nu|lambda sigma) of a small basis, row-major nbf^4 (the
+ * reference's int2e_sph through libcint, used by its exact J/K). */
+int mchip_int4c2e(mchip_basis b, double* out);
+
+/* SAD guess (hfmod::compute_guess, src/hf/hfguess.cpp:142-452): per element,
+ * a restricted SCF of the atom alone in its shells of the molecule's basis
+ * with fractional, spin-averaged occupations (src/desc/molecule.cpp:37-93),
+ * exact four-centre J/K, core guess, no DIIS, RMS(FPS - SPF) < 1e-6; the
+ * atomic alpha densities on the diagonal blocks.  The shells of each atom
+ * must be consecutive, in atom order.  P_alpha: nbf x nbf row-major. */
+int mchip_sad_guess(mchip_basis ao, int natom, const double* Z, const double* xyz, double* P_alpha);
+/* The atomic SCF alone (test hook): atom_basis holds one atom's shells, Z its
+ * charge at xyz; P (nbf^2) its alpha density, energy the electronic energy
+ * tr(HP) + tr(FP), iterations the SCF count, nocc / occ (capacity nbf) the
+ * occupied orbitals and their coefficient factors (sqrt of the occupation). */
+int mchip_sad_atom(mchip_basis atom_basis, double Z, const double* xyz, double threshold, double* P,
+                   double* energy, int* iterations, int* nocc, double* occ);
+
 /* ------------------------------------------------------------------------
  * Three-centre tensor B(X|mu nu), block-sparse (host container).
  * Replaces dbcsr::btensor<3> core mode (src/dbcsrx/dbcsr_btensor.hpp) and the
@@ -138,6 +156,12 @@ int mchip_ctx_set_timing(mchip_ctx c, int on);
  * off; default 3).  FP32 gives way to FP64 after such a stall below 1e-6.  No
  * counterpart in the reference. */
 int mchip_ctx_set_escalation(mchip_ctx c, double rms, int stall);
+/* First density of the next mchip_scf_run on this context: the alpha density
+ * of a SAD guess (mchip_sad_guess; nbf x nbf row-major, copied), used by that
+ * one run, whose first exchange then contracts every column of its pivoted
+ * Cholesky factor (the SAD iteration, hfscf.cpp:389-396, jkdf.cpp:243-246).
+ * NULL clears it (core guess, hfguess.cpp:127-141). */
+int mchip_ctx_set_guess(mchip_ctx c, int nbf, const double* P_alpha);
 /* Release the context's work arrays (they grow to the largest exchange run on
  * the context and are kept for reuse; the plain FP64 exchange holds two of
  * naux * nbf * nocc doubles).  The next build allocates what it needs again.

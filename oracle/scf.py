@@ -16,10 +16,27 @@ Follows src/hf/hfscf.cpp:
                                   max 10; when full the error vector of largest
                                   norm is dropped; B-matrix Lagrange solve)
 
+  compute_guess "SAD"   src/hf/hfguess.cpp:142-452  per element an atomic
+                        SCF (fractional, spin-averaged occupations,
+                        src/desc/molecule.cpp:37-93; exact J and K; core
+                        guess; no DIIS; thr 1e-6, hfmod.hpp:50-53), the
+                        atomic alpha densities on the atoms' diagonal blocks,
+                        the first exchange on the density's pivoted Cholesky
+                        factor (hfguess.cpp:436-452)
+
 Pins: at convergence P S P = P, tr(P S) = nocc, |F P S - S P F| < thr; the
 energy does not depend on DIIS settings; a closed-shell energy from two
-independent K formulations agrees.
+independent K formulations agrees.  SAD: the occupation rule on atoms whose
+configurations are known (He, C, Na, Ar, ...); closed-shell atoms (He, Ne),
+where fractional and integer occupation coincide, reproduce the published
+RHF/cc-pVDZ energies; a one-function atom reproduces E = 2 f h + f^2 (ss|ss)
+(f = occupation), which fixes the occupation weighting and the 2J - K
+combination; tr(P S) = Z / 2.  The oracle's integrals (S, T, V, ERIs) are the
+library's, which are pinned on their own by closed forms and derivative
+relations.
 """
+import bisect
+
 import numpy as np
 
 
@@ -93,8 +110,13 @@ class DIIS:
 
 
 def rhf(S, T, V, Enuc, nocc, jk, bsz=None, filter_eps=1e-10, thr=1e-8, max_iter=100,
-        do_diis=True, diis_start=1, diis_min=2, diis_max=10, log=None):
-    """Restricted HF with a J/K callback jk(P_alpha, C_occ) -> (J, K)."""
+        do_diis=True, diis_start=1, diis_min=2, diis_max=10, log=None, occ=None, guess_P=None):
+    """Restricted HF with a J/K callback jk(P_alpha, C_occ) -> (J, K).
+
+    occ: factors of the occupied columns (fractional occupation,
+    hfdiagfock.cpp scales C_occ by them; P = C_occ diag(occ^2) C_occ^T).
+    guess_P: the first density (SAD); the first callback then gets its
+    pivoted Cholesky factor as C_occ (hfscf.cpp:389-396)."""
     nbf = S.shape[0]
     bsz = bsz or [nbf]
     X = orthogonalizer(S)
@@ -109,14 +131,26 @@ def rhf(S, T, V, Enuc, nocc, jk, bsz=None, filter_eps=1e-10, thr=1e-8, max_iter=
         C = X @ Cp
         return eps, C
 
-    eps, C = diag(H)
-    P = C[:, :nocc] @ C[:, :nocc].T
+    f = np.ones(nocc) if occ is None else np.asarray(occ, dtype=np.float64)
+
+    def density(C):
+        Co = C[:, :nocc] * f
+        return Co @ Co.T, Co
+
+    if guess_P is None:
+        eps, C = diag(H)
+        P, Co = density(C)
+    else:
+        from .mp2 import pivoted_cholesky
+        eps, C = None, None
+        P = np.array(guess_P, dtype=np.float64)
+        Co, _ = pivoted_cholesky(P, filter_eps=filter_eps)
     diis = DIIS(diis_start, diis_min, diis_max)
     E = 0.0
     it = 0
     history = []
     while True:
-        J, K = jk(P, C[:, :nocc])
+        J, K = jk(P, Co)
         F = H + J + K
         e = F @ P @ S - S @ P @ F
         E = float(np.sum(H * P) + np.sum(F * P))
@@ -132,7 +166,61 @@ def rhf(S, T, V, Enuc, nocc, jk, bsz=None, filter_eps=1e-10, thr=1e-8, max_iter=
             diis.compute_extrapolation_parameters(F, e, it)
             F = diis.extrapolate(F, it)
         eps, C = diag(F)
-        P = C[:, :nocc] @ C[:, :nocc].T
+        P, Co = density(C)
         it += 1
     return dict(energy=E + Enuc, e_elec=E, C=C, eps=eps, P=P, F=F, J=J, K=K,
                 iterations=it, history=history)
+
+
+CONF_ORB = (0, 2, 10, 18, 36, 54, 86, 118)   # electrons of the closed shells (hfguess.cpp:24)
+
+
+def sad_occupations(Z):
+    """molecule.cpp:37-93 (fractional, spin averaged, neutral): Z/2 alpha
+    electrons; the orbitals below the last closed shell under Z hold one
+    each, the next shell's orbitals share the rest.  Returns the factors of
+    the occupied columns (square roots of the occupations)."""
+    if not 1 <= Z <= 118:
+        raise ValueError("atomic number out of range")
+    i = bisect.bisect_left(CONF_ORB, Z)     # first closed-shell count >= Z
+    if Z < CONF_ORB[i]:
+        i -= 1
+    if CONF_ORB[i] == Z:
+        ncore, nact = 0, CONF_ORB[i] // 2
+    else:
+        ncore, nact = CONF_ORB[i] // 2, CONF_ORB[i + 1] // 2 - CONF_ORB[i] // 2
+    occ = np.ones(ncore + nact)
+    occ[ncore:] = np.sqrt((0.5 * Z - ncore) / nact)
+    return occ
+
+
+def exact_jk(eri):
+    """Restricted exact J and K of an atom (J[P_alpha + P_beta] and the
+    negated K[P_alpha]) from the full ERI tensor (ij|kl)."""
+    def jk(P, C):
+        J = np.einsum("ijkl,kl->ij", eri, P, optimize=True)
+        K = np.einsum("ikjl,kl->ij", eri, P, optimize=True)
+        return 2.0 * J, -K
+    return jk
+
+
+def atomic_scf(S, T, V, eri, Z, thr=1e-6, max_iter=100):
+    """The atomic SCF of the SAD guess (hfguess.cpp:248-292): fractional
+    occupation, exact J/K, core guess, no DIIS.  Returns rhf()'s dict."""
+    occ = sad_occupations(Z)
+    return rhf(S, T, V, 0.0, len(occ), exact_jk(eri), filter_eps=0.0, thr=thr, max_iter=max_iter,
+               do_diis=False, occ=occ)
+
+
+def sad_density(nbf, atoms, atomic_density):
+    """hfguess.cpp:398-413: the alpha density of the molecule, block diagonal,
+    atoms = [(Z, first function, number of functions)], atomic_density(Z) the
+    element's atomic alpha density (computed once per element from its first
+    atom, hfguess.cpp:146-160)."""
+    P = np.zeros((nbf, nbf))
+    cache = {}
+    for Z, off, n in atoms:
+        if Z not in cache:
+            cache[Z] = atomic_density(Z)
+        P[off:off + n, off:off + n] = cache[Z]
+    return P

@@ -216,3 +216,35 @@ def test_fp64_copy_exchange_and_scf(ctx):
     print("K rel %.2e; dE %.2e (%d iterations)" % (rel, mix.energy - ref.energy, mix.iterations))
     assert rel < 1e-12
     assert mix.converged and abs(mix.energy - ref.energy) < 1e-8
+
+
+@pytest.mark.parametrize("nwater,kmode", [(2, "dfmo"), (4, "mixed_auto")])
+def test_sad_guess(ctx, nwater, kmode):
+    """SAD guess (hfguess.cpp:142-452): the first iteration (J on the guess
+    density, K on its pivoted Cholesky factor) matches the oracle's SAD
+    iteration within 1e-10 Eh; the converged energy matches the oracle's and
+    the core-guess run's; SAD needs no more iterations than the core guess."""
+    from chem_amd.engine import DeviceB, Metric
+    from chem_amd.scf import run_rhf, sad_density
+    mol = desc.Molecule(desc.water_cluster(nwater, 7))
+    print("case: SAD guess water n=%d kmode=%s" % (nwater, kmode), flush=True)
+    S, T, V = ints.int1e(mol)
+    Bh = ints.BTensorHost.build(mol, 1e-10)
+    m = Metric(ctx, ints.int2c2e(mol))
+    dB = DeviceB(ctx, Bh)
+    if kmode == "mixed_auto":
+        dB.make_lowp(m, f32=True, bf16=True, f64=True)
+    sad = run_rhf(mol, dB, m, S, T + V, threshold=1e-8, kmode=kmode, guess="sad")
+    core = run_rhf(mol, dB, m, S, T + V, threshold=1e-8, kmode=kmode)
+    assert sad.converged and core.converged
+    print("E sad %.12f (%d it) core %.12f (%d it)" % (sad.energy, sad.iterations, core.energy, core.iterations))
+    assert abs(sad.energy - core.energy) < 1e-9
+    assert sad.iterations <= core.iterations
+    if kmode == "dfmo":
+        Linv, Vinv = m.get()
+        Bd = Bh.dense()
+        jk = lambda P, C: (dfock.df_j_ext(Bd, Vinv, P), dfock.dfmo_k(Bd, Linv.T, C, bsz=mol.b, xsz=mol.x))
+        ref = oscf.rhf(S, T, V, mol.nuclear_repulsion(), mol.nocc, jk, bsz=mol.b, thr=1e-8,
+                       guess_P=sad_density(mol))
+        assert abs(sad.history_e[0] - ref["history"][0][0]) < 1e-10
+        assert abs(sad.energy - ref["energy"]) < 1e-10

@@ -10,6 +10,9 @@ with the reference's own integrals: parity against libcint is unpinned).
   (so a wrong index, sign, Hermite term or spherical coefficient fails).
 * normalisation of every shell of the three basis sets (S diagonal = 1 and
   orthogonality inside a shell).
+* four-centre (ab|cd) (host, for the SAD guess's atomic SCFs): s closed form,
+  8-fold symmetry and positive semidefiniteness, and the limit of a vanishing
+  ket exponent, which turns (ab|cd) into the three-centre (ab|c) for every l.
 """
 import math
 
@@ -304,3 +307,61 @@ def test_library_schwarz_mask_matches_rule():
         assert np.allclose(nmn, desc.block_norms(Qao, mol.blocks), rtol=1e-14, atol=0)
         assert np.allclose(nx, desc.block_norms(Qx, mol.dfblocks), rtol=1e-14, atol=0)
         assert mask.sum() > 0
+
+
+def test_4c_s_closed_form():
+    """(ab|cd) over four s Gaussians: N 2 pi^(5/2) / (p q sqrt(p+q)) K_ab K_cd F_0(rho R_PQ^2)."""
+    ex = [0.8, 1.3, 0.6, 2.1]
+    cs = [np.array([0.1, -0.2, 0.3]), np.array([0.9, 0.4, -0.5]), np.array([-0.7, 0.2, 1.1]),
+          np.array([0.4, -0.8, -0.3])]
+    mol = _Mol([_shell(0, a, c) for a, c in zip(ex, cs)])
+    g = ints.int4c2e(mol)
+    a, b, c, d = ex
+    p, q = a + b, c + d
+    P = (a * cs[0] + b * cs[1]) / p
+    Q = (c * cs[2] + d * cs[3]) / q
+    Kab = math.exp(-a * b / p * float(((cs[0] - cs[1]) ** 2).sum()))
+    Kcd = math.exp(-c * d / q * float(((cs[2] - cs[3]) ** 2).sum()))
+    N = nrm_s(a) * nrm_s(b) * nrm_s(c) * nrm_s(d)
+    ref = N * 2 * math.pi ** 2.5 / (p * q * math.sqrt(p + q)) * Kab * Kcd * \
+        boys(0, p * q / (p + q) * float(((P - Q) ** 2).sum()))[0]
+    assert abs(g[0, 1, 2, 3] - ref) < 1e-14 * abs(ref)
+
+
+def test_4c_permutational_symmetry():
+    """(ij|kl) = (ji|kl) = (ij|lk) = (kl|ij) over mixed s/p/d/f shells, and
+    (ij|ij) >= 0."""
+    sh = [_shell(2, 0.9, [0.2, -0.1, 0.4]), _shell(1, 0.5, [-0.6, 0.5, 0.1]), _shell(0, 1.4, [0.3, 0.8, -0.9]),
+          _shell(3, 0.7, [0.0, 0.1, 0.2]), _shell(1, 2.2, [0.2, -0.1, 0.4])]
+    g = ints.int4c2e(_Mol(sh))
+    tol = 1e-14 * np.abs(g).max()
+    assert np.abs(g - g.transpose(1, 0, 2, 3)).max() < tol
+    assert np.abs(g - g.transpose(0, 1, 3, 2)).max() < tol
+    assert np.abs(g - g.transpose(2, 3, 0, 1)).max() < tol
+    n = g.shape[0]
+    d = np.einsum("ijij->ij", g)
+    assert (d >= -tol).all()
+    # the pair matrix (ij|kl) is positive semidefinite
+    w = np.linalg.eigvalsh(g.reshape(n * n, n * n))
+    assert w.min() > -1e-12 * w.max()
+
+
+@pytest.mark.parametrize("la,lb,lc", [(0, 0, 0), (1, 0, 0), (2, 1, 0), (0, 0, 2), (1, 1, 1), (2, 2, 3), (3, 0, 1)])
+def test_4c_against_3c_limit(la, lb, lc):
+    """(ab|c d) with d an s Gaussian of vanishing exponent on the centre of c
+    tends to N_d (ab|c), the three-centre integral with c as auxiliary
+    function: the four-centre engine against the separately pinned
+    three-centre one, for every l on the bra and on the ket."""
+    A = [0.2, -0.1, 0.4]
+    Bc = [-0.6, 0.5, 0.1]
+    C = [0.3, 0.8, -0.9]
+    eta = 1e-10
+    a, b, g = 1.1, 0.6, 0.5
+    mol4 = _Mol([_shell(la, a, A), _shell(lb, b, Bc), _shell(lc, g, C), _shell(0, eta, C)])
+    g4 = ints.int4c2e(mol4)
+    na, nb, nc = 2 * la + 1, 2 * lb + 1, 2 * lc + 1
+    got = g4[:na, na:na + nb, na + nb:na + nb + nc, -1] / nrm_s(eta)
+    mol3 = _Mol([_shell(la, a, A), _shell(lb, b, Bc)], [_shell(lc, g, C)])
+    ref = ints.int3c2e_shell(mol3, 0, 1, 0)       # [c][a][b]
+    ref = ref.transpose(1, 2, 0)
+    assert np.abs(got - ref).max() < 1e-8 * max(1.0, np.abs(ref).max()), (la, lb, lc)
