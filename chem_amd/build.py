@@ -86,6 +86,49 @@ def build_driver():
     return DRIVER_EXE
 
 
+WFN_SRC = os.path.join(ROOT, "chem_amd", "wfn", "wfn_h5.cpp")
+WFN_LIB = os.path.join(LIBDIR, "libmchip_wfn.so")
+HDF5_ROOT = os.environ.get("MCHIP_HDF5_ROOT", "/opt/conda")
+
+
+def build_wfn():
+    """libmchip_wfn.so: the HDF5 wavefunction files (host only), against the
+    HDF5 C library of HDF5_ROOT (headers searched after the system ones, run
+    path to its lib).  Skipped when that HDF5 is absent."""
+    if not os.path.exists(os.path.join(HDF5_ROOT, "include", "hdf5.h")):
+        return None
+    tmp = WFN_LIB + ".tmp"
+    # libstdc++ linked in statically: the run path must not pull HDF5_ROOT's
+    # C++ runtime into processes that load the system one
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-Werror", "-static-libstdc++",
+           "-static-libgcc", "-idirafter", os.path.join(HDF5_ROOT, "include"), WFN_SRC,
+           "-L", os.path.join(HDF5_ROOT, "lib"), "-lhdf5",
+           "-Wl,-rpath," + os.path.join(HDF5_ROOT, "lib"), "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("wfn build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(tmp, WFN_LIB)
+    return WFN_LIB
+
+
+WFN_DRIVER_SRC = os.path.join(ROOT, "tests", "native", "wfn_driver.cpp")
+WFN_DRIVER_EXE = os.path.join(ROOT, "tests", "native", "wfn_driver")
+
+
+def build_wfn_driver():
+    """Test driver of the C++ wavefunction-file layer (tests/native/wfn_driver.cpp)."""
+    if not (os.path.exists(WFN_DRIVER_SRC) and os.path.exists(WFN_LIB)):
+        return None
+    tmp = WFN_DRIVER_EXE + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), WFN_DRIVER_SRC,
+           "-L", LIBDIR, "-lmchip_wfn", "-Wl,-rpath,$ORIGIN/../../chem_amd/lib", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("wfn driver build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(tmp, WFN_DRIVER_EXE)
+    return WFN_DRIVER_EXE
+
+
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
@@ -101,6 +144,8 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     build_probe()
     build_driver()
+    build_wfn()
+    build_wfn_driver()
     if verbose:
         print("built", LIB)
     return LIB
