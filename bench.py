@@ -573,11 +573,11 @@ def main():
     d = kernels[dom]
     # the exact kernel instances the autotuned plans run, for the PMC figures
     exact = {}
-    for key, form, pname in (("k_slab_gemm_bf16x3", "bf16x3", "bf16_plan"), ("k_seg_syrk_bf16x3", "bf16x3", "bf16_plan"),
+    for key, kform, pname in (("k_slab_gemm_bf16x3", "bf16x3", "bf16_plan"), ("k_seg_syrk_bf16x3", "bf16x3", "bf16_plan"),
                              ("k_slab_gemm_f32_dl", "f32", "f32_plan"), ("k_seg_syrk_f32_dl", "f32", "f32_plan"),
                              ("k_slab_gemm_f64_dl", "f64", "f64_plan"), ("k_seg_syrk_f64_dl", "f64", "f64_plan")):
         if key in kernels and kernels.get(pname):
-            k1, k3 = exact_kernels(form, kernels[pname])
+            k1, k3 = exact_kernels(kform, kernels[pname])
             exact[key] = k1 if key.startswith("k_slab") else k3
             kernels[key]["instance"] = exact[key]
     traffic = measured_traffic().get(exact.get(dom, dom))
