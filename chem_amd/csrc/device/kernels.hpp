@@ -76,7 +76,7 @@ __global__ void k_slab_gemm_bf16x3_dla(const uint16_t* bhi, const uint16_t* blo,
                                        const int* boff, const Tile* tiles, int naux, int nbf,
                                        const uint16_t* ghi, const uint16_t* glo, int ng, uint16_t* whi,
                                        uint16_t* wlo, int ldo);
-template <int S, int BM>
+template <int S, int BM, int NW>
 __global__ void k_seg_syrk_bf16x3_dl(const uint16_t* whi, const uint16_t* wlo, int nseg, int64_t sstride,
                                      int ldw, int n, const int2* tiles, int ntile, int segs_per_chunk,
                                      float* partial);

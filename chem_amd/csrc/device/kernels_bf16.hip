@@ -727,16 +727,19 @@ __global__ __launch_bounds__(64 * WM * WN, (dl_min_blocks<WM, WN, TM, TN>())) vo
 
 // Segmented SYRK over bf16 hi/lo W, direct-to-LDS pipelined: partial[chunk](m,n)
 // = sum_{X in chunk} sum_i W(X,m,i) W(X,n,i) on the upper BM x BM tiles.
-// BM = 128: 4 waves of 64 x 64; BM = 256: 8 waves of 128 x 64 (each W row
-// block is fetched by half as many tiles).  Rows of W have ldw (multiple of
-// 32) bf16.
-template <int S, int BM>
-__global__ __launch_bounds__(BM * 2) void k_seg_syrk_bf16x3_dl(
+// NW waves in two rows of BM/2 rows each: BM = 128: 4 waves of 64 x 64;
+// BM = 256: 8 waves of 128 x 64 (each W row block is fetched by half as many
+// tiles).  (4 waves of 128 x 128 at BM = 256, one wave per SIMD with half the
+// LDS fragment reads per MFMA, measured the same: the step is bound by the
+// operand fetches into LDS.)  Rows of W have ldw (multiple of 32) bf16.
+template <int S, int BM, int NW>
+__global__ __launch_bounds__(64 * NW) void k_seg_syrk_bf16x3_dl(
     const uint16_t* __restrict__ whi, const uint16_t* __restrict__ wlo, int nseg, int64_t sstride, int ldw,
     int n, const int2* __restrict__ tiles, int ntile, int segs_per_chunk, float* __restrict__ partial)
 {
   static_assert(BM == 128 || BM == 256, "SYRK tile");
-  constexpr int NW = BM / 32, WN = NW / 2;
+  static_assert(NW == 4 || NW == BM / 32, "SYRK wave layout");
+  constexpr int WN = NW / 2;
   constexpr int TM = BM / 32, TN = BM / WN / 16;
   constexpr int LPW = (4 * BM / 16) / NW;
   constexpr int STAGE = 2 * BM * 8;
@@ -764,19 +767,29 @@ __global__ __launch_bounds__(BM * 2) void k_seg_syrk_bf16x3_dl(
   const int ld8 = ldw >> 3;
   const uint4* Wh = reinterpret_cast<const uint4*>(whi);
   const uint4* Wl = reinterpret_cast<const uint4*>(wlo);
-  dl_pipeline<S, LPW>(
-      nsteps,
-      [&](int q, auto k) {
-        const int sg = s0 + q / ksteps;
-        const int kc0 = (q % ksteps) * 4;
-        const int64_t base = (int64_t)sg * (sstride >> 3);
-        issue_stage<BM, BM, NW>(pick_stage<S>(k, lds0, lds1, lds2, lds3), wave, lane, Wh + base, Wl + base,
-                                Wh + base, Wl + base, m0, n - 1, n0, n - 1, ld8, ld8, kc0);
-      },
-      [&](int, auto k) {
-        const uint4* st = pick_stage<S>(k, lds0, lds1, lds2, lds3);
-        bf16x3_step_swz<TM, TN>(st, st + BM * 4, st + BM * 8, st + BM * 12, wm0, wn0, lane, acc);
-      });
+  // a diagonal tile (m0 == n0) has the same W rows on both sides: its stages
+  // hold them once, with half the loads per wave (the waits count them)
+  auto run = [&](auto diag) {
+    constexpr bool D = decltype(diag)::value;
+    dl_pipeline<S, D ? LPW / 2 : LPW>(
+        nsteps,
+        [&](int q, auto k) {
+          const int sg = s0 + q / ksteps;
+          const int kc0 = (q % ksteps) * 4;
+          const int64_t base = (int64_t)sg * (sstride >> 3);
+          issue_stage<BM, D ? 0 : BM, NW>(pick_stage<S>(k, lds0, lds1, lds2, lds3), wave, lane, Wh + base,
+                                          Wl + base, Wh + base, Wl + base, m0, n - 1, n0, n - 1, ld8, ld8, kc0);
+        },
+        [&](int, auto k) {
+          const uint4* st = pick_stage<S>(k, lds0, lds1, lds2, lds3);
+          const uint4* sb = D ? st : st + BM * 8;
+          bf16x3_step_swz<TM, TN>(st, st + BM * 4, sb, sb + BM * 4, wm0, wn0, lane, acc);
+        });
+  };
+  if (m0 == n0)
+    run(std::true_type{});
+  else
+    run(std::false_type{});
   float* P = partial + (int64_t)chunk * n * n;   // the FP32 chunk sum, exactly
   const int fc = lane & 15, fr4 = 4 * (lane >> 4);
 #pragma unroll
@@ -810,14 +823,16 @@ MCHIP_SLAB16DLA_INST(2, 4, 4, 5, 4)
 MCHIP_SLAB16DLA_INST(2, 4, 4, 5, 3)
 MCHIP_SLAB16DLA_INST(2, 4, 4, 3, 4)
 MCHIP_SLAB16DLA_INST(2, 2, 4, 5, 2)
+MCHIP_SLAB16DLA_INST(2, 4, 6, 5, 3)
+MCHIP_SLAB16DLA_INST(2, 4, 6, 5, 2)
 #undef MCHIP_SLAB16DLA_INST
-#define MCHIP_SYRK16DL_INST(S, BM)                                                                     \
-  template __global__ void k_seg_syrk_bf16x3_dl<S, BM>(const uint16_t*, const uint16_t*, int, int64_t, int, \
-                                                       int, const int2*, int, int, float*);
-MCHIP_SYRK16DL_INST(2, 128)
-MCHIP_SYRK16DL_INST(3, 128)
-MCHIP_SYRK16DL_INST(4, 128)
-MCHIP_SYRK16DL_INST(2, 256)
+#define MCHIP_SYRK16DL_INST(S, BM, NW)                                                                 \
+  template __global__ void k_seg_syrk_bf16x3_dl<S, BM, NW>(const uint16_t*, const uint16_t*, int, int64_t,   \
+                                                           int, int, const int2*, int, int, float*);
+MCHIP_SYRK16DL_INST(2, 128, 4)
+MCHIP_SYRK16DL_INST(3, 128, 4)
+MCHIP_SYRK16DL_INST(4, 128, 4)
+MCHIP_SYRK16DL_INST(2, 256, 8)
 #undef MCHIP_SYRK16DL_INST
 
 #define MCHIP_SLAB16_INST(WM, WN, TM, TN, NB)                                                          \
