@@ -181,6 +181,7 @@ def serve_stream(args, log_progress=True):
     mols = {}
     results = []
     with Server(budget_bytes=cfg["budget"], nslots=args.slots) as srv:
+        srv.set_batching(not args.no_batch)
         t0 = time.perf_counter()
         ids = []
         for i in range(nreq):
@@ -207,6 +208,7 @@ def serve_stream(args, log_progress=True):
                         rid, sizes[i], r["energy"], r["iterations"], r["latency"], r["t_admit"] - r["t_submit"],
                         r["slot"], r["t_host"], r["t_build"], r["t_setup"], r["t_scf"]))
         st = srv.stats()
+        bst = srv.batch_stats()
     ok = [r for r in results if r is not None]
     lat = np.array([r["latency"] for r in ok])
     span = max(r["t_done"] for r in ok) - min(r["t_submit"] for r in ok)
@@ -236,6 +238,8 @@ def serve_stream(args, log_progress=True):
                 failures=failures[:5], all_converged=all(r["converged"] for r in ok),
                 max_abs_dE_vs_fp64=float(max(dEs)) if dEs else None, dE_sample=len(dEs),
                 max_abs_dE_vs_fp64_both_1e8=float(max(dEt)) if dEt else None,
+                cross_request_batching=not args.no_batch, exchange_batches=bst["batches"],
+                batched_builds=bst["builds"], largest_batch=bst["largest"],
                 mean_phase_s={k: float(np.mean([r[k] for r in ok])) for k in ("t_host", "t_build", "t_setup",
                                                                               "t_scf")})
 
@@ -342,6 +346,8 @@ def main():
                     help="auto form: iterations without progress before BF16x3 gives way to FP32 (0: off)")
     ap.add_argument("--requests", type=int, default=0, help="c4: number of requests (0: 200)")
     ap.add_argument("--slots", type=int, default=4, help="c4: request workers")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="c4: no cross-request batching of the exchange builds (each on its worker's stream)")
     ap.add_argument("--serve-check", type=int, default=4,
                     help="c4: requests re-run on the plain FP64 path for the energy check")
     ap.add_argument("--no-reorder", action="store_true",

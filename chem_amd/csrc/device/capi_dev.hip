@@ -532,6 +532,33 @@ extern "C" int mchip_dfk_mo_bf16_dev(mchip_dbt t, const double* C_dev, int ldc, 
   MCHIP_CATCH
 }
 
+extern "C" int mchip_dfk_mo_bf16_group_dev(int n, const mchip_dbt* ts, const double* const* C_dev, const int* ldc,
+                                           const int* nocc, double* const* K_dev)
+{
+  MCHIP_TRY
+  if (n <= 0) return MCHIP_OK;
+  if (!ts || !C_dev || !ldc || !nocc || !K_dev) throw std::runtime_error("mchip_dfk_mo_bf16_group_dev: null argument");
+  mchip_ctx ctx = ts[0]->ctx;
+  need_device(ctx);
+  Ctx& c = ctx->c;
+  std::vector<DevBT*> tv(n);
+  std::vector<ExGroupReq> reqs(n);
+  std::vector<double*> K(n);
+  for (int j = 0; j < n; ++j) {
+    if (!ts[j] || ts[j]->ctx != ctx) throw std::runtime_error("mchip_dfk_mo_bf16_group_dev: the tensors must share one context");
+    for (int i = 0; i < j; ++i)
+      if (ts[i] == ts[j]) throw std::runtime_error("mchip_dfk_mo_bf16_group_dev: a tensor appears twice");
+    check_occ("mchip_dfk_mo_bf16_group_dev", ldc[j], nocc[j]);
+    if (nocc[j] <= 0) throw std::runtime_error("mchip_dfk_mo_bf16_group_dev: every build needs nocc > 0");
+    tv[j] = &ts[j]->t;
+    K[j] = K_dev[j];
+    dfk_bf16_group_prepare(c, *tv[j], C_dev[j], ldc[j], nocc[j], reqs[j]);
+  }
+  static thread_local GroupBufs gb;
+  dfk_bf16_group_launch(c.stream, tv, reqs, K, gb);
+  MCHIP_CATCH
+}
+
 extern "C" int mchip_dfk_mo_bf16(mchip_dbt t, const double* C, int nocc, int occ_nbatches, double* K)
 {
   MCHIP_TRY
@@ -895,6 +922,20 @@ extern "C" int mchip_server_result(mchip_server s, int id, int wait, int* done, 
       times[9] = j.t_scf;
     }
   }
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_server_set_batching(mchip_server s, int enabled)
+{
+  MCHIP_TRY
+  s->s->set_batching(enabled != 0);
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_server_batch_stats(mchip_server s, double* stats)
+{
+  MCHIP_TRY
+  s->s->batch_stats(stats);
   MCHIP_CATCH
 }
 

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <algorithm>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -175,6 +176,8 @@ DevBuf upload_vec(const std::vector<T>& v, hipStream_t s)
   return b;
 }
 
+struct DevBT;
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -190,6 +193,10 @@ struct Ctx {
   double escalate_rms = 1e-5;
   int escalate_stall = 3;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // When set, the BF16x3 exchange of one occupied batch is handed to this
+  // function (the request loop's cross-request batcher) instead of being
+  // launched on this context's stream.
+  std::function<void(Ctx&, DevBT&, const double*, int, int, double*)> bf16_exchange;
   // optional per-phase kernel timing (HIP events on this stream), seconds
   bool timing = false;
   hipEvent_t tev[2] = {nullptr, nullptr};
@@ -228,6 +235,25 @@ struct Tile {
   int pad;
 };
 
+// One build of a grouped BF16x3 exchange (several tensors in one launch):
+// the tensor's B~ hi/lo copy and layout, its gathered coefficients, W and
+// the SYRK chunk partials.
+struct ExGroupReq {
+  const uint16_t* bhi;
+  const uint16_t* blo;
+  const int64_t* off16;
+  const int* ldp;
+  const int64_t* ldp_pre;
+  const int* bsz;
+  const int* boff;
+  const uint16_t* ghi;
+  const uint16_t* glo;
+  uint16_t* whi;
+  uint16_t* wlo;
+  float* partial;
+  int naux, nbf, ng, ldw, spc, pad;
+};
+
 // Autotuned exchange pipeline of one precision (BF16x3, FP32 or FP64): slab-GEMM
 // shape, SYRK variant (-1 = not chosen yet) and the timings (ms, -1 = not
 // tried) they were chosen from.
@@ -256,6 +282,7 @@ struct DevBT {
   int64_t maxR = 0;                         // max bsz[N]*ncol[N]
   bool transformed = false;  // data64 holds L^-1 B
   bool lowp_tilde = false;   // data32 / data_hi,lo hold L^-1 B (from either route)
+  DevBuf gx_w, gx_g, gx_part;   // work arrays of this tensor's builds in a grouped BF16x3 exchange
   std::map<std::pair<int, int>, DevBuf> tiles;   // (BN, ng) -> tile table
   std::map<std::pair<int, int>, int> ntiles;
 };

@@ -864,11 +864,199 @@ int dfk_bf16_tuned_bn(const DevBT& t, int ng)
 
 // K = -sum_X W_X^T W_X with W = B~ C, both contractions BF16x3 on MFMA, FP32
 // accumulation inside a chunk of aux functions, FP64 across chunks.
+// ---------------------------------------------------------------------------
+// Grouped BF16x3 exchange: the builds of several tensors in one slab-GEMM
+// launch and one SYRK launch (the request loop's cross-request batching).
+// Every output element sums the same products in the same order as in
+// dfk_mo_bf16 (the same k steps of the slab GEMM, the same 32-aux chunks of
+// the SYRK), so a grouped build equals the build alone bit for bit.
+// ---------------------------------------------------------------------------
+
+// SYRK chunks of 32 aux functions at most, as in dfk_mo_bf16.
+static void bf16_chunks(int naux, int nbf, int* nch_out, int* spc_out)
+{
+  const int nt = (nbf + SG_BM - 1) / SG_BM;
+  const int ntile = nt * (nt + 1) / 2;
+  int nch = std::max(1, std::min(naux, 2048 / std::max(1, ntile)));
+  int spc = (naux + nch - 1) / nch;
+  if (spc > 32) spc = 32;
+  *nch_out = (naux + spc - 1) / spc;
+  *spc_out = spc;
+}
+
+void dfk_bf16_group_prepare(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, ExGroupReq& r)
+{
+  const BLayout& L = t.L;
+  if (!t.lowp_tilde || !t.data_hi.p || t.h_ldp.size() != (size_t)L.nbb)
+    throw std::runtime_error("BF16 copy of L^-1 B missing (mchip_dbt_make_lowp or mchip_dbt_make_bf16)");
+  if (nocc <= 0 || nocc > ldc) throw std::runtime_error("grouped BF16 exchange: nocc out of range");
+  const int naux = L.naux, nbf = L.nbf;
+  const int ldw = (nocc + 31) / 32 * 32;
+  const size_t nW = (size_t)naux * nbf * ldw;
+  const size_t nG = (size_t)t.h_ldp_pre[L.nbb] * nocc;
+  int nch = 1, spc = 1;
+  bf16_chunks(naux, nbf, &nch, &spc);
+  t.gx_w.ensure(nW * 2 * sizeof(uint16_t) + 256);
+  t.gx_g.ensure(2 * ((nG + 127) / 128 * 128) * sizeof(uint16_t) + 256);
+  t.gx_part.ensure((size_t)nch * nbf * nbf * sizeof(float));
+  r.bhi = t.data_hi.as<uint16_t>();
+  r.blo = t.data_lo.as<uint16_t>();
+  r.off16 = t.off16.as<int64_t>();
+  r.ldp = t.ldp.as<int>();
+  r.ldp_pre = t.ldp_pre.as<int64_t>();
+  r.bsz = t.bsz.as<int>();
+  r.boff = t.boff.as<int>();
+  uint16_t* ghi = t.gx_g.as<uint16_t>();
+  uint16_t* glo = ghi + (nG + 127) / 128 * 128;
+  r.ghi = ghi;
+  r.glo = glo;
+  r.whi = t.gx_w.as<uint16_t>();
+  r.wlo = r.whi + nW;
+  r.partial = t.gx_part.as<float>();
+  r.naux = naux;
+  r.nbf = nbf;
+  r.ng = nocc;
+  r.ldw = ldw;
+  r.spc = spc;
+  r.pad = nch;
+  int gmax = 0;
+  for (int n = 0; n < L.nbb; ++n) gmax = std::max(gmax, t.h_ldp[n]);
+  if (gmax > 0 && L.nbb > 0)
+    hipLaunchKernelGGL(k_gather_ct, dim3(nblk((int64_t)nocc * gmax, 256), L.nbb), dim3(256), 0, c.stream, C, ldc,
+                       nocc, t.ncol.as<int>(), t.ldp.as<int>(), t.ldp_pre.as<int64_t>(), t.colmap.as<int>(),
+                       t.colmap_off.as<int>(), ghi, glo);
+  HIP_CHECK(hipGetLastError());
+}
+
+// Shapes of the grouped launches: slab GEMM 128 x 320 (8 waves) for more than
+// 160 occupied columns, 128 x 160 (4 waves) otherwise; SYRK 256-row tiles
+// above 512 basis functions, 128-row tiles otherwise.
+static int group_k1_kind(const ExGroupReq& r) { return r.ng > 160 ? 0 : 1; }
+static int group_k3_kind(const ExGroupReq& r) { return r.nbf > 512 ? 0 : 1; }
+
+void dfk_bf16_group_launch(hipStream_t s, const std::vector<DevBT*>& ts, const std::vector<ExGroupReq>& reqs,
+                           const std::vector<double*>& K, GroupBufs& gb)
+{
+  const int nj = (int)reqs.size();
+  if (nj == 0) return;
+  if ((int)ts.size() != nj || (int)K.size() != nj) throw std::runtime_error("grouped BF16 exchange: job lists differ");
+  gb.reqs.ensure(nj * sizeof(ExGroupReq));
+  HIP_CHECK(hipMemcpyAsync(gb.reqs.p, reqs.data(), nj * sizeof(ExGroupReq), hipMemcpyHostToDevice, s));
+  // slab-GEMM tiles of every build, per shape: whole slabs dealt to 8 XCD
+  // segments heaviest first (as bf16_tiles does for one tensor)
+  for (int kind = 0; kind < 2; ++kind) {
+    const int BM = 128, BN = kind == 0 ? 320 : 160;
+    struct SlabWork {
+      int job, slab;
+      double work;
+    };
+    std::vector<SlabWork> sw;
+    for (int j = 0; j < nj; ++j) {
+      if (group_k1_kind(reqs[j]) != kind) continue;
+      const BLayout& L = ts[j]->L;
+      const int ncb = (reqs[j].ng + BN - 1) / BN;
+      for (int n = 0; n < L.nbb; ++n) {
+        const int64_t rows = (int64_t)L.naux * L.bsz[n];
+        sw.push_back({j, n, (double)((rows + BM - 1) / BM * ncb) * (ts[j]->h_ldp[n] / 32 + 1)});
+      }
+    }
+    if (sw.empty()) continue;
+    std::stable_sort(sw.begin(), sw.end(), [](const SlabWork& a, const SlabWork& b) { return a.work > b.work; });
+    std::vector<std::vector<Tile>> seg(8);
+    double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (const SlabWork& w : sw) {
+      int best = 0;
+      for (int x = 1; x < 8; ++x)
+        if (load[x] < load[best]) best = x;
+      load[best] += w.work;
+      const BLayout& L = ts[w.job]->L;
+      const int64_t rows = (int64_t)L.naux * L.bsz[w.slab];
+      for (int64_t m0 = 0; m0 < rows; m0 += BM)
+        for (int n0 = 0; n0 < reqs[w.job].ng; n0 += BN) seg[best].push_back(Tile{w.slab, (int)m0, n0, w.job});
+    }
+    size_t len = 0;
+    for (auto& v : seg) len = std::max(len, v.size());
+    std::vector<Tile> v;
+    v.reserve(8 * len);
+    for (auto& sv : seg) {
+      v.insert(v.end(), sv.begin(), sv.end());
+      v.resize(v.size() + len - sv.size(), Tile{-1, 0, 0, 0});
+    }
+    if (v.size() >= ((size_t)1 << 31)) throw std::runtime_error("grouped slab GEMM: too many tiles");
+    DevBuf& tb = gb.tiles[kind];
+    tb.ensure(v.size() * sizeof(Tile));
+    HIP_CHECK(hipMemcpyAsync(tb.p, v.data(), v.size() * sizeof(Tile), hipMemcpyHostToDevice, s));
+    if (kind == 0)
+      hipLaunchKernelGGL((k_slab_gemm_bf16x3_dla_group<2, 4, 4, 5, 3>), dim3((unsigned)v.size()), dim3(512), 0, s,
+                         gb.reqs.as<ExGroupReq>(), tb.as<Tile>());
+    else
+      hipLaunchKernelGGL((k_slab_gemm_bf16x3_dla_group<2, 2, 4, 5, 2>), dim3((unsigned)v.size()), dim3(256), 0, s,
+                         gb.reqs.as<ExGroupReq>(), tb.as<Tile>());
+    HIP_CHECK(hipGetLastError());
+    // the host tables must outlive the asynchronous copies above
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
+  // SYRK items (build, chunk, tile row, tile column): the tiles of one chunk
+  // consecutive in one XCD segment, chunks dealt to the least loaded segment
+  for (int kind = 0; kind < 2; ++kind) {
+    const int BM = kind == 0 ? 256 : 128;
+    std::vector<std::vector<int4>> seg(8);
+    double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    bool any = false;
+    for (int j = 0; j < nj; ++j) {
+      const ExGroupReq& r = reqs[j];
+      if (group_k3_kind(r) != kind) continue;
+      any = true;
+      const int nt = (r.nbf + BM - 1) / BM;
+      const double cost = (double)(nt * (nt + 1) / 2) * r.spc * (r.ldw / 32);
+      for (int ch = 0; ch < r.pad; ++ch) {
+        int best = 0;
+        for (int x = 1; x < 8; ++x)
+          if (load[x] < load[best]) best = x;
+        load[best] += cost;
+        for (int a = 0; a < nt; ++a)
+          for (int b = a; b < nt; ++b) seg[best].push_back(int4{j, ch, a, b});
+      }
+    }
+    if (!any) continue;
+    size_t len = 0;
+    for (auto& v : seg) len = std::max(len, v.size());
+    std::vector<int4> v;
+    v.reserve(8 * len);
+    for (auto& sv : seg) {
+      v.insert(v.end(), sv.begin(), sv.end());
+      v.resize(v.size() + len - sv.size(), int4{-1, 0, 0, 0});
+    }
+    if (v.size() >= ((size_t)1 << 31)) throw std::runtime_error("grouped SYRK: too many items");
+    DevBuf& ib = gb.items[kind];
+    ib.ensure(v.size() * sizeof(int4));
+    HIP_CHECK(hipMemcpyAsync(ib.p, v.data(), v.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+    if (kind == 0)
+      hipLaunchKernelGGL((k_seg_syrk_bf16x3_dl_group<2, 256, 8>), dim3((unsigned)v.size()), dim3(512), 0, s,
+                         gb.reqs.as<ExGroupReq>(), ib.as<int4>());
+    else
+      hipLaunchKernelGGL((k_seg_syrk_bf16x3_dl_group<2, 128, 4>), dim3((unsigned)v.size()), dim3(256), 0, s,
+                         gb.reqs.as<ExGroupReq>(), ib.as<int4>());
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
+  for (int j = 0; j < nj; ++j) {
+    const int nbf = reqs[j].nbf;
+    hipLaunchKernelGGL(k_reduce_sym<float>, dim3(nblk((int64_t)nbf * nbf, 256)), dim3(256), 0, s, reqs[j].partial,
+                       reqs[j].pad, nbf, 0, -1.0, K[j]);
+    HIP_CHECK(hipGetLastError());
+  }
+}
+
 void dfk_mo_bf16(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K, int occ_nbatches)
 {
   const BLayout& L = t.L;
   if (!t.lowp_tilde || !t.data_hi.p || t.h_ldp.size() != (size_t)L.nbb)
     throw std::runtime_error("BF16 copy of L^-1 B missing (mchip_dbt_make_lowp or mchip_dbt_make_bf16)");
+  if (c.bf16_exchange && nocc > 0 && occ_nbatches <= 1) {
+    c.bf16_exchange(c, t, C, ldc, nocc, K);
+    return;
+  }
   const int naux = L.naux, nbf = L.nbf;
   if (nocc <= 0) {
     hipLaunchKernelGGL(k_fill, dim3(gs_blocks((int64_t)nbf * nbf)), dim3(256), 0, c.stream, K,
@@ -902,10 +1090,8 @@ void dfk_mo_bf16(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K
   const int2* d_tl = t.tiles[tkey].as<int2>();
   // chunks of at most 32 aux functions: the FP32 sum inside a chunk stays short
   // (3 MFMA accumulations per 32 occupied columns per aux function)
-  int nch = std::max(1, std::min(naux, 2048 / std::max(1, ntile)));
-  int spc = (naux + nch - 1) / nch;
-  if (spc > 32) spc = 32;
-  nch = (naux + spc - 1) / spc;
+  int nch = 1, spc = 1;
+  bf16_chunks(naux, nbf, &nch, &spc);
   // chunk partials are the FP32 chunk sums, kept exactly in FP32
   c.scratch_partial.ensure((size_t)nb * nch * nbf * nbf * sizeof(float));
   int gmax = 0;

@@ -40,6 +40,17 @@ void dbt_make_lowp(Ctx& c, DevBT& t, const Metric& m, bool f32, bool bf16, bool 
 // into out (L.total floats, slab layout); t is left unchanged.
 void tilde_to_f32(Ctx& c, DevBT& t, const Metric& m, float* out);
 void dfk_mo_bf16(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K, int occ_nbatches);
+// Grouped BF16x3 exchange over several tensors (one occupied batch each): the
+// coefficients of one build are gathered on its owner's stream c.stream
+// (prepare), then every build runs in one slab-GEMM and one SYRK launch on
+// stream s (launch), K[j] = -sum W^T W of build j, the same bits as
+// dfk_mo_bf16 gives.  gb keeps the launch tables between calls.
+struct GroupBufs {
+  DevBuf reqs, tiles[2], items[2];
+};
+void dfk_bf16_group_prepare(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, ExGroupReq& r);
+void dfk_bf16_group_launch(hipStream_t s, const std::vector<DevBT*>& ts, const std::vector<ExGroupReq>& reqs,
+                           const std::vector<double*>& K, GroupBufs& gb);
 int dfk_bf16_tuned_bn(const DevBT& t, int ng);
 void dfk_bf16_plan(const DevBT& t, int ng, int* plan, float* ms, int* nms);
 // Tuned FP32 exchange pipeline for ng (same fields as dfk_bf16_plan; plan[4] = 1, plan[5] = stages).

@@ -605,12 +605,11 @@ __global__ __launch_bounds__(64 * WM * WN, (dl_min_blocks<WM, WN, TM, TN>())) vo
 // whole cycle of stages, so a step waits only for its own stage's loads.
 // Same operands, tile table and output as k_slab_gemm_bf16x3_dl (bf16 hi/lo W).
 template <int WM, int WN, int TM, int TN, int SA>
-__global__ __launch_bounds__(64 * WM * WN, (dl_min_blocks<WM, WN, TM, TN>())) void k_slab_gemm_bf16x3_dla(
-    const uint16_t* __restrict__ bhi, const uint16_t* __restrict__ blo, const int64_t* __restrict__ off16,
-    const int* __restrict__ ldp, const int64_t* __restrict__ ldp_pre, const int* __restrict__ bsz,
-    const int* __restrict__ boff, const Tile* __restrict__ tiles, int naux, int nbf,
-    const uint16_t* __restrict__ ghi, const uint16_t* __restrict__ glo, int ng, uint16_t* __restrict__ whi,
-    uint16_t* __restrict__ wlo, int ldo)
+__device__ __forceinline__ void slab_bf16x3_dla_tile(
+    const Tile& t, const uint16_t* __restrict__ bhi, const uint16_t* __restrict__ blo,
+    const int64_t* __restrict__ off16, const int* __restrict__ ldp, const int64_t* __restrict__ ldp_pre,
+    const int* __restrict__ bsz, const int* __restrict__ boff, int naux, int nbf, const uint16_t* __restrict__ ghi,
+    const uint16_t* __restrict__ glo, int ng, uint16_t* __restrict__ whi, uint16_t* __restrict__ wlo, int ldo)
 {
   static_assert(SA >= 2 && SA <= 4, "B~ pipeline depth");
   constexpr int NW = WM * WN;
@@ -633,8 +632,6 @@ __global__ __launch_bounds__(64 * WM * WN, (dl_min_blocks<WM, WN, TM, TN>())) vo
     if constexpr (decltype(k)::value % 2 == 0) return sb0;
     else return sb1;
   };
-  const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
-  if (t.slab < 0) return;
   const int N = t.slab;
   const int bn = bsz[N];
   const int ld8 = ldp[N] >> 3;
@@ -725,6 +722,34 @@ __global__ __launch_bounds__(64 * WM * WN, (dl_min_blocks<WM, WN, TM, TN>())) vo
   store_w_tile<TM, TN>(acc, t, wm0, wn0, lane, rows, bn, nbf, boff[N], ng, nullptr, whi, wlo, ldo);
 }
 
+template <int WM, int WN, int TM, int TN, int SA>
+__global__ __launch_bounds__(64 * WM * WN, (dl_min_blocks<WM, WN, TM, TN>())) void k_slab_gemm_bf16x3_dla(
+    const uint16_t* __restrict__ bhi, const uint16_t* __restrict__ blo, const int64_t* __restrict__ off16,
+    const int* __restrict__ ldp, const int64_t* __restrict__ ldp_pre, const int* __restrict__ bsz,
+    const int* __restrict__ boff, const Tile* __restrict__ tiles, int naux, int nbf,
+    const uint16_t* __restrict__ ghi, const uint16_t* __restrict__ glo, int ng, uint16_t* __restrict__ whi,
+    uint16_t* __restrict__ wlo, int ldo)
+{
+  const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  if (t.slab < 0) return;
+  slab_bf16x3_dla_tile<WM, WN, TM, TN, SA>(t, bhi, blo, off16, ldp, ldp_pre, bsz, boff, naux, nbf, ghi, glo, ng,
+                                           whi, wlo, ldo);
+}
+
+// The same slab GEMM over the tiles of several exchange builds at once (one
+// launch for the live requests of the request loop): tile.pad is the build's
+// index in reqs, whose operands and W it reads and writes.
+template <int WM, int WN, int TM, int TN, int SA>
+__global__ __launch_bounds__(64 * WM * WN, (dl_min_blocks<WM, WN, TM, TN>())) void k_slab_gemm_bf16x3_dla_group(
+    const ExGroupReq* __restrict__ reqs, const Tile* __restrict__ tiles)
+{
+  const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  if (t.slab < 0) return;
+  const ExGroupReq& r = reqs[__builtin_amdgcn_readfirstlane(t.pad)];   // uniform: scalar loads
+  slab_bf16x3_dla_tile<WM, WN, TM, TN, SA>(t, r.bhi, r.blo, r.off16, r.ldp, r.ldp_pre, r.bsz, r.boff, r.naux, r.nbf,
+                                           r.ghi, r.glo, r.ng, r.whi, r.wlo, r.ldw);
+}
+
 // Segmented SYRK over bf16 hi/lo W, direct-to-LDS pipelined: partial[chunk](m,n)
 // = sum_{X in chunk} sum_i W(X,m,i) W(X,n,i) on the upper BM x BM tiles.
 // NW waves in two rows of BM/2 rows each: BM = 128: 4 waves of 64 x 64;
@@ -732,10 +757,12 @@ __global__ __launch_bounds__(64 * WM * WN, (dl_min_blocks<WM, WN, TM, TN>())) vo
 // tiles).  (4 waves of 128 x 128 at BM = 256, one wave per SIMD with half the
 // LDS fragment reads per MFMA, measured the same: the step is bound by the
 // operand fetches into LDS.)  Rows of W have ldw (multiple of 32) bf16.
+// One output tile tl of one aux chunk of the SYRK.
 template <int S, int BM, int NW>
-__global__ __launch_bounds__(64 * NW) void k_seg_syrk_bf16x3_dl(
-    const uint16_t* __restrict__ whi, const uint16_t* __restrict__ wlo, int nseg, int64_t sstride, int ldw,
-    int n, const int2* __restrict__ tiles, int ntile, int segs_per_chunk, float* __restrict__ partial)
+__device__ __forceinline__ void syrk_bf16x3_dl_tile(const uint16_t* __restrict__ whi,
+                                                    const uint16_t* __restrict__ wlo, int nseg, int64_t sstride,
+                                                    int ldw, int n, int chunk, int2 tl, int segs_per_chunk,
+                                                    float* __restrict__ partial)
 {
   static_assert(BM == 128 || BM == 256, "SYRK tile");
   static_assert(NW == 4 || NW == BM / 32, "SYRK wave layout");
@@ -745,14 +772,8 @@ __global__ __launch_bounds__(64 * NW) void k_seg_syrk_bf16x3_dl(
   constexpr int STAGE = 2 * BM * 8;
   static_assert(S >= 2 && S <= 4, "pipeline depth");
   __shared__ uint4 lds0[STAGE], lds1[STAGE], lds2[STAGE], lds3[STAGE];
-  // all tiles of a chunk run on one XCD, dispatched together: the W rows of
-  // the chunk come from HBM about once per XCD and are re-read from its L2
-  // (grid padded to a multiple of 8; padding blocks exit)
-  const int lid = xcd_tile(blockIdx.x, gridDim.x);
-  const int chunk = lid / ntile;
   const int s0 = chunk * segs_per_chunk;
   if (s0 >= nseg) return;
-  const int2 tl = tiles[lid % ntile];
   const int s1 = min(nseg, s0 + segs_per_chunk);
   const int m0 = tl.x * BM, n0 = tl.y * BM;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -804,6 +825,32 @@ __global__ __launch_bounds__(64 * NW) void k_seg_syrk_bf16x3_dl(
       }
 }
 
+template <int S, int BM, int NW>
+__global__ __launch_bounds__(64 * NW) void k_seg_syrk_bf16x3_dl(
+    const uint16_t* __restrict__ whi, const uint16_t* __restrict__ wlo, int nseg, int64_t sstride, int ldw,
+    int n, const int2* __restrict__ tiles, int ntile, int segs_per_chunk, float* __restrict__ partial)
+{
+  // all tiles of a chunk run on one XCD, dispatched together: the W rows of
+  // the chunk come from HBM about once per XCD and are re-read from its L2
+  // (grid padded to a multiple of 8; padding blocks exit)
+  const int lid = xcd_tile(blockIdx.x, gridDim.x);
+  syrk_bf16x3_dl_tile<S, BM, NW>(whi, wlo, nseg, sstride, ldw, n, lid / ntile, tiles[lid % ntile], segs_per_chunk,
+                                 partial);
+}
+
+// The same SYRK over the work items of several exchange builds at once: item
+// (request, chunk, tile row, tile column), request < 0 for padding.
+template <int S, int BM, int NW>
+__global__ __launch_bounds__(64 * NW) void k_seg_syrk_bf16x3_dl_group(const ExGroupReq* __restrict__ reqs,
+                                                                      const int4* __restrict__ items)
+{
+  const int4 it = items[xcd_tile(blockIdx.x, gridDim.x)];
+  if (it.x < 0) return;
+  const ExGroupReq& r = reqs[__builtin_amdgcn_readfirstlane(it.x)];   // uniform: scalar loads
+  syrk_bf16x3_dl_tile<S, BM, NW>(r.whi, r.wlo, r.naux, (int64_t)r.nbf * r.ldw, r.ldw, r.nbf, it.y, int2{it.z, it.w},
+                                 r.spc, r.partial);
+}
+
 #define MCHIP_SLAB16DL_INST(WM, WN, TM, TN, S)                                                         \
   template __global__ void k_slab_gemm_bf16x3_dl<WM, WN, TM, TN, S>(                                  \
       const uint16_t*, const uint16_t*, const int64_t*, const int*, const int64_t*, const int*,         \
@@ -834,6 +881,10 @@ MCHIP_SYRK16DL_INST(3, 128, 4)
 MCHIP_SYRK16DL_INST(4, 128, 4)
 MCHIP_SYRK16DL_INST(2, 256, 8)
 #undef MCHIP_SYRK16DL_INST
+template __global__ void k_slab_gemm_bf16x3_dla_group<2, 4, 4, 5, 3>(const ExGroupReq*, const Tile*);
+template __global__ void k_slab_gemm_bf16x3_dla_group<2, 2, 4, 5, 2>(const ExGroupReq*, const Tile*);
+template __global__ void k_seg_syrk_bf16x3_dl_group<2, 256, 8>(const ExGroupReq*, const int4*);
+template __global__ void k_seg_syrk_bf16x3_dl_group<2, 128, 4>(const ExGroupReq*, const int4*);
 
 #define MCHIP_SLAB16_INST(WM, WN, TM, TN, NB)                                                          \
   template __global__ void k_slab_gemm_bf16x3<WM, WN, TM, TN, NB>(                                     \

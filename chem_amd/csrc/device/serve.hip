@@ -89,9 +89,106 @@ static std::vector<int> block_functions(const Basis& b, const std::vector<int>& 
   return out;
 }
 
+ExchangeBatcher::ExchangeBatcher(int device) : device_(device)
+{
+  HIP_CHECK(hipSetDevice(device_));
+  HIP_CHECK(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+  th_ = std::thread([this] { loop(); });
+}
+
+ExchangeBatcher::~ExchangeBatcher()
+{
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    stop_ = true;
+  }
+  cv_.notify_all();
+  th_.join();
+  (void)hipStreamDestroy(s_);
+}
+
+void ExchangeBatcher::stats(double* s) const
+{
+  std::lock_guard<std::mutex> lk(mu_);
+  s[0] = (double)nbatch_;
+  s[1] = (double)njob_;
+  s[2] = (double)maxbatch_;
+}
+
+void ExchangeBatcher::exchange(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K)
+{
+  // one pair of events per worker thread (each worker owns one context)
+  static thread_local hipEvent_t ready = nullptr, done = nullptr;
+  if (!ready) {
+    HIP_CHECK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    HIP_CHECK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+  }
+  Job j;
+  j.t = &t;
+  j.K = K;
+  j.ready = ready;
+  j.done = done;
+  dfk_bf16_group_prepare(c, t, C, ldc, nocc, j.r);
+  HIP_CHECK(hipEventRecord(ready, c.stream));
+  {
+    std::unique_lock<std::mutex> lk(mu_);
+    q_.push_back(&j);
+    cv_.notify_all();
+    cv_.wait(lk, [&] { return j.finished; });
+  }
+  if (!j.error.empty()) throw std::runtime_error("batched exchange: " + j.error);
+  HIP_CHECK(hipStreamWaitEvent(c.stream, done, 0));
+}
+
+void ExchangeBatcher::loop()
+{
+  HIP_CHECK(hipSetDevice(device_));
+  while (true) {
+    std::vector<Job*> batch;
+    {
+      std::unique_lock<std::mutex> lk(mu_);
+      cv_.wait(lk, [&] { return stop_ || !q_.empty(); });
+      if (q_.empty()) return;   // stopping
+      batch.assign(q_.begin(), q_.end());
+      q_.clear();
+    }
+    std::string err;
+    try {
+      std::vector<DevBT*> ts;
+      std::vector<ExGroupReq> reqs;
+      std::vector<double*> K;
+      for (Job* j : batch) {
+        HIP_CHECK(hipStreamWaitEvent(s_, j->ready, 0));
+        ts.push_back(j->t);
+        reqs.push_back(j->r);
+        K.push_back(j->K);
+      }
+      dfk_bf16_group_launch(s_, ts, reqs, K, gb_);
+      for (Job* j : batch) HIP_CHECK(hipEventRecord(j->done, s_));
+    }
+    catch (const std::exception& e) {
+      err = e.what();
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      for (Job* j : batch) {
+        j->error = err;
+        j->finished = true;
+      }
+      ++nbatch_;
+      njob_ += (long)batch.size();
+      maxbatch_ = std::max(maxbatch_, (int)batch.size());
+    }
+    cv_.notify_all();
+    // one batch in flight: builds handed in meanwhile form the next batch
+    if (err.empty()) (void)hipStreamSynchronize(s_);
+  }
+}
+
 Server::Server(int device, size_t budget_bytes, int nslots, int cpu_threads)
     : device_(device), budget_(budget_bytes), nslots_(std::max(1, nslots)), t0_(now_s())
 {
+  batcher_ = std::make_unique<ExchangeBatcher>(device_);
   cpu_per_worker_ = std::max(1, cpu_threads / nslots_);
   ctx_.resize(nslots_);
   for (int i = 0; i < nslots_; ++i) {
@@ -114,6 +211,7 @@ Server::~Server()
   }
   cv_.notify_all();
   for (auto& w : workers_) w.join();
+  batcher_.reset();
   for (auto& c : ctx_) ctx_free(*c);
 }
 
@@ -382,7 +480,19 @@ void Server::run(Ctx& c, SrvJob& j, bool& admitted)
     KBuildFn kb, kfinal;
     select_k_builders(c, t, &m, j.kmode, nocc, 1, j.threshold, kb, kfinal, j.final_occ_nbatches);
     ScfOutput out;
-    scf_run(c, t, t.transformed ? nullptr : &m, in, out, kb, kfinal);
+    // the BF16x3 builds of this request join the cross-request batches
+    if (batching_)
+      c.bf16_exchange = [this](Ctx& cc, DevBT& tt, const double* C, int ldc, int no, double* K) {
+        batcher_->exchange(cc, tt, C, ldc, no, K);
+      };
+    try {
+      scf_run(c, t, t.transformed ? nullptr : &m, in, out, kb, kfinal);
+    }
+    catch (...) {
+      c.bf16_exchange = nullptr;
+      throw;
+    }
+    c.bf16_exchange = nullptr;
     j.t_scf = now_s() - tp;
     j.energy = out.energy;
     j.energy_iter = out.energy_iter;

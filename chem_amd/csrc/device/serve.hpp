@@ -1,6 +1,7 @@
 // Request loop serving DF-HF requests on one GPU (see serve.hip).
 #pragma once
 
+#include <atomic>
 #include <condition_variable>
 #include <deque>
 #include <memory>
@@ -12,6 +13,7 @@
 #include "../ints/basis.hpp"
 #include "../ints/ints_host.hpp"
 #include "dev.hpp"
+#include "dfock.hpp"
 
 namespace mchip {
 
@@ -36,6 +38,42 @@ struct SrvJob {
   int iterations = 0, nbf = 0, naux = 0;
 };
 
+// Cross-request batching of the BF16x3 exchange: workers hand their builds
+// in, one thread launches everything queued as one grouped slab-GEMM and one
+// grouped SYRK (dfk_bf16_group_launch) on its own stream, one batch in flight
+// at a time, so that builds arriving while a batch runs form the next one.
+class ExchangeBatcher {
+ public:
+  explicit ExchangeBatcher(int device);
+  ~ExchangeBatcher();
+  // From a worker with context c: gathers the coefficients on c.stream, joins
+  // the next batch and returns once it is launched; c.stream then waits for
+  // the batch before K is read.  Results equal the build alone bit for bit.
+  void exchange(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K);
+  void stats(double* s) const;   // batches, builds, largest batch
+
+ private:
+  struct Job {
+    DevBT* t = nullptr;
+    ExGroupReq r{};
+    double* K = nullptr;
+    hipEvent_t ready = nullptr, done = nullptr;
+    bool finished = false;
+    std::string error;
+  };
+  void loop();
+  int device_;
+  hipStream_t s_ = nullptr;
+  mutable std::mutex mu_;
+  std::condition_variable cv_;
+  std::deque<Job*> q_;
+  bool stop_ = false;
+  GroupBufs gb_;
+  long nbatch_ = 0, njob_ = 0;
+  int maxbatch_ = 0;
+  std::thread th_;
+};
+
 class Server {
  public:
   Server(int device, size_t budget_bytes, int nslots, int cpu_threads);
@@ -44,6 +82,10 @@ class Server {
   bool result(int id, bool wait, SrvJob& out);   // copy of the bookkeeping and result
   void stats(double* s) const;   // in_use, peak, completed, failed, max live requests
   double clock() const;          // seconds since the server started
+  // Cross-request batching of the BF16x3 exchange builds (on by default);
+  // applies to requests that start after the call.
+  void set_batching(bool on) { batching_ = on; }
+  void batch_stats(double* s) const { batcher_->stats(s); }
 
  private:
   int take();
@@ -63,6 +105,8 @@ class Server {
   size_t in_use_ = 0, peak_ = 0;
   int live_ = 0, max_live_ = 0, completed_ = 0, failed_ = 0;
   bool stop_ = false;
+  std::atomic<bool> batching_{true};
+  std::unique_ptr<ExchangeBatcher> batcher_;
   std::vector<std::unique_ptr<Ctx>> ctx_;
   std::vector<std::thread> workers_;
 };

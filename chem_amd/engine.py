@@ -266,6 +266,35 @@ class DeviceB:
             rc = fn(self.h, C_dev.ptr, ldc, nocc, occ_nbatches, K_dev.ptr)
         _lib.check(rc)
 
+    @staticmethod
+    def dfk_bf16_group(builds):
+        """BF16x3 exchanges of several tensors in one grouped launch:
+        builds = [(DeviceB, C_dev, nocc, K_dev)], all on one context; each
+        K_dev equals the build alone (dfk_dev "bf16x3") bit for bit."""
+        n = len(builds)
+        if n == 0:
+            return
+        ctx = builds[0][0].ctx
+        for dB, C_dev, nocc, K_dev in builds:
+            if dB.ctx is not ctx:
+                raise ValueError("dfk_bf16_group: the tensors must share one context")
+            for name, a in (("C_dev", C_dev), ("K_dev", K_dev)):
+                if getattr(a, "ptr", None) is None or not a.ptr.value:
+                    raise ValueError("dfk_bf16_group: %s has been released" % name)
+                if len(a.shape) != 2 or a.shape[0] != dB.nbf:
+                    raise ValueError("dfk_bf16_group: %s has shape %s, expected (%d, n)" % (name, a.shape, dB.nbf))
+            if K_dev.shape != (dB.nbf, dB.nbf):
+                raise ValueError("dfk_bf16_group: K_dev has shape %s" % (K_dev.shape,))
+            if not 0 < nocc <= C_dev.shape[1]:
+                raise ValueError("dfk_bf16_group: nocc %d outside 1..%d" % (nocc, C_dev.shape[1]))
+        vp = ctypes.c_void_p * n
+        ts = vp(*[b[0].h for b in builds])
+        cs = vp(*[b[1].ptr.value for b in builds])
+        ks = vp(*[b[3].ptr.value for b in builds])
+        ldc = np.array([b[1].shape[1] for b in builds], dtype=np.int32)
+        nocc = np.array([b[2] for b in builds], dtype=np.int32)
+        _lib.check(_lib.lib().mchip_dfk_mo_bf16_group_dev(n, ts, cs, _lib.iptr(ldc), _lib.iptr(nocc), ks))
+
     def dfk_ao(self, P, metric=None, form="fp64", occ_nbatches=1):
         """Exchange from a density (DFAO_K / DFMEM_K): pivoted Cholesky of P on
         the GPU, then the occupied-factor exchange.  form: fp64, f32, bf16."""

@@ -60,6 +60,15 @@ class Server:
         return dict(bytes_in_use=s[0], peak_bytes=s[1], completed=int(s[2]), failed=int(s[3]),
                     max_live=int(s[4]), clock=s[5])
 
+    def set_batching(self, on):
+        """Cross-request batching of the BF16x3 exchange builds (on by default)."""
+        _lib.check(_lib.lib().mchip_server_set_batching(self.h, int(bool(on))))
+
+    def batch_stats(self):
+        s = np.zeros(3)
+        _lib.check(_lib.lib().mchip_server_batch_stats(self.h, _lib.dptr(s)))
+        return dict(batches=int(s[0]), builds=int(s[1]), largest=int(s[2]))
+
     def clock(self):
         return self.stats()["clock"]
 

@@ -262,6 +262,15 @@ int mchip_dfk_mo_tilde64_dev(mchip_dbt t, const double* C_dev, int ldc, int nocc
                              double* K_dev);
 int mchip_dfk_mo_bf16_dev(mchip_dbt t, const double* C_dev, int ldc, int nocc, int occ_nbatches,
                           double* K_dev);
+/* The BF16x3 exchange of n tensors on one context in one slab-GEMM launch
+ * and one SYRK launch over all their tiles (one occupied batch each; tensor j
+ * with device C_dev[j], row-major nbf_j x ldc[j], first nocc[j] > 0 columns,
+ * into K_dev[j]).  Each K_dev[j] equals what mchip_dfk_mo_bf16_dev gives for
+ * that tensor alone, bit for bit.  The reference builds one molecule per
+ * process; this is the contraction the request loop batches across live
+ * requests.  No synchronisation (the context's stream). */
+int mchip_dfk_mo_bf16_group_dev(int n, const mchip_dbt* ts, const double* const* C_dev, const int* ldc,
+                                const int* nocc, double* const* K_dev);
 int mchip_dfk_mo_bf16(mchip_dbt t, const double* C, int nocc, int occ_nbatches, double* K);
 /* Exchange from a density matrix P (nbf x nbf, symmetric PSD): DFAO_K::compute_K
  * (src/fock/jkdf.cpp:510-648) and DFMEM_K::compute_K (jkdf.cpp:719-859),
@@ -382,6 +391,14 @@ int mchip_server_submit(mchip_server s, mchip_basis ao, mchip_basis aux, int nat
 int mchip_server_result(mchip_server s, int id, int wait, int* done, double* energy, int* iterations,
                         int* converged, double* times);
 int mchip_server_stats(mchip_server s, double* stats);
+/* Cross-request batching of the BF16x3 exchange builds (on by default; for
+ * requests that start after the call): the builds that live requests hand in
+ * while a batch runs form the next batch, launched as one grouped slab GEMM
+ * and one grouped SYRK (mchip_dfk_mo_bf16_group_dev).  A request's result is
+ * the same bits alone and in any batch.  batch_stats[3] = {batches, builds,
+ * largest batch}. */
+int mchip_server_set_batching(mchip_server s, int enabled);
+int mchip_server_batch_stats(mchip_server s, double* batch_stats);
 
 /* ------------------------------------------------------------------------
  * SOS-AO-MP2 on the GPU: mp::mpmod::compute with LLMP_FULL_Z

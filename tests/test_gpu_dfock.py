@@ -609,3 +609,49 @@ def test_two_sided_exchange_vs_oracle(ctx, nbatch):
         # u = C, v = C^T: the symmetric exchange
         Ks = dB.dfk_lmo_uv(U, np.ascontiguousarray(U.T), metric=m, occ_nbatches=nbatch)
         assert _relerr(Ks, dB.dfk_mo(U, m)) < 1e-12
+
+
+def test_grouped_bf16_exchange_matches_alone(ctx):
+    """Several tensors' BF16x3 exchanges in one grouped slab-GEMM launch and
+    one grouped SYRK launch (the request loop's cross-request batch) give each
+    tensor's K of the build alone bit for bit, for both grouped slab-GEMM
+    shapes (more / fewer than 160 occupied columns) and both SYRK tiles (above
+    / below 512 basis functions), with the FP64 exchange as the sanity bar."""
+    from chem_amd import desc
+    from chem_amd.engine import DeviceArray, DeviceB, Metric
+    cases = [(2, 10), (3, 15), (10, 200), (22, 110), (22, 200)]   # (waters, occupied columns)
+    builds, alone, ref = [], [], []
+    for k, (nw, ncol) in enumerate(cases):
+        mol = desc.Molecule(desc.water_cluster(nw, 40 + k))
+        print("case: grouped exchange member %d: %d waters, %d columns, nbf %d" % (k, nw, ncol, mol.nbf), flush=True)
+        dB = DeviceB.build_on_gpu(ctx, mol, 1e-10)
+        m = Metric(ctx, ints.int2c2e(mol))
+        dB.make_lowp(m, f32=False, bf16=True, f64=True)
+        C, _ = _random_occ(mol.nbf, ncol, 70 + k)
+        Cd = DeviceArray(ctx, C.shape, C)
+        Ka = DeviceArray(ctx, (mol.nbf, mol.nbf))
+        dB.dfk_dev("bf16x3", Cd, ncol, Ka)
+        ctx.sync()
+        alone.append(Ka.download())
+        ref.append(dB.dfk_mo_tilde64(C))
+        Kg = DeviceArray(ctx, (mol.nbf, mol.nbf))
+        builds.append((dB, Cd, ncol, Kg))
+        Ka.release()
+    print("case: grouped launch of %d builds" % len(builds), flush=True)
+    DeviceB.dfk_bf16_group(builds)
+    ctx.sync()
+    for k, (dB, Cd, ncol, Kg) in enumerate(builds):
+        K = Kg.download()
+        assert np.array_equal(K, alone[k]), k
+        assert _relerr(K, ref[k]) < BF16_TOL, k
+    # a second grouped launch with a subset reuses the tables and buffers
+    sub = [builds[3], builds[0]]
+    for b in sub:
+        b[3].upload(np.zeros((b[0].nbf, b[0].nbf)))
+    DeviceB.dfk_bf16_group(sub)
+    ctx.sync()
+    assert np.array_equal(sub[0][3].download(), alone[3])
+    assert np.array_equal(sub[1][3].download(), alone[0])
+    for b in builds:
+        b[1].release()
+        b[3].release()

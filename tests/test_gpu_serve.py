@@ -43,6 +43,32 @@ def test_alone_vs_mix():
     assert st["max_live"] >= 2          # the mix did overlap on the device
 
 
+def test_cross_request_batching():
+    """With several requests live, their BF16x3 exchange builds run in shared
+    grouped launches (batches of two or more builds), and every energy equals
+    the one served with batching off, bit for bit."""
+    from chem_amd.serve import Server
+    mols = [_mol(n, 30 + n) for n in (2, 4, 3, 6, 5, 2)]
+    res = {}
+    for on in (False, True):
+        with Server(budget_bytes=60e9, nslots=4) as srv:
+            srv.set_batching(on)
+            print("case: serve 6 requests, batching %s" % on, flush=True)
+            ids = [srv.submit(m, threshold=1e-7, kmode="mixed_auto") for m in mols]
+            res[on] = [srv.result(i) for i in ids]
+            bs = srv.batch_stats()
+            st = srv.stats()
+        print("batching %s: %s, max live %d" % (on, bs, st["max_live"]))
+        assert st["completed"] == len(mols) and st["failed"] == 0
+        if on:
+            assert bs["builds"] > 0 and bs["largest"] >= 2
+        else:
+            assert bs["builds"] == 0
+    for a, b in zip(res[False], res[True]):
+        assert a["converged"] and b["converged"]
+        assert a["energy"] == b["energy"] and a["iterations"] == b["iterations"]
+
+
 def test_served_fp64_matches_direct_scf():
     """kmode 0 through the loop (host integrals and mask in C++) equals the
     direct device SCF driven from Python on the same molecule."""
