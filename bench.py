@@ -9,13 +9,14 @@ stays FP64 for J and the energy): a BF16 hi/lo split, an FP32 copy and an FP64
 copy.  --form auto (default) runs the library's precision escalation: BF16x3
 builds until the SCF error falls below --escalate (or stops improving), then
 FP32 builds, then FP64 ones if FP32 stalls; every build runs on MFMA.  J and
-one FP64 exchange (on the FP64 copy) at the converged orbitals give the
-reported energy with the reference's formulas.
+the FP64 exchange energy tr(K P) (on the FP64 copy, -sum_X ||W_X^T C||^2
+without forming K) at the converged orbitals give the reported energy with
+the reference's formulas.
 
 A step is one SCF iteration's DF-K work in the chosen form.  The timed region
 replays the chosen SCF's sequence of builds (same forms, same count; steps
-defaults to its iteration count) plus the final FP64 build, so value =
-(sum of the step builds + final build) / steps.  The line also carries the
+defaults to its iteration count) plus the final FP64 exchange energy, so
+value = (sum of the step builds + final energy) / steps.  The line also carries the
 plain FP64 seconds per build, |dE| between the two SCFs, the roofline of the
 dominant kernel (K1 of the BF16x3 form) from HIP events on the library stream,
 and the oracle timed on the host.
@@ -527,18 +528,27 @@ def main():
     k16 = dev_build("bf16x3") if form in ("auto", "bf16") else None
     k32 = dev_build("f32") if form in ("auto", "f32") else None
     k64 = dev_build("fp64_copy")
+
+    def kfin():   # the final FP64 exchange energy tr(K P) of the reported energy
+        return dB.exchange_energy_dev(Cd, mol.nocc)
     for _ in range(args.warmup):
-        for k in (k16, k32, k64):
+        for k in (k16, k32, k64, kfin):
             if k:
                 k()
     ctx.sync()
     t = time.perf_counter()
     for st in seq:
         (k16, k32, k64)[st]()
-    k64()   # the final FP64 exchange for the reported energy
+    ek = kfin()
     ctx.sync()
     step_s = (time.perf_counter() - t) / steps
-    final_s = time_builds(k64, 2, 0)
+    final_s = time_builds(kfin, 2, 0)
+    # the energy-only evaluation against the trace of the full FP64 exchange
+    k64()
+    ctx.sync()
+    ek_full = float(np.sum(Kd.download() * (Cc @ Cc.T)))
+    log("final exchange energy %.12f, tr(K P) of the full FP64 exchange %.12f (rel %.1e), %.4f s" % (
+        ek, ek_full, abs(ek - ek_full) / abs(ek_full), final_s))
     t16 = time_builds(k16, 3, 0) if k16 else None
     t32 = time_builds(k32, 3, 0) if k32 else None
 
@@ -640,7 +650,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"auto": "bf16x3 split then f32", "bf16": "bf16x3 split",
-                  "f32": "f32"}[form] + " (K builds, f32 accumulation) + f64 (final K, J, SCF)",
+                  "f32": "f32"}[form] + " (K builds, f32 accumulation) + f64 (final exchange energy, J, SCF)",
         "data": "synthetic seeded water cluster, real integrals",
         "config": {"workload": "DF-K build, %d-water cluster cc-pVDZ/cc-pVDZ-jkfit (%s)" % (
             cfg["nwater"], args.config), "nbf": mol.nbf, "naux": mol.naux, "nocc": mol.nocc,
@@ -648,11 +658,12 @@ def main():
         "plain_fp64_s_per_iter": plain_s,
         "speedup_vs_plain": plain_s / step_s,
         "chosen_form": form,
-        "builds_timed": {"bf16x3": nb[0], "f32": nb[1], "fp64": nb[2], "fp64_final": 1},
-        "builds_scf": {"bf16x3": scf_mix[0], "f32": scf_mix[1], "fp64": scf_mix[2], "fp64_final": 1},
+        "builds_timed": {"bf16x3": nb[0], "f32": nb[1], "fp64": nb[2], "fp64_final_energy": 1},
+        "builds_scf": {"bf16x3": scf_mix[0], "f32": scf_mix[1], "fp64": scf_mix[2], "fp64_final_energy": 1},
         "bf16x3_build_s": t16,
         "f32_build_s": t32,
-        "final_fp64_build_s": final_s,
+        "final_fp64_energy_s": final_s,
+        "final_energy_vs_full_trace_rel": abs(ek - ek_full) / abs(ek_full),
         "lowp_copies_s": t_copies,
         "dE_scf": dE,
         "dE_scf_note": "chosen SCF at %.0e vs plain FP64 SCF at %.0e" % (args.scf_thresh, args.ref_thresh),

@@ -532,6 +532,16 @@ extern "C" int mchip_dfk_mo_bf16_dev(mchip_dbt t, const double* C_dev, int ldc, 
   MCHIP_CATCH
 }
 
+extern "C" int mchip_dfk_energy_tilde64_dev(mchip_dbt t, const double* C_dev, int ldc, int nocc, double* ek)
+{
+  MCHIP_TRY
+  need_device(t->ctx);
+  check_occ("mchip_dfk_energy_tilde64_dev", ldc, nocc);
+  if (!ek) throw std::runtime_error("mchip_dfk_energy_tilde64_dev: null result pointer");
+  *ek = dfk_energy_tilde64(t->ctx->c, t->t, C_dev, ldc, nocc);
+  MCHIP_CATCH
+}
+
 extern "C" int mchip_dfk_mo_bf16_group_dev(int n, const mchip_dbt* ts, const double* const* C_dev, const int* ldc,
                                            const int* nocc, double* const* K_dev)
 {
@@ -706,9 +716,10 @@ extern "C" int mchip_scf_run(mchip_dbt t, mchip_metric m, int nbf, int nocc, int
   const Metric* met = m ? &m->m : nullptr;
   const Metric* jmet = t->t.transformed ? nullptr : met;
   KBuildFn kb, kfinal;
-  select_k_builders(c, t->t, met, kmode, nocc, occ_nbatches, threshold, kb, kfinal);
+  EnergyFn kenergy;
+  select_k_builders(c, t->t, met, kmode, nocc, occ_nbatches, threshold, kb, kfinal, 0, &kenergy);
   ScfOutput out;
-  scf_run(c, t->t, jmet, in, out, kb, kfinal);
+  scf_run(c, t->t, jmet, in, out, kb, kfinal, kenergy);
   last_scf_extra[0] = out.energy_iter;
   last_scf_extra[1] = out.t_final_k;
   *energy = out.energy;

@@ -1503,6 +1503,73 @@ static void dfk_mo_tilde64_dl(Ctx& c, DevBT& t, const double* C, int ldc, int no
   }
 }
 
+// tr(K P) of the FP64 exchange on the FP64 copy of L^-1 B at the occupied C
+// (P = C C^T), without K: W = B~ C by the copy's slab GEMM (the final
+// exchange's K1, same plan), then -sum_X ||W_X^T C||^2 over the upper 128 x
+// 128 tiles of each nocc x nocc W_X^T C (k_exchange_energy), summed
+// compensated in a fixed order.  Host result; synchronises the stream.
+double dfk_energy_tilde64(Ctx& c, DevBT& t, const double* C, int ldc, int nocc)
+{
+  const BLayout& L = t.L;
+  if (!t.data64t.p || t.h_ldp_pre.size() != (size_t)L.nbb + 1)
+    throw std::runtime_error("FP64 copy of L^-1 B missing (mchip_dbt_make_lowp with f64)");
+  if (nocc <= 0) return 0.0;
+  if (nocc > ldc) throw std::runtime_error("exchange energy: nocc exceeds the coefficients' columns");
+  const int naux = L.naux, nbf = L.nbf;
+  const int ldw = (nocc + 7) / 8 * 8;
+  const size_t nW = (size_t)naux * nbf * ldw;
+  const size_t nG = (size_t)t.h_ldp_pre[L.nbb] * nocc;
+  c.scratch_a.ensure((nW + nG) * sizeof(double) + 256);
+  double* W = c.scratch_a.as<double>();
+  double* G = W + nW;
+  int gmax = 0;
+  for (int n = 0; n < L.nbb; ++n) gmax = std::max(gmax, t.h_ldp[n]);
+  if (gmax > 0 && L.nbb > 0)
+    hipLaunchKernelGGL(k_gather_ct_f64, dim3(nblk((int64_t)nocc * gmax, 256), L.nbb), dim3(256), 0, c.stream, C,
+                       ldc, nocc, t.ncol.as<int>(), t.ldp.as<int>(), t.ldp_pre.as<int64_t>(), t.colmap.as<int>(),
+                       t.colmap_off.as<int>(), G);
+  HIP_CHECK(hipGetLastError());
+  ExchangePlan& p = f64_plan_entry(t, nocc);
+  if (p.k1 < 0) {
+    float best = 1e30f;
+    for (int v = 0; v < kNumF64Shapes; ++v) {
+      const int bn = kF64Shapes[v].bn();
+      if (bn > 64 && (nocc + bn - 1) / bn * bn - nocc >= bn / 2) continue;   // mostly empty tiles
+      const float ms = time_launch(c, [&] { slab_gemm_f64dl(c, t, v, G, nocc, W, ldw); });
+      p.k1_ms[v] = ms;
+      if (ms < best) {
+        best = ms;
+        p.k1 = v;
+      }
+    }
+  }
+  slab_gemm_f64dl(c, t, p.k1, G, nocc, W, ldw);
+  // upper tiles of the nocc x nocc products (cached in t)
+  const int nt = (nocc + 127) / 128;
+  const int ntile = nt * (nt + 1) / 2;
+  const auto key = std::make_pair(-3000000, nocc);
+  if (!t.tiles.count(key)) {
+    std::vector<int2> tl;
+    for (int i = 0; i < nt; ++i)
+      for (int j = i; j < nt; ++j) tl.push_back(int2{i, j});
+    t.tiles[key] = upload_vec(tl, c.stream);
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+  }
+  const int64_t nitem = (int64_t)naux * ntile;
+  if (nitem >= ((int64_t)1 << 31)) throw std::runtime_error("exchange energy: too many work items");
+  c.scratch_b.ensure((size_t)(2 * nitem + 8) * sizeof(double));
+  double* part = c.scratch_b.as<double>();
+  double* out = part + 2 * nitem;
+  hipLaunchKernelGGL(k_exchange_energy, dim3((unsigned)((nitem + 7) / 8 * 8)), dim3(256), 0, c.stream, W, ldw, nbf,
+                     nocc, C, ldc, naux, t.tiles[key].as<int2>(), ntile, part);
+  HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c.stream, part, nitem, -1.0, out);
+  HIP_CHECK(hipGetLastError());
+  double e = 0.0;
+  d2h(&e, out, sizeof(double), c.stream);
+  return e;
+}
+
 // Full 128 x 128 tile table of an n x n product (cached in t).
 static const int2* full_tiles(Ctx& c, DevBT& t, int n, int* ntile)
 {

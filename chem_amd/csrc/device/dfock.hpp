@@ -40,6 +40,10 @@ void dbt_make_lowp(Ctx& c, DevBT& t, const Metric& m, bool f32, bool bf16, bool 
 // into out (L.total floats, slab layout); t is left unchanged.
 void tilde_to_f32(Ctx& c, DevBT& t, const Metric& m, float* out);
 void dfk_mo_bf16(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K, int occ_nbatches);
+// tr(K P) of the FP64 exchange on the FP64 copy of L^-1 B at the occupied C
+// (device, row-major nbf x ldc, first nocc columns; P = C C^T), computed as
+// -sum_X ||W_X^T C||^2 without forming K (host result, synchronises).
+double dfk_energy_tilde64(Ctx& c, DevBT& t, const double* C, int ldc, int nocc);
 // Grouped BF16x3 exchange over several tensors (one occupied batch each): the
 // coefficients of one build are gathered on its owner's stream c.stream
 // (prepare), then every build runs in one slab-GEMM and one SYRK launch on

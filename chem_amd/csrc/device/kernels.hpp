@@ -76,6 +76,9 @@ __global__ void k_slab_gemm_bf16x3_dla(const uint16_t* bhi, const uint16_t* blo,
                                        const int* boff, const Tile* tiles, int naux, int nbf,
                                        const uint16_t* ghi, const uint16_t* glo, int ng, uint16_t* whi,
                                        uint16_t* wlo, int ldo);
+__global__ void k_exchange_energy(const double* W, int ldw, int nbf, int nocc, const double* C, int ldc, int naux,
+                                  const int2* tiles, int ntile, double* partial);
+__global__ void k_sum_partials(const double* partial, int64_t n, double scale, double* out);
 template <int WM, int WN, int TM, int TN, int SA>
 __global__ void k_slab_gemm_bf16x3_dla_group(const ExGroupReq* reqs, const Tile* tiles);
 template <int S, int BM, int NW>

@@ -182,7 +182,7 @@ struct DiisState {
 };
 
 void scf_run(Ctx& c, DevBT& bt, const Metric* m, const ScfInput& in, ScfOutput& out,
-             const KBuildFn& kbuild, const KBuildFn& final_k)
+             const KBuildFn& kbuild, const KBuildFn& final_k, const EnergyFn& final_ek)
 {
   const int n = in.nbf, nocc = in.nocc;
   const int64_t nn = (int64_t)n * n;
@@ -368,7 +368,20 @@ void scf_run(Ctx& c, DevBT& bt, const Metric* m, const ScfInput& in, ScfOutput& 
   }
   HIP_CHECK(hipStreamSynchronize(c.stream));
   out.energy_iter = energy + in.enuc;
-  if (final_k) {
+  if (final_ek) {
+    // tr(K P) of the FP64 exchange at the final orbitals; J is already FP64 at
+    // this density: E = tr(H P) + tr((H + J + K) P)
+    HIP_CHECK(hipEventRecord(c.ev0, c.stream));
+    const double ek = final_ek(Cocc.as<double>());
+    HIP_CHECK(hipEventRecord(c.ev1, c.stream));
+    HIP_CHECK(hipEventSynchronize(c.ev1));
+    float tk = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&tk, c.ev0, c.ev1));
+    out.t_final_k = tk * 1e-3;
+    energy = 2.0 * dot_dev(c, H.as<double>(), P.as<double>(), nn, part) +
+             dot_dev(c, J.as<double>(), P.as<double>(), nn, part) + ek;
+  }
+  else if (final_k) {
     // FP64 exchange at the final orbitals; J is already FP64 at this density
     HIP_CHECK(hipEventRecord(c.ev0, c.stream));
     final_k(iter, prev_rms, P.as<double>(), Cocc.as<double>(), K.as<double>());
@@ -400,8 +413,10 @@ void scf_run(Ctx& c, DevBT& bt, const Metric* m, const ScfInput& in, ScfOutput& 
 }
 
 void select_k_builders(Ctx& c, DevBT& bt, const Metric* m, int kmode, int nocc, int occ_nbatches,
-                       double threshold, KBuildFn& kb, KBuildFn& kfinal, int final_occ_nbatches)
+                       double threshold, KBuildFn& kb, KBuildFn& kfinal, int final_occ_nbatches,
+                       EnergyFn* kenergy)
 {
+  if (kenergy) *kenergy = EnergyFn();
   if (final_occ_nbatches <= 0) final_occ_nbatches = occ_nbatches;
   if (kmode == 0 && !m) throw std::runtime_error("SCF: plain K needs the metric");
   if (kmode >= 2 && !bt.transformed && !m)
@@ -422,6 +437,8 @@ void select_k_builders(Ctx& c, DevBT& bt, const Metric* m, int kmode, int nocc, 
       dfk_mo(*cp, *t, km, Cocc, nocc, nocc, K, final_occ_nbatches, t64);
       return false;
     };
+    if (kenergy && t64 && final_occ_nbatches <= 1 && nocc > 0)
+      *kenergy = [cp, t, nocc](const double* Cocc) { return dfk_energy_tilde64(*cp, *t, Cocc, nocc, nocc); };
   }
   if (kmode == 0 || kmode == 1) {
     kb = k64;

@@ -50,12 +50,22 @@ using KBuildFn = std::function<bool(int, double, const double*, const double*, d
 // FP32 to FP64 after such a stall below 1e-6 (straight from BF16x3 to FP64 when
 // the tensor has no FP32 copy); modes 2-4 get an FP64 final builder, with final_occ_nbatches occupied batches,
 // 0 = occ_nbatches).  The builders capture c, bt and m by reference.
+// tr(K P) of the FP64 exchange at occupied coefficients Cocc (device,
+// row-major nbf x nocc), without K.
+using EnergyFn = std::function<double(const double*)>;
+
+// kenergy (optional): when the final FP64 exchange runs on the FP64 copy of
+// L^-1 B in one occupied batch, the function that gives its tr(K P) directly
+// (the reported energy needs K only through that trace); empty otherwise.
 void select_k_builders(Ctx& c, DevBT& bt, const Metric* m, int kmode, int nocc, int occ_nbatches,
-                       double threshold, KBuildFn& kb, KBuildFn& kfinal, int final_occ_nbatches = 0);
+                       double threshold, KBuildFn& kb, KBuildFn& kfinal, int final_occ_nbatches = 0,
+                       EnergyFn* kenergy = nullptr);
 
 // final_k (optional): FP64 K builder used once at the converged orbitals to
-// evaluate the reported energy when kbuild works in reduced precision.
+// evaluate the reported energy when kbuild works in reduced precision;
+// final_ek (optional, used instead of final_k): tr(K P) of that exchange.
+// With final_ek the returned F holds the last iteration's exchange.
 void scf_run(Ctx& c, DevBT& bt, const Metric* m, const ScfInput& in, ScfOutput& out,
-             const KBuildFn& kbuild, const KBuildFn& final_k = KBuildFn());
+             const KBuildFn& kbuild, const KBuildFn& final_k = KBuildFn(), const EnergyFn& final_ek = EnergyFn());
 
 }  // namespace mchip

@@ -478,7 +478,8 @@ void Server::run(Ctx& c, SrvJob& j, bool& admitted)
     in.max_iter = j.max_iter;
     in.filter_eps = j.eps;
     KBuildFn kb, kfinal;
-    select_k_builders(c, t, &m, j.kmode, nocc, 1, j.threshold, kb, kfinal, j.final_occ_nbatches);
+    EnergyFn kenergy;
+    select_k_builders(c, t, &m, j.kmode, nocc, 1, j.threshold, kb, kfinal, j.final_occ_nbatches, &kenergy);
     ScfOutput out;
     // the BF16x3 builds of this request join the cross-request batches
     if (batching_)
@@ -486,7 +487,7 @@ void Server::run(Ctx& c, SrvJob& j, bool& admitted)
         batcher_->exchange(cc, tt, C, ldc, no, K);
       };
     try {
-      scf_run(c, t, t.transformed ? nullptr : &m, in, out, kb, kfinal);
+      scf_run(c, t, t.transformed ? nullptr : &m, in, out, kb, kfinal, kenergy);
     }
     catch (...) {
       c.bf16_exchange = nullptr;

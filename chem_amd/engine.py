@@ -266,6 +266,19 @@ class DeviceB:
             rc = fn(self.h, C_dev.ptr, ldc, nocc, occ_nbatches, K_dev.ptr)
         _lib.check(rc)
 
+    def exchange_energy_dev(self, C_dev, nocc):
+        """tr(K P) of the FP64 exchange on the FP64 copy of L^-1 B at the
+        occupied columns of the DeviceArray C_dev (P = C C^T), without forming
+        K (synchronises)."""
+        if getattr(C_dev, "ptr", None) is None or not C_dev.ptr.value:
+            raise ValueError("exchange_energy_dev: C_dev has been released")
+        if len(C_dev.shape) != 2 or C_dev.shape[0] != self.nbf or not 0 <= nocc <= C_dev.shape[1]:
+            raise ValueError("exchange_energy_dev: C_dev shape %s / nocc %d" % (C_dev.shape, nocc))
+        e = ctypes.c_double()
+        _lib.check(_lib.lib().mchip_dfk_energy_tilde64_dev(self.h, C_dev.ptr, C_dev.shape[1], nocc,
+                                                            ctypes.byref(e)))
+        return e.value
+
     @staticmethod
     def dfk_bf16_group(builds):
         """BF16x3 exchanges of several tensors in one grouped launch:

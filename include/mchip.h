@@ -260,6 +260,12 @@ int mchip_dbt_make_bf16(mchip_dbt t);
  * synchronisation (the context's stream). */
 int mchip_dfk_mo_tilde64_dev(mchip_dbt t, const double* C_dev, int ldc, int nocc, int occ_nbatches,
                              double* K_dev);
+/* tr(K P) of the FP64 exchange on the FP64 copy of L^-1 B at the device
+ * coefficients C_dev (row-major nbf x ldc, first nocc columns; P = C C^T),
+ * computed as -sum_X ||(L^-1 B C)_X^T C||^2 without forming K: the exchange
+ * part of E = tr(H P) + tr(F P) (hfscf.cpp:278-296), which is all the
+ * reported energy needs of the final exchange.  Synchronises; *ek on the host. */
+int mchip_dfk_energy_tilde64_dev(mchip_dbt t, const double* C_dev, int ldc, int nocc, double* ek);
 int mchip_dfk_mo_bf16_dev(mchip_dbt t, const double* C_dev, int ldc, int nocc, int occ_nbatches,
                           double* K_dev);
 /* The BF16x3 exchange of n tensors on one context in one slab-GEMM launch
