@@ -180,6 +180,8 @@ _SIGS = [
     ("mchip_server_stats", c_int, [c_void_p, P_double]),
     ("mchip_scf_last_extra", c_int, [P_double, P_double]),
     ("mchip_ctx_phase_times", c_int, [c_void_p, P_double, c_int]),
+    ("mchip_ctx_set_graphs", c_int, [c_void_p, c_int]),
+    ("mchip_dbt_graph_stats", c_int, [c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
 ]
 
 

@@ -189,6 +189,22 @@ extern "C" int mchip_ctx_set_timing(mchip_ctx c, int on)
   MCHIP_CATCH
 }
 
+extern "C" int mchip_ctx_set_graphs(mchip_ctx c, int on)
+{
+  MCHIP_TRY
+  c->c.graphs = on != 0;
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_dbt_graph_stats(mchip_dbt t, int64_t* ngraphs, int64_t* replays)
+{
+  MCHIP_TRY
+  if (!t) throw std::runtime_error("mchip_dbt_graph_stats: null tensor");
+  if (ngraphs) *ngraphs = (int64_t)t->t.graphs.exec.size();
+  if (replays) *replays = t->t.graphs.replays;
+  MCHIP_CATCH
+}
+
 extern "C" int mchip_ctx_phase_times(mchip_ctx c, double* out, int n)
 {
   MCHIP_TRY

@@ -6,9 +6,11 @@
 
 #include <cstdint>
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <functional>
 #include <map>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -197,6 +199,10 @@ struct Ctx {
   // function (the request loop's cross-request batcher) instead of being
   // launched on this context's stream.
   std::function<void(Ctx&, DevBT&, const double*, int, int, double*)> bf16_exchange;
+  // Replay the launch chains of repeated exchange and DF-J builds from
+  // captured hipGraphs (the chain of one build is captured on its second
+  // call with the same operands and replayed from the third on).
+  bool graphs = true;
   // optional per-phase kernel timing (HIP events on this stream), seconds
   bool timing = false;
   hipEvent_t tev[2] = {nullptr, nullptr};
@@ -262,6 +268,40 @@ struct ExchangePlan {
   std::vector<float> k1_ms, k3_ms;
 };
 
+// Captured launch chains of one tensor's builds: hipGraph executables keyed
+// by everything the chain's launch arguments derive from (operation, operand
+// and work-array pointers, widths, tuned shapes), and the keys run once
+// uncaptured (their first run builds the tile tables and tunes the shapes).
+using ChainKey = std::array<int64_t, 14>;
+struct GraphCache {
+  std::map<ChainKey, hipGraphExec_t> exec;
+  std::set<ChainKey> seen;
+  int64_t replays = 0;
+  GraphCache() = default;
+  GraphCache(const GraphCache&) = delete;
+  GraphCache& operator=(const GraphCache&) = delete;
+  GraphCache(GraphCache&& o) noexcept : exec(std::move(o.exec)), seen(std::move(o.seen)), replays(o.replays)
+  {
+    o.exec.clear();
+  }
+  GraphCache& operator=(GraphCache&& o) noexcept
+  {
+    clear();
+    exec = std::move(o.exec);
+    seen = std::move(o.seen);
+    replays = o.replays;
+    o.exec.clear();
+    return *this;
+  }
+  ~GraphCache() { clear(); }
+  void clear()
+  {
+    for (auto& kv : exec) (void)hipGraphExecDestroy(kv.second);
+    exec.clear();
+    seen.clear();
+  }
+};
+
 struct DevBT {
   BLayout L;                 // host copy of the layout
   DevBuf data64;             // FP64 slabs (B or B~)
@@ -285,6 +325,7 @@ struct DevBT {
   DevBuf gx_w, gx_g, gx_part;   // work arrays of this tensor's builds in a grouped BF16x3 exchange
   std::map<std::pair<int, int>, DevBuf> tiles;   // (BN, ng) -> tile table
   std::map<std::pair<int, int>, int> ntiles;
+  GraphCache graphs;                             // captured build chains on this tensor
 };
 
 struct Metric {

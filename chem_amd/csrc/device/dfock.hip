@@ -21,6 +21,67 @@
 
 namespace mchip {
 
+// Runs the launch chain `launch` of one build on c.stream.  The first run of a
+// key is direct (it may tune shapes and upload tile tables, which synchronise
+// the stream); the second is captured into a hipGraph, instantiated and
+// launched; later runs replay that graph.  Every kernel argument of the chain
+// derives from the key, and the graph reads the operands' current contents,
+// so a replay computes what a direct run would, bit for bit.  Per-phase timing
+// and a context with graphs off run every chain directly.
+template <class F>
+static void run_chain(Ctx& c, DevBT& t, const ChainKey& key, F&& launch)
+{
+  GraphCache& g = t.graphs;
+  if (!c.graphs || c.timing) {
+    launch();
+    return;
+  }
+  auto it = g.exec.find(key);
+  if (it != g.exec.end()) {
+    HIP_CHECK(hipGraphLaunch(it->second, c.stream));
+    ++g.replays;
+    return;
+  }
+  if (g.seen.size() >= 1024) g.seen.clear();   // bounded: callers with ever new buffers
+  if (g.seen.insert(key).second) {
+    launch();
+    return;
+  }
+  if (g.exec.size() >= 32) {   // bounded: drop every graph once no launch of them is in flight
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    for (auto& kv : g.exec) (void)hipGraphExecDestroy(kv.second);
+    g.exec.clear();
+  }
+  hipGraph_t graph = nullptr;
+  HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+  try {
+    launch();
+  }
+  catch (...) {
+    (void)hipStreamEndCapture(c.stream, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  HIP_CHECK(hipStreamEndCapture(c.stream, &graph));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HIP_CHECK(e);
+  g.exec[key] = exec;
+  HIP_CHECK(hipGraphLaunch(exec, c.stream));
+  ++g.replays;
+}
+
+static int64_t pkey(const void* p) { return (int64_t)reinterpret_cast<uintptr_t>(p); }
+
+// The tuned shapes of a plan for width ng (-1 while untuned).
+static int64_t plan_key(const std::map<int, ExchangePlan>& plans, int ng)
+{
+  auto it = plans.find(ng);
+  if (it == plans.end() || it->second.k1 < 0 || it->second.k3 < 0) return -1;
+  return (int64_t)it->second.k1 * 1000 + it->second.k3;
+}
+
 __global__ void k_zero_upper_cm(double* A, int n)
 {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -264,7 +325,12 @@ size_t dfj_workspace_bytes(const DevBT& t)
 void dfj(Ctx& c, DevBT& t, const Metric* m, const double* P, double* J, double pscale)
 {
   c.scratch_c.ensure(dfj_workspace_bytes(t) + 256);
-  dfj_ws(c, t, m, P, J, pscale, c.scratch_c.as<double>());
+  double* ws = c.scratch_c.as<double>();
+  int64_t ps = 0;
+  std::memcpy(&ps, &pscale, sizeof(ps));
+  const ChainKey key{4, pkey(P), pkey(J), ps, pkey(m ? m->Vinv.p : nullptr), pkey(ws), pkey(t.data64.p),
+                     t.transformed ? 1 : 0, 0, 0, 0, 0, 0, 0};
+  run_chain(c, t, key, [&] { dfj_ws(c, t, m, P, J, pscale, ws); });
 }
 
 void dfj_ws(Ctx& c, DevBT& t, const Metric* m, const double* P, double* J, double pscale, double* ws)
@@ -1096,33 +1162,40 @@ void dfk_mo_bf16(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K
   c.scratch_partial.ensure((size_t)nb * nch * nbf * nbf * sizeof(float));
   int gmax = 0;
   for (int n = 0; n < L.nbb; ++n) gmax = std::max(gmax, t.h_ldp[n]);
-  for (int b = 0; b < nb; ++b) {
-    const int i0 = b * per;
-    const int w = std::min(per, nocc - i0);
-    {
-      PhaseTimer tm(c, PH_SPLIT);
-      if (gmax > 0 && L.nbb > 0)
-        hipLaunchKernelGGL(k_gather_ct, dim3(nblk((int64_t)w * gmax, 256), L.nbb), dim3(256), 0, c.stream,
-                           C + i0, ldc, w, t.ncol.as<int>(), t.ldp.as<int>(), t.ldp_pre.as<int64_t>(),
-                           t.colmap.as<int>(), t.colmap_off.as<int>(), ghi, glo);
+  auto chain = [&] {
+    for (int b = 0; b < nb; ++b) {
+      const int i0 = b * per;
+      const int w = std::min(per, nocc - i0);
+      {
+        PhaseTimer tm(c, PH_SPLIT);
+        if (gmax > 0 && L.nbb > 0)
+          hipLaunchKernelGGL(k_gather_ct, dim3(nblk((int64_t)w * gmax, 256), L.nbb), dim3(256), 0, c.stream,
+                             C + i0, ldc, w, t.ncol.as<int>(), t.ldp.as<int>(), t.ldp_pre.as<int64_t>(),
+                             t.colmap.as<int>(), t.colmap_off.as<int>(), ghi, glo);
+        HIP_CHECK(hipGetLastError());
+      }
+      float* partial = c.scratch_partial.as<float>() + (size_t)b * nch * nbf * nbf;
+      const ExchangePlan& plan = tune_bf16(c, t, ghi, glo, w, W, Whi, Wlo, ldw, d_tl, ntile, nch, spc, partial);
+      {
+        PhaseTimer tm(c, PH_K1);
+        // a narrower last batch: its tiles stop at the first column-tile edge
+        // past w, so the columns up to ldw still hold the previous batch
+        if (w < per) zero_w(c, W, nW * sizeof(float));
+        slab_gemm_bf16(c, t, plan.k1, ghi, glo, w, W, Whi, Wlo, ldw);
+      }
+      PhaseTimer tm(c, PH_K3);
+      seg_syrk_bf16(c, t, plan.k3, W, Whi, Wlo, naux, nbf, ldw, d_tl, ntile, nch, spc, partial);
+      if (b == nb - 1)
+        hipLaunchKernelGGL(k_reduce_sym<float>, dim3(nblk((int64_t)nbf * nbf, 256)), dim3(256), 0, c.stream,
+                           c.scratch_partial.as<float>(), nb * nch, nbf, 0, -1.0, K);
       HIP_CHECK(hipGetLastError());
     }
-    float* partial = c.scratch_partial.as<float>() + (size_t)b * nch * nbf * nbf;
-    const ExchangePlan& plan = tune_bf16(c, t, ghi, glo, w, W, Whi, Wlo, ldw, d_tl, ntile, nch, spc, partial);
-    {
-      PhaseTimer tm(c, PH_K1);
-      // a narrower last batch: its tiles stop at the first column-tile edge
-      // past w, so the columns up to ldw still hold the previous batch
-      if (w < per) zero_w(c, W, nW * sizeof(float));
-      slab_gemm_bf16(c, t, plan.k1, ghi, glo, w, W, Whi, Wlo, ldw);
-    }
-    PhaseTimer tm(c, PH_K3);
-    seg_syrk_bf16(c, t, plan.k3, W, Whi, Wlo, naux, nbf, ldw, d_tl, ntile, nch, spc, partial);
-    if (b == nb - 1)
-      hipLaunchKernelGGL(k_reduce_sym<float>, dim3(nblk((int64_t)nbf * nbf, 256)), dim3(256), 0, c.stream,
-                         c.scratch_partial.as<float>(), nb * nch, nbf, 0, -1.0, K);
-    HIP_CHECK(hipGetLastError());
-  }
+  };
+  const int wl = nocc - (nb - 1) * per;   // width of the last batch
+  const ChainKey key{2, pkey(C), ldc, nocc, pkey(K), occ_nbatches, pkey(c.scratch_a.p), pkey(c.scratch_partial.p),
+                     pkey(t.data_hi.p), pkey(t.data_lo.p), plan_key(t.bf16_plan, per), plan_key(t.bf16_plan, wl),
+                     0, 0};
+  run_chain(c, t, key, chain);
 }
 
 // ---------------------------------------------------------------------------
@@ -1316,31 +1389,37 @@ void dfk_mo_f32(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K,
   c.scratch_partial.ensure((size_t)nb * nch * nbf * nbf * sizeof(double));
   int gmax = 0;
   for (int n = 0; n < L.nbb; ++n) gmax = std::max(gmax, t.h_ldp[n]);
-  for (int b = 0; b < nb; ++b) {
-    const int i0 = b * per;
-    const int w = std::min(per, nocc - i0);
-    {
-      PhaseTimer tm(c, PH_SPLIT);
-      if (gmax > 0 && L.nbb > 0)
-        hipLaunchKernelGGL(k_gather_ct_f32, dim3(nblk((int64_t)w * gmax, 256), L.nbb), dim3(256), 0, c.stream,
-                           C + i0, ldc, w, t.ncol.as<int>(), t.ldp.as<int>(), t.ldp_pre.as<int64_t>(),
-                           t.colmap.as<int>(), t.colmap_off.as<int>(), G);
+  auto chain = [&] {
+    for (int b = 0; b < nb; ++b) {
+      const int i0 = b * per;
+      const int w = std::min(per, nocc - i0);
+      {
+        PhaseTimer tm(c, PH_SPLIT);
+        if (gmax > 0 && L.nbb > 0)
+          hipLaunchKernelGGL(k_gather_ct_f32, dim3(nblk((int64_t)w * gmax, 256), L.nbb), dim3(256), 0, c.stream,
+                             C + i0, ldc, w, t.ncol.as<int>(), t.ldp.as<int>(), t.ldp_pre.as<int64_t>(),
+                             t.colmap.as<int>(), t.colmap_off.as<int>(), G);
+        HIP_CHECK(hipGetLastError());
+      }
+      double* partial = c.scratch_partial.as<double>() + (size_t)b * nch * nbf * nbf;
+      const ExchangePlan& plan = tune_f32(c, t, G, w, W, ldw, tl, ntile, nch, spc, partial);
+      {
+        PhaseTimer tm(c, PH_K1);
+        if (w < per) zero_w(c, W, nW * sizeof(float));   // see dfk_mo_bf16
+        slab_gemm_f32dl(c, t, plan.k1, G, w, W, ldw);
+      }
+      PhaseTimer tm(c, PH_K3);
+      seg_syrk_f32dl(c, plan.k3, W, naux, nbf, ldw, tl, ntile, nch, spc, partial);
+      if (b == nb - 1)
+        hipLaunchKernelGGL(k_reduce_sym<double>, dim3(nblk((int64_t)nbf * nbf, 256)), dim3(256), 0, c.stream,
+                           c.scratch_partial.as<double>(), nb * nch, nbf, 0, -1.0, K);
       HIP_CHECK(hipGetLastError());
     }
-    double* partial = c.scratch_partial.as<double>() + (size_t)b * nch * nbf * nbf;
-    const ExchangePlan& plan = tune_f32(c, t, G, w, W, ldw, tl, ntile, nch, spc, partial);
-    {
-      PhaseTimer tm(c, PH_K1);
-      if (w < per) zero_w(c, W, nW * sizeof(float));   // see dfk_mo_bf16
-      slab_gemm_f32dl(c, t, plan.k1, G, w, W, ldw);
-    }
-    PhaseTimer tm(c, PH_K3);
-    seg_syrk_f32dl(c, plan.k3, W, naux, nbf, ldw, tl, ntile, nch, spc, partial);
-    if (b == nb - 1)
-      hipLaunchKernelGGL(k_reduce_sym<double>, dim3(nblk((int64_t)nbf * nbf, 256)), dim3(256), 0, c.stream,
-                         c.scratch_partial.as<double>(), nb * nch, nbf, 0, -1.0, K);
-    HIP_CHECK(hipGetLastError());
-  }
+  };
+  const int wl = nocc - (nb - 1) * per;
+  const ChainKey key{3, pkey(C), ldc, nocc, pkey(K), occ_nbatches, pkey(c.scratch_a.p), pkey(c.scratch_partial.p),
+                     pkey(t.data32.p), 0, plan_key(t.f32_plan, per), plan_key(t.f32_plan, wl), 0, 0};
+  run_chain(c, t, key, chain);
 }
 
 // ---------------------------------------------------------------------------
@@ -1413,14 +1492,31 @@ static ExchangePlan& f64_plan_entry(DevBT& t, int ng)
 // K3 of an FP64 exchange batch: partial sums of -W_X W_X^T over aux chunks on
 // the upper 128 x 128 tiles (FP64 MFMA), reduced into K after the last batch.
 // The SYRK variant is timed once per ng on the live operand (autotuner).
+// Aux chunks of the FP64 exchange SYRK: nch chunks of spc aux functions.
+static void f64_syrk_chunks(int naux, int ntile, int* nch_out, int* spc_out)
+{
+  int nch = std::max(1, std::min(naux, 2048 / std::max(1, ntile)));
+  const int spc = (naux + nch - 1) / nch;
+  *nch_out = (naux + spc - 1) / spc;
+  *spc_out = spc;
+}
+
+// Bytes of the FP64 SYRK's chunk partials for nb occupied batches (128-row tiles).
+static size_t f64_partial_bytes(const DevBT& t, int nb)
+{
+  const int nt = (t.L.nbf + 127) / 128;
+  int nch = 1, spc = 1;
+  f64_syrk_chunks(t.L.naux, nt * (nt + 1) / 2, &nch, &spc);
+  return (size_t)nb * nch * t.L.nbf * t.L.nbf * sizeof(double);
+}
+
 static void syrk_f64_exchange(Ctx& c, DevBT& t, const double* W, int ng, int ldw, int nb, int b, double* K)
 {
   const int naux = t.L.naux, nbf = t.L.nbf;
   int ntile = 0;
   const int2* tl = syrk_tiles(c, t, nbf, 128, &ntile);
-  int nch = std::max(1, std::min(naux, 2048 / std::max(1, ntile)));
-  const int spc = (naux + nch - 1) / nch;
-  nch = (naux + spc - 1) / spc;
+  int nch = 1, spc = 1;
+  f64_syrk_chunks(naux, ntile, &nch, &spc);
   if (b == 0) c.scratch_partial.ensure((size_t)nb * nch * nbf * nbf * sizeof(double));
   double* partial = c.scratch_partial.as<double>() + (size_t)b * nch * nbf * nbf;
   ExchangePlan& p = f64_plan_entry(t, ng);
@@ -1468,39 +1564,46 @@ static void dfk_mo_tilde64_dl(Ctx& c, DevBT& t, const double* C, int ldc, int no
   double* G = W + nW;   // 64-byte aligned: nW is a multiple of 8
   int gmax = 0;
   for (int n = 0; n < L.nbb; ++n) gmax = std::max(gmax, t.h_ldp[n]);
-  for (int b = 0; b < nb; ++b) {
-    const int i0 = b * per;
-    const int w = std::min(per, nocc - i0);
-    {
-      PhaseTimer tm(c, PH_SPLIT);
-      if (gmax > 0 && L.nbb > 0)
-        hipLaunchKernelGGL(k_gather_ct_f64, dim3(nblk((int64_t)w * gmax, 256), L.nbb), dim3(256), 0, c.stream,
-                           C + i0, ldc, w, t.ncol.as<int>(), t.ldp.as<int>(), t.ldp_pre.as<int64_t>(),
-                           t.colmap.as<int>(), t.colmap_off.as<int>(), G);
-      HIP_CHECK(hipGetLastError());
-    }
-    ExchangePlan& p = f64_plan_entry(t, w);
-    if (p.k1 < 0) {
-      float best = 1e30f;
-      for (int v = 0; v < kNumF64Shapes; ++v) {
-        const int bn = kF64Shapes[v].bn();
-        if (bn > 64 && (w + bn - 1) / bn * bn - w >= bn / 2) continue;   // mostly empty tiles
-        const float ms = time_launch(c, [&] { slab_gemm_f64dl(c, t, v, G, w, W, ldw); });
-        p.k1_ms[v] = ms;
-        if (ms < best) {
-          best = ms;
-          p.k1 = v;
+  c.scratch_partial.ensure(f64_partial_bytes(t, nb));   // the SYRK's chunk partials (before the key)
+  auto chain = [&] {
+    for (int b = 0; b < nb; ++b) {
+      const int i0 = b * per;
+      const int w = std::min(per, nocc - i0);
+      {
+        PhaseTimer tm(c, PH_SPLIT);
+        if (gmax > 0 && L.nbb > 0)
+          hipLaunchKernelGGL(k_gather_ct_f64, dim3(nblk((int64_t)w * gmax, 256), L.nbb), dim3(256), 0, c.stream,
+                             C + i0, ldc, w, t.ncol.as<int>(), t.ldp.as<int>(), t.ldp_pre.as<int64_t>(),
+                             t.colmap.as<int>(), t.colmap_off.as<int>(), G);
+        HIP_CHECK(hipGetLastError());
+      }
+      ExchangePlan& p = f64_plan_entry(t, w);
+      if (p.k1 < 0) {
+        float best = 1e30f;
+        for (int v = 0; v < kNumF64Shapes; ++v) {
+          const int bn = kF64Shapes[v].bn();
+          if (bn > 64 && (w + bn - 1) / bn * bn - w >= bn / 2) continue;   // mostly empty tiles
+          const float ms = time_launch(c, [&] { slab_gemm_f64dl(c, t, v, G, w, W, ldw); });
+          p.k1_ms[v] = ms;
+          if (ms < best) {
+            best = ms;
+            p.k1 = v;
+          }
         }
       }
+      {
+        PhaseTimer tm(c, PH_K1);
+        if (w < per) zero_w(c, W, nW * sizeof(double));   // see dfk_mo_bf16
+        slab_gemm_f64dl(c, t, p.k1, G, w, W, ldw);
+      }
+      PhaseTimer tm(c, PH_K3);
+      syrk_f64_exchange(c, t, W, w, ldw, nb, b, K);
     }
-    {
-      PhaseTimer tm(c, PH_K1);
-      if (w < per) zero_w(c, W, nW * sizeof(double));   // see dfk_mo_bf16
-      slab_gemm_f64dl(c, t, p.k1, G, w, W, ldw);
-    }
-    PhaseTimer tm(c, PH_K3);
-    syrk_f64_exchange(c, t, W, w, ldw, nb, b, K);
-  }
+  };
+  const int wl = nocc - (nb - 1) * per;
+  const ChainKey key{5, pkey(C), ldc, nocc, pkey(K), occ_nbatches, pkey(c.scratch_a.p), pkey(c.scratch_partial.p),
+                     pkey(t.data64t.p), 0, plan_key(t.f64_plan, per), plan_key(t.f64_plan, wl), 0, 0};
+  run_chain(c, t, key, chain);
 }
 
 // tr(K P) of the FP64 exchange on the FP64 copy of L^-1 B at the occupied C

@@ -39,6 +39,11 @@ class Context:
     def set_timing(self, on):
         _lib.check(_lib.lib().mchip_ctx_set_timing(self.h, int(bool(on))))
 
+    def set_graphs(self, on):
+        """Replay repeated exchange and DF-J builds from captured hipGraphs
+        (default on); off: direct launches."""
+        _lib.check(_lib.lib().mchip_ctx_set_graphs(self.h, int(bool(on))))
+
     def set_escalation(self, rms=1e-5, stall=3):
         """When the escalation K mode switches BF16x3 -> FP32: at an error RMS
         <= rms (0: off) or after `stall` iterations without progress (0: off)."""
@@ -187,6 +192,12 @@ class DeviceB:
         m = np.zeros((self.nxb, self.nbb, self.nbb), dtype=np.uint8)
         _lib.check(_lib.lib().mchip_dbt_mask(self.h, _lib.u8ptr(m)))
         return m
+
+    def graph_stats(self):
+        """(captured graphs held by this tensor, builds that replayed one)."""
+        n, r = ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(_lib.lib().mchip_dbt_graph_stats(self.h, ctypes.byref(n), ctypes.byref(r)))
+        return n.value, r.value
 
     def nnz_kept(self):
         """Elements inside kept blocks (the algorithmic nnz of B)."""

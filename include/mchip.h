@@ -149,6 +149,19 @@ int mchip_ctx_stream(mchip_ctx c, void** stream);
  * accumulators.  Slots: 0 K1 (B.C), 1 K2 (metric), 2 K3 (SYRK),
  * 3 J total, 4 incremental K, 5 operand splitting. */
 int mchip_ctx_set_timing(mchip_ctx c, int on);
+
+/* Graph replay of repeated builds (default on): the launch chain of an
+ * exchange build (gather -> slab GEMM -> SYRK -> chunk reduction, over every
+ * occupied batch) of the BF16x3, FP32 and FP64-copy forms, and of a DF-J
+ * build, is captured into a hipGraph on its second call with the same
+ * operand pointers, widths and tuned shapes, and replayed from the third on
+ * (one graph launch per build).  A replay gives the same bits as the direct
+ * launches.  The reference builds through DBCSR calls (jkdf.cpp:334-422) and
+ * has no counterpart; this is the context's switch (0 = direct launches),
+ * e.g. for timing both. */
+int mchip_ctx_set_graphs(mchip_ctx c, int on);
+/* Captured graphs held by tensor t and how many builds replayed one. */
+int mchip_dbt_graph_stats(mchip_dbt t, int64_t* ngraphs, int64_t* replays);
 /* When the escalation K mode (4) switches from the BF16x3 to the FP32
  * exchange: once the previous iteration's error RMS is <= rms (0 turns this
  * trigger off; default 1e-5), or after `stall` consecutive iterations below an

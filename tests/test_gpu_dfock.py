@@ -683,3 +683,50 @@ def test_exchange_energy_without_k(ctx, nw, ncol):
         ref = float(np.sum(dfock.dfmo_k(Bh.dense(), Linv.T, C) * P))
         assert abs(ek - ref) <= 1e-12 * abs(ref)
     Cd.release()
+
+
+@pytest.mark.parametrize("nbatch", [1, 3])
+def test_graph_replay_matches_direct(ctx, nbatch):
+    """Repeated builds with the same device operands replay a captured hipGraph
+    of their launch chain (captured on the second call, replayed from the
+    third): every replay gives the bits of the direct launches, for the
+    BF16x3, FP32 and FP64-copy exchanges (one and three occupied batches) and
+    DF-J, with the operands' contents changed between calls."""
+    from chem_amd import _lib
+    from chem_amd.engine import DeviceArray, DeviceB, Metric
+    mol, Bh, V = _system(4)
+    nbf, nocc = mol.nbf, mol.nocc
+    m = Metric(ctx, V)
+    dB = DeviceB(ctx, Bh)
+    dB.make_lowp(m, f32=True, bf16=True, f64=True)
+    Cs = [_random_occ(nbf, nocc, 100 + i)[0] for i in range(5)]
+    Cd = DeviceArray(ctx, (nbf, nocc))
+    Kd = DeviceArray(ctx, (nbf, nbf))
+    Pd = DeviceArray(ctx, (nbf, nbf))
+
+    def run(form, C):
+        if form == "dfj":
+            Pd.upload(C @ C.T)
+            _lib.check(_lib.lib().mchip_dfj_dev(dB.h, m.h, Pd.ptr, 2.0, Kd.ptr))
+        else:
+            Cd.upload(C)
+            dB.dfk_dev(form, Cd, nocc, Kd, occ_nbatches=nbatch)
+        ctx.sync()
+        return Kd.download()
+
+    try:
+        for form in ("bf16x3", "f32", "fp64_copy", "dfj"):
+            print("case: graph replay %s nbatch=%d" % (form, nbatch), flush=True)
+            ctx.set_graphs(False)
+            ref = [run(form, C) for C in Cs]
+            ctx.set_graphs(True)
+            n0, r0 = dB.graph_stats()
+            for i, C in enumerate(Cs):
+                assert np.array_equal(run(form, C), ref[i]), (form, i)
+            n1, r1 = dB.graph_stats()
+            # first call direct (the key's first run), second captured, then replays
+            assert n1 == n0 + 1 and r1 - r0 == len(Cs) - 1, (form, n0, n1, r0, r1)
+    finally:
+        ctx.set_graphs(True)
+        for a in (Cd, Kd, Pd):
+            a.release()
