@@ -148,7 +148,8 @@ def exact_kernels(form, plan):
         k1 = "%s<2,%d,%d,%d,%d>" % (base, wn, tm, tn, plan["stages"])
         v = plan["syrk_variant"]
         k3 = "k_seg_syrk_bf16x3<128>" if v == 0 else (
-            "k_seg_syrk_bf16x3_dl<2,256,8>" if v == 4 else "k_seg_syrk_bf16x3_dl<%d,128,4>" % (v + 1))
+            "k_seg_syrk_bf16x3_dl<2,256,8,%s>" % ("true" if v == 4 else "false") if v >= 4
+            else "k_seg_syrk_bf16x3_dl<%d,128,4,true>" % (v + 1))
     else:
         k1 = "k_slab_gemm_%s_dl<2,%d,%d,%d,%d>" % (form, wn, tm, tn, plan["stages"])
         k3 = "k_seg_syrk_%s_dl<%d>" % (form, (plan["syrk_stages"] if form == "f32" else plan["syrk_variant"] + 2))

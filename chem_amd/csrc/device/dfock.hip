@@ -717,8 +717,9 @@ static const Bf16Shape kBf16Shapes[] = {
 static constexpr int kNumBf16Shapes = sizeof(kBf16Shapes) / sizeof(kBf16Shapes[0]);
 // SYRK variants: 0 = register-staged on FP32 W; 1..3 = direct-to-LDS on bf16
 // hi/lo W with 2, 3, 4 stages
-// 0 staged 128, 1-3 direct-to-LDS 128 (2-4 stages), 4 direct 256
-static constexpr int kNumSyrkVariants = 5;
+// 0 staged 128, 1-3 direct-to-LDS 128 (2-4 stages), 4 direct 256, 5 direct 256
+// with the diagonal tiles loading both sides (every tile of a chunk equally long)
+static constexpr int kNumSyrkVariants = 6;
 
 static void slab_gemm_bf16(Ctx& c, DevBT& t, int shape, const uint16_t* ghi, const uint16_t* glo, int ng,
                            float* W, uint16_t* Whi, uint16_t* Wlo, int ldw)
@@ -792,23 +793,30 @@ static void seg_syrk_bf16(Ctx& c, DevBT& t, int variant, const float* W, const u
                          nbf, tl, ntile, spc, partial);
       break;
     case 1:
-      hipLaunchKernelGGL((k_seg_syrk_bf16x3_dl<2, 128, 4>), grid8, dim3(256), 0, c.stream, Whi, Wlo, naux,
+      hipLaunchKernelGGL((k_seg_syrk_bf16x3_dl<2, 128, 4, true>), grid8, dim3(256), 0, c.stream, Whi, Wlo, naux,
                          (int64_t)nbf * ldw, ldw, nbf, tl, ntile, spc, partial);
       break;
     case 2:
-      hipLaunchKernelGGL((k_seg_syrk_bf16x3_dl<3, 128, 4>), grid8, dim3(256), 0, c.stream, Whi, Wlo, naux,
+      hipLaunchKernelGGL((k_seg_syrk_bf16x3_dl<3, 128, 4, true>), grid8, dim3(256), 0, c.stream, Whi, Wlo, naux,
                          (int64_t)nbf * ldw, ldw, nbf, tl, ntile, spc, partial);
       break;
     case 3:
-      hipLaunchKernelGGL((k_seg_syrk_bf16x3_dl<4, 128, 4>), grid8, dim3(256), 0, c.stream, Whi, Wlo, naux,
+      hipLaunchKernelGGL((k_seg_syrk_bf16x3_dl<4, 128, 4, true>), grid8, dim3(256), 0, c.stream, Whi, Wlo, naux,
                          (int64_t)nbf * ldw, ldw, nbf, tl, ntile, spc, partial);
       break;
     case 4: {
       int nt2 = 0;
       const int2* tl2 = syrk_tiles(c, t, nbf, 256, &nt2);
-      hipLaunchKernelGGL((k_seg_syrk_bf16x3_dl<2, 256, 8>), dim3((unsigned)((nt2 * nch + 7) / 8 * 8)), dim3(512), 0,
+      hipLaunchKernelGGL((k_seg_syrk_bf16x3_dl<2, 256, 8, true>), dim3((unsigned)((nt2 * nch + 7) / 8 * 8)), dim3(512), 0,
                          c.stream, Whi,
                          Wlo, naux, (int64_t)nbf * ldw, ldw, nbf, tl2, nt2, spc, partial);
+      break;
+    }
+    case 5: {
+      int nt2 = 0;
+      const int2* tl2 = syrk_tiles(c, t, nbf, 256, &nt2);
+      hipLaunchKernelGGL((k_seg_syrk_bf16x3_dl<2, 256, 8, false>), dim3((unsigned)((nt2 * nch + 7) / 8 * 8)),
+                         dim3(512), 0, c.stream, Whi, Wlo, naux, (int64_t)nbf * ldw, ldw, nbf, tl2, nt2, spc, partial);
       break;
     }
     default: throw std::runtime_error("BF16 SYRK: unknown variant");

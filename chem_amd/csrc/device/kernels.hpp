@@ -83,7 +83,7 @@ template <int WM, int WN, int TM, int TN, int SA>
 __global__ void k_slab_gemm_bf16x3_dla_group(const ExGroupReq* reqs, const Tile* tiles);
 template <int S, int BM, int NW>
 __global__ void k_seg_syrk_bf16x3_dl_group(const ExGroupReq* reqs, const int4* items);
-template <int S, int BM, int NW>
+template <int S, int BM, int NW, bool DIAG1>
 __global__ void k_seg_syrk_bf16x3_dl(const uint16_t* whi, const uint16_t* wlo, int nseg, int64_t sstride,
                                      int ldw, int n, const int2* tiles, int ntile, int segs_per_chunk,
                                      float* partial);

@@ -413,6 +413,50 @@ __device__ __forceinline__ void bf16x3_step_swz(const uint4* Ah, const uint4* Al
     }
 }
 
+// The same step, software-pipelined: every B fragment is read first, then the
+// A fragments of row i + 1 are read while the 3 TN MFMAs of row i run (two
+// A fragment registers sets), so that LDS latency overlaps the MFMAs.  The
+// order is pinned with scheduling-group barriers (the scheduler otherwise
+// reads fragments just before their first MFMA and waits for each).  Each
+// accumulator receives its three products in the same order as in
+// bf16x3_step_swz, so the results are the same bits.
+template <int TM, int TN>
+__device__ __forceinline__ void bf16x3_step_pipe(const uint4* Ah, const uint4* Al, const uint4* Bh,
+                                                 const uint4* Bl, int wm0, int wn0, int lane,
+                                                 f32x4 (&acc)[TM][TN])
+{
+  const int fr = lane & 15, fk = lane >> 4;
+  uint4 bh[TN], bl[TN], ah[2], al[2];
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int r = wn0 + 16 * j + fr;
+    bh[j] = Bh[r * 4 + swz(r, fk)];
+    bl[j] = Bl[r * 4 + swz(r, fk)];
+  }
+  {
+    const int r = wm0 + fr;
+    ah[0] = Ah[r * 4 + swz(r, fk)];
+    al[0] = Al[r * 4 + swz(r, fk)];
+  }
+  __builtin_amdgcn_sched_group_barrier(0x100, 2 * TN + 2, 0);   // DS reads: B fragments, A row 0
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+    if (i + 1 < TM) {
+      const int r = wm0 + 16 * (i + 1) + fr;
+      ah[(i + 1) & 1] = Ah[r * 4 + swz(r, fk)];
+      al[(i + 1) & 1] = Al[r * 4 + swz(r, fk)];
+      __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);       // DS reads: A row i + 1
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      acc[i][j] = mfma_bf16(al[i & 1], bh[j], acc[i][j]);
+      acc[i][j] = mfma_bf16(ah[i & 1], bl[j], acc[i][j]);
+      acc[i][j] = mfma_bf16(ah[i & 1], bh[j], acc[i][j]);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x008, 3 * TN, 0);    // MFMAs of row i
+  }
+}
+
 // Issue the loads of one stage: the wave-loads j = wave + NW*u (u < LPW) over
 // the list [A hi rows/16][A lo][B hi][B lo].
 template <int BM, int BN, int NW>
@@ -674,17 +718,28 @@ __device__ __forceinline__ void slab_bf16x3_dla_tile(
       lds_load16((lo ? Gl : Gh) + (int64_t)gr * ld8 + s * 4 + swz(r, pos), st + (lo ? BN * 4 : 0) + q * 64);
     }
   };
-  // issue order A(0..SA-2), B(0), then per step A(s+SA-1), B(s+1): every load
-  // that step s needs precedes B(s), so B(s) landing (vmcnt in order) is enough
-  static_for<SA - 1>([&](auto k) {
+  // Issue order: A(0..SA-3), B(0), A(SA-2), then per step s: B(s+1),
+  // A(s+SA-1).  vmcnt counts loads in issue order, so a wait for a load also
+  // waits for every load issued before it.  Step s needs A(s) and B(s):
+  // * SA >= 3: B(s) (issued first in step s - 1) is the later of the two; after
+  //   it come A(s+SA-2), B(s+1) and A(s+SA-1), so vmcnt(2 LA + LB) suffices
+  //   and A(s+1) stays in flight: the B~ stream (from HBM) gets two steps of
+  //   latency cover, the coefficients (from L2) one.  (With A issued before B
+  //   in a step, waiting for B(s) would also wait for A(s+1): one step of
+  //   cover for B~ whatever SA.)
+  // * SA == 2: A(s) (issued last in step s - 1) is the later; after it come
+  //   B(s+1) and A(s+1): vmcnt(LA + LB).
+  static_for<(SA >= 3 ? SA - 2 : 0)>([&](auto k) {
     if (k.value < nsteps) issue_a(k.value, stage_a(k));
   });
   if (nsteps > 0) issue_b(0, stage_b(std::integral_constant<int, 0>{}));
+  if (SA - 2 < nsteps) issue_a(SA - 2, stage_a(std::integral_constant<int, SA - 2>{}));
   auto compute = [&](auto k) {
     const uint4* sa = stage_a(k);
     const uint4* sb = stage_b(k);
     bf16x3_step_swz<TM, TN>(sa, sa + BM * 4, sb, sb + BN * 4, wm0, wn0, lane, acc);
   };
+  constexpr int WFULL = SA >= 3 ? 2 * LA + LB : LA + LB;
   // whole stage cycles in which every step issues both loads (no branches:
   // the wait-count model stays exact across the loop's back edge), then the
   // remaining steps with the issues and waits guarded
@@ -693,9 +748,9 @@ __device__ __forceinline__ void slab_bf16x3_dla_tile(
     static_for<CYC>([&](auto k) {
       const int s = s0 + k.value;
       block_sync();   // every wave is done with step s - 1's buffers
-      issue_a(s + SA - 1, stage_a(std::integral_constant<int, k.value + SA - 1>{}));
       issue_b(s + 1, stage_b(std::integral_constant<int, k.value + 1>{}));
-      wait_vm<LA + LB>();
+      issue_a(s + SA - 1, stage_a(std::integral_constant<int, k.value + SA - 1>{}));
+      wait_vm<WFULL>();
       block_sync();   // step s's operands have landed for every wave
       compute(k);
     });
@@ -706,14 +761,31 @@ __device__ __forceinline__ void slab_bf16x3_dla_tile(
       if (s < nsteps) {
         block_sync();
         const bool na = s + SA - 1 < nsteps, nb = s + 1 < nsteps;
-        if (na) issue_a(s + SA - 1, stage_a(std::integral_constant<int, k.value + SA - 1>{}));
         if (nb) issue_b(s + 1, stage_b(std::integral_constant<int, k.value + 1>{}));
-        if (na)
-          wait_vm<LA + LB>();
-        else if (nb)
-          wait_vm<LB>();
-        else
-          wait_vm<0>();
+        if (na) issue_a(s + SA - 1, stage_a(std::integral_constant<int, k.value + SA - 1>{}));
+        if constexpr (SA >= 3) {
+          // loads issued after B(s): A(s+SA-2) if it exists, then this step's
+          const bool pa = s + SA - 2 < nsteps;
+          if (na)
+            wait_vm<2 * LA + LB>();
+          else if (pa && nb)
+            wait_vm<LA + LB>();
+          else if (pa)
+            wait_vm<LA>();
+          else if (nb)
+            wait_vm<LB>();
+          else
+            wait_vm<0>();
+        }
+        else {
+          // loads issued after A(s): this step's (na == nb when SA == 2)
+          if (na)
+            wait_vm<LA + LB>();
+          else if (nb)
+            wait_vm<LB>();
+          else
+            wait_vm<0>();
+        }
         block_sync();
         compute(k);
       }
@@ -758,7 +830,7 @@ __global__ __launch_bounds__(64 * WM * WN, (dl_min_blocks<WM, WN, TM, TN>())) vo
 // LDS fragment reads per MFMA, measured the same: the step is bound by the
 // operand fetches into LDS.)  Rows of W have ldw (multiple of 32) bf16.
 // One output tile tl of one aux chunk of the SYRK.
-template <int S, int BM, int NW>
+template <int S, int BM, int NW, bool DIAG1 = true>
 __device__ __forceinline__ void syrk_bf16x3_dl_tile(const uint16_t* __restrict__ whi,
                                                     const uint16_t* __restrict__ wlo, int nseg, int64_t sstride,
                                                     int ldw, int n, int chunk, int2 tl, int segs_per_chunk,
@@ -804,10 +876,13 @@ __device__ __forceinline__ void syrk_bf16x3_dl_tile(const uint16_t* __restrict__
         [&](int, auto k) {
           const uint4* st = pick_stage<S>(k, lds0, lds1, lds2, lds3);
           const uint4* sb = D ? st : st + BM * 8;
-          bf16x3_step_swz<TM, TN>(st, st + BM * 4, sb, sb + BM * 4, wm0, wn0, lane, acc);
+          bf16x3_step_pipe<TM, TN>(st, st + BM * 4, sb, sb + BM * 4, wm0, wn0, lane, acc);
         });
   };
-  if (m0 == n0)
+  // DIAG1 = false: diagonal tiles load both sides like the others, so that
+  // every tile of a chunk takes the same time (its tiles share W rows in L2
+  // only while they stay in step)
+  if (DIAG1 && m0 == n0)
     run(std::true_type{});
   else
     run(std::false_type{});
@@ -825,7 +900,7 @@ __device__ __forceinline__ void syrk_bf16x3_dl_tile(const uint16_t* __restrict__
       }
 }
 
-template <int S, int BM, int NW>
+template <int S, int BM, int NW, bool DIAG1>
 __global__ __launch_bounds__(64 * NW) void k_seg_syrk_bf16x3_dl(
     const uint16_t* __restrict__ whi, const uint16_t* __restrict__ wlo, int nseg, int64_t sstride, int ldw,
     int n, const int2* __restrict__ tiles, int ntile, int segs_per_chunk, float* __restrict__ partial)
@@ -834,8 +909,8 @@ __global__ __launch_bounds__(64 * NW) void k_seg_syrk_bf16x3_dl(
   // the chunk come from HBM about once per XCD and are re-read from its L2
   // (grid padded to a multiple of 8; padding blocks exit)
   const int lid = xcd_tile(blockIdx.x, gridDim.x);
-  syrk_bf16x3_dl_tile<S, BM, NW>(whi, wlo, nseg, sstride, ldw, n, lid / ntile, tiles[lid % ntile], segs_per_chunk,
-                                 partial);
+  syrk_bf16x3_dl_tile<S, BM, NW, DIAG1>(whi, wlo, nseg, sstride, ldw, n, lid / ntile, tiles[lid % ntile],
+                                        segs_per_chunk, partial);
 }
 
 // The same SYRK over the work items of several exchange builds at once: item
@@ -874,13 +949,15 @@ MCHIP_SLAB16DLA_INST(2, 4, 6, 5, 3)
 MCHIP_SLAB16DLA_INST(2, 4, 6, 5, 2)
 #undef MCHIP_SLAB16DLA_INST
 #define MCHIP_SYRK16DL_INST(S, BM, NW)                                                                 \
-  template __global__ void k_seg_syrk_bf16x3_dl<S, BM, NW>(const uint16_t*, const uint16_t*, int, int64_t,   \
+  template __global__ void k_seg_syrk_bf16x3_dl<S, BM, NW, true>(const uint16_t*, const uint16_t*, int, int64_t,   \
                                                            int, int, const int2*, int, int, float*);
 MCHIP_SYRK16DL_INST(2, 128, 4)
 MCHIP_SYRK16DL_INST(3, 128, 4)
 MCHIP_SYRK16DL_INST(4, 128, 4)
 MCHIP_SYRK16DL_INST(2, 256, 8)
 #undef MCHIP_SYRK16DL_INST
+template __global__ void k_seg_syrk_bf16x3_dl<2, 256, 8, false>(const uint16_t*, const uint16_t*, int, int64_t, int,
+                                                                int, const int2*, int, int, float*);
 template __global__ void k_slab_gemm_bf16x3_dla_group<2, 4, 4, 5, 3>(const ExGroupReq*, const Tile*);
 template __global__ void k_slab_gemm_bf16x3_dla_group<2, 2, 4, 5, 2>(const ExGroupReq*, const Tile*);
 template __global__ void k_seg_syrk_bf16x3_dl_group<2, 256, 8>(const ExGroupReq*, const int4*);
