@@ -684,7 +684,8 @@ __device__ __forceinline__ void slab_bf16x3_dla_tile(
   const uint4* Al = reinterpret_cast<const uint4*>(blo + off16[N]);
   const uint4* Gh = reinterpret_cast<const uint4*>(ghi + ldp_pre[N] * ng);
   const uint4* Gl = reinterpret_cast<const uint4*>(glo + ldp_pre[N] * ng);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // the wave index as a scalar: the operand bases chosen by it stay in SGPRs
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm0 = (wave / WN) * (16 * TM), wn0 = (wave % WN) * (16 * TN);
   const int rr = lane >> 2, pos = lane & 3;
   f32x4 acc[TM][TN];
@@ -696,15 +697,33 @@ __device__ __forceinline__ void slab_bf16x3_dla_tile(
   const int64_t alast = rows - 1;
   const int blast = ng - 1;
   // wave-loads j = wave + NW*u over [hi rows/16][lo rows/16] of one operand
+  // per wave-load: a 32-bit row offset (in 16-byte units) from a wave-uniform
+  // base (slabs and gathered coefficients stay below 2^32 bytes), so that the
+  // loads take the scalar-base form and the row addresses one VGPR each
+  uint32_t offa[LA], offb[LB];
+#pragma unroll
+  for (int u = 0; u < LA; ++u) {
+    const int j = wave + NW * u;
+    const int q = j >= QA ? j - QA : j;
+    const int r = 16 * q + rr;
+    const int64_t gr = t.m0 + r < alast ? t.m0 + r : alast;
+    offa[u] = (uint32_t)(gr * ld8 + swz(r, pos));
+  }
+#pragma unroll
+  for (int u = 0; u < LB; ++u) {
+    const int j = wave + NW * u;
+    const int q = j >= QB ? j - QB : j;
+    const int r = 16 * q + rr;
+    const int gr = t.n0 + r < blast ? t.n0 + r : blast;
+    offb[u] = (uint32_t)((int64_t)gr * ld8 + swz(r, pos));
+  }
   auto issue_a = [&](int s, uint4* st) {
 #pragma unroll
     for (int u = 0; u < LA; ++u) {
       const int j = wave + NW * u;
       const bool lo = j >= QA;
       const int q = lo ? j - QA : j;
-      const int r = 16 * q + rr;
-      const int64_t gr = t.m0 + r < alast ? t.m0 + r : alast;
-      lds_load16((lo ? Al : Ah) + gr * ld8 + s * 4 + swz(r, pos), st + (lo ? BM * 4 : 0) + q * 64);
+      lds_load16((lo ? Al : Ah) + (offa[u] + (uint32_t)(s * 4)), st + (lo ? BM * 4 : 0) + q * 64);
     }
   };
   auto issue_b = [&](int s, uint4* st) {
@@ -713,9 +732,7 @@ __device__ __forceinline__ void slab_bf16x3_dla_tile(
       const int j = wave + NW * u;
       const bool lo = j >= QB;
       const int q = lo ? j - QB : j;
-      const int r = 16 * q + rr;
-      const int gr = t.n0 + r < blast ? t.n0 + r : blast;
-      lds_load16((lo ? Gl : Gh) + (int64_t)gr * ld8 + s * 4 + swz(r, pos), st + (lo ? BN * 4 : 0) + q * 64);
+      lds_load16((lo ? Gl : Gh) + (offb[u] + (uint32_t)(s * 4)), st + (lo ? BN * 4 : 0) + q * 64);
     }
   };
   // Issue order: A(0..SA-3), B(0), A(SA-2), then per step s: B(s+1),
@@ -737,7 +754,7 @@ __device__ __forceinline__ void slab_bf16x3_dla_tile(
   auto compute = [&](auto k) {
     const uint4* sa = stage_a(k);
     const uint4* sb = stage_b(k);
-    bf16x3_step_swz<TM, TN>(sa, sa + BM * 4, sb, sb + BN * 4, wm0, wn0, lane, acc);
+    bf16x3_step_pipe<TM, TN>(sa, sa + BM * 4, sb, sb + BN * 4, wm0, wn0, lane, acc);
   };
   constexpr int WFULL = SA >= 3 ? 2 * LA + LB : LA + LB;
   // whole stage cycles in which every step issues both loads (no branches:
