@@ -457,6 +457,55 @@ __device__ __forceinline__ void bf16x3_step_pipe(const uint4* Ah, const uint4* A
   }
 }
 
+// The software-pipelined step with the next stages' global loads spread over
+// its first SPAN rows: issue(u) starts wave-load u (u < NL), and row i < SPAN
+// issues loads [NL i / SPAN, NL (i + 1) / SPAN) between its A-fragment reads
+// and its MFMAs.
+// Issued in one burst at the step boundary, a workgroup's stage loads queue
+// in the address unit (64 bytes per clock) while no MFMA runs; spread over
+// the rows they are processed under the MFMAs.  Same products, same order.
+template <int TM, int TN, int NL, int SPAN = TM, class Issue>
+__device__ __forceinline__ void bf16x3_step_pipe_io(const uint4* Ah, const uint4* Al, const uint4* Bh,
+                                                    const uint4* Bl, int wm0, int wn0, int lane,
+                                                    f32x4 (&acc)[TM][TN], Issue&& issue)
+{
+  const int fr = lane & 15, fk = lane >> 4;
+  uint4 bh[TN], bl[TN], ah[2], al[2];
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int r = wn0 + 16 * j + fr;
+    bh[j] = Bh[r * 4 + swz(r, fk)];
+    bl[j] = Bl[r * 4 + swz(r, fk)];
+  }
+  {
+    const int r = wm0 + fr;
+    ah[0] = Ah[r * 4 + swz(r, fk)];
+    al[0] = Al[r * 4 + swz(r, fk)];
+  }
+  __builtin_amdgcn_sched_group_barrier(0x100, 2 * TN + 2, 0);   // DS reads: B fragments, A row 0
+  static_for<TM>([&](auto ii) {
+    constexpr int i = decltype(ii)::value;
+    constexpr int u0 = i < SPAN ? NL * i / SPAN : NL, u1 = i < SPAN ? NL * (i + 1) / SPAN : NL;
+    if constexpr (i + 1 < TM) {
+      const int r = wm0 + 16 * (i + 1) + fr;
+      ah[(i + 1) & 1] = Ah[r * 4 + swz(r, fk)];
+      al[(i + 1) & 1] = Al[r * 4 + swz(r, fk)];
+      __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);       // DS reads: A row i + 1
+    }
+    if constexpr (u1 > u0) {
+      static_for<u1 - u0>([&](auto uu) { issue(std::integral_constant<int, u0 + decltype(uu)::value>{}); });
+      __builtin_amdgcn_sched_group_barrier(0x020, u1 - u0, 0); // global -> LDS loads
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      acc[i][j] = mfma_bf16(al[i & 1], bh[j], acc[i][j]);
+      acc[i][j] = mfma_bf16(ah[i & 1], bl[j], acc[i][j]);
+      acc[i][j] = mfma_bf16(ah[i & 1], bh[j], acc[i][j]);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x008, 3 * TN, 0);    // MFMAs of row i
+  });
+}
+
 // Issue the loads of one stage: the wave-loads j = wave + NW*u (u < LPW) over
 // the list [A hi rows/16][A lo][B hi][B lo].
 template <int BM, int BN, int NW>
@@ -648,6 +697,10 @@ __global__ __launch_bounds__(64 * WM * WN, (dl_min_blocks<WM, WN, TM, TN>())) vo
 // Every stage is an LDS array of its own and the step loop is unrolled over a
 // whole cycle of stages, so a step waits only for its own stage's loads.
 // Same operands, tile table and output as k_slab_gemm_bf16x3_dl (bf16 hi/lo W).
+// Zero rows: 256 rows of 64 bytes (hi or lo plane of a row tile's k-step),
+// the source of the B~ stage of steps past a tile's last (read, never written).
+__device__ uint4 g_zero_rows[1024] = {};
+
 template <int WM, int WN, int TM, int TN, int SA>
 __device__ __forceinline__ void slab_bf16x3_dla_tile(
     const Tile& t, const uint16_t* __restrict__ bhi, const uint16_t* __restrict__ blo,
@@ -735,78 +788,81 @@ __device__ __forceinline__ void slab_bf16x3_dla_tile(
       lds_load16((lo ? Gl : Gh) + (offb[u] + (uint32_t)(s * 4)), st + (lo ? BN * 4 : 0) + q * 64);
     }
   };
-  // Issue order: A(0..SA-3), B(0), A(SA-2), then per step s: B(s+1),
-  // A(s+SA-1).  vmcnt counts loads in issue order, so a wait for a load also
-  // waits for every load issued before it.  Step s needs A(s) and B(s):
-  // * SA >= 3: B(s) (issued first in step s - 1) is the later of the two; after
-  //   it come A(s+SA-2), B(s+1) and A(s+SA-1), so vmcnt(2 LA + LB) suffices
-  //   and A(s+1) stays in flight: the B~ stream (from HBM) gets two steps of
-  //   latency cover, the coefficients (from L2) one.  (With A issued before B
-  //   in a step, waiting for B(s) would also wait for A(s+1): one step of
-  //   cover for B~ whatever SA.)
-  // * SA == 2: A(s) (issued last in step s - 1) is the later; after it come
-  //   B(s+1) and A(s+1): vmcnt(LA + LB).
-  static_for<(SA >= 3 ? SA - 2 : 0)>([&](auto k) {
-    if (k.value < nsteps) issue_a(k.value, stage_a(k));
-  });
-  if (nsteps > 0) issue_b(0, stage_b(std::integral_constant<int, 0>{}));
-  if (SA - 2 < nsteps) issue_a(SA - 2, stage_a(std::integral_constant<int, SA - 2>{}));
-  auto compute = [&](auto k) {
-    const uint4* sa = stage_a(k);
-    const uint4* sb = stage_b(k);
-    bf16x3_step_pipe<TM, TN>(sa, sa + BM * 4, sb, sb + BN * 4, wm0, wn0, lane, acc);
-  };
-  constexpr int WFULL = SA >= 3 ? 2 * LA + LB : LA + LB;
-  // whole stage cycles in which every step issues both loads (no branches:
-  // the wait-count model stays exact across the loop's back edge), then the
-  // remaining steps with the issues and waits guarded
-  int s0 = 0;
-  for (; s0 + CYC + SA - 1 <= nsteps; s0 += CYC) {
-    static_for<CYC>([&](auto k) {
-      const int s = s0 + k.value;
-      block_sync();   // every wave is done with step s - 1's buffers
-      issue_b(s + 1, stage_b(std::integral_constant<int, k.value + 1>{}));
-      issue_a(s + SA - 1, stage_a(std::integral_constant<int, k.value + SA - 1>{}));
-      wait_vm<WFULL>();
-      block_sync();   // step s's operands have landed for every wave
-      compute(k);
-    });
+  // Prologue: A(0..SA-2) and B(0), all landed.  One barrier per step.  Step
+  // s: barrier (every wave's loads for step s have landed, every wave is done
+  // with step s - 1's stages), then the MFMAs of step s with the loads of
+  // B(s+1) and then A(s+SA-1) spread over its rows (into the stages step s - 1
+  // used), then vmcnt(LA) (SA >= 3): B(s+1) and everything issued before it,
+  // A(s+1) among them, have landed.  A(s+1) was issued one step earlier than
+  // B(s+1): the B~ stream (HBM) gets about SA - 1 steps of latency cover, the
+  // coefficients (L2) about one.  SA == 2: A(s+1) is issued after B(s+1) in
+  // step s and must land with it (vmcnt(0)).
+  // No step has a branch around its loads or MFMAs: the compiler's count of
+  // the direct-to-LDS writes then stays exact (after a branch around a load
+  // it waits for all of them, vmcnt(0), before the next LDS read).  The loop
+  // runs whole stage cycles; the steps past the last (fewer than a cycle)
+  // load their B~ stage from zero rows, so their products are exact zeros
+  // and the sums are unchanged, and their coefficient stage from the last
+  // step's columns (in bounds).
+  static_assert(BM <= 256, "zero rows cover 256 rows");
+  constexpr int WAIT = SA >= 3 ? LA : 0;
+  uint32_t offz[LA];
+#pragma unroll
+  for (int u = 0; u < LA; ++u) {
+    const int j = wave + NW * u;
+    const int q = j >= QA ? j - QA : j;
+    const int r = 16 * q + rr;
+    offz[u] = (uint32_t)(r * 4 + swz(r, pos));
   }
-  for (; s0 < nsteps; s0 += CYC) {
-    static_for<CYC>([&](auto k) {
-      const int s = s0 + k.value;
-      if (s < nsteps) {
-        block_sync();
-        const bool na = s + SA - 1 < nsteps, nb = s + 1 < nsteps;
-        if (nb) issue_b(s + 1, stage_b(std::integral_constant<int, k.value + 1>{}));
-        if (na) issue_a(s + SA - 1, stage_a(std::integral_constant<int, k.value + SA - 1>{}));
-        if constexpr (SA >= 3) {
-          // loads issued after B(s): A(s+SA-2) if it exists, then this step's
-          const bool pa = s + SA - 2 < nsteps;
-          if (na)
-            wait_vm<2 * LA + LB>();
-          else if (pa && nb)
-            wait_vm<LA + LB>();
-          else if (pa)
-            wait_vm<LA>();
-          else if (nb)
-            wait_vm<LB>();
-          else
-            wait_vm<0>();
-        }
-        else {
-          // loads issued after A(s): this step's (na == nb when SA == 2)
-          if (na)
-            wait_vm<LA + LB>();
-          else if (nb)
-            wait_vm<LB>();
-          else
-            wait_vm<0>();
-        }
-        block_sync();
-        compute(k);
+  if (nsteps > 0) {
+    const uint4* zr = reinterpret_cast<const uint4*>(g_zero_rows);
+    static_for<SA - 1>([&](auto k) {   // A(k), zero rows for a step past the last
+      uint4* st = stage_a(k);
+      const bool z = k.value >= nsteps;
+#pragma unroll
+      for (int u = 0; u < LA; ++u) {
+        const int j = wave + NW * u;
+        const bool lo = j >= QA;
+        const int q = lo ? j - QA : j;
+        lds_load16(z ? zr + offz[u] : (lo ? Al : Ah) + (offa[u] + (uint32_t)(k.value * 4)),
+                   st + (lo ? BM * 4 : 0) + q * 64);
       }
     });
+    issue_b(0, stage_b(std::integral_constant<int, 0>{}));
+    wait_vm<0>();
+    for (int s0 = 0; s0 < nsteps; s0 += CYC) {
+      static_for<CYC>([&](auto k) {
+        const int s = s0 + k.value;
+        uint4* nb = stage_b(std::integral_constant<int, k.value + 1>{});
+        uint4* na = stage_a(std::integral_constant<int, k.value + SA - 1>{});
+        const int sb1 = s + 1 < nsteps ? s + 1 : nsteps - 1;
+        const int sa1 = s + SA - 1;
+        const bool za = sa1 >= nsteps;   // uniform: zero rows for a step past the last
+        auto issue = [&](auto uu) {
+          constexpr int u = decltype(uu)::value;
+          if constexpr (u < LB) {
+            const int j = wave + NW * u;
+            const bool lo = j >= QB;
+            const int q = lo ? j - QB : j;
+            lds_load16((lo ? Gl : Gh) + (offb[u] + (uint32_t)(sb1 * 4)), nb + (lo ? BN * 4 : 0) + q * 64);
+          }
+          else {
+            constexpr int v = u - LB;
+            const int j = wave + NW * v;
+            const bool lo = j >= QA;
+            const int q = lo ? j - QA : j;
+            const uint4* src = za ? zr + offz[v] : (lo ? Al : Ah) + (offa[v] + (uint32_t)(sa1 * 4));
+            lds_load16(src, na + (lo ? BM * 4 : 0) + q * 64);
+          }
+        };
+        block_sync();
+        const uint4* sa = stage_a(k);
+        const uint4* sb = stage_b(k);
+        bf16x3_step_pipe_io<TM, TN, LA + LB>(sa, sa + BM * 4, sb, sb + BN * 4, wm0, wn0, lane, acc, issue);
+        wait_vm<WAIT>();
+      });
+    }
+    wait_vm<0>();   // no load into this workgroup's LDS outlives it
   }
   store_w_tile<TM, TN>(acc, t, wm0, wn0, lane, rows, bn, nbf, boff[N], ng, nullptr, whi, wlo, ldo);
 }
@@ -879,22 +935,71 @@ __device__ __forceinline__ void syrk_bf16x3_dl_tile(const uint16_t* __restrict__
   const uint4* Wl = reinterpret_cast<const uint4*>(wlo);
   // a diagonal tile (m0 == n0) has the same W rows on both sides: its stages
   // hold them once, with half the loads per wave (the waits count them)
+  // One barrier per step.  Step q: barrier (every wave's loads for step q have
+  // landed, every wave is done with step q - 1's stage), then the MFMAs of
+  // step q with the loads of step q + S - 1 (into the stage step q - 1 used)
+  // spread over the first half of its rows, then vmcnt((S - 2) LP): step q + 1
+  // has landed.  No step branches around its loads (the compiler's count of
+  // direct-to-LDS writes stays exact): whole stage cycles run, and the steps
+  // past the last load their m-side rows from zero rows (exact zero products)
+  // and their n-side rows from the last step (in bounds).
+  const uint4* zr = reinterpret_cast<const uint4*>(g_zero_rows);
+  const int uwave = __builtin_amdgcn_readfirstlane(wave);
   auto run = [&](auto diag) {
     constexpr bool D = decltype(diag)::value;
-    dl_pipeline<S, D ? LPW / 2 : LPW>(
-        nsteps,
-        [&](int q, auto k) {
-          const int sg = s0 + q / ksteps;
-          const int kc0 = (q % ksteps) * 4;
-          const int64_t base = (int64_t)sg * (sstride >> 3);
-          issue_stage<BM, D ? 0 : BM, NW>(pick_stage<S>(k, lds0, lds1, lds2, lds3), wave, lane, Wh + base,
-                                          Wl + base, Wh + base, Wl + base, m0, n - 1, n0, n - 1, ld8, ld8, kc0);
-        },
-        [&](int, auto k) {
+    constexpr int QA = BM / 16, QB = D ? 0 : BM / 16;
+    static_assert((2 * QA + 2 * QB) % NW == 0, "stage loads must split evenly over the waves");
+    constexpr int LP = (2 * QA + 2 * QB) / NW;   // wave-loads per wave per stage
+    const int rr = lane >> 2, pos = lane & 3;
+    uint32_t roff[LP], zoff[LP];
+#pragma unroll
+    for (int u = 0; u < LP; ++u) {
+      const int j = uwave + NW * u;
+      const bool sa = j < 2 * QA;
+      const int jj = sa ? j : j - 2 * QA;
+      const int q = jj % (sa ? QA : QB > 0 ? QB : 1);
+      const int r = 16 * q + rr;
+      const int g = (sa ? m0 : n0) + r;
+      roff[u] = (uint32_t)((int64_t)(g < n - 1 ? g : n - 1) * ld8 + swz(r, pos));
+      zoff[u] = (uint32_t)(r * 4 + swz(r, pos));
+    }
+    // wave-load u of step qs into stage st
+    auto issue_u = [&](int qs, uint4* st, int u) {
+      const int j = uwave + NW * u;
+      const bool sa = j < 2 * QA;
+      const int jj = sa ? j : j - 2 * QA;
+      const int qa = sa ? QA : QB;
+      const bool lo = jj >= qa;
+      const int q = lo ? jj - qa : jj;
+      const bool z = sa && qs >= nsteps;
+      const int qc = qs < nsteps ? qs : nsteps - 1;
+      const int sg = s0 + qc / ksteps;
+      const int64_t base = (int64_t)sg * (sstride >> 3) + (qc % ksteps) * 4;
+      const uint4* src = z ? zr + zoff[u] : (lo ? Wl : Wh) + (base + roff[u]);
+      lds_load16(src, st + (sa ? 0 : BM * 8) + (lo ? BM * 4 : 0) + q * 64);
+    };
+    if (nsteps > 0) {
+      static_for<S - 1>([&](auto k) {
+        uint4* st = pick_stage<S>(k, lds0, lds1, lds2, lds3);
+#pragma unroll
+        for (int u = 0; u < LP; ++u) issue_u(k.value, st, u);
+      });
+      wait_vm<0>();
+      for (int q0 = 0; q0 < nsteps; q0 += S) {
+        static_for<S>([&](auto k) {
+          const int q = q0 + k.value;
+          uint4* ns = pick_stage<S>(std::integral_constant<int, k.value + S - 1>{}, lds0, lds1, lds2, lds3);
+          auto issue = [&](auto uu) { issue_u(q + S - 1, ns, decltype(uu)::value); };
+          block_sync();
           const uint4* st = pick_stage<S>(k, lds0, lds1, lds2, lds3);
           const uint4* sb = D ? st : st + BM * 8;
-          bf16x3_step_pipe<TM, TN>(st, st + BM * 4, sb, sb + BM * 4, wm0, wn0, lane, acc);
+          bf16x3_step_pipe_io<TM, TN, LP, (TM + 1) / 2>(st, st + BM * 4, sb, sb + BM * 4, wm0, wn0, lane, acc,
+                                                         issue);
+          wait_vm<(S - 2) * LP>();
         });
+      }
+      wait_vm<0>();   // no load into this workgroup's LDS outlives it
+    }
   };
   // DIAG1 = false: diagonal tiles load both sides like the others, so that
   // every tile of a chunk takes the same time (its tiles share W rows in L2
