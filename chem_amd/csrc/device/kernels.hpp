@@ -23,9 +23,11 @@ __global__ void k_slab_out_gemm(const T* A, int lda, int klen, const int64_t* sl
                                 const int* colmap_off, const Tile* tiles, int naux, int nbf,
                                 const T* G, int ldg, T* out, const int* pcol, const int* pwidth,
                                 int64_t out_stride);
-template <int BM, int BN>
-__global__ void k_group_apply_f32(const float* P, const float* Plo, const float* in, float* out, int n, int R,
-                                  const int* pcol, const int* pwidth, const int4* tiles);
+__global__ void k_transpose_pad_f32(const float* in, int n, int R, int ldn, float* out);
+template <int S>
+__global__ void k_group_apply_f32_dl(const float* P, int npt, const float* bT, float* out, int n, int ldn, int R,
+                                     int naux, const int* pcol, const int* pwidth, const int4* tiles);
+__global__ void k_to_f32x2_rows(const double* in, int n, int ldn, float* hi, float* lo);
 __global__ void k_slabseg_f32(const float* A, int64_t a_stride, const float* B, const int64_t* seg_off,
                               const int64_t* seg_len, const int* chunk_seg, int nchunk, int n,
                               const int2* tiles, int ntile, double* partial);

@@ -3,6 +3,10 @@
 //   k_gather_ct_f32     per-slab gathered, transposed occupied coefficients in FP32
 //   k_slab_gemm_f32_dl  W(X, mu, i) = sum_nu B~(X, mu nu) C(nu, i) per slab (K1)
 //   k_seg_syrk_f32_dl   partial[chunk] = sum_{X in chunk} W_X W_X^T, upper tiles (K3)
+// and, for the grouped SOS-MP2 graph:
+//   k_transpose_pad_f32   b[X][nu][i] -> b^T[X][i][nu], rows padded to ldn
+//   k_group_apply_f32_dl  M3: P_v,g b_g[X] for every (point, X), P_v as an FP32 pair
+//   k_to_f32x2_rows       FP64 P_v -> FP32 hi/lo rows padded to ldn
 //
 // Operands use the padded slab layout of the BF16 copy: slab N has naux*bsz[N]
 // rows of ldp[N] (multiple of 32) elements at element offset off[N]; the
@@ -210,6 +214,148 @@ __global__ __launch_bounds__(512) void k_seg_syrk_f32_dl(const float* __restrict
         if (r < n && c < n) P[(int64_t)r * n + c] = acc64[i][j][e];
       }
 }
+
+// Rows of an n x R block per aux function, transposed and zero-padded:
+// out[X][c][k] = in[X][k][c] for k < n, 0 for n <= k < ldn (c < R).  32 x 32
+// tiles through LDS (padded rows: no bank conflicts); grid (R/32, ldn/32, X).
+__global__ __launch_bounds__(256) void k_transpose_pad_f32(const float* __restrict__ in, int n, int R, int ldn,
+                                                           float* __restrict__ out)
+{
+  __shared__ float tile[32][33];
+  const int64_t X = blockIdx.z;
+  const int c0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+  const float* src = in + X * n * R;
+  float* dst = out + X * (int64_t)R * ldn;
+#pragma unroll
+  for (int r = ty; r < 32; r += 8) {
+    const int k = k0 + r, c = c0 + tx;
+    tile[r][tx] = (k < n && c < R) ? src[(int64_t)k * R + c] : 0.f;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = ty; r < 32; r += 8) {
+    const int c = c0 + r, k = k0 + tx;
+    if (c < R && k < ldn) dst[(int64_t)c * ldn + k] = tile[tx][r];
+  }
+}
+
+// M3 of the grouped SOS-MP2 contractions (the virtual pseudo-density applied
+// to b for every X and Laplace point, z_full.cpp:224-240), direct-to-LDS:
+// per point g, C_g = (Phi_g + Plo_g) B_g with B_g's columns the pairs (X, c),
+// c < pwidth[g], column index X pwidth[g] + c:
+//   out[X][m][pcol[g] + c] = sum_k P_g(m, k) bT[X][pcol[g] + c][k].
+// Tiles (g, m0, column 0 of the tile, -) of 128 rows m and 128 columns (the
+// columns of all aux functions run on, so a rank below 128 wastes no tile
+// width).  The P rows (hi then lo plane) and the transposed b rows have ldn
+// (multiple of 16) floats, zero past n; each 16-element step adds its FP32
+// MFMA sums (16 products) into FP64 running sums.  Eight waves of 64 x 32.
+template <int S>
+__global__ __launch_bounds__(512, 2) void k_group_apply_f32_dl(const float* __restrict__ P, int npt,
+                                                            const float* __restrict__ bT, float* __restrict__ out,
+                                                            int n, int ldn, int R, int naux,
+                                                            const int* __restrict__ pcol,
+                                                            const int* __restrict__ pwidth,
+                                                            const int4* __restrict__ tiles)
+{
+  constexpr int BM = 128, BN = 128, NW = 8, WN = 4, TM = 4, TN = 2;
+  constexpr int QA = BM / 16, QB = BN / 16;
+  constexpr int LPW = (QA + QB) / NW;
+  static_assert((QA + QB) % NW == 0, "stage loads must split evenly over the waves");
+  constexpr int STAGE = (BM + BN) * 4;
+  static_assert(S >= 2 && S <= 4, "pipeline depth");
+  __shared__ uint4 lds0[STAGE], lds1[STAGE], lds2[STAGE], lds3[STAGE];
+  const int4 t = tiles[blockIdx.x];   // (point, m0, first column, -)
+  const int g = t.x, w = pwidth[g], pc = pcol[g];
+  const int ncols = naux * w;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm0 = (wave / WN) * 64, wn0 = (wave % WN) * 32;
+  // two sets of FP32 accumulators: step s sums into set s % 2 while the set of
+  // step s - 1 (its MFMAs long done) is added into the FP64 sums, so no step
+  // waits for its own MFMA results; the FP64 additions keep the step order
+  static_assert(S % 2 == 0, "accumulator set = stage parity");
+  f32v4 acc[2][TM][TN];
+  double acc64[TM][TN][4];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      acc[0][i][j] = f32v4{0.f, 0.f, 0.f, 0.f};
+      acc[1][i][j] = f32v4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc64[i][j][e] = 0.0;
+    }
+  const int ld4 = ldn >> 2;
+  const int nh = ldn >> 4;   // steps per P plane
+  static_assert(QA == NW && QB == NW, "one A row block and one B column block per wave and stage");
+  const uint4* Phi = reinterpret_cast<const uint4*>(P + (int64_t)g * n * ldn);
+  const int64_t lo_off = (int64_t)npt * n * ld4;   // the lo plane, in 16-byte units
+  const uint4* B4 = reinterpret_cast<const uint4*>(bT);
+  // per wave: A rows 16 wave + rr and B columns (X, c) 16 wave + rr (clamped
+  // to the last valid), offsets in 16-byte units, fixed over the contraction
+  const int rr = lane >> 2, pos = lane & 3;
+  const int r = 16 * wave + rr;
+  const int m = t.y + r < n - 1 ? t.y + r : n - 1;
+  const int64_t offA = (int64_t)m * ld4 + swz(r, pos);
+  const int col = t.z + r < ncols - 1 ? t.z + r : ncols - 1;
+  const int Xc = col / w;
+  const int64_t offB = ((int64_t)Xc * R + pc + (col - Xc * w)) * ld4 + swz(r, pos);
+  dl_pipeline<S, LPW>(
+      2 * nh,
+      [&](int s, auto k) {
+        uint4* st = pick_stage<S>(k, lds0, lds1, lds2, lds3);
+        const bool lo = s >= nh;
+        const int kc = (lo ? s - nh : s) * 4;
+        lds_load16(Phi + (offA + (lo ? lo_off : 0) + kc), st + wave * 64);
+        lds_load16(B4 + (offB + kc), st + BM * 4 + wave * 64);
+      },
+      [&](int, auto k) {
+        constexpr int p = decltype(k)::value & 1;
+        const uint4* st = pick_stage<S>(k, lds0, lds1, lds2, lds3);
+        f32_step_swz<TM, TN>(st, st + BM * 4, wm0, wn0, lane, acc[p]);
+        flush_f64<TM, TN>(acc[1 - p], acc64);   // step s - 1 (zero before the first step)
+      });
+  flush_f64<TM, TN>(acc[(2 * nh - 1) & 1], acc64);   // the last step
+  const int fc = lane & 15, fr4 = 4 * (lane >> 4);
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int col = t.z + wn0 + 16 * j + fc;
+    if (col >= ncols) continue;
+    const int X = col / w, c = col - X * w;
+    float* o = out + (int64_t)X * n * R + pc + c;
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int m = t.y + wm0 + 16 * i + fr4 + e;
+        if (m < n) o[(int64_t)m * R] = (float)acc64[i][j][e];
+      }
+  }
+}
+
+// FP64 n x n -> FP32 hi/lo pair, rows padded with zeros to ldn.
+__global__ void k_to_f32x2_rows(const double* __restrict__ in, int n, int ldn, float* __restrict__ hi,
+                                float* __restrict__ lo)
+{
+  const int64_t total = (int64_t)n * ldn;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / ldn;
+    const int c = (int)(i - r * ldn);
+    float h = 0.f, l = 0.f;
+    if (c < n) {
+      const double v = in[r * n + c];
+      h = (float)v;
+      l = (float)(v - (double)h);
+    }
+    hi[i] = h;
+    lo[i] = l;
+  }
+}
+
+template __global__ void k_group_apply_f32_dl<2>(const float*, int, const float*, float*, int, int, int, int,
+                                                 const int*, const int*, const int4*);
+template __global__ void k_group_apply_f32_dl<4>(const float*, int, const float*, float*, int, int, int, int,
+                                                 const int*, const int*, const int4*);
 
 #define MCHIP_SLABF32DL_INST(WM, WN, TM, TN, S)                                                            \
   template __global__ void k_slab_gemm_f32_dl<WM, WN, TM, TN, S>(                                          \

@@ -316,92 +316,6 @@ __global__ __launch_bounds__(256) void k_slabseg_gemm(
       }
 }
 
-// Grouped dense product, one group per (point g, aux function X):
-//   out[X][m][pcol[g] + j] = sum_v P[g][m][v] in[X][v][pcol[g] + j],  j < pwidth[g]
-// P: n x n row-major per group, in/out: [naux][n][R] row-major.  The virtual
-// pseudo-density applied to b for every X and Laplace point (z_full.cpp:224-240).
-// FP32 MFMA with FP64 running sums flushed after every 16-product step.  With
-// Plo (P = P + Plo as an FP32 pair) the contraction runs over both halves.
-template <int BM, int BN>
-__global__ __launch_bounds__(256) void k_group_apply_f32(const float* __restrict__ P,
-                                                        const float* __restrict__ Plo,
-                                                        const float* __restrict__ in,
-                                                        float* __restrict__ out, int n, int R,
-                                                        const int* __restrict__ pcol,
-                                                        const int* __restrict__ pwidth,
-                                                        const int4* __restrict__ tiles)
-{
-  constexpr int TM = BM / 32, TN = BN / 32;
-  constexpr int LDA = BM + 16, LDB = BN + 16;
-  constexpr int RA = BM / 16, RB = BN / 16;
-  __shared__ float As[2][GM_BK * LDA];
-  __shared__ float Bs[2][GM_BK * LDB];
-  const int4 t = tiles[blockIdx.x];   // (group, X, m0, n0)
-  const int g = t.x;
-  const float* Ahi = P + (int64_t)g * n * n;
-  const float* Alo = Plo ? Plo + (int64_t)g * n * n : nullptr;
-  const float* Bg = in + (int64_t)t.y * n * R + pcol[g];
-  float* Cg = out + (int64_t)t.y * n * R + pcol[g];
-  const int w = pwidth[g];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm0 = (wave >> 1) * (BM / 2), wn0 = (wave & 1) * (BN / 2);
-  Mfma<float>::v4 acc[TM][TN];
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j) acc[i][j] = Mfma<float>::v4{0.f, 0.f, 0.f, 0.f};
-  double acc64[TM][TN][4] = {};
-  const int ak = tid & 15, ar = tid >> 4;
-  const int bk = tid >> 4, bc = tid & 15;
-  float ra[RA], rb[RB];
-  // steps [0, nh) contract with P (hi part), steps [nh, 2 nh) with Plo
-  const int nh = (n + GM_BK - 1) / GM_BK;
-  const int nsteps = Alo ? 2 * nh : nh;
-  auto load = [&](int s) {
-    const float* A = (s < nh) ? Ahi : Alo;
-    const int k0 = (s < nh ? s : s - nh) * GM_BK;
-#pragma unroll
-    for (int j = 0; j < RA; ++j) {
-      const int m = t.z + ar + 16 * j, kk = k0 + ak;
-      ra[j] = (m < n && kk < n) ? A[(int64_t)m * n + kk] : 0.f;
-    }
-#pragma unroll
-    for (int j = 0; j < RB; ++j) {
-      const int c = t.w + bc + 16 * j, kk = k0 + bk;
-      rb[j] = (c < w && kk < n) ? Bg[(int64_t)kk * R + c] : 0.f;
-    }
-  };
-  auto store = [&](int buf) {
-#pragma unroll
-    for (int j = 0; j < RA; ++j) As[buf][ak * LDA + ar + 16 * j] = ra[j];
-#pragma unroll
-    for (int j = 0; j < RB; ++j) Bs[buf][bk * LDB + bc + 16 * j] = rb[j];
-  };
-  if (nsteps > 0) {
-    load(0);
-    store(0);
-  }
-  __syncthreads();
-  for (int s = 0; s < nsteps; ++s) {
-    const int cur = s & 1;
-    if (s + 1 < nsteps) load(s + 1);
-    mfma_step<float, TM, TN, LDA, LDB>(As[cur], Bs[cur], wm0, wn0, lane, acc);
-    flush_f64<TM, TN>(acc, acc64);
-    if (s + 1 < nsteps) store(cur ^ 1);
-    __syncthreads();
-  }
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int r = t.z + wm0 + 16 * i + Mfma<float>::row(lane, e);
-        const int c = t.w + wn0 + 16 * j + (lane & 15);
-        if (r < n && c < w) Cg[(int64_t)r * R + c] = (float)acc64[i][j][e];
-      }
-}
-
 // Grouped slab-segment SYRK in FP32 with FP64-flushed accumulators (8 waves,
 // 64 x 32 each, flush every 4 steps = 64 products):
 //   partial[g][chunk](m,n) = sum_{slabs s in chunk} sum_k A_g(off_s + m*len_s + k) B(off_s + n*len_s + k)
@@ -513,8 +427,6 @@ __global__ __launch_bounds__(512) void k_slabseg_f32(const float* __restrict__ A
       }
 }
 
-template __global__ void k_group_apply_f32<128, 64>(const float*, const float*, const float*, float*, int, int,
-                                                    const int*, const int*, const int4*);
 
 template __global__ void k_slab_out_gemm<double, 128, 128>(const double*, int, int, const int64_t*,
     const int*, const int*, const int*, const int*, const int*, const Tile*, int, int, const double*, int,

@@ -491,7 +491,8 @@ __global__ void k_cols_to_f32(const double* __restrict__ in, int rows, int w, fl
 // slab in FP64 from B):
 //   M2  b[X][nu][i]     = sum_mu B~(X|mu nu) L_all(mu, i)  one slab GEMM over the
 //                                                         concatenated columns of all points
-//   M3  b2_g[X]         = P_v,g b_g[X]                     grouped over (point, X), P_v as an FP32 hi/lo pair
+//   M3  b2_g[X]         = P_v,g b_g[X]                     grouped over (point, X), P_v as an FP32 hi/lo pair,
+//                                                         direct-to-LDS on b transposed per X
 //   M4  b3_g(X|M N)     = sum_i L_g(M,i) b2_g(X,i,N)        grouped over points, B's pattern
 //   M5  Z'_g(X,Y)       = sum b3_g(X|mu nu) B~(Y|mu nu)      grouped over points, FP64-flushed
 // so Z'_g = L^-1 Z_g L^-T and e_g = tr(Z_g V^-1 Z_g V^-1) = sum_XY Z'_g(X,Y)^2
@@ -528,33 +529,38 @@ double mp2_sos_batched(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in,
     if (t_prep) *t_prep = 0.0;
     return 0.0;
   }
-  // FP32 operands: all L_o side by side (n x R), the P_v (n x n each), B~
-  const size_t nn = (size_t)n * n;
+  // FP32 operands: all L_o side by side (n x R), the P_v (n x n each, rows
+  // padded to ldn, a multiple of 16), B~
+  const int ldn = (n + 15) / 16 * 16;
+  const size_t nl = (size_t)n * ldn;
+  if (naux > 65535) throw std::runtime_error("MP2: the grouped graph handles at most 65535 aux functions");
   // P_v as an FP32 pair (hi at [g], lo at [npt + g]): P_v b cancels heavily,
   // and a single FP32 rounding of P_v alone costs ~1e-5 of max|Z|
-  DevBuf Lall((size_t)n * R * sizeof(float)), Pv32((size_t)2 * npt * nn * sizeof(float)),
+  DevBuf Lall((size_t)n * R * sizeof(float)), Pv32((size_t)2 * npt * nl * sizeof(float)),
       B32((size_t)std::max<int64_t>(L.total, 1) * sizeof(float));
   for (int g = 0; g < npt; ++g) {
     if (rank[g] == 0) continue;
     hipLaunchKernelGGL(k_cols_to_f32, dim3(nblk((int64_t)n * rank[g], 256)), dim3(256), 0, c.stream,
                        Lo[g].as<double>(), n, rank[g], Lall.as<float>(), R, pcol[g]);
-    hipLaunchKernelGGL(k_to_f32x2, dim3(gs_blocks((int64_t)nn)), dim3(256), 0, c.stream, Pv[g].as<double>(),
-                       Pv32.as<float>() + (size_t)g * nn, Pv32.as<float>() + (size_t)(npt + g) * nn, (int64_t)nn);
+    hipLaunchKernelGGL(k_to_f32x2_rows, dim3(gs_blocks((int64_t)nl)), dim3(256), 0, c.stream, Pv[g].as<double>(),
+                       n, ldn, Pv32.as<float>() + (size_t)g * nl, Pv32.as<float>() + (size_t)(npt + g) * nl);
   }
   HIP_CHECK(hipGetLastError());
   if (L.total) tilde_to_f32(c, bt, met, B32.as<float>());
   const size_t bsize = (size_t)naux * n * R;
-  DevBuf bb(bsize * sizeof(float)), b2(bsize * sizeof(float)),
+  // b transposed per aux function, rows of ldn: the second operand of M3
+  DevBuf bb(bsize * sizeof(float)), b2(bsize * sizeof(float)), bT((size_t)naux * R * ldn * sizeof(float)),
       b3((size_t)npt * std::max<int64_t>(L.total, 1) * sizeof(float));
   std::vector<int> pw(rank.begin(), rank.end());
   std::vector<int> pc(pcol.begin(), pcol.end() - 1);
   DevBuf d_pcol = upload_vec(pc, c.stream), d_pw = upload_vec(pw, c.stream);
-  // M3 tiles (point, X, m0, n0)
+  // M3 tiles (point, m0, first column): 128 x 128 over the columns (X, c) of
+  // each point, column index X rank + c
   std::vector<int4> t3;
   for (int g = 0; g < npt; ++g)
-    for (int x = 0; x < naux; ++x)
-      for (int m0 = 0; m0 < n; m0 += 128)
-        for (int n0 = 0; n0 < rank[g]; n0 += 64) t3.push_back(int4{g, x, m0, n0});
+    for (int64_t col = 0; col < (int64_t)naux * rank[g]; col += 128)
+      for (int m0 = 0; m0 < n; m0 += 128) t3.push_back(int4{g, m0, (int)col, 0});
+  if ((int64_t)naux * R >= ((int64_t)1 << 31)) throw std::runtime_error("MP2: too many (aux, rank) columns");
   // M4 tiles (slab, m0, c0, point)
   std::vector<Tile> t4;
   for (int g = 0; g < npt; ++g) {
@@ -608,6 +614,38 @@ double mp2_sos_batched(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in,
       }
     }
   }
+  // M3: b transposed (rows of the contraction index), then the grouped
+  // direct-to-LDS GEMM; its pipeline depth is timed on the live operands
+  const dim3 tgrid((unsigned)((R + 31) / 32), (unsigned)((ldn + 31) / 32), (unsigned)naux);
+  auto m3 = [&](int depth) {
+    if (depth == 2)
+      hipLaunchKernelGGL(k_group_apply_f32_dl<2>, dim3((unsigned)t3.size()), dim3(512), 0, c.stream,
+                         Pv32.as<float>(), npt, bT.as<float>(), b2.as<float>(), n, ldn, R, naux, d_pcol.as<int>(),
+                         d_pw.as<int>(), d_t3.as<int4>());
+    else
+      hipLaunchKernelGGL(k_group_apply_f32_dl<4>, dim3((unsigned)t3.size()), dim3(512), 0, c.stream,
+                         Pv32.as<float>(), npt, bT.as<float>(), b2.as<float>(), n, ldn, R, naux, d_pcol.as<int>(),
+                         d_pw.as<int>(), d_t3.as<int4>());
+  };
+  int m3_depth = 2;
+  {
+    hipLaunchKernelGGL(k_transpose_pad_f32, tgrid, dim3(256), 0, c.stream, bb.as<float>(), n, R, ldn,
+                       bT.as<float>());
+    float best = 1e30f;
+    for (int depth : {2, 4}) {
+      m3(depth);
+      HIP_CHECK(hipEventRecord(c.ev0, c.stream));
+      m3(depth);
+      HIP_CHECK(hipEventRecord(c.ev1, c.stream));
+      HIP_CHECK(hipEventSynchronize(c.ev1));
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+      if (ms < best) {
+        best = ms;
+        m3_depth = depth;
+      }
+    }
+  }
   HIP_CHECK(hipStreamSynchronize(c.stream));
   if (t_prep) *t_prep = std::chrono::duration<double>(std::chrono::steady_clock::now() - tp0).count();
   // capture M2..M5 and the chunk reductions in one graph
@@ -615,10 +653,8 @@ double mp2_sos_batched(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in,
   hipGraphExec_t exec = nullptr;
   HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
   slab_gemm_f32(c, bt, B32.as<float>(), Lall.as<float>(), R, R, bb.as<float>(), R, bn, true);
-  hipLaunchKernelGGL((k_group_apply_f32<128, 64>), dim3((unsigned)t3.size()), dim3(256), 0, c.stream,
-                     Pv32.as<float>(), Pv32.as<float>() + (size_t)npt * nn, bb.as<float>(), b2.as<float>(), n, R,
-                     d_pcol.as<int>(), d_pw.as<int>(),
-                     d_t3.as<int4>());
+  hipLaunchKernelGGL(k_transpose_pad_f32, tgrid, dim3(256), 0, c.stream, bb.as<float>(), n, R, ldn, bT.as<float>());
+  m3(m3_depth);
   hipLaunchKernelGGL((k_slab_out_gemm<float, SG_BM, 128>), dim3((unsigned)t4.size()), dim3(256), 0, c.stream,
                      b2.as<float>(), R, 0, bt.slab_off.as<int64_t>(), bt.ncol.as<int>(), bt.bsz.as<int>(),
                      bt.boff.as<int>(), bt.colmap.as<int>(), bt.colmap_off.as<int>(), d_t4.as<Tile>(), naux, n,
