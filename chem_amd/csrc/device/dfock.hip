@@ -1229,6 +1229,58 @@ void dbt_make_f32(Ctx& c, DevBT& t)
   t.lowp_tilde = true;
 }
 
+// Padded FP32 copy (data32) of B~ from its FP32 values in the slab layout
+// (tilde_to_f32's output): the operand of the direct-to-LDS FP32 slab GEMMs.
+void dbt_make_f32_from(Ctx& c, DevBT& t, const float* b32)
+{
+  const BLayout& L = t.L;
+  pad_layout(c, t);
+  t.data32.alloc((size_t)std::max<int64_t>(padded_elems(t), 1) * sizeof(float));
+  for (int n = 0; n < L.nbb; ++n) {
+    const int64_t ne = (int64_t)L.naux * L.bsz[n] * t.h_ldp[n];
+    if (ne == 0) continue;
+    hipLaunchKernelGGL(k_pad_f32_rows, dim3(gs_blocks(ne)), dim3(256), 0, c.stream, b32 + L.slab_off[n], n,
+                       t.ncol.as<int>(), t.bsz.as<int>(), L.naux, t.ldp.as<int>(), t.off16.as<int64_t>(),
+                       t.data32.as<float>());
+    HIP_CHECK(hipGetLastError());
+  }
+  t.f32_plan.clear();
+  t.lowp_tilde = true;
+}
+
+int64_t dbt_padded_cols(const DevBT& t) { return t.h_ldp_pre.empty() ? 0 : t.h_ldp_pre.back(); }
+
+// W = B~ C (W[X][nu][i], rows of ldo) on the padded FP32 copy with FP64 sums
+// of every 16-product step; C is FP32 n x ldc (ng columns), gathered per slab
+// into G (dbt_padded_cols(t) * ng floats).  Asynchronous; the tile table is
+// made on the first call (synchronising) and cached.
+void slab_gemm_f32_f64sum(Ctx& c, DevBT& t, const float* C, int ldc, int ng, float* G, float* out, int ldo,
+                          int depth)
+{
+  const BLayout& L = t.L;
+  if (!t.data32.p || t.h_ldp_pre.size() != (size_t)L.nbb + 1)
+    throw std::runtime_error("slab_gemm_f32_f64sum: no padded FP32 copy");
+  if (ng <= 0) return;
+  int gmax = 0;
+  for (int n = 0; n < L.nbb; ++n) gmax = std::max(gmax, t.h_ldp[n]);
+  if (gmax > 0 && L.nbb > 0)
+    hipLaunchKernelGGL(k_gather_ct_f32f, dim3(nblk((int64_t)ng * gmax, 256), L.nbb), dim3(256), 0, c.stream, C, ldc,
+                       ng, t.ncol.as<int>(), t.ldp.as<int>(), t.ldp_pre.as<int64_t>(), t.colmap.as<int>(),
+                       t.colmap_off.as<int>(), G);
+  int nt = 0;
+  const Tile* tiles = bf16_tiles(c, t, 128, 128, ng, &nt);
+  if (nt == 0) return;
+  if (depth == 2)
+    hipLaunchKernelGGL(k_slab_gemm_f32_dl_f64<2>, dim3(nt), dim3(512), 0, c.stream, t.data32.as<float>(),
+                       t.off16.as<int64_t>(), t.ldp.as<int>(), t.ldp_pre.as<int64_t>(), t.bsz.as<int>(),
+                       t.boff.as<int>(), tiles, L.naux, L.nbf, G, ng, out, ldo);
+  else
+    hipLaunchKernelGGL(k_slab_gemm_f32_dl_f64<4>, dim3(nt), dim3(512), 0, c.stream, t.data32.as<float>(),
+                       t.off16.as<int64_t>(), t.ldp.as<int>(), t.ldp_pre.as<int64_t>(), t.bsz.as<int>(),
+                       t.boff.as<int>(), tiles, L.naux, L.nbf, G, ng, out, ldo);
+  HIP_CHECK(hipGetLastError());
+}
+
 // Launch shapes of the FP32 and FP64 slab GEMMs (direct-to-LDS): waves
 // WM x WN, MFMA tiles per wave TM x TN, S pipeline stages.
 struct DlShape {

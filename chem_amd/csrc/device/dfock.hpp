@@ -39,6 +39,10 @@ void dbt_make_lowp(Ctx& c, DevBT& t, const Metric& m, bool f32, bool bf16, bool 
 // FP32 copy of L^-1 B (made slab by slab in FP64 from the untransformed B)
 // into out (L.total floats, slab layout); t is left unchanged.
 void tilde_to_f32(Ctx& c, DevBT& t, const Metric& m, float* out);
+void dbt_make_f32_from(Ctx& c, DevBT& t, const float* b32);
+int64_t dbt_padded_cols(const DevBT& t);
+void slab_gemm_f32_f64sum(Ctx& c, DevBT& t, const float* C, int ldc, int ng, float* G, float* out, int ldo,
+                          int depth);
 void dfk_mo_bf16(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K, int occ_nbatches);
 // tr(K P) of the FP64 exchange on the FP64 copy of L^-1 B at the occupied C
 // (device, row-major nbf x ldc, first nocc columns; P = C C^T), computed as

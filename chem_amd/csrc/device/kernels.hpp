@@ -28,6 +28,14 @@ template <int S>
 __global__ void k_group_apply_f32_dl(const float* P, int npt, const float* bT, float* out, int n, int ldn, int R,
                                      int naux, const int* pcol, const int* pwidth, const int4* tiles);
 __global__ void k_to_f32x2_rows(const double* in, int n, int ldn, float* hi, float* lo);
+__global__ void k_pad_f32_rows(const float* src, int N, const int* ncol, const int* bsz, int naux, const int* ldp,
+                               const int64_t* off, float* out);
+__global__ void k_gather_ct_f32f(const float* C, int ldc, int ng, const int* ncol, const int* ldp,
+                                 const int64_t* ldp_pre, const int* colmap, const int* colmap_off, float* g);
+template <int S>
+__global__ void k_slab_gemm_f32_dl_f64(const float* data, const int64_t* off, const int* ldp, const int64_t* ldp_pre,
+                                       const int* bsz, const int* boff, const Tile* tiles, int naux, int nbf,
+                                       const float* g, int ng, float* out, int ldo);
 __global__ void k_slabseg_f32(const float* A, int64_t a_stride, const float* B, const int64_t* seg_off,
                               const int64_t* seg_len, const int* chunk_seg, int nchunk, int n,
                               const int2* tiles, int ntile, double* partial);

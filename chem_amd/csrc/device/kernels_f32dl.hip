@@ -7,6 +7,8 @@
 //   k_transpose_pad_f32   b[X][nu][i] -> b^T[X][i][nu], rows padded to ldn
 //   k_group_apply_f32_dl  M3: P_v,g b_g[X] for every (point, X), P_v as an FP32 pair
 //   k_to_f32x2_rows       FP64 P_v -> FP32 hi/lo rows padded to ldn
+//   k_pad_f32_rows, k_gather_ct_f32f, k_slab_gemm_f32_dl_f64
+//                         M2: b = B~ L on the padded FP32 copy, FP64 sums
 //
 // Operands use the padded slab layout of the BF16 copy: slab N has naux*bsz[N]
 // rows of ldp[N] (multiple of 32) elements at element offset off[N]; the
@@ -315,7 +317,7 @@ __global__ __launch_bounds__(512, 2) void k_group_apply_f32_dl(const float* __re
         f32_step_swz<TM, TN>(st, st + BM * 4, wm0, wn0, lane, acc[p]);
         flush_f64<TM, TN>(acc[1 - p], acc64);   // step s - 1 (zero before the first step)
       });
-  flush_f64<TM, TN>(acc[(2 * nh - 1) & 1], acc64);   // the last step
+  flush_f64<TM, TN>(acc[1], acc64);   // the last step: 2 nh - 1 is odd
   const int fc = lane & 15, fr4 = 4 * (lane >> 4);
 #pragma unroll
   for (int j = 0; j < TN; ++j) {
@@ -332,6 +334,130 @@ __global__ __launch_bounds__(512, 2) void k_group_apply_f32_dl(const float* __re
       }
   }
 }
+
+// One slab of FP32 values in the slab layout (rows of ncol[N]) -> rows padded
+// to ldp[N] with zeros, at element offset off[N].
+__global__ void k_pad_f32_rows(const float* __restrict__ src, int N, const int* __restrict__ ncol,
+                               const int* __restrict__ bsz, int naux, const int* __restrict__ ldp,
+                               const int64_t* __restrict__ off, float* __restrict__ out)
+{
+  const int K = ncol[N], ld = ldp[N];
+  const int64_t n = (int64_t)naux * bsz[N] * ld;
+  float* o = out + off[N];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / ld;
+    const int c = (int)(i - r * ld);
+    o[i] = c < K ? src[r * K + c] : 0.f;
+  }
+}
+
+// Per-slab gathered, transposed FP32 coefficients from an FP32 n x ldc matrix:
+// g[ldp_pre[N] ng + i ldp[N] + c] = C(colmap_N[c], i), zero for c >= ncol[N].
+__global__ void k_gather_ct_f32f(const float* __restrict__ C, int ldc, int ng, const int* __restrict__ ncol,
+                                 const int* __restrict__ ldp, const int64_t* __restrict__ ldp_pre,
+                                 const int* __restrict__ colmap, const int* __restrict__ colmap_off,
+                                 float* __restrict__ g)
+{
+  const int N = blockIdx.y;
+  const int ld = ldp[N];
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)ng * ld) return;
+  const int i = (int)(idx / ld), c = (int)(idx - (int64_t)(idx / ld) * ld);
+  g[ldp_pre[N] * ng + idx] = c < ncol[N] ? C[(int64_t)colmap[colmap_off[N] + c] * ldc + i] : 0.f;
+}
+
+// The FP32 slab GEMM with FP64 sums (MP2 M2, b = B~ L): W rows (X, boff[N] +
+// nu) for the rows X*bsz + nu of slab N, columns [0, ldo), zero in [ng, ldo).
+// 128 x 128 tiles, eight waves of 64 x 32; each 16-element step's FP32 MFMA
+// sums (16 products) are added into FP64 running sums, the step s - 1's set
+// while step s accumulates into the other (double-buffered), in step order.
+template <int S>
+__global__ __launch_bounds__(512, 2) void k_slab_gemm_f32_dl_f64(
+    const float* __restrict__ data, const int64_t* __restrict__ off, const int* __restrict__ ldp,
+    const int64_t* __restrict__ ldp_pre, const int* __restrict__ bsz, const int* __restrict__ boff,
+    const Tile* __restrict__ tiles, int naux, int nbf, const float* __restrict__ g, int ng,
+    float* __restrict__ out, int ldo)
+{
+  constexpr int BM = 128, BN = 128, NW = 8, WN = 4, TM = 4, TN = 2;
+  constexpr int QA = BM / 16, QB = BN / 16;
+  static_assert(QA == NW && QB == NW, "one A and one B row block per wave and stage");
+  static_assert(S % 2 == 0 && S >= 2 && S <= 4, "pipeline depth (even: accumulator set = stage parity)");
+  constexpr int STAGE = (BM + BN) * 4;
+  __shared__ uint4 lds0[STAGE], lds1[STAGE], lds2[STAGE], lds3[STAGE];
+  const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  if (t.slab < 0) return;
+  const int N = t.slab;
+  const int bn = bsz[N];
+  const int ld4 = ldp[N] >> 2;
+  const int64_t rows = (int64_t)naux * bn;
+  const uint4* A = reinterpret_cast<const uint4*>(data + off[N]);
+  const uint4* G = reinterpret_cast<const uint4*>(g + ldp_pre[N] * ng);
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm0 = (wave / WN) * (16 * TM), wn0 = (wave % WN) * (16 * TN);
+  f32v4 acc[2][TM][TN];
+  double acc64[TM][TN][4];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      acc[0][i][j] = f32v4{0.f, 0.f, 0.f, 0.f};
+      acc[1][i][j] = f32v4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc64[i][j][e] = 0.0;
+    }
+  const int nsteps = ld4 >> 2;
+  // per wave: A row 16 wave + rr and B row 16 wave + rr (clamped), offsets in
+  // 16-byte units, fixed over the contraction
+  const int rr = lane >> 2, pos = lane & 3;
+  const int r = 16 * wave + rr;
+  const int64_t ar = (int64_t)t.m0 + r < rows - 1 ? (int64_t)t.m0 + r : rows - 1;
+  const int br = t.n0 + r < ng - 1 ? t.n0 + r : ng - 1;
+  const int64_t offA = ar * ld4 + swz(r, pos), offB = (int64_t)br * ld4 + swz(r, pos);
+  dl_pipeline<S, 2>(
+      nsteps,
+      [&](int s, auto k) {
+        uint4* st = pick_stage<S>(k, lds0, lds1, lds2, lds3);
+        lds_load16(A + (offA + s * 4), st + wave * 64);
+        lds_load16(G + (offB + s * 4), st + BM * 4 + wave * 64);
+      },
+      [&](int, auto k) {
+        constexpr int p = decltype(k)::value & 1;
+        const uint4* st = pick_stage<S>(k, lds0, lds1, lds2, lds3);
+        f32_step_swz<TM, TN>(st, st + BM * 4, wm0, wn0, lane, acc[p]);
+        flush_f64<TM, TN>(acc[1 - p], acc64);   // step s - 1 (zero before the first step)
+      });
+  // the last step's set (constant indices: a run-time index would put the
+  // accumulators in scratch memory)
+  if (nsteps > 0) {
+    if ((nsteps - 1) & 1)
+      flush_f64<TM, TN>(acc[1], acc64);
+    else
+      flush_f64<TM, TN>(acc[0], acc64);
+  }
+  const int fc = lane & 15, fr4 = 4 * (lane >> 4);
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t rw = (int64_t)t.m0 + wm0 + 16 * i + fr4 + e;
+      if (rw >= rows) continue;
+      const int64_t x = rw / bn;
+      const int nu = (int)(rw - x * bn);
+      const int64_t orow = (x * nbf + boff[N] + nu) * (int64_t)ldo;
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int c = t.n0 + wn0 + 16 * j + fc;
+        if (c < ldo) out[orow + c] = c < ng ? (float)acc64[i][j][e] : 0.f;
+      }
+    }
+}
+
+template __global__ void k_slab_gemm_f32_dl_f64<2>(const float*, const int64_t*, const int*, const int64_t*,
+                                                   const int*, const int*, const Tile*, int, int, const float*, int,
+                                                   float*, int);
+template __global__ void k_slab_gemm_f32_dl_f64<4>(const float*, const int64_t*, const int*, const int64_t*,
+                                                   const int*, const int*, const Tile*, int, int, const float*, int,
+                                                   float*, int);
 
 // FP64 n x n -> FP32 hi/lo pair, rows padded with zeros to ldn.
 __global__ void k_to_f32x2_rows(const double* __restrict__ in, int n, int ldn, float* __restrict__ hi,

@@ -490,7 +490,8 @@ __global__ void k_cols_to_f32(const double* __restrict__ in, int rows, int w, fl
 // with FP64 accumulation, on B~ = L^-1 B (V = L L^T; FP32 copy made slab by
 // slab in FP64 from B):
 //   M2  b[X][nu][i]     = sum_mu B~(X|mu nu) L_all(mu, i)  one slab GEMM over the
-//                                                         concatenated columns of all points
+//                                                         concatenated columns of all points,
+//                                                         direct-to-LDS on the padded FP32 copy
 //   M3  b2_g[X]         = P_v,g b_g[X]                     grouped over (point, X), P_v as an FP32 hi/lo pair,
 //                                                         direct-to-LDS on b transposed per X
 //   M4  b3_g(X|M N)     = sum_i L_g(M,i) b2_g(X,i,N)        grouped over points, B's pattern
@@ -501,8 +502,8 @@ __global__ void k_cols_to_f32(const double* __restrict__ in, int rows, int w, fl
 // rounding of FP32 products alone moves e_g by ~1e-5 relative; in the
 // orthonormal auxiliary basis e_g is a well-conditioned function of Z'.
 // Every FP32 MFMA chain is cut into FP64 running sums after 16 products
-// (64 in M5).  The column tile of M2 is picked by timing both candidates on
-// the live data.
+// (64 in M5).  The pipeline depths of M2 and M3 are picked by timing both
+// candidates on the live data.
 double mp2_sos_batched(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in, std::vector<double>& terms,
                        double* t_z, double* t_prep, double* Z_host)
 {
@@ -547,6 +548,12 @@ double mp2_sos_batched(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in,
   }
   HIP_CHECK(hipGetLastError());
   if (L.total) tilde_to_f32(c, bt, met, B32.as<float>());
+  // the padded FP32 copy of B~ for M2's direct-to-LDS slab GEMM (made here
+  // unless the tensor holds one; released at the end if made here)
+  const bool had32 = bt.data32.p != nullptr && bt.lowp_tilde;
+  const bool had_lowp = bt.lowp_tilde;
+  if (!had32) dbt_make_f32_from(c, bt, B32.as<float>());
+  DevBuf G2((size_t)std::max<int64_t>(dbt_padded_cols(bt) * R, 1) * sizeof(float));
   const size_t bsize = (size_t)naux * n * R;
   // b transposed per aux function, rows of ldn: the second operand of M3
   DevBuf bb(bsize * sizeof(float)), b2(bsize * sizeof(float)), bT((size_t)naux * R * ldn * sizeof(float)),
@@ -596,21 +603,22 @@ double mp2_sos_batched(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in,
   DevBuf d_t3 = upload_vec(t3, c.stream), d_t4 = upload_vec(t4, c.stream), d_t5 = upload_vec(t5, c.stream),
          d_soff = upload_vec(soff, c.stream), d_slen = upload_vec(slen, c.stream), d_cs = upload_vec(cs, c.stream);
   DevBuf partial((size_t)npt * nch * naux * naux * sizeof(double)), Z((size_t)npt * naux * naux * sizeof(double));
-  // autotune the M2 column tile on the live operands (also creates its tile table)
-  int bn = 128;
+  // M2 on the padded FP32 copy; its pipeline depth is timed on the live
+  // operands (the first call also makes the tile table)
+  int m2_depth = 2;
   {
     float best = 1e30f;
-    for (int cand : {64, 128}) {
-      slab_gemm_f32(c, bt, B32.as<float>(), Lall.as<float>(), R, R, bb.as<float>(), R, cand, true);
+    for (int depth : {2, 4}) {
+      slab_gemm_f32_f64sum(c, bt, Lall.as<float>(), R, R, G2.as<float>(), bb.as<float>(), R, depth);
       HIP_CHECK(hipEventRecord(c.ev0, c.stream));
-      slab_gemm_f32(c, bt, B32.as<float>(), Lall.as<float>(), R, R, bb.as<float>(), R, cand, true);
+      slab_gemm_f32_f64sum(c, bt, Lall.as<float>(), R, R, G2.as<float>(), bb.as<float>(), R, depth);
       HIP_CHECK(hipEventRecord(c.ev1, c.stream));
       HIP_CHECK(hipEventSynchronize(c.ev1));
       float ms = 0.f;
       HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
       if (ms < best) {
         best = ms;
-        bn = cand;
+        m2_depth = depth;
       }
     }
   }
@@ -652,7 +660,7 @@ double mp2_sos_batched(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in,
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-  slab_gemm_f32(c, bt, B32.as<float>(), Lall.as<float>(), R, R, bb.as<float>(), R, bn, true);
+  slab_gemm_f32_f64sum(c, bt, Lall.as<float>(), R, R, G2.as<float>(), bb.as<float>(), R, m2_depth);
   hipLaunchKernelGGL(k_transpose_pad_f32, tgrid, dim3(256), 0, c.stream, bb.as<float>(), n, R, ldn, bT.as<float>());
   m3(m3_depth);
   hipLaunchKernelGGL((k_slab_out_gemm<float, SG_BM, 128>), dim3((unsigned)t4.size()), dim3(256), 0, c.stream,
@@ -679,6 +687,11 @@ double mp2_sos_batched(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in,
   if (t_z) *t_z = ms * 1e-3;
   (void)hipGraphExecDestroy(exec);
   (void)hipGraphDestroy(graph);
+  if (!had32) {   // the tensor as the caller left it
+    bt.data32.release();
+    bt.lowp_tilde = had_lowp;
+    bt.f32_plan.clear();
+  }
   // e_g = sum Z'_g(X,Y)^2 in FP64; Z_g = L Z'_g L^T only when asked for
   DevBuf part(256 * sizeof(double)), Lf, Zg64;
   if (Z_host) {
