@@ -45,6 +45,9 @@ CONFIGS = {
     "c3": dict(nwater=32, seed=1234, nlap=8),
     # serving stream: Poisson arrivals at 4/s, clusters of U{4..64} waters
     "c4": dict(requests=200, rate=4.0, nmin=4, nmax=64, seed=2024, budget=200e9, thresh=1e-6),
+    # a short stream of the c4 kind run after the default c2 line, so that the
+    # default run also reports served latencies (not the c4 workload itself)
+    "c4_sample": dict(requests=24, rate=4.0, nmin=4, nmax=16, seed=2024, budget=200e9, thresh=1e-6),
 }
 MFMA_F64_PEAK = 78.6e12
 MFMA_F32_PEAK = 157.3e12
@@ -167,16 +170,20 @@ def cpu_model():
     return "unknown"
 
 
-def serve_stream(args, log_progress=True):
+def serve_stream(args, log_progress=True, cfg=None, ncheck=None):
     """The DF-HF request stream: seeded Poisson arrivals, water clusters of
     uniformly random size, the chosen form (BF16x3 escalating to FP32, FP64
     final energy) per request, admission against the HBM budget.  Returns the
     latency percentiles, throughput and the energy check against the plain
-    FP64 path on a sample of the requests."""
+    FP64 path on a sample of the requests.  cfg: the stream (default c4)."""
     from chem_amd import desc
     from chem_amd.serve import Server
-    cfg = CONFIGS["c4"]
-    nreq = args.requests or cfg["requests"]
+    if cfg is None:
+        cfg = CONFIGS["c4"]
+        nreq = args.requests or cfg["requests"]
+    else:
+        nreq = cfg["requests"]
+    ncheck = args.serve_check if ncheck is None else ncheck
     rng = np.random.default_rng(cfg["seed"])
     arrivals = np.cumsum(rng.exponential(1.0 / cfg["rate"], nreq))
     sizes = rng.integers(cfg["nmin"], cfg["nmax"] + 1, nreq)
@@ -216,8 +223,8 @@ def serve_stream(args, log_progress=True):
     span = max(r["t_done"] for r in ok) - min(r["t_submit"] for r in ok)
     fock_builds = sum(r["iterations"] + 1 for r in ok)
     # energy check: plain FP64 path (kmode dfmo, the reference formulas) on a sample
-    sample = [i for i in sorted(set(int(i) for i in np.linspace(0, nreq - 1, min(args.serve_check, nreq)).round()))
-              if results[i] is not None]
+    sample = [i for i in sorted(set(int(i) for i in np.linspace(0, nreq - 1, min(ncheck, nreq)).round()))
+              if results[i] is not None] if ncheck > 0 else []
     # (both paths also at 1e-8: at the served threshold two FP64 SCFs that take
     # different iteration paths already differ by up to ~1e-6 Eh, so the tight
     # pair isolates the error of the chosen form's exchange)
@@ -347,6 +354,8 @@ def main():
     ap.add_argument("--stall", type=int, default=3,
                     help="auto form: iterations without progress before BF16x3 gives way to FP32 (0: off)")
     ap.add_argument("--requests", type=int, default=0, help="c4: number of requests (0: 200)")
+    ap.add_argument("--no-serve-sample", action="store_true",
+                    help="c2: skip the short served stream (24 requests of 4-16 waters) after the DF-K line")
     ap.add_argument("--slots", type=int, default=4, help="c4: request workers")
     ap.add_argument("--no-batch", action="store_true",
                     help="c4: no cross-request batching of the exchange builds (each on its worker's stream)")
@@ -677,6 +686,26 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if not args.no_serve_sample and args.config == "c2":
+        # the c2 tensors hold most of HBM: release them, then a short served
+        # stream (reported beside the DF-K value, not as it)
+        for obj in (Cd, Kd, dB, met):
+            obj.release()
+        ctx.close()
+        sv = serve_stream(args, log_progress=False, cfg=CONFIGS["c4_sample"], ncheck=1)
+        sc = CONFIGS["c4_sample"]
+        out["serving_sample"] = {
+            "workload": "request stream: %d requests, Poisson %.0f/s, U{%d..%d} waters, conv %.0e, %d workers "
+                        "(a short stream of the c4 kind; c4 itself: --config c4)" % (
+                            sc["requests"], sc["rate"], sc["nmin"], sc["nmax"], sc["thresh"], sv["slots"]),
+            "p50_s": sv["p50_s"], "p99_s": sv["p99_s"], "mean_s": sv["mean_s"],
+            "requests_per_s": sv["requests_per_s"], "scf_iterations_per_s": sv["scf_iterations_per_s"],
+            "all_converged": sv["all_converged"], "failed": sv["failed"],
+            "max_abs_dE_vs_fp64": sv["max_abs_dE_vs_fp64"],
+            "max_abs_dE_vs_fp64_both_1e8": sv["max_abs_dE_vs_fp64_both_1e8"],
+            "mean_phase_s": sv["mean_phase_s"]}
+        log("serving sample: p50 %.2f s p99 %.2f s, %.2f requests/s" % (sv["p50_s"], sv["p99_s"],
+                                                                        sv["requests_per_s"]))
     print(json.dumps(out), flush=True)
 
 
