@@ -493,8 +493,12 @@ __device__ __forceinline__ void bf16x3_step_pipe_io(const uint4* Ah, const uint4
       __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);       // DS reads: A row i + 1
     }
     if constexpr (u1 > u0) {
-      static_for<u1 - u0>([&](auto uu) { issue(std::integral_constant<int, u0 + decltype(uu)::value>{}); });
-      __builtin_amdgcn_sched_group_barrier(0x020, u1 - u0, 0); // global -> LDS loads
+      // global -> LDS loads, one group each: they keep their program order
+      // (the caller's closing vmcnt count assumes which loads were issued last)
+      static_for<u1 - u0>([&](auto uu) {
+        issue(std::integral_constant<int, u0 + decltype(uu)::value>{});
+        __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+      });
     }
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
