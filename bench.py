@@ -298,17 +298,22 @@ def mp2_batched_bench(args):
 
 
 def kernels_only(args):
-    """The chosen form's exchange builds alone, for profiling: B, metric and
-    the copies of L^-1 B of the 64-water cluster, then `steps` BF16x3, FP32 and
-    FP64-copy builds each at orthonormalised random occupied coefficients (the
-    kernels' work does not depend on the orbitals).  Everything in this mode
-    runs on the library's own kernels (no vendor BLAS call)."""
+    """The chosen form's exchange builds alone, for profiling: B (its GPU build
+    timed), metric and the copies of L^-1 B of the 64-water cluster, then
+    `steps` BF16x3, FP32 and FP64-copy builds each at orthonormalised random
+    occupied coefficients (the kernels' work does not depend on the orbitals).
+    Everything in this mode runs on the library's own kernels (no vendor BLAS
+    call)."""
     from chem_amd import desc, ints
     from chem_amd.engine import Context, DeviceB, Metric
     cfg = CONFIGS[args.config if args.config in ("c1", "c2") else "c2"]
     mol = desc.Molecule(desc.water_cluster(cfg["nwater"], cfg["seed"]), reorder=not args.no_reorder)
     ctx = Context(0)
+    t = time.perf_counter()
     dB = DeviceB.build_on_gpu(ctx, mol, 1e-10)
+    ctx.sync()
+    out = {"b_build_gpu_s": time.perf_counter() - t}
+    log("GPU build of B %.3f s" % out["b_build_gpu_s"])
     met = Metric(ctx, ints.int2c2e(mol))
     dB.make_lowp(met, f64=True)
     S, _, _ = ints.int1e(mol)
@@ -317,7 +322,6 @@ def kernels_only(args):
     w, U = np.linalg.eigh(C.T @ S @ C)
     C = np.ascontiguousarray(C @ U / np.sqrt(w))
     steps = args.steps or 3
-    out = {}
     for name, fn in (("bf16x3", dB.dfk_mo_bf16), ("f32", dB.dfk_mo_f32), ("fp64_copy", dB.dfk_mo_tilde64)):
         fn(C)
         ctx.sync()
