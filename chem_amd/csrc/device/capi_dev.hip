@@ -7,6 +7,7 @@
 #include "../../../include/mchip.h"
 #include "../capi_common.hpp"
 #include "dfock.hpp"
+#include "ints12_dev.hpp"
 #include "ints_dev.hpp"
 
 using namespace mchip;
@@ -242,6 +243,29 @@ extern "C" int mchip_dbt_build_gpu(mchip_ctx c, mchip_basis ao, mchip_basis aux,
   d->device = c->c.device;
   build_B_device(c->c, basis_of(ao), basis_of(aux), bsh, xsh, msk, filter_eps, d->t);
   *out = d.release();
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_int1e_gpu(mchip_ctx c, mchip_basis b, int natom, const double* Z, const double* xyz,
+                               double* S, double* T, double* V)
+{
+  MCHIP_TRY
+  need_device(c);
+  if (natom < 0) throw std::runtime_error("mchip_int1e_gpu: negative charge count");
+  std::vector<Charge> ch(natom);
+  for (int i = 0; i < natom; ++i) {
+    ch[i].Z = Z[i];
+    for (int d = 0; d < 3; ++d) ch[i].xyz[d] = xyz[3 * i + d];
+  }
+  ints_1e_device(c->c, basis_of(b), ch, S, T, V);
+  MCHIP_CATCH
+}
+
+extern "C" int mchip_int2c2e_gpu(mchip_ctx c, mchip_basis aux, double* V)
+{
+  MCHIP_TRY
+  need_device(c);
+  ints_2c2e_device(c->c, basis_of(aux), V);
   MCHIP_CATCH
 }
 

@@ -365,6 +365,10 @@ __global__ __launch_bounds__(64) void k_int3c2e_cls(const I3cArgs A)
 }
 
 
+// Normalisations of the real solid harmonics (host table, checked against
+// the integer shapes).
+void solid_harmonic_norms(double (&inv)[LMAX + 1][2 * LMAX + 1]);
+
 // One launch of class (LA, lb, lc), instantiated per LA in its own file.
 void launch_int3c2e_la0(int lb, int lc, dim3 grid, hipStream_t st, const I3cArgs& a);
 void launch_int3c2e_la1(int lb, int lc, dim3 grid, hipStream_t st, const I3cArgs& a);

@@ -71,6 +71,31 @@ __global__ void k_pair_scatter_slabs(const double* __restrict__ tmp, const int64
   }
 }
 
+// sinv[l][m] with c_sph[l][m][k] = shp(l, m, k) * sinv[l][m]: read off a
+// monomial whose integer weight is a power of two (the division is exact),
+// and the integer shapes checked against the host table.
+void solid_harmonic_norms(double (&inv)[LMAX + 1][2 * LMAX + 1])
+{
+  const SphTable& S = sph_table();
+  for (int l = 0; l <= LMAX; ++l)
+    for (int m = 0; m < 2 * LMAX + 1; ++m) inv[l][m] = 0.0;
+  for (int l = 0; l <= LMAX; ++l)
+    for (int m = 0; m < 2 * l + 1; ++m) {
+      bool found = false;
+      for (int k = 0; k < i3c::nc(l) && !found; ++k) {
+        const int w = i3c::shp(l, m, k);
+        if (w == 1 || w == -1 || w == 2 || w == -2 || w == 4 || w == -4) {
+          inv[l][m] = S.c[l][m][k] / w;
+          found = true;
+        }
+      }
+      if (!found) throw std::runtime_error("GPU integrals: solid harmonic normalisation");
+      for (int k = 0; k < i3c::nc(l); ++k)
+        if (std::fabs(i3c::shp(l, m, k) * inv[l][m] - S.c[l][m][k]) > 1e-15 * std::fabs(inv[l][m]))
+          throw std::runtime_error("GPU integrals: solid harmonic shapes disagree with the host table");
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host driver
 // ---------------------------------------------------------------------------
@@ -170,26 +195,7 @@ void build_B_device(Ctx& c, const Basis& ao, const Basis& aux, const std::vector
   }
   shl0[LMAX + 1] = (int)shl.size();
   I3cArgs ia{};
-  {
-    // c_sph[l][m][k] = shp(l, m, k) * sinv[l][m]; read sinv off a monomial
-    // whose integer weight is a power of two (the division is exact)
-    const SphTable& S = sph_table();
-    for (int l = 0; l <= LMAX; ++l)
-      for (int m = 0; m < 2 * l + 1; ++m) {
-        bool found = false;
-        for (int k = 0; k < i3c::nc(l) && !found; ++k) {
-          const int w = i3c::shp(l, m, k);
-          if (w == 1 || w == -1 || w == 2 || w == -2 || w == 4 || w == -4) {
-            ia.sinv[l][m] = S.c[l][m][k] / w;
-            found = true;
-          }
-        }
-        if (!found) throw std::runtime_error("GPU integrals: solid harmonic normalisation");
-        for (int k = 0; k < i3c::nc(l); ++k)   // the shapes agree with the host table
-          if (std::fabs(i3c::shp(l, m, k) * ia.sinv[l][m] - S.c[l][m][k]) > 1e-15 * std::fabs(ia.sinv[l][m]))
-            throw std::runtime_error("GPU integrals: solid harmonic shapes disagree with the host table");
-      }
-  }
+  solid_harmonic_norms(ia.sinv);
   const auto& btab = boys_table();
   DevBuf d_aonp = upload_vec(aonp, c.stream), d_aopo = upload_vec(aopo, c.stream),
          d_aoO = upload_vec(aoO, c.stream), d_aoex = upload_vec(aoex, c.stream),

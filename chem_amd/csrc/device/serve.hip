@@ -33,6 +33,7 @@
 #include "../ints/btensor.hpp"
 #include "../ints/ints_host.hpp"
 #include "dfock.hpp"
+#include "ints12_dev.hpp"
 #include "ints_dev.hpp"
 #include "scf.hpp"
 #include "serve.hpp"
@@ -395,7 +396,8 @@ void Server::worker(int slot)
 
 void Server::run(Ctx& c, SrvJob& j, bool& admitted)
 {
-  // ---- host side: one-electron integrals, metric, Schwarz mask, estimate
+  // ---- before admission: one-electron integrals and the metric (GPU, a few
+  // hundred MB at most, inside the admission margin), Schwarz mask, estimate
   const double th0 = now_s();
   const Basis& ao = j.ao;
   const Basis& aux = j.aux;
@@ -407,9 +409,9 @@ void Server::run(Ctx& c, SrvJob& j, bool& admitted)
   static thread_local PinnedArena pS, pT, pM;
   HostArray S(pS, (size_t)n * n), T(pT, (size_t)n * n), M(pM, (size_t)naux * naux);
   std::vector<double> V((size_t)n * n);
-  ints_1e(ao, j.charges, S.data(), T.data(), V.data());
+  ints_1e_device(c, ao, j.charges, S.data(), T.data(), V.data());
   for (size_t i = 0; i < S.size(); ++i) T[i] += V[i];   // H = T + V
-  ints_2c2e(aux, M.data());
+  ints_2c2e_device(c, aux, M.data());
   const int nsa = (int)ao.shells.size(), nsx = (int)aux.shells.size();
   std::vector<double> Qao((size_t)nsa * nsa), Qx(nsx);
   schwarz_ao(ao, Qao.data());

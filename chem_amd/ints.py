@@ -31,6 +31,24 @@ def int2c2e(mol):
     return M
 
 
+def int1e_gpu(ctx, mol):
+    """S, T, V of int1e computed on the context's GPU."""
+    n = mol.nbf
+    S, T, V = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n))
+    Z, xyz = mol.charges()
+    _lib.check(_lib.lib().mchip_int1e_gpu(ctx.h, mol.basis_handle.h, len(mol.atoms), _lib.dptr(Z),
+                                          _lib.dptr(xyz), _lib.dptr(S), _lib.dptr(T), _lib.dptr(V)))
+    return S, T, V
+
+
+def int2c2e_gpu(ctx, mol):
+    """The Coulomb metric of int2c2e computed on the context's GPU."""
+    n = mol.naux
+    M = np.zeros((n, n))
+    _lib.check(_lib.lib().mchip_int2c2e_gpu(ctx.h, mol.dfbasis_handle.h, _lib.dptr(M)))
+    return M
+
+
 def int4c2e(mol):
     """Exact four-centre ERIs (ij|kl) over the AO basis, nbf^4 row-major
     (host; the SAD guess's atomic SCFs use them)."""

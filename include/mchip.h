@@ -200,6 +200,12 @@ int mchip_dbt_destroy(mchip_dbt t);
 int mchip_dbt_build_gpu(mchip_ctx c, mchip_basis ao, mchip_basis aux, int nbb,
                         const int* ao_blk_shells, int nxb, const int* aux_blk_shells,
                         const uint8_t* schwarz_mask, double filter_eps, mchip_dbt* out);
+/* The same S, T, V and (X|Y) computed on the context's GPU (AO shells up to
+ * d, uncontracted auxiliary shells up to f), into host arrays; they replace
+ * the same aofactory calls (aofactory.cpp:380-430) on the device path. */
+int mchip_int1e_gpu(mchip_ctx c, mchip_basis b, int natom, const double* Z, const double* xyz,
+                    double* S, double* T, double* V);
+int mchip_int2c2e_gpu(mchip_ctx c, mchip_basis aux, double* V);
 /* dims[0..3] = naux, nbf, nxb, nbb; *nelem = stored elements. */
 int mchip_dbt_info(mchip_dbt t, int* dims, int64_t* nelem);
 int mchip_dbt_mask(mchip_dbt t, uint8_t* mask);

@@ -28,3 +28,25 @@ def test_gpu_B_matches_host(ctx, nwater, dfbasis):
     err = np.abs(got - ref).max() / np.abs(ref).max()
     print("relerr %.3e" % err)
     assert err < 1e-12
+
+
+@pytest.mark.parametrize("nwater,dfbasis", [(1, "cc-pvdz-jkfit"), (3, "cc-pvdz-jkfit"), (2, "cc-pvdz-ri")])
+def test_gpu_1e_and_metric_match_host(ctx, nwater, dfbasis):
+    """S, T, V and the Coulomb metric from the GPU class kernels against the
+    host engine (same formulas; FMA contraction and the top-degree-only
+    auxiliary Hermite expansion differ at rounding level)."""
+    mol = desc.Molecule(desc.water_cluster(nwater, 21), dfbasis=dfbasis)
+    print("case: GPU 1e/2c2e n=%d %s" % (nwater, dfbasis), flush=True)
+    S, T, V = ints.int1e(mol)
+    Sg, Tg, Vg = ints.int1e_gpu(ctx, mol)
+    for name, a, b in (("S", S, Sg), ("T", T, Tg), ("V", V, Vg)):
+        err = np.abs(a - b).max() / np.abs(a).max()
+        print("%s relerr %.3e" % (name, err))
+        assert err < 1e-12
+        assert np.array_equal(b, b.T)
+    M = ints.int2c2e(mol)
+    Mg = ints.int2c2e_gpu(ctx, mol)
+    err = np.abs(M - Mg).max() / np.abs(M).max()
+    print("M relerr %.3e" % err)
+    assert err < 1e-12
+    assert np.array_equal(Mg, Mg.T)
