@@ -13,6 +13,8 @@
 #include <cstring>
 #include <type_traits>
 
+#include <rocsolver/rocsolver.h>
+
 #include "dfock.hpp"
 #include "ints_dev.hpp"
 #include "kernels.hpp"
@@ -1879,28 +1881,80 @@ int dfk_f64_num_shapes(int* nsyrk)
 namespace mchip {
 
 // Exchange from a density (DFAO_K, jkdf.cpp:510-648, and DFMEM_K,
-// jkdf.cpp:719-859): K = -sum_{X,Y} B(X|m l) V^-1(X,Y) P(l,s) B(Y|s n).  The
-// density is factorised on the GPU, P = L L^T by pivoted Cholesky, and K is the
-// occupied-factor exchange with C := L, which equals the fitted-coefficient
-// contraction term by term.  form: 0 FP64, 1 FP32, 2 BF16x3 (the low-precision
-// forms need the transformed B and its copy).
+// jkdf.cpp:719-859): K = -sum_{X,Y} B(X|m l) V^-1(X,Y) P(l,s) B(Y|s n).  K is
+// linear in P, so any factorisation P = C C^T turns it into the
+// occupied-factor exchange with C, term by term.  A positive semidefinite P
+// is factorised on the GPU by pivoted Cholesky; an indefinite one (a
+// difference density, say: the pivoted Cholesky stops at a negative pivot)
+// by its eigenvectors, P = C+ C+^T - C- C-^T with C+- = V+- |w+-|^1/2, and
+// K = K(C+) - K(C-); eigenvalues below n eps max|w| are dropped.  form: 0
+// FP64, 1 FP32, 2 BF16x3 (the low-precision forms need the transformed B and
+// its copy).
 void dfk_ao(Ctx& c, DevBT& t, const Metric* m, const double* P_dev, double* K, int occ_nbatches, int form)
 {
   const int n = t.L.nbf;
-  DevBuf U((size_t)n * n * sizeof(double));
+  auto exchange = [&](const double* C, int r, double* Kout) {
+    switch (form) {
+      case 0: dfk_mo(c, t, t.transformed ? nullptr : m, C, r, r, Kout, occ_nbatches); break;
+      case 1: dfk_mo_f32(c, t, C, r, r, Kout, occ_nbatches); break;
+      case 2: dfk_mo_bf16(c, t, C, r, r, Kout, occ_nbatches); break;
+      default: throw std::runtime_error("dfk_ao: unknown precision form");
+    }
+  };
+  if (form < 0 || form > 2) throw std::runtime_error("dfk_ao: unknown precision form");
+  const size_t nn = (size_t)n * n;
+  DevBuf U(nn * sizeof(double));
   HIP_CHECK(hipMemcpyAsync(U.p, P_dev, U.bytes, hipMemcpyDeviceToDevice, c.stream));
   int rank = 0;
-  std::vector<double> hL = pivoted_cholesky_dev(c, U, n, 0.0, &rank);
-  DevBuf L((size_t)n * std::max(rank, 1) * sizeof(double));
-  if (rank > 0)
-    h2d(L.p, hL.data(), hL.size() * sizeof(double), c.stream);
-  switch (form) {
-    case 0: dfk_mo(c, t, t.transformed ? nullptr : m, L.as<double>(), rank, rank, K, occ_nbatches); break;
-    case 1: dfk_mo_f32(c, t, L.as<double>(), rank, rank, K, occ_nbatches); break;
-    case 2: dfk_mo_bf16(c, t, L.as<double>(), rank, rank, K, occ_nbatches); break;
-    default: throw std::runtime_error("dfk_ao: unknown precision form");
+  bool neg = false;
+  std::vector<double> hL = pivoted_cholesky_dev(c, U, n, 0.0, &rank, &neg);
+  if (!neg) {
+    DevBuf L((size_t)n * std::max(rank, 1) * sizeof(double));
+    if (rank > 0) h2d(L.p, hL.data(), hL.size() * sizeof(double), c.stream);
+    exchange(L.as<double>(), rank, K);
+    HIP_CHECK(hipStreamSynchronize(c.stream));   // L and the host factor die here
+    return;
   }
-  HIP_CHECK(hipStreamSynchronize(c.stream));   // L and the host factor die here
+  // indefinite: eigenvectors of P (column-major view of the symmetric
+  // row-major P is P itself)
+  DevBuf W(nn * sizeof(double)), w((size_t)n * sizeof(double)), ework((size_t)n * sizeof(double));
+  HIP_CHECK(hipMemcpyAsync(W.p, P_dev, W.bytes, hipMemcpyDeviceToDevice, c.stream));
+  RB_CHECK(rocsolver_dsyevd(c.blas, rocblas_evect_original, rocblas_fill_upper, n, W.as<double>(), n,
+                            w.as<double>(), ework.as<double>(), c.info.as<int>()));
+  std::vector<double> hw(n), hV(nn);
+  int info = 0;
+  d2h(&info, c.info.p, sizeof(int), c.stream);
+  if (info != 0) throw std::runtime_error("dfk_ao: eigendecomposition of the density did not converge");
+  d2h(hw.data(), w.p, hw.size() * sizeof(double), c.stream);
+  d2h(hV.data(), W.p, hV.size() * sizeof(double), c.stream);
+  double wmax = 0.0;
+  for (double v : hw) wmax = std::max(wmax, std::fabs(v));
+  const double tol = n * 2.220446049250313e-16 * wmax;
+  std::vector<int> ip, in;
+  for (int k = 0; k < n; ++k) {
+    if (hw[k] > tol) ip.push_back(k);
+    else if (hw[k] < -tol) in.push_back(k);
+  }
+  auto factor = [&](const std::vector<int>& cols) {   // row-major n x r, column j = V_k |w_k|^1/2
+    const int r = (int)cols.size();
+    std::vector<double> C((size_t)n * std::max(r, 1), 0.0);
+    for (int j = 0; j < r; ++j) {
+      const double f = std::sqrt(std::fabs(hw[cols[j]]));
+      for (int i = 0; i < n; ++i) C[(size_t)i * r + j] = hV[(size_t)cols[j] * n + i] * f;
+    }
+    return C;
+  };
+  const std::vector<double> Cp = factor(ip), Cn = factor(in);
+  const int rp = (int)ip.size(), rn = (int)in.size();
+  DevBuf dCp(Cp.size() * sizeof(double)), dCn(Cn.size() * sizeof(double)), Kn(nn * sizeof(double));
+  h2d(dCp.p, Cp.data(), Cp.size() * sizeof(double), c.stream);
+  h2d(dCn.p, Cn.data(), Cn.size() * sizeof(double), c.stream);
+  exchange(dCp.as<double>(), rp, K);
+  exchange(dCn.as<double>(), rn, Kn.as<double>());
+  hipLaunchKernelGGL(k_axpby, dim3(gs_blocks((int64_t)nn)), dim3(256), 0, c.stream, K, Kn.as<double>(),
+                     (int64_t)nn, -1.0, 1.0);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(c.stream));
 }
 
 }  // namespace mchip

@@ -32,75 +32,163 @@
 namespace mchip {
 
 // ---- pivoted Cholesky kernels (row-major N x N symmetric U) -----------------
-// out[0] = index of the first maximum |U_kk| over k >= I, out[1] = U_kk
-__global__ void k_pcd_pivot(const double* __restrict__ U, int N, int I, double* __restrict__ out)
+// Right-looking blocked form (LAPACK dpstrf's scheme): a panel of PCD_NB
+// columns is factorised by one workgroup, then the trailing matrix takes the
+// panel's rank-PCD_NB update.  The updated diagonal d drives the pivot choice
+// (largest |d_k|, first index on ties, as piv_cd.cpp:125-483).  All state
+// (d, the permutation, rank, stop and error flags) lives on the device: the
+// host reads it once per panel.
+constexpr int PCD_NB = 32;
+
+struct PcdState {
+  int rank;      // columns factorised so far
+  int done;      // 1: the factorisation has stopped
+  int neg;       // 1: stopped at a negative pivot above the threshold
+  int pad;
+  double thresh; // N eps |largest diagonal|, set at the first pivot
+};
+
+__global__ void k_pcd_init(const double* __restrict__ U, int N, double* __restrict__ d, int* __restrict__ perm,
+                           PcdState* __restrict__ st)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < N) {
+    d[k] = U[(int64_t)k * N + k];
+    perm[k] = k;
+  }
+  if (k == 0) {
+    st->rank = 0;
+    st->done = N == 0 ? 1 : 0;
+    st->neg = 0;
+    st->pad = 0;
+    st->thresh = 0.0;
+  }
+}
+
+// Columns J0 .. J0 + PCD_NB - 1 (one workgroup of 1024 threads).  U holds the
+// Schur complement updated through column J0 - 1; a column's panel terms
+// are subtracted here, the trailing update of the panel comes after.
+__global__ __launch_bounds__(1024) void k_pcd_panel(double* __restrict__ U, int N, double* __restrict__ L,
+                                                    double* __restrict__ d, int* __restrict__ perm, int J0,
+                                                    PcdState* __restrict__ st)
 {
   __shared__ double bv[1024];
   __shared__ int bi[1024];
-  double best = -1.0;
-  int idx = N;
-  for (int k = I + threadIdx.x; k < N; k += blockDim.x) {
-    const double v = fabs(U[(int64_t)k * N + k]);
-    if (v > best || (v == best && k < idx)) {
-      best = v;
-      idx = k;
-    }
-  }
-  bv[threadIdx.x] = best;
-  bi[threadIdx.x] = idx;
+  __shared__ double s_thresh;
+  const int tid = threadIdx.x;
+  if (st->done) return;
+  if (tid == 0) s_thresh = st->thresh;
   __syncthreads();
-  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) {
-      const double v = bv[threadIdx.x + off];
-      const int j = bi[threadIdx.x + off];
-      if (v > bv[threadIdx.x] || (v == bv[threadIdx.x] && j < bi[threadIdx.x])) {
-        bv[threadIdx.x] = v;
-        bi[threadIdx.x] = j;
+  const int J1 = min(J0 + PCD_NB, N);
+  for (int j = J0; j < J1; ++j) {
+    double best = -1.0;
+    int idx = N;
+    for (int k = j + tid; k < N; k += 1024) {
+      const double v = fabs(d[k]);
+      if (v > best || (v == best && k < idx)) {
+        best = v;
+        idx = k;
+      }
+    }
+    bv[tid] = best;
+    bi[tid] = idx;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+      if (tid < off) {
+        const double v = bv[tid + off];
+        const int i2 = bi[tid + off];
+        if (v > bv[tid] || (v == bv[tid] && i2 < bi[tid])) {
+          bv[tid] = v;
+          bi[tid] = i2;
+        }
+      }
+      __syncthreads();
+    }
+    const int p = bi[0];
+    const double pv = d[p];
+    if (j == 0 && tid == 0) s_thresh = N * 2.220446049250313e-16 * fabs(pv);
+    __syncthreads();
+    const double thresh = s_thresh;
+    if (fabs(pv) < thresh || pv < 0.0) {   // rank reached, or a negative pivot: stop
+      if (tid == 0) {
+        st->rank = j;
+        st->done = 1;
+        st->neg = fabs(pv) < thresh ? 0 : 1;
+        st->thresh = thresh;
+      }
+      return;
+    }
+    if (p != j) {
+      for (int k = tid; k < N; k += 1024) {   // rows j and p of U
+        const double t = U[(int64_t)j * N + k];
+        U[(int64_t)j * N + k] = U[(int64_t)p * N + k];
+        U[(int64_t)p * N + k] = t;
+      }
+      __syncthreads();
+      for (int k = tid; k < N; k += 1024) {   // columns j and p of U
+        const double t = U[(int64_t)k * N + j];
+        U[(int64_t)k * N + j] = U[(int64_t)k * N + p];
+        U[(int64_t)k * N + p] = t;
+      }
+      for (int k = tid; k < j; k += 1024) {   // rows j and p of the factor so far
+        const double t = L[(int64_t)j * N + k];
+        L[(int64_t)j * N + k] = L[(int64_t)p * N + k];
+        L[(int64_t)p * N + k] = t;
+      }
+      if (tid == 0) {
+        const double t = d[j];
+        d[j] = d[p];
+        d[p] = t;
+        const int q = perm[j];
+        perm[j] = perm[p];
+        perm[p] = q;
+      }
+      __syncthreads();
+    }
+    const double sq = sqrt(d[j]);
+    for (int r = j + tid; r < N; r += 1024) {
+      if (r == j) {
+        L[(int64_t)j * N + j] = sq;
+      }
+      else {
+        double v = U[(int64_t)r * N + j];
+        for (int i = J0; i < j; ++i) v -= L[(int64_t)r * N + i] * L[(int64_t)j * N + i];
+        v /= sq;
+        L[(int64_t)r * N + j] = v;
+        d[r] -= v * v;
       }
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    out[0] = (double)bi[0];
-    out[1] = (bi[0] < N) ? U[(int64_t)bi[0] * N + bi[0]] : 0.0;
+  if (tid == 0) {
+    st->rank = J1;
+    st->thresh = s_thresh;
+    if (J1 == N) st->done = 1;
   }
 }
 
-__global__ void k_swap_rows(double* __restrict__ A, int ncols, int ld, int r1, int r2)
+// U[a][b] -= sum over the panel's columns of L[a][i] L[b][i], a, b >= J0 + PCD_NB
+// (16 x 16 tiles of U, the panel rows of both sides staged in LDS)
+__global__ __launch_bounds__(256) void k_pcd_trailing(double* __restrict__ U, int N, const double* __restrict__ L,
+                                                      int J0, const PcdState* __restrict__ st)
 {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= ncols) return;
-  const double t = A[(int64_t)r1 * ld + k];
-  A[(int64_t)r1 * ld + k] = A[(int64_t)r2 * ld + k];
-  A[(int64_t)r2 * ld + k] = t;
-}
-
-__global__ void k_swap_cols(double* __restrict__ A, int nrows, int ld, int c1, int c2)
-{
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nrows) return;
-  const double t = A[(int64_t)k * ld + c1];
-  A[(int64_t)k * ld + c1] = A[(int64_t)k * ld + c2];
-  A[(int64_t)k * ld + c2] = t;
-}
-
-// L[k][I] = U[k][I] / sqrt(piv) for k >= I (row-major L, leading dim ldl)
-__global__ void k_pcd_column(const double* __restrict__ U, int N, int I, double piv,
-                             double* __restrict__ L, int ldl)
-{
-  const int k = I + blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= N) return;
-  const double s = sqrt(piv);
-  L[(int64_t)k * ldl + I] = (k == I) ? s : U[(int64_t)k * N + I] / s;
-}
-
-// U[a][b] -= U[a][I] U[b][I] / piv for a, b > I
-__global__ void k_pcd_update(double* __restrict__ U, int N, int I, double piv)
-{
-  const int a = I + 1 + blockIdx.y;
-  const int b = I + 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (st->done) return;
+  __shared__ double la[16][PCD_NB + 1], lb[16][PCD_NB + 1];
+  const int J1 = J0 + PCD_NB;
+  const int a0 = J1 + blockIdx.y * 16, b0 = J1 + blockIdx.x * 16;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  for (int e = threadIdx.x; e < 16 * PCD_NB; e += 256) {
+    const int r = e / PCD_NB, i = e % PCD_NB;
+    la[r][i] = a0 + r < N ? L[(int64_t)(a0 + r) * N + J0 + i] : 0.0;
+    lb[r][i] = b0 + r < N ? L[(int64_t)(b0 + r) * N + J0 + i] : 0.0;
+  }
+  __syncthreads();
+  const int a = a0 + ty, b = b0 + tx;
   if (a >= N || b >= N) return;
-  U[(int64_t)a * N + b] -= U[(int64_t)a * N + I] * U[(int64_t)b * N + I] / piv;
+  double v = 0.0;
+#pragma unroll 8
+  for (int i = 0; i < PCD_NB; ++i) v += la[ty][i] * lb[tx][i];
+  U[(int64_t)a * N + b] -= v;
 }
 
 // ---- small helpers ---------------------------------------------------------
@@ -158,44 +246,46 @@ __global__ void k_trace_sq(const double* __restrict__ Z, int n, double* __restri
 // ---- pivoted Cholesky driver -------------------------------------------------
 // Decomposes the row-major N x N matrix in U (destroyed); returns L (host,
 // N x rank row-major) in the original row order with LMO-ordered columns.
-std::vector<double> pivoted_cholesky_dev(Ctx& c, DevBuf& U, int N, double filter_eps, int* rank_out)
+// Throws at a negative pivot above the threshold (piv_cd.cpp's "Negative
+// Pivot element. CD not possible."); *neg_out, when given, receives that
+// case instead of the exception.
+std::vector<double> pivoted_cholesky_dev(Ctx& c, DevBuf& U, int N, double filter_eps, int* rank_out, bool* neg_out)
 {
-  DevBuf L((size_t)N * N * sizeof(double)), piv(2 * sizeof(double));
+  DevBuf L((size_t)std::max(N, 1) * std::max(N, 1) * sizeof(double)), d((size_t)std::max(N, 1) * sizeof(double)),
+      perm_d((size_t)std::max(N, 1) * sizeof(int)), st(sizeof(PcdState));
   hipLaunchKernelGGL(k_fill, dim3(gs_blocks((int64_t)N * N)), dim3(256), 0, c.stream, L.as<double>(),
                      (int64_t)N * N, 0.0);
-  std::vector<int> perm(N);
-  std::iota(perm.begin(), perm.end(), 0);
-  double hp[2];
-  hipLaunchKernelGGL(k_pcd_pivot, dim3(1), dim3(1024), 0, c.stream, U.as<double>(), N, 0, piv.as<double>());
-  d2h(hp, piv.p, sizeof(hp), c.stream);
-  const double thresh = N * 2.220446049250313e-16 * std::fabs(hp[1]);
-  int rank = N;
-  for (int I = 0; I < N; ++I) {
-    hipLaunchKernelGGL(k_pcd_pivot, dim3(1), dim3(1024), 0, c.stream, U.as<double>(), N, I, piv.as<double>());
-    d2h(hp, piv.p, sizeof(hp), c.stream);
-    const int j = (int)hp[0];
-    const double pv = hp[1];
-    if (j != I && j < N) {
-      hipLaunchKernelGGL(k_swap_rows, dim3(nblk(N, 256)), dim3(256), 0, c.stream, U.as<double>(), N, N, I, j);
-      hipLaunchKernelGGL(k_swap_cols, dim3(nblk(N, 256)), dim3(256), 0, c.stream, U.as<double>(), N, N, I, j);
-      if (I > 0)
-        hipLaunchKernelGGL(k_swap_rows, dim3(nblk(I, 256)), dim3(256), 0, c.stream, L.as<double>(), I, N, I, j);
-      std::swap(perm[I], perm[j]);
-    }
-    if (pv < 0.0 && std::fabs(pv) > thresh) throw std::runtime_error("Negative Pivot element. CD not possible.");
-    if (std::fabs(pv) < thresh) {
-      rank = I;
-      break;
-    }
-    hipLaunchKernelGGL(k_pcd_column, dim3(nblk(N - I, 256)), dim3(256), 0, c.stream, U.as<double>(), N, I, pv,
-                       L.as<double>(), N);
-    if (I + 1 < N)
-      hipLaunchKernelGGL(k_pcd_update, dim3(nblk(N - I - 1, 256), N - I - 1), dim3(256), 0, c.stream,
-                         U.as<double>(), N, I, pv);
+  hipLaunchKernelGGL(k_pcd_init, dim3(nblk(std::max(N, 1), 256)), dim3(256), 0, c.stream, U.as<double>(), N,
+                     d.as<double>(), perm_d.as<int>(), st.as<PcdState>());
+  PcdState hs{};
+  for (int J0 = 0; J0 < N; J0 += PCD_NB) {
+    hipLaunchKernelGGL(k_pcd_panel, dim3(1), dim3(1024), 0, c.stream, U.as<double>(), N, L.as<double>(),
+                       d.as<double>(), perm_d.as<int>(), J0, st.as<PcdState>());
+    const int rest = N - (J0 + PCD_NB);
+    if (rest > 0)
+      hipLaunchKernelGGL(k_pcd_trailing, dim3(nblk(rest, 16), nblk(rest, 16)), dim3(256), 0, c.stream,
+                         U.as<double>(), N, L.as<double>(), J0, st.as<PcdState>());
+    HIP_CHECK(hipGetLastError());
+    d2h(&hs, st.p, sizeof(hs), c.stream);   // one read of the state per panel
+    if (hs.done) break;
   }
-  HIP_CHECK(hipGetLastError());
+  if (N == 0) hs.rank = 0;
+  if (hs.neg) {
+    if (neg_out) {
+      *neg_out = true;
+      *rank_out = 0;
+      return {};
+    }
+    throw std::runtime_error("Negative Pivot element. CD not possible.");
+  }
+  if (neg_out) *neg_out = false;
+  const int rank = hs.rank;
   std::vector<double> hL((size_t)N * N);
-  d2h(hL.data(), L.p, hL.size() * sizeof(double), c.stream);
+  std::vector<int> perm(N);
+  if (N > 0) {
+    d2h(hL.data(), L.p, hL.size() * sizeof(double), c.stream);
+    d2h(perm.data(), perm_d.p, perm.size() * sizeof(int), c.stream);
+  }
   // rows back to the original order
   std::vector<double> Lo((size_t)N * rank, 0.0);
   for (int r = 0; r < N; ++r)

@@ -34,7 +34,8 @@ void z_point(Ctx& c, DevBT& bt, int zmethod, const double* Lo, int ro, const dou
 // (overwritten): returns L (N x rank, row-major, rows in the original order,
 // columns in LMO position order) with U = L L^T to N * eps * max|diag|
 // (math::pivinc_cd, piv_cd.cpp:125-483).
-std::vector<double> pivoted_cholesky_dev(Ctx& c, DevBuf& U, int N, double filter_eps, int* rank_out);
+std::vector<double> pivoted_cholesky_dev(Ctx& c, DevBuf& U, int N, double filter_eps, int* rank_out,
+                                         bool* neg_out = nullptr);
 
 // Returns E = sum_k e_k (positive, unscaled); terms[k] = e_k; t_z = seconds
 // spent in the Z builder (M2-M5) over all points.  Z_host, if not null,

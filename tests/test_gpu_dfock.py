@@ -454,6 +454,52 @@ def test_dfao_from_density_vs_oracle(ctx, nwater):
     assert _relerr(K16, Kmo) < BF16_TOL
 
 
+@pytest.mark.parametrize("nwater", [1, 3])
+def test_dfao_indefinite_density_vs_oracle(ctx, nwater):
+    """DFAO_K from an indefinite density (a difference of two densities, as in
+    a response or a density update): the pivoted Cholesky stops at a negative
+    pivot, and the exchange runs on the density's eigenvector factors,
+    K(P) = K(C+) - K(C-).  Compared with the oracle's fitted-coefficient DFAO
+    (which takes any P) and with the difference of the two GPU DFMO builds."""
+    from chem_amd.engine import DeviceB, Metric
+    mol, Bh, V = _system(nwater)
+    Bd = Bh.dense()
+    dB = DeviceB(ctx, Bh)
+    m = Metric(ctx, V)
+    Linv, Vinv = m.get()
+    C1, P1 = _random_occ(mol.nbf, mol.nocc, 31 + nwater)
+    C2, P2 = _random_occ(mol.nbf, mol.nocc - 1, 41 + nwater)
+    P = P1 - 0.5 * P2
+    w = np.linalg.eigvalsh(P)
+    assert w.min() < -1e-3 * w.max()          # really indefinite
+    print("case: indefinite density, %d waters" % nwater, flush=True)
+    K = dB.dfk_ao(P, m)
+    Kao = dfock.dfao_k(Bd, dfock.fitting_coefficients(Bd, Vinv), P, bsz=mol.b, xsz=mol.x)
+    Kdiff = dB.dfk_mo(C1, m) - 0.5 * dB.dfk_mo(C2, m)
+    e_ao, e_diff = _relerr(K, Kao), _relerr(K, Kdiff)
+    print("indefinite DFAO: relerr vs oracle DFAO %.2e, vs DFMO difference %.2e" % (e_ao, e_diff))
+    assert e_ao < _jtol(V)
+    assert e_diff < 1e-11
+
+
+def test_pivoted_cholesky_rank_and_factor(ctx):
+    """The device pivoted Cholesky on a rank-deficient PSD matrix across several
+    panels: the rank found is the matrix rank and L L^T reproduces it (through
+    the DFAO route: K(P) equals K of the generating factor)."""
+    from chem_amd.engine import DeviceB, Metric
+    mol, Bh, V = _system(3)
+    dB = DeviceB(ctx, Bh)
+    m = Metric(ctx, V)
+    rng = np.random.default_rng(5)
+    for r in (1, 31, 32, 33, 70):
+        G = rng.standard_normal((mol.nbf, min(r, mol.nbf)))
+        P = G @ G.T
+        print("case: rank %d of %d" % (r, mol.nbf), flush=True)
+        K = dB.dfk_ao(P, m)
+        Kg = dB.dfk_mo(np.ascontiguousarray(G), m)
+        assert _relerr(K, Kg) < 1e-11
+
+
 def test_lowp_copies_from_untransformed_B(ctx):
     """The low-precision copies of L^-1 B made slab by slab from the untransformed
     B give bitwise the same exchange as the copies of the in-place transformed

@@ -70,8 +70,9 @@ def test_cross_request_batching():
 
 
 def test_served_fp64_matches_direct_scf():
-    """kmode 0 through the loop (host integrals and mask in C++) equals the
-    direct device SCF driven from Python on the same molecule."""
+    """kmode 0 through the loop (integrals on the GPU, mask in C++) equals the
+    direct device SCF driven from Python on the same molecule and the same
+    (GPU) one-electron integrals and metric."""
     from chem_amd.engine import Context, DeviceB, Metric
     from chem_amd.scf import run_rhf
     from chem_amd.serve import Server
@@ -80,9 +81,9 @@ def test_served_fp64_matches_direct_scf():
     with Server(budget_bytes=20e9, nslots=1) as srv:
         r = srv.result(srv.submit(mol, threshold=1e-8, kmode="dfmo"))
     ctx = Context(0)
-    S, T, V = ints.int1e(mol)
+    S, T, V = ints.int1e_gpu(ctx, mol)
     dB = DeviceB.build_on_gpu(ctx, mol, 1e-10)
-    met = Metric(ctx, ints.int2c2e(mol))
+    met = Metric(ctx, ints.int2c2e_gpu(ctx, mol))
     d = run_rhf(mol, dB, met, S, T + V, threshold=1e-8, kmode="dfmo")
     print("served %.12f direct %.12f" % (r["energy"], d.energy))
     assert r["converged"] and d.converged
