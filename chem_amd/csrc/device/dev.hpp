@@ -320,6 +320,9 @@ struct DevBT {
   int64_t colmap_total = 0;
   int64_t pg_total = 0;                     // sum of bsz[N]*ncol[N]
   int64_t maxR = 0;                         // max bsz[N]*ncol[N]
+  DevBuf cdn;                               // int2 per slab: first column of block N, first past it
+  int64_t maxRc = 0;                        // max bsz[N]*cdn[N].y (canonical half of a slab)
+  bool sym_mask = false;                    // block mask symmetric in (M, N): DF-J reads the canonical half
   bool transformed = false;  // data64 holds L^-1 B
   bool lowp_tilde = false;   // data32 / data_hi,lo hold L^-1 B (from either route)
   DevBuf gx_w, gx_g, gx_part;   // work arrays of this tensor's builds in a grouped BF16x3 exchange

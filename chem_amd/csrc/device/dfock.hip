@@ -165,6 +165,33 @@ static void dbt_index(Ctx& c, const BLayout& L, DevBT& t)
     t.maxR = std::max<int64_t>(t.maxR, R);
   }
   t.pg_total = t.h_pg_off[L.nbb];
+  // canonical half of each slab (columns of blocks M <= N) for DF-J, when
+  // every significant pair (M, N) has its transpose (N, M)
+  {
+    std::vector<int2> cdn(std::max(L.nbb, 1), make_int2(0, 0));
+    t.maxRc = 0;
+    t.sym_mask = true;
+    for (int x = 0; x < L.nxb && t.sym_mask; ++x)
+      for (int m = 0; m < L.nbb && t.sym_mask; ++m)
+        for (int n = 0; n < m; ++n)
+          if (L.kept(x, m, n) != L.kept(x, n, m)) {
+            t.sym_mask = false;
+            break;
+          }
+    for (int n = 0; n < L.nbb; ++n) {
+      int cd = L.ncol[n], ce = L.ncol[n];
+      for (int p = L.pair_ptr[n]; p < L.pair_ptr[n + 1]; ++p) {
+        const int m = L.pair_m[p];
+        if (m < n) continue;
+        cd = L.pair_col[p];
+        ce = m == n ? L.pair_col[p] + L.bsz[m] : L.pair_col[p];
+        break;
+      }
+      cdn[n] = make_int2(cd, ce);
+      t.maxRc = std::max<int64_t>(t.maxRc, (int64_t)L.bsz[n] * ce);
+    }
+    t.cdn = upload_vec(cdn, c.stream);
+  }
   std::vector<int64_t> so(L.slab_off.begin(), L.slab_off.end());
   t.slab_off = upload_vec(so, c.stream);
   t.ncol = upload_vec(L.ncol, c.stream);
@@ -361,9 +388,14 @@ void dfj_ws(Ctx& c, DevBT& t, const Metric* m, const double* P, double* J, doubl
     hipLaunchKernelGGL(k_pair_gather, dim3(gx, nbb), dim3(256), 0, c.stream, P, nbf, t.ncol.as<int>(),
                        t.bsz.as<int>(), t.boff.as<int>(), t.colmap.as<int>(), t.colmap_off.as<int>(),
                        t.pg_off.as<int64_t>(), Pg);
-    hipLaunchKernelGGL(k_j_gamma, dim3(nblk(naux, J_ROWS), nbb), dim3(256), 0, c.stream,
-                       t.data64.as<double>(), t.slab_off.as<int64_t>(), t.ncol.as<int>(),
-                       t.bsz.as<int>(), t.pg_off.as<int64_t>(), Pg, naux, gpart, gpart_c);
+    if (t.sym_mask)
+      hipLaunchKernelGGL(k_j_gamma_sym, dim3(nblk(naux, J_ROWS), nbb), dim3(256), 0, c.stream,
+                         t.data64.as<double>(), t.slab_off.as<int64_t>(), t.ncol.as<int>(), t.bsz.as<int>(),
+                         t.cdn.as<int2>(), t.pg_off.as<int64_t>(), Pg, naux, gpart, gpart_c);
+    else
+      hipLaunchKernelGGL(k_j_gamma, dim3(nblk(naux, J_ROWS), nbb), dim3(256), 0, c.stream,
+                         t.data64.as<double>(), t.slab_off.as<int64_t>(), t.ncol.as<int>(),
+                         t.bsz.as<int>(), t.pg_off.as<int64_t>(), Pg, naux, gpart, gpart_c);
   }
   else {
     hipLaunchKernelGGL(k_fill, dim3(gs_blocks(2 * n_gp)), dim3(256), 0, c.stream, gpart, (int64_t)(2 * n_gp), 0.0);
@@ -379,7 +411,20 @@ void dfj_ws(Ctx& c, DevBT& t, const Metric* m, const double* P, double* J, doubl
   }
   hipLaunchKernelGGL(k_fill, dim3(gs_blocks((int64_t)nbf * nbf)), dim3(256), 0, c.stream, J,
                      (int64_t)nbf * nbf, 0.0);
-  if (pg > 0) {
+  if (pg > 0 && t.sym_mask) {
+    // canonical half: the entries of Jpart past each slab's canonical columns
+    // are never written; the reduction carries them along unread
+    const unsigned gxc = nblk(std::max<int64_t>(t.maxRc, 1), 256);
+    hipLaunchKernelGGL(k_j_cols_sym, dim3(gxc, nbb, nch), dim3(256), 0, c.stream, t.data64.as<double>(),
+                       t.slab_off.as<int64_t>(), t.ncol.as<int>(), t.bsz.as<int>(), t.cdn.as<int2>(),
+                       t.pg_off.as<int64_t>(), gq, naux, xchunk, pg, Jpart, Jpart_c);
+    hipLaunchKernelGGL(k_reduce_chunks_dd, dim3(gs_blocks(pg)), dim3(256), 0, c.stream, Jpart, Jpart_c, nch, pg,
+                       Jg);
+    hipLaunchKernelGGL(k_pair_scatter_sym, dim3(gxc, nbb), dim3(256), 0, c.stream, Jg, nbf, t.ncol.as<int>(),
+                       t.bsz.as<int>(), t.boff.as<int>(), t.cdn.as<int2>(), t.colmap.as<int>(),
+                       t.colmap_off.as<int>(), t.pg_off.as<int64_t>(), J);
+  }
+  else if (pg > 0) {
     hipLaunchKernelGGL(k_j_cols, dim3(gx, nbb, nch), dim3(256), 0, c.stream, t.data64.as<double>(),
                        t.slab_off.as<int64_t>(), t.ncol.as<int>(), t.bsz.as<int>(), t.pg_off.as<int64_t>(),
                        gq, naux, xchunk, pg, Jpart, Jpart_c);

@@ -53,6 +53,15 @@ __global__ void k_pair_gather(const double* P, int nbf, const int* ncol, const i
 __global__ void k_pair_scatter(const double* Jg, int nbf, const int* ncol, const int* bsz,
                                const int* boff, const int* colmap, const int* colmap_off,
                                const int64_t* pg_off, double* J);
+__global__ void k_j_gamma_sym(const double* bdata, const int64_t* slab_off, const int* ncol, const int* bsz,
+                              const int2* cdn, const int64_t* pg_off, const double* Pg, int naux, double* gpart,
+                              double* gpart_c);
+__global__ void k_j_cols_sym(const double* bdata, const int64_t* slab_off, const int* ncol, const int* bsz,
+                             const int2* cdn, const int64_t* pg_off, const double* gq, int naux, int xchunk,
+                             int64_t pg_total, double* Jpart, double* Jpart_c);
+__global__ void k_pair_scatter_sym(const double* Jg, int nbf, const int* ncol, const int* bsz, const int* boff,
+                                   const int2* cdn, const int* colmap, const int* colmap_off, const int64_t* pg_off,
+                                   double* J);
 __global__ void k_j_gamma(const double* bdata, const int64_t* slab_off, const int* ncol,
                           const int* bsz, const int64_t* pg_off, const double* Pg, int naux,
                           double* gpart, double* gpart_c);

@@ -80,6 +80,25 @@ __global__ void k_pair_scatter(const double* __restrict__ Jg, int nbf, const int
   J[(int64_t)mu * nbf + boff[N] + nu] = Jg[pg_off[N] + e];
 }
 
+// J from the canonical half: J[mu][boff[N]+nu] for the columns c < cdn[N].y,
+// and the transpose J[boff[N]+nu][mu] for the columns of blocks M < N
+__global__ void k_pair_scatter_sym(const double* __restrict__ Jg, int nbf, const int* __restrict__ ncol,
+                                   const int* __restrict__ bsz, const int* __restrict__ boff,
+                                   const int2* __restrict__ cdn, const int* __restrict__ colmap,
+                                   const int* __restrict__ colmap_off, const int64_t* __restrict__ pg_off,
+                                   double* __restrict__ J)
+{
+  const int N = blockIdx.y;
+  const int bn = bsz[N], nc = ncol[N], cd = cdn[N].x, ce = cdn[N].y;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= bn * ce) return;
+  const int nu = q / ce, c = q - nu * ce;
+  const int mu = colmap[colmap_off[N] + c];
+  const double v = Jg[pg_off[N] + nu * nc + c];
+  J[(int64_t)mu * nbf + boff[N] + nu] = v;
+  if (c < cd) J[(int64_t)(boff[N] + nu) * nbf + mu] = v;
+}
+
 // gpart[N*naux + x] = sum_e slab_N[x][e] * Pg_N[e] as a compensated pair (sum
 // in gpart, rounding errors in gpart_c); one wave per row, J_ROWS rows per block
 __global__ __launch_bounds__(256) void k_j_gamma(
@@ -98,6 +117,39 @@ __global__ __launch_bounds__(256) void k_j_gamma(
     const double* row = slab + (int64_t)x * R;
     dsum s;
     for (int e = lane; e < R; e += 64) dot2_add(s, row[e], pg[e]);
+    dsum_wave(s);
+    if (lane == 0) {
+      gpart[(int64_t)N * naux + x] = s.s;
+      gpart_c[(int64_t)N * naux + x] = s.c;
+    }
+  }
+}
+
+// The same gamma sums over the canonical half of the slabs: B(X|mu nu) =
+// B(X|nu mu) and P symmetric, so slab N contributes its columns of blocks
+// M < N twice and of its own block M = N once (cdn[N] = (first column of
+// block N, first column past it); columns are in block order), and the
+// columns of blocks M > N, the transposes of other slabs' entries, are not
+// read.  One wave per aux row, J_ROWS rows per block.
+__global__ __launch_bounds__(256) void k_j_gamma_sym(
+    const double* __restrict__ bdata, const int64_t* __restrict__ slab_off, const int* __restrict__ ncol,
+    const int* __restrict__ bsz, const int2* __restrict__ cdn, const int64_t* __restrict__ pg_off,
+    const double* __restrict__ Pg, int naux, double* __restrict__ gpart, double* __restrict__ gpart_c)
+{
+  const int N = blockIdx.y;
+  const int bn = bsz[N], nc = ncol[N], cd = cdn[N].x, ce = cdn[N].y;
+  const double* slab = bdata + slab_off[N];
+  const double* pg = Pg + pg_off[N];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int rr = wave; rr < J_ROWS; rr += 4) {
+    const int x = blockIdx.x * J_ROWS + rr;
+    if (x >= naux) break;
+    dsum s;
+    for (int nu = 0; nu < bn; ++nu) {
+      const double* row = slab + ((int64_t)x * bn + nu) * nc;
+      const double* pr = pg + (int64_t)nu * nc;
+      for (int c = lane; c < ce; c += 64) dot2_add(s, c < cd ? 2.0 * row[c] : row[c], pr[c]);
+    }
     dsum_wave(s);
     if (lane == 0) {
       gpart[(int64_t)N * naux + x] = s.s;
@@ -149,6 +201,29 @@ __global__ void k_j_cols(const double* __restrict__ bdata, const int64_t* __rest
   const int R = bsz[N] * ncol[N];
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= R) return;
+  const double* slab = bdata + slab_off[N];
+  const int x0 = chunk * xchunk, x1 = min(naux, x0 + xchunk);
+  dsum s;
+  for (int x = x0; x < x1; ++x) dot2_add(s, gq[x], slab[(int64_t)x * R + e]);
+  Jpart[(int64_t)chunk * pg_total + pg_off[N] + e] = s.s;
+  Jpart_c[(int64_t)chunk * pg_total + pg_off[N] + e] = s.c;
+}
+
+// k_j_cols over the canonical half: the columns c < cdn[N].y of slab N only
+// (thread q = nu * ce + c); the other entries of Jpart are left as they were
+__global__ void k_j_cols_sym(const double* __restrict__ bdata, const int64_t* __restrict__ slab_off,
+                             const int* __restrict__ ncol, const int* __restrict__ bsz, const int2* __restrict__ cdn,
+                             const int64_t* __restrict__ pg_off, const double* __restrict__ gq, int naux,
+                             int xchunk, int64_t pg_total, double* __restrict__ Jpart, double* __restrict__ Jpart_c)
+{
+  const int N = blockIdx.y;
+  const int chunk = blockIdx.z;
+  const int bn = bsz[N], nc = ncol[N], ce = cdn[N].y;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= bn * ce) return;
+  const int nu = q / ce, c = q - nu * ce;
+  const int e = nu * nc + c;
+  const int64_t R = (int64_t)bn * nc;
   const double* slab = bdata + slab_off[N];
   const int x0 = chunk * xchunk, x1 = min(naux, x0 + xchunk);
   dsum s;
