@@ -394,8 +394,9 @@ int mchip_scf_last_extra(double* energy_iter, double* t_final_k);
  * metric, Schwarz block mask, HBM estimate), wait until the estimate fits
  * budget_bytes next to the resident requests (admission in arrival order),
  * then build B on the GPU and run the SCF with everything resident.  Work of
- * different requests overlaps on the device.  cpu_threads is split over the
- * workers for the host integrals.
+ * different requests overlaps on the device.  The one-electron integrals and
+ * the metric are computed on the GPU, the Schwarz factors on the host
+ * (cpu_threads split over the workers).
  * kmode as for mchip_scf_run (0-4); screen_eps is the block threshold.
  * Requests get no FP32 copy, and kmode 4 here escalates on a stall only (no
  * RMS trigger): BF16x3 until the SCF converges or stalls, then FP64.
