@@ -126,6 +126,42 @@ constexpr int shp(int l, int m, int k)
 // accumulators (group x ncart(la) x ncart(lb) doubles) stay within 24
 constexpr int mgroup(int ncab, int nm) { return 24 / ncab < 1 ? 1 : (24 / ncab > nm ? nm : 24 / ncab); }
 
+// Compile-time checks of the index helpers: the Hermite decode inverts the
+// index for every triple up to degree 8, and the Cartesian exponents of
+// every component sum to its degree and run in the order of md::cart_exps
+// (x descending, then y descending); each solid harmonic's integer shape
+// has at least one monomial and no monomial outside the degree.
+constexpr bool hermite_index_ok()
+{
+  for (int i = 0; i < nh(8); ++i) {
+    if (hix(ht(i), hu(i), hv(i)) != i) return false;
+    if (ht(i) < 0 || hu(i) < 0 || hv(i) < 0 || ht(i) + hu(i) + hv(i) != hdeg(i)) return false;
+  }
+  return true;
+}
+constexpr bool cartesian_order_ok()
+{
+  for (int l = 0; l <= 4; ++l)
+    for (int k = 0; k + 1 < nc(l); ++k) {
+      if (cx(l, k) + cy(l, k) + cz(l, k) != l || cz(l, k) < 0) return false;
+      if (cx(l, k) < cx(l, k + 1) || (cx(l, k) == cx(l, k + 1) && cy(l, k) <= cy(l, k + 1))) return false;
+    }
+  return true;
+}
+constexpr bool shapes_ok()
+{
+  for (int l = 0; l <= 3; ++l)
+    for (int m = 0; m < 2 * l + 1; ++m) {
+      int nz = 0;
+      for (int k = 0; k < nc(l); ++k) nz += shp(l, m, k) != 0;
+      if (nz == 0) return false;
+    }
+  return true;
+}
+static_assert(hermite_index_ok(), "Hermite index decode");
+static_assert(cartesian_order_ok(), "Cartesian component order");
+static_assert(shapes_ok(), "solid harmonic shapes");
+
 }  // namespace i3c
 
 // E^{ij}_t of one direction for i <= LA, j <= LB (ints/md.hpp, hermite_E)
