@@ -33,6 +33,11 @@ __global__ void k_pad_f32_rows(const float* src, int N, const int* ncol, const i
 __global__ void k_gather_ct_f32f(const float* C, int ldc, int ng, const int* ncol, const int* ldp,
                                  const int64_t* ldp_pre, const int* colmap, const int* colmap_off, float* g);
 template <int S>
+__global__ void k_slab_out_gemm_f32_dl(const float* A, int ldr, const int64_t* slab_off, const int* ncol,
+                                       const int* bsz, const int* boff, const int* colmap, const int* colmap_off,
+                                       const Tile* tiles, int naux, int nbf, const float* G, float* out,
+                                       const int* pcol, const int* pwidth, int64_t out_stride);
+template <int S>
 __global__ void k_slab_gemm_f32_dl_f64(const float* data, const int64_t* off, const int* ldp, const int64_t* ldp_pre,
                                        const int* bsz, const int* boff, const Tile* tiles, int naux, int nbf,
                                        const float* g, int ng, float* out, int ldo);

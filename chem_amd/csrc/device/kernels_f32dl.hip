@@ -7,7 +7,7 @@
 //   k_transpose_pad_f32   b[X][nu][i] -> b^T[X][i][nu], rows padded to ldn
 //   k_group_apply_f32_dl  M3: P_v,g b_g[X] for every (point, X), P_v as an FP32 pair
 //   k_to_f32x2_rows       FP64 P_v -> FP32 hi/lo rows padded to ldn
-//   k_pad_f32_rows, k_gather_ct_f32f, k_slab_gemm_f32_dl_f64
+//   k_pad_f32_rows, k_gather_ct_f32f, k_slab_gemm_f32_dl_f64, k_slab_out_gemm_f32_dl
 //                         M2: b = B~ L on the padded FP32 copy, FP64 sums
 //
 // Operands use the padded slab layout of the BF16 copy: slab N has naux*bsz[N]
@@ -458,6 +458,104 @@ template __global__ void k_slab_gemm_f32_dl_f64<2>(const float*, const int64_t*,
 template __global__ void k_slab_gemm_f32_dl_f64<4>(const float*, const int64_t*, const int*, const int64_t*,
                                                    const int*, const int*, const Tile*, int, int, const float*, int,
                                                    float*, int);
+
+// MP2 M4 on direct-to-LDS pipelines, grouped over the Laplace points:
+// out_g[slab_off[N] + r nc + c] = sum_{i < pwidth[g]} A[(x nbf + boff[N] + nu) ldr
+// + pcol[g] + i] G[colmap_N[c] ldr + pcol[g] + i] for the rows r = x bn + nu
+// and columns c < nc of slab N (b3 = L_o b2 on B's pair pattern).  Both
+// operands are contraction-contiguous rows of ldr floats; ldr, pcol[g] and
+// pwidth[g] are multiples of 16 (whole 64-byte steps).  Tile = (slab,
+// m0, n0, point g in pad): 128 x 128, eight waves of 64 x 32, one A and one B
+// row block per wave and stage; each step's FP32 MFMA sums (16 products) go
+// into FP64 running sums, double-buffered as in the M2 kernel.
+template <int S>
+__global__ __launch_bounds__(512, 2) void k_slab_out_gemm_f32_dl(
+    const float* __restrict__ A, int ldr, const int64_t* __restrict__ slab_off, const int* __restrict__ ncol,
+    const int* __restrict__ bsz, const int* __restrict__ boff, const int* __restrict__ colmap,
+    const int* __restrict__ colmap_off, const Tile* __restrict__ tiles, int naux, int nbf,
+    const float* __restrict__ G, float* __restrict__ out, const int* __restrict__ pcol,
+    const int* __restrict__ pwidth, int64_t out_stride)
+{
+  constexpr int BM = 128, BN = 128, NW = 8, WN = 4, TM = 4, TN = 2;
+  constexpr int QA = BM / 16, QB = BN / 16;
+  static_assert(QA == NW && QB == NW, "one A and one B row block per wave and stage");
+  static_assert(S % 2 == 0 && S >= 2 && S <= 4, "pipeline depth (even: accumulator set = stage parity)");
+  constexpr int STAGE = (BM + BN) * 4;
+  __shared__ uint4 lds0[STAGE], lds1[STAGE], lds2[STAGE], lds3[STAGE];
+  const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  if (t.slab < 0) return;
+  const int N = t.slab, g = t.pad;
+  const int bn = bsz[N], nc = ncol[N];
+  const int64_t rows = (int64_t)naux * bn;
+  const int ld4 = ldr >> 2, pc4 = pcol[g] >> 2;
+  const uint4* A4 = reinterpret_cast<const uint4*>(A);
+  const uint4* G4 = reinterpret_cast<const uint4*>(G);
+  const int* cmap = colmap + colmap_off[N];
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm0 = (wave / WN) * (16 * TM), wn0 = (wave % WN) * (16 * TN);
+  f32v4 acc[2][TM][TN];
+  double acc64[TM][TN][4];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      acc[0][i][j] = f32v4{0.f, 0.f, 0.f, 0.f};
+      acc[1][i][j] = f32v4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc64[i][j][e] = 0.0;
+    }
+  const int nsteps = pwidth[g] >> 4;
+  // per wave: A row 16 wave + rr (a row (x, nu) of b2) and B row 16 wave + rr
+  // (the L_o row of column c), both clamped into range, offsets in 16-byte
+  // units, fixed over the contraction
+  const int rr = lane >> 2, pos = lane & 3;
+  const int r = 16 * wave + rr;
+  const int64_t ra = (int64_t)t.m0 + r < rows - 1 ? (int64_t)t.m0 + r : rows - 1;
+  const int64_t x = ra / bn;
+  const int64_t arow = x * nbf + boff[N] + (ra - x * bn);
+  const int cb = t.n0 + r < nc - 1 ? t.n0 + r : nc - 1;
+  const int64_t offA = arow * ld4 + pc4 + swz(r, pos), offB = (int64_t)cmap[cb] * ld4 + pc4 + swz(r, pos);
+  dl_pipeline<S, 2>(
+      nsteps,
+      [&](int s, auto k) {
+        uint4* st = pick_stage<S>(k, lds0, lds1, lds2, lds3);
+        lds_load16(A4 + (offA + s * 4), st + wave * 64);
+        lds_load16(G4 + (offB + s * 4), st + BM * 4 + wave * 64);
+      },
+      [&](int, auto k) {
+        constexpr int p = decltype(k)::value & 1;
+        const uint4* st = pick_stage<S>(k, lds0, lds1, lds2, lds3);
+        f32_step_swz<TM, TN>(st, st + BM * 4, wm0, wn0, lane, acc[p]);
+        flush_f64<TM, TN>(acc[1 - p], acc64);   // step s - 1 (zero before the first step)
+      });
+  if (nsteps > 0) {
+    if ((nsteps - 1) & 1)
+      flush_f64<TM, TN>(acc[1], acc64);
+    else
+      flush_f64<TM, TN>(acc[0], acc64);
+  }
+  float* o = out + (int64_t)g * out_stride + slab_off[N];
+  const int fc = lane & 15, fr4 = 4 * (lane >> 4);
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t rw = (int64_t)t.m0 + wm0 + 16 * i + fr4 + e;
+      if (rw >= rows) continue;
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int c = t.n0 + wn0 + 16 * j + fc;
+        if (c < nc) o[rw * nc + c] = (float)acc64[i][j][e];
+      }
+    }
+}
+
+template __global__ void k_slab_out_gemm_f32_dl<2>(const float*, int, const int64_t*, const int*, const int*,
+                                                   const int*, const int*, const int*, const Tile*, int, int,
+                                                   const float*, float*, const int*, const int*, int64_t);
+template __global__ void k_slab_out_gemm_f32_dl<4>(const float*, int, const int64_t*, const int*, const int*,
+                                                   const int*, const int*, const int*, const Tile*, int, int,
+                                                   const float*, float*, const int*, const int*, int64_t);
 
 // FP64 n x n -> FP32 hi/lo pair, rows padded with zeros to ldn.
 __global__ void k_to_f32x2_rows(const double* __restrict__ in, int n, int ldn, float* __restrict__ hi,

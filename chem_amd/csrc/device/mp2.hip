@@ -584,7 +584,8 @@ __global__ void k_cols_to_f32(const double* __restrict__ in, int rows, int w, fl
 //                                                         direct-to-LDS on the padded FP32 copy
 //   M3  b2_g[X]         = P_v,g b_g[X]                     grouped over (point, X), P_v as an FP32 hi/lo pair,
 //                                                         direct-to-LDS on b transposed per X
-//   M4  b3_g(X|M N)     = sum_i L_g(M,i) b2_g(X,i,N)        grouped over points, B's pattern
+//   M4  b3_g(X|M N)     = sum_i L_g(M,i) b2_g(X,i,N)        grouped over points, B's pattern,
+//                                                         direct-to-LDS (rank blocks padded to 16)
 //   M5  Z'_g(X,Y)       = sum b3_g(X|mu nu) B~(Y|mu nu)      grouped over points, FP64-flushed
 // so Z'_g = L^-1 Z_g L^-T and e_g = tr(Z_g V^-1 Z_g V^-1) = sum_XY Z'_g(X,Y)^2
 // (FP64, after the graph).  The reference's form applies V^-1 after Z: Z then
@@ -607,10 +608,13 @@ double mp2_sos_batched(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in,
   Mp2Setup s;
   mp2_setup(c, in, s);
   std::vector<DevBuf> Lo(npt), Pv(npt);
-  std::vector<int> rank(npt), pcol(npt + 1, 0);
+  // each point's rank block padded with zero columns to a multiple of 16
+  // (whole 64-byte contraction steps for M4's direct-to-LDS loads)
+  std::vector<int> rank(npt), rank16(npt), pcol(npt + 1, 0);
   for (int g = 0; g < npt; ++g) {
     rank[g] = mp2_point_factors(c, s, in, g, Lo[g], Pv[g]);
-    pcol[g + 1] = pcol[g] + rank[g];
+    rank16[g] = (rank[g] + 15) / 16 * 16;
+    pcol[g + 1] = pcol[g] + rank16[g];
   }
   const int R = pcol[npt];
   terms.assign(npt, 0.0);
@@ -629,6 +633,7 @@ double mp2_sos_batched(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in,
   // and a single FP32 rounding of P_v alone costs ~1e-5 of max|Z|
   DevBuf Lall((size_t)n * R * sizeof(float)), Pv32((size_t)2 * npt * nl * sizeof(float)),
       B32((size_t)std::max<int64_t>(L.total, 1) * sizeof(float));
+  HIP_CHECK(hipMemsetAsync(Lall.p, 0, Lall.bytes, c.stream));   // the padding columns (before any capture)
   for (int g = 0; g < npt; ++g) {
     if (rank[g] == 0) continue;
     hipLaunchKernelGGL(k_cols_to_f32, dim3(nblk((int64_t)n * rank[g], 256)), dim3(256), 0, c.stream,
@@ -648,14 +653,14 @@ double mp2_sos_batched(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in,
   // b transposed per aux function, rows of ldn: the second operand of M3
   DevBuf bb(bsize * sizeof(float)), b2(bsize * sizeof(float)), bT((size_t)naux * R * ldn * sizeof(float)),
       b3((size_t)npt * std::max<int64_t>(L.total, 1) * sizeof(float));
-  std::vector<int> pw(rank.begin(), rank.end());
+  std::vector<int> pw(rank16.begin(), rank16.end());
   std::vector<int> pc(pcol.begin(), pcol.end() - 1);
   DevBuf d_pcol = upload_vec(pc, c.stream), d_pw = upload_vec(pw, c.stream);
   // M3 tiles (point, m0, first column): 128 x 128 over the columns (X, c) of
   // each point, column index X rank + c
   std::vector<int4> t3;
   for (int g = 0; g < npt; ++g)
-    for (int64_t col = 0; col < (int64_t)naux * rank[g]; col += 128)
+    for (int64_t col = 0; col < (int64_t)naux * rank16[g]; col += 128)
       for (int m0 = 0; m0 < n; m0 += 128) t3.push_back(int4{g, m0, (int)col, 0});
   if ((int64_t)naux * R >= ((int64_t)1 << 31)) throw std::runtime_error("MP2: too many (aux, rank) columns");
   // M4 tiles (slab, m0, c0, point)
@@ -668,6 +673,7 @@ double mp2_sos_batched(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in,
         for (int c0 = 0; c0 < L.ncol[nb]; c0 += 128) t4.push_back(Tile{nb, (int)m0, c0, g});
     }
   }
+  while (t4.size() % 8) t4.push_back(Tile{-1, 0, 0, 0});   // whole XCD segments (empty tiles return)
   // M5: upper 128 x 128 tiles of Z, slabs grouped in chunks of similar length
   const int ntz = (naux + SG_BM - 1) / SG_BM;
   std::vector<int2> t5;
@@ -744,6 +750,37 @@ double mp2_sos_batched(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in,
       }
     }
   }
+  // M4 on the direct-to-LDS pipeline; its depth is timed on the live operands
+  // (b2 from the tuning runs above)
+  auto m4 = [&](int depth) {
+    if (depth == 2)
+      hipLaunchKernelGGL(k_slab_out_gemm_f32_dl<2>, dim3((unsigned)t4.size()), dim3(512), 0, c.stream,
+                         b2.as<float>(), R, bt.slab_off.as<int64_t>(), bt.ncol.as<int>(), bt.bsz.as<int>(),
+                         bt.boff.as<int>(), bt.colmap.as<int>(), bt.colmap_off.as<int>(), d_t4.as<Tile>(), naux, n,
+                         Lall.as<float>(), b3.as<float>(), d_pcol.as<int>(), d_pw.as<int>(), (int64_t)L.total);
+    else
+      hipLaunchKernelGGL(k_slab_out_gemm_f32_dl<4>, dim3((unsigned)t4.size()), dim3(512), 0, c.stream,
+                         b2.as<float>(), R, bt.slab_off.as<int64_t>(), bt.ncol.as<int>(), bt.bsz.as<int>(),
+                         bt.boff.as<int>(), bt.colmap.as<int>(), bt.colmap_off.as<int>(), d_t4.as<Tile>(), naux, n,
+                         Lall.as<float>(), b3.as<float>(), d_pcol.as<int>(), d_pw.as<int>(), (int64_t)L.total);
+  };
+  int m4_depth = 2;
+  if (!t4.empty()) {
+    float best = 1e30f;
+    for (int depth : {2, 4}) {
+      m4(depth);
+      HIP_CHECK(hipEventRecord(c.ev0, c.stream));
+      m4(depth);
+      HIP_CHECK(hipEventRecord(c.ev1, c.stream));
+      HIP_CHECK(hipEventSynchronize(c.ev1));
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+      if (ms < best) {
+        best = ms;
+        m4_depth = depth;
+      }
+    }
+  }
   HIP_CHECK(hipStreamSynchronize(c.stream));
   if (t_prep) *t_prep = std::chrono::duration<double>(std::chrono::steady_clock::now() - tp0).count();
   // capture M2..M5 and the chunk reductions in one graph
@@ -753,10 +790,7 @@ double mp2_sos_batched(Ctx& c, DevBT& bt, const Metric& met, const Mp2Input& in,
   slab_gemm_f32_f64sum(c, bt, Lall.as<float>(), R, R, G2.as<float>(), bb.as<float>(), R, m2_depth);
   hipLaunchKernelGGL(k_transpose_pad_f32, tgrid, dim3(256), 0, c.stream, bb.as<float>(), n, R, ldn, bT.as<float>());
   m3(m3_depth);
-  hipLaunchKernelGGL((k_slab_out_gemm<float, SG_BM, 128>), dim3((unsigned)t4.size()), dim3(256), 0, c.stream,
-                     b2.as<float>(), R, 0, bt.slab_off.as<int64_t>(), bt.ncol.as<int>(), bt.bsz.as<int>(),
-                     bt.boff.as<int>(), bt.colmap.as<int>(), bt.colmap_off.as<int>(), d_t4.as<Tile>(), naux, n,
-                     Lall.as<float>(), R, b3.as<float>(), d_pcol.as<int>(), d_pw.as<int>(), (int64_t)L.total);
+  if (!t4.empty()) m4(m4_depth);
   hipLaunchKernelGGL(k_slabseg_f32, dim3((unsigned)(ntile * nch * npt)), dim3(512), 0, c.stream, b3.as<float>(),
                      (int64_t)L.total, B32.as<float>(), d_soff.as<int64_t>(), d_slen.as<int64_t>(), d_cs.as<int>(),
                      nch, naux, d_t5.as<int2>(), ntile, partial.as<double>());
