@@ -83,6 +83,7 @@ _SIGS = [
     ("mchip_int2c2e", c_int, [c_void_p, P_double]),
     ("mchip_int1e_gpu", c_int, [c_void_p, c_void_p, c_int, P_double, P_double, P_double, P_double, P_double]),
     ("mchip_int2c2e_gpu", c_int, [c_void_p, c_void_p, P_double]),
+    ("mchip_schwarz_ao_gpu", c_int, [c_void_p, c_void_p, P_double]),
     ("mchip_schwarz", c_int, [c_void_p, c_void_p, P_double, P_double]),
     ("mchip_int3c2e_shell", c_int, [c_void_p, c_void_p, c_int, c_int, c_int, P_double]),
     ("mchip_btensor_build", c_int, [c_void_p, c_void_p, c_int, P_int, c_int, P_int, P_u8,

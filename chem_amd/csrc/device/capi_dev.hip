@@ -269,6 +269,14 @@ extern "C" int mchip_int2c2e_gpu(mchip_ctx c, mchip_basis aux, double* V)
   MCHIP_CATCH
 }
 
+extern "C" int mchip_schwarz_ao_gpu(mchip_ctx c, mchip_basis ao, double* Qao)
+{
+  MCHIP_TRY
+  need_device(c);
+  schwarz_ao_device(c->c, basis_of(ao), Qao);
+  MCHIP_CATCH
+}
+
 extern "C" int mchip_dbt_info(mchip_dbt t, int* dims, int64_t* nelem)
 {
   MCHIP_TRY

@@ -1,10 +1,12 @@
-// One-electron integrals (overlap, kinetic, nuclear attraction) and the
-// two-centre Coulomb metric (X|Y) on the GPU, one kernel per angular momentum
-// class, for the request set-up of the serving loop and the C-ABI.
+// One-electron integrals (overlap, kinetic, nuclear attraction), the
+// two-centre Coulomb metric (X|Y) and the AO Schwarz factors on the GPU, one
+// kernel per angular momentum class, for the request set-up of the serving
+// loop and the C-ABI.
 //
-// Same McMurchie-Davidson formulas as the host engine (ints_host.cpp, ints_1e
-// and ints_2c2e; the reference's libcint calls int1e_ovlp_sph, int1e_kin_sph,
-// int1e_nuc_sph and int2c2e_sph, aofactory.cpp:380-430), with the recursions
+// Same McMurchie-Davidson formulas as the host engine (ints_host.cpp, ints_1e,
+// ints_2c2e and schwarz_ao; the reference's libcint calls int1e_ovlp_sph,
+// int1e_kin_sph, int1e_nuc_sph and int2c2e_sph, aofactory.cpp:380-430, and its
+// Schwarz factors, integrals.cpp:511-689), with the recursions
 // of the three-centre class kernels (ints3c_cls.hpp): one thread per shell
 // pair, everything unrolled with constant indices.
 #include <algorithm>
@@ -205,6 +207,144 @@ __global__ __launch_bounds__(64) void k_int2c2e_cls(const I2cArgs A)
   });
 }
 
+struct SchArgs {
+  const int* np;
+  const int* po;
+  const double* O;
+  const double* ex;
+  const double* cf;
+  const int* sha;          // shells of degree LA
+  const int* shb;          // shells of degree LB
+  int nsha, nshb;
+  const double* boys;
+  double* diag;            // [pair][component]: (ab|ab) of one spherical component pair
+  double sinv[LMAX + 1][2 * LMAX + 1];
+};
+
+// Diagonal (s s | s s) of one spherical component pair s = (sa, sb) of the
+// shell pair (a of degree LA, b of degree LB; LA == LB: b at or below a):
+// sum over primitive pairs k1, k2 of 2 pi^5/2 / (p q sqrt(p + q)) K1 K2
+// sum_tuv G1[tuv] sum_t'u'v' (-1)^(t'+u'+v') G2[t'u'v'] R[t+t', u+u', v+v']
+// with G the pair's Hermite coefficients of component s (ints_host.cpp,
+// schwarz_ao).  Thread = (pair, component).
+template <int LA, int LB>
+__global__ __launch_bounds__(64) void k_schwarz_ao_cls(const SchArgs A)
+{
+  using namespace i3c;
+  constexpr int L = LA + LB, TD = L + 1, NE = (LA + 1) * (LB + 1) * TD;
+  constexpr int NSA = 2 * LA + 1, NSB = 2 * LB + 1, NS = NSA * NSB;
+  const int64_t idx = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (idx >= (int64_t)A.nsha * A.nshb * NS) return;
+  const int64_t pidx = idx / NS;
+  const int sc = (int)(idx - pidx * NS);
+  const int ia = (int)(pidx / A.nshb), ib = (int)(pidx % A.nshb);
+  if (LA == LB && ib > ia) return;
+  const int sa = sc / NSB, sb = sc - (sc / NSB) * NSB;
+  const int a = A.sha[ia], b = A.shb[ib];
+  const double* __restrict__ ex = A.ex;
+  const double* __restrict__ cf = A.cf;
+  const double Ax = A.O[3 * a], Ay = A.O[3 * a + 1], Az = A.O[3 * a + 2];
+  const double Bx = A.O[3 * b], By = A.O[3 * b + 1], Bz = A.O[3 * b + 2];
+  const double AB2 = (Ax - Bx) * (Ax - Bx) + (Ay - By) * (Ay - By) + (Az - Bz) * (Az - Bz);
+  const int pa0 = A.po[a], pb0 = A.po[b], npa = A.np[a], npb = A.np[b];
+  auto ei = [](int i, int j, int t) { return (i * (LB + 1) + j) * TD + t; };
+  // G[tuv] of component (sa, sb) from the E coefficients of one primitive pair
+  // (the integer shapes; the normalisations are applied at the end)
+  auto hermite = [&](const double (&Ex)[NE], const double (&Ey)[NE], const double (&Ez)[NE], double (&G)[nh(L)]) {
+    static_for<nh(L)>([&](auto h) { G[h] = 0.0; });
+    static_for<NSA>([&](auto sA) {
+      if (sA.value != sa) return;
+      static_for<NSB>([&](auto sB) {
+        if (sB.value != sb) return;
+        static_for<nc(LA)>([&](auto ka) {
+          constexpr int wa = shp(LA, decltype(sA)::value, ka);
+          if constexpr (wa != 0) {
+            constexpr int ax = cx(LA, ka), ay = cy(LA, ka), az = cz(LA, ka);
+            static_for<nc(LB)>([&](auto kb) {
+              constexpr int wb = shp(LB, decltype(sB)::value, kb);
+              if constexpr (wb != 0) {
+                constexpr int bx = cx(LB, kb), by = cy(LB, kb), bz = cz(LB, kb);
+                static_for<ax + bx + 1>([&](auto t) {
+                  static_for<ay + by + 1>([&](auto u) {
+                    static_for<az + bz + 1>([&](auto v) {
+                      G[hix(t, u, v)] += (double)(wa * wb) * Ex[ei(ax, bx, t)] * Ey[ei(ay, by, u)] * Ez[ei(az, bz, v)];
+                    });
+                  });
+                });
+              }
+            });
+          }
+        });
+      });
+    });
+  };
+  double dsum = 0.0;
+  for (int i1 = 0; i1 < npa; ++i1)
+    for (int j1 = 0; j1 < npb; ++j1) {
+      const double a1 = ex[pa0 + i1], b1 = ex[pb0 + j1];
+      const double p = a1 + b1, mu1 = a1 * b1 / p;
+      if (mu1 * AB2 > 41.4) continue;   // exp(-mu R^2) < 1e-18, as the host's pair list
+      const double K1 = cf[pa0 + i1] * cf[pb0 + j1] * exp(-mu1 * AB2);
+      const double P1x = (a1 * Ax + b1 * Bx) / p, P1y = (a1 * Ay + b1 * By) / p, P1z = (a1 * Az + b1 * Bz) / p;
+      double G1[nh(L)];
+      {
+        double Ex[NE], Ey[NE], Ez[NE];
+        herm_E_t<LA, LB>(P1x - Ax, P1x - Bx, 0.5 / p, Ex);
+        herm_E_t<LA, LB>(P1y - Ay, P1y - By, 0.5 / p, Ey);
+        herm_E_t<LA, LB>(P1z - Az, P1z - Bz, 0.5 / p, Ez);
+        hermite(Ex, Ey, Ez, G1);
+      }
+      for (int i2 = 0; i2 < npa; ++i2)
+        for (int j2 = 0; j2 < npb; ++j2) {
+          const double a2 = ex[pa0 + i2], b2 = ex[pb0 + j2];
+          const double q = a2 + b2, mu2 = a2 * b2 / q;
+          if (mu2 * AB2 > 41.4) continue;
+          const double K2 = cf[pa0 + i2] * cf[pb0 + j2] * exp(-mu2 * AB2);
+          const double P2x = (a2 * Ax + b2 * Bx) / q, P2y = (a2 * Ay + b2 * By) / q, P2z = (a2 * Az + b2 * Bz) / q;
+          double G2[nh(L)];
+          {
+            double Ex[NE], Ey[NE], Ez[NE];
+            herm_E_t<LA, LB>(P2x - Ax, P2x - Bx, 0.5 / q, Ex);
+            herm_E_t<LA, LB>(P2y - Ay, P2y - By, 0.5 / q, Ey);
+            herm_E_t<LA, LB>(P2z - Az, P2z - Bz, 0.5 / q, Ez);
+            hermite(Ex, Ey, Ez, G2);
+          }
+          double R[nh(2 * L)];
+          herm_R_t<2 * L>(p * q / (p + q), P1x - P2x, P1y - P2y, P1z - P2z, A.boys, R);
+          double acc = 0.0;
+          static_for<nh(L)>([&](auto h1) {
+            constexpr int t = ht(h1), u = hu(h1), v = hv(h1);
+            double inner = 0.0;
+            static_for<nh(L)>([&](auto h2) {
+              constexpr int t2 = ht(h2), u2 = hu(h2), v2 = hv(h2);
+              constexpr double sg = ((t2 + u2 + v2) & 1) ? -1.0 : 1.0;
+              inner += sg * G2[h2] * R[hix(t + t2, u + u2, v + v2)];
+            });
+            acc += G1[h1] * inner;
+          });
+          dsum += 2.0 * PI * PI * sqrt(PI) / (p * q * sqrt(p + q)) * K1 * K2 * acc;
+        }
+    }
+  const double f = A.sinv[LA][sa] * A.sinv[LB][sb];
+  A.diag[pidx * NS + sc] = dsum * f * f;
+}
+
+// Q[a][b] = Q[b][a] = sqrt(sum over the components of |diag|) of the class's pairs
+__global__ void k_schwarz_ao_reduce(const double* __restrict__ diag, int ns_comp, const int* __restrict__ sha,
+                                    const int* __restrict__ shb, int nsha, int nshb, int same, int nshell,
+                                    double* __restrict__ Q)
+{
+  const int64_t pidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pidx >= (int64_t)nsha * nshb) return;
+  const int ia = (int)(pidx / nshb), ib = (int)(pidx % nshb);
+  if (same && ib > ia) return;
+  double n = 0.0;
+  for (int s = 0; s < ns_comp; ++s) n += fabs(diag[pidx * ns_comp + s]);
+  const int a = sha[ia], b = shb[ib];
+  Q[(int64_t)a * nshell + b] = sqrt(n);
+  Q[(int64_t)b * nshell + a] = sqrt(n);
+}
+
 namespace {
 
 std::vector<std::vector<int>> shells_by_degree(const Basis& b, int lmax)
@@ -347,6 +487,67 @@ void ints_2c2e_device(Ctx& c, const Basis& aux, double* M)
       HIP_CHECK(hipGetLastError());
     }
   d2h(M, dM.p, bytes, c.stream);
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+}
+
+void schwarz_ao_device(Ctx& c, const Basis& bas, double* Q)
+{
+  HIP_CHECK(hipSetDevice(c.device));
+  const int ns = (int)bas.shells.size();
+  for (const auto& sh : bas.shells)
+    if (sh.l > AO_L) throw std::runtime_error("GPU integrals: AO shells above d are not supported");
+  std::vector<int> np, po;
+  std::vector<double> O, ex, cf;
+  for (const auto& sh : bas.shells) {
+    np.push_back((int)sh.alpha.size());
+    po.push_back((int)ex.size());
+    for (int d = 0; d < 3; ++d) O.push_back(sh.O[d]);
+    ex.insert(ex.end(), sh.alpha.begin(), sh.alpha.end());
+    cf.insert(cf.end(), sh.coeff.begin(), sh.coeff.end());
+  }
+  const auto byl = shells_by_degree(bas, AO_L);
+  std::vector<int> shl, shl0(AO_L + 2, 0);
+  for (int l = 0; l <= AO_L; ++l) {
+    shl0[l] = (int)shl.size();
+    shl.insert(shl.end(), byl[l].begin(), byl[l].end());
+  }
+  shl0[AO_L + 1] = (int)shl.size();
+  if (ns == 0) return;
+  DevBuf d_np = upload_vec(np, c.stream), d_po = upload_vec(po, c.stream), d_O = upload_vec(O, c.stream),
+         d_ex = upload_vec(ex, c.stream), d_cf = upload_vec(cf, c.stream), d_shl = upload_vec(shl, c.stream),
+         d_boys = upload_vec(boys_table(), c.stream);
+  DevBuf dQ((size_t)ns * ns * sizeof(double));
+  SchArgs a{};
+  solid_harmonic_norms(a.sinv);
+  a.np = d_np.as<int>();
+  a.po = d_po.as<int>();
+  a.O = d_O.as<double>();
+  a.ex = d_ex.as<double>();
+  a.cf = d_cf.as<double>();
+  a.boys = d_boys.as<double>();
+  for (int la = 0; la <= AO_L; ++la)
+    for (int lb = 0; lb <= la; ++lb) {
+      a.sha = d_shl.as<int>() + shl0[la];
+      a.shb = d_shl.as<int>() + shl0[lb];
+      a.nsha = shl0[la + 1] - shl0[la];
+      a.nshb = shl0[lb + 1] - shl0[lb];
+      if (a.nsha == 0 || a.nshb == 0) continue;
+      const int nsc = (2 * la + 1) * (2 * lb + 1);
+      const int64_t npair = (int64_t)a.nsha * a.nshb;
+      DevBuf diag((size_t)npair * nsc * sizeof(double));
+      a.diag = diag.as<double>();
+      const dim3 grid(blocks_of(npair * nsc));
+#define MCHIP_SCH(LA, LB) \
+  if (la == LA && lb == LB) hipLaunchKernelGGL((k_schwarz_ao_cls<LA, LB>), grid, dim3(64), 0, c.stream, a);
+      MCHIP_SCH(0, 0) MCHIP_SCH(1, 0) MCHIP_SCH(1, 1) MCHIP_SCH(2, 0) MCHIP_SCH(2, 1) MCHIP_SCH(2, 2)
+#undef MCHIP_SCH
+      HIP_CHECK(hipGetLastError());
+      hipLaunchKernelGGL(k_schwarz_ao_reduce, dim3(blocks_of(npair)), dim3(64), 0, c.stream, diag.as<double>(), nsc,
+                         a.sha, a.shb, a.nsha, a.nshb, la == lb ? 1 : 0, ns, dQ.as<double>());
+      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipStreamSynchronize(c.stream));   // diag dies at the end of this scope
+    }
+  d2h(Q, dQ.p, dQ.bytes, c.stream);
   HIP_CHECK(hipStreamSynchronize(c.stream));
 }
 

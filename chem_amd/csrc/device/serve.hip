@@ -414,7 +414,7 @@ void Server::run(Ctx& c, SrvJob& j, bool& admitted)
   ints_2c2e_device(c, aux, M.data());
   const int nsa = (int)ao.shells.size(), nsx = (int)aux.shells.size();
   std::vector<double> Qao((size_t)nsa * nsa), Qx(nsx);
-  schwarz_ao(ao, Qao.data());
+  schwarz_ao_device(c, ao, Qao.data());
   schwarz_aux(aux, Qx.data());
   const std::vector<double> nmn = block_norms_2d(Qao, nsa, j.ao_blk);
   const std::vector<double> nx = block_norms_1d(Qx, j.aux_blk);

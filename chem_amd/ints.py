@@ -49,6 +49,14 @@ def int2c2e_gpu(ctx, mol):
     return M
 
 
+def schwarz_ao_gpu(ctx, mol):
+    """AO Schwarz factors (nshell x nshell) computed on the context's GPU."""
+    n = len(mol.shells)
+    Q = np.zeros((n, n))
+    _lib.check(_lib.lib().mchip_schwarz_ao_gpu(ctx.h, mol.basis_handle.h, _lib.dptr(Q)))
+    return Q
+
+
 def int4c2e(mol):
     """Exact four-centre ERIs (ij|kl) over the AO basis, nbf^4 row-major
     (host; the SAD guess's atomic SCFs use them)."""

@@ -206,6 +206,10 @@ int mchip_dbt_build_gpu(mchip_ctx c, mchip_basis ao, mchip_basis aux, int nbb,
 int mchip_int1e_gpu(mchip_ctx c, mchip_basis b, int natom, const double* Z, const double* xyz,
                     double* S, double* T, double* V);
 int mchip_int2c2e_gpu(mchip_ctx c, mchip_basis aux, double* V);
+/* AO Schwarz factors Q[a][b] = sqrt(sum_s |(ab|ab)_s|), nshell x nshell, on
+ * the context's GPU (the same factors as mchip_schwarz's Qao; the reference's
+ * Schwarz screener, integrals.cpp:511-689 and screening.cpp:8-30). */
+int mchip_schwarz_ao_gpu(mchip_ctx c, mchip_basis ao, double* Qao);
 /* dims[0..3] = naux, nbf, nxb, nbb; *nelem = stored elements. */
 int mchip_dbt_info(mchip_dbt t, int* dims, int64_t* nelem);
 int mchip_dbt_mask(mchip_dbt t, uint8_t* mask);

@@ -50,3 +50,24 @@ def test_gpu_1e_and_metric_match_host(ctx, nwater, dfbasis):
     print("M relerr %.3e" % err)
     assert err < 1e-12
     assert np.array_equal(Mg, Mg.T)
+
+
+@pytest.mark.parametrize("nwater,seed", [(1, 21), (3, 21), (8, 2)])
+def test_gpu_schwarz_matches_host(ctx, nwater, seed):
+    """AO Schwarz factors from the GPU class kernels against the host engine:
+    values to 1e-12 relative, symmetric, and the screening mask built from
+    them equal bit for bit to the library's (host) mask."""
+    mol = desc.Molecule(desc.water_cluster(nwater, seed))
+    print("case: GPU Schwarz n=%d seed=%d" % (nwater, seed), flush=True)
+    Qh, Qx = ints.schwarz(mol)
+    Qg = ints.schwarz_ao_gpu(ctx, mol)
+    assert np.array_equal(Qg, Qg.T)
+    err = np.abs(Qg - Qh).max() / np.abs(Qh).max()
+    big = Qh > 1e-8 * Qh.max()
+    rel = (np.abs(Qg - Qh)[big] / Qh[big]).max()
+    print("relerr %.3e  elementwise %.3e" % (err, rel))
+    assert err < 1e-12 and rel < 1e-11
+    for eps in (1e-10, 1e-8, 1e-6):
+        mask, _, _ = ints.schwarz_mask(mol, eps)
+        mine = desc.schwarz_block_mask(desc.block_norms(Qg, mol.blocks), desc.block_norms(Qx, mol.dfblocks), eps)
+        assert np.array_equal(mask, mine)
