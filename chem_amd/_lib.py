@@ -140,6 +140,7 @@ _SIGS = [
     ("mchip_ctx_set_guess", c_int, [c_void_p, c_int, P_double]),
     ("mchip_server_set_batching", c_int, [c_void_p, c_int]),
     ("mchip_dfk_energy_tilde64_dev", c_int, [c_void_p, c_void_p, c_int, c_int, P_double]),
+    ("mchip_dfk_energy_dev", c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, P_double]),
     ("mchip_server_batch_stats", c_int, [c_void_p, P_double]),
     ("mchip_dfk_mo_bf16_group_dev", c_int, [c_int, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), P_int,
                                             P_int, ctypes.POINTER(c_void_p)]),

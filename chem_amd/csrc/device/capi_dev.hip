@@ -590,6 +590,18 @@ extern "C" int mchip_dfk_energy_tilde64_dev(mchip_dbt t, const double* C_dev, in
   MCHIP_CATCH
 }
 
+extern "C" int mchip_dfk_energy_dev(mchip_dbt t, mchip_metric m, const double* C_dev, int ldc, int nocc,
+                                    int occ_nbatches, double* ek)
+{
+  MCHIP_TRY
+  need_device(t->ctx);
+  check_occ("mchip_dfk_energy_dev", ldc, nocc);
+  if (!ek) throw std::runtime_error("mchip_dfk_energy_dev: null result pointer");
+  if (!m) throw std::runtime_error("mchip_dfk_energy_dev: the metric is required");
+  *ek = dfk_energy_metric(t->ctx->c, t->t, m->m, C_dev, ldc, nocc, occ_nbatches);
+  MCHIP_CATCH
+}
+
 extern "C" int mchip_dfk_mo_bf16_group_dev(int n, const mchip_dbt* ts, const double* const* C_dev, const int* ldc,
                                            const int* nocc, double* const* K_dev)
 {

@@ -1780,6 +1780,65 @@ double dfk_energy_tilde64(Ctx& c, DevBT& t, const double* C, int ldc, int nocc)
   return e;
 }
 
+// tr(K P) of the plain FP64 exchange (untransformed B, metric step), without
+// K.  Per occupied batch i0 <= i < i0 + w:
+//   W[x][nu][i]  = sum_mu B(x|mu nu) C(mu, i0 + i)      (K1 of the plain path)
+//   M[x][j][i]   = sum_nu C(nu, j) W[x][nu][i]          (one strided-batched DGEMM)
+//   Mt           = L^-1 M over x                        (the K2 triangular multiply
+//                                                        on nocc instead of nbf rows)
+//   E           -= sum Mt^2                             (compensated, fixed order)
+// The metric step then costs naux^2 nocc^2 instead of naux^2 nbf nocc flops.
+// Work: W (naux nbf ldw) and M (naux nocc ldw) in scratch_a, Mt in scratch_b,
+// within the plain exchange's HT1 + HT2 of the same batch width.
+double dfk_energy_metric(Ctx& c, DevBT& t, const Metric& m, const double* C, int ldc, int nocc, int occ_nbatches)
+{
+  const BLayout& L = t.L;
+  if (t.transformed) throw std::runtime_error("exchange energy: B is transformed (use the FP64 copy form)");
+  if (m.naux != L.naux) throw std::runtime_error("exchange energy: metric and B have different aux dimensions");
+  if (nocc <= 0) return 0.0;
+  if (nocc > ldc) throw std::runtime_error("exchange energy: nocc exceeds the coefficients' columns");
+  const int naux = L.naux, nbf = L.nbf;
+  occ_nbatches = std::max(1, std::min(occ_nbatches, nocc));
+  const int per = (nocc + occ_nbatches - 1) / occ_nbatches;
+  const int nb = (nocc + per - 1) / per;
+  const int ldw = (per + 7) / 8 * 8;
+  const size_t nW = (size_t)naux * nbf * ldw, nM = (size_t)naux * nocc * ldw;
+  c.scratch_a.ensure((nW + nM) * sizeof(double));
+  c.scratch_b.ensure(nM * sizeof(double));
+  constexpr int NPART = 1024;
+  c.scratch_partial.ensure((size_t)(2 * NPART * nb + 8) * sizeof(double));
+  double* W = c.scratch_a.as<double>();
+  double* M = W + nW;
+  double* Mt = c.scratch_b.as<double>();
+  double* part = c.scratch_partial.as<double>();
+  double* out = part + 2 * NPART * nb;
+  const double one = 1.0, zero = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    const int i0 = b * per;
+    const int w = std::min(per, nocc - i0);
+    if (ldw != w)
+      hipLaunchKernelGGL(k_fill, dim3(gs_blocks((int64_t)nW)), dim3(256), 0, c.stream, W, (int64_t)nW, 0.0);
+    slab_gemm_t<double>(c, t, t.data64.as<double>(), C + i0, ldc, w, W, ldw);
+    // column-major per x: M_x (ldw x nocc) = W_x (ldw x nbf) * C^T (nbf x nocc),
+    // C^T being the column-major view of row-major C (ldc x nbf)
+    RB_CHECK(rocblas_dgemm_strided_batched(c.blas, rocblas_operation_none, rocblas_operation_transpose, ldw, nocc,
+                                           nbf, &one, W, ldw, (rocblas_stride)nbf * ldw, C, ldc, 0, &zero, M, ldw,
+                                           (rocblas_stride)nocc * ldw, naux));
+    // Mt = L^-1 M over x: column-major (nocc ldw x naux) view times (L^-1)^T
+    RB_CHECK(rocblas_dtrmm_64(c.blas, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                              rocblas_diagonal_non_unit, (int64_t)nocc * ldw, naux, &one, m.Linv.as<double>(), naux,
+                              M, (int64_t)nocc * ldw, Mt, (int64_t)nocc * ldw));
+    hipLaunchKernelGGL(k_sumsq_pairs, dim3(NPART), dim3(256), 0, c.stream, Mt, (int64_t)nM,
+                       part + (size_t)2 * NPART * b);
+    HIP_CHECK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c.stream, part, (int64_t)NPART * nb, -1.0, out);
+  HIP_CHECK(hipGetLastError());
+  double e = 0.0;
+  d2h(&e, out, sizeof(double), c.stream);
+  return e;
+}
+
 // Full 128 x 128 tile table of an n x n product (cached in t).
 static const int2* full_tiles(Ctx& c, DevBT& t, int n, int* ntile)
 {

@@ -48,6 +48,10 @@ void dfk_mo_bf16(Ctx& c, DevBT& t, const double* C, int ldc, int nocc, double* K
 // (device, row-major nbf x ldc, first nocc columns; P = C C^T), computed as
 // -sum_X ||W_X^T C||^2 without forming K (host result, synchronises).
 double dfk_energy_tilde64(Ctx& c, DevBT& t, const double* C, int ldc, int nocc);
+// tr(K P) of the FP64 exchange on the untransformed B with the metric, without
+// K: -sum_X ||(L^-1 M)_X||^2 with M_X = C^T B_X C, in occupied batches of the
+// first index.  Host result; synchronises the stream.
+double dfk_energy_metric(Ctx& c, DevBT& t, const Metric& m, const double* C, int ldc, int nocc, int occ_nbatches);
 // Grouped BF16x3 exchange over several tensors (one occupied batch each): the
 // coefficients of one build are gathered on its owner's stream c.stream
 // (prepare), then every build runs in one slab-GEMM and one SYRK launch on

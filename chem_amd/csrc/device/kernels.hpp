@@ -103,6 +103,7 @@ __global__ void k_slab_gemm_bf16x3_dla(const uint16_t* bhi, const uint16_t* blo,
 __global__ void k_exchange_energy(const double* W, int ldw, int nbf, int nocc, const double* C, int ldc, int naux,
                                   const int2* tiles, int ntile, double* partial);
 __global__ void k_sum_partials(const double* partial, int64_t n, double scale, double* out);
+__global__ void k_sumsq_pairs(const double* a, int64_t n, double* partial);
 template <int WM, int WN, int TM, int TN, int SA>
 __global__ void k_slab_gemm_bf16x3_dla_group(const ExGroupReq* reqs, const Tile* tiles);
 template <int S, int BM, int NW>

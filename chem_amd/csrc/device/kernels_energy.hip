@@ -12,6 +12,9 @@
 //                      compensated sum of squares of the tile (off-diagonal
 //                      tiles count twice) into partial[workgroup]
 //   k_sum_partials     the fixed-order compensated sum of the partials
+//   k_sumsq_pairs      compensated sum of squares of a dense array (the
+//                      metric-applied M_X of an untransformed B), one (sum,
+//                      error) pair per workgroup
 #include "dd.hpp"
 #include "gemm_mfma.hpp"
 #include "kernels.hpp"
@@ -103,6 +106,34 @@ __global__ __launch_bounds__(256) void k_exchange_energy(const double* __restric
     const double f = tl.x == tl.y ? 1.0 : 2.0;   // an upper off-diagonal tile stands for its mirror too
     partial[2 * (int64_t)item] = f * t.s;
     partial[2 * (int64_t)item + 1] = f * t.c;
+  }
+}
+
+// partial[2 b], partial[2 b + 1] = (sum, error) of sum a[i]^2 over the
+// grid-stride share of workgroup b (fixed order: the same bits on every run)
+__global__ __launch_bounds__(256) void k_sumsq_pairs(const double* __restrict__ a, int64_t n,
+                                                     double* __restrict__ partial)
+{
+  __shared__ double red[2][4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  dsum acc;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + tid; i < n; i += (int64_t)gridDim.x * 256) dot2_add(acc, a[i], a[i]);
+  dsum_wave(acc);
+  if (lane == 0) {
+    red[0][wave] = acc.s;
+    red[1][wave] = acc.c;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    dsum t;
+    for (int w = 0; w < 4; ++w) {
+      dsum o;
+      o.s = red[0][w];
+      o.c = red[1][w];
+      dsum_add(t, o);
+    }
+    partial[2 * blockIdx.x] = t.s;
+    partial[2 * blockIdx.x + 1] = t.c;
   }
 }
 

@@ -439,6 +439,10 @@ void select_k_builders(Ctx& c, DevBT& bt, const Metric* m, int kmode, int nocc, 
     };
     if (kenergy && t64 && final_occ_nbatches <= 1 && nocc > 0)
       *kenergy = [cp, t, nocc](const double* Cocc) { return dfk_energy_tilde64(*cp, *t, Cocc, nocc, nocc); };
+    else if (kenergy && !t64 && km && nocc > 0)
+      *kenergy = [cp, t, km, nocc, final_occ_nbatches](const double* Cocc) {
+        return dfk_energy_metric(*cp, *t, *km, Cocc, nocc, nocc, final_occ_nbatches);
+      };
   }
   if (kmode == 0 || kmode == 1) {
     kb = k64;

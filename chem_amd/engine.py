@@ -290,6 +290,19 @@ class DeviceB:
                                                             ctypes.byref(e)))
         return e.value
 
+    def exchange_energy_metric_dev(self, metric, C_dev, nocc, occ_nbatches=1):
+        """The same tr(K P) from the untransformed B and the metric (plain
+        FP64 exchange formula) without forming K: -sum_X ||(L^-1 M)_X||^2,
+        M_X = C^T B_X C (synchronises)."""
+        if getattr(C_dev, "ptr", None) is None or not C_dev.ptr.value:
+            raise ValueError("exchange_energy_metric_dev: C_dev has been released")
+        if len(C_dev.shape) != 2 or C_dev.shape[0] != self.nbf or not 0 <= nocc <= C_dev.shape[1]:
+            raise ValueError("exchange_energy_metric_dev: C_dev shape %s / nocc %d" % (C_dev.shape, nocc))
+        e = ctypes.c_double()
+        _lib.check(_lib.lib().mchip_dfk_energy_dev(self.h, metric.h, C_dev.ptr, C_dev.shape[1], nocc,
+                                                    int(occ_nbatches), ctypes.byref(e)))
+        return e.value
+
     @staticmethod
     def dfk_bf16_group(builds):
         """BF16x3 exchanges of several tensors in one grouped launch:

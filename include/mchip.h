@@ -289,6 +289,12 @@ int mchip_dfk_mo_tilde64_dev(mchip_dbt t, const double* C_dev, int ldc, int nocc
  * part of E = tr(H P) + tr(F P) (hfscf.cpp:278-296), which is all the
  * reported energy needs of the final exchange.  Synchronises; *ek on the host. */
 int mchip_dfk_energy_tilde64_dev(mchip_dbt t, const double* C_dev, int ldc, int nocc, double* ek);
+/* The same tr(K P) from the untransformed B and the metric (the plain FP64
+ * exchange's formula, jkdf.cpp:221-462), without K and without the nbf-sized
+ * metric step: -sum_X ||(L^-1 M)_X||^2 with M_X = C^T B_X C (nocc x nocc),
+ * in occ_nbatches batches of the first occupied index.  Synchronises. */
+int mchip_dfk_energy_dev(mchip_dbt t, mchip_metric m, const double* C_dev, int ldc, int nocc, int occ_nbatches,
+                         double* ek);
 int mchip_dfk_mo_bf16_dev(mchip_dbt t, const double* C_dev, int ldc, int nocc, int occ_nbatches,
                           double* K_dev);
 /* The BF16x3 exchange of n tensors on one context in one slab-GEMM launch

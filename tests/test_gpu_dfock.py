@@ -731,6 +731,34 @@ def test_exchange_energy_without_k(ctx, nw, ncol):
     Cd.release()
 
 
+@pytest.mark.parametrize("nw,ncol,nbatch", [(1, 5, 1), (6, 129, 1), (6, 129, 2), (12, 300, 3)])
+def test_exchange_energy_metric_without_k(ctx, nw, ncol, nbatch):
+    """tr(K P) of the plain FP64 exchange (untransformed B, metric step)
+    computed as -sum_X ||(L^-1 M)_X||^2, M_X = C^T B_X C, without forming K
+    (the final energy of the chosen SCF when no FP64 copy is held), equals the
+    trace of the full plain exchange to 1e-12 relative, in one to three
+    occupied batches; at one water also the oracle's."""
+    from chem_amd import desc
+    from chem_amd.engine import DeviceArray, DeviceB, Metric
+    mol = desc.Molecule(desc.water_cluster(nw, 60 + nw))
+    print("case: metric exchange energy, %d waters, %d columns, %d batches" % (nw, ncol, nbatch), flush=True)
+    Bh = ints.BTensorHost.build(mol, 1e-10)
+    dB = DeviceB(ctx, Bh)
+    m = Metric(ctx, ints.int2c2e(mol))
+    C, P = _random_occ(mol.nbf, ncol, 90 + nw)
+    Cd = DeviceArray(ctx, C.shape, C)
+    ek = dB.exchange_energy_metric_dev(m, Cd, ncol, nbatch)
+    full = float(np.sum(dB.dfk_mo(C, m) * P))
+    print("energy-only %.15e full trace %.15e rel %.2e" % (ek, full, abs(ek - full) / abs(full)))
+    assert ek < 0
+    assert abs(ek - full) <= 1e-12 * abs(full)
+    if nw == 1:
+        Linv, _ = m.get()
+        ref = float(np.sum(dfock.dfmo_k(Bh.dense(), Linv.T, C) * P))
+        assert abs(ek - ref) <= 1e-12 * abs(ref)
+    Cd.release()
+
+
 @pytest.mark.parametrize("nbatch", [1, 3])
 def test_graph_replay_matches_direct(ctx, nbatch):
     """Repeated builds with the same device operands replay a captured hipGraph
