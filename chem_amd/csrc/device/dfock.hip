@@ -306,8 +306,11 @@ void metric_create(Ctx& c, int naux, const double* V_host, Metric& m)
 // exchange (DFMO_K K2, jkdf.cpp:384).  Asynchronous.
 void linv_apply(Ctx& c, const Metric& m, const double* src, int64_t R, double* dst)
 {
-  dgemm_rm(c.stream, false, false, m.naux, R, m.naux, m.Linv.as<double>(), m.naux, src, R, dst, R, 1.0, 0.0,
-           DG_TRI_A_LOWER);
+  // dst = L^-1 src over x (src row-major naux x R): the column-major R x naux
+  // view times (L^-1)^T, out of place, on rocBLAS's triangular multiply
+  const double one = 1.0;
+  RB_CHECK(rocblas_dtrmm_64(c.blas, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                            rocblas_diagonal_non_unit, R, m.naux, &one, m.Linv.as<double>(), m.naux, src, R, dst, R));
 }
 
 void transform_B(Ctx& c, DevBT& t, const Metric& m)
