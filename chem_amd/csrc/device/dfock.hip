@@ -605,6 +605,76 @@ static void bf16_layout(Ctx& c, DevBT& t)
   t.bf16_plan.clear();
 }
 
+// The padded copies of B~ = L^-1 B from the canonical half of a symmetric
+// tensor: per slab N, its columns of blocks M <= N (the leading ce columns,
+// blocks in ascending order) packed into rows of length ce, transformed by
+// L^-1 over x, written into slab N's copies, and the columns of each block
+// M < N mirrored into slab M's columns for block N (B~(X|mu nu) = B~(X|nu mu)).
+// Every column of every slab is written once: the non-canonical columns of
+// slab M are the mirrors of the slabs of their (larger) blocks.  About half the
+// triangular-multiply flops of the full transform.
+static void tilde_slabs_sym(Ctx& c, DevBT& t, const Metric& m, float* out32, bool bf16, double* out64)
+{
+  const BLayout& L = t.L;
+  const int naux = L.naux;
+  // per slab: canonical width ce and the mirror segments of its blocks M < N
+  std::vector<int> ce(L.nbb, 0), seg_off(L.nbb + 1, 0);
+  std::vector<int4> seg;
+  int64_t maxc = 1;
+  for (int n = 0; n < L.nbb; ++n) {
+    int e = L.ncol[n];
+    for (int p = L.pair_ptr[n]; p < L.pair_ptr[n + 1]; ++p) {
+      const int mb = L.pair_m[p];
+      if (mb > n) {
+        e = L.pair_col[p];
+        break;
+      }
+      if (mb == n) {
+        e = L.pair_col[p] + L.bsz[n];
+        break;
+      }
+      // block mb < n: its mirror is slab mb's column block n
+      int q = -1;
+      for (int pp = L.pair_ptr[mb]; pp < L.pair_ptr[mb + 1]; ++pp)
+        if (L.pair_m[pp] == n) q = L.pair_col[pp];
+      if (q < 0) throw std::runtime_error("L^-1 B copies: the block mask is not symmetric");
+      seg.push_back(make_int4(mb, L.pair_col[p], q, 0));
+    }
+    ce[n] = e;
+    seg_off[n + 1] = (int)seg.size();
+    maxc = std::max<int64_t>(maxc, (int64_t)naux * L.bsz[n] * e);
+  }
+  DevBuf packed((size_t)maxc * sizeof(double)), T((size_t)maxc * sizeof(double));
+  DevBuf d_seg = upload_vec(seg.empty() ? std::vector<int4>(1) : seg, c.stream);
+  uint16_t* hi = bf16 ? t.data_hi.as<uint16_t>() : nullptr;
+  uint16_t* lo = bf16 ? t.data_lo.as<uint16_t>() : nullptr;
+  for (int n = 0; n < L.nbb; ++n) {
+    const int K = L.ncol[n], bN = L.bsz[n], e = ce[n];
+    const int64_t rows = (int64_t)naux * bN;
+    if (rows == 0 || K == 0) continue;
+    if (e > 0) {
+      const double* slab = t.data64.as<double>() + L.slab_off[n];
+      HIP_CHECK(hipMemcpy2DAsync(packed.p, (size_t)e * sizeof(double), slab, (size_t)K * sizeof(double),
+                                 (size_t)e * sizeof(double), (size_t)rows, hipMemcpyDeviceToDevice, c.stream));
+      linv_apply(c, m, packed.as<double>(), (int64_t)bN * e, T.as<double>());
+    }
+    hipLaunchKernelGGL(k_tilde_canon, dim3(gs_blocks(rows * t.h_ldp[n])), dim3(256), 0, c.stream, T.as<double>(), n,
+                       e, t.ncol.as<int>(), t.bsz.as<int>(), naux, t.ldp.as<int>(), t.off16.as<int64_t>(), out64,
+                       out32, hi, lo);
+    const int ns = seg_off[n + 1] - seg_off[n];
+    if (ns > 0) {
+      int64_t big = 0;
+      for (int s = seg_off[n]; s < seg_off[n + 1]; ++s) big = std::max<int64_t>(big, (int64_t)L.bsz[seg[s].x] * bN);
+      const unsigned gx = (unsigned)std::min<int64_t>(std::max<int64_t>(1, (int64_t)naux * big / 256), 256);
+      hipLaunchKernelGGL(k_tilde_mirror, dim3(gx, ns), dim3(256), 0, c.stream, T.as<double>(), n, e,
+                         d_seg.as<int4>() + seg_off[n], t.bsz.as<int>(), naux, t.ldp.as<int>(),
+                         t.off16.as<int64_t>(), out64, out32, hi, lo);
+    }
+    HIP_CHECK(hipGetLastError());
+  }
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+}
+
 // Low-precision copies of B~ = L^-1 B made slab by slab from the untransformed
 // B, which stays in data64: J, the reported energy and every FP64 exchange
 // keep the reference's formula (explicit V^-1 in J, L^-1 applied per build in
@@ -617,6 +687,10 @@ static void tilde_slabs(Ctx& c, DevBT& t, const Metric& m, float* out32, bool pa
                         double* out64 = nullptr)
 {
   const BLayout& L = t.L;
+  if (t.sym_mask && (pad32 || !out32)) {
+    tilde_slabs_sym(c, t, m, out32, bf16, out64);
+    return;
+  }
   int64_t maxslab = 0;
   for (int n = 0; n < L.nbb; ++n) maxslab = std::max<int64_t>(maxslab, L.slab_off[n + 1] - L.slab_off[n]);
   DevBuf tmp((size_t)std::max<int64_t>(maxslab, 1) * sizeof(double));

@@ -112,6 +112,12 @@ template <int S, int BM, int NW, bool DIAG1>
 __global__ void k_seg_syrk_bf16x3_dl(const uint16_t* whi, const uint16_t* wlo, int nseg, int64_t sstride,
                                      int ldw, int n, const int2* tiles, int ntile, int segs_per_chunk,
                                      float* partial);
+__global__ void k_tilde_canon(const double* T, int N, int ce, const int* ncol, const int* bsz, int naux,
+                              const int* ldp, const int64_t* off, double* out64, float* out32, uint16_t* hi,
+                              uint16_t* lo);
+__global__ void k_tilde_mirror(const double* T, int N, int ce, const int4* seg, const int* bsz, int naux,
+                               const int* ldp, const int64_t* off, double* out64, float* out32, uint16_t* hi,
+                               uint16_t* lo);
 __global__ void k_split_slab(const double* src, int N, const int* ncol, const int* bsz, int naux,
                              const int* ldp, const int64_t* off16, uint16_t* hi, uint16_t* lo);
 __global__ void k_gather_ct(const double* C, int ldc, int ng, const int* ncol, const int* ldp,

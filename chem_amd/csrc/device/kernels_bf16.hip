@@ -85,6 +85,65 @@ __global__ void k_split_slabs(const double* __restrict__ data64, const int64_t* 
   }
 }
 
+// Copies of L^-1 B from the canonical half of a symmetric tensor
+// (B(X|mu nu) = B(X|nu mu) and every kept pair (M, N) has its transpose kept).
+// T holds the transformed canonical columns c < ce of slab N, rows (x, nu) of
+// length ce.  The values go into the padded copies (element offsets off[],
+// row length ldp[]; any of out64, out32, hi/lo may be null):
+//   k_tilde_canon   slab N's columns c < ce, and zeros in its padding
+//                   columns [ncol, ldp) (columns [ce, ncol) are the mirrors
+//                   written for the slabs of their blocks)
+//   k_tilde_mirror  the columns of each block M < N (segment s: block M at
+//                   column col of slab N, width bM) into slab M's columns for
+//                   block N (from column q): element (x, mu, nu) of the
+//                   segment goes to row (x, mu), column q + nu of slab M
+//                   (nu fastest, so the stores run along a row)
+__device__ __forceinline__ void tilde_store(double v, int64_t j, double* out64, float* out32, uint16_t* hi,
+                                            uint16_t* lo)
+{
+  if (out64) out64[j] = v;
+  if (out32) out32[j] = (float)v;
+  if (hi) {
+    uint16_t a, b;
+    split64(v, a, b);
+    hi[j] = a;
+    lo[j] = b;
+  }
+}
+
+__global__ void k_tilde_canon(const double* __restrict__ T, int N, int ce, const int* __restrict__ ncol,
+                              const int* __restrict__ bsz, int naux, const int* __restrict__ ldp,
+                              const int64_t* __restrict__ off, double* __restrict__ out64,
+                              float* __restrict__ out32, uint16_t* __restrict__ hi, uint16_t* __restrict__ lo)
+{
+  const int K = ncol[N], ld = ldp[N];
+  const int64_t n = (int64_t)naux * bsz[N] * ld, o = off[N];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / ld;
+    const int c = (int)(i - r * ld);
+    if (c >= ce && c < K) continue;
+    tilde_store(c < ce ? T[r * ce + c] : 0.0, o + i, out64, out32, hi, lo);
+  }
+}
+
+__global__ void k_tilde_mirror(const double* __restrict__ T, int N, int ce, const int4* __restrict__ seg,
+                               const int* __restrict__ bsz, int naux, const int* __restrict__ ldp,
+                               const int64_t* __restrict__ off, double* __restrict__ out64,
+                               float* __restrict__ out32, uint16_t* __restrict__ hi, uint16_t* __restrict__ lo)
+{
+  const int4 sg = seg[blockIdx.y];   // (M, column of block M in slab N, column q of block N in slab M, unused)
+  const int M = sg.x, bM = bsz[M], bN = bsz[N], ldM = ldp[M];
+  const int64_t n = (int64_t)naux * bM * bN, oM = off[M];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t = i / bN;
+    const int nu = (int)(i - t * bN);
+    const int64_t x = t / bM;
+    const int mu = (int)(t - x * bM);
+    const double v = T[(x * bN + nu) * ce + sg.y + mu];
+    tilde_store(v, oM + (x * bM + mu) * ldM + sg.z + nu, out64, out32, hi, lo);
+  }
+}
+
 // One slab N whose FP64 values are at src (same row layout as in data64).
 __global__ void k_split_slab(const double* __restrict__ src, int N, const int* __restrict__ ncol,
                              const int* __restrict__ bsz, int naux, const int* __restrict__ ldp,

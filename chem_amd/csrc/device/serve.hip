@@ -442,15 +442,16 @@ void Server::run(Ctx& c, SrvJob& j, bool& admitted)
                              : 0.0);
   // FP64 exchange work space (HT1 and HT2 unless B is transformed; the final
   // exchange of the low-precision modes is the plain FP64 one, run in occupied
-  // batches of at most 8 GB per work array) and the per-slab transform buffer
-  // of the low-precision copies
+  // batches of at most 8 GB per work array) and the per-slab transform buffers
+  // of the low-precision copies (packed canonical columns and their transform,
+  // at most two slabs' worth)
   const double ht_full = 8.0 * naux * (double)n * nocc;
   j.final_occ_nbatches = j.kmode >= 2 ? std::max(1, (int)std::ceil(ht_full / 8e9)) : 1;
   const double ht = (j.kmode == 1 ? 1.0 : 2.0) * ht_full / j.final_occ_nbatches +
                     (j.kmode >= 2 ? 4.0 * naux * (double)n * ldw : 0.0);
   double maxslab = 0.0;
   for (int b = 0; b < L.nbb; ++b) maxslab = std::max(maxslab, (double)(L.slab_off[b + 1] - L.slab_off[b]));
-  est += std::max(4.0 * tot, copies + ht + 8.0 * maxslab);   // build scratch vs. resident copies + work
+  est += std::max(4.0 * tot, copies + ht + 16.0 * maxslab);   // build scratch vs. resident copies + work
   j.est_bytes = (size_t)est;
   j.t_host = now_s() - th0;
 
