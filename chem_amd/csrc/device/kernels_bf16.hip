@@ -893,11 +893,9 @@ __device__ __forceinline__ void slab_bf16x3_dla_tile(
     });
     issue_b(0, stage_b(std::integral_constant<int, 0>{}));
     wait_vm<0>();
-    for (int s0 = 0; s0 < nsteps; s0 += CYC) {
-      static_for<CYC>([&](auto k) {
-        const int s = s0 + k.value;
-        uint4* nb = stage_b(std::integral_constant<int, k.value + 1>{});
-        uint4* na = stage_a(std::integral_constant<int, k.value + SA - 1>{});
+    // step s: MFMAs on stages (sa, sb), loads of B(s + 1) into nb and of
+    // A(s + SA - 1) into na
+    auto step = [&](uint4* nb, uint4* na, const uint4* sa, const uint4* sb, int s) __attribute__((always_inline)) {
         const int sb1 = s + 1 < nsteps ? s + 1 : nsteps - 1;
         const int sa1 = s + SA - 1;
         const bool za = sa1 >= nsteps;   // uniform: zero rows for a step past the last
@@ -919,11 +917,28 @@ __device__ __forceinline__ void slab_bf16x3_dla_tile(
           }
         };
         block_sync();
-        const uint4* sa = stage_a(k);
-        const uint4* sb = stage_b(k);
         bf16x3_step_pipe_io<TM, TN, LA + LB>(sa, sa + BM * 4, sb, sb + BN * 4, wm0, wn0, lane, acc, issue);
         wait_vm<WAIT>();
+    };
+    // whole stage cycles with no branch (stages fixed at compile time), then
+    // the last nsteps mod CYC steps in a loop with the stages chosen at run
+    // time (the compiler may wait for every load there; the steps past the
+    // last are not run at all)
+    const int nfull = nsteps - nsteps % CYC;
+    for (int s0 = 0; s0 < nfull; s0 += CYC)
+      static_for<CYC>([&](auto k) {
+        step(stage_b(std::integral_constant<int, k.value + 1>{}),
+             stage_a(std::integral_constant<int, k.value + SA - 1>{}), stage_a(k), stage_b(k), s0 + k.value);
       });
+    auto sas = [&](int i) -> uint4* {
+      return i == 0   ? stage_a(std::integral_constant<int, 0>{})
+             : i == 1 ? stage_a(std::integral_constant<int, 1>{})
+             : i == 2 ? stage_a(std::integral_constant<int, 2>{})
+                      : stage_a(std::integral_constant<int, 3>{});
+    };
+    for (int s = nfull; s < nsteps; ++s) {
+      const int k = s - nfull;
+      step((k & 1) ? sb0 : sb1, sas((k + SA - 1) % SA), sas(k % SA), (k & 1) ? sb1 : sb0, s);
     }
     wait_vm<0>();   // no load into this workgroup's LDS outlives it
   }
