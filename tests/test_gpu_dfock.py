@@ -284,6 +284,52 @@ def test_f64_every_pipeline(ctx, nocc):
     assert np.array_equal(dB.dfk_mo(C, m), Kplain)
 
 
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_lowp_copies_canonical_half_and_full(ctx, symmetric):
+    """The L^-1 B copies (FP64, FP32, BF16 hi/lo) made from the canonical
+    half of a symmetric tensor with mirrored blocks (ragged blocks, an empty
+    slab), and over every column when one kept pair lacks its transpose: the
+    exchanges on the copies equal the plain DFMO exchange to each form's
+    tolerance."""
+    from chem_amd.engine import DeviceB, Metric
+    rng = np.random.default_rng(4242)
+    bsz = [14, 5, 5, 3, 14, 1, 7, 9, 30, 12]
+    xsz = [23, 70, 9, 40, 17]
+    nbf, naux = sum(bsz), sum(xsz)
+    mask = (rng.random((len(xsz), len(bsz), len(bsz))) < 0.6).astype(np.uint8)
+    mask = np.maximum(mask, mask.transpose(0, 2, 1))
+    mask[:, 5, :] = 0
+    mask[:, :, 5] = 0
+    if not symmetric:
+        mask[:, 2, 7] = 1
+        mask[:, 7, 2] = 0
+        mask[1, 7, 2] = 0
+    boff = np.concatenate([[0], np.cumsum(bsz)])
+    xoff = np.concatenate([[0], np.cumsum(xsz)])
+    dense = rng.standard_normal((naux, nbf, nbf))
+    dense = 0.5 * (dense + dense.transpose(0, 2, 1))
+    for x in range(len(xsz)):
+        for a in range(len(bsz)):
+            for b in range(len(bsz)):
+                if not mask[x, a, b]:
+                    dense[xoff[x]:xoff[x + 1], boff[a]:boff[a + 1], boff[b]:boff[b + 1]] = 0
+    print("case: low-precision copies, symmetric mask %s" % symmetric, flush=True)
+    assert np.array_equal(mask, mask.transpose(0, 2, 1)) == symmetric
+    assert np.array_equal(dense, dense.transpose(0, 2, 1)) == symmetric
+    Bh = ints.BTensorHost.from_dense(dense, xsz, bsz, mask)
+    A = rng.standard_normal((naux, naux))
+    m = Metric(ctx, A @ A.T + naux * np.eye(naux))
+    C = rng.standard_normal((nbf, 37)) * 0.3
+    dB = DeviceB(ctx, Bh)
+    Kplain = dB.dfk_mo(C, m)
+    dB.make_lowp(m, f32=True, bf16=True, f64=True)
+    for name, K, tol in (("fp64_copy", dB.dfk_mo_tilde64(C), 1e-12), ("f32", dB.dfk_mo_f32(C), 2e-6),
+                         ("bf16x3", dB.dfk_mo_bf16(C), BF16_TOL)):
+        rel = _relerr(K, Kplain)
+        print("  %s relerr vs plain %.2e" % (name, rel))
+        assert rel < tol
+
+
 def test_device_resident_exchange_matches_host(ctx):
     """The _dev entry points (occupied coefficients and K in device arrays,
     a row stride larger than the occupied count) give bitwise the host-pointer
